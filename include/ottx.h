@@ -1,0 +1,148 @@
+/*
+ * ottx.h — C ABI of the MI355X (gfx950) log-domain Sinkhorn library.
+ *
+ * The library replaces the XLA fusions that OTT compiles for
+ * ``ott.solvers.linear.solve(PointCloud(x, y), a, b)`` in log-domain mode with
+ * hand-written HIP kernels.  OTT itself has no FFI: its boundary is Python.
+ * Each entry point below names the reference interface it stands in for
+ * (paths relative to ott-jax/ott ``src/ott``); the Python package ``ott_amd``
+ * binds them with ctypes and mirrors the reference's classes on top.
+ *
+ * Conventions
+ *   - Every array pointer is device memory owned by the caller (torch tensors
+ *     in the Python binding), contiguous, float32 unless stated.
+ *   - Points are row-major: x is [n, d], y is [m, d].
+ *   - Calls are ordered on the caller's ``stream`` (a hipStream_t passed as
+ *     void*; NULL is the default stream).
+ *   - Return value: OTTX_OK (0) or a negative OTTX_E* code; nothing throws or
+ *     aborts across the ABI.  Numerical failure (NaN, no convergence) is
+ *     reported in the outputs exactly as OTT reports it, never as an error code.
+ *   - The library allocates no device memory.  Scratch space is a caller
+ *     workspace sized by the matching ``*_workspace_bytes`` query.
+ *   - Precision: OTTX_PREC_FP32 computes the cross term -2<x,y> in float32
+ *     (MFMA f32), OTTX_PREC_BF16_SPLIT with hi/lo bfloat16 operands
+ *     (hi*hi + hi*lo + lo*hi, float32 accumulation).  Potentials, exp and
+ *     log-sum-exp are float32; marginal-error and cost reductions are float64.
+ */
+#ifndef OTTX_H_
+#define OTTX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OTTX_ABI_VERSION 1
+
+enum ottx_status {
+  OTTX_OK = 0,
+  OTTX_EINVAL = -1,        /* invalid argument (shape, pointer, option) */
+  OTTX_EUNSUPPORTED = -2,  /* valid OTT option this library does not run */
+  OTTX_EHIP = -3,          /* a HIP runtime call failed */
+  OTTX_EWORKSPACE = -4,    /* workspace too small or misaligned */
+};
+
+enum ottx_precision {
+  OTTX_PREC_FP32 = 0,
+  OTTX_PREC_BF16_SPLIT = 1,
+};
+
+/* A point cloud geometry with squared-Euclidean cost,
+ * C_ij = inv_scale_cost * ||x_i - y_j||^2, and its linear problem
+ * (weights a, b and unbalancedness tau_a, tau_b).
+ * Reference: geometry/pointcloud.py:54-72 (PointCloud), geometry/costs.py:547-561
+ * (SqEuclidean), problems/linear/linear_problem.py:51-63 (LinearProblem). */
+typedef struct ottx_problem {
+  int64_t n;               /* rows of x */
+  int64_t m;               /* rows of y */
+  int32_t d;               /* feature dimension (1..256) */
+  int32_t precision;       /* enum ottx_precision */
+  const float* x;          /* [n, d] */
+  const float* y;          /* [m, d] */
+  const float* a;          /* [n] weights, >= 0 */
+  const float* b;          /* [m] weights, >= 0 */
+  double epsilon;          /* resolved target epsilon, > 0 (geometry.py:173-176) */
+  double inv_scale_cost;   /* s in C = s * ||x - y||^2, > 0 (pointcloud.py:334-340) */
+  double tau_a;            /* in (0, 1] (linear_problem.py:51-63) */
+  double tau_b;
+} ottx_problem;
+
+/* Options of the Sinkhorn solver: solvers/linear/sinkhorn.py:718-780. */
+typedef struct ottx_sinkhorn_opts {
+  double threshold;         /* default 1e-3 */
+  int32_t norm_error;       /* p of the p-norm, >= 1 (default 1) */
+  int32_t inner_iterations; /* default 10 */
+  int32_t min_iterations;   /* default 0 */
+  int32_t max_iterations;   /* default 2000 */
+  double momentum;          /* constant Momentum weight (acceleration.py:161-172), 1 = none */
+  int32_t tile_config;      /* -1 = tuned default, else an index from ottx_tile_configs */
+  int32_t use_graph;        /* 1 = replay each block of iterations as a hipGraph */
+} ottx_sinkhorn_opts;
+
+/* Outputs of one solve (SinkhornOutput fields, sinkhorn.py:280-323). */
+typedef struct ottx_sinkhorn_out {
+  float* f;                 /* [n] device: initial potential in, final potential out */
+  float* g;                 /* [m] device: initial potential in, final potential out */
+  float* errors;            /* [ceil(max_iterations / inner_iterations)] device */
+  double reg_ot_cost;       /* host, written on return (sinkhorn.py:224-277) */
+  double total_mass;        /* host: sum of the coupling at (f, g) */
+  int32_t converged;        /* host, sinkhorn.py:1020-1023 */
+  int32_t n_iters;          /* host, sinkhorn.py:431-434 */
+  int32_t n_blocks;         /* host: blocks of inner_iterations run */
+  int32_t _pad;
+} ottx_sinkhorn_out;
+
+/* Library identity. */
+int ottx_abi_version(void);
+const char* ottx_strerror(int status);
+
+/* Mean of the cost matrix in closed form, float64:
+ * s * (mean ||x||^2 + mean ||y||^2 - 2 <mean x, mean y>).
+ * Replaces geometry/geometry.py:112-117 -> pointcloud.py:170-186 (the
+ * relative_epsilon="mean" path of geometry.py:147-171).  *out_host is written
+ * after the stream has been synchronised. */
+size_t ottx_mean_cost_workspace_bytes(int64_t n, int64_t m, int32_t d);
+int ottx_mean_cost(int64_t n, int64_t m, int32_t d, const float* x,
+                   const float* y, double inv_scale_cost, double* out_host,
+                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* The Geometry operator protocol of the reference, for one PointCloud.
+ * mode selects the output written to out[k] (k over rows if axis == 1,
+ * columns if axis == 0):
+ *   OTTX_OP_APPLY_LSE   apply_lse_kernel(f, g, eps, axis)[0]  (geometry.py:260-299)
+ *   OTTX_OP_UPDATE      update_potential(f, g, log_marginal, axis)   (geometry.py:376-398);
+ *                       log_marginal ([m] if axis == 0 else [n]) is required for this op only
+ *   OTTX_OP_MARGINAL    marginal_from_potentials(f, g, axis)   (geometry.py:326-349)
+ * ``eps`` is the epsilon used in the exponent (the problem's epsilon for
+ * OTTX_OP_MARGINAL). */
+enum ottx_op {
+  OTTX_OP_APPLY_LSE = 0,
+  OTTX_OP_UPDATE = 1,
+  OTTX_OP_MARGINAL = 2,
+};
+size_t ottx_op_workspace_bytes(const ottx_problem* problem);
+int ottx_pointcloud_op(const ottx_problem* problem, int32_t op, int32_t axis,
+                       const float* f, const float* g, const float* log_marginal,
+                       double eps, float* out, void* workspace,
+                       size_t workspace_bytes, void* stream);
+
+/* Full log-domain Sinkhorn solve: Sinkhorn.__call__ -> run -> iterations
+ * (sinkhorn.py:782-801, 1052-1099) with the fixed-point loop of
+ * math/fixed_point_loop.py:23-93, the error of sinkhorn.py:127-221 fused into
+ * the half-steps, and reg_ot_cost of sinkhorn.py:224-277 in float64.
+ * out->f and out->g hold the initial potentials on entry (zeros with -inf on
+ * zero weights reproduces DefaultInitializer, initializers.py:70-131).
+ * Blocks until the loop has stopped; outputs are final on return. */
+size_t ottx_sinkhorn_workspace_bytes(const ottx_problem* problem,
+                                     const ottx_sinkhorn_opts* opts);
+int ottx_sinkhorn_solve(const ottx_problem* problem,
+                        const ottx_sinkhorn_opts* opts, ottx_sinkhorn_out* out,
+                        void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif  /* OTTX_H_ */

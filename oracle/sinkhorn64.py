@@ -1,0 +1,456 @@
+"""FP64 CPU oracle: OTT's log-domain Sinkhorn on a squared-Euclidean point cloud.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP path: it may
+be imported only by ``tests/``, by ``__graft_entry__.smoke()`` and by the
+``cpu_baseline`` leg of ``bench.py``.  Nothing in ``ott_amd`` imports it, and
+the product path never routes through it.
+
+Every function restates, operation for operation, the reference function it
+cites (paths are relative to the ott-jax/ott source tree ``src/ott``).  The
+arithmetic is float64 by default; passing ``dtype=np.float32`` gives an
+op-order twin that keeps OTT's float32 operation order ((f + g - C) / eps, then
+a max-shifted log-sum-exp, expanded squared-Euclidean cost), so float32-level
+agreement with the GPU path can be shown as well.
+
+Pinning (what checks this oracle independently of its own formulas) lives in
+``tests/test_oracle.py``: brute-force costs, ``scipy.special.logsumexp``, the
+m=1 closed form of a half-step, the exact marginal after a half-step, an
+L-BFGS solve of the smooth semi-dual, the Hungarian limit at small epsilon, the
+epsilon -> infinity limit, scale invariance and gauge invariance.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+from typing import Callable, Optional, Sequence, Tuple
+
+import numpy as np
+from scipy import special as sps
+
+__all__ = [
+    "sqeuclidean_cost",
+    "mean_cost",
+    "std_cost",
+    "resolve_epsilon",
+    "epsilon_at",
+    "DenseLSE",
+    "apply_lse_kernel",
+    "update_potential",
+    "marginal_from_potentials",
+    "grad_of_marginal_fit",
+    "marginal_error",
+    "solution_error",
+    "compute_kl_reg_cost",
+    "recenter",
+    "OracleOutput",
+    "sinkhorn",
+    "ent_reg_cost",
+    "dual_cost",
+]
+
+DEFAULT_EPSILON_SCALE = 0.05  # geometry/epsilon_scheduler.py:22
+
+
+# --------------------------------------------------------------------------
+# Costs and epsilon
+# --------------------------------------------------------------------------
+def sqeuclidean_cost(x, y, scale: float = 1.0, *, expanded: bool = False,
+                     dtype=np.float64) -> np.ndarray:
+  """Squared-Euclidean cost matrix ``s * ||x_i - y_j||^2``.
+
+  geometry/costs.py:554-561 (``norm(x) + norm(y) - 2 vdot``) applied pairwise
+  (costs.py:82-92) and scaled by ``inv_scale_cost`` (pointcloud.py:334-340).
+  ``expanded=True`` keeps the reference's expanded form; the default forms the
+  differences directly, which is exact up to one rounding per term.
+  """
+  x = np.asarray(x, dtype=dtype)
+  y = np.asarray(y, dtype=dtype)
+  if expanded:
+    nx = np.sum(x ** 2, axis=-1)
+    ny = np.sum(y ** 2, axis=-1)
+    c = nx[:, None] + ny[None, :] + (-2.0 * (x @ y.T)).astype(dtype)
+  else:
+    diff = x[:, None, :] - y[None, :, :]
+    c = np.sum(diff * diff, axis=-1)
+  return (np.asarray(scale, dtype=dtype) * c).astype(dtype)
+
+
+def mean_cost(x, y, scale: float = 1.0) -> float:
+  """Mean of the cost matrix in closed form.
+
+  geometry/geometry.py:112-117 calls ``apply_cost`` with a uniform vector, which
+  PointCloud answers with pointcloud.py:170-186:
+  ``sum_j (1/m) * s * (nx . vec + ny * sum(vec) - 2 y . (x^T vec))``.
+  """
+  x = np.asarray(x, dtype=np.float64)
+  y = np.asarray(y, dtype=np.float64)
+  n, m = x.shape[0], y.shape[0]
+  vec = np.full((n,), 1.0 / n)
+  nx = np.sum(x ** 2, axis=-1)
+  ny = np.sum(y ** 2, axis=-1)
+  applied = nx @ vec + ny * np.sum(vec) - 2.0 * (y @ (x.T @ vec))
+  return float(np.sum((1.0 / m) * scale * applied))
+
+
+def std_cost(x, y, scale: float = 1.0) -> float:
+  """Standard deviation of the cost matrix (geometry/geometry.py:119-136)."""
+  c = sqeuclidean_cost(x, y, scale)
+  n, m = c.shape
+  tmp = np.sum((1.0 / m) * np.sum(c ** 2, axis=0) / n) - mean_cost(x, y, scale) ** 2
+  return float(np.sqrt(max(tmp, 0.0)))
+
+
+def resolve_epsilon(x, y, epsilon=None, relative_epsilon=None,
+                    scale: float = 1.0) -> float:
+  """Target epsilon of a PointCloud (geometry/geometry.py:147-176)."""
+  if relative_epsilon is None:
+    if epsilon is not None:
+      return float(epsilon)
+    return DEFAULT_EPSILON_SCALE * std_cost(x, y, scale)
+  if relative_epsilon == "std":
+    stat = std_cost(x, y, scale)
+  elif relative_epsilon == "mean":
+    stat = mean_cost(x, y, scale)
+  else:
+    raise ValueError(f"Invalid relative epsilon: {relative_epsilon}.")
+  mult = DEFAULT_EPSILON_SCALE if epsilon is None else epsilon
+  return float(mult * stat)
+
+
+def epsilon_at(target: float, it: Optional[int], init: float = 1.0,
+               decay: float = 1.0) -> float:
+  """Scheduled epsilon (geometry/epsilon_scheduler.py:47-60)."""
+  if it is None:
+    return target
+  return max(init * decay ** it, 1.0) * target
+
+
+# --------------------------------------------------------------------------
+# Log-domain kernel operators (the Geometry protocol)
+# --------------------------------------------------------------------------
+def _lse(z: np.ndarray, axis: int) -> np.ndarray:
+  """Max-shifted log-sum-exp; -inf for an all -inf slice.
+
+  math/utils.py:132-138 forwards to ``jax.scipy.special.logsumexp``; scipy's
+  routine computes the same max-shifted quantity.
+  """
+  with np.errstate(invalid="ignore", over="ignore"):
+    return sps.logsumexp(z, axis=axis)
+
+
+class DenseLSE:
+  """``C``-backed implementation of the Geometry LSE protocol.
+
+  ``lse(f, g, eps, axis)`` returns ``eps * LSE((f[:, None] + g[None, :] - C) / eps)``
+  over ``axis`` (geometry/geometry.py:423-442, ``_center`` then ``_softmax``).
+  """
+
+  def __init__(self, cost: np.ndarray):
+    self.cost = cost
+
+  @property
+  def shape(self) -> Tuple[int, int]:
+    return self.cost.shape
+
+  def lse(self, f, g, eps, axis):
+    dt = self.cost.dtype
+    with np.errstate(invalid="ignore"):
+      centered = f.astype(dt)[:, None] + g.astype(dt)[None, :] - self.cost
+      return (np.asarray(eps, dt) * _lse(centered / np.asarray(eps, dt), axis)
+              ).astype(dt)
+
+
+class PointCloudLSE:
+  """Cost-free implementation of the same protocol for large clouds.
+
+  Backed by ``liboracle_sinkhorn.so`` (oracle/sinkhorn_cpu.c, FP64, OpenMP):
+  the online mode of pointcloud.py:85-103 with the cost recomputed per block.
+  """
+
+  def __init__(self, x, y, scale: float = 1.0, threads: int = 0):
+    import ctypes
+    import os
+    lib_path = os.path.join(os.path.dirname(__file__), "liboracle_sinkhorn.so")
+    if not os.path.exists(lib_path):
+      raise RuntimeError(f"oracle C library not built: {lib_path} (run make -C oracle)")
+    self._lib = ctypes.CDLL(lib_path)
+    self._lib.ox_lse.restype = ctypes.c_int
+    self._ct = ctypes
+    self.x = np.ascontiguousarray(x, dtype=np.float64)
+    self.y = np.ascontiguousarray(y, dtype=np.float64)
+    self.scale = float(scale)
+    self.threads = int(threads)
+
+  @property
+  def shape(self) -> Tuple[int, int]:
+    return self.x.shape[0], self.y.shape[0]
+
+  def lse(self, f, g, eps, axis):
+    ct = self._ct
+    n, m = self.shape
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    out = np.empty((m if axis == 0 else n,), dtype=np.float64)
+    dp = ct.POINTER(ct.c_double)
+    rc = self._lib.ox_lse(
+        self.x.ctypes.data_as(dp), self.y.ctypes.data_as(dp),
+        ct.c_int64(n), ct.c_int64(m), ct.c_int32(self.x.shape[1]),
+        ct.c_double(self.scale), f.ctypes.data_as(dp), g.ctypes.data_as(dp),
+        ct.c_double(float(eps)), ct.c_int(axis), out.ctypes.data_as(dp),
+        ct.c_int(self.threads))
+    if rc != 0:
+      raise ValueError("ox_lse rejected its arguments")
+    return out
+
+
+def apply_lse_kernel(ops, f, g, eps, axis=0):
+  """geometry/geometry.py:260-299 (vec=None): softmax minus the finite h."""
+  res = ops.lse(f, g, eps, axis)
+  remove = f if axis == 1 else g
+  with np.errstate(invalid="ignore"):
+    return res - np.where(np.isfinite(remove), remove, 0.0)
+
+
+def update_potential(ops, f, g, log_marginal, eps, axis=0):
+  """geometry/geometry.py:376-398."""
+  app = apply_lse_kernel(ops, f, g, eps, axis)
+  with np.errstate(invalid="ignore"):
+    return eps * log_marginal - np.where(np.isfinite(app), app, 0.0)
+
+
+def marginal_from_potentials(ops, f, g, eps, axis=0):
+  """geometry/geometry.py:326-349 (always the target epsilon)."""
+  h = f if axis == 1 else g
+  z = apply_lse_kernel(ops, f, g, eps, axis)
+  with np.errstate(invalid="ignore", over="ignore"):
+    return np.exp((z + h) / eps)
+
+
+# --------------------------------------------------------------------------
+# Errors and costs
+# --------------------------------------------------------------------------
+def rho(eps, tau):
+  """math/unbalanced_functions.py:89-90."""
+  return (eps * tau) / (1.0 - tau)
+
+
+def phi_star(h, r):
+  """math/unbalanced_functions.py:19-21."""
+  return r * (np.exp(h / r) - 1.0)
+
+
+def grad_of_marginal_fit(c, h, tau, eps):
+  """math/unbalanced_functions.py:30-50."""
+  if tau == 1.0:
+    return c
+  r = rho(eps, tau)
+  with np.errstate(over="ignore", invalid="ignore"):
+    return np.where(c > 0, c * np.exp(-h / r), 0.0)
+
+
+def marginal_error(ops, f, g, target, eps, axis=0, norm_error=(1,)):
+  """solvers/linear/sinkhorn.py:191-221."""
+  marginal = marginal_from_potentials(ops, f, g, eps, axis)
+  p = np.asarray(norm_error, dtype=marginal.dtype)
+  return np.sum(np.abs(marginal - target)[None, :] ** p[:, None], axis=1) ** (1.0 / p)
+
+
+def solution_error(ops, f, g, a, b, eps, tau_a, tau_b, norm_error=(1,),
+                   parallel_dual_updates=False):
+  """solvers/linear/sinkhorn.py:127-188 (lse_mode=True)."""
+  if tau_a == 1.0 and tau_b == 1.0 and not parallel_dual_updates:
+    return marginal_error(ops, f, g, b, eps, 0, norm_error)
+  grad_a = grad_of_marginal_fit(a, f, tau_a, eps)
+  grad_b = grad_of_marginal_fit(b, g, tau_b, eps)
+  err = marginal_error(ops, f, g, grad_a, eps, 1, norm_error)
+  err = err + marginal_error(ops, f, g, grad_b, eps, 0, norm_error)
+  return err
+
+
+def compute_kl_reg_cost(ops, f, g, a, b, eps, tau_a=1.0, tau_b=1.0):
+  """solvers/linear/sinkhorn.py:224-277 (lse_mode=True)."""
+  supp_a, supp_b = a > 0, b > 0
+  with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+    fa = eps * np.log(a)  # geometry.py:524-534 potential_from_scaling
+    if tau_a == 1.0:
+      div_a = np.sum(np.where(supp_a, a * (f - fa), 0.0))
+    else:
+      div_a = -np.sum(np.where(supp_a, a * phi_star(-(f - fa), rho(eps, tau_a)), 0.0))
+    gb = eps * np.log(b)
+    if tau_b == 1.0:
+      div_b = np.sum(np.where(supp_b, b * (g - gb), 0.0))
+    else:
+      div_b = -np.sum(np.where(supp_b, b * phi_star(-(g - gb), rho(eps, tau_b)), 0.0))
+  total_sum = np.sum(marginal_from_potentials(ops, f, g, eps, 0))
+  return div_a + div_b + eps * (np.sum(a) * np.sum(b) - total_sum)
+
+
+def ent_reg_cost(reg_ot_cost, a, b, eps):
+  """solvers/linear/sinkhorn.py:350-370: reg cost minus eps * (H(a) + H(b))."""
+  return reg_ot_cost - eps * (np.sum(sps.entr(a)) + np.sum(sps.entr(b)))
+
+
+def dual_cost(f, g, a, b):
+  """solvers/linear/sinkhorn.py:337-343."""
+  return (np.sum(np.where(a > 0.0, a * f, 0.0)) +
+          np.sum(np.where(b > 0.0, b * g, 0.0)))
+
+
+def recenter(f, g, a, b, eps, tau_a, tau_b):
+  """solvers/linear/sinkhorn.py:76-114."""
+  if tau_a == 1.0 and tau_b == 1.0:
+    fin = np.isfinite(f)
+    shift = np.sum(np.where(fin, f, 0.0)) / np.sum(fin)
+    return f - shift, g + shift
+  if tau_a == 1.0 or tau_b == 1.0:
+    return f, g
+  ra, rb = rho(eps, tau_a), rho(eps, tau_b)
+  tau = ra * rb / (ra + rb)
+  shift = tau * (sps.logsumexp(-f / ra, b=a) - sps.logsumexp(-g / rb, b=b))
+  return f + shift, g - shift
+
+
+# --------------------------------------------------------------------------
+# The solver loop
+# --------------------------------------------------------------------------
+@dataclasses.dataclass
+class OracleOutput:
+  f: np.ndarray
+  g: np.ndarray
+  errors: np.ndarray
+  reg_ot_cost: float
+  converged: bool
+  n_iters: int
+  epsilon: float
+  threshold: float
+  inner_iterations: int
+
+
+def sinkhorn(
+    ops,
+    a: np.ndarray,
+    b: np.ndarray,
+    epsilon: float,
+    *,
+    tau_a: float = 1.0,
+    tau_b: float = 1.0,
+    threshold: float = 1e-3,
+    norm_error: int = 1,
+    inner_iterations: int = 10,
+    min_iterations: int = 0,
+    max_iterations: int = 2000,
+    momentum: float = 1.0,
+    parallel_dual_updates: bool = False,
+    recenter_potentials: bool = False,
+    init: Optional[Tuple[np.ndarray, np.ndarray]] = None,
+    eps_init: float = 1.0,
+    eps_decay: float = 1.0,
+    progress_fn: Optional[Callable] = None,
+    dtype=np.float64,
+) -> OracleOutput:
+  """Log-domain Sinkhorn restated from OTT.
+
+  - initializer: initializers/linear/initializers.py:70-131 (zeros, then -inf
+    on zero weights) unless ``init`` is given (sinkhorn.py:782-801);
+  - loop: math/fixed_point_loop.py:23-93 (blocks of ``inner_iterations``,
+    stop test between blocks, ``lax.scan`` when min == max);
+  - one iteration: sinkhorn.py:879-938 with ``lse_step`` sinkhorn.py:803-857
+    (recentering terms omitted: only used when ``recenter_potentials`` and both
+    taus < 1, which this oracle rejects) and constant ``Momentum``
+    (acceleration.py:161-172);
+  - termination: sinkhorn.py:940-962; outputs: sinkhorn.py:974-1029, n_iters
+    sinkhorn.py:431-434; cost: sinkhorn.py:224-277 through ``run``
+    sinkhorn.py:1052-1062.
+  """
+  if recenter_potentials and tau_a < 1.0 and tau_b < 1.0:
+    raise NotImplementedError("unbalanced recentering is not restated")
+  a = np.asarray(a, dtype=dtype)
+  b = np.asarray(b, dtype=dtype)
+  eps_t = np.asarray(epsilon, dtype=dtype)
+  n, m = ops.shape
+  with np.errstate(divide="ignore"):
+    log_a, log_b = np.log(a), np.log(b)
+  if init is None:
+    f = np.where(a > 0.0, 0.0, -np.inf).astype(dtype)
+    g = np.where(b > 0.0, 0.0, -np.inf).astype(dtype)
+  else:
+    f = np.asarray(init[0], dtype=dtype).copy()
+    g = np.asarray(init[1], dtype=dtype).copy()
+
+  outer = int(np.ceil(max_iterations / inner_iterations))
+  errors = -np.ones((outer,), dtype=dtype)
+  force_scan = min_iterations == max_iterations
+
+  def one_iteration(it, f, g, compute_error):
+    eps_it = dtype(epsilon_at(float(epsilon), it, eps_init, eps_decay))
+    w = dtype(momentum)
+    old_f, old_g = f, g
+    new_g = tau_b * update_potential(ops, old_f, old_g, log_b, eps_it, axis=0)
+    with np.errstate(invalid="ignore"):
+      g = (1.0 - w) * np.where(np.isfinite(old_g), old_g, 0.0) + w * new_g
+    g_for_f = old_g if parallel_dual_updates else g
+    new_f = tau_a * update_potential(ops, old_f, g_for_f, log_a, eps_it, axis=1)
+    with np.errstate(invalid="ignore"):
+      f = (1.0 - w) * np.where(np.isfinite(old_f), old_f, 0.0) + w * new_f
+    if it == max_iterations - 1 or (compute_error and it >= min_iterations):
+      ff, gg = f, g
+      if recenter_potentials:
+        ff, gg = recenter(f, g, a, b, eps_t, tau_a, tau_b)
+      err = solution_error(ops, ff, gg, a, b, eps_t, tau_a, tau_b,
+                           (norm_error,), parallel_dual_updates)[0]
+    else:
+      err = np.inf
+    errors[it // inner_iterations] = err
+    if progress_fn is not None:
+      progress_fn((it, inner_iterations, max_iterations, errors))
+    return f, g
+
+  def run_block(it, f, g):
+    for j in range(inner_iterations):
+      f, g = one_iteration(it, f, g, j == inner_iterations - 1)
+      it += 1
+    return it, f, g
+
+  it = 0
+  if force_scan:
+    for _ in range(max_iterations // inner_iterations):
+      it, f, g = run_block(it, f, g)
+  else:
+    while True:
+      if not it < max_iterations:
+        break
+      if not it < min_iterations:
+        err = errors[it // inner_iterations - 1]
+        converged = it > 0 and err < threshold
+        diverged = not np.isfinite(err)
+        if converged or diverged:
+          break
+      it, f, g = run_block(it, f, g)
+
+  if recenter_potentials:
+    f, g = recenter(f, g, a, b, eps_t, tau_a, tau_b)
+  converged = bool((not np.any(np.isnan(errors))) and errors[-1] < threshold)
+  n_iters = int(np.sum(errors != -1) * inner_iterations)
+  reg = compute_kl_reg_cost(ops, f, g, a, b, eps_t, tau_a, tau_b)
+  return OracleOutput(f=f, g=g, errors=errors, reg_ot_cost=float(reg),
+                      converged=converged, n_iters=n_iters,
+                      epsilon=float(epsilon), threshold=threshold,
+                      inner_iterations=inner_iterations)
+
+
+def solve_pointcloud(x, y, a=None, b=None, *, epsilon=None,
+                     relative_epsilon=None, scale_cost: float = 1.0,
+                     dtype=np.float64, expanded: bool = False, **kwargs):
+  """``linear.solve(PointCloud(x, y, ...), a, b, **kwargs)`` with a dense cost.
+
+  solvers/linear/_solve.py:29-64 and problems/linear/linear_problem.py:65-79
+  (``None`` weights become uniform).
+  """
+  x = np.asarray(x, dtype=np.float64)
+  y = np.asarray(y, dtype=np.float64)
+  n, m = x.shape[0], y.shape[0]
+  a = np.full((n,), 1.0 / n) if a is None else np.asarray(a, np.float64)
+  b = np.full((m,), 1.0 / m) if b is None else np.asarray(b, np.float64)
+  eps = resolve_epsilon(x, y, epsilon, relative_epsilon, 1.0 / scale_cost)
+  cost = sqeuclidean_cost(x, y, 1.0 / scale_cost, expanded=expanded, dtype=dtype)
+  return sinkhorn(DenseLSE(cost), a, b, eps, dtype=dtype, **kwargs)

@@ -1,0 +1,159 @@
+"""ctypes binding of ``libottx.so`` (the C ABI in ``include/ottx.h``).
+
+The library is built in-tree (``ott_amd/lib/libottx.so``) by ``ott_amd._build``.
+There is no fallback: every entry point here raises when the library or a GPU
+is missing.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from typing import Optional
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libottx.so")
+
+OTTX_OK = 0
+OTTX_EINVAL = -1
+OTTX_EUNSUPPORTED = -2
+OTTX_EHIP = -3
+OTTX_EWORKSPACE = -4
+
+PREC_FP32 = 0
+PREC_BF16_SPLIT = 1
+PRECISIONS = {"fp32": PREC_FP32, "bf16_split": PREC_BF16_SPLIT}
+
+OP_APPLY_LSE = 0
+OP_UPDATE = 1
+OP_MARGINAL = 2
+
+
+class Problem(ctypes.Structure):
+  _fields_ = [
+      ("n", ctypes.c_int64),
+      ("m", ctypes.c_int64),
+      ("d", ctypes.c_int32),
+      ("precision", ctypes.c_int32),
+      ("x", ctypes.c_void_p),
+      ("y", ctypes.c_void_p),
+      ("a", ctypes.c_void_p),
+      ("b", ctypes.c_void_p),
+      ("epsilon", ctypes.c_double),
+      ("inv_scale_cost", ctypes.c_double),
+      ("tau_a", ctypes.c_double),
+      ("tau_b", ctypes.c_double),
+  ]
+
+
+class SinkhornOpts(ctypes.Structure):
+  _fields_ = [
+      ("threshold", ctypes.c_double),
+      ("norm_error", ctypes.c_int32),
+      ("inner_iterations", ctypes.c_int32),
+      ("min_iterations", ctypes.c_int32),
+      ("max_iterations", ctypes.c_int32),
+      ("momentum", ctypes.c_double),
+      ("tile_config", ctypes.c_int32),
+      ("use_graph", ctypes.c_int32),
+  ]
+
+
+class SinkhornOut(ctypes.Structure):
+  _fields_ = [
+      ("f", ctypes.c_void_p),
+      ("g", ctypes.c_void_p),
+      ("errors", ctypes.c_void_p),
+      ("reg_ot_cost", ctypes.c_double),
+      ("total_mass", ctypes.c_double),
+      ("converged", ctypes.c_int32),
+      ("n_iters", ctypes.c_int32),
+      ("n_blocks", ctypes.c_int32),
+      ("_pad", ctypes.c_int32),
+  ]
+
+
+# (name, restype, argtypes) of every exported symbol declared in include/*.h.
+_SIGNATURES = [
+    ("ottx_abi_version", ctypes.c_int, []),
+    ("ottx_strerror", ctypes.c_char_p, [ctypes.c_int]),
+    ("ottx_mean_cost_workspace_bytes", ctypes.c_size_t,
+     [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32]),
+    ("ottx_mean_cost", ctypes.c_int,
+     [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+      ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    ("ottx_op_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(Problem)]),
+    ("ottx_pointcloud_op", ctypes.c_int,
+     [ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    ("ottx_sinkhorn_workspace_bytes", ctypes.c_size_t,
+     [ctypes.POINTER(Problem), ctypes.POINTER(SinkhornOpts)]),
+    ("ottx_sinkhorn_solve", ctypes.c_int,
+     [ctypes.POINTER(Problem), ctypes.POINTER(SinkhornOpts),
+      ctypes.POINTER(SinkhornOut), ctypes.c_void_p, ctypes.c_size_t,
+      ctypes.c_void_p]),
+]
+
+EXPORTED_SYMBOLS = tuple(name for name, _, _ in _SIGNATURES)
+
+_lib_lock = threading.Lock()
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+  """Load ``libottx.so`` and set every prototype; raise if it is not built."""
+  global _lib
+  with _lib_lock:
+    if _lib is not None:
+      return _lib
+    if not os.path.exists(path):
+      raise RuntimeError(
+          f"ottx HIP library not found at {path}; build it with "
+          "`python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(path)
+    for name, restype, argtypes in _SIGNATURES:
+      fn = getattr(lib, name)
+      fn.restype = restype
+      fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+class OttxError(RuntimeError):
+  """A non-zero status returned by the C ABI."""
+
+  def __init__(self, fn: str, status: int):
+    msg = load_library().ottx_strerror(status).decode()
+    super().__init__(f"{fn} failed: {msg} ({status})")
+    self.status = status
+
+
+def check(fn: str, status: int) -> None:
+  if status == OTTX_EUNSUPPORTED:
+    raise NotImplementedError(f"{fn}: option not supported by the HIP kernels")
+  if status != OTTX_OK:
+    raise OttxError(fn, status)
+
+
+def require_device() -> torch.device:
+  """The GPU the kernels run on; raise loudly when none is present."""
+  if not torch.cuda.is_available():
+    raise RuntimeError("ott_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+  return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr() -> ctypes.c_void_p:
+  return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def workspace(nbytes: int, device: torch.device) -> torch.Tensor:
+  return torch.empty((max(int(nbytes), 256),), dtype=torch.uint8, device=device)
+
+
+def ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+  return None if t is None else t.data_ptr()

@@ -1,0 +1,92 @@
+// Shared device/host definitions of the ottx library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ottx.h"
+
+namespace ottx {
+
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kLn2 = 0.6931471805599453f;
+constexpr int kWave = 64;
+
+// Rows of every packed point array are padded to this multiple, so that any
+// streamed row tile and any output column tile stays inside the allocation.
+constexpr int kRowPad = 256;
+
+// Half-step mode bits (uniform per launch).
+enum : uint32_t {
+  kModeWritePot = 1u << 0,   // write the (momentum-blended) new potential
+  kModeSaveOld = 1u << 1,    // copy the old out-side potential to out_save
+  kModeErrCol = 1u << 2,     // accumulate the column error + mass, finalize the block
+  kModeErrRow = 1u << 3,     // accumulate the row error of the new potential
+  kModeApply = 1u << 4,      // out_vec = apply_lse_kernel
+  kModeMarginal = 1u << 5,   // out_vec = marginal_from_potentials
+  kModeCheckDone = 1u << 6,  // return at once when state->done is set
+};
+
+// Loop state shared by the kernels of one solve (device memory).
+struct SolveState {
+  int32_t done;        // set by the block finalizer when the loop must stop
+  int32_t block;       // index of the next block whose error is finalized
+  int32_t n_blocks;    // blocks run when the loop stopped
+  int32_t pad0;
+  uint32_t tiles_done; // arrival counter of the column tiles of an error pass
+  uint32_t pad1[3];
+  double total_mass;   // sum of the coupling at the last error pass
+  double last_err;     // last raw error (before the inf/record rule)
+};
+
+// Arguments of one half-step launch: the out side gets the new potential, the
+// in side is streamed.  For the g-update out = y (m columns), in = x (n rows).
+struct HalfStepArgs {
+  const void* in_frag;
+  const float* in_pot;
+  const float* in_norm;
+  const void* out_frag;
+  float* out_pot;       // read (old) and written (new) when kModeWritePot
+  float* out_save;      // kModeSaveOld target
+  float* out_vec;       // kModeApply / kModeMarginal target
+  const float* out_norm;
+  const float* out_w;   // weights of the out side
+  const float* out_logw; // optional log-marginal replacing log(out_w) in the update
+  int64_t n_in;
+  int64_t n_out;
+  int32_t n_in_rb;      // padded in rows / 16
+  int32_t rb_per_split; // row blocks streamed by one split (multiple of MT)
+  int32_t splits;
+  int32_t tiles;        // column tiles of this launch
+  uint32_t mode;
+  int32_t norm_p;
+  float inv2s;          // 1 / (2 s)
+  float two_s;          // 2 s
+  float c;              // 2 s log2(e) / eps
+  float eps;            // epsilon of this launch's exponent
+  float eps_marg;       // epsilon of the marginal (target epsilon)
+  float tau;            // tau of the out side
+  float inv_rho;        // 1 / rho_out (0 when tau == 1)
+  float momentum;       // constant momentum weight
+  int32_t balanced;
+  float2* partial;      // [splits][n_out_pad] running (max, sum) per column
+  uint32_t* tile_counter;   // [tiles]
+  double* err_partial;  // [tiles] column (or row) error partials
+  double* mass_partial; // [tiles]
+  // block finalizer (kModeErrCol)
+  SolveState* state;
+  float* errors;
+  const double* row_err_partial;
+  int32_t row_tiles;
+  int32_t inner;
+  int32_t min_it;
+  int32_t max_it;
+  int32_t outer;
+  int32_t force_scan;
+  float threshold;
+};
+
+}  // namespace ottx

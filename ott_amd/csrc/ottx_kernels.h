@@ -1,0 +1,354 @@
+// Device kernels of the point-cloud Sinkhorn path (gfx950 / CDNA4).
+//
+// The central kernel is the fused log-domain half-step.  For the g-update
+// (reference: geometry/geometry.py:376-398 -> pointcloud.py:74-83 ->
+// geometry.py:260-299, 423-442 with C from costs.py:554-561) it computes, for
+// every column j of the out side,
+//
+//   LSE_i((f_i - C_ij) / eps),   C_ij = s (|x_i|^2 + |y_j|^2 - 2 <x_i, y_j>)
+//
+// without materialising C: the cross term runs on MFMA with the row bias
+// beta_i = (f_i - s|x_i|^2) / (2 s) loaded as the accumulator input, so the
+// MFMA result is acc_ij = <x_i, y_j> + beta_i and the exponent in base 2 is
+// c * acc_ij with c = 2 s log2(e) / eps.  A streaming max / sum-of-exp2 per
+// column reduces the tile, wave shuffles combine lane groups, and a split-K
+// combine (last-arriving workgroup per column tile) merges row splits.  The
+// epilogue turns (max, sum) into the new potential, the marginal of the old
+// state and the float64 error terms, and the last column tile of an error
+// pass finalizes the block: errors[], convergence, the stop flag.
+#pragma once
+
+#include "ottx_common.h"
+
+namespace ottx {
+
+__device__ __forceinline__ float exp2_fast(float x) { return __builtin_amdgcn_exp2f(x); }
+
+__device__ __forceinline__ uint32_t f32_to_bf16_rne(float v) {
+  uint32_t u = __float_as_uint(v);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
+
+// Combine two (max, sum) pairs of base-2 scaled log-sum-exp accumulators.
+__device__ __forceinline__ void lse_merge(float& mx, float& sm, float om, float os, float c) {
+  const float mn = fmaxf(mx, om);
+  float s = 0.f;
+  if (mn != -INFINITY) s = sm * exp2_fast((mx - mn) * c) + os * exp2_fast((om - mn) * c);
+  mx = mn;
+  sm = s;
+}
+
+__device__ __forceinline__ double load_f64_coherent(const double* p) {
+  const uint64_t v = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double(static_cast<long long>(v));
+}
+
+__device__ __forceinline__ float2 load_f2_coherent(const float2* p) {
+  const uint64_t v = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  float2 r;
+  r.x = __uint_as_float(static_cast<uint32_t>(v));
+  r.y = __uint_as_float(static_cast<uint32_t>(v >> 32));
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Block finalizer: sinkhorn.py:914-931 (errors layout), :940-953 and
+// fixed_point_loop.py:63-70 (continue test), run by one thread of the last
+// column tile of an error pass.
+// ---------------------------------------------------------------------------
+__device__ inline void finalize_block(const HalfStepArgs& p) {
+  double col = 0.0, mass = 0.0;
+  for (int t = 0; t < p.tiles; ++t) {
+    col += load_f64_coherent(p.err_partial + t);
+    mass += load_f64_coherent(p.mass_partial + t);
+  }
+  const double inv_p = 1.0 / static_cast<double>(p.norm_p);
+  double err = (p.norm_p == 1) ? col : pow(col, inv_p);
+  if (!p.balanced) {
+    double row = 0.0;
+    for (int t = 0; t < p.row_tiles; ++t) row += p.row_err_partial[t];
+    err += (p.norm_p == 1) ? row : pow(row, inv_p);
+  }
+  SolveState* st = p.state;
+  const int k = st->block;
+  const int t_last = k * p.inner + p.inner - 1;
+  const bool record = (t_last >= p.min_it) || (t_last == p.max_it - 1);
+  const float e = record ? static_cast<float>(err) : INFINITY;
+  if (k < p.outer) p.errors[k] = e;
+  st->block = k + 1;
+  st->last_err = err;
+  st->total_mass = mass;
+  const int t_next = (k + 1) * p.inner;
+  bool cont;
+  if (p.force_scan) {
+    cont = (k + 1) < (p.max_it / p.inner);
+  } else {
+    cont = (t_next < p.max_it) &&
+           ((t_next < p.min_it) || (isfinite(e) && !(e < p.threshold)));
+  }
+  if (!cont) {
+    st->n_blocks = k + 1;
+    st->done = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The fused half-step.
+//   PREC 1 (bf16 split): KK = K blocks of 32 features, fragments [rb][kb][hi|lo][lane][8 bf16]
+//   PREC 0 (fp32)      : KK = k-steps of 4 features,  fragments [rb][kq][lane][4 f32]
+//   NB  = 16-column blocks per wave, MT = 16-row blocks per streamed tile,
+//   W   = waves per workgroup.
+// MFMA operand layouts (16x16 output): lane l holds A row (l % 16), B column
+// (l % 16), and output rows 4 (l / 16) + r, r < 4, of column l % 16.
+// ---------------------------------------------------------------------------
+template <int PREC, int KK, int NB, int MT, int W>
+__global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) {
+  if ((p.mode & kModeCheckDone) && p.state->done) return;
+  constexpr int NT = W * 64;
+  constexpr int COLS = W * NB * 16;
+  constexpr int KQ = (KK + 3) / 4;
+  __shared__ float2 s_ms[COLS];
+  __shared__ double s_red[2][NT];
+  __shared__ int s_last;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int tile = blockIdx.x;
+  const int split = blockIdx.y;
+  const int cb0 = (tile * W + wave) * NB;
+  const float c = p.c;
+
+  // Out-side operands stay in registers for the whole stream.
+  bf16x8 bh[PREC == 1 ? NB : 1][PREC == 1 ? KK : 1];
+  bf16x8 bl[PREC == 1 ? NB : 1][PREC == 1 ? KK : 1];
+  float4 bq[PREC == 0 ? NB : 1][PREC == 0 ? KQ : 1];
+  if constexpr (PREC == 1) {
+    const uint4* __restrict__ of = static_cast<const uint4*>(p.out_frag);
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int kb = 0; kb < KK; ++kb) {
+        const size_t idx = ((static_cast<size_t>(cb0 + nb) * KK + kb) * 2) * 64 + lane;
+        const uint4 h = of[idx];
+        const uint4 l = of[idx + 64];
+        bh[nb][kb] = __builtin_bit_cast(bf16x8, h);
+        bl[nb][kb] = __builtin_bit_cast(bf16x8, l);
+      }
+  } else {
+    const float4* __restrict__ of = static_cast<const float4*>(p.out_frag);
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int q = 0; q < KQ; ++q)
+        bq[nb][q] = of[(static_cast<size_t>(cb0 + nb) * KQ + q) * 64 + lane];
+  }
+
+  float mx[NB], sm[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    mx[nb] = -INFINITY;
+    sm[nb] = 0.f;
+  }
+
+  const int rb_begin = split * p.rb_per_split;
+  const int rb_end = min(rb_begin + p.rb_per_split, p.n_in_rb);
+  const int rsub = (lane >> 4) * 4;
+
+  for (int rb = rb_begin; rb < rb_end; rb += MT) {
+    f32x4 acc[MT][NB];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int64_t row0 = static_cast<int64_t>(rb + mt) * 16 + rsub;
+      f32x4 bias;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t i = row0 + r;
+        bias[r] = (i < p.n_in) ? (p.in_pot[i] - p.in_norm[i]) * p.inv2s : -INFINITY;
+      }
+      if constexpr (PREC == 1) {
+        const uint4* __restrict__ inf = static_cast<const uint4*>(p.in_frag);
+        bf16x8 ah[KK], al[KK];
+#pragma unroll
+        for (int kb = 0; kb < KK; ++kb) {
+          const size_t idx = ((static_cast<size_t>(rb + mt) * KK + kb) * 2) * 64 + lane;
+          ah[kb] = __builtin_bit_cast(bf16x8, inf[idx]);
+          al[kb] = __builtin_bit_cast(bf16x8, inf[idx + 64]);
+        }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], bh[nb][0], bias, 0, 0, 0);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], bl[nb][0], acc[mt][nb], 0, 0, 0);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0], bh[nb][0], acc[mt][nb], 0, 0, 0);
+#pragma unroll
+        for (int kb = 1; kb < KK; ++kb) {
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb)
+            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], bh[nb][kb], acc[mt][nb], 0, 0, 0);
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb)
+            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], bl[nb][kb], acc[mt][nb], 0, 0, 0);
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb)
+            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kb], bh[nb][kb], acc[mt][nb], 0, 0, 0);
+        }
+      } else {
+        const float4* __restrict__ inf = static_cast<const float4*>(p.in_frag);
+        float4 aq[KQ];
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) aq[q] = inf[(static_cast<size_t>(rb + mt) * KQ + q) * 64 + lane];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[0].x, bq[nb][0].x, bias, 0, 0, 0);
+#pragma unroll
+        for (int ks = 1; ks < KK; ++ks) {
+          const int q = ks >> 2, e = ks & 3;
+#pragma unroll
+          for (int nb = 0; nb < NB; ++nb) {
+            const float av = (e == 0) ? aq[q].x : (e == 1) ? aq[q].y : (e == 2) ? aq[q].z : aq[q].w;
+            const float bv = (e == 0) ? bq[nb][q].x : (e == 1) ? bq[nb][q].y : (e == 2) ? bq[nb][q].z : bq[nb][q].w;
+            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[mt][nb], 0, 0, 0);
+          }
+        }
+      }
+    }
+    // Streaming log-sum-exp, base 2: one rescale per column per tile.
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      float tmax = -INFINITY;
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tmax = fmaxf(tmax, acc[mt][nb][r]);
+      const float mnew = fmaxf(mx[nb], tmax);
+      if (mnew != -INFINITY) {
+        const float nmc = -c * mnew;
+        float s = sm[nb] * exp2_fast(fmaf(mx[nb], c, nmc));
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) s += exp2_fast(fmaf(acc[mt][nb][r], c, nmc));
+        sm[nb] = s;
+        mx[nb] = mnew;
+      }
+    }
+  }
+
+  // Combine the four 16-lane row groups of each column.
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+    for (int off = 16; off <= 32; off <<= 1) {
+      const float om = __shfl_xor(mx[nb], off, 64);
+      const float os = __shfl_xor(sm[nb], off, 64);
+      lse_merge(mx[nb], sm[nb], om, os, c);
+    }
+    if (lane < 16) s_ms[(wave * NB + nb) * 16 + lane] = make_float2(mx[nb], sm[nb]);
+  }
+  __syncthreads();
+
+  const int64_t col0 = static_cast<int64_t>(tile) * COLS;
+  if (p.splits > 1) {
+    const int64_t stride = static_cast<int64_t>(p.tiles) * COLS;
+    float2* part = p.partial + static_cast<int64_t>(split) * stride;
+    for (int t = tid; t < COLS; t += NT) part[col0 + t] = s_ms[t];
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+      const uint32_t prev = atomicAdd(p.tile_counter + tile, 1u);
+      s_last = (prev == static_cast<uint32_t>(p.splits - 1)) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    for (int t = tid; t < COLS; t += NT) {
+      float M = -INFINITY, S = 0.f;
+      for (int sp = 0; sp < p.splits; ++sp) {
+        const float2 v = load_f2_coherent(p.partial + sp * stride + col0 + t);
+        lse_merge(M, S, v.x, v.y, c);
+      }
+      s_ms[t] = make_float2(M, S);
+    }
+    if (tid == 0) p.tile_counter[tile] = 0u;
+    __syncthreads();
+  }
+
+  // Epilogue: one column per thread.
+  const uint32_t mode = p.mode;
+  const bool want_err = (mode & (kModeErrCol | kModeErrRow)) != 0;
+  double e_acc = 0.0, m_acc = 0.0;
+  for (int t = tid; t < COLS; t += NT) {
+    const int64_t j = col0 + t;
+    if (j >= p.n_out) continue;
+    const float2 ms = s_ms[t];
+    float app;
+    if (ms.x == -INFINITY || !(ms.y > 0.f)) {
+      app = (ms.y != ms.y) ? ms.y : -INFINITY;
+    } else {
+      app = fmaf(p.two_s, ms.x, -p.out_norm[j]) + p.eps * logf(ms.y);
+    }
+    const float h_old = p.out_pot[j];
+    float h_marg = h_old;
+    if (mode & kModeWritePot) {
+      const float lw = p.out_logw ? p.out_logw[j] : logf(p.out_w[j]);
+      float upd = p.eps * lw - (isfinite(app) ? app : 0.f);
+      upd *= p.tau;
+      float nv = upd;
+      if (p.momentum != 1.f) nv = (1.f - p.momentum) * (isfinite(h_old) ? h_old : 0.f) + p.momentum * upd;
+      p.out_pot[j] = nv;
+      if (mode & kModeErrRow) h_marg = nv;
+    }
+    if (mode & kModeSaveOld) p.out_save[j] = h_old;
+    if (want_err) {
+      const float marg = expf((h_marg + app) / p.eps_marg);
+      const float w = p.out_w[j];
+      const float target = (p.tau == 1.f) ? w : ((w > 0.f) ? w * expf(-h_marg * p.inv_rho) : 0.f);
+      const double diff = fabs(static_cast<double>(marg) - static_cast<double>(target));
+      e_acc += (p.norm_p == 1) ? diff : pow(diff, static_cast<double>(p.norm_p));
+      m_acc += static_cast<double>(marg);
+    }
+    if (mode & kModeApply) p.out_vec[j] = (h_old == -INFINITY || h_old != h_old) ? h_old : app;
+    if (mode & kModeMarginal) p.out_vec[j] = expf((h_old + app) / p.eps_marg);
+  }
+
+  if (want_err) {
+    s_red[0][tid] = e_acc;
+    s_red[1][tid] = m_acc;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s > 0; s >>= 1) {
+      if (tid < s) {
+        s_red[0][tid] += s_red[0][tid + s];
+        s_red[1][tid] += s_red[1][tid + s];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      p.err_partial[tile] = s_red[0][0];
+      if (p.mass_partial) p.mass_partial[tile] = s_red[1][0];
+    }
+    if ((mode & kModeErrCol) && p.state != nullptr) {
+      __threadfence();
+      __syncthreads();
+      if (tid == 0) {
+        const uint32_t prev = atomicAdd(&p.state->tiles_done, 1u);
+        if (prev == static_cast<uint32_t>(p.tiles - 1)) {
+          __threadfence();
+          finalize_block(p);
+          p.state->tiles_done = 0u;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace ottx

@@ -1,0 +1,801 @@
+// ottx: log-domain Sinkhorn on squared-Euclidean point clouds for gfx950.
+// Host side of the C ABI declared in include/ottx.h plus the small kernels
+// around the fused half-step (statistics, packing, loop state, dual terms).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "ottx_common.h"
+#include "ottx_kernels.h"
+
+namespace ottx {
+
+// ---------------------------------------------------------------------------
+// Small kernels
+// ---------------------------------------------------------------------------
+
+// Per-chunk, per-feature float64 sums of x and x^2 (deterministic two-stage
+// reduction for mean(C), pointcloud.py:170-186, and for the centring shift).
+__global__ void stats_partial_kernel(const float* __restrict__ pts, int64_t np, int d,
+                                     int64_t rows_per_chunk, double* __restrict__ part) {
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_chunk;
+  const int64_t r1 = min(np, r0 + rows_per_chunk);
+  for (int k = threadIdx.x; k < d; k += blockDim.x) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int64_t r = r0; r < r1; ++r) {
+      const double v = pts[r * d + k];
+      s1 += v;
+      s2 += v * v;
+    }
+    part[(static_cast<int64_t>(blockIdx.x) * d + k) * 2 + 0] = s1;
+    part[(static_cast<int64_t>(blockIdx.x) * d + k) * 2 + 1] = s2;
+  }
+}
+
+// One workgroup: combine the chunks, write mean(C) (out[0]) and the common
+// centring shift mu = (mean x + mean y) / 2 (float, [d]).
+__global__ void stats_final_kernel(const double* __restrict__ px, int cx, const double* __restrict__ py,
+                                   int cy, int d, int64_t n, int64_t m, double s,
+                                   double* __restrict__ out, float* __restrict__ mu) {
+  __shared__ double red[3][256];
+  double nx2 = 0.0, ny2 = 0.0, dot = 0.0;
+  for (int k = threadIdx.x; k < d; k += blockDim.x) {
+    double sx1 = 0.0, sx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
+    for (int c = 0; c < cx; ++c) {
+      sx1 += px[(static_cast<int64_t>(c) * d + k) * 2];
+      sx2 += px[(static_cast<int64_t>(c) * d + k) * 2 + 1];
+    }
+    for (int c = 0; c < cy; ++c) {
+      sy1 += py[(static_cast<int64_t>(c) * d + k) * 2];
+      sy2 += py[(static_cast<int64_t>(c) * d + k) * 2 + 1];
+    }
+    const double mxk = sx1 / static_cast<double>(n);
+    const double myk = sy1 / static_cast<double>(m);
+    if (mu) mu[k] = static_cast<float>(0.5 * (mxk + myk));
+    nx2 += sx2;
+    ny2 += sy2;
+    dot += mxk * myk;
+  }
+  red[0][threadIdx.x] = nx2;
+  red[1][threadIdx.x] = ny2;
+  red[2][threadIdx.x] = dot;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s * (red[0][0] / static_cast<double>(n) + red[1][0] / static_cast<double>(m) - 2.0 * red[2][0]);
+  }
+}
+
+// bf16 hi/lo fragments: [rb][kb][hi|lo][lane][8], lane -> row rb*16 + lane%16,
+// features kb*32 + 8*(lane/16) + e.
+__global__ void pack_bf16_kernel(const float* __restrict__ pts, int64_t np, int d,
+                                 const float* __restrict__ mu, int KB, int64_t n_threads,
+                                 uint4* __restrict__ frag) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n_threads) return;
+  const int lane = static_cast<int>(gid & 63);
+  const int64_t chunk = gid >> 6;
+  const int kb = static_cast<int>(chunk % KB);
+  const int64_t rb = chunk / KB;
+  const int64_t row = rb * 16 + (lane & 15);
+  const int k0 = kb * 32 + 8 * (lane >> 4);
+  uint32_t hw[4], lw[4];
+#pragma unroll
+  for (int e = 0; e < 8; e += 2) {
+    uint32_t h2[2], l2[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int k = k0 + e + q;
+      const float v = (row < np && k < d) ? pts[row * d + k] - mu[k] : 0.f;
+      const uint32_t h = f32_to_bf16_rne(v);
+      const uint32_t l = f32_to_bf16_rne(v - bf16_to_f32(h));
+      h2[q] = h;
+      l2[q] = l;
+    }
+    hw[e / 2] = h2[0] | (h2[1] << 16);
+    lw[e / 2] = l2[0] | (l2[1] << 16);
+  }
+  frag[(chunk * 2 + 0) * 64 + lane] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+  frag[(chunk * 2 + 1) * 64 + lane] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+}
+
+// fp32 fragments: [rb][kq][lane][4], element r -> feature (4 kq + r) * 4 + lane/16.
+__global__ void pack_f32_kernel(const float* __restrict__ pts, int64_t np, int d,
+                                const float* __restrict__ mu, int KQ, int64_t n_threads,
+                                float4* __restrict__ frag) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n_threads) return;
+  const int lane = static_cast<int>(gid & 63);
+  const int64_t chunk = gid >> 6;
+  const int kq = static_cast<int>(chunk % KQ);
+  const int64_t rb = chunk / KQ;
+  const int64_t row = rb * 16 + (lane & 15);
+  float v[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int k = (4 * kq + r) * 4 + (lane >> 4);
+    v[r] = (row < np && k < d) ? pts[row * d + k] - mu[k] : 0.f;
+  }
+  frag[chunk * 64 + lane] = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// s * |p - mu|^2 per point (float64 accumulation), zero on padded rows.
+__global__ void norms_kernel(const float* __restrict__ pts, int64_t np, int d, const float* __restrict__ mu,
+                             double s, int64_t np_pad, float* __restrict__ norm) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= np_pad) return;
+  if (i >= np) {
+    norm[i] = 0.f;
+    return;
+  }
+  double acc = 0.0;
+  for (int k = 0; k < d; ++k) {
+    const double v = static_cast<double>(pts[i * d + k] - mu[k]);
+    acc += v * v;
+  }
+  norm[i] = static_cast<float>(s * acc);
+}
+
+// Loop state reset: errors = -1 (sinkhorn.py:964-972), counters = 0.
+__global__ void init_state_kernel(SolveState* st, float* errors, int outer, uint32_t* counters, int n_counters) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    st->done = 0;
+    st->block = 0;
+    st->n_blocks = 0;
+    st->tiles_done = 0u;
+    st->total_mass = 0.0;
+    st->last_err = 0.0;
+  }
+  for (int k = i; k < outer; k += gridDim.x * blockDim.x) errors[k] = -1.f;
+  for (int k = i; k < n_counters; k += gridDim.x * blockDim.x) counters[k] = 0u;
+}
+
+__global__ void copy_f32_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Float64 dual terms of compute_kl_reg_cost (sinkhorn.py:224-277) and the
+// errors summary of output_from_state (sinkhorn.py:1020-1023, 431-434).
+//   block 0: out[0] = div_a, out[1] = sum a;  block 1: out[2] = div_b, out[3] = sum b
+//   block 2: out[4] = any NaN in errors, out[5] = #(errors != -1), out[6] = errors[-1]
+__global__ void dual_terms_kernel(const float* __restrict__ f, const float* __restrict__ a, int64_t n,
+                                  const float* __restrict__ g, const float* __restrict__ b, int64_t m,
+                                  double eps, double tau_a, double tau_b, const float* __restrict__ errors,
+                                  int outer, double* __restrict__ out) {
+  __shared__ double red[2][256];
+  const int tid = threadIdx.x;
+  double s0 = 0.0, s1 = 0.0;
+  if (blockIdx.x < 2) {
+    const float* pot = blockIdx.x == 0 ? f : g;
+    const float* w = blockIdx.x == 0 ? a : b;
+    const int64_t len = blockIdx.x == 0 ? n : m;
+    const double tau = blockIdx.x == 0 ? tau_a : tau_b;
+    const double rho = eps * tau / (1.0 - tau);
+    for (int64_t i = tid; i < len; i += blockDim.x) {
+      const double wi = w[i];
+      s1 += wi;
+      if (wi > 0.0) {
+        const double h = static_cast<double>(pot[i]) - eps * log(wi);
+        if (tau == 1.0) {
+          s0 += wi * h;
+        } else {
+          s0 -= wi * rho * (exp(-h / rho) - 1.0);
+        }
+      }
+    }
+  } else {
+    for (int i = tid; i < outer; i += blockDim.x) {
+      const float e = errors[i];
+      if (e != e) s0 += 1.0;
+      if (e != -1.f) s1 += 1.0;
+    }
+  }
+  red[0][tid] = s0;
+  red[1][tid] = s1;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) {
+      red[0][tid] += red[0][tid + st];
+      red[1][tid] += red[1][tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (blockIdx.x < 2) {
+      out[blockIdx.x * 2 + 0] = red[0][0];
+      out[blockIdx.x * 2 + 1] = red[1][0];
+    } else {
+      out[4] = red[0][0];
+      out[5] = red[1][0];
+      out[6] = outer > 0 ? static_cast<double>(errors[outer - 1]) : 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel selection
+// ---------------------------------------------------------------------------
+typedef void (*HalfStepFn)(const HalfStepArgs);
+
+struct KernelCfg {
+  int prec;
+  int kk;       // bf16: K blocks of 32; fp32: k-steps of 4
+  int nb, mt, w;
+  HalfStepFn fn;
+  int cols() const { return 16 * nb * w; }
+  int frag_chunks() const { return prec == 1 ? kk : (kk + 3) / 4; }  // 1 KiB chunks per 16 rows
+};
+
+template <int PREC, int KK, int NB, int MT, int W>
+KernelCfg make_cfg() {
+  return KernelCfg{PREC, KK, NB, MT, W, &halfstep_kernel<PREC, KK, NB, MT, W>};
+}
+
+bool select_kernel(int prec, int d, KernelCfg* out) {
+  if (d < 1 || d > 256) return false;
+  if (prec == OTTX_PREC_BF16_SPLIT) {
+    const int kb = (d + 31) / 32;
+    switch (kb) {
+      case 1: *out = make_cfg<1, 1, 4, 4, 4>(); return true;
+      case 2: *out = make_cfg<1, 2, 4, 4, 4>(); return true;
+      case 3: *out = make_cfg<1, 3, 2, 4, 4>(); return true;
+      case 4: *out = make_cfg<1, 4, 2, 4, 4>(); return true;
+      case 5: *out = make_cfg<1, 5, 1, 4, 4>(); return true;
+      case 6: *out = make_cfg<1, 6, 1, 4, 4>(); return true;
+      case 7: *out = make_cfg<1, 7, 1, 4, 4>(); return true;
+      case 8: *out = make_cfg<1, 8, 1, 4, 4>(); return true;
+      default: return false;
+    }
+  }
+  if (prec == OTTX_PREC_FP32) {
+    const int ks = (d + 3) / 4;
+    if (ks <= 1) { *out = make_cfg<0, 1, 4, 4, 4>(); return true; }
+    if (ks <= 2) { *out = make_cfg<0, 2, 4, 4, 4>(); return true; }
+    if (ks <= 4) { *out = make_cfg<0, 4, 4, 4, 4>(); return true; }
+    if (ks <= 8) { *out = make_cfg<0, 8, 4, 4, 4>(); return true; }
+    if (ks <= 12) { *out = make_cfg<0, 12, 4, 4, 4>(); return true; }
+    if (ks <= 16) { *out = make_cfg<0, 16, 4, 4, 4>(); return true; }
+    if (ks <= 24) { *out = make_cfg<0, 24, 2, 4, 4>(); return true; }
+    if (ks <= 32) { *out = make_cfg<0, 32, 2, 4, 4>(); return true; }
+    if (ks <= 48) { *out = make_cfg<0, 48, 1, 4, 4>(); return true; }
+    if (ks <= 64) { *out = make_cfg<0, 64, 1, 4, 4>(); return true; }
+    return false;
+  }
+  return false;
+}
+
+int device_cus() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) return 256;
+  return v;
+}
+
+int64_t pad_rows(int64_t n) { return ((n + kRowPad - 1) / kRowPad) * kRowPad; }
+
+struct PassPlan {
+  int tiles = 0;
+  int splits = 1;
+  int rb_per_split = 0;
+  int n_in_rb = 0;
+};
+
+PassPlan plan_pass(const KernelCfg& k, int64_t n_in, int64_t n_out, int cus) {
+  PassPlan p;
+  const int64_t in_pad = pad_rows(n_in), out_pad = pad_rows(n_out);
+  p.tiles = static_cast<int>(out_pad / k.cols());
+  p.n_in_rb = static_cast<int>(in_pad / 16);
+  const int in_tiles = p.n_in_rb / k.mt;
+  const int target = 4 * cus;
+  int splits = (target + p.tiles - 1) / p.tiles;
+  splits = std::max(1, std::min(splits, in_tiles));
+  const int tiles_per_split = (in_tiles + splits - 1) / splits;
+  p.rb_per_split = tiles_per_split * k.mt;
+  p.splits = (p.n_in_rb + p.rb_per_split - 1) / p.rb_per_split;
+  return p;
+}
+
+size_t align_up(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+struct Layout {
+  size_t frag_x, frag_y, norm_x, norm_y, g_save, partial, counters, err_col, mass, err_row, state,
+      dual, stats_x, stats_y, stats_out, mu, total;
+  int chunks_x, chunks_y;
+  int64_t rows_chunk_x, rows_chunk_y;
+};
+
+void stats_chunks(int64_t np, int* chunks, int64_t* rows) {
+  int64_t r = std::max<int64_t>(64, (np + 255) / 256);
+  *rows = r;
+  *chunks = static_cast<int>((np + r - 1) / r);
+}
+
+Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& gp, const PassPlan& fp, int outer) {
+  Layout L{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align_up(off + bytes);
+    return at;
+  };
+  const int64_t n_pad = pad_rows(pb->n), m_pad = pad_rows(pb->m);
+  const size_t chunk_bytes = static_cast<size_t>(k.frag_chunks()) * (k.prec == 1 ? 2048 : 1024);
+  L.frag_x = take(static_cast<size_t>(n_pad / 16) * chunk_bytes);
+  L.frag_y = take(static_cast<size_t>(m_pad / 16) * chunk_bytes);
+  L.norm_x = take(n_pad * sizeof(float));
+  L.norm_y = take(m_pad * sizeof(float));
+  L.g_save = take(m_pad * sizeof(float));
+  const size_t part_g = static_cast<size_t>(gp.splits) * gp.tiles * k.cols();
+  const size_t part_f = static_cast<size_t>(fp.splits) * fp.tiles * k.cols();
+  L.partial = take(std::max(part_g, part_f) * sizeof(float2));
+  const int max_tiles = std::max(gp.tiles, fp.tiles);
+  L.counters = take(static_cast<size_t>(max_tiles) * sizeof(uint32_t));
+  L.err_col = take(static_cast<size_t>(max_tiles) * sizeof(double));
+  L.mass = take(static_cast<size_t>(max_tiles) * sizeof(double));
+  L.err_row = take(static_cast<size_t>(max_tiles) * sizeof(double));
+  L.state = take(sizeof(SolveState));
+  L.dual = take(16 * sizeof(double));
+  stats_chunks(pb->n, &L.chunks_x, &L.rows_chunk_x);
+  stats_chunks(pb->m, &L.chunks_y, &L.rows_chunk_y);
+  L.stats_x = take(static_cast<size_t>(L.chunks_x) * pb->d * 2 * sizeof(double));
+  L.stats_y = take(static_cast<size_t>(L.chunks_y) * pb->d * 2 * sizeof(double));
+  L.stats_out = take(8 * sizeof(double));
+  L.mu = take(static_cast<size_t>(pb->d) * sizeof(float));
+  (void)outer;
+  L.total = off;
+  return L;
+}
+
+int check_problem(const ottx_problem* pb) {
+  if (!pb || pb->n <= 0 || pb->m <= 0 || pb->d <= 0 || pb->d > 256) return OTTX_EINVAL;
+  if (pb->n > (int64_t{1} << 30) || pb->m > (int64_t{1} << 30)) return OTTX_EINVAL;
+  if (!pb->x || !pb->y || !pb->a || !pb->b) return OTTX_EINVAL;
+  if (!(pb->epsilon > 0.0) || !(pb->inv_scale_cost > 0.0)) return OTTX_EINVAL;
+  if (!(pb->tau_a > 0.0 && pb->tau_a <= 1.0) || !(pb->tau_b > 0.0 && pb->tau_b <= 1.0)) return OTTX_EINVAL;
+  if (pb->precision != OTTX_PREC_FP32 && pb->precision != OTTX_PREC_BF16_SPLIT) return OTTX_EINVAL;
+  return OTTX_OK;
+}
+
+#define OTTX_HIP(call)                              \
+  do {                                              \
+    if ((call) != hipSuccess) return OTTX_EHIP;     \
+  } while (0)
+
+#define OTTX_LAUNCHED()                                  \
+  do {                                                   \
+    if (hipGetLastError() != hipSuccess) return OTTX_EHIP; \
+  } while (0)
+
+struct Prepared {
+  char* base;
+  Layout L;
+  void* frag_x;
+  void* frag_y;
+  float* norm_x;
+  float* norm_y;
+  float* mu;
+  double* stats_out;
+};
+
+// Statistics, centring shift, fragments and norms of both clouds.
+int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* ws, hipStream_t st,
+            Prepared* P) {
+  char* base = static_cast<char*>(ws);
+  P->base = base;
+  P->L = L;
+  P->frag_x = base + L.frag_x;
+  P->frag_y = base + L.frag_y;
+  P->norm_x = reinterpret_cast<float*>(base + L.norm_x);
+  P->norm_y = reinterpret_cast<float*>(base + L.norm_y);
+  P->mu = reinterpret_cast<float*>(base + L.mu);
+  P->stats_out = reinterpret_cast<double*>(base + L.stats_out);
+  double* sx = reinterpret_cast<double*>(base + L.stats_x);
+  double* sy = reinterpret_cast<double*>(base + L.stats_y);
+  hipLaunchKernelGGL(stats_partial_kernel, dim3(L.chunks_x), dim3(256), 0, st, pb->x, pb->n, pb->d,
+                     L.rows_chunk_x, sx);
+  hipLaunchKernelGGL(stats_partial_kernel, dim3(L.chunks_y), dim3(256), 0, st, pb->y, pb->m, pb->d,
+                     L.rows_chunk_y, sy);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, st, sx, L.chunks_x, sy, L.chunks_y, pb->d,
+                     pb->n, pb->m, pb->inv_scale_cost, P->stats_out, P->mu);
+  OTTX_LAUNCHED();
+  const int64_t n_pad = pad_rows(pb->n), m_pad = pad_rows(pb->m);
+  const int chunks = k.frag_chunks();
+  for (int side = 0; side < 2; ++side) {
+    const float* pts = side == 0 ? pb->x : pb->y;
+    const int64_t np = side == 0 ? pb->n : pb->m;
+    const int64_t np_pad = side == 0 ? n_pad : m_pad;
+    void* frag = side == 0 ? P->frag_x : P->frag_y;
+    float* norm = side == 0 ? P->norm_x : P->norm_y;
+    const int64_t n_threads = (np_pad / 16) * chunks * 64;
+    const int blocks = static_cast<int>((n_threads + 255) / 256);
+    if (k.prec == 1) {
+      hipLaunchKernelGGL(pack_bf16_kernel, dim3(blocks), dim3(256), 0, st, pts, np, pb->d, P->mu, chunks,
+                         n_threads, static_cast<uint4*>(frag));
+    } else {
+      hipLaunchKernelGGL(pack_f32_kernel, dim3(blocks), dim3(256), 0, st, pts, np, pb->d, P->mu, chunks,
+                         n_threads, static_cast<float4*>(frag));
+    }
+    hipLaunchKernelGGL(norms_kernel, dim3(static_cast<int>((np_pad + 255) / 256)), dim3(256), 0, st, pts, np,
+                       pb->d, P->mu, pb->inv_scale_cost, np_pad, norm);
+    OTTX_LAUNCHED();
+  }
+  return OTTX_OK;
+}
+
+// Arguments common to every half-step of one problem.  side_out = 1 for the
+// g-update (out = y), 0 for the f-update (out = x).
+HalfStepArgs base_args(const ottx_problem* pb, const KernelCfg& k, const PassPlan& plan, const Prepared& P,
+                       int side_out, float* in_pot, float* out_pot, double eps) {
+  HalfStepArgs a{};
+  const bool g_upd = side_out == 1;
+  a.in_frag = g_upd ? P.frag_x : P.frag_y;
+  a.in_pot = in_pot;
+  a.in_norm = g_upd ? P.norm_x : P.norm_y;
+  a.out_frag = g_upd ? P.frag_y : P.frag_x;
+  a.out_pot = out_pot;
+  a.out_norm = g_upd ? P.norm_y : P.norm_x;
+  a.out_w = g_upd ? pb->b : pb->a;
+  a.n_in = g_upd ? pb->n : pb->m;
+  a.n_out = g_upd ? pb->m : pb->n;
+  a.n_in_rb = plan.n_in_rb;
+  a.rb_per_split = plan.rb_per_split;
+  a.splits = plan.splits;
+  a.tiles = plan.tiles;
+  a.norm_p = 1;
+  const double s = pb->inv_scale_cost;
+  a.inv2s = static_cast<float>(1.0 / (2.0 * s));
+  a.two_s = static_cast<float>(2.0 * s);
+  a.c = static_cast<float>(2.0 * s * 1.4426950408889634 / eps);
+  a.eps = static_cast<float>(eps);
+  a.eps_marg = static_cast<float>(pb->epsilon);
+  const double tau = g_upd ? pb->tau_b : pb->tau_a;
+  a.tau = static_cast<float>(tau);
+  a.inv_rho = tau == 1.0 ? 0.f : static_cast<float>((1.0 - tau) / (pb->epsilon * tau));
+  a.momentum = 1.f;
+  a.balanced = (pb->tau_a == 1.0 && pb->tau_b == 1.0) ? 1 : 0;
+  char* base = P.base;
+  a.partial = reinterpret_cast<float2*>(base + P.L.partial);
+  a.tile_counter = reinterpret_cast<uint32_t*>(base + P.L.counters);
+  a.err_partial = reinterpret_cast<double*>(base + (g_upd ? P.L.err_col : P.L.err_row));
+  a.mass_partial = g_upd ? reinterpret_cast<double*>(base + P.L.mass) : nullptr;
+  a.state = reinterpret_cast<SolveState*>(base + P.L.state);
+  (void)k;
+  return a;
+}
+
+int launch_halfstep(const KernelCfg& k, const HalfStepArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k.fn, dim3(a.tiles, a.splits), dim3(64 * k.w), 0, st, a);
+  OTTX_LAUNCHED();
+  return OTTX_OK;
+}
+
+// Per-thread host helpers: a non-blocking stream for status reads and a
+// pinned mailbox they land in.
+struct HostHelpers {
+  hipStream_t side = nullptr;
+  void* mailbox = nullptr;
+  int device = -1;
+};
+
+int host_helpers(HostHelpers** out) {
+  static thread_local HostHelpers h;
+  int dev = 0;
+  OTTX_HIP(hipGetDevice(&dev));
+  if (h.device != dev) {
+    if (h.side) (void)hipStreamDestroy(h.side);
+    if (h.mailbox) (void)hipHostFree(h.mailbox);
+    h.side = nullptr;
+    h.mailbox = nullptr;
+    OTTX_HIP(hipStreamCreateWithFlags(&h.side, hipStreamNonBlocking));
+    OTTX_HIP(hipHostMalloc(&h.mailbox, 4096, hipHostMallocDefault));
+    h.device = dev;
+  }
+  *out = &h;
+  return OTTX_OK;
+}
+
+}  // namespace ottx
+
+using namespace ottx;
+
+extern "C" {
+
+int ottx_abi_version(void) { return OTTX_ABI_VERSION; }
+
+const char* ottx_strerror(int status) {
+  switch (status) {
+    case OTTX_OK: return "ok";
+    case OTTX_EINVAL: return "invalid argument";
+    case OTTX_EUNSUPPORTED: return "unsupported option";
+    case OTTX_EHIP: return "HIP runtime error";
+    case OTTX_EWORKSPACE: return "workspace too small or misaligned";
+    default: return "unknown status";
+  }
+}
+
+size_t ottx_mean_cost_workspace_bytes(int64_t n, int64_t m, int32_t d) {
+  if (n <= 0 || m <= 0 || d <= 0) return 0;
+  int cx, cy;
+  int64_t rx, ry;
+  stats_chunks(n, &cx, &rx);
+  stats_chunks(m, &cy, &ry);
+  return align_up(static_cast<size_t>(cx) * d * 16) + align_up(static_cast<size_t>(cy) * d * 16) +
+         align_up(8 * sizeof(double));
+}
+
+int ottx_mean_cost(int64_t n, int64_t m, int32_t d, const float* x, const float* y, double inv_scale_cost,
+                   double* out_host, void* workspace, size_t workspace_bytes, void* stream) {
+  if (n <= 0 || m <= 0 || d <= 0 || !x || !y || !out_host || !workspace) return OTTX_EINVAL;
+  if (workspace_bytes < ottx_mean_cost_workspace_bytes(n, m, d)) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int cx, cy;
+  int64_t rx, ry;
+  stats_chunks(n, &cx, &rx);
+  stats_chunks(m, &cy, &ry);
+  char* base = static_cast<char*>(workspace);
+  double* sx = reinterpret_cast<double*>(base);
+  double* sy = reinterpret_cast<double*>(base + align_up(static_cast<size_t>(cx) * d * 16));
+  double* o = reinterpret_cast<double*>(base + align_up(static_cast<size_t>(cx) * d * 16) +
+                                        align_up(static_cast<size_t>(cy) * d * 16));
+  hipLaunchKernelGGL(stats_partial_kernel, dim3(cx), dim3(256), 0, st, x, n, d, rx, sx);
+  hipLaunchKernelGGL(stats_partial_kernel, dim3(cy), dim3(256), 0, st, y, m, d, ry, sy);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, st, sx, cx, sy, cy, d, n, m, inv_scale_cost, o,
+                     static_cast<float*>(nullptr));
+  OTTX_LAUNCHED();
+  OTTX_HIP(hipMemcpyAsync(out_host, o, sizeof(double), hipMemcpyDeviceToHost, st));
+  OTTX_HIP(hipStreamSynchronize(st));
+  return OTTX_OK;
+}
+
+size_t ottx_op_workspace_bytes(const ottx_problem* pb) {
+  if (check_problem(pb) != OTTX_OK) return 0;
+  KernelCfg k;
+  if (!select_kernel(pb->precision, pb->d, &k)) return 0;
+  const int cus = device_cus();
+  const PassPlan gp = plan_pass(k, pb->n, pb->m, cus);
+  const PassPlan fp = plan_pass(k, pb->m, pb->n, cus);
+  return make_layout(pb, k, gp, fp, 1).total;
+}
+
+int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const float* f, const float* g,
+                       const float* log_marginal, double eps, float* out, void* workspace, size_t workspace_bytes,
+                       void* stream) {
+  int rc = check_problem(pb);
+  if (rc != OTTX_OK) return rc;
+  if (!f || !g || !out || !workspace || !(eps > 0.0)) return OTTX_EINVAL;
+  if (axis != 0 && axis != 1) return OTTX_EINVAL;
+  if (op != OTTX_OP_APPLY_LSE && op != OTTX_OP_UPDATE && op != OTTX_OP_MARGINAL) return OTTX_EINVAL;
+  KernelCfg k;
+  if (!select_kernel(pb->precision, pb->d, &k)) return OTTX_EUNSUPPORTED;
+  const int cus = device_cus();
+  const PassPlan gp = plan_pass(k, pb->n, pb->m, cus);
+  const PassPlan fp = plan_pass(k, pb->m, pb->n, cus);
+  const Layout L = make_layout(pb, k, gp, fp, 1);
+  if (workspace_bytes < L.total) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Prepared P;
+  rc = prepare(pb, k, L, workspace, st, &P);
+  if (rc != OTTX_OK) return rc;
+  uint32_t* counters = reinterpret_cast<uint32_t*>(P.base + L.counters);
+  SolveState* state = reinterpret_cast<SolveState*>(P.base + L.state);
+  float* dummy_err = reinterpret_cast<float*>(P.base + L.dual);
+  hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(256), 0, st, state, dummy_err, 0, counters,
+                     std::max(gp.tiles, fp.tiles));
+  OTTX_LAUNCHED();
+  const bool g_upd = axis == 0;
+  const PassPlan& plan = g_upd ? gp : fp;
+  const float* h = g_upd ? g : f;
+  const float* in_pot = g_upd ? f : g;
+  HalfStepArgs a = base_args(pb, k, plan, P, g_upd ? 1 : 0, const_cast<float*>(in_pot), const_cast<float*>(h), eps);
+  a.state = nullptr;
+  const int64_t n_out = g_upd ? pb->m : pb->n;
+  if (op == OTTX_OP_UPDATE) {
+    hipLaunchKernelGGL(copy_f32_kernel, dim3(static_cast<int>((n_out + 255) / 256)), dim3(256), 0, st, h, out,
+                       n_out);
+    OTTX_LAUNCHED();
+    if (!log_marginal) return OTTX_EINVAL;
+    a.out_pot = out;
+    a.out_logw = log_marginal;
+    a.tau = 1.f;
+    a.mode = kModeWritePot;
+  } else if (op == OTTX_OP_APPLY_LSE) {
+    a.out_vec = out;
+    a.mode = kModeApply;
+  } else {
+    a.out_vec = out;
+    a.eps_marg = static_cast<float>(eps);
+    a.mode = kModeMarginal;
+  }
+  return launch_halfstep(k, a, st);
+}
+
+size_t ottx_sinkhorn_workspace_bytes(const ottx_problem* pb, const ottx_sinkhorn_opts* o) {
+  if (check_problem(pb) != OTTX_OK || !o) return 0;
+  KernelCfg k;
+  if (!select_kernel(pb->precision, pb->d, &k)) return 0;
+  const int cus = device_cus();
+  const PassPlan gp = plan_pass(k, pb->n, pb->m, cus);
+  const PassPlan fp = plan_pass(k, pb->m, pb->n, cus);
+  return make_layout(pb, k, gp, fp, 1).total;
+}
+
+int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ottx_sinkhorn_out* out, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+  int rc = check_problem(pb);
+  if (rc != OTTX_OK) return rc;
+  if (!o || !out || !out->f || !out->g || !out->errors || !workspace) return OTTX_EINVAL;
+  if (o->inner_iterations < 1 || o->max_iterations < 0 || o->min_iterations < 0 || o->norm_error < 1)
+    return OTTX_EINVAL;
+  if (!(o->momentum > 0.0 && o->momentum < 2.0)) return OTTX_EINVAL;
+  KernelCfg k;
+  if (!select_kernel(pb->precision, pb->d, &k)) return OTTX_EUNSUPPORTED;
+  const int cus = device_cus();
+  const PassPlan gp = plan_pass(k, pb->n, pb->m, cus);
+  const PassPlan fp = plan_pass(k, pb->m, pb->n, cus);
+  const Layout L = make_layout(pb, k, gp, fp, 1);
+  if (workspace_bytes < L.total) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HostHelpers* hh = nullptr;
+  rc = host_helpers(&hh);
+  if (rc != OTTX_OK) return rc;
+
+  const int inner = o->inner_iterations;
+  const int outer = (o->max_iterations + inner - 1) / inner;
+  const bool force_scan = o->min_iterations == o->max_iterations;
+  const int total_blocks = force_scan ? o->max_iterations / inner : outer;
+
+  Prepared P;
+  rc = prepare(pb, k, L, workspace, st, &P);
+  if (rc != OTTX_OK) return rc;
+  char* base = P.base;
+  SolveState* state = reinterpret_cast<SolveState*>(base + L.state);
+  uint32_t* counters = reinterpret_cast<uint32_t*>(base + L.counters);
+  float* g_save = reinterpret_cast<float*>(base + L.g_save);
+  double* dual = reinterpret_cast<double*>(base + L.dual);
+  hipLaunchKernelGGL(init_state_kernel, dim3(4), dim3(256), 0, st, state, out->errors, outer, counters,
+                     std::max(gp.tiles, fp.tiles));
+  OTTX_LAUNCHED();
+
+  const double eps = pb->epsilon;
+  HalfStepArgs ga = base_args(pb, k, gp, P, 1, out->f, out->g, eps);
+  HalfStepArgs fa = base_args(pb, k, fp, P, 0, out->g, out->f, eps);
+  for (HalfStepArgs* a : {&ga, &fa}) {
+    a->norm_p = o->norm_error;
+    a->momentum = static_cast<float>(o->momentum);
+    a->errors = out->errors;
+    a->row_err_partial = reinterpret_cast<const double*>(base + L.err_row);
+    a->row_tiles = fp.tiles;
+    a->inner = inner;
+    a->min_it = o->min_iterations;
+    a->max_it = o->max_iterations;
+    a->outer = outer;
+    a->force_scan = force_scan ? 1 : 0;
+    a->threshold = static_cast<float>(o->threshold);
+  }
+  ga.out_save = g_save;
+  const bool balanced = (pb->tau_a == 1.0 && pb->tau_b == 1.0);
+  HalfStepArgs g_plain = ga, g_err = ga, f_plain = fa, f_err = fa;
+  g_plain.mode = kModeWritePot | kModeCheckDone;
+  g_err.mode = kModeWritePot | kModeCheckDone | kModeSaveOld | kModeErrCol;
+  f_plain.mode = kModeWritePot | kModeCheckDone;
+  f_err.mode = kModeWritePot | kModeCheckDone | (balanced ? 0u : kModeErrRow);
+
+  auto launch_block = [&](hipStream_t s) -> int {
+    for (int it = 0; it < inner; ++it) {
+      int r = launch_halfstep(k, it == inner - 1 ? f_err : f_plain, s);
+      if (r != OTTX_OK) return r;
+      r = launch_halfstep(k, it == inner - 1 ? g_err : g_plain, s);
+      if (r != OTTX_OK) return r;
+    }
+    return OTTX_OK;
+  };
+
+  SolveState* mail = static_cast<SolveState*>(hh->mailbox);
+  if (total_blocks > 0) {
+    rc = launch_halfstep(k, g_plain, st);  // g-update of iteration 0
+    if (rc != OTTX_OK) return rc;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipStream_t cap = nullptr;
+    if (o->use_graph) {
+      OTTX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      OTTX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+      rc = launch_block(cap);
+      hipGraph_t g = nullptr;
+      const hipError_t e1 = hipStreamEndCapture(cap, &g);
+      if (rc != OTTX_OK || e1 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cap);
+        return rc != OTTX_OK ? rc : OTTX_EHIP;
+      }
+      graph = g;
+      if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        (void)hipStreamDestroy(cap);
+        return OTTX_EHIP;
+      }
+    }
+    hipEvent_t ev[2];
+    OTTX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    OTTX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int status = OTTX_OK;
+    for (int blk = 0; blk < total_blocks; ++blk) {
+      if (exec) {
+        if (hipGraphLaunch(exec, st) != hipSuccess) { status = OTTX_EHIP; break; }
+      } else {
+        status = launch_block(st);
+        if (status != OTTX_OK) break;
+      }
+      if (hipEventRecord(ev[blk & 1], st) != hipSuccess) { status = OTTX_EHIP; break; }
+      if (blk >= 1) {
+        if (hipEventSynchronize(ev[(blk - 1) & 1]) != hipSuccess) { status = OTTX_EHIP; break; }
+        if (hipMemcpyAsync(mail, state, sizeof(SolveState), hipMemcpyDeviceToHost, hh->side) != hipSuccess ||
+            hipStreamSynchronize(hh->side) != hipSuccess) {
+          status = OTTX_EHIP;
+          break;
+        }
+        if (mail->done) break;
+      }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+    if (status != OTTX_OK) return status;
+  } else {
+    // No iteration runs: the outputs are the initial potentials.
+    HalfStepArgs mass = ga;
+    mass.mode = kModeSaveOld | kModeErrCol;
+    mass.state = nullptr;
+    rc = launch_halfstep(k, mass, st);
+    if (rc != OTTX_OK) return rc;
+  }
+
+  hipLaunchKernelGGL(dual_terms_kernel, dim3(3), dim3(256), 0, st, out->f, pb->a, pb->n, g_save, pb->b, pb->m,
+                     pb->epsilon, pb->tau_a, pb->tau_b, out->errors, outer, dual);
+  OTTX_LAUNCHED();
+  hipLaunchKernelGGL(copy_f32_kernel, dim3(static_cast<int>((pb->m + 255) / 256)), dim3(256), 0, st, g_save,
+                     out->g, pb->m);
+  OTTX_LAUNCHED();
+  char* mb = static_cast<char*>(hh->mailbox);
+  OTTX_HIP(hipMemcpyAsync(mb, state, sizeof(SolveState), hipMemcpyDeviceToHost, st));
+  OTTX_HIP(hipMemcpyAsync(mb + 256, dual, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (total_blocks == 0) {
+    OTTX_HIP(hipMemcpyAsync(mb + 512, base + L.mass, static_cast<size_t>(gp.tiles) * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+  }
+  OTTX_HIP(hipStreamSynchronize(st));
+  const SolveState* fs = reinterpret_cast<const SolveState*>(mb);
+  const double* du = reinterpret_cast<const double*>(mb + 256);
+  double total = fs->total_mass;
+  if (total_blocks == 0) {
+    total = 0.0;
+    const double* mp = reinterpret_cast<const double*>(mb + 512);
+    for (int t = 0; t < gp.tiles && 512 + (t + 1) * 8 <= 4096; ++t) total += mp[t];
+  }
+  out->total_mass = total;
+  out->reg_ot_cost = du[0] + du[2] + pb->epsilon * (du[1] * du[3] - total);
+  const bool any_nan = du[4] > 0.0;
+  const float last = static_cast<float>(du[6]);
+  out->converged = (!any_nan && outer > 0 && last < static_cast<float>(o->threshold)) ? 1 : 0;
+  out->n_iters = static_cast<int32_t>(du[5]) * inner;
+  out->n_blocks = fs->n_blocks;
+  return OTTX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,229 @@
+"""PointCloud geometry backed by the ottx HIP kernels.
+
+Mirrors ``ott.geometry.pointcloud.PointCloud`` (geometry/pointcloud.py:27-454)
+and the parts of ``ott.geometry.geometry.Geometry`` the Sinkhorn solver calls
+(the operator protocol, geometry/geometry.py:256-420): ``apply_lse_kernel``,
+``update_potential``, ``marginal_from_potentials``, ``epsilon``,
+``epsilon_scheduler``, ``shape``, ``dtype``.  The cost matrix is never
+materialised: every operator is one fused half-step launch.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Any, Literal, Optional, Tuple, Union
+
+import numpy as np
+import torch
+
+from ott_amd import _lib
+from ott_amd.geometry import costs
+from ott_amd.geometry import epsilon_scheduler as eps_scheduler
+
+__all__ = ["PointCloud"]
+
+Precision = Literal["fp32", "bf16_split"]
+
+
+def as_device_f32(arr, device: torch.device) -> torch.Tensor:
+  """Contiguous float32 tensor on ``device`` (numpy and CPU tensors are copied)."""
+  if isinstance(arr, np.ndarray):
+    arr = torch.from_numpy(np.ascontiguousarray(arr))
+  if not isinstance(arr, torch.Tensor):
+    arr = torch.as_tensor(arr)
+  return arr.to(device=device, dtype=torch.float32).contiguous()
+
+
+class PointCloud:
+  """Two point clouds with the squared-Euclidean cost.
+
+  Args mirror geometry/pointcloud.py:54-72 and geometry/geometry.py:76-89:
+    x: ``[n, d]`` points.  y: ``[m, d]`` points, ``None`` means ``x``.
+    cost_fn: only :class:`~ott_amd.geometry.costs.SqEuclidean` (the default).
+    batch_size: accepted for signature compatibility; the kernels are always
+      online (the cost is recomputed per tile and never stored).
+    scale_cost: a positive float ``s`` so that ``C = ||x - y||^2 / s``.
+    epsilon: float, :class:`Epsilon`, or the multiplier when
+      ``relative_epsilon`` is set.
+    relative_epsilon: ``"mean"`` (closed form on the GPU) or ``None``.
+    precision: ``"fp32"`` (f32 MFMA cross term) or ``"bf16_split"`` (hi/lo
+      bfloat16 operands, f32 accumulation).  Not part of OTT's signature.
+  """
+
+  def __init__(
+      self,
+      x,
+      y=None,
+      cost_fn: Optional[costs.CostFn] = None,
+      batch_size: Optional[int] = None,
+      scale_cost: Union[float, str] = 1.0,
+      epsilon: Union[float, eps_scheduler.Epsilon, None] = None,
+      relative_epsilon: Optional[Literal["mean", "std"]] = None,
+      precision: Precision = "fp32",
+      **kwargs: Any,
+  ):
+    if kwargs:
+      raise TypeError(f"unexpected PointCloud arguments: {sorted(kwargs)}")
+    self.device = _lib.require_device()
+    self.x = as_device_f32(x, self.device)
+    self.y = self.x if y is None else as_device_f32(y, self.device)
+    if self.x.ndim != 2 or self.y.ndim != 2 or self.x.shape[1] != self.y.shape[1]:
+      raise ValueError(f"x and y must be [n, d] and [m, d], got {tuple(self.x.shape)} "
+                       f"and {tuple(self.y.shape)}")
+    self.cost_fn = costs.SqEuclidean() if cost_fn is None else cost_fn
+    if batch_size is not None:
+      assert batch_size > 0, f"`batch_size={batch_size}` must be positive."
+    self._batch_size = batch_size
+    self._scale_cost = scale_cost
+    self._epsilon_init = epsilon
+    self._relative_epsilon = relative_epsilon
+    if precision not in _lib.PRECISIONS:
+      raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
+    self.precision = precision
+    self._eps_cache: Optional[eps_scheduler.Epsilon] = None
+    self._mean_cache: Optional[float] = None
+
+  # -- shape / dtype ---------------------------------------------------------
+  @property
+  def shape(self) -> Tuple[int, int]:
+    return self.x.shape[0], self.y.shape[0]
+
+  @property
+  def dtype(self) -> torch.dtype:
+    return self.x.dtype
+
+  @property
+  def is_squared_euclidean(self) -> bool:
+    return isinstance(self.cost_fn, costs.SqEuclidean)
+
+  @property
+  def is_online(self) -> bool:
+    return self._batch_size is not None
+
+  @property
+  def batch_size(self) -> Optional[int]:
+    if self._batch_size is None:
+      return None
+    n, m = self.shape
+    return min(n, m, self._batch_size)
+
+  @property
+  def cost_rank(self) -> int:
+    return self.x.shape[1]
+
+  # -- cost scaling and epsilon ---------------------------------------------
+  @property
+  def inv_scale_cost(self) -> float:
+    """pointcloud.py:343-383; float scales only."""
+    s = self._scale_cost
+    if isinstance(s, (int, float, np.floating)) and not isinstance(s, bool):
+      if not float(s) > 0.0:
+        raise ValueError(f"scale_cost must be positive, got {s}")
+      return 1.0 / float(s)
+    if isinstance(s, str) and s in ("mean", "max_cost", "median", "max_norm", "max_bound"):
+      raise NotImplementedError(f"scale_cost={s!r} is not supported by ott_amd yet")
+    raise ValueError(f"Scaling {s} not implemented.")
+
+  @property
+  def mean_cost_matrix(self) -> float:
+    """geometry.py:112-117 via pointcloud.py:170-186, closed form on the GPU."""
+    if self._mean_cache is None:
+      lib = _lib.load_library()
+      n, m = self.shape
+      d = self.x.shape[1]
+      nbytes = lib.ottx_mean_cost_workspace_bytes(n, m, d)
+      ws = _lib.workspace(nbytes, self.device)
+      out = ctypes.c_double(0.0)
+      _lib.check("ottx_mean_cost", lib.ottx_mean_cost(
+          n, m, d, self.x.data_ptr(), self.y.data_ptr(), self.inv_scale_cost,
+          ctypes.byref(out), ws.data_ptr(), ws.numel(), _lib.stream_ptr()))
+      self._mean_cache = float(out.value)
+    return self._mean_cache
+
+  @property
+  def epsilon_scheduler(self) -> eps_scheduler.Epsilon:
+    """geometry.py:147-171."""
+    if self._eps_cache is not None:
+      return self._eps_cache
+    init = self._epsilon_init
+    if isinstance(init, eps_scheduler.Epsilon):
+      sched = init
+    elif self._relative_epsilon is None:
+      if init is None:
+        raise NotImplementedError(
+            "the default epsilon (0.05 * std of the cost) is not supported yet; "
+            "pass epsilon=... or relative_epsilon='mean'")
+      sched = eps_scheduler.Epsilon(float(init))
+    elif self._relative_epsilon == "mean":
+      mult = eps_scheduler.DEFAULT_EPSILON_SCALE if init is None else float(init)
+      sched = eps_scheduler.Epsilon(mult * self.mean_cost_matrix)
+    elif self._relative_epsilon == "std":
+      raise NotImplementedError("relative_epsilon='std' is not supported yet")
+    else:
+      raise ValueError(f"Invalid relative epsilon: {self._relative_epsilon}.")
+    self._eps_cache = sched
+    return sched
+
+  @property
+  def epsilon(self) -> float:
+    return float(self.epsilon_scheduler.target)
+
+  # -- the operator protocol -------------------------------------------------
+  def _problem(self, a: Optional[torch.Tensor] = None, b: Optional[torch.Tensor] = None,
+               tau_a: float = 1.0, tau_b: float = 1.0,
+               epsilon: Optional[float] = None) -> Tuple[_lib.Problem, Tuple]:
+    if not self.is_squared_euclidean:
+      raise NotImplementedError("ott_amd accelerates the squared-Euclidean cost only")
+    n, m = self.shape
+    if a is None:
+      a = torch.full((n,), 1.0 / n, dtype=torch.float32, device=self.device)
+    if b is None:
+      b = torch.full((m,), 1.0 / m, dtype=torch.float32, device=self.device)
+    pb = _lib.Problem(
+        n=n, m=m, d=self.x.shape[1], precision=_lib.PRECISIONS[self.precision],
+        x=self.x.data_ptr(), y=self.y.data_ptr(), a=a.data_ptr(), b=b.data_ptr(),
+        epsilon=float(self.epsilon if epsilon is None else epsilon),
+        inv_scale_cost=self.inv_scale_cost, tau_a=float(tau_a), tau_b=float(tau_b))
+    return pb, (a, b)
+
+  def _op(self, op: int, f, g, eps: float, axis: int,
+          log_marginal: Optional[torch.Tensor] = None) -> torch.Tensor:
+    if axis not in (0, 1):
+      raise ValueError(f"axis must be 0 or 1, got {axis}")
+    lib = _lib.load_library()
+    f = as_device_f32(f, self.device)
+    g = as_device_f32(g, self.device)
+    n, m = self.shape
+    if f.shape != (n,) or g.shape != (m,):
+      raise ValueError(f"potentials must be [{n}] and [{m}]")
+    pb, keep = self._problem(epsilon=eps)
+    out = torch.empty((m if axis == 0 else n,), dtype=torch.float32, device=self.device)
+    lm = None if log_marginal is None else as_device_f32(log_marginal, self.device)
+    nbytes = lib.ottx_op_workspace_bytes(ctypes.byref(pb))
+    ws = _lib.workspace(nbytes, self.device)
+    _lib.check("ottx_pointcloud_op", lib.ottx_pointcloud_op(
+        ctypes.byref(pb), op, axis, f.data_ptr(), g.data_ptr(), _lib.ptr(lm),
+        float(eps), out.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_ptr()))
+    del keep
+    return out
+
+  def apply_lse_kernel(self, f, g, eps: float, vec=None, axis: int = 0):
+    """geometry.py:260-299 (vec=None) -> ``(eps*LSE(...) - h, sign)``."""
+    if vec is not None:
+      raise NotImplementedError("apply_lse_kernel with vec is not supported yet")
+    res = self._op(_lib.OP_APPLY_LSE, f, g, float(eps), axis)
+    return res, torch.ones((1,), dtype=torch.float32, device=self.device)
+
+  def update_potential(self, f, g, log_marginal, iteration: Optional[int] = None,
+                       axis: int = 0) -> torch.Tensor:
+    """geometry.py:376-398."""
+    eps = self.epsilon_scheduler(iteration)
+    return self._op(_lib.OP_UPDATE, f, g, float(eps), axis, log_marginal=log_marginal)
+
+  def marginal_from_potentials(self, f, g, axis: int = 0) -> torch.Tensor:
+    """geometry.py:326-349 (target epsilon)."""
+    return self._op(_lib.OP_MARGINAL, f, g, self.epsilon, axis)
+
+  def __repr__(self) -> str:
+    n, m = self.shape
+    return (f"PointCloud(n={n}, m={m}, d={self.x.shape[1]}, precision={self.precision!r})")

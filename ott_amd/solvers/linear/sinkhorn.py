@@ -1,0 +1,224 @@
+"""Log-domain Sinkhorn solver on the ottx HIP kernels.
+
+Mirrors ``ott.solvers.linear.sinkhorn`` (solvers/linear/sinkhorn.py): the
+``Sinkhorn`` constructor (:718-780), ``__call__`` (:782-801), and
+``SinkhornOutput`` (:280-538).  The whole loop -- fixed-point blocks
+(math/fixed_point_loop.py:23-93), lse steps (:803-857), errors (:127-221),
+termination (:940-962), outputs (:974-1029) and ``reg_ot_cost`` (:224-277) --
+runs in ``ottx_sinkhorn_solve`` (include/ottx.h); this module validates the
+options, prepares buffers and wraps the results.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Any, Callable, NamedTuple, Optional, Tuple
+
+import torch
+
+from ott_amd import _lib
+from ott_amd.geometry.pointcloud import PointCloud, as_device_f32
+from ott_amd.initializers.linear import initializers as init_lib
+from ott_amd.problems.linear import linear_problem
+from ott_amd.solvers.linear import acceleration
+
+__all__ = ["Sinkhorn", "SinkhornOutput"]
+
+
+class SinkhornOutput(NamedTuple):
+  """sinkhorn.py:280-323: potentials, errors, reg_ot_cost, ot_prob, threshold,
+  converged, inner_iterations."""
+
+  potentials: Tuple[torch.Tensor, torch.Tensor]
+  errors: Optional[torch.Tensor] = None
+  reg_ot_cost: Optional[float] = None
+  ot_prob: Optional[linear_problem.LinearProblem] = None
+  threshold: Optional[float] = None
+  converged: Optional[bool] = None
+  inner_iterations: Optional[int] = None
+
+  def set(self, **kwargs: Any) -> "SinkhornOutput":
+    return self._replace(**kwargs)
+
+  @property
+  def f(self) -> torch.Tensor:
+    return self.potentials[0]
+
+  @property
+  def g(self) -> torch.Tensor:
+    return self.potentials[1]
+
+  @property
+  def geom(self) -> PointCloud:
+    return self.ot_prob.geom
+
+  @property
+  def a(self) -> torch.Tensor:
+    return self.ot_prob.a
+
+  @property
+  def b(self) -> torch.Tensor:
+    return self.ot_prob.b
+
+  @property
+  def n_iters(self) -> int:
+    """sinkhorn.py:431-434."""
+    return int(torch.sum(self.errors != -1).item()) * self.inner_iterations
+
+  @property
+  def dual_cost(self) -> float:
+    """sinkhorn.py:337-343."""
+    a, b = self.ot_prob.a, self.ot_prob.b
+    fa = torch.where(a > 0.0, a.double() * self.f.double(), torch.zeros_like(a, dtype=torch.float64))
+    gb = torch.where(b > 0.0, b.double() * self.g.double(), torch.zeros_like(b, dtype=torch.float64))
+    return float(fa.sum() + gb.sum())
+
+  @property
+  def ent_reg_cost(self) -> float:
+    """sinkhorn.py:350-370: ``reg_ot_cost - eps * (H(a) + H(b))``."""
+    def entr(w):
+      w = w.double()
+      return torch.where(w > 0, -w * torch.log(w), torch.zeros_like(w)).sum()
+    return float(self.reg_ot_cost - self.geom.epsilon * (entr(self.a) + entr(self.b)))
+
+  @property
+  def kl_reg_cost(self) -> float:
+    return self.reg_ot_cost
+
+  def marginal(self, axis: int) -> torch.Tensor:
+    """sinkhorn.py:451-452 -> geometry.marginal_from_potentials."""
+    return self.ot_prob.geom.marginal_from_potentials(self.f, self.g, axis=axis)
+
+  @property
+  def transport_mass(self) -> float:
+    return float(self.marginal(0).double().sum())
+
+
+class Sinkhorn:
+  """Sinkhorn solver with OTT's constructor (sinkhorn.py:718-780).
+
+  Supported on the GPU path: ``lse_mode=True``; any ``threshold``,
+  ``norm_error`` (p >= 1), ``inner_iterations``, ``min_iterations``,
+  ``max_iterations``; constant ``Momentum`` (``start == 0``); sequential
+  updates; balanced or unbalanced problems; warm starts through ``init`` or an
+  ``initializer``.  Options outside that set raise ``NotImplementedError``;
+  there is no silent fallback.  ``implicit_diff`` and ``use_danskin`` only
+  affect reverse mode, which this package does not provide.
+
+  Extra (not in OTT): ``use_graph`` replays each block of ``inner_iterations``
+  as a hipGraph.
+  """
+
+  def __init__(
+      self,
+      lse_mode: bool = True,
+      threshold: float = 1e-3,
+      norm_error: int = 1,
+      inner_iterations: int = 10,
+      min_iterations: int = 0,
+      max_iterations: int = 2000,
+      momentum: Optional[acceleration.Momentum] = None,
+      anderson: Optional[acceleration.AndersonAcceleration] = None,
+      parallel_dual_updates: bool = False,
+      recenter_potentials: bool = False,
+      use_danskin: Optional[bool] = None,
+      implicit_diff: Any = True,
+      initializer: Optional[init_lib.SinkhornInitializer] = None,
+      progress_fn: Optional[Callable] = None,
+      use_graph: bool = True,
+  ):
+    self.lse_mode = lse_mode
+    self.threshold = threshold
+    self.inner_iterations = inner_iterations
+    self.min_iterations = min_iterations
+    self.max_iterations = max_iterations
+    self._norm_error = norm_error
+    self.anderson = anderson
+    self.implicit_diff = implicit_diff
+    if momentum is not None:
+      self.momentum = acceleration.Momentum(momentum.start, momentum.error_threshold,
+                                            momentum.value, self.inner_iterations)
+    else:
+      self.momentum = acceleration.Momentum()
+    self.parallel_dual_updates = parallel_dual_updates
+    self.recenter_potentials = recenter_potentials
+    self.initializer = init_lib.DefaultInitializer() if initializer is None else initializer
+    self.progress_fn = progress_fn
+    self.use_danskin = (self.implicit_diff is not None) if use_danskin is None else use_danskin
+    self.use_graph = use_graph
+
+  @property
+  def norm_error(self) -> Tuple[int, ...]:
+    return (self._norm_error,)
+
+  @property
+  def outer_iterations(self) -> int:
+    """sinkhorn.py:964-972 (errors rows)."""
+    return int(math.ceil(self.max_iterations / self.inner_iterations))
+
+  def _check_supported(self, ot_prob: linear_problem.LinearProblem) -> None:
+    if not self.lse_mode:
+      raise NotImplementedError("ott_amd runs the log-domain (lse_mode=True) solver only")
+    if self.anderson is not None:
+      raise NotImplementedError("Anderson acceleration is not supported by ott_amd")
+    if self.parallel_dual_updates:
+      raise NotImplementedError("parallel_dual_updates is not supported by ott_amd yet")
+    if self.recenter_potentials:
+      raise NotImplementedError("recenter_potentials is not supported by ott_amd yet")
+    if not self.momentum.is_constant:
+      raise NotImplementedError("adaptive momentum is not supported by ott_amd yet")
+    if self.progress_fn is not None:
+      raise NotImplementedError("progress_fn is not supported by ott_amd yet")
+    if not isinstance(ot_prob.geom, PointCloud):
+      raise NotImplementedError("ott_amd accelerates PointCloud geometries only")
+    if not ot_prob.geom.epsilon_scheduler.is_target_only:
+      raise NotImplementedError("epsilon schedules with init > 1 are not supported yet")
+    if int(self._norm_error) < 1:
+      raise ValueError("norm_error must be >= 1")
+
+  def __call__(self, ot_prob: linear_problem.LinearProblem,
+               init: Optional[Tuple[Any, Any]] = None, **kwargs: Any) -> SinkhornOutput:
+    """sinkhorn.py:782-801."""
+    self._check_supported(ot_prob)
+    geom = ot_prob.geom
+    if init is None:
+      init = self.initializer(ot_prob, lse_mode=self.lse_mode, **kwargs)
+    f0, g0 = init
+    f = as_device_f32(f0, geom.device).clone()
+    g = as_device_f32(g0, geom.device).clone()
+    return run(ot_prob, self, (f, g))
+
+
+def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
+        init: Tuple[torch.Tensor, torch.Tensor]) -> SinkhornOutput:
+  """sinkhorn.py:1052-1099: the loop, then ``set_cost`` and ``ot_prob``."""
+  lib = _lib.load_library()
+  geom: PointCloud = ot_prob.geom
+  f, g = init
+  n, m = geom.shape
+  if f.shape != (n,) or g.shape != (m,):
+    raise ValueError(f"init potentials must be [{n}] and [{m}]")
+  a, b = ot_prob.a, ot_prob.b
+  pb, keep = geom._problem(a=a, b=b, tau_a=ot_prob.tau_a, tau_b=ot_prob.tau_b)
+  opts = _lib.SinkhornOpts(
+      threshold=float(solver.threshold), norm_error=int(solver._norm_error),
+      inner_iterations=int(solver.inner_iterations),
+      min_iterations=int(solver.min_iterations),
+      max_iterations=int(solver.max_iterations),
+      momentum=float(solver.momentum.value), tile_config=-1,
+      use_graph=1 if solver.use_graph else 0)
+  errors = torch.empty((solver.outer_iterations,), dtype=torch.float32, device=geom.device)
+  out = _lib.SinkhornOut(f=f.data_ptr(), g=g.data_ptr(), errors=errors.data_ptr())
+  nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts))
+  if nbytes == 0:
+    raise NotImplementedError(f"problem not supported by the HIP kernels: {geom!r}")
+  ws = _lib.workspace(nbytes, geom.device)
+  _lib.check("ottx_sinkhorn_solve", lib.ottx_sinkhorn_solve(
+      ctypes.byref(pb), ctypes.byref(opts), ctypes.byref(out), ws.data_ptr(),
+      ws.numel(), _lib.stream_ptr()))
+  del keep, ws
+  return SinkhornOutput(
+      potentials=(f, g), errors=errors, reg_ot_cost=float(out.reg_ot_cost),
+      ot_prob=ot_prob, threshold=float(solver.threshold),
+      converged=bool(out.converged), inner_iterations=int(solver.inner_iterations))

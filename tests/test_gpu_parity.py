@@ -1,0 +1,203 @@
+"""GPU parity of the HIP path against the FP64 oracle.
+
+Every test calls the C ABI through the ``ott_amd`` binding on seeded inputs and
+compares with ``oracle/sinkhorn64.py``.  Tolerances: float32 path rel 1e-4,
+bf16-split path rel 1e-3 (on potentials, relative to their scale, and on
+reg_ot_cost).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import sinkhorn64 as oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 1e-4, "bf16_split": 1e-3}
+
+
+def _cloud(n, m, d, seed, shift=0.2, zero_a=None, zero_b=None):
+  rng = np.random.default_rng(seed)
+  x = (rng.normal(size=(n, d)) / np.sqrt(d)).astype(np.float32)
+  y = (rng.normal(size=(m, d)) / np.sqrt(d) + shift).astype(np.float32)
+  a = rng.uniform(0.2, 1.2, size=n)
+  b = rng.uniform(0.2, 1.2, size=m)
+  if zero_a is not None:
+    a[zero_a] = 0.0
+  if zero_b is not None:
+    b[zero_b] = 0.0
+  return x, y, (a / a.sum()).astype(np.float32), (b / b.sum()).astype(np.float32)
+
+
+def _geom(x, y, precision, **kw):
+  from ott_amd.geometry import pointcloud
+  return pointcloud.PointCloud(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(),
+                               precision=precision, **kw)
+
+
+def _close(got, want, rtol, what):
+  got = np.asarray(got, dtype=np.float64)
+  want = np.asarray(want, dtype=np.float64)
+  fin = np.isfinite(want)
+  assert np.array_equal(fin, np.isfinite(got)), f"{what}: finiteness differs"
+  scale = max(1.0, float(np.max(np.abs(want[fin])))) if fin.any() else 1.0
+  err = float(np.max(np.abs(got[fin] - want[fin]))) if fin.any() else 0.0
+  assert err <= rtol * scale, f"{what}: max abs err {err:.3e} > {rtol:.1e} * {scale:.3e}"
+  if (~fin).any():
+    assert np.array_equal(got[~fin], want[~fin]), f"{what}: non-finite entries differ"
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("d", [2, 5, 16, 64, 100])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_apply_lse_kernel(precision, d, axis):
+  x, y, a, b = _cloud(300, 517, d, seed=d, zero_b=[3])
+  rng = np.random.default_rng(7)
+  f = rng.normal(size=300).astype(np.float32) * 0.1
+  g = rng.normal(size=517).astype(np.float32) * 0.1
+  g[3] = -np.inf
+  eps = 0.07
+  geom = _geom(x, y, precision, epsilon=eps)
+  got, sign = geom.apply_lse_kernel(torch.from_numpy(f).cuda(), torch.from_numpy(g).cuda(),
+                                    eps, axis=axis)
+  ops = oracle.DenseLSE(oracle.sqeuclidean_cost(x, y))
+  want = oracle.apply_lse_kernel(ops, f.astype(np.float64), g.astype(np.float64), eps, axis)
+  _close(got.cpu().numpy(), want, TOL[precision], f"apply_lse axis={axis}")
+  assert float(sign[0]) == 1.0
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_update_potential_and_marginal(precision, axis):
+  x, y, a, b = _cloud(257, 129, 32, seed=11, zero_a=[0], zero_b=[5])
+  f = np.where(a > 0, 0.0, -np.inf).astype(np.float32)
+  g = np.where(b > 0, 0.0, -np.inf).astype(np.float32)
+  geom = _geom(x, y, precision, epsilon=0.05, relative_epsilon="mean")
+  eps = geom.epsilon
+  want_eps = oracle.resolve_epsilon(x, y, 0.05, "mean")
+  assert abs(eps - want_eps) <= 1e-6 * want_eps
+  ops = oracle.DenseLSE(oracle.sqeuclidean_cost(x, y))
+  logw = np.log(b if axis == 0 else a)
+  with np.errstate(divide="ignore"):
+    got = geom.update_potential(torch.from_numpy(f).cuda(), torch.from_numpy(g).cuda(),
+                                torch.from_numpy(logw.astype(np.float32)).cuda(), axis=axis)
+  want = oracle.update_potential(ops, f.astype(np.float64), g.astype(np.float64),
+                                 logw, eps, axis)
+  _close(got.cpu().numpy(), want, TOL[precision], "update_potential")
+  # The marginal after a half-step equals the target exactly (a property of Sinkhorn).
+  fg = (f, got.cpu().numpy()) if axis == 0 else (got.cpu().numpy(), g)
+  marg = geom.marginal_from_potentials(torch.from_numpy(fg[0]).cuda(),
+                                       torch.from_numpy(fg[1]).cuda(), axis=axis)
+  target = b if axis == 0 else a
+  np.testing.assert_allclose(marg.cpu().numpy(), target, rtol=20 * TOL[precision], atol=1e-7)
+
+
+def _solve(x, y, a, b, precision, eps_kw, **kw):
+  from ott_amd.solvers import linear
+  geom = _geom(x, y, precision, **eps_kw)
+  at = None if a is None else torch.from_numpy(a).cuda()
+  bt = None if b is None else torch.from_numpy(b).cuda()
+  tau_a = kw.pop("tau_a", 1.0)
+  tau_b = kw.pop("tau_b", 1.0)
+  return linear.solve(geom, a=at, b=bt, tau_a=tau_a, tau_b=tau_b, **kw)
+
+
+def _check_solve(out, ref, precision, same_iters=True):
+  tol = TOL[precision]
+  assert out.converged == ref.converged
+  if same_iters:
+    assert out.n_iters == ref.n_iters, (out.n_iters, ref.n_iters)
+    errs = out.errors.cpu().numpy()
+    assert np.array_equal(errs == -1, ref.errors == -1)
+    assert np.array_equal(np.isinf(errs), np.isinf(ref.errors))
+    run = (ref.errors != -1) & np.isfinite(ref.errors)
+    np.testing.assert_allclose(errs[run], ref.errors[run], rtol=20 * tol, atol=1e-6)
+    _close(out.f.cpu().numpy(), ref.f, tol, "f")
+    _close(out.g.cpu().numpy(), ref.g, tol, "g")
+  rel = abs(out.reg_ot_cost - ref.reg_ot_cost) / max(abs(ref.reg_ot_cost), 1e-12)
+  assert rel <= tol, f"reg_ot_cost {out.reg_ot_cost} vs {ref.reg_ot_cost} (rel {rel:.2e})"
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+def test_solve_uniform_shifted_d2(precision):
+  from ott_amd import datasets
+  pb = datasets.uniform_shifted(1024, 1024, 2, seed=0)
+  out = _solve(pb.x, pb.y, pb.a, pb.b, precision, dict(epsilon=0.05, relative_epsilon="mean"))
+  ref = oracle.solve_pointcloud(pb.x, pb.y, pb.a, pb.b, epsilon=0.05, relative_epsilon="mean")
+  _check_solve(out, ref, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_solve_zero_weights(precision, use_graph):
+  x, y, a, b = _cloud(173, 291, 4, seed=3, zero_a=[0], zero_b=[3])
+  out = _solve(x, y, a, b, precision, dict(epsilon=0.1), use_graph=use_graph)
+  ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.1)
+  _check_solve(out, ref, precision)
+  assert np.isneginf(out.f.cpu().numpy()[0]) and np.isneginf(out.g.cpu().numpy()[3])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("tau", [(0.8, 0.9), (1.0, 0.9), (0.95, 0.95)])
+def test_solve_unbalanced(precision, tau):
+  x, y, a, b = _cloud(333, 222, 16, seed=5)
+  out = _solve(x, y, a, b, precision, dict(epsilon=0.1), tau_a=tau[0], tau_b=tau[1])
+  ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.1, tau_a=tau[0], tau_b=tau[1])
+  _check_solve(out, ref, precision)
+
+
+def test_solve_min_iterations_and_scan():
+  x, y, a, b = _cloud(120, 90, 4, seed=9)
+  out = _solve(x, y, a, b, "fp32", dict(epsilon=0.1), min_iterations=34)
+  ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.1, min_iterations=34)
+  _check_solve(out, ref, "fp32")
+  errs = out.errors.cpu().numpy()
+  assert np.isinf(errs[:3]).all() and errs[3] > 0
+  out = _solve(x, y, a, b, "fp32", dict(epsilon=0.1), min_iterations=50, max_iterations=50)
+  ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.1, min_iterations=50, max_iterations=50)
+  _check_solve(out, ref, "fp32")
+  assert out.converged and 0 < float(out.errors[-1]) < 1e-3
+
+
+@pytest.mark.parametrize("kw", [dict(norm_error=2), dict(inner_iterations=7),
+                                dict(momentum_value=1.3), dict(max_iterations=25),
+                                dict(threshold=1e-5, max_iterations=40)])
+def test_solve_options(kw):
+  from ott_amd.solvers.linear import acceleration
+  x, y, a, b = _cloud(150, 140, 8, seed=21)
+  okw = dict(kw)
+  if "momentum_value" in okw:
+    v = okw.pop("momentum_value")
+    gkw = dict(okw, momentum=acceleration.Momentum(value=v))
+    ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.1, momentum=v, **okw)
+  else:
+    gkw = okw
+    ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.1, **okw)
+  out = _solve(x, y, a, b, "fp32", dict(epsilon=0.1), **gkw)
+  _check_solve(out, ref, "fp32")
+
+
+def test_solve_warm_start_restart():
+  """A restart from converged potentials stops after one block (sinkhorn_test.py:491-546)."""
+  from ott_amd.geometry import pointcloud
+  from ott_amd.problems.linear import linear_problem
+  from ott_amd.solvers.linear import sinkhorn
+  x, y, a, b = _cloud(200, 180, 8, seed=4)
+  geom = _geom(x, y, "fp32", epsilon=0.05)
+  prob = linear_problem.LinearProblem(geom, torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+  solver = sinkhorn.Sinkhorn(threshold=1e-4)
+  out = solver(prob)
+  assert out.converged and out.n_iters > 10
+  out2 = solver(prob, init=(out.f, out.g))
+  assert out2.converged and out2.n_iters == 10
+  rel = abs(out2.reg_ot_cost - out.reg_ot_cost) / abs(out.reg_ot_cost)
+  assert rel < 1e-5
+
+
+def test_determinism():
+  x, y, a, b = _cloud(700, 900, 64, seed=8)
+  o1 = _solve(x, y, a, b, "bf16_split", dict(epsilon=0.05, relative_epsilon="mean"))
+  o2 = _solve(x, y, a, b, "bf16_split", dict(epsilon=0.05, relative_epsilon="mean"))
+  assert torch.equal(o1.f, o2.f) and torch.equal(o1.g, o2.g)
+  assert o1.reg_ot_cost == o2.reg_ot_cost

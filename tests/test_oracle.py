@@ -1,0 +1,246 @@
+"""Pins the FP64 oracle with checks that do not reuse its own formulas.
+
+The reference (JAX) cannot run here and its tests hold no golden values for
+this path, so each oracle function is checked against something independent:
+brute force, scipy, closed forms, an L-BFGS solve of the semi-dual, the
+Hungarian and epsilon -> infinity limits, and invariances.  A plausible slip
+(dropped norm term, wrong sign, swapped axis, wrong error cadence) fails one of
+these.
+"""
+
+import math
+
+import numpy as np
+import pytest
+from scipy import optimize, special
+
+from oracle import sinkhorn64 as oracle
+
+
+def _problem(n, m, d, seed, zero_a=None, zero_b=None, shift=0.25):
+  rng = np.random.default_rng(seed)
+  x = rng.uniform(size=(n, d))
+  y = rng.uniform(size=(m, d)) + shift
+  a = rng.uniform(size=n) + 0.2
+  b = rng.uniform(size=m) + 0.2
+  if zero_a is not None:
+    a[zero_a] = 0.0
+  if zero_b is not None:
+    b[zero_b] = 0.0
+  return x, y, a / a.sum(), b / b.sum()
+
+
+def test_cost_brute_force():
+  x, y, _, _ = _problem(7, 5, 3, 0)
+  want = np.array([[sum((x[i, k] - y[j, k]) ** 2 for k in range(3)) for j in range(5)]
+                   for i in range(7)])
+  np.testing.assert_allclose(oracle.sqeuclidean_cost(x, y), want, rtol=1e-14)
+  np.testing.assert_allclose(oracle.sqeuclidean_cost(x, y, expanded=True), want, rtol=1e-12)
+  np.testing.assert_allclose(oracle.sqeuclidean_cost(x, y, 0.5), 0.5 * want, rtol=1e-14)
+
+
+def test_mean_cost_and_epsilon():
+  x, y, _, _ = _problem(40, 30, 4, 1)
+  c = oracle.sqeuclidean_cost(x, y)
+  assert math.isclose(oracle.mean_cost(x, y), c.mean(), rel_tol=1e-12)
+  assert math.isclose(oracle.std_cost(x, y), c.std(), rel_tol=1e-10)
+  assert math.isclose(oracle.resolve_epsilon(x, y, 0.05, "mean"), 0.05 * c.mean(), rel_tol=1e-12)
+  assert math.isclose(oracle.resolve_epsilon(x, y, None, None), 0.05 * c.std(), rel_tol=1e-10)
+  assert oracle.resolve_epsilon(x, y, 0.3, None) == 0.3
+  assert oracle.epsilon_at(2.0, 3, init=10.0, decay=0.5) == max(10 * 0.125, 1.0) * 2.0
+
+
+def test_lse_naive_and_neginf():
+  rng = np.random.default_rng(2)
+  z = rng.normal(size=(6, 9))
+  np.testing.assert_allclose(oracle._lse(z, 0), np.log(np.exp(z).sum(0)), rtol=1e-13)
+  np.testing.assert_allclose(oracle._lse(z, 1), np.log(np.exp(z).sum(1)), rtol=1e-13)
+  z[:, 2] = -np.inf
+  assert np.isneginf(oracle._lse(z, 0)[2])
+
+
+def test_half_step_closed_form_m1():
+  """m = 1: g = eps log b - eps LSE_i((f_i - C_i) / eps), written with math.exp."""
+  x, y, a, _ = _problem(9, 1, 2, 3)
+  b = np.array([1.0])
+  f = np.linspace(-0.3, 0.2, 9)
+  eps = 0.07
+  c = [sum((x[i, k] - y[0, k]) ** 2 for k in range(2)) for i in range(9)]
+  s = sum(math.exp((f[i] - c[i]) / eps) for i in range(9))
+  want = eps * math.log(1.0) - eps * math.log(s)
+  ops = oracle.DenseLSE(oracle.sqeuclidean_cost(x, y))
+  got = oracle.update_potential(ops, f, np.zeros(1), np.log(b), eps, axis=0)
+  assert math.isclose(got[0], want, rel_tol=1e-12, abs_tol=1e-14)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_half_step_gives_exact_marginal(axis):
+  x, y, a, b = _problem(13, 11, 3, 4, zero_a=[2], zero_b=[5])
+  ops = oracle.DenseLSE(oracle.sqeuclidean_cost(x, y))
+  eps = 0.05
+  rng = np.random.default_rng(0)
+  f = np.where(a > 0, rng.normal(size=13) * 0.1, -np.inf)
+  g = np.where(b > 0, rng.normal(size=11) * 0.1, -np.inf)
+  with np.errstate(divide="ignore"):
+    if axis == 0:
+      g = oracle.update_potential(ops, f, g, np.log(b), eps, 0)
+      target = b
+    else:
+      f = oracle.update_potential(ops, f, g, np.log(a), eps, 1)
+      target = a
+  marg = oracle.marginal_from_potentials(ops, f, g, eps, axis)
+  np.testing.assert_allclose(marg, target, rtol=1e-12, atol=1e-15)
+  # and the marginal equals the row/column sums of the explicit plan
+  plan = np.exp((f[:, None] + g[None, :] - ops.cost) / eps)
+  np.testing.assert_allclose(marg, plan.sum(axis=axis), rtol=1e-12, atol=1e-15)
+
+
+def test_fixed_point_matches_lbfgs_semidual():
+  x, y, a, b = _problem(12, 9, 2, 5)
+  eps = 0.1
+  c = oracle.sqeuclidean_cost(x, y)
+  out = oracle.sinkhorn(oracle.DenseLSE(c), a, b, eps, threshold=1e-12, max_iterations=20000)
+  assert out.converged
+
+  def neg_semidual(gv):
+    f = eps * np.log(a) - eps * special.logsumexp((gv[None, :] - c) / eps, axis=1)
+    val = a @ f + b @ gv
+    p = np.exp((f[:, None] + gv[None, :] - c) / eps)
+    grad = b - p.sum(0)
+    return -val, -grad
+
+  res = optimize.minimize(neg_semidual, np.zeros(9), jac=True, method="L-BFGS-B",
+                          options=dict(gtol=1e-13, ftol=1e-15, maxiter=5000))
+  g_star = res.x - res.x.mean()
+  g_or = out.g - out.g.mean()
+  np.testing.assert_allclose(g_or, g_star, atol=1e-6)
+  # At the optimum the semi-dual value plus eps (H(a) + H(b)) is reg_ot_cost:
+  # OTT's potentials carry eps log a, eps log b (KL relative to a b^T).
+  ent = -(a @ np.log(a)) - (b @ np.log(b))
+  assert math.isclose(out.reg_ot_cost, -res.fun + eps * ent, rel_tol=1e-8)
+
+
+def test_hungarian_limit():
+  rng = np.random.default_rng(6)
+  n = 8
+  x = rng.uniform(size=(n, 2))
+  y = rng.uniform(size=(n, 2))
+  c = oracle.sqeuclidean_cost(x, y)
+  a = b = np.full(n, 1.0 / n)
+  out = oracle.sinkhorn(oracle.DenseLSE(c), a, b, 0.002, threshold=1e-6, max_iterations=50000)
+  plan = np.exp((out.f[:, None] + out.g[None, :] - c) / 0.002)
+  rows, cols = optimize.linear_sum_assignment(c)
+  hung = np.zeros_like(plan)
+  hung[rows, cols] = 1.0 / n
+  # sinkhorn_test.py:68-90 criterion: cosine distance to the Hungarian plan < 0.2
+  cos = plan.ravel() @ hung.ravel() / (np.linalg.norm(plan) * np.linalg.norm(hung))
+  assert 1.0 - cos < 0.2
+  assert plan[rows, cols].sum() > 0.95
+  # sinkhorn_test.py:56-66: x against itself at small eps puts > 0.99 on the
+  # diagonal (17 points in [0, 1]^4 as in the reference; eps 0.002 because
+  # these numpy-seeded points are not the reference's jax-seeded ones)
+  x = rng.uniform(size=(17, 4))
+  a = np.full(17, 1.0 / 17)
+  cxx = oracle.sqeuclidean_cost(x, x)
+  out = oracle.sinkhorn(oracle.DenseLSE(cxx), a, a, 0.002, threshold=1e-6, max_iterations=50000)
+  diag = np.exp((out.f + out.g - np.diag(cxx)) / 0.002)
+  assert 0.99 < diag.sum() < 1.0
+
+
+def test_large_epsilon_limit():
+  x, y, a, b = _problem(10, 7, 3, 7)
+  c = oracle.sqeuclidean_cost(x, y)
+  out = oracle.sinkhorn(oracle.DenseLSE(c), a, b, 1e4, threshold=1e-12)
+  assert math.isclose(out.reg_ot_cost, float(a @ c @ b), rel_tol=1e-5)
+
+
+def test_scale_invariance():
+  x, y, a, b = _problem(15, 12, 3, 8)
+  s = 3.0
+  o1 = oracle.solve_pointcloud(x, y, a, b, epsilon=0.05, threshold=1e-10, max_iterations=10000)
+  o2 = oracle.solve_pointcloud(s * x, s * y, a, b, epsilon=0.05 * s * s, threshold=1e-10,
+                               max_iterations=10000)
+  np.testing.assert_allclose(o2.f, s * s * o1.f, rtol=1e-6, atol=1e-9)
+  assert math.isclose(o2.reg_ot_cost, s * s * o1.reg_ot_cost, rel_tol=1e-8)
+
+
+def test_gauge_invariance_of_cost():
+  x, y, a, b = _problem(11, 10, 2, 9, zero_a=[1], zero_b=[4])
+  ops = oracle.DenseLSE(oracle.sqeuclidean_cost(x, y))
+  out = oracle.sinkhorn(ops, a, b, 0.1, threshold=1e-9)
+  r1 = oracle.compute_kl_reg_cost(ops, out.f, out.g, a, b, 0.1)
+  r2 = oracle.compute_kl_reg_cost(ops, out.f + 0.3, out.g - 0.3, a, b, 0.1)
+  assert math.isclose(r1, r2, rel_tol=1e-12)
+  assert np.isneginf(out.f[1]) and np.isneginf(out.g[4])
+  assert np.isfinite(out.reg_ot_cost)
+
+
+def test_unbalanced_scalar_fixed_point():
+  """n = m = 1: f = ta (A - g), g = tb (B - f) has a closed-form fixed point."""
+  x = np.array([[0.0, 0.0]])
+  y = np.array([[0.6, 0.8]])
+  a = np.array([0.7])
+  b = np.array([1.3])
+  eps, ta, tb = 0.2, 0.8, 0.9
+  c0 = 1.0
+  A = eps * math.log(0.7) + c0
+  B = eps * math.log(1.3) + c0
+  f_star = ta * (A - tb * B) / (1 - ta * tb)
+  g_star = tb * (B - f_star)
+  ops = oracle.DenseLSE(oracle.sqeuclidean_cost(x, y))
+  out = oracle.sinkhorn(ops, a, b, eps, tau_a=ta, tau_b=tb, threshold=1e-13, max_iterations=5000)
+  assert math.isclose(out.f[0], f_star, rel_tol=1e-9)
+  assert math.isclose(out.g[0], g_star, rel_tol=1e-9)
+
+
+def test_loop_bookkeeping():
+  x, y, a, b = _problem(17, 29, 4, 10)
+  kw = dict(epsilon=0.1)
+  out = oracle.solve_pointcloud(x, y, a, b, min_iterations=34, **kw)
+  assert np.isinf(out.errors[:3]).all() and out.errors[3] > 0
+  assert out.converged
+  assert out.n_iters == int(np.sum(out.errors != -1)) * 10
+  out = oracle.solve_pointcloud(x, y, a, b, min_iterations=50, max_iterations=50, **kw)
+  assert out.converged and 0 < out.errors[-1] < 1e-3 and np.isinf(out.errors[:-1]).all()
+  out = oracle.solve_pointcloud(x, y, a, b, max_iterations=25, threshold=1e-12, **kw)
+  assert out.errors.shape == (3,) and not out.converged and out.n_iters == 30
+  # a min_iterations multiple of inner stops on the +inf of the previous block
+  out = oracle.solve_pointcloud(x, y, a, b, min_iterations=40, **kw)
+  assert out.n_iters == 40 and np.isinf(out.errors[3]) and out.errors[4] == -1
+  # progress callback once per iteration
+  calls = []
+  oracle.solve_pointcloud(x, y, a, b, progress_fn=lambda s: calls.append(s[0]), **kw)
+  assert calls == list(range(len(calls))) and len(calls) % 10 == 0
+
+
+def test_unbalanced_converges_and_semibalanced():
+  x, y, a, b = _problem(30, 20, 3, 11)
+  for ta, tb in [(0.8, 0.9), (1.0, 0.9)]:
+    out = oracle.solve_pointcloud(x, y, a, b, epsilon=0.1, tau_a=ta, tau_b=tb)
+    err = out.errors[out.errors > -1][-1]
+    assert out.converged and 0 < err < 1e-3
+
+
+def test_c_oracle_matches_dense():
+  x, y, a, b = _problem(130, 70, 5, 12, zero_b=[3])
+  dense = oracle.DenseLSE(oracle.sqeuclidean_cost(x, y))
+  online = oracle.PointCloudLSE(x, y)
+  rng = np.random.default_rng(1)
+  f = rng.normal(size=130) * 0.1
+  g = rng.normal(size=70) * 0.1
+  g[3] = -np.inf
+  for axis in (0, 1):
+    want = dense.lse(f, g, 0.05, axis)
+    got = online.lse(f, g, 0.05, axis)
+    fin = np.isfinite(want)
+    np.testing.assert_allclose(got[fin], want[fin], rtol=1e-12)
+    assert np.array_equal(np.isfinite(got), fin)
+
+
+def test_fp32_twin_agrees():
+  x, y, a, b = _problem(64, 48, 3, 13)
+  o64 = oracle.solve_pointcloud(x, y, a, b, epsilon=0.05)
+  o32 = oracle.solve_pointcloud(x, y, a, b, epsilon=0.05, dtype=np.float32, expanded=True)
+  assert o32.n_iters == o64.n_iters
+  np.testing.assert_allclose(o32.f, o64.f, atol=2e-5)
+  assert math.isclose(o32.reg_ot_cost, o64.reg_ot_cost, rel_tol=1e-5)
