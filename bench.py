@@ -1,0 +1,256 @@
+#!/usr/bin/env python
+"""Benchmark: time to marginal error <= 1e-3 of log-domain Sinkhorn on one GPU.
+
+Workload (default): n = m = 16384 points in d = 64, x ~ N(0, I/d),
+y ~ N(0.3/sqrt(d) 1, I/d), uniform weights, epsilon = 0.05 * mean cost,
+threshold 1e-3 (L1 column-marginal error), max 5000 iterations.
+
+A "step" is one full solve from solve entry (geometry + epsilon + initializer
++ Sinkhorn loop + float64 cost) to outputs on the host, inputs already in HBM.
+Two variants are timed:
+  A  bf16-split cross term (hi/lo bfloat16 MFMA, f32 accumulation), warm start
+     from the learned initializer when its weights are present, else zeros;
+  B  plain float32 cross term (f32 MFMA), zero initialisation.
+``value`` is A's milliseconds per solve; B and the ratio B / A are reported
+beside it, together with the roofline of the dominant kernel (the fused
+half-step, timed live with hipEvents) and the FP64 CPU oracle timed on a
+bounded sample of the same workload on the host cores.
+
+  python bench.py --gpus 1 --steps 3 --warmup 1
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("time-to-marginal-err≤1e-3 (ms) n=m=16384 d=64: learned init+bf16 vs plain fp32")
+PEAK_BF16_TFLOPS = 2500.0     # MI355X dense bf16 MFMA
+PEAK_F32_MFMA_TFLOPS = 157.3  # MI355X f32 matrix
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+  print(msg, file=sys.stderr, flush=True)
+
+
+def workload(name: str):
+  from ott_amd import datasets
+  if name == "gaussian_d64":
+    return datasets.gaussian_shifted(16384, 16384, 64, seed=1), 5000
+  if name == "uniform_d2":
+    return datasets.uniform_shifted(1024, 1024, 2, seed=0), 2000
+  raise SystemExit(f"unknown workload {name}")
+
+
+def golden(name: str):
+  path = {"gaussian_d64": "gaussian_shifted_n16384_d64.json",
+          "uniform_d2": "uniform_shifted_n1024_d2.json"}[name]
+  path = os.path.join(ROOT, "tests", "golden", path)
+  if os.path.exists(path):
+    with open(path) as fh:
+      return json.load(fh)
+  return None
+
+
+def time_solves(pb, max_iterations, precision, init_fn, steps, warmup):
+  import torch
+  from ott_amd.geometry import pointcloud
+  from ott_amd.problems.linear import linear_problem
+  from ott_amd.solvers.linear import sinkhorn
+
+  x = torch.from_numpy(pb.x).cuda()
+  y = torch.from_numpy(pb.y).cuda()
+  a = None if pb.a is None else torch.from_numpy(pb.a).cuda()
+  b = None if pb.b is None else torch.from_numpy(pb.b).cuda()
+  solver_kw = dict(threshold=1e-3, max_iterations=max_iterations)
+
+  def one():
+    geom = pointcloud.PointCloud(x, y, epsilon=pb.eps_mult, relative_epsilon="mean",
+                                 precision=precision)
+    prob = linear_problem.LinearProblem(geom, a, b, tau_a=pb.tau_a, tau_b=pb.tau_b)
+    solver = sinkhorn.Sinkhorn(**solver_kw)
+    init = None if init_fn is None else init_fn(prob)
+    return solver(prob, init=init)
+
+  for _ in range(warmup):
+    out = one()
+  torch.cuda.synchronize()
+  times = []
+  for _ in range(steps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = one()
+    torch.cuda.synchronize()
+    times.append((time.perf_counter() - t0) * 1e3)
+  return times, out
+
+
+def halfstep_roofline(pb, precision, reps=20):
+  """Mean duration of the fused g-update half-step, from hipEvents on the launch stream."""
+  import torch
+  from ott_amd import _lib
+  from ott_amd.geometry import pointcloud
+  lib = _lib.load_library()
+  geom = pointcloud.PointCloud(torch.from_numpy(pb.x).cuda(), torch.from_numpy(pb.y).cuda(),
+                               epsilon=pb.eps_mult, relative_epsilon="mean", precision=precision)
+  n, m = geom.shape
+  d = pb.x.shape[1]
+  prob, keep = geom._problem()
+  opts = _lib.SinkhornOpts(threshold=1e-3, norm_error=1, inner_iterations=10, min_iterations=0,
+                           max_iterations=10, momentum=1.0, tile_config=-1, use_graph=0)
+  nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(prob), ctypes.byref(opts))
+  ws = _lib.workspace(nbytes, geom.device)
+  f = torch.zeros(n, device=geom.device)
+  g = torch.zeros(m, device=geom.device)
+  avg = ctypes.c_double(0.0)
+  _lib.check("ottx_benchmark_halfstep", lib.ottx_benchmark_halfstep(
+      ctypes.byref(prob), 0, reps, f.data_ptr(), g.data_ptr(), ctypes.byref(avg), ws.data_ptr(),
+      ws.numel(), _lib.stream_ptr()))
+  del keep
+  ms = avg.value
+  flops = 2.0 * n * m * d            # algorithmic cross-term flops of one half-step
+  pairs = float(n) * m
+  hbm = (n + m) * d * 4 + 3 * (n + m) * 4
+  peak = PEAK_BF16_TFLOPS if precision == "bf16_split" else PEAK_F32_MFMA_TFLOPS
+  achieved = flops / (ms * 1e-3) / 1e12
+  return {
+      "kernel": f"halfstep_kernel ({precision}, g-update)",
+      "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+      "frac": round(achieved / peak, 5), "traffic": None,
+      "avg_launch_ms": round(ms, 5), "flops_per_launch": flops,
+      "pairs_per_s": pairs / (ms * 1e-3),
+      "exp_per_s": pairs / (ms * 1e-3),
+      "algorithmic_bytes_per_launch": hbm,
+  }
+
+
+def cpu_baseline(pb, n_iters_needed, budget_s=15.0):
+  """FP64 oracle (C, OpenMP) on a bounded number of iterations of the same workload."""
+  import numpy as np
+  from oracle import sinkhorn64 as oracle
+  threads = len(os.sched_getaffinity(0))
+  env = os.environ.get("OMP_NUM_THREADS")
+  if env and env.isdigit():
+    threads = min(threads, int(env))
+  x = pb.x.astype(np.float64)
+  y = pb.y.astype(np.float64)
+  n, m = x.shape[0], y.shape[0]
+  a = np.full(n, 1.0 / n) if pb.a is None else pb.a.astype(np.float64)
+  b = np.full(m, 1.0 / m) if pb.b is None else pb.b.astype(np.float64)
+  eps = oracle.resolve_epsilon(x, y, pb.eps_mult, "mean")
+  ops = oracle.PointCloudLSE(x, y, threads=threads)
+  f = np.zeros(n)
+  g = np.zeros(m)
+  la, lb = np.log(a), np.log(b)
+  iters = 0
+  t0 = time.perf_counter()
+  while True:
+    g = oracle.update_potential(ops, f, g, lb, eps, axis=0)
+    f = oracle.update_potential(ops, f, g, la, eps, axis=1)
+    iters += 1
+    if time.perf_counter() - t0 > budget_s or iters >= 50:
+      break
+  per_iter_ms = (time.perf_counter() - t0) * 1e3 / iters
+  return {
+      "value": round(per_iter_ms * n_iters_needed, 1), "unit": "ms", "cores": threads,
+      "kind": "port",
+      "sample": (f"FP64 C oracle (oracle/sinkhorn_cpu.c, OpenMP) timed over {iters} full "
+                 f"iterations of the same {n}x{m} problem; value = per-iteration "
+                 f"{per_iter_ms:.1f} ms x {n_iters_needed} iterations to reach 1e-3"),
+      "per_iteration_ms": round(per_iter_ms, 2),
+  }
+
+
+def main() -> int:
+  ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+  ap.add_argument("--gpus", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=3)
+  ap.add_argument("--warmup", type=int, default=1)
+  ap.add_argument("--workload", default="gaussian_d64", choices=["gaussian_d64", "uniform_d2"])
+  ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+  ap.add_argument("--cpu-budget", type=float, default=15.0)
+  args = ap.parse_args()
+  if args.gpus != 1:
+    print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
+          f"--gpus {args.gpus} is not run", file=sys.stderr)
+    return 2
+
+  import torch
+  if not torch.cuda.is_available():
+    print("bench.py needs a GPU", file=sys.stderr)
+    return 1
+  from ott_amd import _lib
+  _lib.load_library()
+  from ott_amd.initializers.neural import learned
+  pb, max_it = workload(args.workload)
+  log(f"workload {pb.name}: n={pb.x.shape[0]} m={pb.y.shape[0]} d={pb.x.shape[1]}")
+
+  init_fn, init_name = learned.bench_initializer(pb.x.shape[1])
+  t_a, out_a = time_solves(pb, max_it, "bf16_split", init_fn, args.steps, args.warmup)
+  log(f"A bf16_split+{init_name}: {min(t_a):.2f} ms n_iters={out_a.n_iters} "
+      f"reg={out_a.reg_ot_cost:.8f}")
+  t_b, out_b = time_solves(pb, max_it, "fp32", None, args.steps, args.warmup)
+  log(f"B fp32+zeros: {min(t_b):.2f} ms n_iters={out_b.n_iters} reg={out_b.reg_ot_cost:.8f}")
+  roof = halfstep_roofline(pb, "bf16_split")
+  roof_b = halfstep_roofline(pb, "fp32")
+  log(f"half-step bf16 {roof['avg_launch_ms']:.4f} ms, fp32 {roof_b['avg_launch_ms']:.4f} ms")
+
+  ms_a = sum(t_a) / len(t_a)
+  ms_b = sum(t_b) / len(t_b)
+  gold = golden(args.workload)
+  accuracy = None
+  if gold is not None:
+    ref = gold["reg_ot_cost"]
+    accuracy = {
+        "oracle_reg_ot_cost": ref, "oracle_n_iters": gold["n_iters"],
+        "A_rel_err": abs(out_a.reg_ot_cost - ref) / abs(ref),
+        "B_rel_err": abs(out_b.reg_ot_cost - ref) / abs(ref),
+    }
+  cpu = None
+  if not args.no_cpu:
+    log("timing the CPU oracle sample ...")
+    cpu = cpu_baseline(pb, out_b.n_iters, args.cpu_budget)
+    log(f"cpu per iteration {cpu['per_iteration_ms']} ms on {cpu['cores']} threads")
+
+  line = {
+      "metric": METRIC, "value": round(ms_a, 3), "unit": "ms", "n_gpus": 1,
+      "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_a, 3),
+      "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+      "dtype": "bf16x3 (hi/lo split) MFMA cross term, f32 exp/lse, f64 reductions",
+      "data": "synthetic (numpy default_rng seed 1)",
+      "config": {
+          "workload": (f"{pb.name}: x~N(0,I/d), y~N(0.3/sqrt(d),I/d), uniform weights, "
+                       f"eps=0.05*mean(C), threshold 1e-3 L1, max {max_it} iterations"),
+          "n": int(pb.x.shape[0]), "m": int(pb.y.shape[0]), "d": int(pb.x.shape[1]),
+          "parallelism": "single GPU",
+      },
+      "variants": {
+          "A_bf16_split": {"init": init_name, "ms": round(ms_a, 3), "ms_min": round(min(t_a), 3),
+                           "n_iters": out_a.n_iters, "converged": out_a.converged,
+                           "reg_ot_cost": out_a.reg_ot_cost,
+                           "iters_per_s": round(out_a.n_iters / (ms_a * 1e-3), 1)},
+          "B_fp32_zero_init": {"ms": round(ms_b, 3), "ms_min": round(min(t_b), 3),
+                               "n_iters": out_b.n_iters, "converged": out_b.converged,
+                               "reg_ot_cost": out_b.reg_ot_cost,
+                               "iters_per_s": round(out_b.n_iters / (ms_b * 1e-3), 1)},
+          "speedup_B_over_A": round(ms_b / ms_a, 3),
+      },
+      "accuracy_vs_fp64_oracle": accuracy,
+      "roofline": roof,
+      "roofline_fp32_variant": roof_b,
+      "cpu_baseline": cpu,
+  }
+  print(json.dumps(line), flush=True)
+  return 0
+
+
+if __name__ == "__main__":
+  sys.exit(main())

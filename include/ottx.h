@@ -141,6 +141,17 @@ int ottx_sinkhorn_solve(const ottx_problem* problem,
                         const ottx_sinkhorn_opts* opts, ottx_sinkhorn_out* out,
                         void* workspace, size_t workspace_bytes, void* stream);
 
+/* Measurement helper (no reference counterpart; used by bench.py for the
+ * roofline figure).  Prepares the problem once, then runs ``reps`` real
+ * half-steps back to back on ``stream`` -- the g-update (axis 0: new g from f)
+ * or the f-update (axis 1) -- bracketed by hipEvents, and writes the mean
+ * duration of one half-step launch in milliseconds to *avg_ms_host.  f and g
+ * are updated in place.  Uses the workspace of ottx_sinkhorn_workspace_bytes. */
+int ottx_benchmark_halfstep(const ottx_problem* problem, int32_t axis,
+                            int32_t reps, float* f, float* g,
+                            double* avg_ms_host, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

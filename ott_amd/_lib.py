@@ -97,6 +97,10 @@ _SIGNATURES = [
      [ctypes.POINTER(Problem), ctypes.POINTER(SinkhornOpts),
       ctypes.POINTER(SinkhornOut), ctypes.c_void_p, ctypes.c_size_t,
       ctypes.c_void_p]),
+    ("ottx_benchmark_halfstep", ctypes.c_int,
+     [ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p,
+      ctypes.c_size_t, ctypes.c_void_p]),
 ]
 
 EXPORTED_SYMBOLS = tuple(name for name, _, _ in _SIGNATURES)

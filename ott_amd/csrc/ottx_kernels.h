@@ -98,68 +98,58 @@ __device__ inline void finalize_block(const HalfStepArgs& p) {
 }
 
 // ---------------------------------------------------------------------------
-// The fused half-step.
+// Streaming core of the fused half-step, shared by the single-problem and the
+// batched kernels.
 //   PREC 1 (bf16 split): KK = K blocks of 32 features, fragments [rb][kb][hi|lo][lane][8 bf16]
 //   PREC 0 (fp32)      : KK = k-steps of 4 features,  fragments [rb][kq][lane][4 f32]
-//   NB  = 16-column blocks per wave, MT = 16-row blocks per streamed tile,
-//   W   = waves per workgroup.
+//   NB  = 16-column blocks per wave, MT = 16-row blocks per streamed tile.
 // MFMA operand layouts (16x16 output): lane l holds A row (l % 16), B column
 // (l % 16), and output rows 4 (l / 16) + r, r < 4, of column l % 16.
 // ---------------------------------------------------------------------------
-template <int PREC, int KK, int NB, int MT, int W>
-__global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) {
-  if ((p.mode & kModeCheckDone) && p.state->done) return;
-  constexpr int NT = W * 64;
-  constexpr int COLS = W * NB * 16;
-  constexpr int KQ = (KK + 3) / 4;
-  __shared__ float2 s_ms[COLS];
-  __shared__ double s_red[2][NT];
-  __shared__ int s_last;
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int tile = blockIdx.x;
-  const int split = blockIdx.y;
-  const int cb0 = (tile * W + wave) * NB;
-  const float c = p.c;
-
-  // Out-side operands stay in registers for the whole stream.
+template <int PREC, int KK, int NB>
+struct OutOperand {
+  static constexpr int KQ = (KK + 3) / 4;
   bf16x8 bh[PREC == 1 ? NB : 1][PREC == 1 ? KK : 1];
   bf16x8 bl[PREC == 1 ? NB : 1][PREC == 1 ? KK : 1];
   float4 bq[PREC == 0 ? NB : 1][PREC == 0 ? KQ : 1];
-  if constexpr (PREC == 1) {
-    const uint4* __restrict__ of = static_cast<const uint4*>(p.out_frag);
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-      for (int kb = 0; kb < KK; ++kb) {
-        const size_t idx = ((static_cast<size_t>(cb0 + nb) * KK + kb) * 2) * 64 + lane;
-        const uint4 h = of[idx];
-        const uint4 l = of[idx + 64];
-        bh[nb][kb] = __builtin_bit_cast(bf16x8, h);
-        bl[nb][kb] = __builtin_bit_cast(bf16x8, l);
-      }
-  } else {
-    const float4* __restrict__ of = static_cast<const float4*>(p.out_frag);
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-      for (int q = 0; q < KQ; ++q)
-        bq[nb][q] = of[(static_cast<size_t>(cb0 + nb) * KQ + q) * 64 + lane];
-  }
 
-  float mx[NB], sm[NB];
+  // Fragments of column blocks cb0 .. cb0 + NB - 1 (16 points each).
+  __device__ __forceinline__ void load(const void* frag, int64_t cb0, int lane) {
+    if constexpr (PREC == 1) {
+      const uint4* __restrict__ of = static_cast<const uint4*>(frag);
 #pragma unroll
-  for (int nb = 0; nb < NB; ++nb) {
-    mx[nb] = -INFINITY;
-    sm[nb] = 0.f;
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int kb = 0; kb < KK; ++kb) {
+          const size_t idx = ((static_cast<size_t>(cb0 + nb) * KK + kb) * 2) * 64 + lane;
+          bh[nb][kb] = __builtin_bit_cast(bf16x8, of[idx]);
+          bl[nb][kb] = __builtin_bit_cast(bf16x8, of[idx + 64]);
+        }
+    } else {
+      const float4* __restrict__ of = static_cast<const float4*>(frag);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) bq[nb][q] = of[(static_cast<size_t>(cb0 + nb) * KQ + q) * 64 + lane];
+    }
   }
+};
 
-  const int rb_begin = split * p.rb_per_split;
-  const int rb_end = min(rb_begin + p.rb_per_split, p.n_in_rb);
+__device__ __forceinline__ float f4_get(const float4& v, int e) {
+  return (e == 0) ? v.x : (e == 1) ? v.y : (e == 2) ? v.z : v.w;
+}
+
+// Streams in-side row blocks [rb_begin, rb_end) (multiples of MT) against the
+// NB column blocks held in B, updating each lane's running (max, sum) of
+// exp2(c * acc) per column.  in_frag / in_pot / in_norm are relative to the
+// first row of the in side; rows i >= n_in get a -inf bias.
+template <int PREC, int KK, int NB, int MT>
+__device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, const void* in_frag,
+                                            const float* __restrict__ in_pot, const float* __restrict__ in_norm,
+                                            int64_t n_in, int rb_begin, int rb_end, float inv2s, float c,
+                                            int lane, float (&mx)[NB], float (&sm)[NB]) {
+  constexpr int KQ = (KK + 3) / 4;
   const int rsub = (lane >> 4) * 4;
-
   for (int rb = rb_begin; rb < rb_end; rb += MT) {
     f32x4 acc[MT][NB];
 #pragma unroll
@@ -169,10 +159,10 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t i = row0 + r;
-        bias[r] = (i < p.n_in) ? (p.in_pot[i] - p.in_norm[i]) * p.inv2s : -INFINITY;
+        bias[r] = (i < n_in) ? (in_pot[i] - in_norm[i]) * inv2s : -INFINITY;
       }
       if constexpr (PREC == 1) {
-        const uint4* __restrict__ inf = static_cast<const uint4*>(p.in_frag);
+        const uint4* __restrict__ inf = static_cast<const uint4*>(in_frag);
         bf16x8 ah[KK], al[KK];
 #pragma unroll
         for (int kb = 0; kb < KK; ++kb) {
@@ -182,42 +172,40 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
         }
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], bh[nb][0], bias, 0, 0, 0);
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], B.bh[nb][0], bias, 0, 0, 0);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], bl[nb][0], acc[mt][nb], 0, 0, 0);
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], B.bl[nb][0], acc[mt][nb], 0, 0, 0);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0], bh[nb][0], acc[mt][nb], 0, 0, 0);
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0], B.bh[nb][0], acc[mt][nb], 0, 0, 0);
 #pragma unroll
         for (int kb = 1; kb < KK; ++kb) {
 #pragma unroll
           for (int nb = 0; nb < NB; ++nb)
-            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], bh[nb][kb], acc[mt][nb], 0, 0, 0);
+            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], B.bh[nb][kb], acc[mt][nb], 0, 0, 0);
 #pragma unroll
           for (int nb = 0; nb < NB; ++nb)
-            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], bl[nb][kb], acc[mt][nb], 0, 0, 0);
+            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], B.bl[nb][kb], acc[mt][nb], 0, 0, 0);
 #pragma unroll
           for (int nb = 0; nb < NB; ++nb)
-            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kb], bh[nb][kb], acc[mt][nb], 0, 0, 0);
+            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kb], B.bh[nb][kb], acc[mt][nb], 0, 0, 0);
         }
       } else {
-        const float4* __restrict__ inf = static_cast<const float4*>(p.in_frag);
+        const float4* __restrict__ inf = static_cast<const float4*>(in_frag);
         float4 aq[KQ];
 #pragma unroll
         for (int q = 0; q < KQ; ++q) aq[q] = inf[(static_cast<size_t>(rb + mt) * KQ + q) * 64 + lane];
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[0].x, bq[nb][0].x, bias, 0, 0, 0);
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[0].x, B.bq[nb][0].x, bias, 0, 0, 0);
 #pragma unroll
         for (int ks = 1; ks < KK; ++ks) {
-          const int q = ks >> 2, e = ks & 3;
 #pragma unroll
-          for (int nb = 0; nb < NB; ++nb) {
-            const float av = (e == 0) ? aq[q].x : (e == 1) ? aq[q].y : (e == 2) ? aq[q].z : aq[q].w;
-            const float bv = (e == 0) ? bq[nb][q].x : (e == 1) ? bq[nb][q].y : (e == 2) ? bq[nb][q].z : bq[nb][q].w;
-            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[mt][nb], 0, 0, 0);
-          }
+          for (int nb = 0; nb < NB; ++nb)
+            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(f4_get(aq[ks >> 2], ks & 3),
+                                                               f4_get(B.bq[nb][ks >> 2], ks & 3), acc[mt][nb], 0,
+                                                               0, 0);
         }
       }
     }
@@ -242,8 +230,13 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
       }
     }
   }
+}
 
-  // Combine the four 16-lane row groups of each column.
+// Combine the four 16-lane row groups of each column; lanes < 16 store the
+// column's (max, sum) to s_ms[(wave * NB + nb) * 16 + lane].
+template <int NB>
+__device__ __forceinline__ void combine_lanes(float (&mx)[NB], float (&sm)[NB], float c, int lane, int wave,
+                                              float2* s_ms) {
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
 #pragma unroll
@@ -254,6 +247,106 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
     }
     if (lane < 16) s_ms[(wave * NB + nb) * 16 + lane] = make_float2(mx[nb], sm[nb]);
   }
+}
+
+// Scalars of the column epilogue.
+struct EpiScalars {
+  uint32_t mode;
+  int32_t norm_p;
+  float two_s, eps, eps_marg, tau, inv_rho, momentum;
+};
+
+// Epilogue of one out-side point j (pointers already offset to the problem):
+// turns the column's (max, sum) into apply_lse_kernel's value ``app`` =
+// eps * LSE_i((h_i - C_ij) / eps), the update of geometry.py:376-398 scaled by
+// tau (sinkhorn.py:836-855) and blended by constant momentum
+// (acceleration.py:161-172), and the marginal-error / mass terms of
+// sinkhorn.py:191-221 with the unbalanced target of unbalanced_functions.py:30-50.
+__device__ __forceinline__ void column_epilogue(const EpiScalars& e, float2 ms, int64_t j,
+                                                const float* __restrict__ out_norm, float* out_pot,
+                                                const float* __restrict__ out_w, const float* __restrict__ out_logw,
+                                                float* out_save, float* out_vec, double& e_acc, double& m_acc) {
+  float app;
+  if (ms.x == -INFINITY || !(ms.y > 0.f)) {
+    app = (ms.y != ms.y) ? ms.y : -INFINITY;
+  } else {
+    app = fmaf(e.two_s, ms.x, -out_norm[j]) + e.eps * logf(ms.y);
+  }
+  const float h_old = out_pot[j];
+  float h_marg = h_old;
+  if (e.mode & kModeWritePot) {
+    const float lw = out_logw ? out_logw[j] : logf(out_w[j]);
+    float upd = e.eps * lw - (isfinite(app) ? app : 0.f);
+    upd *= e.tau;
+    float nv = upd;
+    if (e.momentum != 1.f) nv = (1.f - e.momentum) * (isfinite(h_old) ? h_old : 0.f) + e.momentum * upd;
+    out_pot[j] = nv;
+    if (e.mode & kModeErrRow) h_marg = nv;
+  }
+  if (e.mode & kModeSaveOld) out_save[j] = h_old;
+  if (e.mode & (kModeErrCol | kModeErrRow)) {
+    const float marg = expf((h_marg + app) / e.eps_marg);
+    const float w = out_w[j];
+    const float target = (e.tau == 1.f) ? w : ((w > 0.f) ? w * expf(-h_marg * e.inv_rho) : 0.f);
+    const double diff = fabs(static_cast<double>(marg) - static_cast<double>(target));
+    e_acc += (e.norm_p == 1) ? diff : pow(diff, static_cast<double>(e.norm_p));
+    m_acc += static_cast<double>(marg);
+  }
+  if (e.mode & kModeApply) out_vec[j] = (h_old == -INFINITY || h_old != h_old) ? h_old : app;
+  if (e.mode & kModeMarginal) out_vec[j] = expf((h_old + app) / e.eps_marg);
+}
+
+// Deterministic tree sum of two float64 values over the workgroup; the
+// result is valid in thread 0.
+template <int NT>
+__device__ __forceinline__ void block_sum2(double (*s_red)[NT], int tid, double a, double b) {
+  s_red[0][tid] = a;
+  s_red[1][tid] = b;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      s_red[0][tid] += s_red[0][tid + s];
+      s_red[1][tid] += s_red[1][tid + s];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Single-problem half-step: grid (column tiles, row splits), W waves each
+// owning NB column blocks; the last split to arrive on a column tile merges
+// the partial (max, sum) pairs and runs the epilogue.
+// ---------------------------------------------------------------------------
+template <int PREC, int KK, int NB, int MT, int W>
+__global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) {
+  if ((p.mode & kModeCheckDone) && p.state->done) return;
+  constexpr int NT = W * 64;
+  constexpr int COLS = W * NB * 16;
+  __shared__ float2 s_ms[COLS];
+  __shared__ double s_red[2][NT];
+  __shared__ int s_last;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int tile = blockIdx.x;
+  const int split = blockIdx.y;
+  const float c = p.c;
+
+  OutOperand<PREC, KK, NB> B;
+  B.load(p.out_frag, static_cast<int64_t>(tile * W + wave) * NB, lane);
+  float mx[NB], sm[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    mx[nb] = -INFINITY;
+    sm[nb] = 0.f;
+  }
+  const int rb_begin = split * p.rb_per_split;
+  const int rb_end = min(rb_begin + p.rb_per_split, p.n_in_rb);
+  stream_rows<PREC, KK, NB, MT>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s, c, lane, mx,
+                                sm);
+  combine_lanes<NB>(mx, sm, c, lane, wave, s_ms);
   __syncthreads();
 
   const int64_t col0 = static_cast<int64_t>(tile) * COLS;
@@ -282,61 +375,23 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
     __syncthreads();
   }
 
-  // Epilogue: one column per thread.
-  const uint32_t mode = p.mode;
-  const bool want_err = (mode & (kModeErrCol | kModeErrRow)) != 0;
+  const EpiScalars es{p.mode, p.norm_p, p.two_s, p.eps, p.eps_marg, p.tau, p.inv_rho, p.momentum};
+  const bool want_err = (p.mode & (kModeErrCol | kModeErrRow)) != 0;
   double e_acc = 0.0, m_acc = 0.0;
   for (int t = tid; t < COLS; t += NT) {
     const int64_t j = col0 + t;
     if (j >= p.n_out) continue;
-    const float2 ms = s_ms[t];
-    float app;
-    if (ms.x == -INFINITY || !(ms.y > 0.f)) {
-      app = (ms.y != ms.y) ? ms.y : -INFINITY;
-    } else {
-      app = fmaf(p.two_s, ms.x, -p.out_norm[j]) + p.eps * logf(ms.y);
-    }
-    const float h_old = p.out_pot[j];
-    float h_marg = h_old;
-    if (mode & kModeWritePot) {
-      const float lw = p.out_logw ? p.out_logw[j] : logf(p.out_w[j]);
-      float upd = p.eps * lw - (isfinite(app) ? app : 0.f);
-      upd *= p.tau;
-      float nv = upd;
-      if (p.momentum != 1.f) nv = (1.f - p.momentum) * (isfinite(h_old) ? h_old : 0.f) + p.momentum * upd;
-      p.out_pot[j] = nv;
-      if (mode & kModeErrRow) h_marg = nv;
-    }
-    if (mode & kModeSaveOld) p.out_save[j] = h_old;
-    if (want_err) {
-      const float marg = expf((h_marg + app) / p.eps_marg);
-      const float w = p.out_w[j];
-      const float target = (p.tau == 1.f) ? w : ((w > 0.f) ? w * expf(-h_marg * p.inv_rho) : 0.f);
-      const double diff = fabs(static_cast<double>(marg) - static_cast<double>(target));
-      e_acc += (p.norm_p == 1) ? diff : pow(diff, static_cast<double>(p.norm_p));
-      m_acc += static_cast<double>(marg);
-    }
-    if (mode & kModeApply) p.out_vec[j] = (h_old == -INFINITY || h_old != h_old) ? h_old : app;
-    if (mode & kModeMarginal) p.out_vec[j] = expf((h_old + app) / p.eps_marg);
+    column_epilogue(es, s_ms[t], j, p.out_norm, p.out_pot, p.out_w, p.out_logw, p.out_save, p.out_vec, e_acc,
+                    m_acc);
   }
 
   if (want_err) {
-    s_red[0][tid] = e_acc;
-    s_red[1][tid] = m_acc;
-    __syncthreads();
-#pragma unroll
-    for (int s = NT / 2; s > 0; s >>= 1) {
-      if (tid < s) {
-        s_red[0][tid] += s_red[0][tid + s];
-        s_red[1][tid] += s_red[1][tid + s];
-      }
-      __syncthreads();
-    }
+    block_sum2<NT>(s_red, tid, e_acc, m_acc);
     if (tid == 0) {
       p.err_partial[tile] = s_red[0][0];
       if (p.mass_partial) p.mass_partial[tile] = s_red[1][0];
     }
-    if ((mode & kModeErrCol) && p.state != nullptr) {
+    if ((p.mode & kModeErrCol) && p.state != nullptr) {
       __threadfence();
       __syncthreads();
       if (tid == 0) {

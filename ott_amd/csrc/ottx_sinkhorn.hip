@@ -799,3 +799,54 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
 }
 
 }  // extern "C"
+
+extern "C" int ottx_benchmark_halfstep(const ottx_problem* pb, int32_t axis, int32_t reps, float* f, float* g,
+                                       double* avg_ms_host, void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_problem(pb);
+  if (rc != OTTX_OK) return rc;
+  if ((axis != 0 && axis != 1) || reps < 1 || !f || !g || !avg_ms_host || !workspace) return OTTX_EINVAL;
+  KernelCfg k;
+  if (!select_kernel(pb->precision, pb->d, &k)) return OTTX_EUNSUPPORTED;
+  const int cus = device_cus();
+  const PassPlan gp = plan_pass(k, pb->n, pb->m, cus);
+  const PassPlan fp = plan_pass(k, pb->m, pb->n, cus);
+  const Layout L = make_layout(pb, k, gp, fp, 1);
+  if (workspace_bytes < L.total) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Prepared P;
+  rc = prepare(pb, k, L, workspace, st, &P);
+  if (rc != OTTX_OK) return rc;
+  uint32_t* counters = reinterpret_cast<uint32_t*>(P.base + L.counters);
+  SolveState* state = reinterpret_cast<SolveState*>(P.base + L.state);
+  float* dummy_err = reinterpret_cast<float*>(P.base + L.dual);
+  hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(256), 0, st, state, dummy_err, 0, counters,
+                     std::max(gp.tiles, fp.tiles));
+  OTTX_LAUNCHED();
+  HalfStepArgs a = axis == 0 ? base_args(pb, k, gp, P, 1, f, g, pb->epsilon)
+                             : base_args(pb, k, fp, P, 0, g, f, pb->epsilon);
+  a.state = nullptr;
+  a.mode = kModeWritePot;
+  for (int w = 0; w < 2; ++w) {
+    rc = launch_halfstep(k, a, st);
+    if (rc != OTTX_OK) return rc;
+  }
+  hipEvent_t e0, e1;
+  OTTX_HIP(hipEventCreate(&e0));
+  OTTX_HIP(hipEventCreate(&e1));
+  OTTX_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r) {
+    rc = launch_halfstep(k, a, st);
+    if (rc != OTTX_OK) break;
+  }
+  OTTX_HIP(hipEventRecord(e1, st));
+  OTTX_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  const hipError_t e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != OTTX_OK) return rc;
+  if (e != hipSuccess) return OTTX_EHIP;
+  *avg_ms_host = static_cast<double>(ms) / reps;
+  return OTTX_OK;
+}
