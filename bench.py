@@ -112,7 +112,7 @@ def halfstep_roofline(pb, precision, reps=20):
   g = torch.zeros(m, device=geom.device)
   avg = ctypes.c_double(0.0)
   _lib.check("ottx_benchmark_halfstep", lib.ottx_benchmark_halfstep(
-      ctypes.byref(prob), 0, reps, f.data_ptr(), g.data_ptr(), ctypes.byref(avg), ws.data_ptr(),
+      ctypes.byref(prob), 0, -1, reps, f.data_ptr(), g.data_ptr(), ctypes.byref(avg), ws.data_ptr(),
       ws.numel(), _lib.stream_ptr()))
   del keep
   ms = avg.value

@@ -63,10 +63,13 @@ typedef struct ottx_problem {
   const float* y;          /* [m, d] */
   const float* a;          /* [n] weights, >= 0 */
   const float* b;          /* [m] weights, >= 0 */
-  double epsilon;          /* resolved target epsilon, > 0 (geometry.py:173-176) */
+  double epsilon;          /* target epsilon, > 0, used when relative_epsilon == 0 */
   double inv_scale_cost;   /* s in C = s * ||x - y||^2, > 0 (pointcloud.py:334-340) */
   double tau_a;            /* in (0, 1] (linear_problem.py:51-63) */
   double tau_b;
+  double relative_epsilon; /* > 0: epsilon = relative_epsilon * mean(C), resolved on the
+                              device by ottx_sinkhorn_solve (relative_epsilon="mean",
+                              geometry.py:147-171); 0: use epsilon */
 } ottx_problem;
 
 /* Options of the Sinkhorn solver: solvers/linear/sinkhorn.py:718-780. */
@@ -77,7 +80,7 @@ typedef struct ottx_sinkhorn_opts {
   int32_t min_iterations;   /* default 0 */
   int32_t max_iterations;   /* default 2000 */
   double momentum;          /* constant Momentum weight (acceleration.py:161-172), 1 = none */
-  int32_t tile_config;      /* -1 = tuned default, else an index from ottx_tile_configs */
+  int32_t tile_config;      /* -1 = default, else a value listed by ottx_tile_configs */
   int32_t use_graph;        /* 1 = replay each block of iterations as a hipGraph */
 } ottx_sinkhorn_opts;
 
@@ -87,6 +90,7 @@ typedef struct ottx_sinkhorn_out {
   float* g;                 /* [m] device: initial potential in, final potential out */
   float* errors;            /* [ceil(max_iterations / inner_iterations)] device */
   double reg_ot_cost;       /* host, written on return (sinkhorn.py:224-277) */
+  double epsilon;           /* host: the target epsilon the solve used */
   double total_mass;        /* host: sum of the coupling at (f, g) */
   int32_t converged;        /* host, sinkhorn.py:1020-1023 */
   int32_t n_iters;          /* host, sinkhorn.py:431-434 */
@@ -141,15 +145,26 @@ int ottx_sinkhorn_solve(const ottx_problem* problem,
                         const ottx_sinkhorn_opts* opts, ottx_sinkhorn_out* out,
                         void* workspace, size_t workspace_bytes, void* stream);
 
+/* Tile configurations of the fused half-step for (precision, d): the search
+ * space of the autotuner (no reference counterpart; OTT leaves tiling to XLA).
+ * Returns the number of configurations; when desc is non-NULL writes up to
+ * max_configs records of 5 int32: {tile_config, NB (16-column blocks per wave),
+ * MT (16-row blocks per streamed tile), W (waves per workgroup), target
+ * workgroups per CU}.  tile_config is what ottx_sinkhorn_opts.tile_config and
+ * ottx_benchmark_halfstep accept (-1 = default). */
+int ottx_tile_configs(int32_t precision, int32_t d, int32_t* desc,
+                      int32_t max_configs);
+
 /* Measurement helper (no reference counterpart; used by bench.py for the
- * roofline figure).  Prepares the problem once, then runs ``reps`` real
- * half-steps back to back on ``stream`` -- the g-update (axis 0: new g from f)
- * or the f-update (axis 1) -- bracketed by hipEvents, and writes the mean
- * duration of one half-step launch in milliseconds to *avg_ms_host.  f and g
- * are updated in place.  Uses the workspace of ottx_sinkhorn_workspace_bytes. */
+ * roofline figure and by the autotuner).  Prepares the problem once, then
+ * runs ``reps`` real half-steps back to back on ``stream`` -- the g-update
+ * (axis 0: new g from f) or the f-update (axis 1) -- with the given
+ * tile_config, bracketed by hipEvents, and writes the mean duration of one
+ * half-step launch in milliseconds to *avg_ms_host.  f and g are updated in
+ * place.  Uses the workspace of ottx_sinkhorn_workspace_bytes. */
 int ottx_benchmark_halfstep(const ottx_problem* problem, int32_t axis,
-                            int32_t reps, float* f, float* g,
-                            double* avg_ms_host, void* workspace,
+                            int32_t tile_config, int32_t reps, float* f,
+                            float* g, double* avg_ms_host, void* workspace,
                             size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus

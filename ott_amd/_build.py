@@ -27,6 +27,9 @@ HIPCC_FLAGS = [
     "-shared",
     # Packed FP32 instructions are kept out of every kernel.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results stay in architectural VGPRs, so the exp/log-sum-exp
+    # epilogue reads them without v_accvgpr_read copies.
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 

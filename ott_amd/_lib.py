@@ -15,7 +15,8 @@ from typing import Optional
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libottx.so")
+# OTTX_LIBRARY overrides the in-tree build (for A/B comparisons of builds).
+LIB_PATH = os.environ.get("OTTX_LIBRARY", os.path.join(_HERE, "lib", "libottx.so"))
 
 OTTX_OK = 0
 OTTX_EINVAL = -1
@@ -46,6 +47,7 @@ class Problem(ctypes.Structure):
       ("inv_scale_cost", ctypes.c_double),
       ("tau_a", ctypes.c_double),
       ("tau_b", ctypes.c_double),
+      ("relative_epsilon", ctypes.c_double),
   ]
 
 
@@ -68,6 +70,7 @@ class SinkhornOut(ctypes.Structure):
       ("g", ctypes.c_void_p),
       ("errors", ctypes.c_void_p),
       ("reg_ot_cost", ctypes.c_double),
+      ("epsilon", ctypes.c_double),
       ("total_mass", ctypes.c_double),
       ("converged", ctypes.c_int32),
       ("n_iters", ctypes.c_int32),
@@ -97,10 +100,12 @@ _SIGNATURES = [
      [ctypes.POINTER(Problem), ctypes.POINTER(SinkhornOpts),
       ctypes.POINTER(SinkhornOut), ctypes.c_void_p, ctypes.c_size_t,
       ctypes.c_void_p]),
+    ("ottx_tile_configs", ctypes.c_int,
+     [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     ("ottx_benchmark_halfstep", ctypes.c_int,
-     [ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
-      ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p,
-      ctypes.c_size_t, ctypes.c_void_p]),
+     [ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+      ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
 ]
 
 EXPORTED_SYMBOLS = tuple(name for name, _, _ in _SIGNATURES)

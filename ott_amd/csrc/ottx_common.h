@@ -10,6 +10,7 @@ namespace ottx {
 
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
 
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLn2 = 0.6931471805599453f;
@@ -17,7 +18,7 @@ constexpr int kWave = 64;
 
 // Rows of every packed point array are padded to this multiple, so that any
 // streamed row tile and any output column tile stays inside the allocation.
-constexpr int kRowPad = 256;
+constexpr int kRowPad = 512;
 
 // Half-step mode bits (uniform per launch).
 enum : uint32_t {
@@ -37,9 +38,19 @@ struct SolveState {
   int32_t n_blocks;    // blocks run when the loop stopped
   int32_t pad0;
   uint32_t tiles_done; // arrival counter of the column tiles of an error pass
-  uint32_t pad1[3];
+  int32_t shift_ok[2]; // a log-sum-exp per column exists for side 0 (f) / 1 (g)
+  uint32_t pad1;
   double total_mass;   // sum of the coupling at the last error pass
   double last_err;     // last raw error (before the inf/record rule)
+};
+
+// Scalars resolved on the device at solve start (relative epsilon).
+struct SolveScalars {
+  double eps;        // target epsilon
+  float eps_f;       // float copy used by the kernels
+  float c;           // 2 s log2(e) / eps
+  float inv_rho_a;   // (1 - tau_a) / (eps tau_a), 0 when tau_a == 1
+  float inv_rho_b;
 };
 
 // Arguments of one half-step launch: the out side gets the new potential, the
@@ -72,6 +83,10 @@ struct HalfStepArgs {
   float inv_rho;        // 1 / rho_out (0 when tau == 1)
   float momentum;       // constant momentum weight
   int32_t balanced;
+  int32_t side;         // 1: out = y (tau_b), 0: out = x (tau_a)
+  const SolveScalars* scal;  // when set, overrides c, eps, eps_marg and inv_rho
+  const float* lse_shift;    // per-column log-sum-exp (acc units) of the previous same-side pass
+  float* lse_store;          // where this pass stores its per-column log-sum-exp
   float2* partial;      // [splits][n_out_pad] running (max, sum) per column
   uint32_t* tile_counter;   // [tiles]
   double* err_partial;  // [tiles] column (or row) error partials

@@ -139,95 +139,206 @@ __device__ __forceinline__ float f4_get(const float4& v, int e) {
   return (e == 0) ? v.x : (e == 1) ? v.y : (e == 2) ? v.z : v.w;
 }
 
+// Number of 1 KiB fragment chunks per 16-row block.
+template <int PREC, int KK>
+struct FragChunks {
+  static constexpr int value = (PREC == 1) ? KK * 2 : (KK + 3) / 4;
+};
+
+// Loads one streamed tile (contiguous fragment chunks of MT row blocks) and
+// its row biases into registers.
+template <int TILE_U4, int NT, int PER_T, int ROWS, int CH>
+__device__ __forceinline__ void tile_prefetch(const u32x4* __restrict__ src, const float* __restrict__ in_pot,
+                                              const float* __restrict__ in_norm, int64_t n_in, int rb, float inv2s,
+                                              int tid, u32x4 (&pre)[PER_T], float& pre_bias) {
+  const u32x4* base = src + static_cast<size_t>(rb) * CH * 64;
+#pragma unroll
+  for (int q = 0; q < PER_T; ++q) {
+    const int idx = tid + q * NT;
+    if (TILE_U4 % NT == 0 || idx < TILE_U4) pre[q] = base[idx];
+  }
+  if (tid < ROWS) {
+    const int64_t i = static_cast<int64_t>(rb) * 16 + tid;
+    pre_bias = (i < n_in) ? (in_pot[i] - in_norm[i]) * inv2s : -INFINITY;
+  }
+}
+
 // Streams in-side row blocks [rb_begin, rb_end) (multiples of MT) against the
 // NB column blocks held in B, updating each lane's running (max, sum) of
 // exp2(c * acc) per column.  in_frag / in_pot / in_norm are relative to the
 // first row of the in side; rows i >= n_in get a -inf bias.
-template <int PREC, int KK, int NB, int MT>
+//
+// SHIFT: mx[] holds a fixed per-column shift (the column's log-sum-exp from
+// the previous half-step of the same side) and only the sum is accumulated;
+// otherwise mx[] is the running max, rescaled once per tile.
+//
+// The W waves of the workgroup stream the same rows: each tile of MT row
+// blocks (contiguous in the fragment layout) and its row biases are copied
+// once into LDS (double-buffered, next tile prefetched into registers while
+// the current one is on MFMA), and every wave reads its A operands from LDS.
+// One streamed tile (in LDS at L, row biases at LB) against the NB column
+// blocks in B: MFMA cross terms into acc, then the log-sum-exp update of
+// (mx, sm).  SHIFT: mx[] is a fixed per-column shift and only sm is summed;
+// otherwise mx[] is the running max, rescaled once per tile.
+template <int PREC, int KK, int NB, int MT, bool SHIFT>
+__device__ __forceinline__ void compute_tile(const OutOperand<PREC, KK, NB>& B, const u32x4* L, const float* LB,
+                                             int lane, float c, float (&mx)[NB], float (&sm)[NB]) {
+  constexpr int KQ = (KK + 3) / 4;
+  const int rsub = (lane >> 4) * 4;
+  f32x4 acc[MT][NB];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const float4 bv = *reinterpret_cast<const float4*>(LB + mt * 16 + rsub);
+    f32x4 bias;
+    bias[0] = bv.x;
+    bias[1] = bv.y;
+    bias[2] = bv.z;
+    bias[3] = bv.w;
+    if constexpr (PREC == 1) {
+      bf16x8 ah[KK], al[KK];
+#pragma unroll
+      for (int kb = 0; kb < KK; ++kb) {
+        ah[kb] = __builtin_bit_cast(bf16x8, L[((mt * KK + kb) * 2) * 64 + lane]);
+        al[kb] = __builtin_bit_cast(bf16x8, L[((mt * KK + kb) * 2 + 1) * 64 + lane]);
+      }
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], B.bh[nb][0], bias, 0, 0, 0);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], B.bl[nb][0], acc[mt][nb], 0, 0, 0);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0], B.bh[nb][0], acc[mt][nb], 0, 0, 0);
+#pragma unroll
+      for (int kb = 1; kb < KK; ++kb) {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], B.bh[nb][kb], acc[mt][nb], 0, 0, 0);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], B.bl[nb][kb], acc[mt][nb], 0, 0, 0);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kb], B.bh[nb][kb], acc[mt][nb], 0, 0, 0);
+      }
+    } else {
+      float4 aq[KQ];
+#pragma unroll
+      for (int q = 0; q < KQ; ++q) aq[q] = __builtin_bit_cast(float4, L[(mt * KQ + q) * 64 + lane]);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[0].x, B.bq[nb][0].x, bias, 0, 0, 0);
+#pragma unroll
+      for (int ks = 1; ks < KK; ++ks) {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(f4_get(aq[ks >> 2], ks & 3),
+                                                             f4_get(B.bq[nb][ks >> 2], ks & 3), acc[mt][nb], 0,
+                                                             0, 0);
+      }
+    }
+  }
+  if constexpr (SHIFT) {
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const float nmc = -c * mx[nb];
+      float part[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) part[r] = exp2_fast(fmaf(acc[0][nb][r], c, nmc));
+#pragma unroll
+      for (int mt = 1; mt < MT; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[r] += exp2_fast(fmaf(acc[mt][nb][r], c, nmc));
+      sm[nb] += (part[0] + part[1]) + (part[2] + part[3]);
+    }
+  } else {
+    // Streaming log-sum-exp, base 2: one rescale per column per tile.  The
+    // tile max is a tree and the sum uses four partial accumulators, so no
+    // VALU chain is longer than MT + 2 dependent operations.
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      float m4[MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+        m4[mt] = fmaxf(fmaxf(acc[mt][nb][0], acc[mt][nb][1]), fmaxf(acc[mt][nb][2], acc[mt][nb][3]));
+#pragma unroll
+      for (int w = 1; w < MT; w <<= 1)
+#pragma unroll
+        for (int mt = 0; mt + w < MT; mt += 2 * w) m4[mt] = fmaxf(m4[mt], m4[mt + w]);
+      const float mnew = fmaxf(mx[nb], m4[0]);
+      if (mnew != -INFINITY) {
+        const float nmc = -c * mnew;
+        float part[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[r] = exp2_fast(fmaf(acc[0][nb][r], c, nmc));
+#pragma unroll
+        for (int mt = 1; mt < MT; ++mt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) part[r] += exp2_fast(fmaf(acc[mt][nb][r], c, nmc));
+        sm[nb] = fmaf(sm[nb], exp2_fast(fmaf(mx[nb], c, nmc)), (part[0] + part[1]) + (part[2] + part[3]));
+        mx[nb] = mnew;
+      }
+    }
+  }}
+
+// Streams in-side row blocks [rb_begin, rb_end) (multiples of MT) against the
+// NB column blocks held in B, updating each lane's running (max, sum) of
+// exp2(c * acc) per column.  in_frag / in_pot / in_norm are relative to the
+// first row of the in side; rows i >= n_in get a -inf bias.
+//
+// The W waves of the workgroup stream the same rows: each tile of MT row
+// blocks (contiguous in the fragment layout) and its row biases are copied
+// once into LDS and every wave reads its A operands from there.  Two tiles are
+// in flight in registers ahead of the one on MFMA (prefetch distance 2) and
+// LDS is double-buffered, with one barrier per tile.
+template <int PREC, int KK, int NB, int MT, int W, bool SHIFT>
 __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, const void* in_frag,
                                             const float* __restrict__ in_pot, const float* __restrict__ in_norm,
                                             int64_t n_in, int rb_begin, int rb_end, float inv2s, float c,
-                                            int lane, float (&mx)[NB], float (&sm)[NB]) {
-  constexpr int KQ = (KK + 3) / 4;
-  const int rsub = (lane >> 4) * 4;
-  for (int rb = rb_begin; rb < rb_end; rb += MT) {
-    f32x4 acc[MT][NB];
+                                            int tid, float (&mx)[NB], float (&sm)[NB], u32x4* s_tile,
+                                            float* s_bias) {
+  constexpr int CH = FragChunks<PREC, KK>::value;
+  constexpr int NT = W * 64;
+  constexpr int TILE_U4 = MT * CH * 64;
+  constexpr int PER_T = (TILE_U4 + NT - 1) / NT;
+  constexpr int ROWS = MT * 16;
+  const int lane = tid & 63;
+  const u32x4* __restrict__ src = static_cast<const u32x4*>(in_frag);
+  u32x4* L0 = s_tile;
+  u32x4* L1 = s_tile + TILE_U4;
+  float* B0 = s_bias;
+  float* B1 = s_bias + ROWS;
+
+  u32x4 pre0[PER_T], pre1[PER_T];
+  float pb0 = -INFINITY, pb1 = -INFINITY;
+  if (rb_begin < rb_end)
+    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb_begin, inv2s, tid, pre0, pb0);
+  if (rb_begin + MT < rb_end)
+    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb_begin + MT, inv2s, tid, pre1, pb1);
+  for (int rb = rb_begin; rb < rb_end; rb += 2 * MT) {
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      const int64_t row0 = static_cast<int64_t>(rb + mt) * 16 + rsub;
-      f32x4 bias;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t i = row0 + r;
-        bias[r] = (i < n_in) ? (in_pot[i] - in_norm[i]) * inv2s : -INFINITY;
-      }
-      if constexpr (PREC == 1) {
-        const uint4* __restrict__ inf = static_cast<const uint4*>(in_frag);
-        bf16x8 ah[KK], al[KK];
-#pragma unroll
-        for (int kb = 0; kb < KK; ++kb) {
-          const size_t idx = ((static_cast<size_t>(rb + mt) * KK + kb) * 2) * 64 + lane;
-          ah[kb] = __builtin_bit_cast(bf16x8, inf[idx]);
-          al[kb] = __builtin_bit_cast(bf16x8, inf[idx + 64]);
-        }
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], B.bh[nb][0], bias, 0, 0, 0);
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], B.bl[nb][0], acc[mt][nb], 0, 0, 0);
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0], B.bh[nb][0], acc[mt][nb], 0, 0, 0);
-#pragma unroll
-        for (int kb = 1; kb < KK; ++kb) {
-#pragma unroll
-          for (int nb = 0; nb < NB; ++nb)
-            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], B.bh[nb][kb], acc[mt][nb], 0, 0, 0);
-#pragma unroll
-          for (int nb = 0; nb < NB; ++nb)
-            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], B.bl[nb][kb], acc[mt][nb], 0, 0, 0);
-#pragma unroll
-          for (int nb = 0; nb < NB; ++nb)
-            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kb], B.bh[nb][kb], acc[mt][nb], 0, 0, 0);
-        }
-      } else {
-        const float4* __restrict__ inf = static_cast<const float4*>(in_frag);
-        float4 aq[KQ];
-#pragma unroll
-        for (int q = 0; q < KQ; ++q) aq[q] = inf[(static_cast<size_t>(rb + mt) * KQ + q) * 64 + lane];
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[0].x, B.bq[nb][0].x, bias, 0, 0, 0);
-#pragma unroll
-        for (int ks = 1; ks < KK; ++ks) {
-#pragma unroll
-          for (int nb = 0; nb < NB; ++nb)
-            acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(f4_get(aq[ks >> 2], ks & 3),
-                                                               f4_get(B.bq[nb][ks >> 2], ks & 3), acc[mt][nb], 0,
-                                                               0, 0);
-        }
-      }
+    for (int q = 0; q < PER_T; ++q) {
+      const int idx = tid + q * NT;
+      if (TILE_U4 % NT == 0 || idx < TILE_U4) L0[idx] = pre0[q];
     }
-    // Streaming log-sum-exp, base 2: one rescale per column per tile.
+    if (tid < ROWS) B0[tid] = pb0;
+    __syncthreads();
+    if (rb + 2 * MT < rb_end)
+      tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb + 2 * MT, inv2s, tid, pre0, pb0);
+    compute_tile<PREC, KK, NB, MT, SHIFT>(B, L0, B0, lane, c, mx, sm);
+    if (rb + MT < rb_end) {
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-      float tmax = -INFINITY;
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) tmax = fmaxf(tmax, acc[mt][nb][r]);
-      const float mnew = fmaxf(mx[nb], tmax);
-      if (mnew != -INFINITY) {
-        const float nmc = -c * mnew;
-        float s = sm[nb] * exp2_fast(fmaf(mx[nb], c, nmc));
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) s += exp2_fast(fmaf(acc[mt][nb][r], c, nmc));
-        sm[nb] = s;
-        mx[nb] = mnew;
+      for (int q = 0; q < PER_T; ++q) {
+        const int idx = tid + q * NT;
+        if (TILE_U4 % NT == 0 || idx < TILE_U4) L1[idx] = pre1[q];
       }
+      if (tid < ROWS) B1[tid] = pb1;
+      __syncthreads();
+      if (rb + 3 * MT < rb_end)
+        tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb + 3 * MT, inv2s, tid, pre1,
+                                                    pb1);
+      compute_tile<PREC, KK, NB, MT, SHIFT>(B, L1, B1, lane, c, mx, sm);
     }
   }
 }
@@ -323,6 +434,9 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
   if ((p.mode & kModeCheckDone) && p.state->done) return;
   constexpr int NT = W * 64;
   constexpr int COLS = W * NB * 16;
+  constexpr int TILE_U4 = MT * FragChunks<PREC, KK>::value * 64;
+  __shared__ u32x4 s_tile[2 * TILE_U4];
+  __shared__ float s_bias[2 * MT * 16];
   __shared__ float2 s_ms[COLS];
   __shared__ double s_red[2][NT];
   __shared__ int s_last;
@@ -332,21 +446,63 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
   const int wave = tid >> 6;
   const int tile = blockIdx.x;
   const int split = blockIdx.y;
-  const float c = p.c;
+  float c = p.c, eps = p.eps, eps_marg = p.eps_marg, inv_rho = p.inv_rho;
+  if (p.scal != nullptr) {
+    c = p.scal->c;
+    eps = eps_marg = p.scal->eps_f;
+    inv_rho = p.side ? p.scal->inv_rho_b : p.scal->inv_rho_a;
+  }
 
   OutOperand<PREC, KK, NB> B;
   B.load(p.out_frag, static_cast<int64_t>(tile * W + wave) * NB, lane);
   float mx[NB], sm[NB];
-#pragma unroll
-  for (int nb = 0; nb < NB; ++nb) {
-    mx[nb] = -INFINITY;
-    sm[nb] = 0.f;
-  }
   const int rb_begin = split * p.rb_per_split;
   const int rb_end = min(rb_begin + p.rb_per_split, p.n_in_rb);
-  stream_rows<PREC, KK, NB, MT>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s, c, lane, mx,
-                                sm);
-  combine_lanes<NB>(mx, sm, c, lane, wave, s_ms);
+  const int64_t cbase = static_cast<int64_t>(tile * W + wave) * NB * 16 + (lane & 15);
+  // Fixed-shift streaming when the previous half-step of this side left a
+  // finite log-sum-exp for every column of the tile; the workgroup falls back
+  // to the running max if the resulting sums leave [2^-60, 2^60].
+  bool shifted = p.lse_shift != nullptr && p.state != nullptr && p.state->shift_ok[p.side] != 0;
+  if (shifted) {
+    if (tid == 0) s_last = 1;
+    __syncthreads();
+    bool ok = true;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      mx[nb] = p.lse_shift[cbase + nb * 16];
+      sm[nb] = 0.f;
+      ok = ok && isfinite(mx[nb]);
+    }
+    if (!ok) s_last = 0;
+    __syncthreads();
+    shifted = s_last != 0;
+    __syncthreads();
+  }
+  if (shifted) {
+    stream_rows<PREC, KK, NB, MT, W, true>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s, c,
+                                           tid, mx, sm, s_tile, s_bias);
+    combine_lanes<NB>(mx, sm, c, lane, wave, s_ms);
+    if (tid == 0) s_last = 1;
+    __syncthreads();
+    const bool has_rows = static_cast<int64_t>(rb_begin) * 16 < p.n_in;
+    for (int t = tid; t < COLS; t += NT) {
+      const float S = s_ms[t].y;
+      if (has_rows && !(S >= 8.673617e-19f && S <= 1.1529215e18f)) s_last = 0;
+    }
+    __syncthreads();
+    shifted = s_last != 0;
+    __syncthreads();
+  }
+  if (!shifted) {
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      mx[nb] = -INFINITY;
+      sm[nb] = 0.f;
+    }
+    stream_rows<PREC, KK, NB, MT, W, false>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s,
+                                            c, tid, mx, sm, s_tile, s_bias);
+    combine_lanes<NB>(mx, sm, c, lane, wave, s_ms);
+  }
   __syncthreads();
 
   const int64_t col0 = static_cast<int64_t>(tile) * COLS;
@@ -375,7 +531,14 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
     __syncthreads();
   }
 
-  const EpiScalars es{p.mode, p.norm_p, p.two_s, p.eps, p.eps_marg, p.tau, p.inv_rho, p.momentum};
+  if (p.lse_store != nullptr) {
+    for (int t = tid; t < COLS; t += NT) {
+      const float2 ms = s_ms[t];
+      p.lse_store[col0 + t] = (ms.y > 0.f) ? ms.x + log2f(ms.y) / c : -INFINITY;
+    }
+    if (tid == 0 && p.state != nullptr) p.state->shift_ok[p.side] = 1;
+  }
+  const EpiScalars es{p.mode, p.norm_p, p.two_s, eps, eps_marg, p.tau, inv_rho, p.momentum};
   const bool want_err = (p.mode & (kModeErrCol | kModeErrRow)) != 0;
   double e_acc = 0.0, m_acc = 0.0;
   for (int t = tid; t < COLS; t += NT) {

@@ -20,57 +20,103 @@ namespace ottx {
 
 // Per-chunk, per-feature float64 sums of x and x^2 (deterministic two-stage
 // reduction for mean(C), pointcloud.py:170-186, and for the centring shift).
+// Thread t < G*d (G = 256 / d row groups) owns feature t % d and rows
+// r0 + t / d, r0 + t / d + G, ...; groups are summed in a fixed order.
 __global__ void stats_partial_kernel(const float* __restrict__ pts, int64_t np, int d,
                                      int64_t rows_per_chunk, double* __restrict__ part) {
+  __shared__ double s1[256], s2[256];
+  const int t = threadIdx.x;
+  const int G = max(1, 256 / d);
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_chunk;
   const int64_t r1 = min(np, r0 + rows_per_chunk);
-  for (int k = threadIdx.x; k < d; k += blockDim.x) {
-    double s1 = 0.0, s2 = 0.0;
-    for (int64_t r = r0; r < r1; ++r) {
+  double a1 = 0.0, a2 = 0.0;
+  if (t < G * d) {
+    const int k = t % d, g = t / d;
+    for (int64_t r = r0 + g; r < r1; r += G) {
       const double v = pts[r * d + k];
-      s1 += v;
-      s2 += v * v;
+      a1 += v;
+      a2 += v * v;
     }
-    part[(static_cast<int64_t>(blockIdx.x) * d + k) * 2 + 0] = s1;
-    part[(static_cast<int64_t>(blockIdx.x) * d + k) * 2 + 1] = s2;
+  }
+  s1[t] = a1;
+  s2[t] = a2;
+  __syncthreads();
+  if (t < d) {
+    double b1 = 0.0, b2 = 0.0;
+    for (int g = 0; g < G; ++g) {
+      b1 += s1[g * d + t];
+      b2 += s2[g * d + t];
+    }
+    part[(static_cast<int64_t>(blockIdx.x) * d + t) * 2 + 0] = b1;
+    part[(static_cast<int64_t>(blockIdx.x) * d + t) * 2 + 1] = b2;
   }
 }
 
-// One workgroup: combine the chunks, write mean(C) (out[0]) and the common
-// centring shift mu = (mean x + mean y) / 2 (float, [d]).
+// One workgroup: combine the chunks, write mean(C) (out[0]), the common
+// centring shift mu = (mean x + mean y) / 2 (float, [d]) and, when ``sc`` is
+// given, the solve's scalars: eps = rel * mean(C) if rel > 0 else eps_abs.
 __global__ void stats_final_kernel(const double* __restrict__ px, int cx, const double* __restrict__ py,
                                    int cy, int d, int64_t n, int64_t m, double s,
-                                   double* __restrict__ out, float* __restrict__ mu) {
-  __shared__ double red[3][256];
-  double nx2 = 0.0, ny2 = 0.0, dot = 0.0;
-  for (int k = threadIdx.x; k < d; k += blockDim.x) {
-    double sx1 = 0.0, sx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
-    for (int c = 0; c < cx; ++c) {
+                                   double* __restrict__ out, float* __restrict__ mu, SolveScalars* sc,
+                                   double rel, double eps_abs, double tau_a, double tau_b) {
+  __shared__ double red[4][256];
+  __shared__ double acc3[3][256];
+  const int t = threadIdx.x;
+  const int G = max(1, 256 / d);
+  double sx1 = 0.0, sx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
+  if (t < G * d) {
+    const int k = t % d, g = t / d;
+    for (int c = g; c < cx; c += G) {
       sx1 += px[(static_cast<int64_t>(c) * d + k) * 2];
       sx2 += px[(static_cast<int64_t>(c) * d + k) * 2 + 1];
     }
-    for (int c = 0; c < cy; ++c) {
+    for (int c = g; c < cy; c += G) {
       sy1 += py[(static_cast<int64_t>(c) * d + k) * 2];
       sy2 += py[(static_cast<int64_t>(c) * d + k) * 2 + 1];
     }
-    const double mxk = sx1 / static_cast<double>(n);
-    const double myk = sy1 / static_cast<double>(m);
-    if (mu) mu[k] = static_cast<float>(0.5 * (mxk + myk));
-    nx2 += sx2;
-    ny2 += sy2;
-    dot += mxk * myk;
   }
-  red[0][threadIdx.x] = nx2;
-  red[1][threadIdx.x] = ny2;
-  red[2][threadIdx.x] = dot;
+  red[0][t] = sx1;
+  red[1][t] = sx2;
+  red[2][t] = sy1;
+  red[3][t] = sy2;
+  __syncthreads();
+  double nx2 = 0.0, ny2 = 0.0, dot = 0.0;
+  if (t < d) {
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+    for (int g = 0; g < G; ++g) {
+      a1 += red[0][g * d + t];
+      a2 += red[1][g * d + t];
+      b1 += red[2][g * d + t];
+      b2 += red[3][g * d + t];
+    }
+    const double mxk = a1 / static_cast<double>(n);
+    const double myk = b1 / static_cast<double>(m);
+    if (mu) mu[t] = static_cast<float>(0.5 * (mxk + myk));
+    nx2 = a2;
+    ny2 = b2;
+    dot = mxk * myk;
+  }
+  acc3[0][t] = nx2;
+  acc3[1][t] = ny2;
+  acc3[2][t] = dot;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st)
-      for (int q = 0; q < 3; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    if (t < st)
+      for (int q = 0; q < 3; ++q) acc3[q][t] += acc3[q][t + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    out[0] = s * (red[0][0] / static_cast<double>(n) + red[1][0] / static_cast<double>(m) - 2.0 * red[2][0]);
+  if (t == 0) {
+    const double mean_c =
+        s * (acc3[0][0] / static_cast<double>(n) + acc3[1][0] / static_cast<double>(m) - 2.0 * acc3[2][0]);
+    out[0] = mean_c;
+    if (sc) {
+      const double eps = rel > 0.0 ? rel * mean_c : eps_abs;
+      sc->eps = eps;
+      sc->eps_f = static_cast<float>(eps);
+      sc->c = static_cast<float>(2.0 * s * 1.4426950408889634 / eps);
+      sc->inv_rho_a = tau_a == 1.0 ? 0.f : static_cast<float>((1.0 - tau_a) / (eps * tau_a));
+      sc->inv_rho_b = tau_b == 1.0 ? 0.f : static_cast<float>((1.0 - tau_b) / (eps * tau_b));
+    }
   }
 }
 
@@ -152,6 +198,8 @@ __global__ void init_state_kernel(SolveState* st, float* errors, int outer, uint
     st->block = 0;
     st->n_blocks = 0;
     st->tiles_done = 0u;
+    st->shift_ok[0] = 0;
+    st->shift_ok[1] = 0;
     st->total_mass = 0.0;
     st->last_err = 0.0;
   }
@@ -165,23 +213,30 @@ __global__ void copy_f32_kernel(const float* __restrict__ src, float* __restrict
 }
 
 // Float64 dual terms of compute_kl_reg_cost (sinkhorn.py:224-277) and the
-// errors summary of output_from_state (sinkhorn.py:1020-1023, 431-434).
-//   block 0: out[0] = div_a, out[1] = sum a;  block 1: out[2] = div_b, out[3] = sum b
-//   block 2: out[4] = any NaN in errors, out[5] = #(errors != -1), out[6] = errors[-1]
-__global__ void dual_terms_kernel(const float* __restrict__ f, const float* __restrict__ a, int64_t n,
-                                  const float* __restrict__ g, const float* __restrict__ b, int64_t m,
-                                  double eps, double tau_a, double tau_b, const float* __restrict__ errors,
-                                  int outer, double* __restrict__ out) {
+// errors summary of output_from_state (sinkhorn.py:1020-1023, 431-434), as a
+// deterministic two-stage reduction.  Stage 1, grid (kDualBlocks, 3):
+//   y = 0: partial div_a and sum a over a slice;  y = 1: same for b / g
+//   y = 2 (block 0 only): #NaN and #(errors != -1)
+constexpr int kDualBlocks = 32;
+
+__global__ void dual_partial_kernel(const float* __restrict__ f, const float* __restrict__ a, int64_t n,
+                                    const float* __restrict__ g, const float* __restrict__ b, int64_t m,
+                                    const SolveScalars* __restrict__ sc, double tau_a, double tau_b,
+                                    const float* __restrict__ errors, int outer, double* __restrict__ part) {
   __shared__ double red[2][256];
   const int tid = threadIdx.x;
+  const int side = blockIdx.y;
   double s0 = 0.0, s1 = 0.0;
-  if (blockIdx.x < 2) {
-    const float* pot = blockIdx.x == 0 ? f : g;
-    const float* w = blockIdx.x == 0 ? a : b;
-    const int64_t len = blockIdx.x == 0 ? n : m;
-    const double tau = blockIdx.x == 0 ? tau_a : tau_b;
+  if (side < 2) {
+    const float* pot = side == 0 ? f : g;
+    const float* w = side == 0 ? a : b;
+    const int64_t len = side == 0 ? n : m;
+    const double tau = side == 0 ? tau_a : tau_b;
+    const double eps = sc->eps;
     const double rho = eps * tau / (1.0 - tau);
-    for (int64_t i = tid; i < len; i += blockDim.x) {
+    const int64_t per = (len + kDualBlocks - 1) / kDualBlocks;
+    const int64_t i0 = blockIdx.x * per, i1 = min(len, i0 + per);
+    for (int64_t i = i0 + tid; i < i1; i += blockDim.x) {
       const double wi = w[i];
       s1 += wi;
       if (wi > 0.0) {
@@ -193,7 +248,7 @@ __global__ void dual_terms_kernel(const float* __restrict__ f, const float* __re
         }
       }
     }
-  } else {
+  } else if (blockIdx.x == 0) {
     for (int i = tid; i < outer; i += blockDim.x) {
       const float e = errors[i];
       if (e != e) s0 += 1.0;
@@ -211,15 +266,27 @@ __global__ void dual_terms_kernel(const float* __restrict__ f, const float* __re
     __syncthreads();
   }
   if (tid == 0) {
-    if (blockIdx.x < 2) {
-      out[blockIdx.x * 2 + 0] = red[0][0];
-      out[blockIdx.x * 2 + 1] = red[1][0];
-    } else {
-      out[4] = red[0][0];
-      out[5] = red[1][0];
-      out[6] = outer > 0 ? static_cast<double>(errors[outer - 1]) : 0.0;
-    }
+    part[(side * kDualBlocks + blockIdx.x) * 2 + 0] = red[0][0];
+    part[(side * kDualBlocks + blockIdx.x) * 2 + 1] = red[1][0];
   }
+}
+
+// Stage 2 (one thread): out[0] = div_a, [1] = sum a, [2] = div_b, [3] = sum b,
+// [4] = #NaN errors, [5] = #(errors != -1), [6] = errors[-1], [7] = eps.
+__global__ void dual_final_kernel(const double* __restrict__ part, const float* __restrict__ errors, int outer,
+                                  const SolveScalars* __restrict__ sc, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  for (int side = 0; side < 3; ++side) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int b = 0; b < kDualBlocks; ++b) {
+      s0 += part[(side * kDualBlocks + b) * 2 + 0];
+      s1 += part[(side * kDualBlocks + b) * 2 + 1];
+    }
+    out[side * 2 + 0] = s0;
+    out[side * 2 + 1] = s1;
+  }
+  out[6] = outer > 0 ? static_cast<double>(errors[outer - 1]) : 0.0;
+  out[7] = sc->eps;
 }
 
 // ---------------------------------------------------------------------------
@@ -241,37 +308,84 @@ KernelCfg make_cfg() {
   return KernelCfg{PREC, KK, NB, MT, W, &halfstep_kernel<PREC, KK, NB, MT, W>};
 }
 
-bool select_kernel(int prec, int d, KernelCfg* out) {
-  if (d < 1 || d > 256) return false;
+// Tile variants of the fused half-step per (precision, K): the search space
+// of the autotuner.  Index 0 is the default.
+int kernel_variants(int prec, int d, KernelCfg* out, int max_out) {
+  int n = 0;
+  auto add = [&](const KernelCfg& k) {
+    if (n < max_out) out[n] = k;
+    ++n;
+  };
+  if (d < 1 || d > 256) return 0;
   if (prec == OTTX_PREC_BF16_SPLIT) {
     const int kb = (d + 31) / 32;
     switch (kb) {
-      case 1: *out = make_cfg<1, 1, 4, 4, 4>(); return true;
-      case 2: *out = make_cfg<1, 2, 4, 4, 4>(); return true;
-      case 3: *out = make_cfg<1, 3, 2, 4, 4>(); return true;
-      case 4: *out = make_cfg<1, 4, 2, 4, 4>(); return true;
-      case 5: *out = make_cfg<1, 5, 1, 4, 4>(); return true;
-      case 6: *out = make_cfg<1, 6, 1, 4, 4>(); return true;
-      case 7: *out = make_cfg<1, 7, 1, 4, 4>(); return true;
-      case 8: *out = make_cfg<1, 8, 1, 4, 4>(); return true;
-      default: return false;
+      case 1:
+        add(make_cfg<1, 1, 4, 4, 4>()); add(make_cfg<1, 1, 4, 2, 4>());
+        add(make_cfg<1, 1, 2, 4, 4>()); add(make_cfg<1, 1, 8, 2, 4>());
+        break;
+      case 2:
+        add(make_cfg<1, 2, 4, 4, 4>()); add(make_cfg<1, 2, 4, 2, 4>());
+        add(make_cfg<1, 2, 2, 4, 4>()); add(make_cfg<1, 2, 8, 2, 4>());
+        break;
+      case 3: add(make_cfg<1, 3, 2, 4, 4>()); add(make_cfg<1, 3, 2, 2, 4>()); add(make_cfg<1, 3, 1, 4, 4>()); break;
+      case 4: add(make_cfg<1, 4, 2, 4, 4>()); add(make_cfg<1, 4, 2, 2, 4>()); add(make_cfg<1, 4, 1, 4, 4>()); break;
+      case 5: add(make_cfg<1, 5, 1, 2, 4>()); break;
+      case 6: add(make_cfg<1, 6, 1, 2, 4>()); break;
+      case 7: add(make_cfg<1, 7, 1, 2, 4>()); break;
+      case 8: add(make_cfg<1, 8, 1, 2, 4>()); break;
+      default: return 0;
     }
+    return n;
   }
   if (prec == OTTX_PREC_FP32) {
     const int ks = (d + 3) / 4;
-    if (ks <= 1) { *out = make_cfg<0, 1, 4, 4, 4>(); return true; }
-    if (ks <= 2) { *out = make_cfg<0, 2, 4, 4, 4>(); return true; }
-    if (ks <= 4) { *out = make_cfg<0, 4, 4, 4, 4>(); return true; }
-    if (ks <= 8) { *out = make_cfg<0, 8, 4, 4, 4>(); return true; }
-    if (ks <= 12) { *out = make_cfg<0, 12, 4, 4, 4>(); return true; }
-    if (ks <= 16) { *out = make_cfg<0, 16, 4, 4, 4>(); return true; }
-    if (ks <= 24) { *out = make_cfg<0, 24, 2, 4, 4>(); return true; }
-    if (ks <= 32) { *out = make_cfg<0, 32, 2, 4, 4>(); return true; }
-    if (ks <= 48) { *out = make_cfg<0, 48, 1, 4, 4>(); return true; }
-    if (ks <= 64) { *out = make_cfg<0, 64, 1, 4, 4>(); return true; }
-    return false;
+    if (ks <= 1) {
+      add(make_cfg<0, 1, 4, 4, 4>()); add(make_cfg<0, 1, 4, 2, 4>()); add(make_cfg<0, 1, 8, 2, 4>());
+    } else if (ks <= 2) {
+      add(make_cfg<0, 2, 4, 4, 4>()); add(make_cfg<0, 2, 4, 2, 4>()); add(make_cfg<0, 2, 8, 2, 4>());
+    } else if (ks <= 4) {
+      add(make_cfg<0, 4, 4, 4, 4>()); add(make_cfg<0, 4, 4, 2, 4>()); add(make_cfg<0, 4, 2, 4, 4>());
+    } else if (ks <= 8) {
+      add(make_cfg<0, 8, 4, 4, 4>()); add(make_cfg<0, 8, 4, 2, 4>()); add(make_cfg<0, 8, 2, 4, 4>());
+    } else if (ks <= 12) {
+      add(make_cfg<0, 12, 4, 4, 4>()); add(make_cfg<0, 12, 4, 2, 4>()); add(make_cfg<0, 12, 2, 4, 4>());
+    } else if (ks <= 16) {
+      add(make_cfg<0, 16, 4, 4, 4>()); add(make_cfg<0, 16, 4, 2, 4>()); add(make_cfg<0, 16, 2, 4, 4>());
+    } else if (ks <= 24) {
+      add(make_cfg<0, 24, 2, 4, 4>()); add(make_cfg<0, 24, 2, 2, 4>()); add(make_cfg<0, 24, 1, 4, 4>());
+    } else if (ks <= 32) {
+      add(make_cfg<0, 32, 2, 4, 4>()); add(make_cfg<0, 32, 2, 2, 4>()); add(make_cfg<0, 32, 1, 4, 4>());
+    } else if (ks <= 48) {
+      add(make_cfg<0, 48, 1, 2, 4>());
+    } else if (ks <= 64) {
+      add(make_cfg<0, 64, 1, 2, 4>());
+    } else {
+      return 0;
+    }
+    return n;
   }
-  return false;
+  return 0;
+}
+
+// Occupancy targets (workgroups per CU the row splits aim for).
+constexpr int kTargets[] = {4, 2, 8, 1, 16, 3, 6};
+constexpr int kNumTargets = 7;
+
+// tile_config: -1 = default, else variant | (target index << 8).
+bool select_kernel(int prec, int d, int tile_config, KernelCfg* out, int* target_mult) {
+  KernelCfg v[8];
+  const int n = kernel_variants(prec, d, v, 8);
+  if (n <= 0) return false;
+  int vi = 0, ti = 0;
+  if (tile_config >= 0) {
+    vi = tile_config & 0xff;
+    ti = (tile_config >> 8) & 0xff;
+    if (vi >= n || ti >= kNumTargets) return false;
+  }
+  *out = v[vi];
+  *target_mult = kTargets[ti];
+  return true;
 }
 
 int device_cus() {
@@ -291,13 +405,13 @@ struct PassPlan {
   int n_in_rb = 0;
 };
 
-PassPlan plan_pass(const KernelCfg& k, int64_t n_in, int64_t n_out, int cus) {
+PassPlan plan_pass(const KernelCfg& k, int64_t n_in, int64_t n_out, int cus, int target_mult) {
   PassPlan p;
   const int64_t in_pad = pad_rows(n_in), out_pad = pad_rows(n_out);
   p.tiles = static_cast<int>(out_pad / k.cols());
   p.n_in_rb = static_cast<int>(in_pad / 16);
   const int in_tiles = p.n_in_rb / k.mt;
-  const int target = 4 * cus;
+  const int target = target_mult * cus;
   int splits = (target + p.tiles - 1) / p.tiles;
   splits = std::max(1, std::min(splits, in_tiles));
   const int tiles_per_split = (in_tiles + splits - 1) / splits;
@@ -309,8 +423,8 @@ PassPlan plan_pass(const KernelCfg& k, int64_t n_in, int64_t n_out, int cus) {
 size_t align_up(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 
 struct Layout {
-  size_t frag_x, frag_y, norm_x, norm_y, g_save, partial, counters, err_col, mass, err_row, state,
-      dual, stats_x, stats_y, stats_out, mu, total;
+  size_t frag_x, frag_y, norm_x, norm_y, g_save, lse_x, lse_y, partial, counters, err_col, mass, err_row, state,
+      dual, dual_part, scal, stats_x, stats_y, stats_out, mu, total;
   int chunks_x, chunks_y;
   int64_t rows_chunk_x, rows_chunk_y;
 };
@@ -336,6 +450,8 @@ Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& g
   L.norm_x = take(n_pad * sizeof(float));
   L.norm_y = take(m_pad * sizeof(float));
   L.g_save = take(m_pad * sizeof(float));
+  L.lse_x = take(n_pad * sizeof(float));
+  L.lse_y = take(m_pad * sizeof(float));
   const size_t part_g = static_cast<size_t>(gp.splits) * gp.tiles * k.cols();
   const size_t part_f = static_cast<size_t>(fp.splits) * fp.tiles * k.cols();
   L.partial = take(std::max(part_g, part_f) * sizeof(float2));
@@ -346,6 +462,8 @@ Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& g
   L.err_row = take(static_cast<size_t>(max_tiles) * sizeof(double));
   L.state = take(sizeof(SolveState));
   L.dual = take(16 * sizeof(double));
+  L.dual_part = take(3 * kDualBlocks * 2 * sizeof(double));
+  L.scal = take(sizeof(SolveScalars));
   stats_chunks(pb->n, &L.chunks_x, &L.rows_chunk_x);
   stats_chunks(pb->m, &L.chunks_y, &L.rows_chunk_y);
   L.stats_x = take(static_cast<size_t>(L.chunks_x) * pb->d * 2 * sizeof(double));
@@ -357,11 +475,44 @@ Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& g
   return L;
 }
 
+struct Plan {
+  KernelCfg k;
+  PassPlan gp, fp;
+  Layout L;
+};
+
+bool make_plan(const ottx_problem* pb, int tile_config, Plan* P) {
+  int tm = 4;
+  if (!select_kernel(pb->precision, pb->d, tile_config, &P->k, &tm)) return false;
+  const int cus = device_cus();
+  // Every column tile (16 NB W points) and streamed tile (16 MT rows) must
+  // divide the row padding, so tiles never straddle the end of an array.
+  if (kRowPad % P->k.cols() != 0 || kRowPad % (16 * P->k.mt) != 0) return false;
+  P->gp = plan_pass(P->k, pb->n, pb->m, cus, tm);
+  P->fp = plan_pass(P->k, pb->m, pb->n, cus, tm);
+  P->L = make_layout(pb, P->k, P->gp, P->fp, 1);
+  return true;
+}
+
+// Workspace large enough for every tile configuration of the problem.
+size_t max_workspace(const ottx_problem* pb) {
+  KernelCfg v[8];
+  const int n = kernel_variants(pb->precision, pb->d, v, 8);
+  size_t best = 0;
+  for (int vi = 0; vi < n; ++vi)
+    for (int ti = 0; ti < kNumTargets; ++ti) {
+      Plan P;
+      if (make_plan(pb, vi | (ti << 8), &P)) best = std::max(best, P.L.total);
+    }
+  return best;
+}
+
 int check_problem(const ottx_problem* pb) {
   if (!pb || pb->n <= 0 || pb->m <= 0 || pb->d <= 0 || pb->d > 256) return OTTX_EINVAL;
   if (pb->n > (int64_t{1} << 30) || pb->m > (int64_t{1} << 30)) return OTTX_EINVAL;
   if (!pb->x || !pb->y || !pb->a || !pb->b) return OTTX_EINVAL;
-  if (!(pb->epsilon > 0.0) || !(pb->inv_scale_cost > 0.0)) return OTTX_EINVAL;
+  if (!(pb->epsilon > 0.0) && !(pb->relative_epsilon > 0.0)) return OTTX_EINVAL;
+  if (!(pb->inv_scale_cost > 0.0) || pb->relative_epsilon < 0.0) return OTTX_EINVAL;
   if (!(pb->tau_a > 0.0 && pb->tau_a <= 1.0) || !(pb->tau_b > 0.0 && pb->tau_b <= 1.0)) return OTTX_EINVAL;
   if (pb->precision != OTTX_PREC_FP32 && pb->precision != OTTX_PREC_BF16_SPLIT) return OTTX_EINVAL;
   return OTTX_OK;
@@ -386,6 +537,7 @@ struct Prepared {
   float* norm_y;
   float* mu;
   double* stats_out;
+  SolveScalars* scal;
 };
 
 // Statistics, centring shift, fragments and norms of both clouds.
@@ -406,8 +558,10 @@ int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* w
                      L.rows_chunk_x, sx);
   hipLaunchKernelGGL(stats_partial_kernel, dim3(L.chunks_y), dim3(256), 0, st, pb->y, pb->m, pb->d,
                      L.rows_chunk_y, sy);
+  P->scal = reinterpret_cast<SolveScalars*>(base + L.scal);
   hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, st, sx, L.chunks_x, sy, L.chunks_y, pb->d,
-                     pb->n, pb->m, pb->inv_scale_cost, P->stats_out, P->mu);
+                     pb->n, pb->m, pb->inv_scale_cost, P->stats_out, P->mu, P->scal, pb->relative_epsilon,
+                     pb->epsilon, pb->tau_a, pb->tau_b);
   OTTX_LAUNCHED();
   const int64_t n_pad = pad_rows(pb->n), m_pad = pad_rows(pb->m);
   const int chunks = k.frag_chunks();
@@ -456,7 +610,7 @@ HalfStepArgs base_args(const ottx_problem* pb, const KernelCfg& k, const PassPla
   const double s = pb->inv_scale_cost;
   a.inv2s = static_cast<float>(1.0 / (2.0 * s));
   a.two_s = static_cast<float>(2.0 * s);
-  a.c = static_cast<float>(2.0 * s * 1.4426950408889634 / eps);
+  a.c = static_cast<float>(2.0 * s * 1.4426950408889634 / (eps > 0.0 ? eps : 1.0));
   a.eps = static_cast<float>(eps);
   a.eps_marg = static_cast<float>(pb->epsilon);
   const double tau = g_upd ? pb->tau_b : pb->tau_a;
@@ -470,6 +624,8 @@ HalfStepArgs base_args(const ottx_problem* pb, const KernelCfg& k, const PassPla
   a.err_partial = reinterpret_cast<double*>(base + (g_upd ? P.L.err_col : P.L.err_row));
   a.mass_partial = g_upd ? reinterpret_cast<double*>(base + P.L.mass) : nullptr;
   a.state = reinterpret_cast<SolveState*>(base + P.L.state);
+  a.scal = P.scal;
+  a.side = side_out;
   (void)k;
   return a;
 }
@@ -552,7 +708,7 @@ int ottx_mean_cost(int64_t n, int64_t m, int32_t d, const float* x, const float*
   hipLaunchKernelGGL(stats_partial_kernel, dim3(cx), dim3(256), 0, st, x, n, d, rx, sx);
   hipLaunchKernelGGL(stats_partial_kernel, dim3(cy), dim3(256), 0, st, y, m, d, ry, sy);
   hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, st, sx, cx, sy, cy, d, n, m, inv_scale_cost, o,
-                     static_cast<float*>(nullptr));
+                     static_cast<float*>(nullptr), static_cast<SolveScalars*>(nullptr), 0.0, 0.0, 1.0, 1.0);
   OTTX_LAUNCHED();
   OTTX_HIP(hipMemcpyAsync(out_host, o, sizeof(double), hipMemcpyDeviceToHost, st));
   OTTX_HIP(hipStreamSynchronize(st));
@@ -561,12 +717,7 @@ int ottx_mean_cost(int64_t n, int64_t m, int32_t d, const float* x, const float*
 
 size_t ottx_op_workspace_bytes(const ottx_problem* pb) {
   if (check_problem(pb) != OTTX_OK) return 0;
-  KernelCfg k;
-  if (!select_kernel(pb->precision, pb->d, &k)) return 0;
-  const int cus = device_cus();
-  const PassPlan gp = plan_pass(k, pb->n, pb->m, cus);
-  const PassPlan fp = plan_pass(k, pb->m, pb->n, cus);
-  return make_layout(pb, k, gp, fp, 1).total;
+  return max_workspace(pb);
 }
 
 int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const float* f, const float* g,
@@ -577,12 +728,12 @@ int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const f
   if (!f || !g || !out || !workspace || !(eps > 0.0)) return OTTX_EINVAL;
   if (axis != 0 && axis != 1) return OTTX_EINVAL;
   if (op != OTTX_OP_APPLY_LSE && op != OTTX_OP_UPDATE && op != OTTX_OP_MARGINAL) return OTTX_EINVAL;
-  KernelCfg k;
-  if (!select_kernel(pb->precision, pb->d, &k)) return OTTX_EUNSUPPORTED;
-  const int cus = device_cus();
-  const PassPlan gp = plan_pass(k, pb->n, pb->m, cus);
-  const PassPlan fp = plan_pass(k, pb->m, pb->n, cus);
-  const Layout L = make_layout(pb, k, gp, fp, 1);
+  Plan PL;
+  if (!make_plan(pb, -1, &PL)) return OTTX_EUNSUPPORTED;
+  const KernelCfg& k = PL.k;
+  const PassPlan& gp = PL.gp;
+  const PassPlan& fp = PL.fp;
+  const Layout& L = PL.L;
   if (workspace_bytes < L.total) return OTTX_EWORKSPACE;
   if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -601,6 +752,7 @@ int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const f
   const float* in_pot = g_upd ? f : g;
   HalfStepArgs a = base_args(pb, k, plan, P, g_upd ? 1 : 0, const_cast<float*>(in_pot), const_cast<float*>(h), eps);
   a.state = nullptr;
+  a.scal = nullptr;
   const int64_t n_out = g_upd ? pb->m : pb->n;
   if (op == OTTX_OP_UPDATE) {
     hipLaunchKernelGGL(copy_f32_kernel, dim3(static_cast<int>((n_out + 255) / 256)), dim3(256), 0, st, h, out,
@@ -624,12 +776,7 @@ int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const f
 
 size_t ottx_sinkhorn_workspace_bytes(const ottx_problem* pb, const ottx_sinkhorn_opts* o) {
   if (check_problem(pb) != OTTX_OK || !o) return 0;
-  KernelCfg k;
-  if (!select_kernel(pb->precision, pb->d, &k)) return 0;
-  const int cus = device_cus();
-  const PassPlan gp = plan_pass(k, pb->n, pb->m, cus);
-  const PassPlan fp = plan_pass(k, pb->m, pb->n, cus);
-  return make_layout(pb, k, gp, fp, 1).total;
+  return max_workspace(pb);
 }
 
 int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ottx_sinkhorn_out* out, void* workspace,
@@ -640,12 +787,12 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   if (o->inner_iterations < 1 || o->max_iterations < 0 || o->min_iterations < 0 || o->norm_error < 1)
     return OTTX_EINVAL;
   if (!(o->momentum > 0.0 && o->momentum < 2.0)) return OTTX_EINVAL;
-  KernelCfg k;
-  if (!select_kernel(pb->precision, pb->d, &k)) return OTTX_EUNSUPPORTED;
-  const int cus = device_cus();
-  const PassPlan gp = plan_pass(k, pb->n, pb->m, cus);
-  const PassPlan fp = plan_pass(k, pb->m, pb->n, cus);
-  const Layout L = make_layout(pb, k, gp, fp, 1);
+  Plan PL;
+  if (!make_plan(pb, o->tile_config, &PL)) return o->tile_config >= 0 ? OTTX_EINVAL : OTTX_EUNSUPPORTED;
+  const KernelCfg& k = PL.k;
+  const PassPlan& gp = PL.gp;
+  const PassPlan& fp = PL.fp;
+  const Layout& L = PL.L;
   if (workspace_bytes < L.total) return OTTX_EWORKSPACE;
   if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -687,6 +834,8 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     a->threshold = static_cast<float>(o->threshold);
   }
   ga.out_save = g_save;
+  ga.lse_shift = ga.lse_store = reinterpret_cast<float*>(base + L.lse_y);
+  fa.lse_shift = fa.lse_store = reinterpret_cast<float*>(base + L.lse_x);
   const bool balanced = (pb->tau_a == 1.0 && pb->tau_b == 1.0);
   HalfStepArgs g_plain = ga, g_err = ga, f_plain = fa, f_err = fa;
   g_plain.mode = kModeWritePot | kModeCheckDone;
@@ -766,8 +915,10 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     if (rc != OTTX_OK) return rc;
   }
 
-  hipLaunchKernelGGL(dual_terms_kernel, dim3(3), dim3(256), 0, st, out->f, pb->a, pb->n, g_save, pb->b, pb->m,
-                     pb->epsilon, pb->tau_a, pb->tau_b, out->errors, outer, dual);
+  double* dual_part = reinterpret_cast<double*>(base + L.dual_part);
+  hipLaunchKernelGGL(dual_partial_kernel, dim3(kDualBlocks, 3), dim3(256), 0, st, out->f, pb->a, pb->n, g_save,
+                     pb->b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors, outer, dual_part);
+  hipLaunchKernelGGL(dual_final_kernel, dim3(1), dim3(64), 0, st, dual_part, out->errors, outer, P.scal, dual);
   OTTX_LAUNCHED();
   hipLaunchKernelGGL(copy_f32_kernel, dim3(static_cast<int>((pb->m + 255) / 256)), dim3(256), 0, st, g_save,
                      out->g, pb->m);
@@ -789,7 +940,8 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     for (int t = 0; t < gp.tiles && 512 + (t + 1) * 8 <= 4096; ++t) total += mp[t];
   }
   out->total_mass = total;
-  out->reg_ot_cost = du[0] + du[2] + pb->epsilon * (du[1] * du[3] - total);
+  out->epsilon = du[7];
+  out->reg_ot_cost = du[0] + du[2] + du[7] * (du[1] * du[3] - total);
   const bool any_nan = du[4] > 0.0;
   const float last = static_cast<float>(du[6]);
   out->converged = (!any_nan && outer > 0 && last < static_cast<float>(o->threshold)) ? 1 : 0;
@@ -800,17 +952,38 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
 
 }  // extern "C"
 
-extern "C" int ottx_benchmark_halfstep(const ottx_problem* pb, int32_t axis, int32_t reps, float* f, float* g,
-                                       double* avg_ms_host, void* workspace, size_t workspace_bytes, void* stream) {
+extern "C" int ottx_tile_configs(int32_t precision, int32_t d, int32_t* desc, int32_t max_configs) {
+  KernelCfg v[8];
+  const int n = kernel_variants(precision, d, v, 8);
+  if (n <= 0) return 0;
+  int out = 0;
+  for (int vi = 0; vi < n; ++vi)
+    for (int ti = 0; ti < kNumTargets; ++ti) {
+      if (desc && out < max_configs) {
+        int32_t* e = desc + 5 * out;
+        e[0] = vi | (ti << 8);
+        e[1] = v[vi].nb;
+        e[2] = v[vi].mt;
+        e[3] = v[vi].w;
+        e[4] = kTargets[ti];
+      }
+      ++out;
+    }
+  return out;
+}
+
+extern "C" int ottx_benchmark_halfstep(const ottx_problem* pb, int32_t axis, int32_t tile_config, int32_t reps,
+                                       float* f, float* g, double* avg_ms_host, void* workspace,
+                                       size_t workspace_bytes, void* stream) {
   int rc = check_problem(pb);
   if (rc != OTTX_OK) return rc;
   if ((axis != 0 && axis != 1) || reps < 1 || !f || !g || !avg_ms_host || !workspace) return OTTX_EINVAL;
-  KernelCfg k;
-  if (!select_kernel(pb->precision, pb->d, &k)) return OTTX_EUNSUPPORTED;
-  const int cus = device_cus();
-  const PassPlan gp = plan_pass(k, pb->n, pb->m, cus);
-  const PassPlan fp = plan_pass(k, pb->m, pb->n, cus);
-  const Layout L = make_layout(pb, k, gp, fp, 1);
+  Plan PL;
+  if (!make_plan(pb, tile_config, &PL)) return tile_config >= 0 ? OTTX_EINVAL : OTTX_EUNSUPPORTED;
+  const KernelCfg& k = PL.k;
+  const PassPlan& gp = PL.gp;
+  const PassPlan& fp = PL.fp;
+  const Layout& L = PL.L;
   if (workspace_bytes < L.total) return OTTX_EWORKSPACE;
   if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -825,7 +998,7 @@ extern "C" int ottx_benchmark_halfstep(const ottx_problem* pb, int32_t axis, int
   OTTX_LAUNCHED();
   HalfStepArgs a = axis == 0 ? base_args(pb, k, gp, P, 1, f, g, pb->epsilon)
                              : base_args(pb, k, fp, P, 0, g, f, pb->epsilon);
-  a.state = nullptr;
+  a.lse_shift = a.lse_store = reinterpret_cast<float*>(P.base + (axis == 0 ? L.lse_y : L.lse_x));
   a.mode = kModeWritePot;
   for (int w = 0; w < 2; ++w) {
     rc = launch_halfstep(k, a, st);

@@ -168,10 +168,31 @@ class PointCloud:
   def epsilon(self) -> float:
     return float(self.epsilon_scheduler.target)
 
+  def _epsilon_spec(self) -> Tuple[float, float]:
+    """(absolute epsilon, relative multiplier) for the C ABI.
+
+    A not-yet-resolved ``relative_epsilon="mean"`` is passed as a multiplier so
+    the solve resolves it on the device without a host round trip.
+    """
+    if (self._eps_cache is None and self._relative_epsilon == "mean" and
+        not isinstance(self._epsilon_init, eps_scheduler.Epsilon)):
+      mult = (eps_scheduler.DEFAULT_EPSILON_SCALE if self._epsilon_init is None
+              else float(self._epsilon_init))
+      return 0.0, mult
+    return self.epsilon, 0.0
+
+  def _set_resolved_epsilon(self, eps: float) -> None:
+    if self._eps_cache is None:
+      self._eps_cache = eps_scheduler.Epsilon(float(eps))
+
+  def _schedule_is_target_only(self) -> bool:
+    init = self._epsilon_init
+    return not isinstance(init, eps_scheduler.Epsilon) or init.is_target_only
+
   # -- the operator protocol -------------------------------------------------
   def _problem(self, a: Optional[torch.Tensor] = None, b: Optional[torch.Tensor] = None,
-               tau_a: float = 1.0, tau_b: float = 1.0,
-               epsilon: Optional[float] = None) -> Tuple[_lib.Problem, Tuple]:
+               tau_a: float = 1.0, tau_b: float = 1.0, epsilon: Optional[float] = None,
+               relative_epsilon: float = 0.0) -> Tuple[_lib.Problem, Tuple]:
     if not self.is_squared_euclidean:
       raise NotImplementedError("ott_amd accelerates the squared-Euclidean cost only")
     n, m = self.shape
@@ -183,7 +204,8 @@ class PointCloud:
         n=n, m=m, d=self.x.shape[1], precision=_lib.PRECISIONS[self.precision],
         x=self.x.data_ptr(), y=self.y.data_ptr(), a=a.data_ptr(), b=b.data_ptr(),
         epsilon=float(self.epsilon if epsilon is None else epsilon),
-        inv_scale_cost=self.inv_scale_cost, tau_a=float(tau_a), tau_b=float(tau_b))
+        inv_scale_cost=self.inv_scale_cost, tau_a=float(tau_a), tau_b=float(tau_b),
+        relative_epsilon=float(relative_epsilon))
     return pb, (a, b)
 
   def _op(self, op: int, f, g, eps: float, axis: int,

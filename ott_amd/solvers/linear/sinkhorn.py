@@ -17,7 +17,7 @@ from typing import Any, Callable, NamedTuple, Optional, Tuple
 
 import torch
 
-from ott_amd import _lib
+from ott_amd import _lib, tune
 from ott_amd.geometry.pointcloud import PointCloud, as_device_f32
 from ott_amd.initializers.linear import initializers as init_lib
 from ott_amd.problems.linear import linear_problem
@@ -172,7 +172,7 @@ class Sinkhorn:
       raise NotImplementedError("progress_fn is not supported by ott_amd yet")
     if not isinstance(ot_prob.geom, PointCloud):
       raise NotImplementedError("ott_amd accelerates PointCloud geometries only")
-    if not ot_prob.geom.epsilon_scheduler.is_target_only:
+    if not ot_prob.geom._schedule_is_target_only():
       raise NotImplementedError("epsilon schedules with init > 1 are not supported yet")
     if int(self._norm_error) < 1:
       raise ValueError("norm_error must be >= 1")
@@ -200,13 +200,16 @@ def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
   if f.shape != (n,) or g.shape != (m,):
     raise ValueError(f"init potentials must be [{n}] and [{m}]")
   a, b = ot_prob.a, ot_prob.b
-  pb, keep = geom._problem(a=a, b=b, tau_a=ot_prob.tau_a, tau_b=ot_prob.tau_b)
+  eps_abs, eps_rel = geom._epsilon_spec()
+  pb, keep = geom._problem(a=a, b=b, tau_a=ot_prob.tau_a, tau_b=ot_prob.tau_b,
+                           epsilon=eps_abs, relative_epsilon=eps_rel)
   opts = _lib.SinkhornOpts(
       threshold=float(solver.threshold), norm_error=int(solver._norm_error),
       inner_iterations=int(solver.inner_iterations),
       min_iterations=int(solver.min_iterations),
       max_iterations=int(solver.max_iterations),
-      momentum=float(solver.momentum.value), tile_config=-1,
+      momentum=float(solver.momentum.value),
+      tile_config=tune.lookup(geom.precision, geom.x.shape[1], n, m),
       use_graph=1 if solver.use_graph else 0)
   errors = torch.empty((solver.outer_iterations,), dtype=torch.float32, device=geom.device)
   out = _lib.SinkhornOut(f=f.data_ptr(), g=g.data_ptr(), errors=errors.data_ptr())
@@ -218,6 +221,7 @@ def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
       ctypes.byref(pb), ctypes.byref(opts), ctypes.byref(out), ws.data_ptr(),
       ws.numel(), _lib.stream_ptr()))
   del keep, ws
+  geom._set_resolved_epsilon(out.epsilon)
   return SinkhornOutput(
       potentials=(f, g), errors=errors, reg_ot_cost=float(out.reg_ot_cost),
       ot_prob=ot_prob, threshold=float(solver.threshold),

@@ -1,0 +1,130 @@
+"""Tile autotuner for the fused half-step kernel.
+
+OTT leaves tiling to XLA; here the kernel exposes a set of tile
+configurations (``ottx_tile_configs``: 16-column blocks per wave, 16-row
+blocks per streamed tile, waves per workgroup, target workgroups per CU).
+``autotune`` times every configuration with ``ottx_benchmark_halfstep`` on the
+problem at hand (both half-step directions) and records the fastest in a JSON
+cache keyed by device, precision, feature dimension and the power-of-two size
+class of n and m.  ``lookup`` returns the cached choice, falling back to the
+defaults shipped in ``tuned_configs.json``, then to -1 (the kernel default).
+
+The cache path is ``$OTTX_TUNE_CACHE`` or ``~/.cache/ott_amd/tune.json``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import os
+import threading
+from typing import Dict, List, Optional
+
+import torch
+
+from ott_amd import _lib
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULTS_PATH = os.path.join(_HERE, "tuned_configs.json")
+_lock = threading.Lock()
+_cache: Optional[Dict[str, int]] = None
+
+
+def cache_path() -> str:
+  return os.environ.get("OTTX_TUNE_CACHE",
+                        os.path.join(os.path.expanduser("~"), ".cache", "ott_amd", "tune.json"))
+
+
+def _device_name() -> str:
+  try:
+    return torch.cuda.get_device_properties(torch.cuda.current_device()).gcnArchName.split(":")[0]
+  except Exception:  # noqa: BLE001 - name only keys the cache
+    return "gfx950"
+
+
+def _size_class(v: int) -> int:
+  return int(math.ceil(math.log2(max(int(v), 1))))
+
+
+def key(precision: str, d: int, n: int, m: int) -> str:
+  return f"{_device_name()}/{precision}/d{d}/n{_size_class(n)}/m{_size_class(m)}"
+
+
+def tile_configs(precision: str, d: int) -> List[dict]:
+  lib = _lib.load_library()
+  prec = _lib.PRECISIONS[precision]
+  count = lib.ottx_tile_configs(prec, d, None, 0)
+  buf = (ctypes.c_int32 * (5 * count))()
+  lib.ottx_tile_configs(prec, d, buf, count)
+  return [dict(tile_config=buf[5 * i], nb=buf[5 * i + 1], mt=buf[5 * i + 2], w=buf[5 * i + 3],
+               target=buf[5 * i + 4]) for i in range(count)]
+
+
+def _load() -> Dict[str, int]:
+  global _cache
+  if _cache is None:
+    merged: Dict[str, int] = {}
+    for path in (DEFAULTS_PATH, cache_path()):
+      if os.path.exists(path):
+        try:
+          with open(path) as fh:
+            merged.update({k: int(v) for k, v in json.load(fh).items()})
+        except (OSError, ValueError):
+          pass
+    _cache = merged
+  return _cache
+
+
+def lookup(precision: str, d: int, n: int, m: int) -> int:
+  with _lock:
+    return _load().get(key(precision, d, n, m), -1)
+
+
+def benchmark(geom, tile_config: int, axis: int, reps: int = 10) -> float:
+  """Mean milliseconds of one half-step of ``geom`` with ``tile_config``."""
+  lib = _lib.load_library()
+  prob, keep = geom._problem()
+  opts = _lib.SinkhornOpts(threshold=1e-3, norm_error=1, inner_iterations=10, min_iterations=0,
+                           max_iterations=10, momentum=1.0, tile_config=-1, use_graph=0)
+  nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(prob), ctypes.byref(opts))
+  ws = _lib.workspace(nbytes, geom.device)
+  n, m = geom.shape
+  f = torch.zeros(n, device=geom.device)
+  g = torch.zeros(m, device=geom.device)
+  avg = ctypes.c_double(0.0)
+  _lib.check("ottx_benchmark_halfstep", lib.ottx_benchmark_halfstep(
+      ctypes.byref(prob), axis, tile_config, reps, f.data_ptr(), g.data_ptr(),
+      ctypes.byref(avg), ws.data_ptr(), ws.numel(), _lib.stream_ptr()))
+  del keep
+  return float(avg.value)
+
+
+def autotune(geom, reps: int = 10, save: bool = True, verbose: bool = False) -> dict:
+  """Time every tile configuration on ``geom``; record and return the fastest."""
+  n, m = geom.shape
+  d = geom.x.shape[1]
+  results = []
+  for cfg in tile_configs(geom.precision, d):
+    ms = benchmark(geom, cfg["tile_config"], 0, reps) + benchmark(geom, cfg["tile_config"], 1, reps)
+    results.append(dict(cfg, ms_per_iteration=ms))
+    if verbose:
+      print(f"  tile {cfg}: {ms:.4f} ms per iteration", flush=True)
+  best = min(results, key=lambda r: r["ms_per_iteration"])
+  k = key(geom.precision, d, n, m)
+  with _lock:
+    _load()[k] = int(best["tile_config"])
+    if save:
+      path = cache_path()
+      os.makedirs(os.path.dirname(path), exist_ok=True)
+      existing = {}
+      if os.path.exists(path):
+        try:
+          with open(path) as fh:
+            existing = json.load(fh)
+        except (OSError, ValueError):
+          existing = {}
+      existing[k] = int(best["tile_config"])
+      with open(path, "w") as fh:
+        json.dump(existing, fh, indent=1, sort_keys=True)
+  return dict(best=best, key=k, results=results)

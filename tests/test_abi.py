@@ -42,9 +42,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
   from ott_amd import _lib
-  assert ctypes.sizeof(_lib.Problem) == 8 + 8 + 4 + 4 + 4 * 8 + 4 * 8
+  assert ctypes.sizeof(_lib.Problem) == 8 + 8 + 4 + 4 + 4 * 8 + 5 * 8
   assert ctypes.sizeof(_lib.SinkhornOpts) == 40
-  assert ctypes.sizeof(_lib.SinkhornOut) == 3 * 8 + 2 * 8 + 4 * 4
+  assert ctypes.sizeof(_lib.SinkhornOut) == 3 * 8 + 3 * 8 + 4 * 4
 
 
 def test_invalid_arguments_are_rejected_without_gpu_work():
@@ -58,3 +58,20 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
   assert lib.ottx_mean_cost_workspace_bytes(0, 3, 2) == 0
   rc = lib.ottx_sinkhorn_solve(ctypes.byref(pb), None, None, None, 0, None)
   assert rc == _lib.OTTX_EINVAL
+
+
+@pytest.mark.parametrize("n,m,d", [(257, 129, 32), (1, 1, 1), (17, 29, 4), (5000, 3, 256)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_workspace_planning_all_shapes(n, m, d, precision):
+  """Host-side planning of every tile configuration (no kernel launch)."""
+  from ott_amd import _lib
+  if not os.path.exists(_lib.LIB_PATH):
+    pytest.skip("libottx.so not built")
+  lib = _lib.load_library()
+  pb = _lib.Problem(n=n, m=m, d=d, precision=precision, x=1, y=1, a=1, b=1, epsilon=0.1,
+                    inv_scale_cost=1.0, tau_a=1.0, tau_b=1.0, relative_epsilon=0.0)
+  opts = _lib.SinkhornOpts(threshold=1e-3, norm_error=1, inner_iterations=10, min_iterations=0,
+                           max_iterations=100, momentum=1.0, tile_config=-1, use_graph=1)
+  assert lib.ottx_op_workspace_bytes(ctypes.byref(pb)) > 0
+  assert lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts)) > 0
+  assert lib.ottx_tile_configs(precision, d, None, 0) > 0
