@@ -167,6 +167,65 @@ int ottx_benchmark_halfstep(const ottx_problem* problem, int32_t axis,
                             float* g, double* avg_ms_host, void* workspace,
                             size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Learned warm start: a DeepSets network predicting g from the two clouds.
+ * No direct reference counterpart: OTT's learned initializer
+ * (initializers/neural/meta_initializer.py:35-213) maps the weights of a fixed
+ * geometry to f with flax and optax.adam; this network reads the points, is
+ * trained with the kernels below (forward, backward, Adam), and its output is
+ * turned into a warm start (f0 from one f-update of g) by the Python layer.
+ * Parameters are one flat float32 buffer of ottx_ds_param_count(d) values
+ * (layout in ott_amd/csrc/ottx_deepsets.hip: param_layout); bf16 copies of the
+ * weight matrices (ottx_ds_weight_bf16_count(d) values) feed the MFMA GEMMs.
+ * ------------------------------------------------------------------------ */
+typedef struct ottx_ds_model {
+  int32_t d;               /* feature dimension of the clouds */
+  int32_t _pad;
+  float* params;           /* [ottx_ds_param_count(d)] device, float32 master weights */
+  void* weights_bf16;      /* [ottx_ds_weight_bf16_count(d)] device, bf16 copies */
+} ottx_ds_model;
+
+/* A batch of problems; all arrays are device memory.  Encoder rows are the
+ * points ordered x_0, y_0, x_1, y_1, ... (segment 2 b + side); head rows are
+ * the y points in y order. */
+typedef struct ottx_ds_batch {
+  int32_t count;           /* problems B */
+  int32_t d;
+  int32_t n_total;         /* sum of n_b */
+  int32_t m_total;         /* sum of m_b */
+  const float* x;          /* [n_total, d] problems concatenated */
+  const float* y;          /* [m_total, d] */
+  const float* eps;        /* [B] target epsilon of each problem */
+  const int32_t* x_off;    /* [B + 1] row offsets into x */
+  const int32_t* y_off;    /* [B + 1] row offsets into y */
+  const int32_t* y_prob;   /* [roundup(m_total, 64)] problem of each head row, 0 on padding */
+  const int32_t* enc_src;  /* [n_total + m_total] row in x (side 0) or y (side 1) of encoder row r */
+  const int32_t* seg_of_row; /* [n_total + m_total] segment 2 b + side of encoder row r */
+  const int32_t* seg_off;  /* [2 B + 1] encoder row offsets of the segments */
+} ottx_ds_batch;
+
+int64_t ottx_ds_param_count(int32_t d);
+int64_t ottx_ds_weight_bf16_count(int32_t d);
+size_t ottx_ds_workspace_bytes(const ottx_ds_batch* batch);
+/* Recompute the bf16 weight copies from the float32 parameters. */
+int ottx_ds_refresh_bf16(const ottx_ds_model* model, void* stream);
+/* g_out[m_total]: predicted potentials g_j = eps_b * psi(...) (up to a constant). */
+int ottx_ds_forward(const ottx_ds_model* model, const ottx_ds_batch* batch,
+                    float* g_out, void* workspace, size_t workspace_bytes,
+                    void* stream);
+/* Centred mean-squared loss against target[m_total] (potentials divided by
+ * eps, any per-problem constant), its gradient into grad[param_count] and the
+ * per-problem losses into loss_out[B] (nullable). */
+int ottx_ds_loss_backward(const ottx_ds_model* model,
+                          const ottx_ds_batch* batch, const float* target,
+                          float* grad, float* loss_out, void* workspace,
+                          size_t workspace_bytes, void* stream);
+/* One Adam step (bias-corrected, optax.adam semantics) on the float32
+ * parameters, then a refresh of the bf16 copies. */
+int ottx_ds_adam_step(const ottx_ds_model* model, const float* grad, float* m,
+                      float* v, int32_t step, float lr, float beta1,
+                      float beta2, float eps, void* stream);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -17,7 +17,7 @@ CSRC = os.path.join(_HERE, "csrc")
 LIB_DIR = os.path.join(_HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libottx.so")
 
-SOURCES = ["ottx_sinkhorn.hip"]
+SOURCES = ["ottx_sinkhorn.hip", "ottx_deepsets.hip"]
 ARCH = os.environ.get("OTTX_OFFLOAD_ARCH", "gfx950")
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

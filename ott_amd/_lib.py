@@ -79,6 +79,33 @@ class SinkhornOut(ctypes.Structure):
   ]
 
 
+class DsModel(ctypes.Structure):
+  _fields_ = [
+      ("d", ctypes.c_int32),
+      ("_pad", ctypes.c_int32),
+      ("params", ctypes.c_void_p),
+      ("weights_bf16", ctypes.c_void_p),
+  ]
+
+
+class DsBatch(ctypes.Structure):
+  _fields_ = [
+      ("count", ctypes.c_int32),
+      ("d", ctypes.c_int32),
+      ("n_total", ctypes.c_int32),
+      ("m_total", ctypes.c_int32),
+      ("x", ctypes.c_void_p),
+      ("y", ctypes.c_void_p),
+      ("eps", ctypes.c_void_p),
+      ("x_off", ctypes.c_void_p),
+      ("y_off", ctypes.c_void_p),
+      ("y_prob", ctypes.c_void_p),
+      ("enc_src", ctypes.c_void_p),
+      ("seg_of_row", ctypes.c_void_p),
+      ("seg_off", ctypes.c_void_p),
+  ]
+
+
 # (name, restype, argtypes) of every exported symbol declared in include/*.h.
 _SIGNATURES = [
     ("ottx_abi_version", ctypes.c_int, []),
@@ -106,6 +133,19 @@ _SIGNATURES = [
      [ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    ("ottx_ds_param_count", ctypes.c_int64, [ctypes.c_int32]),
+    ("ottx_ds_weight_bf16_count", ctypes.c_int64, [ctypes.c_int32]),
+    ("ottx_ds_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(DsBatch)]),
+    ("ottx_ds_refresh_bf16", ctypes.c_int, [ctypes.POINTER(DsModel), ctypes.c_void_p]),
+    ("ottx_ds_forward", ctypes.c_int,
+     [ctypes.POINTER(DsModel), ctypes.POINTER(DsBatch), ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_size_t, ctypes.c_void_p]),
+    ("ottx_ds_loss_backward", ctypes.c_int,
+     [ctypes.POINTER(DsModel), ctypes.POINTER(DsBatch), ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    ("ottx_ds_adam_step", ctypes.c_int,
+     [ctypes.POINTER(DsModel), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+      ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]),
 ]
 
 EXPORTED_SYMBOLS = tuple(name for name, _, _ in _SIGNATURES)

@@ -1,0 +1,885 @@
+// DeepSets warm-start network for the Sinkhorn solver: forward, backward and
+// Adam kernels (gfx950).
+//
+// OTT's learned initializer (initializers/neural/meta_initializer.py:35-213)
+// is an MLP on the weight vectors of a fixed geometry, trained by flax/optax.
+// This network instead reads the point clouds themselves:
+//   encoder  phi: R^d -> R^256, 3 layers (ReLU, ReLU, linear), shared by x and y
+//   pooling  e_x = mean_i phi(x~_i),  e_y = mean_j phi(y~_j)
+//   head     g~_j = psi(y~_j, e_x, e_y, kappa), 3 layers (ReLU, ReLU, linear)
+// with x~ = (x - mu) / sigma, y~ = (y - mu) / sigma per problem
+// (mu = (mean x + mean y) / 2, sigma^2 = mean(C) / 2) and kappa = eps / sigma^2.
+// The prediction is g_j = eps * g~_j (a potential, up to a constant).
+//
+// Every matrix product runs through one bf16 MFMA GEMM, C = A . Bt^T with both
+// operands K-contiguous ("NT"), f32 accumulation, and an epilogue that adds a
+// column bias and/or a per-row-group bias, applies ReLU or a ReLU mask, and
+// writes f32, bf16 and transposed-bf16 copies of the result as requested.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/ottx.h"
+
+namespace ottx_ds {
+
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+typedef uint16_t bf16_t;
+
+__device__ __forceinline__ bf16_t to_bf16(float v) {
+  uint32_t u = __float_as_uint(v);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<bf16_t>(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(bf16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// ---------------------------------------------------------------------------
+// GEMM: C[M, N] (+)= A[M, K] . Bt[N, K]^T.  M, N multiples of 64, K multiple
+// of 32 (per split).  Tile 64 x 64, 4 waves each owning 32 x 32.
+// ---------------------------------------------------------------------------
+struct GemmArgs {
+  const bf16_t* A;
+  const bf16_t* Bt;
+  int64_t lda, ldb;      // row strides (elements)
+  int M, N, K;           // K = reduction length of one split
+  int splits;            // gridDim.z; split z covers k in [z K, (z + 1) K)
+  const float* bias;     // [N] column bias, or null
+  const float* gbias;    // [groups, N] per-row-group bias, or null
+  const int* row_group;  // [M] group of each row (for gbias), or null
+  int relu;              // apply max(0, .) after the biases
+  const bf16_t* mask;    // [M, ldm] multiply by (mask > 0), or null
+  int64_t ldm;
+  float* Cf;             // f32 output [M, ldc] (or [splits][M, ldc] partials)
+  int64_t ldc;
+  bf16_t* Cb;            // bf16 output [M, ldcb], or null
+  int64_t ldcb;
+  bf16_t* CbT;           // bf16 transposed output [N, ldct], or null
+  int64_t ldct;
+  int valid_rows;        // rows >= valid_rows are written as zero
+};
+
+__global__ __launch_bounds__(256) void gemm_nt_kernel(const GemmArgs g) {
+  constexpr int LDS_STRIDE = 40;  // bf16 per LDS row (32 + 8 pad)
+  __shared__ bf16_t As[64 * LDS_STRIDE];
+  __shared__ bf16_t Bs[64 * LDS_STRIDE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int64_t kbase = static_cast<int64_t>(blockIdx.z) * g.K;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int lr = tid >> 2, lc = (tid & 3) * 8;  // loader: row, 8-element column group
+  for (int k0 = 0; k0 < g.K; k0 += 32) {
+    const u32x4 av = *reinterpret_cast<const u32x4*>(g.A + (m0 + lr) * g.lda + kbase + k0 + lc);
+    const u32x4 bv = *reinterpret_cast<const u32x4*>(g.Bt + (n0 + lr) * g.ldb + kbase + k0 + lc);
+    __syncthreads();
+    *reinterpret_cast<u32x4*>(As + lr * LDS_STRIDE + lc) = av;
+    *reinterpret_cast<u32x4*>(Bs + lr * LDS_STRIDE + lc) = bv;
+    __syncthreads();
+    bf16x8 af[2], bf[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                                             As + (wr + i * 16 + (lane & 15)) * LDS_STRIDE + 8 * (lane >> 4)));
+      bf[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                                             Bs + (wc + i * 16 + (lane & 15)) * LDS_STRIDE + 8 * (lane >> 4)));
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
+  }
+  // Epilogue: element (row 4 (lane / 16) + r, column lane % 16) of each block.
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wr + i * 16 + 4 * (lane >> 4) + r;
+        const int col = n0 + wc + j * 16 + (lane & 15);
+        float v = acc[i][j][r];
+        if (g.splits > 1) {
+          g.Cf[static_cast<int64_t>(blockIdx.z) * g.M * g.ldc + static_cast<int64_t>(row) * g.ldc + col] = v;
+          continue;
+        }
+        if (g.bias) v += g.bias[col];
+        if (g.gbias) v += g.gbias[static_cast<int64_t>(g.row_group[row]) * g.N + col];
+        if (g.relu) v = fmaxf(v, 0.f);
+        if (g.mask && !(from_bf16(g.mask[static_cast<int64_t>(row) * g.ldm + col]) > 0.f)) v = 0.f;
+        if (row >= g.valid_rows) v = 0.f;
+        if (g.Cf) g.Cf[static_cast<int64_t>(row) * g.ldc + col] = v;
+        if (g.Cb) g.Cb[static_cast<int64_t>(row) * g.ldcb + col] = to_bf16(v);
+        if (g.CbT) g.CbT[static_cast<int64_t>(col) * g.ldct + row] = to_bf16(v);
+      }
+}
+
+// Sum of split-K partials: out[i] = sum_z part[z][i] (deterministic order).
+__global__ void sum_splits_kernel(const float* __restrict__ part, int splits, int64_t count, float* __restrict__ out,
+                                  int accumulate) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float s = 0.f;
+  for (int z = 0; z < splits; ++z) s += part[z * count + i];
+  out[i] = accumulate ? out[i] + s : s;
+}
+
+// ---------------------------------------------------------------------------
+// Normalisation statistics per problem: mu = (mean x + mean y) / 2 and
+// sigma^2 = mean(C) / 2 with mean(C) = mean|x|^2 + mean|y|^2 - 2 <mean x, mean y>.
+// One workgroup per problem (thread per feature, rows strided).
+// ---------------------------------------------------------------------------
+__global__ void stats_kernel(const float* __restrict__ x, const float* __restrict__ y, const int* __restrict__ xoff,
+                             const int* __restrict__ yoff, int d, float* __restrict__ mu, float* __restrict__ sig2) {
+  __shared__ double red[3][256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int G = max(1, 256 / d);
+  const int k = t % d, gi = t / d;
+  double sx = 0, sxx = 0, sy = 0, syy = 0;
+  const int x0 = xoff[b], x1 = xoff[b + 1], y0 = yoff[b], y1 = yoff[b + 1];
+  __shared__ double fs[4][256];
+  if (t < G * d) {
+    for (int r = x0 + gi; r < x1; r += G) {
+      const double v = x[static_cast<int64_t>(r) * d + k];
+      sx += v;
+      sxx += v * v;
+    }
+    for (int r = y0 + gi; r < y1; r += G) {
+      const double v = y[static_cast<int64_t>(r) * d + k];
+      sy += v;
+      syy += v * v;
+    }
+  }
+  fs[0][t] = sx;
+  fs[1][t] = sxx;
+  fs[2][t] = sy;
+  fs[3][t] = syy;
+  __syncthreads();
+  double nx2 = 0, ny2 = 0, dot = 0;
+  if (t < d) {
+    double a1 = 0, a2 = 0, b1 = 0, b2 = 0;
+    for (int q = 0; q < G; ++q) {
+      a1 += fs[0][q * d + t];
+      a2 += fs[1][q * d + t];
+      b1 += fs[2][q * d + t];
+      b2 += fs[3][q * d + t];
+    }
+    const double mx = a1 / (x1 - x0), my = b1 / (y1 - y0);
+    mu[b * d + t] = static_cast<float>(0.5 * (mx + my));
+    nx2 = a2;
+    ny2 = b2;
+    dot = mx * my;
+  }
+  red[0][t] = nx2;
+  red[1][t] = ny2;
+  red[2][t] = dot;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int q = 0; q < 3; ++q) red[q][t] += red[q][t + s];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double mc = red[0][0] / (x1 - x0) + red[1][0] / (y1 - y0) - 2.0 * red[2][0];
+    sig2[b] = static_cast<float>(fmax(0.5 * mc, 1e-30));
+  }
+}
+
+// Normalised bf16 inputs of the encoder: row r (rows ordered x_0, y_0, x_1,
+// y_1, ... by segment 2 b + side) comes from x or y row src[r]; written to
+// X [rows_pad, kpad] and transposed XT [kpad, rows_pad], zero-padded.
+__global__ void enc_normalize_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                     const int* __restrict__ src, const int* __restrict__ seg_of_row, int64_t rows,
+                                     int64_t rows_pad, int d, int kpad, const float* __restrict__ mu,
+                                     const float* __restrict__ sig2, bf16_t* __restrict__ X, bf16_t* __restrict__ XT) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows_pad * kpad) return;
+  const int64_t r = i / kpad;
+  const int k = static_cast<int>(i % kpad);
+  float v = 0.f;
+  if (r < rows && k < d) {
+    const int seg = seg_of_row[r];
+    const int b = seg >> 1;
+    const float* pts = (seg & 1) ? y : x;
+    v = (pts[static_cast<int64_t>(src[r]) * d + k] - mu[b * d + k]) * rsqrtf(sig2[b]);
+  }
+  const bf16_t h = to_bf16(v);
+  X[i] = h;
+  XT[static_cast<int64_t>(k) * rows_pad + r] = h;
+}
+
+// Normalised bf16 inputs of the head: the y points in their batch order.
+__global__ void head_normalize_kernel(const float* __restrict__ y, const int* __restrict__ prob, int64_t rows,
+                                      int64_t rows_pad, int d, int kpad, const float* __restrict__ mu,
+                                      const float* __restrict__ sig2, bf16_t* __restrict__ Y, bf16_t* __restrict__ YT) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows_pad * kpad) return;
+  const int64_t r = i / kpad;
+  const int k = static_cast<int>(i % kpad);
+  float v = 0.f;
+  if (r < rows && k < d) {
+    const int b = prob[r];
+    v = (y[r * d + k] - mu[b * d + k]) * rsqrtf(sig2[b]);
+  }
+  const bf16_t h = to_bf16(v);
+  Y[i] = h;
+  YT[static_cast<int64_t>(k) * rows_pad + r] = h;
+}
+
+// Mean pooling of the encoder output H [P, 256] per (problem, side), with
+// float partial sums over row chunks: part[(b * 2 + side) * chunks + c][256].
+__global__ void pool_partial_kernel(const float* __restrict__ H, const int* __restrict__ seg_off, int chunks,
+                                    float* __restrict__ part) {
+  const int seg = blockIdx.y, c = blockIdx.x, f = threadIdx.x;
+  const int r0 = seg_off[seg], r1 = seg_off[seg + 1];
+  const int per = (r1 - r0 + chunks - 1) / chunks;
+  const int a = r0 + c * per, e = min(r1, a + per);
+  float s = 0.f;
+  for (int r = a; r < e; ++r) s += H[static_cast<int64_t>(r) * 256 + f];
+  part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f] = s;
+}
+
+// ctx[b] = [e_x (256), e_y (256), kappa = eps / sigma^2, 0 pad ...] (ld 544).
+__global__ void pool_final_kernel(const float* __restrict__ part, const int* __restrict__ seg_off, int chunks,
+                                  const float* __restrict__ eps, const float* __restrict__ sig2,
+                                  float* __restrict__ ctx) {
+  const int b = blockIdx.x, f = threadIdx.x;
+  for (int side = 0; side < 2; ++side) {
+    const int seg = b * 2 + side;
+    float s = 0.f;
+    for (int c = 0; c < chunks; ++c) s += part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f];
+    ctx[b * 544 + side * 256 + f] = s / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
+  }
+  if (f < 32) ctx[b * 544 + 512 + f] = (f == 0) ? eps[b] / sig2[b] : 0.f;
+}
+
+// hctx[b, o] = sum_k Wc[o, k] ctx[b, k] + bh[o]   (Wc [256, 544] f32).
+__global__ void ctx_forward_kernel(const float* __restrict__ Wc, const float* __restrict__ bh,
+                                   const float* __restrict__ ctx, float* __restrict__ hctx) {
+  const int b = blockIdx.x, o = threadIdx.x;
+  float s = bh[o];
+  for (int k = 0; k < 544; ++k) s = fmaf(Wc[o * 544 + k], ctx[b * 544 + k], s);
+  hctx[b * 256 + o] = s;
+}
+
+// Last layer: out[q] = sum_k w5[k] A[q, k] + b5 (A f32 [P, 256]).
+__global__ void head_out_kernel(const float* __restrict__ A, const float* __restrict__ w5, const float* __restrict__ b5,
+                                int64_t rows, float* __restrict__ out) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= rows) return;
+  float s = b5[0];
+  for (int k = 0; k < 256; ++k) s = fmaf(w5[k], A[q * 256 + k], s);
+  out[q] = s;
+}
+
+// Centred MSE per problem on the head points:
+// L = (1/B) sum_b (1/m_b) sum_j (r_j - mean_b r)^2,  r = out - target;
+// dout_j = 2 (r_j - mean_b r) / (B m_b).  loss[b] receives L_b.
+__global__ void loss_kernel(const float* __restrict__ out, const float* __restrict__ target,
+                            const int* __restrict__ yoff, int nprob, float* __restrict__ dout,
+                            float* __restrict__ loss) {
+  __shared__ double red[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int q0 = yoff[b], q1 = yoff[b + 1];
+  double s = 0;
+  for (int q = q0 + t; q < q1; q += 256) s += static_cast<double>(out[q]) - target[q];
+  red[t] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (t < k) red[t] += red[t + k];
+    __syncthreads();
+  }
+  const double mean = red[0] / (q1 - q0);
+  __syncthreads();
+  double l = 0;
+  for (int q = q0 + t; q < q1; q += 256) {
+    const double r = static_cast<double>(out[q]) - target[q] - mean;
+    l += r * r;
+    if (dout) dout[q] = static_cast<float>(2.0 * r / (static_cast<double>(nprob) * (q1 - q0)));
+  }
+  red[t] = l;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (t < k) red[t] += red[t + k];
+    __syncthreads();
+  }
+  if (t == 0) loss[b] = static_cast<float>(red[0] / (q1 - q0));
+}
+
+// Backward of the last layer: dZ[q, k] = dout[q] w5[k] (A[q, k] > 0), bf16 and
+// transposed bf16; rows >= rows are zero.
+__global__ void head_out_backward_kernel(const float* __restrict__ dout, const float* __restrict__ w5,
+                                         const float* __restrict__ A, int64_t rows, int64_t rows_pad,
+                                         bf16_t* __restrict__ dZ, bf16_t* __restrict__ dZT) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows_pad * 256) return;
+  const int64_t q = i / 256;
+  const int k = static_cast<int>(i % 256);
+  float v = 0.f;
+  if (q < rows && A[q * 256 + k] > 0.f) v = dout[q] * w5[k];
+  const bf16_t h = to_bf16(v);
+  dZ[i] = h;
+  dZT[static_cast<int64_t>(k) * rows_pad + q] = h;
+}
+
+// Column sums over row chunks (bias gradients and per-group sums):
+// part[c][N] = sum_{rows of chunk c} M[r, :], with M bf16 [rows, N] or f32.
+__global__ void colsum_partial_kernel(const bf16_t* __restrict__ Mb, const float* __restrict__ Mf, int64_t rows,
+                                      int N, int chunks, float* __restrict__ part) {
+  const int c = blockIdx.x;
+  const int64_t per = (rows + chunks - 1) / chunks;
+  const int64_t r0 = c * per, r1 = min(rows, r0 + per);
+  for (int f = threadIdx.x; f < N; f += blockDim.x) {
+    float s = 0.f;
+    for (int64_t r = r0; r < r1; ++r) s += Mb ? from_bf16(Mb[r * N + f]) : Mf[r * N + f];
+    part[static_cast<int64_t>(c) * N + f] = s;
+  }
+}
+
+// Per-group row sums: out[b, :] = sum_{r in [off[b], off[b+1])} M[r, :] (bf16 M).
+__global__ void group_sum_kernel(const bf16_t* __restrict__ M, const int* __restrict__ off, int N,
+                                 float* __restrict__ out) {
+  const int b = blockIdx.x;
+  for (int f = threadIdx.x; f < N; f += blockDim.x) {
+    float s = 0.f;
+    for (int r = off[b]; r < off[b + 1]; ++r) s += from_bf16(M[static_cast<int64_t>(r) * N + f]);
+    out[static_cast<int64_t>(b) * N + f] = s;
+  }
+}
+
+// dw5[k] = sum_q dout[q] A[q, k] and db5 = sum_q dout[q], over row chunks.
+__global__ void head_out_wgrad_kernel(const float* __restrict__ dout, const float* __restrict__ A, int64_t rows,
+                                      int chunks, float* __restrict__ part) {
+  const int c = blockIdx.x, k = threadIdx.x;
+  const int64_t per = (rows + chunks - 1) / chunks;
+  const int64_t r0 = c * per, r1 = min(rows, r0 + per);
+  float s = 0.f, sb = 0.f;
+  for (int64_t q = r0; q < r1; ++q) {
+    s = fmaf(dout[q], A[q * 256 + k], s);
+    if (k == 0) sb += dout[q];
+  }
+  part[static_cast<int64_t>(c) * 257 + k] = s;
+  if (k == 0) part[static_cast<int64_t>(c) * 257 + 256] = sb;
+}
+
+// Context backward: dWc[o, k] += sum_b dh[b, o] ctx[b, k]; dctx[b, k] = sum_o Wc[o, k] dh[b, o].
+__global__ void ctx_backward_kernel(const float* __restrict__ Wc, const float* __restrict__ ctx,
+                                    const float* __restrict__ dh, int nprob, float* __restrict__ dWc,
+                                    float* __restrict__ dctx) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 544) return;
+  for (int o = 0; o < 256; ++o) {
+    float s = 0.f;
+    for (int b = 0; b < nprob; ++b) s = fmaf(dh[b * 256 + o], ctx[b * 544 + k], s);
+    dWc[o * 544 + k] = s;
+  }
+  for (int b = 0; b < nprob; ++b) {
+    float s = 0.f;
+    for (int o = 0; o < 256; ++o) s = fmaf(Wc[o * 544 + k], dh[b * 256 + o], s);
+    dctx[b * 544 + k] = s;
+  }
+}
+
+// Pool backward: dH[r, f] = dctx[b(r), side(r) * 256 + f] / count(segment),
+// bf16 and transposed bf16, zero on padded rows.
+__global__ void pool_backward_kernel(const float* __restrict__ dctx, const int* __restrict__ seg_of_row,
+                                     const int* __restrict__ seg_off, int64_t rows, int64_t rows_pad,
+                                     bf16_t* __restrict__ dH, bf16_t* __restrict__ dHT) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= rows_pad * 256) return;
+  const int64_t r = i / 256;
+  const int f = static_cast<int>(i % 256);
+  float v = 0.f;
+  if (r < rows) {
+    const int seg = seg_of_row[r];
+    v = dctx[(seg >> 1) * 544 + (seg & 1) * 256 + f] / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
+  }
+  const bf16_t h = to_bf16(v);
+  dH[i] = h;
+  dHT[static_cast<int64_t>(f) * rows_pad + r] = h;
+}
+
+// Adam on a flat f32 parameter buffer (optax.adam semantics with bias
+// correction), then the bf16 copies: W (row-major) and WT (transposed) for the
+// tensors listed in the copy table.
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, int64_t count, float lr, float b1, float b2, float eps,
+                            float bc1, float bc2) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ src, int rows, int cols, int ld_src, bf16_t* __restrict__ dst,
+                               int ld_dst, bf16_t* __restrict__ dstT, int ld_dstT) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<int64_t>(rows) * cols) return;
+  const int r = static_cast<int>(i / cols), c = static_cast<int>(i % cols);
+  const bf16_t h = to_bf16(src[static_cast<int64_t>(r) * ld_src + c]);
+  if (dst) dst[static_cast<int64_t>(r) * ld_dst + c] = h;
+  if (dstT) dstT[static_cast<int64_t>(c) * ld_dstT + r] = h;
+}
+
+// The head's prediction as a potential: g[q] = eps[b(q)] * out[q].
+__global__ void scale_out_kernel(const float* __restrict__ out, const int* __restrict__ prob,
+                                 const float* __restrict__ eps, int64_t rows, float* __restrict__ g) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q < rows) g[q] = eps[prob[q]] * out[q];
+}
+
+}  // namespace ottx_ds
+
+using namespace ottx_ds;
+
+namespace {
+
+#define DS_LAUNCHED()                                     \
+  do {                                                    \
+    if (hipGetLastError() != hipSuccess) return OTTX_EHIP; \
+  } while (0)
+
+inline int blocks_for(int64_t n, int t = 256) { return static_cast<int>((n + t - 1) / t); }
+inline int64_t pad64(int64_t v) { return (v + 63) / 64 * 64; }
+inline size_t al(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+// Parameter layout inside the flat f32 buffer (offsets in floats).
+struct ParamLayout {
+  int kin;  // padded input width (multiple of 64: a GEMM N dimension in the gradients)
+  int64_t W0, b0, W1, b1, W2, b2, Wy, Wc, bh, W4, b4, w5, b5, total;
+};
+
+ParamLayout param_layout(int d) {
+  ParamLayout L;
+  L.kin = (d + 63) / 64 * 64;
+  int64_t o = 0;
+  L.W0 = o; o += 256LL * L.kin;
+  L.b0 = o; o += 256;
+  L.W1 = o; o += 256LL * 256;
+  L.b1 = o; o += 256;
+  L.W2 = o; o += 256LL * 256;
+  L.b2 = o; o += 256;
+  L.Wy = o; o += 256LL * L.kin;
+  L.Wc = o; o += 256LL * 544;
+  L.bh = o; o += 256;
+  L.W4 = o; o += 256LL * 256;
+  L.b4 = o; o += 256;
+  L.w5 = o; o += 256;
+  L.b5 = o; o += 64;
+  L.total = o;
+  return L;
+}
+
+// bf16 weight copies (row-major W and transposed WT) for the GEMM operands.
+struct WeightCopies {
+  int64_t W0, W0T, W1, W1T, W2, W2T, Wy, WyT, W4, W4T, total;  // offsets in bf16 elements
+};
+
+WeightCopies weight_copies(const ParamLayout& L) {
+  WeightCopies C;
+  int64_t o = 0;
+  C.W0 = o; o += 256LL * L.kin;
+  C.W0T = o; o += 256LL * L.kin;
+  C.W1 = o; o += 65536;
+  C.W1T = o; o += 65536;
+  C.W2 = o; o += 65536;
+  C.W2T = o; o += 65536;
+  C.Wy = o; o += 256LL * L.kin;
+  C.WyT = o; o += 256LL * L.kin;
+  C.W4 = o; o += 65536;
+  C.W4T = o; o += 65536;
+  C.total = o;
+  return C;
+}
+
+// Workspace layout for one batch.
+struct DsLayout {
+  int64_t Pe, Pe_pad, Ph, Ph_pad;  // encoder / head rows (valid, padded)
+  int nprob, chunks, kchunks;
+  size_t mu, sig2, kappa, X0, Y0, A0, A0T, A1, A1T, H2f, part, ctx, hctx, Z1h, Z1hT, A2h, A2hT, A2f, out, dout, loss,
+      dZ, dZT, dA, dAT, dctx, dh, gpart, total;
+};
+
+DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P) {
+  DsLayout L{};
+  L.nprob = bt->count;
+  L.Pe = bt->n_total + bt->m_total;
+  L.Pe_pad = pad64(L.Pe);
+  L.Ph = bt->m_total;
+  L.Ph_pad = pad64(L.Ph);
+  L.chunks = 16;
+  L.kchunks = 64;
+  size_t o = 0;
+  auto take = [&](size_t b) {
+    const size_t at = o;
+    o = al(o + b);
+    return at;
+  };
+  const int64_t Pe = L.Pe_pad, Ph = L.Ph_pad;
+  const int B = L.nprob;
+  L.mu = take(static_cast<size_t>(B) * bt->d * 4);
+  L.sig2 = take(static_cast<size_t>(B) * 4);
+  L.kappa = take(static_cast<size_t>(B) * 4);
+  L.X0 = take(static_cast<size_t>(Pe) * P.kin * 2 * 2);  // X0 and X0T
+  L.Y0 = take(static_cast<size_t>(Ph) * P.kin * 2 * 2);  // Y0 and Y0T
+  L.A0 = take(static_cast<size_t>(Pe) * 256 * 2);
+  L.A0T = take(static_cast<size_t>(Pe) * 256 * 2);
+  L.A1 = take(static_cast<size_t>(Pe) * 256 * 2);
+  L.A1T = take(static_cast<size_t>(Pe) * 256 * 2);
+  L.H2f = take(static_cast<size_t>(Pe) * 256 * 4);
+  L.part = take(static_cast<size_t>(2 * B) * L.chunks * 256 * 4);
+  L.ctx = take(static_cast<size_t>(B) * 544 * 4);
+  L.hctx = take(static_cast<size_t>(B) * 256 * 4);
+  L.Z1h = take(static_cast<size_t>(Ph) * 256 * 2);
+  L.Z1hT = take(static_cast<size_t>(Ph) * 256 * 2);
+  L.A2h = take(static_cast<size_t>(Ph) * 256 * 2);
+  L.A2hT = take(static_cast<size_t>(Ph) * 256 * 2);
+  L.A2f = take(static_cast<size_t>(Ph) * 256 * 4);
+  L.out = take(static_cast<size_t>(Ph) * 4);
+  L.dout = take(static_cast<size_t>(Ph) * 4);
+  L.loss = take(static_cast<size_t>(B) * 4);
+  const int64_t Pm = Pe > Ph ? Pe : Ph;
+  L.dZ = take(static_cast<size_t>(Pm) * 256 * 2);
+  L.dZT = take(static_cast<size_t>(Pm) * 256 * 2);
+  L.dA = take(static_cast<size_t>(Pm) * 256 * 2);
+  L.dAT = take(static_cast<size_t>(Pm) * 256 * 2);
+  L.dctx = take(static_cast<size_t>(B) * 544 * 4);
+  L.dh = take(static_cast<size_t>(B) * 256 * 4);
+  L.gpart = take(static_cast<size_t>(L.kchunks) * 256 * 544 * 4);
+  L.total = o;
+  return L;
+}
+
+int gemm(hipStream_t st, const GemmArgs& a) {
+  if (a.M % 64 || a.N % 64 || a.K % 32 || a.splits < 1) return OTTX_EINVAL;
+  hipLaunchKernelGGL(gemm_nt_kernel, dim3(a.N / 64, a.M / 64, a.splits), dim3(256), 0, st, a);
+  DS_LAUNCHED();
+  return OTTX_OK;
+}
+
+GemmArgs gemm_args(const bf16_t* A, int64_t lda, const bf16_t* Bt, int64_t ldb, int M, int N, int K) {
+  GemmArgs g{};
+  g.A = A;
+  g.Bt = Bt;
+  g.lda = lda;
+  g.ldb = ldb;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.splits = 1;
+  g.valid_rows = M;
+  return g;
+}
+
+// dW[N_out, K_in] = dZ^T . X  as split-K GEMM over the rows, summed into grad.
+int weight_grad(hipStream_t st, const bf16_t* dZT, int64_t rows_pad, const bf16_t* XT, int n_out, int k_in,
+                float* gpart, int kchunks, float* grad) {
+  int splits = kchunks;
+  while (splits > 1 && (rows_pad % splits != 0 || (rows_pad / splits) % 32 != 0)) --splits;
+  GemmArgs g = gemm_args(dZT, rows_pad, XT, rows_pad, n_out, k_in, static_cast<int>(rows_pad / splits));
+  g.splits = splits;
+  g.Cf = gpart;
+  g.ldc = k_in;
+  int rc = gemm(st, g);
+  if (rc != OTTX_OK) return rc;
+  const int64_t count = static_cast<int64_t>(n_out) * k_in;
+  hipLaunchKernelGGL(sum_splits_kernel, dim3(blocks_for(count)), dim3(256), 0, st, gpart, splits, count, grad, 0);
+  DS_LAUNCHED();
+  return OTTX_OK;
+}
+
+// grad[0:N] = column sums of M (bf16 [rows, N]).
+int bias_grad(hipStream_t st, const bf16_t* M, int64_t rows, int N, float* part, int chunks, float* grad) {
+  hipLaunchKernelGGL(colsum_partial_kernel, dim3(chunks), dim3(256), 0, st, M, static_cast<const float*>(nullptr), rows,
+                     N, chunks, part);
+  hipLaunchKernelGGL(sum_splits_kernel, dim3(blocks_for(N)), dim3(256), 0, st, part, chunks, static_cast<int64_t>(N),
+                     grad, 0);
+  DS_LAUNCHED();
+  return OTTX_OK;
+}
+
+struct Forward {
+  DsLayout L;
+  ParamLayout P;
+  WeightCopies C;
+  char* ws;
+  const float* params;
+  const bf16_t* wb;
+};
+
+template <typename T>
+T* at(char* base, size_t off) {
+  return reinterpret_cast<T*>(base + off);
+}
+
+// The forward pass up to the head output (out[Ph]).
+int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, hipStream_t st, Forward* F) {
+  F->P = param_layout(bt->d);
+  F->C = weight_copies(F->P);
+  F->L = ds_layout(bt, F->P);
+  const ParamLayout& P = F->P;
+  const WeightCopies& C = F->C;
+  const DsLayout& L = F->L;
+  char* ws = static_cast<char*>(workspace);
+  F->ws = ws;
+  F->params = md->params;
+  F->wb = reinterpret_cast<const bf16_t*>(md->weights_bf16);
+  const float* prm = md->params;
+  const bf16_t* wb = F->wb;
+  const int B = L.nprob;
+  const int64_t Pe = L.Pe_pad, Ph = L.Ph_pad;
+  float* mu = at<float>(ws, L.mu);
+  float* sig2 = at<float>(ws, L.sig2);
+  hipLaunchKernelGGL(stats_kernel, dim3(B), dim3(256), 0, st, bt->x, bt->y, bt->x_off, bt->y_off, bt->d, mu, sig2);
+  DS_LAUNCHED();
+  bf16_t* X0 = at<bf16_t>(ws, L.X0);
+  bf16_t* X0T = X0 + Pe * P.kin;
+  bf16_t* Y0 = at<bf16_t>(ws, L.Y0);
+  bf16_t* Y0T = Y0 + Ph * P.kin;
+  hipLaunchKernelGGL(enc_normalize_kernel, dim3(blocks_for(Pe * P.kin)), dim3(256), 0, st, bt->x, bt->y, bt->enc_src,
+                     bt->seg_of_row, L.Pe, Pe, bt->d, P.kin, mu, sig2, X0, X0T);
+  hipLaunchKernelGGL(head_normalize_kernel, dim3(blocks_for(Ph * P.kin)), dim3(256), 0, st, bt->y, bt->y_prob, L.Ph,
+                     Ph, bt->d, P.kin, mu, sig2, Y0, Y0T);
+  DS_LAUNCHED();
+  int rc;
+  // encoder layer 0: A0 = relu(X0 W0^T + b0)
+  GemmArgs g = gemm_args(X0, P.kin, wb + C.W0, P.kin, static_cast<int>(Pe), 256, P.kin);
+  g.bias = prm + P.b0;
+  g.relu = 1;
+  g.Cb = at<bf16_t>(ws, L.A0);
+  g.ldcb = 256;
+  g.CbT = at<bf16_t>(ws, L.A0T);
+  g.ldct = Pe;
+  g.valid_rows = static_cast<int>(L.Pe);
+  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+  // encoder layer 1: A1 = relu(A0 W1^T + b1)
+  g = gemm_args(at<bf16_t>(ws, L.A0), 256, wb + C.W1, 256, static_cast<int>(Pe), 256, 256);
+  g.bias = prm + P.b1;
+  g.relu = 1;
+  g.Cb = at<bf16_t>(ws, L.A1);
+  g.ldcb = 256;
+  g.CbT = at<bf16_t>(ws, L.A1T);
+  g.ldct = Pe;
+  g.valid_rows = static_cast<int>(L.Pe);
+  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+  // encoder layer 2 (linear): H2 = A1 W2^T + b2 (f32 for the pooling)
+  g = gemm_args(at<bf16_t>(ws, L.A1), 256, wb + C.W2, 256, static_cast<int>(Pe), 256, 256);
+  g.bias = prm + P.b2;
+  g.Cf = at<float>(ws, L.H2f);
+  g.ldc = 256;
+  g.valid_rows = static_cast<int>(L.Pe);
+  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+  // pooling per (problem, side) and the context vector
+  hipLaunchKernelGGL(pool_partial_kernel, dim3(L.chunks, 2 * B), dim3(256), 0, st, at<float>(ws, L.H2f), bt->seg_off,
+                     L.chunks, at<float>(ws, L.part));
+  hipLaunchKernelGGL(pool_final_kernel, dim3(B), dim3(256), 0, st, at<float>(ws, L.part), bt->seg_off, L.chunks,
+                     bt->eps, sig2, at<float>(ws, L.ctx));
+  hipLaunchKernelGGL(ctx_forward_kernel, dim3(B), dim3(256), 0, st, prm + P.Wc, prm + P.bh, at<float>(ws, L.ctx),
+                     at<float>(ws, L.hctx));
+  DS_LAUNCHED();
+  // head layer 1: Z1 = relu(Y0 Wy^T + hctx[b])
+  g = gemm_args(Y0, P.kin, wb + C.Wy, P.kin, static_cast<int>(Ph), 256, P.kin);
+  g.gbias = at<float>(ws, L.hctx);
+  g.row_group = bt->y_prob;
+  g.relu = 1;
+  g.Cb = at<bf16_t>(ws, L.Z1h);
+  g.ldcb = 256;
+  g.CbT = at<bf16_t>(ws, L.Z1hT);
+  g.ldct = Ph;
+  g.valid_rows = static_cast<int>(L.Ph);
+  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+  // head layer 2: A2 = relu(Z1 W4^T + b4)
+  g = gemm_args(at<bf16_t>(ws, L.Z1h), 256, wb + C.W4, 256, static_cast<int>(Ph), 256, 256);
+  g.bias = prm + P.b4;
+  g.relu = 1;
+  g.Cf = at<float>(ws, L.A2f);
+  g.ldc = 256;
+  g.Cb = at<bf16_t>(ws, L.A2h);
+  g.ldcb = 256;
+  g.CbT = at<bf16_t>(ws, L.A2hT);
+  g.ldct = Ph;
+  g.valid_rows = static_cast<int>(L.Ph);
+  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+  hipLaunchKernelGGL(head_out_kernel, dim3(blocks_for(L.Ph)), dim3(256), 0, st, at<float>(ws, L.A2f), prm + P.w5,
+                     prm + P.b5, L.Ph, at<float>(ws, L.out));
+  DS_LAUNCHED();
+  return OTTX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ottx_ds_param_count(int32_t d) {
+  if (d < 1 || d > 256) return 0;
+  return param_layout(d).total;
+}
+
+int64_t ottx_ds_weight_bf16_count(int32_t d) {
+  if (d < 1 || d > 256) return 0;
+  return weight_copies(param_layout(d)).total;
+}
+
+size_t ottx_ds_workspace_bytes(const ottx_ds_batch* bt) {
+  if (!bt || bt->count < 1 || bt->d < 1 || bt->d > 256 || bt->n_total < 1 || bt->m_total < 1) return 0;
+  return ds_layout(bt, param_layout(bt->d)).total;
+}
+
+int ottx_ds_refresh_bf16(const ottx_ds_model* md, void* stream) {
+  if (!md || !md->params || !md->weights_bf16 || md->d < 1 || md->d > 256) return OTTX_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const ParamLayout P = param_layout(md->d);
+  const WeightCopies C = weight_copies(P);
+  bf16_t* wb = reinterpret_cast<bf16_t*>(md->weights_bf16);
+  const float* p = md->params;
+  struct Item {
+    int64_t src, dst, dstT;
+    int rows, cols;
+  } items[] = {{P.W0, C.W0, C.W0T, 256, P.kin}, {P.W1, C.W1, C.W1T, 256, 256}, {P.W2, C.W2, C.W2T, 256, 256},
+               {P.Wy, C.Wy, C.WyT, 256, P.kin}, {P.W4, C.W4, C.W4T, 256, 256}};
+  for (const Item& it : items) {
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(static_cast<int64_t>(it.rows) * it.cols)), dim3(256), 0, st,
+                       p + it.src, it.rows, it.cols, it.cols, wb + it.dst, it.cols, wb + it.dstT, it.rows);
+  }
+  DS_LAUNCHED();
+  return OTTX_OK;
+}
+
+int ottx_ds_forward(const ottx_ds_model* md, const ottx_ds_batch* bt, float* g_out, void* workspace,
+                    size_t workspace_bytes, void* stream) {
+  if (!md || !bt || !g_out || !workspace || md->d != bt->d) return OTTX_EINVAL;
+  const size_t need = ottx_ds_workspace_bytes(bt);
+  if (need == 0) return OTTX_EINVAL;
+  if (workspace_bytes < need || (reinterpret_cast<uintptr_t>(workspace) & 255)) return OTTX_EWORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Forward F;
+  int rc = forward(bt, md, workspace, st, &F);
+  if (rc != OTTX_OK) return rc;
+  hipLaunchKernelGGL(scale_out_kernel, dim3(blocks_for(F.L.Ph)), dim3(256), 0, st, at<float>(F.ws, F.L.out),
+                     bt->y_prob, bt->eps, F.L.Ph, g_out);
+  DS_LAUNCHED();
+  return OTTX_OK;
+}
+
+int ottx_ds_loss_backward(const ottx_ds_model* md, const ottx_ds_batch* bt, const float* target, float* grad,
+                          float* loss_out, void* workspace, size_t workspace_bytes, void* stream) {
+  if (!md || !bt || !target || !grad || !workspace || md->d != bt->d) return OTTX_EINVAL;
+  const size_t need = ottx_ds_workspace_bytes(bt);
+  if (need == 0) return OTTX_EINVAL;
+  if (workspace_bytes < need || (reinterpret_cast<uintptr_t>(workspace) & 255)) return OTTX_EWORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Forward F;
+  int rc = forward(bt, md, workspace, st, &F);
+  if (rc != OTTX_OK) return rc;
+  const ParamLayout& P = F.P;
+  const WeightCopies& C = F.C;
+  const DsLayout& L = F.L;
+  char* ws = F.ws;
+  const float* prm = md->params;
+  const bf16_t* wb = F.wb;
+  const int B = L.nprob;
+  const int64_t Pe = L.Pe_pad, Ph = L.Ph_pad;
+  float* dout = at<float>(ws, L.dout);
+  float* lossb = loss_out ? loss_out : at<float>(ws, L.loss);
+  hipLaunchKernelGGL(loss_kernel, dim3(B), dim3(256), 0, st, at<float>(ws, L.out), target, bt->y_off, B, dout, lossb);
+  DS_LAUNCHED();
+  float* gpart = at<float>(ws, L.gpart);
+  // last layer
+  hipLaunchKernelGGL(head_out_wgrad_kernel, dim3(L.kchunks), dim3(256), 0, st, dout, at<float>(ws, L.A2f), L.Ph,
+                     L.kchunks, gpart);
+  hipLaunchKernelGGL(sum_splits_kernel, dim3(blocks_for(257)), dim3(256), 0, st, gpart, L.kchunks,
+                     static_cast<int64_t>(257), grad + P.w5, 0);
+  DS_LAUNCHED();
+  bf16_t* dZ = at<bf16_t>(ws, L.dZ);
+  bf16_t* dZT = at<bf16_t>(ws, L.dZT);
+  bf16_t* dA = at<bf16_t>(ws, L.dA);
+  bf16_t* dAT = at<bf16_t>(ws, L.dAT);
+  hipLaunchKernelGGL(head_out_backward_kernel, dim3(blocks_for(Ph * 256)), dim3(256), 0, st, dout, prm + P.w5,
+                     at<float>(ws, L.A2f), L.Ph, Ph, dZ, dZT);
+  DS_LAUNCHED();
+  // head layer 2: dW4 = dZ^T Z1, db4 = colsum dZ, dZ1 = (dZ W4) * (Z1 > 0)
+  if ((rc = weight_grad(st, dZT, Ph, at<bf16_t>(ws, L.Z1hT), 256, 256, gpart, L.kchunks, grad + P.W4)) != OTTX_OK)
+    return rc;
+  if ((rc = bias_grad(st, dZ, Ph, 256, gpart, L.kchunks, grad + P.b4)) != OTTX_OK) return rc;
+  GemmArgs g = gemm_args(dZ, 256, wb + C.W4T, 256, static_cast<int>(Ph), 256, 256);
+  g.mask = at<bf16_t>(ws, L.Z1h);
+  g.ldm = 256;
+  g.Cb = dA;
+  g.ldcb = 256;
+  g.CbT = dAT;
+  g.ldct = Ph;
+  g.valid_rows = static_cast<int>(L.Ph);
+  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+  // head layer 1: dWy = dZ1^T Y0, dh[b] = per-problem sums of dZ1, bh grad = sum_b dh
+  if ((rc = weight_grad(st, dAT, Ph, at<bf16_t>(ws, L.Y0) + Ph * P.kin, 256, P.kin, gpart, L.kchunks,
+                        grad + P.Wy)) != OTTX_OK)
+    return rc;
+  float* dh = at<float>(ws, L.dh);
+  hipLaunchKernelGGL(group_sum_kernel, dim3(B), dim3(256), 0, st, dA, bt->y_off, 256, dh);
+  hipLaunchKernelGGL(sum_splits_kernel, dim3(1), dim3(256), 0, st, dh, B, static_cast<int64_t>(256), grad + P.bh, 0);
+  float* dctx = at<float>(ws, L.dctx);
+  hipLaunchKernelGGL(ctx_backward_kernel, dim3(blocks_for(544)), dim3(256), 0, st, prm + P.Wc, at<float>(ws, L.ctx),
+                     dh, B, grad + P.Wc, dctx);
+  DS_LAUNCHED();
+  // pool backward -> dH2 (encoder output gradient)
+  hipLaunchKernelGGL(pool_backward_kernel, dim3(blocks_for(Pe * 256)), dim3(256), 0, st, dctx, bt->seg_of_row,
+                     bt->seg_off, L.Pe, Pe, dZ, dZT);
+  DS_LAUNCHED();
+  // encoder layer 2 (linear): dW2 = dH2^T A1, db2, dA1 = (dH2 W2) * (A1 > 0)
+  if ((rc = weight_grad(st, dZT, Pe, at<bf16_t>(ws, L.A1T), 256, 256, gpart, L.kchunks, grad + P.W2)) != OTTX_OK)
+    return rc;
+  if ((rc = bias_grad(st, dZ, Pe, 256, gpart, L.kchunks, grad + P.b2)) != OTTX_OK) return rc;
+  g = gemm_args(dZ, 256, wb + C.W2T, 256, static_cast<int>(Pe), 256, 256);
+  g.mask = at<bf16_t>(ws, L.A1);
+  g.ldm = 256;
+  g.Cb = dA;
+  g.ldcb = 256;
+  g.CbT = dAT;
+  g.ldct = Pe;
+  g.valid_rows = static_cast<int>(L.Pe);
+  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+  // encoder layer 1: dW1 = dZ1^T A0, db1, dA0 = (dZ1 W1) * (A0 > 0)
+  if ((rc = weight_grad(st, dAT, Pe, at<bf16_t>(ws, L.A0T), 256, 256, gpart, L.kchunks, grad + P.W1)) != OTTX_OK)
+    return rc;
+  if ((rc = bias_grad(st, dA, Pe, 256, gpart, L.kchunks, grad + P.b1)) != OTTX_OK) return rc;
+  g = gemm_args(dA, 256, wb + C.W1T, 256, static_cast<int>(Pe), 256, 256);
+  g.mask = at<bf16_t>(ws, L.A0);
+  g.ldm = 256;
+  g.Cb = dZ;
+  g.ldcb = 256;
+  g.CbT = dZT;
+  g.ldct = Pe;
+  g.valid_rows = static_cast<int>(L.Pe);
+  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+  // encoder layer 0: dW0 = dZ0^T X0, db0
+  if ((rc = weight_grad(st, dZT, Pe, at<bf16_t>(ws, L.X0) + Pe * P.kin, 256, P.kin, gpart, L.kchunks,
+                        grad + P.W0)) != OTTX_OK)
+    return rc;
+  if ((rc = bias_grad(st, dZ, Pe, 256, gpart, L.kchunks, grad + P.b0)) != OTTX_OK) return rc;
+  return OTTX_OK;
+}
+
+int ottx_ds_adam_step(const ottx_ds_model* md, const float* grad, float* m, float* v, int32_t step, float lr,
+                      float beta1, float beta2, float eps, void* stream) {
+  if (!md || !grad || !m || !v || step < 1) return OTTX_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t count = param_layout(md->d).total;
+  const float bc1 = 1.f - powf(beta1, static_cast<float>(step));
+  const float bc2 = 1.f - powf(beta2, static_cast<float>(step));
+  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(count)), dim3(256), 0, st, md->params, grad, m, v, count, lr, beta1,
+                     beta2, eps, bc1, bc2);
+  DS_LAUNCHED();
+  return ottx_ds_refresh_bf16(md, stream);
+}
+
+}  // extern "C"

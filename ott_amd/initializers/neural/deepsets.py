@@ -1,0 +1,262 @@
+"""DeepSets warm start for the Sinkhorn solver.
+
+The network and its training kernels live in ``ott_amd/csrc/ottx_deepsets.hip``
+(C ABI ``ottx_ds_*`` in ``include/ottx.h``); this module holds the parameter
+layout, builds the batch index arrays, and wraps forward, backward and Adam.
+
+Role in the solver: OTT's learned initializer
+(initializers/neural/meta_initializer.py:35-213) predicts f and derives g with
+one ``update_potential``.  With OTT's sequential updates the first g-update
+discards any initial g (geometry.py:297-299, 396-398), so the warm start built
+here predicts g^ from the clouds and returns ``(f0, g^)`` with
+``f0 = update_potential(., g^, log a, axis=1)`` -- the f-update that follows g^
+-- which is what the loop's first g-update then consumes.  The converged solver
+judges the result; zero initialisation remains the default.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import os
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from ott_amd import _lib
+from ott_amd.initializers.linear.initializers import SinkhornInitializer
+
+__all__ = ["ParamLayout", "param_layout", "DeepSets", "Batch", "make_batch", "DeepSetsInitializer"]
+
+WIDTH = 256
+CTX = 544  # [e_x (256), e_y (256), kappa, zero padding]
+
+
+@dataclasses.dataclass(frozen=True)
+class ParamLayout:
+  """Offsets (in floats) of each tensor inside the flat parameter buffer."""
+  d: int
+  kin: int
+  shapes: Tuple[Tuple[str, Tuple[int, ...]], ...]
+  offsets: Tuple[Tuple[str, int], ...]
+  total: int
+
+  def slice(self, name: str) -> Tuple[int, Tuple[int, ...]]:
+    off = dict(self.offsets)[name]
+    shape = dict(self.shapes)[name]
+    return off, shape
+
+
+def param_layout(d: int) -> ParamLayout:
+  """Mirror of ``param_layout`` in ottx_deepsets.hip."""
+  kin = (d + 63) // 64 * 64
+  shapes = (
+      ("W0", (WIDTH, kin)), ("b0", (WIDTH,)), ("W1", (WIDTH, WIDTH)), ("b1", (WIDTH,)),
+      ("W2", (WIDTH, WIDTH)), ("b2", (WIDTH,)), ("Wy", (WIDTH, kin)), ("Wc", (WIDTH, CTX)),
+      ("bh", (WIDTH,)), ("W4", (WIDTH, WIDTH)), ("b4", (WIDTH,)), ("w5", (WIDTH,)), ("b5", (64,)),
+  )
+  offsets, o = [], 0
+  for name, shape in shapes:
+    offsets.append((name, o))
+    o += int(np.prod(shape))
+  return ParamLayout(d=d, kin=kin, shapes=shapes, offsets=tuple(offsets), total=o)
+
+
+@dataclasses.dataclass
+class Batch:
+  """Device-side description of a batch of problems (see ottx_ds_batch)."""
+  count: int
+  d: int
+  n_total: int
+  m_total: int
+  x: torch.Tensor
+  y: torch.Tensor
+  eps: torch.Tensor
+  x_off: torch.Tensor
+  y_off: torch.Tensor
+  y_prob: torch.Tensor
+  enc_src: torch.Tensor
+  seg_of_row: torch.Tensor
+  seg_off: torch.Tensor
+
+  def struct(self) -> _lib.DsBatch:
+    return _lib.DsBatch(
+        count=self.count, d=self.d, n_total=self.n_total, m_total=self.m_total,
+        x=self.x.data_ptr(), y=self.y.data_ptr(), eps=self.eps.data_ptr(),
+        x_off=self.x_off.data_ptr(), y_off=self.y_off.data_ptr(), y_prob=self.y_prob.data_ptr(),
+        enc_src=self.enc_src.data_ptr(), seg_of_row=self.seg_of_row.data_ptr(),
+        seg_off=self.seg_off.data_ptr())
+
+
+def make_batch(problems: Sequence[Tuple[torch.Tensor, torch.Tensor, float]],
+               device: Optional[torch.device] = None) -> Batch:
+  """Concatenate ``(x, y, eps)`` problems and build the index arrays."""
+  device = device or _lib.require_device()
+  xs = [torch.as_tensor(p[0], dtype=torch.float32).to(device) for p in problems]
+  ys = [torch.as_tensor(p[1], dtype=torch.float32).to(device) for p in problems]
+  d = xs[0].shape[1]
+  ns = np.array([x.shape[0] for x in xs], dtype=np.int64)
+  ms = np.array([y.shape[0] for y in ys], dtype=np.int64)
+  B = len(problems)
+  x_off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int32)
+  y_off = np.concatenate([[0], np.cumsum(ms)]).astype(np.int32)
+  m_total, n_total = int(ms.sum()), int(ns.sum())
+  m_pad = (m_total + 63) // 64 * 64
+  y_prob = np.zeros(m_pad, dtype=np.int32)
+  y_prob[:m_total] = np.repeat(np.arange(B, dtype=np.int32), ms)
+  seg_sizes = np.stack([ns, ms], axis=1).reshape(-1)
+  seg_off = np.concatenate([[0], np.cumsum(seg_sizes)]).astype(np.int32)
+  seg_of_row = np.repeat(np.arange(2 * B, dtype=np.int32), seg_sizes)
+  enc_src = np.empty(n_total + m_total, dtype=np.int32)
+  for b in range(B):
+    s0 = seg_off[2 * b]
+    enc_src[s0:s0 + ns[b]] = np.arange(x_off[b], x_off[b + 1])
+    s1 = seg_off[2 * b + 1]
+    enc_src[s1:s1 + ms[b]] = np.arange(y_off[b], y_off[b + 1])
+  as_dev = lambda a: torch.from_numpy(a).to(device)  # noqa: E731
+  return Batch(
+      count=B, d=d, n_total=n_total, m_total=m_total,
+      x=torch.cat(xs).contiguous(), y=torch.cat(ys).contiguous(),
+      eps=torch.tensor([float(p[2]) for p in problems], dtype=torch.float32, device=device),
+      x_off=as_dev(x_off), y_off=as_dev(y_off), y_prob=as_dev(y_prob), enc_src=as_dev(enc_src),
+      seg_of_row=as_dev(seg_of_row), seg_off=as_dev(seg_off))
+
+
+class DeepSets:
+  """Parameters (float32 master + bf16 copies) and Adam state on the GPU."""
+
+  def __init__(self, d: int, seed: int = 0, device: Optional[torch.device] = None):
+    self.device = device or _lib.require_device()
+    self.d = d
+    self.layout = param_layout(d)
+    lib = _lib.load_library()
+    assert lib.ottx_ds_param_count(d) == self.layout.total, "parameter layout mismatch"
+    self.params = torch.zeros(self.layout.total, dtype=torch.float32, device=self.device)
+    self.weights_bf16 = torch.zeros(int(lib.ottx_ds_weight_bf16_count(d)), dtype=torch.int16,
+                                    device=self.device)
+    self.adam_m = torch.zeros_like(self.params)
+    self.adam_v = torch.zeros_like(self.params)
+    self.step = 0
+    self.init_params(seed)
+
+  # -- parameters ---------------------------------------------------------
+  def tensor(self, name: str) -> torch.Tensor:
+    off, shape = self.layout.slice(name)
+    return self.params[off:off + int(np.prod(shape))].view(*shape)
+
+  def init_params(self, seed: int) -> None:
+    """He-uniform weights for the ReLU layers; a zero output layer."""
+    gen = torch.Generator().manual_seed(seed)
+    host = torch.zeros(self.layout.total, dtype=torch.float32)
+    fan_in = {"W0": self.d, "W1": WIDTH, "W2": WIDTH, "Wy": self.d + 2 * WIDTH + 1,
+              "Wc": self.d + 2 * WIDTH + 1, "W4": WIDTH}
+    for name, shape in self.layout.shapes:
+      off = dict(self.layout.offsets)[name]
+      n = int(np.prod(shape))
+      if name in fan_in:
+        bound = math.sqrt(6.0 / fan_in[name])
+        w = (torch.rand(shape, generator=gen) * 2 - 1) * bound
+        if name in ("W0", "Wy"):
+          w[:, self.d:] = 0.0
+        if name == "Wc":
+          w[:, 2 * WIDTH + 1:] = 0.0
+        host[off:off + n] = w.reshape(-1)
+    self.params.copy_(host.to(self.device))
+    self.refresh()
+
+  def refresh(self) -> None:
+    lib = _lib.load_library()
+    _lib.check("ottx_ds_refresh_bf16", lib.ottx_ds_refresh_bf16(ctypes.byref(self.struct()),
+                                                                 _lib.stream_ptr()))
+
+  def struct(self) -> _lib.DsModel:
+    return _lib.DsModel(d=self.d, params=self.params.data_ptr(),
+                        weights_bf16=self.weights_bf16.data_ptr())
+
+  def save(self, path: str, **meta) -> None:
+    np.savez_compressed(path, d=np.array(self.d), params=self.params.cpu().numpy(),
+                        **{k: np.asarray(v) for k, v in meta.items()})
+
+  @classmethod
+  def load(cls, path: str, device: Optional[torch.device] = None) -> "DeepSets":
+    with np.load(path, allow_pickle=False) as z:
+      model = cls(int(z["d"]), device=device)
+      model.params.copy_(torch.from_numpy(z["params"]).to(model.device))
+    model.refresh()
+    return model
+
+  # -- compute ------------------------------------------------------------
+  def _workspace(self, batch: Batch) -> Tuple[torch.Tensor, _lib.DsBatch]:
+    lib = _lib.load_library()
+    bs = batch.struct()
+    nbytes = lib.ottx_ds_workspace_bytes(ctypes.byref(bs))
+    if nbytes == 0:
+      raise ValueError("invalid DeepSets batch")
+    return _lib.workspace(nbytes, self.device), bs
+
+  def forward(self, batch: Batch) -> torch.Tensor:
+    """Predicted potentials g (per head row, batch order), up to a constant per problem."""
+    lib = _lib.load_library()
+    ws, bs = self._workspace(batch)
+    out = torch.empty(batch.m_total, dtype=torch.float32, device=self.device)
+    _lib.check("ottx_ds_forward", lib.ottx_ds_forward(
+        ctypes.byref(self.struct()), ctypes.byref(bs), out.data_ptr(), ws.data_ptr(), ws.numel(),
+        _lib.stream_ptr()))
+    return out
+
+  def loss_backward(self, batch: Batch, target: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Per-problem centred MSE (of g / eps) and the gradient of its mean."""
+    lib = _lib.load_library()
+    ws, bs = self._workspace(batch)
+    grad = torch.zeros_like(self.params)
+    loss = torch.empty(batch.count, dtype=torch.float32, device=self.device)
+    target = target.to(device=self.device, dtype=torch.float32).contiguous()
+    _lib.check("ottx_ds_loss_backward", lib.ottx_ds_loss_backward(
+        ctypes.byref(self.struct()), ctypes.byref(bs), target.data_ptr(), grad.data_ptr(),
+        loss.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_ptr()))
+    return loss, grad
+
+  def adam_step(self, grad: torch.Tensor, lr: float = 1e-3, beta1: float = 0.9,
+                beta2: float = 0.999, eps: float = 1e-8) -> None:
+    """optax.adam(lr) update (meta_initializer.py:77-78 uses Adam at 1e-3)."""
+    lib = _lib.load_library()
+    self.step += 1
+    _lib.check("ottx_ds_adam_step", lib.ottx_ds_adam_step(
+        ctypes.byref(self.struct()), grad.data_ptr(), self.adam_m.data_ptr(),
+        self.adam_v.data_ptr(), self.step, lr, beta1, beta2, eps, _lib.stream_ptr()))
+
+
+class DeepSetsInitializer(SinkhornInitializer):
+  """Warm start ``(f0, g^)`` from a trained :class:`DeepSets` model.
+
+  Follows SinkhornInitializer.__call__ (initializers.py:70-99): potentials of
+  zero-weight points are set to -inf.
+  """
+
+  def __init__(self, model: DeepSets):
+    self.model = model
+
+  def __call__(self, ot_prob, lse_mode: bool = True, **kwargs):
+    if not lse_mode:
+      raise NotImplementedError("the learned warm start is for lse_mode=True")
+    geom = ot_prob.geom
+    eps = geom.epsilon
+    batch = make_batch([(geom.x, geom.y, eps)], device=geom.device)
+    g_hat = self.model.forward(batch)
+    b = ot_prob.b
+    g_hat = torch.where(b > 0.0, g_hat, torch.full_like(g_hat, -math.inf))
+    f0 = geom.update_potential(torch.zeros_like(ot_prob.a), g_hat, torch.log(ot_prob.a), axis=1)
+    f0 = ot_prob.tau_a * f0 if ot_prob.tau_a != 1.0 else f0
+    a = ot_prob.a
+    f0 = torch.where(a > 0.0, f0, torch.full_like(f0, -math.inf))
+    return f0, g_hat
+
+
+WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
+
+
+def shipped_model_path(d: int) -> Optional[str]:
+  path = os.path.join(WEIGHTS_DIR, f"deepsets_d{d}.npz")
+  return path if os.path.exists(path) else None

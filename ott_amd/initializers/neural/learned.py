@@ -1,14 +1,24 @@
 """Selection of the learned warm start used by the benchmark.
 
-Returns the initializer whose trained weights ship with the package for a
-given feature dimension, or ``(None, "zeros")`` when none is present, in which
-case the solver starts from zero potentials (OTT's DefaultInitializer).
+Returns an initializer backed by the DeepSets weights shipped for a feature
+dimension, or ``(None, "zeros")`` when none is present, in which case the
+solver starts from zero potentials (OTT's DefaultInitializer).
 """
 
 from __future__ import annotations
 
 from typing import Callable, Optional, Tuple
 
+from ott_amd.initializers.neural import deepsets
+
+_cache = {}
+
 
 def bench_initializer(d: int) -> Tuple[Optional[Callable], str]:
-  return None, "zeros"
+  path = deepsets.shipped_model_path(d)
+  if path is None:
+    return None, "zeros"
+  if d not in _cache:
+    _cache[d] = deepsets.DeepSetsInitializer(deepsets.DeepSets.load(path))
+  init = _cache[d]
+  return (lambda prob: init(prob)), f"deepsets ({path.rsplit('/', 1)[-1]})"

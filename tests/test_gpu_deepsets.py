@@ -1,0 +1,163 @@
+"""DeepSets warm-start kernels against a float64 torch-CPU autograd reference.
+
+The HIP forward rounds inputs, weight matrices and hidden activations to
+bfloat16 (float32 accumulation); the reference applies the same roundings in
+float64, so forward values agree to float32-accumulation level.  Gradients are
+compared with float64 autograd of that same function: the kernels additionally
+round back-propagated activations to bfloat16, so the bound is a relative L2
+error of 3e-2 per tensor (and cosine > 0.999).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(t):
+  return t.to(torch.bfloat16).to(torch.float64)
+
+
+def _reference(params, layout, problems, target, bf16=True):
+  """Float64 forward + loss of the network; returns (g list, loss, named params)."""
+  P = {}
+  for name, shape in layout.shapes:
+    off = dict(layout.offsets)[name]
+    n = int(np.prod(shape))
+    P[name] = torch.tensor(params[off:off + n], dtype=torch.float64).view(*shape).requires_grad_(True)
+  rnd = _bf if bf16 else (lambda t: t)
+  W = {k: (rnd(P[k].detach()) + (P[k] - P[k].detach())) for k in ("W0", "W1", "W2", "Wy", "W4")}
+  outs, losses = [], []
+  q = 0
+  for (x, y, eps) in problems:
+    x = torch.tensor(x, dtype=torch.float64)
+    y = torch.tensor(y, dtype=torch.float64)
+    d = x.shape[1]
+    mx, my = x.mean(0), y.mean(0)
+    mu = 0.5 * (mx + my)
+    meanc = (x * x).sum(1).mean() + (y * y).sum(1).mean() - 2 * (mx @ my)
+    sig2 = 0.5 * meanc
+    def norm(p):
+      z = torch.zeros(p.shape[0], layout.kin, dtype=torch.float64)
+      z[:, :d] = (p - mu) / torch.sqrt(sig2)
+      return rnd(z)
+    xt, yt = norm(x), norm(y)
+    def enc(p):
+      a0 = torch.relu(p @ W["W0"].T + P["b0"])
+      a0 = rnd(a0.detach()) + (a0 - a0.detach())
+      a1 = torch.relu(a0 @ W["W1"].T + P["b1"])
+      a1 = rnd(a1.detach()) + (a1 - a1.detach())
+      return a1 @ W["W2"].T + P["b2"]
+    ex, ey = enc(xt).mean(0), enc(yt).mean(0)
+    ctx = torch.cat([ex, ey, (eps / sig2).reshape(1), torch.zeros(31, dtype=torch.float64)])
+    hctx = P["Wc"] @ ctx + P["bh"]
+    z1 = torch.relu(yt @ W["Wy"].T + hctx)
+    z1 = rnd(z1.detach()) + (z1 - z1.detach())
+    a2 = torch.relu(z1 @ W["W4"].T + P["b4"])
+    out = a2 @ P["w5"] + P["b5"][0]
+    m = y.shape[0]
+    r = out - torch.tensor(target[q:q + m], dtype=torch.float64)
+    losses.append(((r - r.mean()) ** 2).mean())
+    outs.append(eps * out)
+    q += m
+  loss = torch.stack(losses).mean()
+  return outs, loss, losses, P
+
+
+def _problems(d, seed):
+  rng = np.random.default_rng(seed)
+  shapes = [(50, 70), (130, 40), (64, 64)]
+  out = []
+  for n, m in shapes:
+    x = rng.normal(size=(n, d)).astype(np.float32) / np.sqrt(d)
+    y = (rng.normal(size=(m, d)) / np.sqrt(d) + 0.2).astype(np.float32)
+    out.append((x, y, float(rng.uniform(0.03, 0.2))))
+  return out
+
+
+@pytest.mark.parametrize("d", [5, 40])
+def test_forward_and_gradients_match_float64_autograd(d):
+  from ott_amd.initializers.neural import deepsets
+  model = deepsets.DeepSets(d, seed=3)
+  # give the output layer non-zero weights so every gradient path is exercised
+  rng = np.random.default_rng(4)
+  with torch.no_grad():
+    model.tensor("w5").copy_(torch.from_numpy(rng.normal(size=256).astype(np.float32) * 0.05))
+    model.tensor("b5")[0] = 0.3
+  model.refresh()
+  probs = _problems(d, seed=d)
+  batch = deepsets.make_batch([(torch.from_numpy(x), torch.from_numpy(y), e) for x, y, e in probs])
+  m_total = sum(y.shape[0] for _, y, _ in probs)
+  target = rng.normal(size=m_total).astype(np.float32)
+
+  g = model.forward(batch).cpu().numpy().astype(np.float64)
+  params = model.params.cpu().numpy()
+  outs, loss_ref, losses_ref, P = _reference(params, model.layout, probs, target)
+  g_ref = torch.cat(outs).detach().numpy()
+  scale = np.max(np.abs(g_ref))
+  assert np.max(np.abs(g - g_ref)) < 2e-3 * scale, np.max(np.abs(g - g_ref)) / scale
+
+  loss, grad = model.loss_backward(batch, torch.from_numpy(target))
+  np.testing.assert_allclose(loss.cpu().numpy(), [float(l.detach()) for l in losses_ref], rtol=5e-3)
+  loss_ref.backward()
+  grad = grad.cpu().numpy().astype(np.float64)
+  for name, shape in model.layout.shapes:
+    off = dict(model.layout.offsets)[name]
+    n = int(np.prod(shape))
+    got = grad[off:off + n]
+    want = P[name].grad.detach().numpy().reshape(-1)
+    if name == "b5":
+      # the centred loss is invariant to a constant output shift: zero gradient
+      assert abs(got[0]) < 1e-6 * np.linalg.norm(grad) and np.all(got[1:] == 0.0), got[:2]
+      continue
+    nrm = np.linalg.norm(want)
+    if nrm == 0.0:
+      assert np.linalg.norm(got) < 1e-6, name
+      continue
+    rel = np.linalg.norm(got - want) / nrm
+    cos = got @ want / (np.linalg.norm(got) * nrm)
+    assert rel < 3e-2 and cos > 0.999, (name, rel, cos)
+
+
+def test_adam_step_and_save_load(tmp_path):
+  from ott_amd.initializers.neural import deepsets
+  d = 8
+  model = deepsets.DeepSets(d, seed=1)
+  probs = _problems(d, seed=11)
+  batch = deepsets.make_batch([(torch.from_numpy(x), torch.from_numpy(y), e) for x, y, e in probs])
+  m_total = sum(y.shape[0] for _, y, _ in probs)
+  target = torch.from_numpy(np.random.default_rng(2).normal(size=m_total).astype(np.float32))
+  p0 = model.params.clone()
+  _, grad = model.loss_backward(batch, target)
+  model.adam_step(grad, lr=1e-3)
+  # first Adam step: p1 = p0 - lr * g / (|g| + eps') with bias correction
+  g = grad.double()
+  m = 0.1 * g / (1 - 0.9)
+  v = 0.001 * g * g / (1 - 0.999)
+  want = p0.double() - 1e-3 * m / (torch.sqrt(v) + 1e-8)
+  torch.testing.assert_close(model.params.double(), want, rtol=1e-5, atol=1e-7)
+  path = str(tmp_path / "m.npz")
+  model.save(path)
+  other = deepsets.DeepSets.load(path)
+  torch.testing.assert_close(other.forward(batch), model.forward(batch))
+
+
+def test_training_reduces_loss():
+  """A few Adam steps on a fixed batch reduce the centred MSE."""
+  from ott_amd.initializers.neural import deepsets
+  d = 4
+  model = deepsets.DeepSets(d, seed=0)
+  probs = _problems(d, seed=5)
+  batch = deepsets.make_batch([(torch.from_numpy(x), torch.from_numpy(y), e) for x, y, e in probs])
+  ys = np.concatenate([y for _, y, _ in probs])
+  target = torch.from_numpy((ys @ np.arange(1, d + 1) + (ys ** 2).sum(1)).astype(np.float32))
+  first = None
+  for _ in range(60):
+    loss, grad = model.loss_backward(batch, target)
+    first = float(loss.mean()) if first is None else first
+    model.adam_step(grad, lr=3e-3)
+  last = float(model.loss_backward(batch, target)[0].mean())
+  assert last < 0.2 * first, (first, last)
