@@ -82,6 +82,9 @@ typedef struct ottx_sinkhorn_opts {
   double momentum;          /* constant Momentum weight (acceleration.py:161-172), 1 = none */
   int32_t tile_config;      /* -1 = default, else a value listed by ottx_tile_configs */
   int32_t use_graph;        /* 1 = replay each block of iterations as a hipGraph */
+  int32_t init_from_g;      /* 1 = f starts as the f-update of the initial g (a warm start that
+                               predicts g only); 0 = f and g are used as given */
+  int32_t _pad;
 } ottx_sinkhorn_opts;
 
 /* Outputs of one solve (SinkhornOutput fields, sinkhorn.py:280-323). */
@@ -202,19 +205,22 @@ typedef struct ottx_ds_batch {
   const int32_t* enc_src;  /* [n_total + m_total] row in x (side 0) or y (side 1) of encoder row r */
   const int32_t* seg_of_row; /* [n_total + m_total] segment 2 b + side of encoder row r */
   const int32_t* seg_off;  /* [2 B + 1] encoder row offsets of the segments */
+  const float* eps_rel;    /* [B] epsilon / mean cost of each problem, or NULL (then eps / sigma^2 / 2) */
 } ottx_ds_batch;
 
 int64_t ottx_ds_param_count(int32_t d);
 int64_t ottx_ds_weight_bf16_count(int32_t d);
-size_t ottx_ds_workspace_bytes(const ottx_ds_batch* batch);
+/* Workspace of ottx_ds_forward (training = 0) or ottx_ds_loss_backward (training = 1). */
+size_t ottx_ds_workspace_bytes(const ottx_ds_batch* batch, int32_t training);
 /* Recompute the bf16 weight copies from the float32 parameters. */
 int ottx_ds_refresh_bf16(const ottx_ds_model* model, void* stream);
-/* g_out[m_total]: predicted potentials g_j = eps_b * psi(...) (up to a constant). */
+/* g_out[m_total]: predicted potentials g_j = sigma_b^2 * psi(...) (up to a constant), with
+   sigma_b^2 half the mean squared-Euclidean cost of problem b. */
 int ottx_ds_forward(const ottx_ds_model* model, const ottx_ds_batch* batch,
                     float* g_out, void* workspace, size_t workspace_bytes,
                     void* stream);
 /* Centred mean-squared loss against target[m_total] (potentials divided by
- * eps, any per-problem constant), its gradient into grad[param_count] and the
+ * sigma_b^2, any per-problem constant), its gradient into grad[param_count] and the
  * per-problem losses into loss_out[B] (nullable). */
 int ottx_ds_loss_backward(const ottx_ds_model* model,
                           const ottx_ds_batch* batch, const float* target,

@@ -61,6 +61,8 @@ class SinkhornOpts(ctypes.Structure):
       ("momentum", ctypes.c_double),
       ("tile_config", ctypes.c_int32),
       ("use_graph", ctypes.c_int32),
+      ("init_from_g", ctypes.c_int32),
+      ("_pad", ctypes.c_int32),
   ]
 
 
@@ -103,6 +105,7 @@ class DsBatch(ctypes.Structure):
       ("enc_src", ctypes.c_void_p),
       ("seg_of_row", ctypes.c_void_p),
       ("seg_off", ctypes.c_void_p),
+      ("eps_rel", ctypes.c_void_p),
   ]
 
 
@@ -135,7 +138,7 @@ _SIGNATURES = [
       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     ("ottx_ds_param_count", ctypes.c_int64, [ctypes.c_int32]),
     ("ottx_ds_weight_bf16_count", ctypes.c_int64, [ctypes.c_int32]),
-    ("ottx_ds_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(DsBatch)]),
+    ("ottx_ds_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(DsBatch), ctypes.c_int32]),
     ("ottx_ds_refresh_bf16", ctypes.c_int, [ctypes.POINTER(DsModel), ctypes.c_void_p]),
     ("ottx_ds_forward", ctypes.c_int,
      [ctypes.POINTER(DsModel), ctypes.POINTER(DsBatch), ctypes.c_void_p, ctypes.c_void_p,

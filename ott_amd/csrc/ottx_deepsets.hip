@@ -9,7 +9,7 @@
 //   head     g~_j = psi(y~_j, e_x, e_y, kappa), 3 layers (ReLU, ReLU, linear)
 // with x~ = (x - mu) / sigma, y~ = (y - mu) / sigma per problem
 // (mu = (mean x + mean y) / 2, sigma^2 = mean(C) / 2) and kappa = eps / sigma^2.
-// The prediction is g_j = eps * g~_j (a potential, up to a constant).
+// The prediction is g_j = sigma^2 * g~_j (a potential, up to a constant).
 //
 // Every matrix product runs through one bf16 MFMA GEMM, C = A . Bt^T with both
 // operands K-contiguous ("NT"), f32 accumulation, and an epilogue that adds a
@@ -191,45 +191,47 @@ __global__ void stats_kernel(const float* __restrict__ x, const float* __restric
   }
 }
 
-// Normalised bf16 inputs of the encoder: row r (rows ordered x_0, y_0, x_1,
-// y_1, ... by segment 2 b + side) comes from x or y row src[r]; written to
-// X [rows_pad, kpad] and transposed XT [kpad, rows_pad], zero-padded.
-__global__ void enc_normalize_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                     const int* __restrict__ src, const int* __restrict__ seg_of_row, int64_t rows,
-                                     int64_t rows_pad, int d, int kpad, const float* __restrict__ mu,
-                                     const float* __restrict__ sig2, bf16_t* __restrict__ X, bf16_t* __restrict__ XT) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= rows_pad * kpad) return;
-  const int64_t r = i / kpad;
-  const int k = static_cast<int>(i % kpad);
-  float v = 0.f;
-  if (r < rows && k < d) {
-    const int seg = seg_of_row[r];
-    const int b = seg >> 1;
-    const float* pts = (seg & 1) ? y : x;
-    v = (pts[static_cast<int64_t>(src[r]) * d + k] - mu[b * d + k]) * rsqrtf(sig2[b]);
+// Normalised bf16 inputs.  Row r holds p~ = (p - mu_b) / sigma_b in columns
+// [0, d), the feature |p~|^2 - 1 in column d and zeros up to kpad; rows past
+// the valid count are zero.  XT (transposed, [kpad, rows_pad]) is written
+// when non-null (training needs it for the first-layer weight gradient).
+// Encoder rows are ordered x_0, y_0, x_1, y_1, ... (segment 2 b + side) and
+// come from x or y row src[r]; head rows are the y points (src null, prob[r]).
+__global__ void normalize_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                 const int* __restrict__ src, const int* __restrict__ seg_of_row,
+                                 const int* __restrict__ prob, int64_t rows, int64_t rows_pad, int d, int kpad,
+                                 const float* __restrict__ mu, const float* __restrict__ sig2, bf16_t* __restrict__ X,
+                                 bf16_t* __restrict__ XT) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= rows_pad) return;
+  const float* p = nullptr;
+  int b = 0;
+  if (r < rows) {
+    if (src) {
+      const int seg = seg_of_row[r];
+      b = seg >> 1;
+      p = ((seg & 1) ? y : x) + static_cast<int64_t>(src[r]) * d;
+    } else {
+      b = prob[r];
+      p = y + r * d;
+    }
   }
-  const bf16_t h = to_bf16(v);
-  X[i] = h;
-  XT[static_cast<int64_t>(k) * rows_pad + r] = h;
-}
-
-// Normalised bf16 inputs of the head: the y points in their batch order.
-__global__ void head_normalize_kernel(const float* __restrict__ y, const int* __restrict__ prob, int64_t rows,
-                                      int64_t rows_pad, int d, int kpad, const float* __restrict__ mu,
-                                      const float* __restrict__ sig2, bf16_t* __restrict__ Y, bf16_t* __restrict__ YT) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= rows_pad * kpad) return;
-  const int64_t r = i / kpad;
-  const int k = static_cast<int>(i % kpad);
-  float v = 0.f;
-  if (r < rows && k < d) {
-    const int b = prob[r];
-    v = (y[r * d + k] - mu[b * d + k]) * rsqrtf(sig2[b]);
+  const float inv = p ? rsqrtf(sig2[b]) : 0.f;
+  float nrm = 0.f;
+  for (int k = 0; k < kpad; ++k) {
+    float v = 0.f;
+    if (p) {
+      if (k < d) {
+        v = (p[k] - mu[b * d + k]) * inv;
+        nrm = fmaf(v, v, nrm);
+      } else if (k == d) {
+        v = nrm - 1.f;
+      }
+    }
+    const bf16_t h = to_bf16(v);
+    X[r * kpad + k] = h;
+    if (XT) XT[static_cast<int64_t>(k) * rows_pad + r] = h;
   }
-  const bf16_t h = to_bf16(v);
-  Y[i] = h;
-  YT[static_cast<int64_t>(k) * rows_pad + r] = h;
 }
 
 // Mean pooling of the encoder output H [P, 256] per (problem, side), with
@@ -247,8 +249,8 @@ __global__ void pool_partial_kernel(const float* __restrict__ H, const int* __re
 
 // ctx[b] = [e_x (256), e_y (256), kappa = eps / sigma^2, 0 pad ...] (ld 544).
 __global__ void pool_final_kernel(const float* __restrict__ part, const int* __restrict__ seg_off, int chunks,
-                                  const float* __restrict__ eps, const float* __restrict__ sig2,
-                                  float* __restrict__ ctx) {
+                                  const float* __restrict__ eps, const float* __restrict__ eps_rel,
+                                  const float* __restrict__ sig2, float* __restrict__ ctx) {
   const int b = blockIdx.x, f = threadIdx.x;
   for (int side = 0; side < 2; ++side) {
     const int seg = b * 2 + side;
@@ -256,7 +258,9 @@ __global__ void pool_final_kernel(const float* __restrict__ part, const int* __r
     for (int c = 0; c < chunks; ++c) s += part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f];
     ctx[b * 544 + side * 256 + f] = s / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
   }
-  if (f < 32) ctx[b * 544 + 512 + f] = (f == 0) ? eps[b] / sig2[b] : 0.f;
+  // kappa = eps / sigma^2; with eps = c mean(C) and sigma^2 = mean(C) / 2 it is 2 c.
+  const float kappa = (eps_rel && eps_rel[b] > 0.f) ? 2.f * eps_rel[b] : eps[b] / sig2[b];
+  if (f < 32) ctx[b * 544 + 512 + f] = (f == 0) ? kappa : 0.f;
 }
 
 // hctx[b, o] = sum_k Wc[o, k] ctx[b, k] + bh[o]   (Wc [256, 544] f32).
@@ -431,11 +435,11 @@ __global__ void to_bf16_kernel(const float* __restrict__ src, int rows, int cols
   if (dstT) dstT[static_cast<int64_t>(c) * ld_dstT + r] = h;
 }
 
-// The head's prediction as a potential: g[q] = eps[b(q)] * out[q].
+// The head's prediction as a potential: g[q] = sigma_b^2 * out[q].
 __global__ void scale_out_kernel(const float* __restrict__ out, const int* __restrict__ prob,
-                                 const float* __restrict__ eps, int64_t rows, float* __restrict__ g) {
+                                 const float* __restrict__ sig2, int64_t rows, float* __restrict__ g) {
   const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q < rows) g[q] = eps[prob[q]] * out[q];
+  if (q < rows) g[q] = sig2[prob[q]] * out[q];
 }
 
 }  // namespace ottx_ds
@@ -461,7 +465,7 @@ struct ParamLayout {
 
 ParamLayout param_layout(int d) {
   ParamLayout L;
-  L.kin = (d + 63) / 64 * 64;
+  L.kin = (d + 1 + 63) / 64 * 64;  // d coordinates + the squared-norm feature
   int64_t o = 0;
   L.W0 = o; o += 256LL * L.kin;
   L.b0 = o; o += 256;
@@ -510,7 +514,7 @@ struct DsLayout {
       dZ, dZT, dA, dAT, dctx, dh, gpart, total;
 };
 
-DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P) {
+DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training) {
   DsLayout L{};
   L.nprob = bt->count;
   L.Pe = bt->n_total + bt->m_total;
@@ -518,44 +522,45 @@ DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P) {
   L.Ph = bt->m_total;
   L.Ph_pad = pad64(L.Ph);
   L.chunks = 16;
-  L.kchunks = 64;
+  L.kchunks = training ? 64 : 0;
   size_t o = 0;
   auto take = [&](size_t b) {
     const size_t at = o;
     o = al(o + b);
     return at;
   };
+  auto take_t = [&](size_t b) { return take(training ? b : 0); };  // training-only buffers
   const int64_t Pe = L.Pe_pad, Ph = L.Ph_pad;
   const int B = L.nprob;
   L.mu = take(static_cast<size_t>(B) * bt->d * 4);
   L.sig2 = take(static_cast<size_t>(B) * 4);
   L.kappa = take(static_cast<size_t>(B) * 4);
-  L.X0 = take(static_cast<size_t>(Pe) * P.kin * 2 * 2);  // X0 and X0T
-  L.Y0 = take(static_cast<size_t>(Ph) * P.kin * 2 * 2);  // Y0 and Y0T
+  L.X0 = take(static_cast<size_t>(Pe) * P.kin * 2 * (training ? 2 : 1));  // X0 (and X0T)
+  L.Y0 = take(static_cast<size_t>(Ph) * P.kin * 2 * (training ? 2 : 1));  // Y0 (and Y0T)
   L.A0 = take(static_cast<size_t>(Pe) * 256 * 2);
-  L.A0T = take(static_cast<size_t>(Pe) * 256 * 2);
+  L.A0T = take_t(static_cast<size_t>(Pe) * 256 * 2);
   L.A1 = take(static_cast<size_t>(Pe) * 256 * 2);
-  L.A1T = take(static_cast<size_t>(Pe) * 256 * 2);
+  L.A1T = take_t(static_cast<size_t>(Pe) * 256 * 2);
   L.H2f = take(static_cast<size_t>(Pe) * 256 * 4);
   L.part = take(static_cast<size_t>(2 * B) * L.chunks * 256 * 4);
   L.ctx = take(static_cast<size_t>(B) * 544 * 4);
   L.hctx = take(static_cast<size_t>(B) * 256 * 4);
   L.Z1h = take(static_cast<size_t>(Ph) * 256 * 2);
-  L.Z1hT = take(static_cast<size_t>(Ph) * 256 * 2);
+  L.Z1hT = take_t(static_cast<size_t>(Ph) * 256 * 2);
   L.A2h = take(static_cast<size_t>(Ph) * 256 * 2);
-  L.A2hT = take(static_cast<size_t>(Ph) * 256 * 2);
+  L.A2hT = take_t(static_cast<size_t>(Ph) * 256 * 2);
   L.A2f = take(static_cast<size_t>(Ph) * 256 * 4);
   L.out = take(static_cast<size_t>(Ph) * 4);
-  L.dout = take(static_cast<size_t>(Ph) * 4);
+  L.dout = take_t(static_cast<size_t>(Ph) * 4);
   L.loss = take(static_cast<size_t>(B) * 4);
   const int64_t Pm = Pe > Ph ? Pe : Ph;
-  L.dZ = take(static_cast<size_t>(Pm) * 256 * 2);
-  L.dZT = take(static_cast<size_t>(Pm) * 256 * 2);
-  L.dA = take(static_cast<size_t>(Pm) * 256 * 2);
-  L.dAT = take(static_cast<size_t>(Pm) * 256 * 2);
-  L.dctx = take(static_cast<size_t>(B) * 544 * 4);
-  L.dh = take(static_cast<size_t>(B) * 256 * 4);
-  L.gpart = take(static_cast<size_t>(L.kchunks) * 256 * 544 * 4);
+  L.dZ = take_t(static_cast<size_t>(Pm) * 256 * 2);
+  L.dZT = take_t(static_cast<size_t>(Pm) * 256 * 2);
+  L.dA = take_t(static_cast<size_t>(Pm) * 256 * 2);
+  L.dAT = take_t(static_cast<size_t>(Pm) * 256 * 2);
+  L.dctx = take_t(static_cast<size_t>(B) * 544 * 4);
+  L.dh = take_t(static_cast<size_t>(B) * 256 * 4);
+  L.gpart = take_t(static_cast<size_t>(L.kchunks) * 256 * 544 * 4);
   L.total = o;
   return L;
 }
@@ -623,10 +628,11 @@ T* at(char* base, size_t off) {
 }
 
 // The forward pass up to the head output (out[Ph]).
-int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, hipStream_t st, Forward* F) {
+int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, hipStream_t st, bool training,
+            Forward* F) {
   F->P = param_layout(bt->d);
   F->C = weight_copies(F->P);
-  F->L = ds_layout(bt, F->P);
+  F->L = ds_layout(bt, F->P, training);
   const ParamLayout& P = F->P;
   const WeightCopies& C = F->C;
   const DsLayout& L = F->L;
@@ -646,10 +652,12 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   bf16_t* X0T = X0 + Pe * P.kin;
   bf16_t* Y0 = at<bf16_t>(ws, L.Y0);
   bf16_t* Y0T = Y0 + Ph * P.kin;
-  hipLaunchKernelGGL(enc_normalize_kernel, dim3(blocks_for(Pe * P.kin)), dim3(256), 0, st, bt->x, bt->y, bt->enc_src,
-                     bt->seg_of_row, L.Pe, Pe, bt->d, P.kin, mu, sig2, X0, X0T);
-  hipLaunchKernelGGL(head_normalize_kernel, dim3(blocks_for(Ph * P.kin)), dim3(256), 0, st, bt->y, bt->y_prob, L.Ph,
-                     Ph, bt->d, P.kin, mu, sig2, Y0, Y0T);
+  hipLaunchKernelGGL(normalize_kernel, dim3(blocks_for(Pe)), dim3(256), 0, st, bt->x, bt->y, bt->enc_src,
+                     bt->seg_of_row, static_cast<const int*>(nullptr), L.Pe, Pe, bt->d, P.kin, mu, sig2, X0,
+                     training ? X0T : nullptr);
+  hipLaunchKernelGGL(normalize_kernel, dim3(blocks_for(Ph)), dim3(256), 0, st, bt->x, bt->y,
+                     static_cast<const int*>(nullptr), static_cast<const int*>(nullptr), bt->y_prob, L.Ph, Ph, bt->d,
+                     P.kin, mu, sig2, Y0, training ? Y0T : nullptr);
   DS_LAUNCHED();
   int rc;
   // encoder layer 0: A0 = relu(X0 W0^T + b0)
@@ -658,7 +666,7 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   g.relu = 1;
   g.Cb = at<bf16_t>(ws, L.A0);
   g.ldcb = 256;
-  g.CbT = at<bf16_t>(ws, L.A0T);
+  g.CbT = training ? at<bf16_t>(ws, L.A0T) : nullptr;
   g.ldct = Pe;
   g.valid_rows = static_cast<int>(L.Pe);
   if ((rc = gemm(st, g)) != OTTX_OK) return rc;
@@ -668,7 +676,7 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   g.relu = 1;
   g.Cb = at<bf16_t>(ws, L.A1);
   g.ldcb = 256;
-  g.CbT = at<bf16_t>(ws, L.A1T);
+  g.CbT = training ? at<bf16_t>(ws, L.A1T) : nullptr;
   g.ldct = Pe;
   g.valid_rows = static_cast<int>(L.Pe);
   if ((rc = gemm(st, g)) != OTTX_OK) return rc;
@@ -683,7 +691,7 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   hipLaunchKernelGGL(pool_partial_kernel, dim3(L.chunks, 2 * B), dim3(256), 0, st, at<float>(ws, L.H2f), bt->seg_off,
                      L.chunks, at<float>(ws, L.part));
   hipLaunchKernelGGL(pool_final_kernel, dim3(B), dim3(256), 0, st, at<float>(ws, L.part), bt->seg_off, L.chunks,
-                     bt->eps, sig2, at<float>(ws, L.ctx));
+                     bt->eps, bt->eps_rel, sig2, at<float>(ws, L.ctx));
   hipLaunchKernelGGL(ctx_forward_kernel, dim3(B), dim3(256), 0, st, prm + P.Wc, prm + P.bh, at<float>(ws, L.ctx),
                      at<float>(ws, L.hctx));
   DS_LAUNCHED();
@@ -694,7 +702,7 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   g.relu = 1;
   g.Cb = at<bf16_t>(ws, L.Z1h);
   g.ldcb = 256;
-  g.CbT = at<bf16_t>(ws, L.Z1hT);
+  g.CbT = training ? at<bf16_t>(ws, L.Z1hT) : nullptr;
   g.ldct = Ph;
   g.valid_rows = static_cast<int>(L.Ph);
   if ((rc = gemm(st, g)) != OTTX_OK) return rc;
@@ -706,7 +714,7 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   g.ldc = 256;
   g.Cb = at<bf16_t>(ws, L.A2h);
   g.ldcb = 256;
-  g.CbT = at<bf16_t>(ws, L.A2hT);
+  g.CbT = training ? at<bf16_t>(ws, L.A2hT) : nullptr;
   g.ldct = Ph;
   g.valid_rows = static_cast<int>(L.Ph);
   if ((rc = gemm(st, g)) != OTTX_OK) return rc;
@@ -721,22 +729,22 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
 extern "C" {
 
 int64_t ottx_ds_param_count(int32_t d) {
-  if (d < 1 || d > 256) return 0;
+  if (d < 1 || d > 255) return 0;
   return param_layout(d).total;
 }
 
 int64_t ottx_ds_weight_bf16_count(int32_t d) {
-  if (d < 1 || d > 256) return 0;
+  if (d < 1 || d > 255) return 0;
   return weight_copies(param_layout(d)).total;
 }
 
-size_t ottx_ds_workspace_bytes(const ottx_ds_batch* bt) {
-  if (!bt || bt->count < 1 || bt->d < 1 || bt->d > 256 || bt->n_total < 1 || bt->m_total < 1) return 0;
-  return ds_layout(bt, param_layout(bt->d)).total;
+size_t ottx_ds_workspace_bytes(const ottx_ds_batch* bt, int32_t training) {
+  if (!bt || bt->count < 1 || bt->d < 1 || bt->d > 255 || bt->n_total < 1 || bt->m_total < 1) return 0;
+  return ds_layout(bt, param_layout(bt->d), training != 0).total;
 }
 
 int ottx_ds_refresh_bf16(const ottx_ds_model* md, void* stream) {
-  if (!md || !md->params || !md->weights_bf16 || md->d < 1 || md->d > 256) return OTTX_EINVAL;
+  if (!md || !md->params || !md->weights_bf16 || md->d < 1 || md->d > 255) return OTTX_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const ParamLayout P = param_layout(md->d);
   const WeightCopies C = weight_copies(P);
@@ -758,15 +766,15 @@ int ottx_ds_refresh_bf16(const ottx_ds_model* md, void* stream) {
 int ottx_ds_forward(const ottx_ds_model* md, const ottx_ds_batch* bt, float* g_out, void* workspace,
                     size_t workspace_bytes, void* stream) {
   if (!md || !bt || !g_out || !workspace || md->d != bt->d) return OTTX_EINVAL;
-  const size_t need = ottx_ds_workspace_bytes(bt);
+  const size_t need = ottx_ds_workspace_bytes(bt, 0);
   if (need == 0) return OTTX_EINVAL;
   if (workspace_bytes < need || (reinterpret_cast<uintptr_t>(workspace) & 255)) return OTTX_EWORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
   Forward F;
-  int rc = forward(bt, md, workspace, st, &F);
+  int rc = forward(bt, md, workspace, st, false, &F);
   if (rc != OTTX_OK) return rc;
   hipLaunchKernelGGL(scale_out_kernel, dim3(blocks_for(F.L.Ph)), dim3(256), 0, st, at<float>(F.ws, F.L.out),
-                     bt->y_prob, bt->eps, F.L.Ph, g_out);
+                     bt->y_prob, at<float>(F.ws, F.L.sig2), F.L.Ph, g_out);
   DS_LAUNCHED();
   return OTTX_OK;
 }
@@ -774,12 +782,12 @@ int ottx_ds_forward(const ottx_ds_model* md, const ottx_ds_batch* bt, float* g_o
 int ottx_ds_loss_backward(const ottx_ds_model* md, const ottx_ds_batch* bt, const float* target, float* grad,
                           float* loss_out, void* workspace, size_t workspace_bytes, void* stream) {
   if (!md || !bt || !target || !grad || !workspace || md->d != bt->d) return OTTX_EINVAL;
-  const size_t need = ottx_ds_workspace_bytes(bt);
+  const size_t need = ottx_ds_workspace_bytes(bt, 1);
   if (need == 0) return OTTX_EINVAL;
   if (workspace_bytes < need || (reinterpret_cast<uintptr_t>(workspace) & 255)) return OTTX_EWORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
   Forward F;
-  int rc = forward(bt, md, workspace, st, &F);
+  int rc = forward(bt, md, workspace, st, true, &F);
   if (rc != OTTX_OK) return rc;
   const ParamLayout& P = F.P;
   const WeightCopies& C = F.C;

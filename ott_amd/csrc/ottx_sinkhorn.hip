@@ -853,6 +853,16 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     return OTTX_OK;
   };
 
+  if (o->init_from_g) {
+    // f <- tau_a (eps log a - eps LSE_j((g_j - C_ij) / eps)), a plain update
+    HalfStepArgs f_init = fa;
+    f_init.mode = kModeWritePot;
+    f_init.momentum = 1.f;
+    f_init.state = nullptr;
+    rc = launch_halfstep(k, f_init, st);
+    if (rc != OTTX_OK) return rc;
+  }
+
   SolveState* mail = static_cast<SolveState*>(hh->mailbox);
   if (total_blocks > 0) {
     rc = launch_halfstep(k, g_plain, st);  // g-update of iteration 0

@@ -50,8 +50,8 @@ class ParamLayout:
 
 
 def param_layout(d: int) -> ParamLayout:
-  """Mirror of ``param_layout`` in ottx_deepsets.hip."""
-  kin = (d + 63) // 64 * 64
+  """Mirror of ``param_layout`` in ottx_deepsets.hip (inputs: d coordinates + |p~|^2 - 1)."""
+  kin = (d + 1 + 63) // 64 * 64
   shapes = (
       ("W0", (WIDTH, kin)), ("b0", (WIDTH,)), ("W1", (WIDTH, WIDTH)), ("b1", (WIDTH,)),
       ("W2", (WIDTH, WIDTH)), ("b2", (WIDTH,)), ("Wy", (WIDTH, kin)), ("Wc", (WIDTH, CTX)),
@@ -80,6 +80,7 @@ class Batch:
   enc_src: torch.Tensor
   seg_of_row: torch.Tensor
   seg_off: torch.Tensor
+  eps_rel: Optional[torch.Tensor] = None
 
   def struct(self) -> _lib.DsBatch:
     return _lib.DsBatch(
@@ -87,12 +88,16 @@ class Batch:
         x=self.x.data_ptr(), y=self.y.data_ptr(), eps=self.eps.data_ptr(),
         x_off=self.x_off.data_ptr(), y_off=self.y_off.data_ptr(), y_prob=self.y_prob.data_ptr(),
         enc_src=self.enc_src.data_ptr(), seg_of_row=self.seg_of_row.data_ptr(),
-        seg_off=self.seg_off.data_ptr())
+        seg_off=self.seg_off.data_ptr(),
+        eps_rel=self.eps_rel.data_ptr() if self.eps_rel is not None else None)
 
 
-def make_batch(problems: Sequence[Tuple[torch.Tensor, torch.Tensor, float]],
-               device: Optional[torch.device] = None) -> Batch:
-  """Concatenate ``(x, y, eps)`` problems and build the index arrays."""
+def make_batch(problems: Sequence[Tuple], device: Optional[torch.device] = None) -> Batch:
+  """Concatenate ``(x, y, eps)`` or ``(x, y, eps, eps_rel)`` problems and build the index arrays.
+
+  ``eps_rel`` is epsilon divided by the mean cost; without it the network's
+  scale input is derived from eps and the clouds.
+  """
   device = device or _lib.require_device()
   xs = [torch.as_tensor(p[0], dtype=torch.float32).to(device) for p in problems]
   ys = [torch.as_tensor(p[1], dtype=torch.float32).to(device) for p in problems]
@@ -109,19 +114,27 @@ def make_batch(problems: Sequence[Tuple[torch.Tensor, torch.Tensor, float]],
   seg_sizes = np.stack([ns, ms], axis=1).reshape(-1)
   seg_off = np.concatenate([[0], np.cumsum(seg_sizes)]).astype(np.int32)
   seg_of_row = np.repeat(np.arange(2 * B, dtype=np.int32), seg_sizes)
-  enc_src = np.empty(n_total + m_total, dtype=np.int32)
-  for b in range(B):
-    s0 = seg_off[2 * b]
-    enc_src[s0:s0 + ns[b]] = np.arange(x_off[b], x_off[b + 1])
-    s1 = seg_off[2 * b + 1]
-    enc_src[s1:s1 + ms[b]] = np.arange(y_off[b], y_off[b + 1])
-  as_dev = lambda a: torch.from_numpy(a).to(device)  # noqa: E731
+  # encoder row r of segment s = 2 b + side reads row (r - seg_off[s]) + off_side[b]
+  side_base = np.stack([x_off[:-1], y_off[:-1]], axis=1).reshape(-1)
+  enc_src = (np.arange(n_total + m_total, dtype=np.int64) - seg_off[seg_of_row] +
+             side_base[seg_of_row]).astype(np.int32)
+  eps = np.array([float(p[2]) for p in problems], dtype=np.float32)
+  has_rel = all(len(p) > 3 and p[3] is not None for p in problems)
+  rel = np.array([float(p[3]) for p in problems] if has_rel else [], dtype=np.float32)
+  # one host-to-device copy for every index array
+  parts = [x_off, y_off, y_prob, enc_src, seg_of_row, seg_off, eps.view(np.int32), rel.view(np.int32)]
+  flat = torch.from_numpy(np.concatenate(parts)).to(device)
+  views, o = [], 0
+  for p in parts:
+    views.append(flat[o:o + p.size])
+    o += p.size
+  x = xs[0].contiguous() if B == 1 else torch.cat(xs).contiguous()
+  y = ys[0].contiguous() if B == 1 else torch.cat(ys).contiguous()
   return Batch(
-      count=B, d=d, n_total=n_total, m_total=m_total,
-      x=torch.cat(xs).contiguous(), y=torch.cat(ys).contiguous(),
-      eps=torch.tensor([float(p[2]) for p in problems], dtype=torch.float32, device=device),
-      x_off=as_dev(x_off), y_off=as_dev(y_off), y_prob=as_dev(y_prob), enc_src=as_dev(enc_src),
-      seg_of_row=as_dev(seg_of_row), seg_off=as_dev(seg_off))
+      count=B, d=d, n_total=n_total, m_total=m_total, x=x, y=y,
+      eps=views[6].view(torch.float32), x_off=views[0], y_off=views[1], y_prob=views[2],
+      enc_src=views[3], seg_of_row=views[4], seg_off=views[5],
+      eps_rel=views[7].view(torch.float32) if has_rel else None)
 
 
 class DeepSets:
@@ -150,8 +163,8 @@ class DeepSets:
     """He-uniform weights for the ReLU layers; a zero output layer."""
     gen = torch.Generator().manual_seed(seed)
     host = torch.zeros(self.layout.total, dtype=torch.float32)
-    fan_in = {"W0": self.d, "W1": WIDTH, "W2": WIDTH, "Wy": self.d + 2 * WIDTH + 1,
-              "Wc": self.d + 2 * WIDTH + 1, "W4": WIDTH}
+    fan_in = {"W0": self.d + 1, "W1": WIDTH, "W2": WIDTH, "Wy": self.d + 1 + 2 * WIDTH + 1,
+              "Wc": self.d + 1 + 2 * WIDTH + 1, "W4": WIDTH}
     for name, shape in self.layout.shapes:
       off = dict(self.layout.offsets)[name]
       n = int(np.prod(shape))
@@ -159,7 +172,7 @@ class DeepSets:
         bound = math.sqrt(6.0 / fan_in[name])
         w = (torch.rand(shape, generator=gen) * 2 - 1) * bound
         if name in ("W0", "Wy"):
-          w[:, self.d:] = 0.0
+          w[:, self.d + 1:] = 0.0
         if name == "Wc":
           w[:, 2 * WIDTH + 1:] = 0.0
         host[off:off + n] = w.reshape(-1)
@@ -188,18 +201,22 @@ class DeepSets:
     return model
 
   # -- compute ------------------------------------------------------------
-  def _workspace(self, batch: Batch) -> Tuple[torch.Tensor, _lib.DsBatch]:
+  def _workspace(self, batch: Batch, training: bool) -> Tuple[torch.Tensor, _lib.DsBatch]:
+    """Workspace for one call; kept and reused by later calls that fit in it."""
     lib = _lib.load_library()
     bs = batch.struct()
-    nbytes = lib.ottx_ds_workspace_bytes(ctypes.byref(bs))
+    nbytes = lib.ottx_ds_workspace_bytes(ctypes.byref(bs), int(training))
     if nbytes == 0:
       raise ValueError("invalid DeepSets batch")
-    return _lib.workspace(nbytes, self.device), bs
+    ws = getattr(self, "_ws", None)
+    if ws is None or ws.numel() < nbytes:
+      self._ws = ws = _lib.workspace(nbytes, self.device)
+    return ws, bs
 
   def forward(self, batch: Batch) -> torch.Tensor:
     """Predicted potentials g (per head row, batch order), up to a constant per problem."""
     lib = _lib.load_library()
-    ws, bs = self._workspace(batch)
+    ws, bs = self._workspace(batch, training=False)
     out = torch.empty(batch.m_total, dtype=torch.float32, device=self.device)
     _lib.check("ottx_ds_forward", lib.ottx_ds_forward(
         ctypes.byref(self.struct()), ctypes.byref(bs), out.data_ptr(), ws.data_ptr(), ws.numel(),
@@ -207,9 +224,9 @@ class DeepSets:
     return out
 
   def loss_backward(self, batch: Batch, target: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
-    """Per-problem centred MSE (of g / eps) and the gradient of its mean."""
+    """Per-problem centred MSE (of g / sigma^2) and the gradient of its mean."""
     lib = _lib.load_library()
-    ws, bs = self._workspace(batch)
+    ws, bs = self._workspace(batch, training=True)
     grad = torch.zeros_like(self.params)
     loss = torch.empty(batch.count, dtype=torch.float32, device=self.device)
     target = target.to(device=self.device, dtype=torch.float32).contiguous()
@@ -235,18 +252,33 @@ class DeepSetsInitializer(SinkhornInitializer):
   zero-weight points are set to -inf.
   """
 
+  predicts_g = True  # Sinkhorn derives f0 from g^ with its own first f-update
+
   def __init__(self, model: DeepSets):
     self.model = model
+
+  def predict_g(self, ot_prob) -> torch.Tensor:
+    """g^ on the target points, -inf where b is zero."""
+    geom = ot_prob.geom
+    inv_s = geom.inv_scale_cost  # the network works in units of |x - y|^2
+    eps_abs, eps_rel = geom._epsilon_spec()
+    if eps_abs > 0.0:
+      eps, rel = eps_abs / inv_s, None
+    else:
+      # relative epsilon not resolved yet: the network only needs eps / mean cost
+      eps, rel = 0.0, eps_rel
+    batch = make_batch([(geom.x, geom.y, eps, rel)], device=geom.device)
+    g_hat = self.model.forward(batch)
+    if inv_s != 1.0:
+      g_hat = g_hat * inv_s
+    b = ot_prob.b
+    return torch.where(b > 0.0, g_hat, torch.full_like(g_hat, -math.inf))
 
   def __call__(self, ot_prob, lse_mode: bool = True, **kwargs):
     if not lse_mode:
       raise NotImplementedError("the learned warm start is for lse_mode=True")
     geom = ot_prob.geom
-    eps = geom.epsilon
-    batch = make_batch([(geom.x, geom.y, eps)], device=geom.device)
-    g_hat = self.model.forward(batch)
-    b = ot_prob.b
-    g_hat = torch.where(b > 0.0, g_hat, torch.full_like(g_hat, -math.inf))
+    g_hat = self.predict_g(ot_prob)
     f0 = geom.update_potential(torch.zeros_like(ot_prob.a), g_hat, torch.log(ot_prob.a), axis=1)
     f0 = ot_prob.tau_a * f0 if ot_prob.tau_a != 1.0 else f0
     a = ot_prob.a

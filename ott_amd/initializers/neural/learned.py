@@ -7,18 +7,17 @@ solver starts from zero potentials (OTT's DefaultInitializer).
 
 from __future__ import annotations
 
-from typing import Callable, Optional, Tuple
+from typing import Optional, Tuple
 
 from ott_amd.initializers.neural import deepsets
 
 _cache = {}
 
 
-def bench_initializer(d: int) -> Tuple[Optional[Callable], str]:
+def bench_initializer(d: int) -> Tuple[Optional[deepsets.DeepSetsInitializer], str]:
   path = deepsets.shipped_model_path(d)
   if path is None:
     return None, "zeros"
   if d not in _cache:
     _cache[d] = deepsets.DeepSetsInitializer(deepsets.DeepSets.load(path))
-  init = _cache[d]
-  return (lambda prob: init(prob)), f"deepsets ({path.rsplit('/', 1)[-1]})"
+  return _cache[d], f"deepsets ({path.rsplit('/', 1)[-1]})"

@@ -182,16 +182,23 @@ class Sinkhorn:
     """sinkhorn.py:782-801."""
     self._check_supported(ot_prob)
     geom = ot_prob.geom
+    init_from_g = False
+    if init is None and getattr(self.initializer, "predicts_g", False):
+      # A warm start that predicts g only: the solver derives f0 from it with
+      # its own first f-update instead of a separate pass over the geometry.
+      g = as_device_f32(self.initializer.predict_g(ot_prob), geom.device).clone()
+      f = torch.zeros((geom.shape[0],), dtype=torch.float32, device=geom.device)
+      return run(ot_prob, self, (f, g), init_from_g=True)
     if init is None:
       init = self.initializer(ot_prob, lse_mode=self.lse_mode, **kwargs)
     f0, g0 = init
     f = as_device_f32(f0, geom.device).clone()
     g = as_device_f32(g0, geom.device).clone()
-    return run(ot_prob, self, (f, g))
+    return run(ot_prob, self, (f, g), init_from_g=init_from_g)
 
 
 def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
-        init: Tuple[torch.Tensor, torch.Tensor]) -> SinkhornOutput:
+        init: Tuple[torch.Tensor, torch.Tensor], init_from_g: bool = False) -> SinkhornOutput:
   """sinkhorn.py:1052-1099: the loop, then ``set_cost`` and ``ot_prob``."""
   lib = _lib.load_library()
   geom: PointCloud = ot_prob.geom
@@ -210,7 +217,8 @@ def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
       max_iterations=int(solver.max_iterations),
       momentum=float(solver.momentum.value),
       tile_config=tune.lookup(geom.precision, geom.x.shape[1], n, m),
-      use_graph=1 if solver.use_graph else 0)
+      use_graph=1 if solver.use_graph else 0,
+      init_from_g=1 if init_from_g else 0)
   errors = torch.empty((solver.outer_iterations,), dtype=torch.float32, device=geom.device)
   out = _lib.SinkhornOut(f=f.data_ptr(), g=g.data_ptr(), errors=errors.data_ptr())
   nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts))

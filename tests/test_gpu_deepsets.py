@@ -32,17 +32,22 @@ def _reference(params, layout, problems, target, bf16=True):
   W = {k: (rnd(P[k].detach()) + (P[k] - P[k].detach())) for k in ("W0", "W1", "W2", "Wy", "W4")}
   outs, losses = [], []
   q = 0
-  for (x, y, eps) in problems:
+  for prob in problems:
+    x, y, eps = prob[:3]
+    rel = prob[3] if len(prob) > 3 else None
     x = torch.tensor(x, dtype=torch.float64)
     y = torch.tensor(y, dtype=torch.float64)
     d = x.shape[1]
+    # x x^T etc. in float64; the kernel's float32 statistics agree to ~1e-6
     mx, my = x.mean(0), y.mean(0)
     mu = 0.5 * (mx + my)
     meanc = (x * x).sum(1).mean() + (y * y).sum(1).mean() - 2 * (mx @ my)
     sig2 = 0.5 * meanc
     def norm(p):
       z = torch.zeros(p.shape[0], layout.kin, dtype=torch.float64)
-      z[:, :d] = (p - mu) / torch.sqrt(sig2)
+      pt = ((p - mu) / torch.sqrt(sig2)).to(torch.float32).to(torch.float64)
+      z[:, :d] = pt
+      z[:, d] = (pt * pt).sum(1) - 1.0
       return rnd(z)
     xt, yt = norm(x), norm(y)
     def enc(p):
@@ -52,7 +57,8 @@ def _reference(params, layout, problems, target, bf16=True):
       a1 = rnd(a1.detach()) + (a1 - a1.detach())
       return a1 @ W["W2"].T + P["b2"]
     ex, ey = enc(xt).mean(0), enc(yt).mean(0)
-    ctx = torch.cat([ex, ey, (eps / sig2).reshape(1), torch.zeros(31, dtype=torch.float64)])
+    kappa = torch.tensor(2.0 * rel, dtype=torch.float64) if rel is not None else eps / sig2
+    ctx = torch.cat([ex, ey, kappa.reshape(1), torch.zeros(31, dtype=torch.float64)])
     hctx = P["Wc"] @ ctx + P["bh"]
     z1 = torch.relu(yt @ W["Wy"].T + hctx)
     z1 = rnd(z1.detach()) + (z1 - z1.detach())
@@ -61,7 +67,7 @@ def _reference(params, layout, problems, target, bf16=True):
     m = y.shape[0]
     r = out - torch.tensor(target[q:q + m], dtype=torch.float64)
     losses.append(((r - r.mean()) ** 2).mean())
-    outs.append(eps * out)
+    outs.append(sig2 * out)
     q += m
   loss = torch.stack(losses).mean()
   return outs, loss, losses, P
@@ -78,8 +84,8 @@ def _problems(d, seed):
   return out
 
 
-@pytest.mark.parametrize("d", [5, 40])
-def test_forward_and_gradients_match_float64_autograd(d):
+@pytest.mark.parametrize("d,relative", [(5, False), (40, False), (63, True)])
+def test_forward_and_gradients_match_float64_autograd(d, relative):
   from ott_amd.initializers.neural import deepsets
   model = deepsets.DeepSets(d, seed=3)
   # give the output layer non-zero weights so every gradient path is exercised
@@ -89,8 +95,10 @@ def test_forward_and_gradients_match_float64_autograd(d):
     model.tensor("b5")[0] = 0.3
   model.refresh()
   probs = _problems(d, seed=d)
-  batch = deepsets.make_batch([(torch.from_numpy(x), torch.from_numpy(y), e) for x, y, e in probs])
-  m_total = sum(y.shape[0] for _, y, _ in probs)
+  if relative:
+    probs = [(x, y, 0.0, e) for x, y, e in probs]  # only eps / mean cost given
+  batch = deepsets.make_batch([(torch.from_numpy(p[0]), torch.from_numpy(p[1]), *p[2:]) for p in probs])
+  m_total = sum(p[1].shape[0] for p in probs)
   target = rng.normal(size=m_total).astype(np.float32)
 
   g = model.forward(batch).cpu().numpy().astype(np.float64)
@@ -161,3 +169,51 @@ def test_training_reduces_loss():
     model.adam_step(grad, lr=3e-3)
   last = float(model.loss_backward(batch, target)[0].mean())
   assert last < 0.2 * first, (first, last)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+def test_initializer_fast_path_matches_explicit_init(precision):
+  """Sinkhorn(initializer=DeepSetsInitializer) derives f0 inside the solve
+  (init_from_g); it must agree with passing init=(f0, g^) computed outside."""
+  from ott_amd.geometry import pointcloud
+  from ott_amd.initializers.neural import deepsets
+  from ott_amd.problems.linear import linear_problem
+  from ott_amd.solvers.linear import sinkhorn
+  d = 16
+  model = deepsets.DeepSets(d, seed=7)
+  rng = np.random.default_rng(8)
+  with torch.no_grad():
+    model.tensor("w5").copy_(torch.from_numpy(rng.normal(size=256).astype(np.float32) * 0.05))
+  model.refresh()
+  init = deepsets.DeepSetsInitializer(model)
+  x = torch.from_numpy(rng.normal(size=(700, d)).astype(np.float32) / 4).cuda()
+  y = torch.from_numpy((rng.normal(size=(513, d)) / 4 + 0.1).astype(np.float32)).cuda()
+  outs = []
+  for fast in (True, False):
+    geom = pointcloud.PointCloud(x, y, epsilon=0.05, relative_epsilon="mean", precision=precision)
+    prob = linear_problem.LinearProblem(geom)
+    solver = sinkhorn.Sinkhorn(threshold=1e-4, inner_iterations=1, initializer=init)
+    outs.append(solver(prob) if fast else solver(prob, init=init(prob)))
+  a, b = outs
+  assert a.n_iters == b.n_iters
+  assert abs(a.reg_ot_cost - b.reg_ot_cost) <= 1e-6 * abs(b.reg_ot_cost)
+  torch.testing.assert_close(a.f, b.f, rtol=1e-4, atol=1e-4 * float(b.f.abs().max()))
+
+
+def test_relative_epsilon_and_absolute_give_same_prediction():
+  """kappa from eps / mean cost (unresolved relative epsilon) equals eps / sigma^2."""
+  from ott_amd.initializers.neural import deepsets
+  d = 8
+  model = deepsets.DeepSets(d, seed=2)
+  rng = np.random.default_rng(3)
+  with torch.no_grad():
+    model.tensor("w5").copy_(torch.from_numpy(rng.normal(size=256).astype(np.float32) * 0.05))
+    model.tensor("Wc").normal_(0.0, 0.1)
+  model.refresh()
+  x = rng.normal(size=(300, d)).astype(np.float32)
+  y = (rng.normal(size=(200, d)) + 0.5).astype(np.float32)
+  meanc = float(((x[:, None, :].astype(np.float64) - y[None]) ** 2).sum(-1).mean())
+  c = 0.07
+  g_abs = model.forward(deepsets.make_batch([(torch.from_numpy(x), torch.from_numpy(y), c * meanc)]))
+  g_rel = model.forward(deepsets.make_batch([(torch.from_numpy(x), torch.from_numpy(y), 0.0, c)]))
+  torch.testing.assert_close(g_abs, g_rel, rtol=1e-4, atol=1e-4 * float(g_rel.abs().max()))
