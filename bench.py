@@ -123,15 +123,45 @@ def halfstep_roofline(pb, precision, reps=20):
   hbm = (n + m) * d * 4 + 3 * (n + m) * 4
   peak = PEAK_BF16_TFLOPS if precision == "bf16_split" else PEAK_F32_MFMA_TFLOPS
   achieved = flops / (ms * 1e-3) / 1e12
+  traffic = load_traffic(precision, tile)
   return {
       "kernel": f"halfstep_kernel ({precision}, g-update, tile_config {tile})",
       "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-      "frac": round(achieved / peak, 5), "traffic": None,
+      "frac": round(achieved / peak, 5),
+      "traffic": None if traffic is None else traffic["hbm_bytes_per_launch"],
       "avg_launch_ms": round(ms, 5), "flops_per_launch": flops,
       "pairs_per_s": pairs / (ms * 1e-3),
       "exp_per_s": pairs / (ms * 1e-3),
       "algorithmic_bytes_per_launch": hbm,
   }
+
+
+def copy_calibration(nbytes=1 << 28, reps=5):
+  """A plain device copy of known size (read nbytes + write nbytes per launch).
+
+  Profiled beside the half-step so FETCH_SIZE / WRITE_SIZE readings can be
+  scaled to bytes: factor = known bytes / counter reading of this copy.
+  """
+  import torch
+  src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+  dst = torch.empty_like(src)
+  for _ in range(reps):
+    dst.copy_(src)
+  torch.cuda.synchronize()
+  return {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}
+
+
+def load_traffic(precision, tile):
+  """Per-launch HBM bytes of the half-step measured with rocprofv3 --pmc, if recorded."""
+  path = os.path.join(ROOT, "profiles", "halfstep_traffic.json")
+  if not os.path.exists(path):
+    return None
+  with open(path) as fh:
+    rec = json.load(fh)
+  ent = rec.get(precision)
+  if not ent or ent.get("tile_config") != tile:
+    return None
+  return ent
 
 
 def cpu_baseline(pb, n_iters_needed, budget_s=15.0):
@@ -179,6 +209,8 @@ def main() -> int:
   ap.add_argument("--workload", default="gaussian_d64", choices=["gaussian_d64", "uniform_d2"])
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
   ap.add_argument("--cpu-budget", type=float, default=15.0)
+  ap.add_argument("--halfstep-only", action="store_true",
+                  help="only launch the half-step kernels and a calibration copy (for profiler passes)")
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
@@ -194,6 +226,11 @@ def main() -> int:
   from ott_amd.initializers.neural import learned
   pb, max_it = workload(args.workload)
   log(f"workload {pb.name}: n={pb.x.shape[0]} m={pb.y.shape[0]} d={pb.x.shape[1]}")
+  if args.halfstep_only:
+    res = {p: halfstep_roofline(pb, p, reps=5) for p in ("bf16_split", "fp32")}
+    res["calibration"] = copy_calibration()
+    print(json.dumps(res), flush=True)
+    return 0
 
   init, init_name = learned.bench_initializer(pb.x.shape[1])
   # The error is fused into every g-update, so checking it after each iteration

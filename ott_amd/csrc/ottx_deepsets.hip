@@ -135,45 +135,61 @@ __global__ void sum_splits_kernel(const float* __restrict__ part, int splits, in
 // sigma^2 = mean(C) / 2 with mean(C) = mean|x|^2 + mean|y|^2 - 2 <mean x, mean y>.
 // One workgroup per problem (thread per feature, rows strided).
 // ---------------------------------------------------------------------------
-__global__ void stats_kernel(const float* __restrict__ x, const float* __restrict__ y, const int* __restrict__ xoff,
-                             const int* __restrict__ yoff, int d, float* __restrict__ mu, float* __restrict__ sig2) {
-  __shared__ double red[3][256];
-  const int b = blockIdx.x, t = threadIdx.x;
+constexpr int kStatChunks = 64;
+
+// Per-(problem, chunk) float64 partial sums of the coordinates and their
+// squares, for x and y: part[((b * kStatChunks + c) * 4 + q) * d + k] with
+// q = sum x, sum x^2, sum y, sum y^2.  Grid (kStatChunks, B).
+__global__ void stats_partial_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                     const int* __restrict__ xoff, const int* __restrict__ yoff, int d,
+                                     double* __restrict__ part) {
+  __shared__ double fs[4][256];
+  const int c = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
   const int G = max(1, 256 / d);
   const int k = t % d, gi = t / d;
-  double sx = 0, sxx = 0, sy = 0, syy = 0;
-  const int x0 = xoff[b], x1 = xoff[b + 1], y0 = yoff[b], y1 = yoff[b + 1];
-  __shared__ double fs[4][256];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (t < G * d) {
-    for (int r = x0 + gi; r < x1; r += G) {
-      const double v = x[static_cast<int64_t>(r) * d + k];
-      sx += v;
-      sxx += v * v;
-    }
-    for (int r = y0 + gi; r < y1; r += G) {
-      const double v = y[static_cast<int64_t>(r) * d + k];
-      sy += v;
-      syy += v * v;
+    for (int side = 0; side < 2; ++side) {
+      const int* off = side ? yoff : xoff;
+      const float* p = side ? y : x;
+      const int r0 = off[b], r1 = off[b + 1];
+      const int per = (r1 - r0 + kStatChunks - 1) / kStatChunks;
+      const int lo = r0 + c * per, hi = min(r1, lo + per);
+      for (int r = lo + gi; r < hi; r += G) {
+        const double v = p[static_cast<int64_t>(r) * d + k];
+        acc[2 * side] += v;
+        acc[2 * side + 1] += v * v;
+      }
     }
   }
-  fs[0][t] = sx;
-  fs[1][t] = sxx;
-  fs[2][t] = sy;
-  fs[3][t] = syy;
+  for (int q = 0; q < 4; ++q) fs[q][t] = acc[q];
   __syncthreads();
+  if (t < d) {
+    for (int q = 0; q < 4; ++q) {
+      double s = 0.0;
+      for (int g = 0; g < G; ++g) s += fs[q][g * d + t];
+      part[((static_cast<int64_t>(b) * kStatChunks + c) * 4 + q) * d + t] = s;
+    }
+  }
+}
+
+// mu_b = (mean x + mean y) / 2 and sigma_b^2 = mean(C) / 2 with
+// mean(C) = mean |x|^2 + mean |y|^2 - 2 <mean x, mean y>.  Grid B.
+__global__ void stats_final_kernel(const double* __restrict__ part, const int* __restrict__ xoff,
+                                   const int* __restrict__ yoff, int d, float* __restrict__ mu,
+                                   float* __restrict__ sig2) {
+  __shared__ double red[3][256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int nx = xoff[b + 1] - xoff[b], ny = yoff[b + 1] - yoff[b];
   double nx2 = 0, ny2 = 0, dot = 0;
   if (t < d) {
-    double a1 = 0, a2 = 0, b1 = 0, b2 = 0;
-    for (int q = 0; q < G; ++q) {
-      a1 += fs[0][q * d + t];
-      a2 += fs[1][q * d + t];
-      b1 += fs[2][q * d + t];
-      b2 += fs[3][q * d + t];
-    }
-    const double mx = a1 / (x1 - x0), my = b1 / (y1 - y0);
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < kStatChunks; ++c)
+      for (int j = 0; j < 4; ++j) q[j] += part[((static_cast<int64_t>(b) * kStatChunks + c) * 4 + j) * d + t];
+    const double mx = q[0] / nx, my = q[2] / ny;
     mu[b * d + t] = static_cast<float>(0.5 * (mx + my));
-    nx2 = a2;
-    ny2 = b2;
+    nx2 = q[1];
+    ny2 = q[3];
     dot = mx * my;
   }
   red[0][t] = nx2;
@@ -186,23 +202,19 @@ __global__ void stats_kernel(const float* __restrict__ x, const float* __restric
     __syncthreads();
   }
   if (t == 0) {
-    const double mc = red[0][0] / (x1 - x0) + red[1][0] / (y1 - y0) - 2.0 * red[2][0];
+    const double mc = red[0][0] / nx + red[1][0] / ny - 2.0 * red[2][0];
     sig2[b] = static_cast<float>(fmax(0.5 * mc, 1e-30));
   }
 }
 
-// Normalised bf16 inputs.  Row r holds p~ = (p - mu_b) / sigma_b in columns
-// [0, d), the feature |p~|^2 - 1 in column d and zeros up to kpad; rows past
-// the valid count are zero.  XT (transposed, [kpad, rows_pad]) is written
-// when non-null (training needs it for the first-layer weight gradient).
-// Encoder rows are ordered x_0, y_0, x_1, y_1, ... (segment 2 b + side) and
-// come from x or y row src[r]; head rows are the y points (src null, prob[r]).
 __global__ void normalize_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                  const int* __restrict__ src, const int* __restrict__ seg_of_row,
                                  const int* __restrict__ prob, int64_t rows, int64_t rows_pad, int d, int kpad,
                                  const float* __restrict__ mu, const float* __restrict__ sig2, bf16_t* __restrict__ X,
                                  bf16_t* __restrict__ XT) {
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  // one 64-lane wave per row: coalesced loads and stores, wave-reduced |p~|^2
+  const int lane = threadIdx.x & 63;
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (r >= rows_pad) return;
   const float* p = nullptr;
   int b = 0;
@@ -217,16 +229,19 @@ __global__ void normalize_kernel(const float* __restrict__ x, const float* __res
     }
   }
   const float inv = p ? rsqrtf(sig2[b]) : 0.f;
-  float nrm = 0.f;
-  for (int k = 0; k < kpad; ++k) {
+  float part = 0.f;
+  for (int k = lane; k < d; k += 64) {
+    if (p) {
+      const float v = (p[k] - mu[b * d + k]) * inv;
+      part = fmaf(v, v, part);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+  for (int k = lane; k < kpad; k += 64) {
     float v = 0.f;
     if (p) {
-      if (k < d) {
-        v = (p[k] - mu[b * d + k]) * inv;
-        nrm = fmaf(v, v, nrm);
-      } else if (k == d) {
-        v = nrm - 1.f;
-      }
+      if (k < d) v = (p[k] - mu[b * d + k]) * inv;
+      else if (k == d) v = part - 1.f;
     }
     const bf16_t h = to_bf16(v);
     X[r * kpad + k] = h;
@@ -510,7 +525,7 @@ WeightCopies weight_copies(const ParamLayout& L) {
 struct DsLayout {
   int64_t Pe, Pe_pad, Ph, Ph_pad;  // encoder / head rows (valid, padded)
   int nprob, chunks, kchunks;
-  size_t mu, sig2, kappa, X0, Y0, A0, A0T, A1, A1T, H2f, part, ctx, hctx, Z1h, Z1hT, A2h, A2hT, A2f, out, dout, loss,
+  size_t mu, sig2, spart, kappa, X0, Y0, A0, A0T, A1, A1T, H2f, part, ctx, hctx, Z1h, Z1hT, A2h, A2hT, A2f, out, dout, loss,
       dZ, dZT, dA, dAT, dctx, dh, gpart, total;
 };
 
@@ -521,7 +536,7 @@ DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training)
   L.Pe_pad = pad64(L.Pe);
   L.Ph = bt->m_total;
   L.Ph_pad = pad64(L.Ph);
-  L.chunks = 16;
+  L.chunks = 128;
   L.kchunks = training ? 64 : 0;
   size_t o = 0;
   auto take = [&](size_t b) {
@@ -534,6 +549,7 @@ DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training)
   const int B = L.nprob;
   L.mu = take(static_cast<size_t>(B) * bt->d * 4);
   L.sig2 = take(static_cast<size_t>(B) * 4);
+  L.spart = take(static_cast<size_t>(B) * kStatChunks * 4 * bt->d * 8);
   L.kappa = take(static_cast<size_t>(B) * 4);
   L.X0 = take(static_cast<size_t>(Pe) * P.kin * 2 * (training ? 2 : 1));  // X0 (and X0T)
   L.Y0 = take(static_cast<size_t>(Ph) * P.kin * 2 * (training ? 2 : 1));  // Y0 (and Y0T)
@@ -646,16 +662,19 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   const int64_t Pe = L.Pe_pad, Ph = L.Ph_pad;
   float* mu = at<float>(ws, L.mu);
   float* sig2 = at<float>(ws, L.sig2);
-  hipLaunchKernelGGL(stats_kernel, dim3(B), dim3(256), 0, st, bt->x, bt->y, bt->x_off, bt->y_off, bt->d, mu, sig2);
+  double* spart = at<double>(ws, L.spart);
+  hipLaunchKernelGGL(stats_partial_kernel, dim3(kStatChunks, B), dim3(256), 0, st, bt->x, bt->y, bt->x_off,
+                     bt->y_off, bt->d, spart);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(B), dim3(256), 0, st, spart, bt->x_off, bt->y_off, bt->d, mu, sig2);
   DS_LAUNCHED();
   bf16_t* X0 = at<bf16_t>(ws, L.X0);
   bf16_t* X0T = X0 + Pe * P.kin;
   bf16_t* Y0 = at<bf16_t>(ws, L.Y0);
   bf16_t* Y0T = Y0 + Ph * P.kin;
-  hipLaunchKernelGGL(normalize_kernel, dim3(blocks_for(Pe)), dim3(256), 0, st, bt->x, bt->y, bt->enc_src,
+  hipLaunchKernelGGL(normalize_kernel, dim3(static_cast<unsigned>((Pe + 3) / 4)), dim3(256), 0, st, bt->x, bt->y, bt->enc_src,
                      bt->seg_of_row, static_cast<const int*>(nullptr), L.Pe, Pe, bt->d, P.kin, mu, sig2, X0,
                      training ? X0T : nullptr);
-  hipLaunchKernelGGL(normalize_kernel, dim3(blocks_for(Ph)), dim3(256), 0, st, bt->x, bt->y,
+  hipLaunchKernelGGL(normalize_kernel, dim3(static_cast<unsigned>((Ph + 3) / 4)), dim3(256), 0, st, bt->x, bt->y,
                      static_cast<const int*>(nullptr), static_cast<const int*>(nullptr), bt->y_prob, L.Ph, Ph, bt->d,
                      P.kin, mu, sig2, Y0, training ? Y0T : nullptr);
   DS_LAUNCHED();

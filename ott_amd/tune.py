@@ -128,3 +128,38 @@ def autotune(geom, reps: int = 10, save: bool = True, verbose: bool = False) -> 
       with open(path, "w") as fh:
         json.dump(existing, fh, indent=1, sort_keys=True)
   return dict(best=best, key=k, results=results)
+
+
+def main(argv=None) -> int:
+  """Autotune a list of shapes: ``python -m ott_amd.tune 64x16384x16384 2x1024x1024 ...``.
+
+  Each shape is ``d x n x m``; uniform-cube points (seeded) are used since the
+  kernel's time does not depend on the values.  Results go to the cache file
+  (``$OTTX_TUNE_CACHE``) and are printed as JSON.
+  """
+  import argparse
+  import numpy as np
+  from ott_amd.geometry import pointcloud
+  ap = argparse.ArgumentParser(description="tile autotuner")
+  ap.add_argument("shapes", nargs="+")
+  ap.add_argument("--precisions", default="bf16_split,fp32")
+  ap.add_argument("--reps", type=int, default=10)
+  args = ap.parse_args(argv)
+  out = {}
+  for shape in args.shapes:
+    d, n, m = (int(v) for v in shape.split("x"))
+    rng = np.random.default_rng(0)
+    x = torch.from_numpy(rng.uniform(size=(n, d)).astype(np.float32)).cuda()
+    y = torch.from_numpy(rng.uniform(size=(m, d)).astype(np.float32)).cuda()
+    for prec in args.precisions.split(","):
+      geom = pointcloud.PointCloud(x, y, epsilon=0.05, relative_epsilon="mean", precision=prec)
+      res = autotune(geom, reps=args.reps)
+      worst = max(r["ms_per_iteration"] for r in res["results"])
+      default = [r for r in res["results"] if r["tile_config"] == res["results"][0]["tile_config"]][0]
+      out[res["key"]] = dict(best=res["best"], default_ms=default["ms_per_iteration"], worst_ms=worst)
+      print(json.dumps({res["key"]: out[res["key"]]}), flush=True)
+  return 0
+
+
+if __name__ == "__main__":
+  raise SystemExit(main())
