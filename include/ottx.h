@@ -101,6 +101,60 @@ typedef struct ottx_sinkhorn_out {
   int32_t _pad;
 } ottx_sinkhorn_out;
 
+/* A batch of independent problems of one feature dimension, concatenated:
+ * problem k owns rows x_off[k] .. x_off[k+1]-1 of x and a, and rows
+ * y_off[k] .. y_off[k+1]-1 of y and b.
+ * Reference: tools/segment_sinkhorn.py:24-138 with geometry/segment.py:23-190,
+ * which pad every problem to max_measure_size with zero-weight zero vectors
+ * and vmap one solve per problem.  Nothing is padded here; mean_rows_x/y
+ * reproduce the padded statistics a relative epsilon sees there (padding
+ * rows have zero weight, so they change nothing else). */
+typedef struct ottx_batch {
+  int32_t count;            /* K problems (>= 1) */
+  int32_t d;                /* feature dimension (1..256) */
+  int32_t precision;        /* OTTX_PREC_* */
+  int32_t _pad0;
+  const float* x;           /* [x_off[K], d] device */
+  const float* y;           /* [y_off[K], d] device */
+  const float* a;           /* [x_off[K]] device */
+  const float* b;           /* [y_off[K]] device */
+  const int64_t* x_off;     /* [K + 1] host, x_off[0] = 0, non-decreasing with n_k >= 1 */
+  const int64_t* y_off;     /* [K + 1] host */
+  const double* epsilon;    /* [K] host absolute epsilons; NULL when relative_epsilon > 0 */
+  double relative_epsilon;  /* > 0: eps_k = relative_epsilon * mean(C_k) */
+  double mean_rows_x;       /* > 0: mean(C_k) averages over this many x rows, as over a cloud
+                               padded to that size with zero vectors; 0: over n_k */
+  double mean_rows_y;       /* the same for y; 0: over m_k */
+  double inv_scale_cost;
+  double tau_a;
+  double tau_b;
+} ottx_batch;
+
+typedef struct ottx_batch_out {
+  float* f;                 /* [x_off[K]] device: initial potentials in, final out */
+  float* g;                 /* [y_off[K]] device */
+  float* errors;            /* [K, ceil(max_iterations / inner_iterations)] device */
+  double* reg_ot_cost;      /* [K] host */
+  double* epsilon;          /* [K] host, nullable */
+  int32_t* converged;       /* [K] host */
+  int32_t* n_iters;         /* [K] host */
+  int32_t n_blocks;         /* host: blocks of inner_iterations launched */
+  int32_t _pad;
+} ottx_batch_out;
+
+/* Workspace of ottx_sinkhorn_batched_solve. */
+size_t ottx_sinkhorn_batched_workspace_bytes(const ottx_batch* batch, const ottx_sinkhorn_opts* opts);
+/* Every problem of the batch solved as by ottx_sinkhorn_solve, each with its
+ * own epsilon, errors and convergence, in shared launches: one launch per
+ * half-step covers the column tiles and row splits of all problems, and a
+ * problem whose loop has finished drops out of later launches.
+ * Reference: tools/segment_sinkhorn.py:24-138 (returns reg_ot_cost per
+ * segment; this returns every SinkhornOutput field per problem).
+ * opts->init_from_g is not supported here; max_iterations must be >= inner_iterations. */
+int ottx_sinkhorn_batched_solve(const ottx_batch* batch, const ottx_sinkhorn_opts* opts,
+                                ottx_batch_out* out, void* workspace, size_t workspace_bytes,
+                                void* stream);
+
 /* Library identity. */
 int ottx_abi_version(void);
 const char* ottx_strerror(int status);

@@ -46,6 +46,8 @@ __all__ = [
     "sinkhorn",
     "ent_reg_cost",
     "dual_cost",
+    "segment_point_cloud",
+    "segment_sinkhorn",
 ]
 
 DEFAULT_EPSILON_SCALE = 0.05  # geometry/epsilon_scheduler.py:22
@@ -454,3 +456,46 @@ def solve_pointcloud(x, y, a=None, b=None, *, epsilon=None,
   eps = resolve_epsilon(x, y, epsilon, relative_epsilon, 1.0 / scale_cost)
   cost = sqeuclidean_cost(x, y, 1.0 / scale_cost, expanded=expanded, dtype=dtype)
   return sinkhorn(DenseLSE(cost), a, b, eps, dtype=dtype, **kwargs)
+
+
+def segment_point_cloud(x, a, num_per_segment, max_measure_size, padding_vector=None):
+  """geometry/segment.py:23-124, second interface (contiguous segments).
+
+  Returns padded points [K, P, d] and weights [K, P] with P =
+  min(max_measure_size, num); rows beyond P in a segment are dropped, as in
+  the reference's dynamic_slice.
+  """
+  x = np.asarray(x, dtype=np.float64)
+  num, dim = x.shape
+  counts = np.asarray(num_per_segment, dtype=np.int64)
+  P = min(int(max_measure_size), num)
+  if a is None:
+    a = np.repeat(1.0 / counts, counts)
+  a = np.asarray(a, dtype=np.float64)
+  pad = np.zeros((dim,)) if padding_vector is None else np.asarray(padding_vector, np.float64).reshape(dim)
+  seg_x = np.tile(pad, (len(counts), P, 1))
+  seg_a = np.zeros((len(counts), P))
+  off = 0
+  for k, c in enumerate(counts):
+    take = min(int(c), P)
+    seg_x[k, :take] = x[off:off + take]
+    seg_a[k, :take] = a[off:off + take]
+    off += int(c)
+  return seg_x, seg_a
+
+
+def segment_sinkhorn(x, y, num_per_segment_x, num_per_segment_y, max_measure_size=None,
+                     weights_x=None, weights_y=None, sinkhorn_kwargs=None, **pc_kwargs):
+  """tools/segment_sinkhorn.py:24-138: pad every segment, solve each padded pair.
+
+  Returns one OracleOutput per segment, on the padded problem (padding rows
+  carry zero weight and -inf potentials).
+  """
+  if max_measure_size is None:
+    max_measure_size = max(max(num_per_segment_x), max(num_per_segment_y))
+  sx, sa = segment_point_cloud(x, weights_x, num_per_segment_x, max_measure_size)
+  sy, sb = segment_point_cloud(y, weights_y, num_per_segment_y, max_measure_size)
+  outs = []
+  for k in range(sx.shape[0]):
+    outs.append(solve_pointcloud(sx[k], sy[k], sa[k], sb[k], **pc_kwargs, **(sinkhorn_kwargs or {})))
+  return outs

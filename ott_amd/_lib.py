@@ -81,6 +81,42 @@ class SinkhornOut(ctypes.Structure):
   ]
 
 
+class Batch(ctypes.Structure):
+  _fields_ = [
+      ("count", ctypes.c_int32),
+      ("d", ctypes.c_int32),
+      ("precision", ctypes.c_int32),
+      ("_pad0", ctypes.c_int32),
+      ("x", ctypes.c_void_p),
+      ("y", ctypes.c_void_p),
+      ("a", ctypes.c_void_p),
+      ("b", ctypes.c_void_p),
+      ("x_off", ctypes.POINTER(ctypes.c_int64)),
+      ("y_off", ctypes.POINTER(ctypes.c_int64)),
+      ("epsilon", ctypes.POINTER(ctypes.c_double)),
+      ("relative_epsilon", ctypes.c_double),
+      ("mean_rows_x", ctypes.c_double),
+      ("mean_rows_y", ctypes.c_double),
+      ("inv_scale_cost", ctypes.c_double),
+      ("tau_a", ctypes.c_double),
+      ("tau_b", ctypes.c_double),
+  ]
+
+
+class BatchOut(ctypes.Structure):
+  _fields_ = [
+      ("f", ctypes.c_void_p),
+      ("g", ctypes.c_void_p),
+      ("errors", ctypes.c_void_p),
+      ("reg_ot_cost", ctypes.POINTER(ctypes.c_double)),
+      ("epsilon", ctypes.POINTER(ctypes.c_double)),
+      ("converged", ctypes.POINTER(ctypes.c_int32)),
+      ("n_iters", ctypes.POINTER(ctypes.c_int32)),
+      ("n_blocks", ctypes.c_int32),
+      ("_pad", ctypes.c_int32),
+  ]
+
+
 class DsModel(ctypes.Structure):
   _fields_ = [
       ("d", ctypes.c_int32),
@@ -126,6 +162,11 @@ _SIGNATURES = [
       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     ("ottx_sinkhorn_workspace_bytes", ctypes.c_size_t,
      [ctypes.POINTER(Problem), ctypes.POINTER(SinkhornOpts)]),
+    ("ottx_sinkhorn_batched_workspace_bytes", ctypes.c_size_t,
+     [ctypes.POINTER(Batch), ctypes.POINTER(SinkhornOpts)]),
+    ("ottx_sinkhorn_batched_solve", ctypes.c_int,
+     [ctypes.POINTER(Batch), ctypes.POINTER(SinkhornOpts), ctypes.POINTER(BatchOut), ctypes.c_void_p,
+      ctypes.c_size_t, ctypes.c_void_p]),
     ("ottx_sinkhorn_solve", ctypes.c_int,
      [ctypes.POINTER(Problem), ctypes.POINTER(SinkhornOpts),
       ctypes.POINTER(SinkhornOut), ctypes.c_void_p, ctypes.c_size_t,

@@ -102,6 +102,7 @@ struct HalfStepArgs {
   int32_t outer;
   int32_t force_scan;
   float threshold;
+  uint32_t* batch_done;  // batched solves: count of problems whose loop finished
 };
 
 }  // namespace ottx

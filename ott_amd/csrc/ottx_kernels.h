@@ -94,6 +94,7 @@ __device__ inline void finalize_block(const HalfStepArgs& p) {
   if (!cont) {
     st->n_blocks = k + 1;
     st->done = 1;
+    if (p.batch_done != nullptr) atomicAdd(p.batch_done, 1u);
   }
 }
 
@@ -425,12 +426,13 @@ __device__ __forceinline__ void block_sum2(double (*s_red)[NT], int tid, double 
 }
 
 // ---------------------------------------------------------------------------
-// Single-problem half-step: grid (column tiles, row splits), W waves each
-// owning NB column blocks; the last split to arrive on a column tile merges
-// the partial (max, sum) pairs and runs the epilogue.
+// One workgroup of a half-step: column tile ``tile`` of the output side and
+// row split ``split`` of the input side.  W waves each own NB column blocks;
+// the last split to arrive on a column tile merges the partial (max, sum)
+// pairs and runs the epilogue.
 // ---------------------------------------------------------------------------
 template <int PREC, int KK, int NB, int MT, int W>
-__global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) {
+__device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int tile, const int split) {
   if ((p.mode & kModeCheckDone) && p.state->done) return;
   constexpr int NT = W * 64;
   constexpr int COLS = W * NB * 16;
@@ -444,8 +446,6 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int tile = blockIdx.x;
-  const int split = blockIdx.y;
   float c = p.c, eps = p.eps, eps_marg = p.eps_marg, inv_rho = p.inv_rho;
   if (p.scal != nullptr) {
     c = p.scal->c;
@@ -567,6 +567,26 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
       }
     }
   }
+}
+
+// Single-problem half-step: grid (column tiles, row splits).
+template <int PREC, int KK, int NB, int MT, int W>
+__global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) {
+  halfstep_block<PREC, KK, NB, MT, W>(p, blockIdx.x, blockIdx.y);
+}
+
+// Batched half-step over many independent problems in one launch.  Workgroup
+// b runs (problem, column tile, row split) = work[b].x, .y, .z with that
+// problem's arguments; ``mode`` replaces the per-problem mode.  Problems whose
+// loop is done exit at once (kModeCheckDone), so converged problems stop
+// costing time while the rest continue.
+template <int PREC, int KK, int NB, int MT, int W>
+__global__ __launch_bounds__(W * 64) void halfstep_batched_kernel(const HalfStepArgs* __restrict__ args,
+                                                                  const int4* __restrict__ work, uint32_t mode) {
+  const int4 w = work[blockIdx.x];
+  HalfStepArgs p = args[w.x];
+  p.mode = mode;
+  halfstep_block<PREC, KK, NB, MT, W>(p, w.y, w.z);
 }
 
 }  // namespace ottx

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "ottx_common.h"
 #include "ottx_kernels.h"
@@ -122,11 +123,9 @@ __global__ void stats_final_kernel(const double* __restrict__ px, int cx, const 
 
 // bf16 hi/lo fragments: [rb][kb][hi|lo][lane][8], lane -> row rb*16 + lane%16,
 // features kb*32 + 8*(lane/16) + e.
-__global__ void pack_bf16_kernel(const float* __restrict__ pts, int64_t np, int d,
-                                 const float* __restrict__ mu, int KB, int64_t n_threads,
-                                 uint4* __restrict__ frag) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= n_threads) return;
+__device__ __forceinline__ void pack_bf16_one(const float* __restrict__ pts, int64_t np, int d,
+                                              const float* __restrict__ mu, int KB, int64_t gid,
+                                              uint4* __restrict__ frag) {
   const int lane = static_cast<int>(gid & 63);
   const int64_t chunk = gid >> 6;
   const int kb = static_cast<int>(chunk % KB);
@@ -153,12 +152,17 @@ __global__ void pack_bf16_kernel(const float* __restrict__ pts, int64_t np, int 
   frag[(chunk * 2 + 1) * 64 + lane] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
 }
 
-// fp32 fragments: [rb][kq][lane][4], element r -> feature (4 kq + r) * 4 + lane/16.
-__global__ void pack_f32_kernel(const float* __restrict__ pts, int64_t np, int d,
-                                const float* __restrict__ mu, int KQ, int64_t n_threads,
-                                float4* __restrict__ frag) {
+__global__ void pack_bf16_kernel(const float* __restrict__ pts, int64_t np, int d,
+                                 const float* __restrict__ mu, int KB, int64_t n_threads,
+                                 uint4* __restrict__ frag) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= n_threads) return;
+  if (gid < n_threads) pack_bf16_one(pts, np, d, mu, KB, gid, frag);
+}
+
+// fp32 fragments: [rb][kq][lane][4], element r -> feature (4 kq + r) * 4 + lane/16.
+__device__ __forceinline__ void pack_f32_one(const float* __restrict__ pts, int64_t np, int d,
+                                             const float* __restrict__ mu, int KQ, int64_t gid,
+                                             float4* __restrict__ frag) {
   const int lane = static_cast<int>(gid & 63);
   const int64_t chunk = gid >> 6;
   const int kq = static_cast<int>(chunk % KQ);
@@ -171,6 +175,13 @@ __global__ void pack_f32_kernel(const float* __restrict__ pts, int64_t np, int d
     v[r] = (row < np && k < d) ? pts[row * d + k] - mu[k] : 0.f;
   }
   frag[chunk * 64 + lane] = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+__global__ void pack_f32_kernel(const float* __restrict__ pts, int64_t np, int d,
+                                const float* __restrict__ mu, int KQ, int64_t n_threads,
+                                float4* __restrict__ frag) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid < n_threads) pack_f32_one(pts, np, d, mu, KQ, gid, frag);
 }
 
 // s * |p - mu|^2 per point (float64 accumulation), zero on padded rows.
@@ -293,19 +304,22 @@ __global__ void dual_final_kernel(const double* __restrict__ part, const float* 
 // Kernel selection
 // ---------------------------------------------------------------------------
 typedef void (*HalfStepFn)(const HalfStepArgs);
+typedef void (*BatchedFn)(const HalfStepArgs*, const int4*, uint32_t);
 
 struct KernelCfg {
   int prec;
   int kk;       // bf16: K blocks of 32; fp32: k-steps of 4
   int nb, mt, w;
   HalfStepFn fn;
+  BatchedFn bfn;
   int cols() const { return 16 * nb * w; }
   int frag_chunks() const { return prec == 1 ? kk : (kk + 3) / 4; }  // 1 KiB chunks per 16 rows
 };
 
 template <int PREC, int KK, int NB, int MT, int W>
 KernelCfg make_cfg() {
-  return KernelCfg{PREC, KK, NB, MT, W, &halfstep_kernel<PREC, KK, NB, MT, W>};
+  return KernelCfg{PREC, KK, NB, MT, W, &halfstep_kernel<PREC, KK, NB, MT, W>,
+                   &halfstep_batched_kernel<PREC, KK, NB, MT, W>};
 }
 
 // Tile variants of the fused half-step per (precision, K): the search space
@@ -396,7 +410,7 @@ int device_cus() {
   return v;
 }
 
-int64_t pad_rows(int64_t n) { return ((n + kRowPad - 1) / kRowPad) * kRowPad; }
+int64_t pad_rows(int64_t n, int64_t pad = kRowPad) { return ((n + pad - 1) / pad) * pad; }
 
 struct PassPlan {
   int tiles = 0;
@@ -405,9 +419,10 @@ struct PassPlan {
   int n_in_rb = 0;
 };
 
-PassPlan plan_pass(const KernelCfg& k, int64_t n_in, int64_t n_out, int cus, int target_mult) {
+PassPlan plan_pass(const KernelCfg& k, int64_t n_in, int64_t n_out, int cus, int target_mult,
+                   int64_t pad = kRowPad) {
   PassPlan p;
-  const int64_t in_pad = pad_rows(n_in), out_pad = pad_rows(n_out);
+  const int64_t in_pad = pad_rows(n_in, pad), out_pad = pad_rows(n_out, pad);
   p.tiles = static_cast<int>(out_pad / k.cols());
   p.n_in_rb = static_cast<int>(in_pad / 16);
   const int in_tiles = p.n_in_rb / k.mt;
@@ -435,7 +450,8 @@ void stats_chunks(int64_t np, int* chunks, int64_t* rows) {
   *chunks = static_cast<int>((np + r - 1) / r);
 }
 
-Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& gp, const PassPlan& fp, int outer) {
+Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& gp, const PassPlan& fp, int outer,
+                   int64_t pad = kRowPad) {
   Layout L{};
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -443,7 +459,7 @@ Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& g
     off = align_up(off + bytes);
     return at;
   };
-  const int64_t n_pad = pad_rows(pb->n), m_pad = pad_rows(pb->m);
+  const int64_t n_pad = pad_rows(pb->n, pad), m_pad = pad_rows(pb->m, pad);
   const size_t chunk_bytes = static_cast<size_t>(k.frag_chunks()) * (k.prec == 1 ? 2048 : 1024);
   L.frag_x = take(static_cast<size_t>(n_pad / 16) * chunk_bytes);
   L.frag_y = take(static_cast<size_t>(m_pad / 16) * chunk_bytes);
@@ -1031,5 +1047,663 @@ extern "C" int ottx_benchmark_halfstep(const ottx_problem* pb, int32_t axis, int
   if (rc != OTTX_OK) return rc;
   if (e != hipSuccess) return OTTX_EHIP;
   *avg_ms_host = static_cast<double>(ms) / reps;
+  return OTTX_OK;
+}
+
+// ===========================================================================
+// Batched solves: many independent problems in shared launches.
+// Reference: tools/segment_sinkhorn.py:24-138 / geometry/segment.py:23-190,
+// which pad every problem to one size and vmap the solve; here every problem
+// keeps its own size, epsilon, loop state and errors, and each half-step is a
+// single launch over a work table of (problem, column tile, row split).
+// ===========================================================================
+namespace ottx {
+
+// Per-problem device view of one batched problem (pointers into the
+// problem's inputs and its own region of the workspace).
+struct BatchItem {
+  const float* x;
+  const float* y;
+  const float* a;
+  const float* b;
+  float* f;
+  float* g;
+  float* g_save;
+  float* errors;
+  int64_t n, m, n_pad, m_pad;
+  double n_div, m_div;   // row counts the mean-cost statistics divide by
+  double eps_abs;        // absolute epsilon (relative epsilon when 0)
+  void* frag_x;
+  void* frag_y;
+  float* norm_x;
+  float* norm_y;
+  float* mu;
+  SolveScalars* scal;
+  SolveState* state;
+  uint32_t* counters;
+  double* dual_part;
+  int32_t n_counters;
+  int32_t pad_;
+};
+
+// One workgroup per problem: float64 feature sums of x and y, the centring
+// shift, mean(C) over (n_div, m_div) rows and the solve scalars
+// (pointcloud.py:170-186, geometry.py:147-171).
+__global__ void batch_stats_kernel(const BatchItem* __restrict__ items, int d, double s, double rel, double tau_a,
+                                   double tau_b) {
+  __shared__ double red[4][256];
+  __shared__ double acc3[3][256];
+  const BatchItem& it = items[blockIdx.x];
+  const int t = threadIdx.x;
+  const int G = max(1, 256 / d);
+  double sx1 = 0.0, sx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
+  if (t < G * d) {
+    const int k = t % d, g = t / d;
+    for (int64_t r = g; r < it.n; r += G) {
+      const double v = it.x[r * d + k];
+      sx1 += v;
+      sx2 += v * v;
+    }
+    for (int64_t r = g; r < it.m; r += G) {
+      const double v = it.y[r * d + k];
+      sy1 += v;
+      sy2 += v * v;
+    }
+  }
+  red[0][t] = sx1;
+  red[1][t] = sx2;
+  red[2][t] = sy1;
+  red[3][t] = sy2;
+  __syncthreads();
+  double nx2 = 0.0, ny2 = 0.0, dot = 0.0;
+  if (t < d) {
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+    for (int g = 0; g < G; ++g) {
+      a1 += red[0][g * d + t];
+      a2 += red[1][g * d + t];
+      b1 += red[2][g * d + t];
+      b2 += red[3][g * d + t];
+    }
+    it.mu[t] = static_cast<float>(0.5 * (a1 / static_cast<double>(it.n) + b1 / static_cast<double>(it.m)));
+    nx2 = a2;
+    ny2 = b2;
+    dot = (a1 / it.n_div) * (b1 / it.m_div);
+  }
+  acc3[0][t] = nx2;
+  acc3[1][t] = ny2;
+  acc3[2][t] = dot;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st)
+      for (int q = 0; q < 3; ++q) acc3[q][t] += acc3[q][t + st];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double mean_c = s * (acc3[0][0] / it.n_div + acc3[1][0] / it.m_div - 2.0 * acc3[2][0]);
+    const double eps = rel > 0.0 ? rel * mean_c : it.eps_abs;
+    SolveScalars* sc = it.scal;
+    sc->eps = eps;
+    sc->eps_f = static_cast<float>(eps);
+    sc->c = static_cast<float>(2.0 * s * 1.4426950408889634 / eps);
+    sc->inv_rho_a = tau_a == 1.0 ? 0.f : static_cast<float>((1.0 - tau_a) / (eps * tau_a));
+    sc->inv_rho_b = tau_b == 1.0 ? 0.f : static_cast<float>((1.0 - tau_b) / (eps * tau_b));
+  }
+}
+
+// Fragments of both clouds of every problem: grid (x blocks, K, side).
+__global__ void batch_pack_kernel(const BatchItem* __restrict__ items, int d, int chunks, int prec) {
+  const BatchItem& it = items[blockIdx.y];
+  const int side = blockIdx.z;
+  const int64_t np = side ? it.m : it.n;
+  const int64_t np_pad = side ? it.m_pad : it.n_pad;
+  const int64_t n_threads = (np_pad / 16) * chunks * 64;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= n_threads) return;
+  const float* pts = side ? it.y : it.x;
+  void* frag = side ? it.frag_y : it.frag_x;
+  if (prec == 1) {
+    pack_bf16_one(pts, np, d, it.mu, chunks, gid, static_cast<uint4*>(frag));
+  } else {
+    pack_f32_one(pts, np, d, it.mu, chunks, gid, static_cast<float4*>(frag));
+  }
+}
+
+// s |p - mu|^2 of every point, zero on padded rows: grid (x blocks, K, side).
+__global__ void batch_norms_kernel(const BatchItem* __restrict__ items, int d, double s) {
+  const BatchItem& it = items[blockIdx.y];
+  const int side = blockIdx.z;
+  const int64_t np = side ? it.m : it.n;
+  const int64_t np_pad = side ? it.m_pad : it.n_pad;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= np_pad) return;
+  float* norm = side ? it.norm_y : it.norm_x;
+  if (i >= np) {
+    norm[i] = 0.f;
+    return;
+  }
+  const float* pts = side ? it.y : it.x;
+  double acc = 0.0;
+  for (int k = 0; k < d; ++k) {
+    const double v = static_cast<double>(pts[i * d + k] - it.mu[k]);
+    acc += v * v;
+  }
+  norm[i] = static_cast<float>(s * acc);
+}
+
+// Loop state of every problem (sinkhorn.py:964-972) and the batch counter.
+__global__ void batch_init_kernel(const BatchItem* __restrict__ items, int outer, uint32_t* batch_done) {
+  const BatchItem& it = items[blockIdx.x];
+  const int t = threadIdx.x;
+  if (t == 0) {
+    SolveState* st = it.state;
+    st->done = 0;
+    st->block = 0;
+    st->n_blocks = 0;
+    st->tiles_done = 0u;
+    st->shift_ok[0] = 0;
+    st->shift_ok[1] = 0;
+    st->total_mass = 0.0;
+    st->last_err = 0.0;
+    if (blockIdx.x == 0) *batch_done = 0u;
+  }
+  for (int k = t; k < outer; k += blockDim.x) it.errors[k] = -1.f;
+  for (int k = t; k < it.n_counters; k += blockDim.x) it.counters[k] = 0u;
+}
+
+// Stage 1 of the float64 dual terms of every problem: grid (kDualBlocks, 3, K),
+// as dual_partial_kernel for one problem.
+__global__ void batch_dual_partial_kernel(const BatchItem* __restrict__ items, double tau_a, double tau_b,
+                                          int outer) {
+  __shared__ double red[2][256];
+  const BatchItem& it = items[blockIdx.z];
+  const int tid = threadIdx.x;
+  const int side = blockIdx.y;
+  double s0 = 0.0, s1 = 0.0;
+  if (side < 2) {
+    const float* pot = side == 0 ? it.f : it.g_save;
+    const float* w = side == 0 ? it.a : it.b;
+    const int64_t len = side == 0 ? it.n : it.m;
+    const double tau = side == 0 ? tau_a : tau_b;
+    const double eps = it.scal->eps;
+    const double rho = eps * tau / (1.0 - tau);
+    const int64_t per = (len + kDualBlocks - 1) / kDualBlocks;
+    const int64_t i0 = blockIdx.x * per, i1 = min(len, i0 + per);
+    for (int64_t i = i0 + tid; i < i1; i += blockDim.x) {
+      const double wi = w[i];
+      s1 += wi;
+      if (wi > 0.0) {
+        const double h = static_cast<double>(pot[i]) - eps * log(wi);
+        if (tau == 1.0) {
+          s0 += wi * h;
+        } else {
+          s0 -= wi * rho * (exp(-h / rho) - 1.0);
+        }
+      }
+    }
+  } else if (blockIdx.x == 0) {
+    for (int i = tid; i < outer; i += blockDim.x) {
+      const float e = it.errors[i];
+      if (e != e) s0 += 1.0;
+      if (e != -1.f) s1 += 1.0;
+    }
+  }
+  red[0][tid] = s0;
+  red[1][tid] = s1;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (tid < st) {
+      red[0][tid] += red[0][tid + st];
+      red[1][tid] += red[1][tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    it.dual_part[(side * kDualBlocks + blockIdx.x) * 2 + 0] = red[0][0];
+    it.dual_part[(side * kDualBlocks + blockIdx.x) * 2 + 1] = red[1][0];
+  }
+}
+
+// Stage 2: out[16 k + ...] = div_a, sum a, div_b, sum b, #NaN errors,
+// #(errors != -1), errors[-1], eps, total mass, blocks run.  Grid K.
+__global__ void batch_dual_final_kernel(const BatchItem* __restrict__ items, int outer, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  const BatchItem& it = items[blockIdx.x];
+  double* o = out + 16 * static_cast<int64_t>(blockIdx.x);
+  for (int side = 0; side < 3; ++side) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int b = 0; b < kDualBlocks; ++b) {
+      s0 += it.dual_part[(side * kDualBlocks + b) * 2 + 0];
+      s1 += it.dual_part[(side * kDualBlocks + b) * 2 + 1];
+    }
+    o[side * 2 + 0] = s0;
+    o[side * 2 + 1] = s1;
+  }
+  o[6] = outer > 0 ? static_cast<double>(it.errors[outer - 1]) : 0.0;
+  o[7] = it.scal->eps;
+  o[8] = it.state->total_mass;
+  o[9] = static_cast<double>(it.state->n_blocks);
+}
+
+// g <- g_save for every problem: grid (x blocks, K).
+__global__ void batch_copy_g_kernel(const BatchItem* __restrict__ items) {
+  const BatchItem& it = items[blockIdx.y];
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < it.m) it.g[i] = it.g_save[i];
+}
+
+// Host plan of a batched solve: one kernel configuration for all problems,
+// each problem padded to a multiple of ``pad`` rows and given its own layout.
+struct BatchPlan {
+  KernelCfg k;
+  int64_t pad = 0;
+  int K = 0;
+  int outer = 0;
+  std::vector<ottx_problem> pbs;
+  std::vector<PassPlan> gp, fp;
+  std::vector<Layout> L;
+  std::vector<size_t> base;  // offset of each problem's layout
+  size_t off_done = 0, off_items = 0, off_args_g = 0, off_args_f = 0, off_work_g = 0, off_work_f = 0,
+         off_out = 0, total = 0;
+  int64_t n_work_g = 0, n_work_f = 0;
+  int64_t max_np = 0;
+};
+
+int64_t round_up(int64_t v, int64_t q) { return (v + q - 1) / q * q; }
+
+// Rows of the input side one workgroup streams: sized so the whole batch
+// makes about ``target`` workgroups per half-step direction.
+int batch_rows_per_split(const KernelCfg& k, const BatchPlan& B, bool g_dir, int target) {
+  double pairs = 0.0;
+  int64_t max_in = 0;
+  for (int i = 0; i < B.K; ++i) {
+    const int64_t n_pad = pad_rows(B.pbs[i].n, B.pad), m_pad = pad_rows(B.pbs[i].m, B.pad);
+    pairs += static_cast<double>(n_pad) * static_cast<double>(m_pad);
+    max_in = std::max(max_in, g_dir ? n_pad : m_pad);
+  }
+  const int64_t unit = 16 * k.mt;
+  int64_t rows = static_cast<int64_t>(pairs / (static_cast<double>(k.cols()) * target));
+  rows = std::max<int64_t>(unit, round_up(std::max<int64_t>(rows, 1), unit));
+  return static_cast<int>(std::min<int64_t>(rows, round_up(max_in, unit)));
+}
+
+int check_batch(const ottx_batch* bt) {
+  // count <= 65535: problems index the y and z grid dimensions of the batched kernels
+  if (!bt || bt->count < 1 || bt->count > 65535 || bt->d < 1 || bt->d > 256) return OTTX_EINVAL;
+  if (!bt->x || !bt->y || !bt->a || !bt->b || !bt->x_off || !bt->y_off) return OTTX_EINVAL;
+  if (bt->precision != OTTX_PREC_FP32 && bt->precision != OTTX_PREC_BF16_SPLIT) return OTTX_EINVAL;
+  if (!(bt->inv_scale_cost > 0.0) || bt->relative_epsilon < 0.0 || bt->mean_rows_x < 0.0 || bt->mean_rows_y < 0.0)
+    return OTTX_EINVAL;
+  if (!(bt->relative_epsilon > 0.0) && !bt->epsilon) return OTTX_EINVAL;
+  if (!(bt->tau_a > 0.0 && bt->tau_a <= 1.0) || !(bt->tau_b > 0.0 && bt->tau_b <= 1.0)) return OTTX_EINVAL;
+  if (bt->x_off[0] != 0 || bt->y_off[0] != 0) return OTTX_EINVAL;
+  for (int i = 0; i < bt->count; ++i) {
+    const int64_t n = bt->x_off[i + 1] - bt->x_off[i], m = bt->y_off[i + 1] - bt->y_off[i];
+    if (n < 1 || m < 1 || n > (int64_t{1} << 30) || m > (int64_t{1} << 30)) return OTTX_EINVAL;
+    if ((bt->mean_rows_x > 0.0 && bt->mean_rows_x < static_cast<double>(n)) ||
+        (bt->mean_rows_y > 0.0 && bt->mean_rows_y < static_cast<double>(m)))
+      return OTTX_EINVAL;
+    if (!(bt->relative_epsilon > 0.0) && !(bt->epsilon[i] > 0.0)) return OTTX_EINVAL;
+  }
+  return OTTX_OK;
+}
+
+int check_opts(const ottx_sinkhorn_opts* o) {
+  if (!o) return OTTX_EINVAL;
+  if (o->inner_iterations < 1 || o->max_iterations < 0 || o->min_iterations < 0 || o->norm_error < 1)
+    return OTTX_EINVAL;
+  if (!(o->momentum > 0.0 && o->momentum < 2.0)) return OTTX_EINVAL;
+  return OTTX_OK;
+}
+
+// Kernel choice for a batch: the variant with the smallest row padding (the
+// least waste on small problems) unless ``tile_config`` names one.
+bool batch_kernel(int prec, int d, int tile_config, KernelCfg* k, int* target_mult) {
+  KernelCfg v[8];
+  const int n = kernel_variants(prec, d, v, 8);
+  if (n <= 0) return false;
+  if (tile_config >= 0) return select_kernel(prec, d, tile_config, k, target_mult);
+  int best = 0;
+  for (int i = 1; i < n; ++i) {
+    const int pi = std::max(v[i].cols(), 16 * v[i].mt), pb = std::max(v[best].cols(), 16 * v[best].mt);
+    if (pi < pb) best = i;
+  }
+  *k = v[best];
+  *target_mult = 8;
+  return true;
+}
+
+int make_batch_plan(const ottx_batch* bt, const ottx_sinkhorn_opts* o, BatchPlan* B) {
+  int tm = 8;
+  if (!batch_kernel(bt->precision, bt->d, o->tile_config, &B->k, &tm))
+    return o->tile_config >= 0 ? OTTX_EINVAL : OTTX_EUNSUPPORTED;
+  const KernelCfg& k = B->k;
+  B->pad = std::max<int64_t>(k.cols(), 16 * k.mt);
+  B->K = bt->count;
+  B->outer = (o->max_iterations + o->inner_iterations - 1) / o->inner_iterations;
+  B->pbs.resize(B->K);
+  for (int i = 0; i < B->K; ++i) {
+    ottx_problem& p = B->pbs[i];
+    p = ottx_problem{};
+    p.n = bt->x_off[i + 1] - bt->x_off[i];
+    p.m = bt->y_off[i + 1] - bt->y_off[i];
+    p.d = bt->d;
+    p.precision = bt->precision;
+    p.x = bt->x + bt->x_off[i] * bt->d;
+    p.y = bt->y + bt->y_off[i] * bt->d;
+    p.a = bt->a + bt->x_off[i];
+    p.b = bt->b + bt->y_off[i];
+    p.epsilon = bt->epsilon ? bt->epsilon[i] : 0.0;
+    p.relative_epsilon = bt->relative_epsilon;
+    p.inv_scale_cost = bt->inv_scale_cost;
+    p.tau_a = bt->tau_a;
+    p.tau_b = bt->tau_b;
+    B->max_np = std::max(B->max_np, std::max(p.n, p.m));
+  }
+  const int target = tm * device_cus();
+  const int rows_g = batch_rows_per_split(k, *B, true, target);
+  const int rows_f = batch_rows_per_split(k, *B, false, target);
+  B->gp.resize(B->K);
+  B->fp.resize(B->K);
+  B->L.resize(B->K);
+  B->base.resize(B->K);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align_up(off + bytes);
+    return at;
+  };
+  B->off_done = take(256);
+  B->off_items = take(sizeof(BatchItem) * B->K);
+  B->off_args_g = take(sizeof(HalfStepArgs) * B->K);
+  B->off_args_f = take(sizeof(HalfStepArgs) * B->K);
+  B->n_work_g = B->n_work_f = 0;
+  for (int i = 0; i < B->K; ++i) {
+    for (int dir = 0; dir < 2; ++dir) {
+      const int64_t n_in = dir == 0 ? B->pbs[i].n : B->pbs[i].m;
+      const int64_t n_out = dir == 0 ? B->pbs[i].m : B->pbs[i].n;
+      PassPlan p;
+      p.tiles = static_cast<int>(pad_rows(n_out, B->pad) / k.cols());
+      p.n_in_rb = static_cast<int>(pad_rows(n_in, B->pad) / 16);
+      p.rb_per_split = (dir == 0 ? rows_g : rows_f) / 16;
+      p.splits = (p.n_in_rb + p.rb_per_split - 1) / p.rb_per_split;
+      (dir == 0 ? B->gp[i] : B->fp[i]) = p;
+      (dir == 0 ? B->n_work_g : B->n_work_f) += static_cast<int64_t>(p.tiles) * p.splits;
+    }
+  }
+  if (B->n_work_g > (int64_t{1} << 31) - 1 || B->n_work_f > (int64_t{1} << 31) - 1) return OTTX_EINVAL;
+  B->off_work_g = take(sizeof(int4) * B->n_work_g);
+  B->off_work_f = take(sizeof(int4) * B->n_work_f);
+  B->off_out = take(sizeof(double) * 16 * B->K);
+  for (int i = 0; i < B->K; ++i) {
+    B->L[i] = make_layout(&B->pbs[i], k, B->gp[i], B->fp[i], 1, B->pad);
+    B->base[i] = take(B->L[i].total);
+  }
+  B->total = off;
+  return OTTX_OK;
+}
+
+// Pointers of one problem's layout inside the workspace (no launches).
+Prepared prepared_view(char* base, const Layout& L) {
+  Prepared P;
+  P.base = base;
+  P.L = L;
+  P.frag_x = base + L.frag_x;
+  P.frag_y = base + L.frag_y;
+  P.norm_x = reinterpret_cast<float*>(base + L.norm_x);
+  P.norm_y = reinterpret_cast<float*>(base + L.norm_y);
+  P.mu = reinterpret_cast<float*>(base + L.mu);
+  P.stats_out = reinterpret_cast<double*>(base + L.stats_out);
+  P.scal = reinterpret_cast<SolveScalars*>(base + L.scal);
+  return P;
+}
+
+// Pinned host staging for the batch tables, kept per thread and grown on demand.
+struct HostStaging {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+int host_staging(size_t bytes, void** out) {
+  static thread_local HostStaging h;
+  if (h.bytes < bytes) {
+    if (h.ptr) (void)hipHostFree(h.ptr);
+    h.ptr = nullptr;
+    h.bytes = 0;
+    OTTX_HIP(hipHostMalloc(&h.ptr, bytes, hipHostMallocDefault));
+    h.bytes = bytes;
+  }
+  *out = h.ptr;
+  return OTTX_OK;
+}
+
+}  // namespace ottx
+
+extern "C" size_t ottx_sinkhorn_batched_workspace_bytes(const ottx_batch* bt, const ottx_sinkhorn_opts* o) {
+  if (check_batch(bt) != OTTX_OK || check_opts(o) != OTTX_OK) return 0;
+  BatchPlan B;
+  if (make_batch_plan(bt, o, &B) != OTTX_OK) return 0;
+  return B.total;
+}
+
+extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sinkhorn_opts* o, ottx_batch_out* out,
+                                           void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_batch(bt);
+  if (rc != OTTX_OK) return rc;
+  if ((rc = check_opts(o)) != OTTX_OK) return rc;
+  if (!out || !out->f || !out->g || !out->errors || !out->reg_ot_cost || !out->converged || !out->n_iters ||
+      !workspace)
+    return OTTX_EINVAL;
+  if (o->init_from_g) return OTTX_EUNSUPPORTED;
+  const int inner = o->inner_iterations;
+  const int outer = (o->max_iterations + inner - 1) / inner;
+  if (outer < 1) return OTTX_EUNSUPPORTED;
+  BatchPlan B;
+  if ((rc = make_batch_plan(bt, o, &B)) != OTTX_OK) return rc;
+  if (workspace_bytes < B.total) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  const KernelCfg& k = B.k;
+  const int K = B.K;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HostHelpers* hh = nullptr;
+  if ((rc = host_helpers(&hh)) != OTTX_OK) return rc;
+  char* ws = static_cast<char*>(workspace);
+  const bool force_scan = o->min_iterations == o->max_iterations;
+  const int total_blocks = force_scan ? o->max_iterations / inner : outer;
+  const bool balanced = (bt->tau_a == 1.0 && bt->tau_b == 1.0);
+  uint32_t* batch_done = reinterpret_cast<uint32_t*>(ws + B.off_done);
+
+  // Host tables (items, per-problem arguments of both directions, work lists)
+  // are staged in pinned memory with the same offsets they have in the
+  // workspace, so one copy moves them.  The staging buffer is free whenever no
+  // call is running on this thread: every call ends with a stream sync after
+  // its last copy.
+  const size_t tbl_bytes = B.off_out - B.off_items;
+  void* stage = nullptr;
+  if ((rc = host_staging(std::max(tbl_bytes, sizeof(double) * 16 * K), &stage)) != OTTX_OK) return rc;
+  char* sb = static_cast<char*>(stage);
+  BatchItem* items = reinterpret_cast<BatchItem*>(sb);
+  HalfStepArgs* args_g = reinterpret_cast<HalfStepArgs*>(sb + (B.off_args_g - B.off_items));
+  HalfStepArgs* args_f = reinterpret_cast<HalfStepArgs*>(sb + (B.off_args_f - B.off_items));
+  int4* work_g = reinterpret_cast<int4*>(sb + (B.off_work_g - B.off_items));
+  int4* work_f = reinterpret_cast<int4*>(sb + (B.off_work_f - B.off_items));
+  int64_t wg = 0, wf = 0;
+  for (int i = 0; i < K; ++i) {
+    const ottx_problem& pb = B.pbs[i];
+    const Layout& L = B.L[i];
+    char* base = ws + B.base[i];
+    const Prepared P = prepared_view(base, L);
+    BatchItem& it = items[i];
+    it = BatchItem{};
+    it.x = pb.x;
+    it.y = pb.y;
+    it.a = pb.a;
+    it.b = pb.b;
+    it.f = out->f + bt->x_off[i];
+    it.g = out->g + bt->y_off[i];
+    it.g_save = reinterpret_cast<float*>(base + L.g_save);
+    it.errors = out->errors + static_cast<int64_t>(i) * outer;
+    it.n = pb.n;
+    it.m = pb.m;
+    it.n_pad = pad_rows(pb.n, B.pad);
+    it.m_pad = pad_rows(pb.m, B.pad);
+    it.n_div = bt->mean_rows_x > 0.0 ? bt->mean_rows_x : static_cast<double>(pb.n);
+    it.m_div = bt->mean_rows_y > 0.0 ? bt->mean_rows_y : static_cast<double>(pb.m);
+    it.eps_abs = pb.epsilon;
+    it.frag_x = P.frag_x;
+    it.frag_y = P.frag_y;
+    it.norm_x = P.norm_x;
+    it.norm_y = P.norm_y;
+    it.mu = P.mu;
+    it.scal = P.scal;
+    it.state = reinterpret_cast<SolveState*>(base + L.state);
+    it.counters = reinterpret_cast<uint32_t*>(base + L.counters);
+    it.dual_part = reinterpret_cast<double*>(base + L.dual_part);
+    it.n_counters = std::max(B.gp[i].tiles, B.fp[i].tiles);
+    const double eps_host = pb.epsilon > 0.0 ? pb.epsilon : 1.0;  // replaced on the device by scal
+    HalfStepArgs ga = base_args(&pb, k, B.gp[i], P, 1, it.f, it.g, eps_host);
+    HalfStepArgs fa = base_args(&pb, k, B.fp[i], P, 0, it.g, it.f, eps_host);
+    for (HalfStepArgs* a : {&ga, &fa}) {
+      a->norm_p = o->norm_error;
+      a->momentum = static_cast<float>(o->momentum);
+      a->errors = it.errors;
+      a->row_err_partial = reinterpret_cast<const double*>(base + L.err_row);
+      a->row_tiles = B.fp[i].tiles;
+      a->inner = inner;
+      a->min_it = o->min_iterations;
+      a->max_it = o->max_iterations;
+      a->outer = outer;
+      a->force_scan = force_scan ? 1 : 0;
+      a->threshold = static_cast<float>(o->threshold);
+      a->batch_done = batch_done;
+    }
+    ga.out_save = it.g_save;
+    ga.lse_shift = ga.lse_store = reinterpret_cast<float*>(base + L.lse_y);
+    fa.lse_shift = fa.lse_store = reinterpret_cast<float*>(base + L.lse_x);
+    args_g[i] = ga;
+    args_f[i] = fa;
+    for (int t = 0; t < B.gp[i].tiles; ++t)
+      for (int sp = 0; sp < B.gp[i].splits; ++sp) work_g[wg++] = make_int4(i, t, sp, 0);
+    for (int t = 0; t < B.fp[i].tiles; ++t)
+      for (int sp = 0; sp < B.fp[i].splits; ++sp) work_f[wf++] = make_int4(i, t, sp, 0);
+  }
+  BatchItem* d_items = reinterpret_cast<BatchItem*>(ws + B.off_items);
+  HalfStepArgs* d_args_g = reinterpret_cast<HalfStepArgs*>(ws + B.off_args_g);
+  HalfStepArgs* d_args_f = reinterpret_cast<HalfStepArgs*>(ws + B.off_args_f);
+  int4* d_work_g = reinterpret_cast<int4*>(ws + B.off_work_g);
+  int4* d_work_f = reinterpret_cast<int4*>(ws + B.off_work_f);
+  OTTX_HIP(hipMemcpyAsync(d_items, stage, tbl_bytes, hipMemcpyHostToDevice, st));
+
+  // Preparation of every problem.
+  const double s = bt->inv_scale_cost;
+  hipLaunchKernelGGL(batch_stats_kernel, dim3(K), dim3(256), 0, st, d_items, bt->d, s, bt->relative_epsilon,
+                     bt->tau_a, bt->tau_b);
+  const int64_t max_pad = pad_rows(B.max_np, B.pad);
+  const int chunks = k.frag_chunks();
+  const int64_t pack_threads = (max_pad / 16) * chunks * 64;
+  hipLaunchKernelGGL(batch_pack_kernel, dim3(static_cast<unsigned>((pack_threads + 255) / 256), K, 2), dim3(256),
+                     0, st, d_items, bt->d, chunks, k.prec);
+  hipLaunchKernelGGL(batch_norms_kernel, dim3(static_cast<unsigned>((max_pad + 255) / 256), K, 2), dim3(256), 0,
+                     st, d_items, bt->d, s);
+  hipLaunchKernelGGL(batch_init_kernel, dim3(K), dim3(256), 0, st, d_items, outer, batch_done);
+  OTTX_LAUNCHED();
+
+  const uint32_t g_plain = kModeWritePot | kModeCheckDone;
+  const uint32_t g_err = kModeWritePot | kModeCheckDone | kModeSaveOld | kModeErrCol;
+  const uint32_t f_plain = kModeWritePot | kModeCheckDone;
+  const uint32_t f_err = kModeWritePot | kModeCheckDone | (balanced ? 0u : kModeErrRow);
+  const dim3 block(64 * k.w);
+  auto launch = [&](bool g_dir, uint32_t mode, hipStream_t s2) -> int {
+    hipLaunchKernelGGL(k.bfn, dim3(static_cast<unsigned>(g_dir ? B.n_work_g : B.n_work_f)), block, 0, s2,
+                       g_dir ? d_args_g : d_args_f, g_dir ? d_work_g : d_work_f, mode);
+    OTTX_LAUNCHED();
+    return OTTX_OK;
+  };
+  auto launch_block = [&](hipStream_t s2) -> int {
+    for (int itn = 0; itn < inner; ++itn) {
+      int r = launch(false, itn == inner - 1 ? f_err : f_plain, s2);
+      if (r != OTTX_OK) return r;
+      r = launch(true, itn == inner - 1 ? g_err : g_plain, s2);
+      if (r != OTTX_OK) return r;
+    }
+    return OTTX_OK;
+  };
+
+  if ((rc = launch(true, g_plain, st)) != OTTX_OK) return rc;  // g-update of iteration 0
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipStream_t cap = nullptr;
+  if (o->use_graph) {
+    OTTX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    OTTX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    rc = launch_block(cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e1 = hipStreamEndCapture(cap, &g);
+    if (rc != OTTX_OK || e1 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipStreamDestroy(cap);
+      return rc != OTTX_OK ? rc : OTTX_EHIP;
+    }
+    graph = g;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      (void)hipStreamDestroy(cap);
+      return OTTX_EHIP;
+    }
+  }
+  hipEvent_t ev[2];
+  OTTX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  OTTX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  uint32_t* mail = static_cast<uint32_t*>(hh->mailbox);
+  int status = OTTX_OK;
+  int launched = 0;
+  for (int blk = 0; blk < total_blocks; ++blk) {
+    if (exec) {
+      if (hipGraphLaunch(exec, st) != hipSuccess) { status = OTTX_EHIP; break; }
+    } else {
+      status = launch_block(st);
+      if (status != OTTX_OK) break;
+    }
+    ++launched;
+    if (hipEventRecord(ev[blk & 1], st) != hipSuccess) { status = OTTX_EHIP; break; }
+    if (blk >= 1) {
+      if (hipEventSynchronize(ev[(blk - 1) & 1]) != hipSuccess) { status = OTTX_EHIP; break; }
+      if (hipMemcpyAsync(mail, batch_done, sizeof(uint32_t), hipMemcpyDeviceToHost, hh->side) != hipSuccess ||
+          hipStreamSynchronize(hh->side) != hipSuccess) {
+        status = OTTX_EHIP;
+        break;
+      }
+      if (mail[0] >= static_cast<uint32_t>(K)) break;
+    }
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (cap) (void)hipStreamDestroy(cap);
+  if (status != OTTX_OK) return status;
+
+  double* d_out = reinterpret_cast<double*>(ws + B.off_out);
+  hipLaunchKernelGGL(batch_dual_partial_kernel, dim3(kDualBlocks, 3, K), dim3(256), 0, st, d_items, bt->tau_a,
+                     bt->tau_b, outer);
+  hipLaunchKernelGGL(batch_dual_final_kernel, dim3(K), dim3(64), 0, st, d_items, outer, d_out);
+  hipLaunchKernelGGL(batch_copy_g_kernel, dim3(static_cast<unsigned>((B.max_np + 255) / 256), K), dim3(256), 0, st,
+                     d_items);
+  OTTX_LAUNCHED();
+  // The tables' copy precedes these launches on the stream, so the staging
+  // buffer can take the results.
+  void* res = stage;
+  OTTX_HIP(hipMemcpyAsync(res, d_out, sizeof(double) * 16 * K, hipMemcpyDeviceToHost, st));
+  OTTX_HIP(hipStreamSynchronize(st));
+  const double* du = static_cast<const double*>(res);
+  for (int i = 0; i < K; ++i) {
+    const double* q = du + 16 * i;
+    out->reg_ot_cost[i] = q[0] + q[2] + q[7] * (q[1] * q[3] - q[8]);
+    if (out->epsilon) out->epsilon[i] = q[7];
+    const bool any_nan = q[4] > 0.0;
+    const float last = static_cast<float>(q[6]);
+    out->converged[i] = (!any_nan && last < static_cast<float>(o->threshold)) ? 1 : 0;
+    out->n_iters[i] = static_cast<int32_t>(q[5]) * inner;
+  }
+  out->n_blocks = launched;
   return OTTX_OK;
 }

@@ -158,6 +158,14 @@ class Sinkhorn:
     return int(math.ceil(self.max_iterations / self.inner_iterations))
 
   def _check_supported(self, ot_prob: linear_problem.LinearProblem) -> None:
+    self._check_options()
+    if not isinstance(ot_prob.geom, PointCloud):
+      raise NotImplementedError("ott_amd accelerates PointCloud geometries only")
+    if not ot_prob.geom._schedule_is_target_only():
+      raise NotImplementedError("epsilon schedules with init > 1 are not supported yet")
+
+  def _check_options(self) -> None:
+    """Solver options the HIP path runs; the others raise NotImplementedError."""
     if not self.lse_mode:
       raise NotImplementedError("ott_amd runs the log-domain (lse_mode=True) solver only")
     if self.anderson is not None:
@@ -170,10 +178,6 @@ class Sinkhorn:
       raise NotImplementedError("adaptive momentum is not supported by ott_amd yet")
     if self.progress_fn is not None:
       raise NotImplementedError("progress_fn is not supported by ott_amd yet")
-    if not isinstance(ot_prob.geom, PointCloud):
-      raise NotImplementedError("ott_amd accelerates PointCloud geometries only")
-    if not ot_prob.geom._schedule_is_target_only():
-      raise NotImplementedError("epsilon schedules with init > 1 are not supported yet")
     if int(self._norm_error) < 1:
       raise ValueError("norm_error must be >= 1")
 

@@ -45,6 +45,34 @@ def test_struct_layouts_match_header():
   assert ctypes.sizeof(_lib.Problem) == 8 + 8 + 4 + 4 + 4 * 8 + 5 * 8
   assert ctypes.sizeof(_lib.SinkhornOpts) == 48
   assert ctypes.sizeof(_lib.SinkhornOut) == 3 * 8 + 3 * 8 + 4 * 4
+  assert ctypes.sizeof(_lib.Batch) == 4 * 4 + 4 * 8 + 3 * 8 + 6 * 8
+  assert ctypes.sizeof(_lib.BatchOut) == 3 * 8 + 4 * 8 + 2 * 4
+
+
+def test_batched_workspace_query_rejects_bad_batches():
+  from ott_amd import _lib
+  lib = _lib.load_library()
+  x_off = (ctypes.c_int64 * 3)(0, 5, 12)
+  y_off = (ctypes.c_int64 * 3)(0, 7, 9)
+  eps = (ctypes.c_double * 2)(0.1, 0.2)
+  dummy = 256
+  def batch(**kw):
+    args = dict(count=2, d=3, precision=0, x=dummy, y=dummy, a=dummy, b=dummy, x_off=x_off, y_off=y_off,
+                epsilon=eps, relative_epsilon=0.0, mean_rows_x=0.0, mean_rows_y=0.0, inv_scale_cost=1.0,
+                tau_a=1.0, tau_b=1.0)
+    args.update(kw)
+    return _lib.Batch(**args)
+  opts = _lib.SinkhornOpts(threshold=1e-3, norm_error=1, inner_iterations=10, min_iterations=0,
+                           max_iterations=100, momentum=1.0, tile_config=-1, use_graph=1)
+  assert lib.ottx_sinkhorn_batched_workspace_bytes(ctypes.byref(batch()), ctypes.byref(opts)) > 0
+  bad_off = (ctypes.c_int64 * 3)(0, 5, 5)  # an empty problem
+  for kw in (dict(count=0), dict(d=0), dict(x_off=bad_off), dict(epsilon=None), dict(mean_rows_x=4.0),
+             dict(tau_a=0.0), dict(precision=7)):
+    assert lib.ottx_sinkhorn_batched_workspace_bytes(ctypes.byref(batch(**kw)), ctypes.byref(opts)) == 0, kw
+  # relative epsilon needs no per-problem epsilons
+  assert lib.ottx_sinkhorn_batched_workspace_bytes(
+      ctypes.byref(batch(epsilon=None, relative_epsilon=0.05, mean_rows_x=7.0, mean_rows_y=7.0)),
+      ctypes.byref(opts)) > 0
 
 
 def test_invalid_arguments_are_rejected_without_gpu_work():
