@@ -41,6 +41,15 @@ __device__ __forceinline__ void lse_merge(float& mx, float& sm, float om, float 
   sm = s;
 }
 
+// Cross-workgroup hand-off without L2 invalidation.  On gfx950 an agent-scope
+// __threadfence() is buffer_wbl2 + buffer_inv: the invalidate drops every
+// cached line of the XCD's L2, which the other workgroups streaming through
+// that L2 then refetch.  Writers here only need a release (write back their
+// dirty lines before the arrival counter's atomic); readers take the data with
+// agent-coherent (sc1) loads, which bypass a stale L2, after the counter's
+// atomic returned.
+__device__ __forceinline__ void release_to_device() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+
 __device__ __forceinline__ double load_f64_coherent(const double* p) {
   const uint64_t v = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
@@ -510,7 +519,7 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
     const int64_t stride = static_cast<int64_t>(p.tiles) * COLS;
     float2* part = p.partial + static_cast<int64_t>(split) * stride;
     for (int t = tid; t < COLS; t += NT) part[col0 + t] = s_ms[t];
-    __threadfence();
+    release_to_device();
     __syncthreads();
     if (tid == 0) {
       const uint32_t prev = atomicAdd(p.tile_counter + tile, 1u);
@@ -518,7 +527,6 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
     }
     __syncthreads();
     if (!s_last) return;
-    __threadfence();
     for (int t = tid; t < COLS; t += NT) {
       float M = -INFINITY, S = 0.f;
       for (int sp = 0; sp < p.splits; ++sp) {
@@ -555,12 +563,10 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
       if (p.mass_partial) p.mass_partial[tile] = s_red[1][0];
     }
     if ((p.mode & kModeErrCol) && p.state != nullptr) {
-      __threadfence();
-      __syncthreads();
       if (tid == 0) {
+        release_to_device();
         const uint32_t prev = atomicAdd(&p.state->tiles_done, 1u);
         if (prev == static_cast<uint32_t>(p.tiles - 1)) {
-          __threadfence();
           finalize_block(p);
           p.state->tiles_done = 0u;
         }

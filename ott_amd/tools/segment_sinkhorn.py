@@ -63,8 +63,13 @@ def segment_sinkhorn_outputs(
     indices_are_sorted: bool = False, num_per_segment_x: Optional[Sequence[int]] = None,
     num_per_segment_y: Optional[Sequence[int]] = None, weights_x=None, weights_y=None,
     sinkhorn_kwargs: Mapping[str, Any] = MappingProxyType({}), tau_a: float = 1.0, tau_b: float = 1.0,
-    **kwargs: Any) -> SegmentSinkhornOutput:
-  """All outputs of the batched solve (see ``segment_sinkhorn``)."""
+    padded_statistics: bool = True, **kwargs: Any) -> SegmentSinkhornOutput:
+  """All outputs of the batched solve (see ``segment_sinkhorn``).
+
+  ``padded_statistics=True`` (OTT's behaviour) computes a relative epsilon
+  over each segment padded to ``max_measure_size`` rows; ``False`` uses each
+  segment's own points, i.e. what one ``PointCloud`` per segment would give.
+  """
   if cost_fn is not None and not isinstance(cost_fn, costs.SqEuclidean):
     raise NotImplementedError("ott_amd accelerates the squared-Euclidean cost only")
   precision = kwargs.pop("precision", "fp32")
@@ -123,8 +128,8 @@ def segment_sinkhorn_outputs(
   bt = _lib.Batch(count=K, d=x.shape[1], precision=_lib.PRECISIONS[precision], x=xs.data_ptr(),
                   y=ys.data_ptr(), a=a.data_ptr(), b=b.data_ptr(), x_off=x_off_c, y_off=y_off_c,
                   epsilon=eps_c, relative_epsilon=eps_rel,
-                  mean_rows_x=float(px) if eps_rel > 0 else 0.0,
-                  mean_rows_y=float(py) if eps_rel > 0 else 0.0,
+                  mean_rows_x=float(px) if eps_rel > 0 and padded_statistics else 0.0,
+                  mean_rows_y=float(py) if eps_rel > 0 and padded_statistics else 0.0,
                   inv_scale_cost=1.0 / float(scale_cost), tau_a=float(tau_a), tau_b=float(tau_b))
   opts = _lib.SinkhornOpts(
       threshold=float(solver.threshold), norm_error=int(solver._norm_error),

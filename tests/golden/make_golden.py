@@ -5,7 +5,7 @@ golden values for this path, so these fixtures are oracle outputs on seeded
 inputs (regenerated from ``ott_amd.datasets``): they let GPU tests and the
 benchmark check full-size results without re-running a minutes-long FP64 solve.
 
-  python tests/golden/make_golden.py [uniform_d2] [gaussian_d64] [mixture_d128]
+  python tests/golden/make_golden.py [uniform_d2] [gaussian_d64] [mixture_d128] [batched_d32]
 """
 
 from __future__ import annotations
@@ -67,6 +67,19 @@ def main(which):
     rec, out = _run(pb, 2000)
     json.dump(rec, open(os.path.join(HERE, "mixture_pair_n32768_m8192_d128.json"), "w"), indent=1)
     print(rec["name"], rec["n_iters"], rec["reg_ot_cost"], rec["oracle_wall_s"], flush=True)
+  if "batched_d32" in which:
+    # 512 uneven problems; each solved alone, error checked every iteration
+    probs = datasets.batched_uneven()
+    recs = []
+    t0 = time.time()
+    for pb in probs:
+      rec, _ = _run(pb, 2000, inner_iterations=1)
+      recs.append(dict(n=rec["n"], m=rec["m"], epsilon=rec["epsilon"], reg_ot_cost=rec["reg_ot_cost"],
+                       n_iters=rec["n_iters"], converged=rec["converged"]))
+    json.dump(dict(name="batched_uneven_512_d32", threshold=1e-3, inner_iterations=1, problems=recs,
+                   oracle_wall_s=time.time() - t0),
+              open(os.path.join(HERE, "batched_uneven_512_d32.json"), "w"))
+    print("batched_uneven_512_d32", sum(r["n_iters"] for r in recs), time.time() - t0, flush=True)
 
 
 if __name__ == "__main__":
