@@ -156,24 +156,31 @@ struct FragChunks {
 };
 
 // Loads one streamed tile (contiguous fragment chunks of MT row blocks) and
-// its row biases into registers.
+// the raw potential and norm of its rows into registers.  Every thread issues
+// the same loads unconditionally (row indices are clamped, tiles past the end
+// of the split are clamped by the caller): the compiler counts outstanding
+// loads per wave, and a load skipped by a branch would make it fall back to
+// waiting for all of them (vmcnt(0)), which drains the prefetch.  For the same
+// reason the row bias (h_i - norm_i) / (2 s) is only formed when the tile is
+// copied to LDS.
 template <int TILE_U4, int NT, int PER_T, int ROWS, int CH>
 __device__ __forceinline__ void tile_prefetch(const u32x4* __restrict__ src, const float* __restrict__ in_pot,
-                                              const float* __restrict__ in_norm, int64_t n_in, int rb, float inv2s,
-                                              int tid, u32x4 (&pre)[PER_T], float& pre_bias) {
+                                              const float* __restrict__ in_norm, int64_t n_in, int rb, int tid,
+                                              u32x4 (&pre)[PER_T], float2& pre_row) {
   const u32x4* base = src + static_cast<size_t>(rb) * CH * 64;
 #pragma unroll
-  for (int q = 0; q < PER_T; ++q) {
-    const int idx = tid + q * NT;
-    if (TILE_U4 % NT == 0 || idx < TILE_U4) pre[q] = base[idx];
-  }
-  if (tid < ROWS) {
-    const int64_t i = static_cast<int64_t>(rb) * 16 + tid;
-    pre_bias = (i < n_in) ? (in_pot[i] - in_norm[i]) * inv2s : -INFINITY;
-  }
+  for (int q = 0; q < PER_T; ++q) pre[q] = base[min(tid + q * NT, TILE_U4 - 1)];
+  const int64_t i = min(static_cast<int64_t>(rb) * 16 + (tid & (ROWS - 1)), n_in - 1);
+  pre_row.x = in_pot[i];
+  pre_row.y = in_norm[i];
 }
 
-// Streams in-side row blocks [rb_begin, rb_end) (multiples of MT) against the
+// Bias of row rb * 16 + tid (tid < ROWS) from its prefetched potential and
+// norm; rows past n_in get -inf so they add nothing.
+__device__ __forceinline__ float row_bias(float2 r, int64_t row, int64_t n_in, float inv2s) {
+  return row < n_in ? (r.x - r.y) * inv2s : -INFINITY;
+}
+
 // NB column blocks held in B, updating each lane's running (max, sum) of
 // exp2(c * acc) per column.  in_frag / in_pot / in_norm are relative to the
 // first row of the in side; rows i >= n_in get a -inf bias.
@@ -301,7 +308,9 @@ __device__ __forceinline__ void compute_tile(const OutOperand<PREC, KK, NB>& B, 
 // blocks (contiguous in the fragment layout) and its row biases are copied
 // once into LDS and every wave reads its A operands from there.  Two tiles are
 // in flight in registers ahead of the one on MFMA (prefetch distance 2) and
-// LDS is double-buffered, with one barrier per tile.
+// LDS is double-buffered, with one barrier per tile.  [rb_begin, rb_end)
+// holds an even number of tiles (the planners guarantee it), so each loop
+// trip computes two tiles without a branch in between.
 template <int PREC, int KK, int NB, int MT, int W, bool SHIFT>
 __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, const void* in_frag,
                                             const float* __restrict__ in_pot, const float* __restrict__ in_norm,
@@ -313,43 +322,37 @@ __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, c
   constexpr int TILE_U4 = MT * CH * 64;
   constexpr int PER_T = (TILE_U4 + NT - 1) / NT;
   constexpr int ROWS = MT * 16;
+  static_assert((ROWS & (ROWS - 1)) == 0 && ROWS <= NT, "MT * 16 rows must be a power of two");
   const int lane = tid & 63;
   const u32x4* __restrict__ src = static_cast<const u32x4*>(in_frag);
   u32x4* L0 = s_tile;
   u32x4* L1 = s_tile + TILE_U4;
   float* B0 = s_bias;
   float* B1 = s_bias + ROWS;
+  const int rb_last = rb_end - MT;  // start of the last tile; prefetches past it reload it
 
   u32x4 pre0[PER_T], pre1[PER_T];
-  float pb0 = -INFINITY, pb1 = -INFINITY;
-  if (rb_begin < rb_end)
-    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb_begin, inv2s, tid, pre0, pb0);
-  if (rb_begin + MT < rb_end)
-    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb_begin + MT, inv2s, tid, pre1, pb1);
+  float2 pb0, pb1;
+  tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb_begin, tid, pre0, pb0);
+  tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb_begin + MT, rb_last), tid, pre1,
+                                              pb1);
   for (int rb = rb_begin; rb < rb_end; rb += 2 * MT) {
 #pragma unroll
-    for (int q = 0; q < PER_T; ++q) {
-      const int idx = tid + q * NT;
-      if (TILE_U4 % NT == 0 || idx < TILE_U4) L0[idx] = pre0[q];
-    }
-    if (tid < ROWS) B0[tid] = pb0;
+    for (int q = 0; q < PER_T; ++q)
+      if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
+    if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb) * 16 + tid, n_in, inv2s);
     __syncthreads();
-    if (rb + 2 * MT < rb_end)
-      tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb + 2 * MT, inv2s, tid, pre0, pb0);
+    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 2 * MT, rb_last), tid, pre0,
+                                                pb0);
     compute_tile<PREC, KK, NB, MT, SHIFT>(B, L0, B0, lane, c, mx, sm);
-    if (rb + MT < rb_end) {
 #pragma unroll
-      for (int q = 0; q < PER_T; ++q) {
-        const int idx = tid + q * NT;
-        if (TILE_U4 % NT == 0 || idx < TILE_U4) L1[idx] = pre1[q];
-      }
-      if (tid < ROWS) B1[tid] = pb1;
-      __syncthreads();
-      if (rb + 3 * MT < rb_end)
-        tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb + 3 * MT, inv2s, tid, pre1,
-                                                    pb1);
-      compute_tile<PREC, KK, NB, MT, SHIFT>(B, L1, B1, lane, c, mx, sm);
-    }
+    for (int q = 0; q < PER_T; ++q)
+      if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L1[tid + q * NT] = pre1[q];
+    if (tid < ROWS) B1[tid] = row_bias(pb1, static_cast<int64_t>(rb + MT) * 16 + tid, n_in, inv2s);
+    __syncthreads();
+    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 3 * MT, rb_last), tid, pre1,
+                                                pb1);
+    compute_tile<PREC, KK, NB, MT, SHIFT>(B, L1, B1, lane, c, mx, sm);
   }
 }
 

@@ -428,14 +428,23 @@ PassPlan plan_pass(const KernelCfg& k, int64_t n_in, int64_t n_out, int cus, int
   const int in_tiles = p.n_in_rb / k.mt;
   const int target = target_mult * cus;
   int splits = (target + p.tiles - 1) / p.tiles;
-  splits = std::max(1, std::min(splits, in_tiles));
-  const int tiles_per_split = (in_tiles + splits - 1) / splits;
+  splits = std::max(1, std::min(splits, in_tiles / 2));
+  // an even number of streamed tiles per split (the stream loop computes two
+  // per trip); in_tiles is even since the row padding is a multiple of 32 MT
+  int tiles_per_split = (in_tiles + splits - 1) / splits;
+  tiles_per_split += tiles_per_split & 1;
   p.rb_per_split = tiles_per_split * k.mt;
   p.splits = (p.n_in_rb + p.rb_per_split - 1) / p.rb_per_split;
   return p;
 }
 
 size_t align_up(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+// The stream loop computes two tiles of MT row blocks per trip: every split
+// (and so the padded row count) must hold an even number of tiles.
+bool even_tiles(const KernelCfg& k, const PassPlan& p) {
+  return p.rb_per_split > 0 && p.rb_per_split % (2 * k.mt) == 0 && p.n_in_rb % (2 * k.mt) == 0;
+}
 
 struct Layout {
   size_t frag_x, frag_y, norm_x, norm_y, g_save, lse_x, lse_y, partial, counters, err_col, mass, err_row, state,
@@ -503,9 +512,10 @@ bool make_plan(const ottx_problem* pb, int tile_config, Plan* P) {
   const int cus = device_cus();
   // Every column tile (16 NB W points) and streamed tile (16 MT rows) must
   // divide the row padding, so tiles never straddle the end of an array.
-  if (kRowPad % P->k.cols() != 0 || kRowPad % (16 * P->k.mt) != 0) return false;
+  if (kRowPad % P->k.cols() != 0 || kRowPad % (32 * P->k.mt) != 0) return false;
   P->gp = plan_pass(P->k, pb->n, pb->m, cus, tm);
   P->fp = plan_pass(P->k, pb->m, pb->n, cus, tm);
+  if (!even_tiles(P->k, P->gp) || !even_tiles(P->k, P->fp)) return false;
   P->L = make_layout(pb, P->k, P->gp, P->fp, 1);
   return true;
 }
@@ -1320,7 +1330,7 @@ int batch_rows_per_split(const KernelCfg& k, const BatchPlan& B, bool g_dir, int
     pairs += static_cast<double>(n_pad) * static_cast<double>(m_pad);
     max_in = std::max(max_in, g_dir ? n_pad : m_pad);
   }
-  const int64_t unit = 16 * k.mt;
+  const int64_t unit = 32 * k.mt;  // an even number of streamed tiles per split
   int64_t rows = static_cast<int64_t>(pairs / (static_cast<double>(k.cols()) * target));
   rows = std::max<int64_t>(unit, round_up(std::max<int64_t>(rows, 1), unit));
   return static_cast<int>(std::min<int64_t>(rows, round_up(max_in, unit)));
@@ -1364,7 +1374,7 @@ bool batch_kernel(int prec, int d, int tile_config, KernelCfg* k, int* target_mu
   if (tile_config >= 0) return select_kernel(prec, d, tile_config, k, target_mult);
   int best = 0;
   for (int i = 1; i < n; ++i) {
-    const int pi = std::max(v[i].cols(), 16 * v[i].mt), pb = std::max(v[best].cols(), 16 * v[best].mt);
+    const int pi = std::max(v[i].cols(), 32 * v[i].mt), pb = std::max(v[best].cols(), 32 * v[best].mt);
     if (pi < pb) best = i;
   }
   *k = v[best];
@@ -1377,7 +1387,7 @@ int make_batch_plan(const ottx_batch* bt, const ottx_sinkhorn_opts* o, BatchPlan
   if (!batch_kernel(bt->precision, bt->d, o->tile_config, &B->k, &tm))
     return o->tile_config >= 0 ? OTTX_EINVAL : OTTX_EUNSUPPORTED;
   const KernelCfg& k = B->k;
-  B->pad = std::max<int64_t>(k.cols(), 16 * k.mt);
+  B->pad = std::max<int64_t>(k.cols(), 32 * k.mt);
   B->K = bt->count;
   B->outer = (o->max_iterations + o->inner_iterations - 1) / o->inner_iterations;
   B->pbs.resize(B->K);
@@ -1426,6 +1436,7 @@ int make_batch_plan(const ottx_batch* bt, const ottx_sinkhorn_opts* o, BatchPlan
       p.n_in_rb = static_cast<int>(pad_rows(n_in, B->pad) / 16);
       p.rb_per_split = (dir == 0 ? rows_g : rows_f) / 16;
       p.splits = (p.n_in_rb + p.rb_per_split - 1) / p.rb_per_split;
+      if (!even_tiles(k, p)) return OTTX_EINVAL;
       (dir == 0 ? B->gp[i] : B->fp[i]) = p;
       (dir == 0 ? B->n_work_g : B->n_work_f) += static_cast<int64_t>(p.tiles) * p.splits;
     }
