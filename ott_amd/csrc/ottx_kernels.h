@@ -197,12 +197,11 @@ __device__ __forceinline__ float row_bias(float2 r, int64_t row, int64_t n_in, f
 // blocks in B: MFMA cross terms into acc, then the log-sum-exp update of
 // (mx, sm).  SHIFT: mx[] is a fixed per-column shift and only sm is summed;
 // otherwise mx[] is the running max, rescaled once per tile.
-template <int PREC, int KK, int NB, int MT, bool SHIFT>
-__device__ __forceinline__ void compute_tile(const OutOperand<PREC, KK, NB>& B, const u32x4* L, const float* LB,
-                                             int lane, float c, float (&mx)[NB], float (&sm)[NB]) {
+template <int PREC, int KK, int NB, int MT>
+__device__ __forceinline__ void tile_mfma(const OutOperand<PREC, KK, NB>& B, const u32x4* L, const float* LB,
+                                          int lane, f32x4 (&acc)[MT][NB]) {
   constexpr int KQ = (KK + 3) / 4;
   const int rsub = (lane >> 4) * 4;
-  f32x4 acc[MT][NB];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
     const float4 bv = *reinterpret_cast<const float4*>(LB + mt * 16 + rsub);
@@ -256,19 +255,34 @@ __device__ __forceinline__ void compute_tile(const OutOperand<PREC, KK, NB>& B, 
       }
     }
   }
+}
+
+// exp2 of one tile's accumulators against the fixed per-column shift mx[],
+// summed into sm[] (the SHIFT half of compute_tile).
+template <int NB, int MT>
+__device__ __forceinline__ void tile_exp_shift(const f32x4 (&acc)[MT][NB], float c, const float (&mx)[NB],
+                                               float (&sm)[NB]) {
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const float nmc = -c * mx[nb];
+    float part[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) part[r] = exp2_fast(fmaf(acc[0][nb][r], c, nmc));
+#pragma unroll
+    for (int mt = 1; mt < MT; ++mt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) part[r] += exp2_fast(fmaf(acc[mt][nb][r], c, nmc));
+    sm[nb] += (part[0] + part[1]) + (part[2] + part[3]);
+  }
+}
+
+template <int PREC, int KK, int NB, int MT, bool SHIFT>
+__device__ __forceinline__ void compute_tile(const OutOperand<PREC, KK, NB>& B, const u32x4* L, const float* LB,
+                                             int lane, float c, float (&mx)[NB], float (&sm)[NB]) {
+  f32x4 acc[MT][NB];
+  tile_mfma<PREC, KK, NB, MT>(B, L, LB, lane, acc);
   if constexpr (SHIFT) {
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-      const float nmc = -c * mx[nb];
-      float part[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) part[r] = exp2_fast(fmaf(acc[0][nb][r], c, nmc));
-#pragma unroll
-      for (int mt = 1; mt < MT; ++mt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) part[r] += exp2_fast(fmaf(acc[mt][nb][r], c, nmc));
-      sm[nb] += (part[0] + part[1]) + (part[2] + part[3]);
-    }
+    tile_exp_shift<NB, MT>(acc, c, mx, sm);
   } else {
     // Streaming log-sum-exp, base 2: one rescale per column per tile.  The
     // tile max is a tree and the sum uses four partial accumulators, so no
@@ -297,7 +311,8 @@ __device__ __forceinline__ void compute_tile(const OutOperand<PREC, KK, NB>& B, 
         mx[nb] = mnew;
       }
     }
-  }}
+  }
+}
 
 // Streams in-side row blocks [rb_begin, rb_end) (multiples of MT) against the
 // NB column blocks held in B, updating each lane's running (max, sum) of
@@ -336,6 +351,48 @@ __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, c
   tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb_begin, tid, pre0, pb0);
   tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb_begin + MT, rb_last), tid, pre1,
                                               pb1);
+  if constexpr (SHIFT) {
+    // Software pipeline across tiles: the exp2 / sum of tile t runs while the
+    // MFMAs of tile t + 1 are in flight (independent instructions for the
+    // scheduler to interleave), so one wave keeps both the matrix core and
+    // the VALU busy.
+    f32x4 acc0[MT][NB], acc1[MT][NB];
+#pragma unroll
+    for (int q = 0; q < PER_T; ++q)
+      if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
+    if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb_begin) * 16 + tid, n_in, inv2s);
+    __syncthreads();
+    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb_begin + 2 * MT, rb_last), tid,
+                                                pre0, pb0);
+    tile_mfma<PREC, KK, NB, MT>(B, L0, B0, lane, acc0);
+    for (int rb = rb_begin; rb < rb_end; rb += 2 * MT) {
+      // tile rb + MT: stage, MFMA, and finish tile rb
+#pragma unroll
+      for (int q = 0; q < PER_T; ++q)
+        if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L1[tid + q * NT] = pre1[q];
+      if (tid < ROWS) B1[tid] = row_bias(pb1, static_cast<int64_t>(rb + MT) * 16 + tid, n_in, inv2s);
+      __syncthreads();
+      tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 3 * MT, rb_last), tid,
+                                                  pre1, pb1);
+      tile_mfma<PREC, KK, NB, MT>(B, L1, B1, lane, acc1);
+      tile_exp_shift<NB, MT>(acc0, c, mx, sm);
+      if (rb + 2 * MT >= rb_end) {
+        tile_exp_shift<NB, MT>(acc1, c, mx, sm);
+        break;
+      }
+      // tile rb + 2 MT: stage, MFMA, and finish tile rb + MT
+#pragma unroll
+      for (int q = 0; q < PER_T; ++q)
+        if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
+      if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb + 2 * MT) * 16 + tid, n_in, inv2s);
+      __syncthreads();
+      tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 4 * MT, rb_last), tid,
+                                                  pre0, pb0);
+      tile_mfma<PREC, KK, NB, MT>(B, L0, B0, lane, acc0);
+      tile_exp_shift<NB, MT>(acc1, c, mx, sm);
+    }
+    return;
+  }
   for (int rb = rb_begin; rb < rb_end; rb += 2 * MT) {
 #pragma unroll
     for (int q = 0; q < PER_T; ++q)
