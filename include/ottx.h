@@ -49,6 +49,12 @@ enum ottx_precision {
   OTTX_PREC_BF16_SPLIT = 1,
 };
 
+enum ottx_init_mode {
+  OTTX_INIT_GIVEN = 0,
+  OTTX_INIT_F_FROM_G = 1,
+  OTTX_INIT_ZEROS = 2,
+};
+
 /* A point cloud geometry with squared-Euclidean cost,
  * C_ij = inv_scale_cost * ||x_i - y_j||^2, and its linear problem
  * (weights a, b and unbalancedness tau_a, tau_b).
@@ -82,8 +88,11 @@ typedef struct ottx_sinkhorn_opts {
   double momentum;          /* constant Momentum weight (acceleration.py:161-172), 1 = none */
   int32_t tile_config;      /* -1 = default, else a value listed by ottx_tile_configs */
   int32_t use_graph;        /* 1 = replay each block of iterations as a hipGraph */
-  int32_t init_from_g;      /* 1 = f starts as the f-update of the initial g (a warm start that
-                               predicts g only); 0 = f and g are used as given */
+  int32_t init_mode;        /* OTTX_INIT_GIVEN: f and g are used as given;
+                               OTTX_INIT_F_FROM_G: f starts as the f-update of the given g (a warm
+                               start that predicts g only);
+                               OTTX_INIT_ZEROS: DefaultInitializer (initializers.py:70-99), f and g
+                               start at 0, -inf where the weight is 0 (their contents are ignored) */
   int32_t _pad;
 } ottx_sinkhorn_opts;
 
@@ -150,7 +159,7 @@ size_t ottx_sinkhorn_batched_workspace_bytes(const ottx_batch* batch, const ottx
  * problem whose loop has finished drops out of later launches.
  * Reference: tools/segment_sinkhorn.py:24-138 (returns reg_ot_cost per
  * segment; this returns every SinkhornOutput field per problem).
- * opts->init_from_g is not supported here; max_iterations must be >= inner_iterations. */
+ * opts->init_mode OTTX_INIT_F_FROM_G is not supported here; max_iterations must be >= inner_iterations. */
 int ottx_sinkhorn_batched_solve(const ottx_batch* batch, const ottx_sinkhorn_opts* opts,
                                 ottx_batch_out* out, void* workspace, size_t workspace_bytes,
                                 void* stream);

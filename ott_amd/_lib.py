@@ -27,6 +27,7 @@ OTTX_EWORKSPACE = -4
 PREC_FP32 = 0
 PREC_BF16_SPLIT = 1
 PRECISIONS = {"fp32": PREC_FP32, "bf16_split": PREC_BF16_SPLIT}
+INIT_GIVEN, INIT_F_FROM_G, INIT_ZEROS = 0, 1, 2
 
 OP_APPLY_LSE = 0
 OP_UPDATE = 1
@@ -61,7 +62,7 @@ class SinkhornOpts(ctypes.Structure):
       ("momentum", ctypes.c_double),
       ("tile_config", ctypes.c_int32),
       ("use_graph", ctypes.c_int32),
-      ("init_from_g", ctypes.c_int32),
+      ("init_mode", ctypes.c_int32),
       ("_pad", ctypes.c_int32),
   ]
 

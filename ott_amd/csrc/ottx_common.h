@@ -36,12 +36,21 @@ struct SolveState {
   int32_t done;        // set by the block finalizer when the loop must stop
   int32_t block;       // index of the next block whose error is finalized
   int32_t n_blocks;    // blocks run when the loop stopped
-  int32_t pad0;
+  int32_t finalized;   // the outputs of the stopped loop have been written
   uint32_t tiles_done; // arrival counter of the column tiles of an error pass
   int32_t shift_ok[2]; // a log-sum-exp per column exists for side 0 (f) / 1 (g)
-  uint32_t pad1;
+  uint32_t fin_count;  // arrival counter of the output finalizer's workgroups
   double total_mass;   // sum of the coupling at the last error pass
   double last_err;     // last raw error (before the inf/record rule)
+};
+
+// Results of a solve, written by the device into pinned host memory once the
+// loop has stopped (``ready`` last, equal to the solve's sequence number).
+struct SolveMail {
+  double du[8];        // div_a, sum a, div_b, sum b, #NaN errors, #(errors != -1), errors[-1], eps
+  double total_mass;
+  int32_t n_blocks;
+  uint32_t ready;
 };
 
 // Scalars resolved on the device at solve start (relative epsilon).

@@ -201,21 +201,46 @@ __global__ void norms_kernel(const float* __restrict__ pts, int64_t np, int d, c
   norm[i] = static_cast<float>(s * acc);
 }
 
-// Loop state reset: errors = -1 (sinkhorn.py:964-972), counters = 0.
-__global__ void init_state_kernel(SolveState* st, float* errors, int outer, uint32_t* counters, int n_counters) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Loop state reset: errors = -1 (sinkhorn.py:964-972), counters = 0.  With
+// ``zero_init`` the potentials start as DefaultInitializer's: 0, or -inf where
+// the weight is 0 (initializers.py:70-99).
+__global__ void init_state_kernel(SolveState* st, float* errors, int outer, uint32_t* counters, int n_counters,
+                                  int zero_init, float* f, const float* a, int64_t n, float* g, const float* b,
+                                  int64_t m) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   if (i == 0) {
     st->done = 0;
     st->block = 0;
     st->n_blocks = 0;
+    st->finalized = 0;
     st->tiles_done = 0u;
     st->shift_ok[0] = 0;
     st->shift_ok[1] = 0;
+    st->fin_count = 0u;
     st->total_mass = 0.0;
     st->last_err = 0.0;
   }
-  for (int k = i; k < outer; k += gridDim.x * blockDim.x) errors[k] = -1.f;
-  for (int k = i; k < n_counters; k += gridDim.x * blockDim.x) counters[k] = 0u;
+  for (int64_t k = i; k < outer; k += stride) errors[k] = -1.f;
+  for (int64_t k = i; k < n_counters; k += stride) counters[k] = 0u;
+  if (zero_init) {
+    for (int64_t k = i; k < n; k += stride) f[k] = a[k] > 0.f ? 0.f : -INFINITY;
+    for (int64_t k = i; k < m; k += stride) g[k] = b[k] > 0.f ? 0.f : -INFINITY;
+  }
+}
+
+// out[0] = sum of v[0, n) (one workgroup, fixed order).
+__global__ void sum_f64_kernel(const double* __restrict__ v, int n, double* __restrict__ out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += v[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
 }
 
 __global__ void copy_f32_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
@@ -298,6 +323,92 @@ __global__ void dual_final_kernel(const double* __restrict__ part, const float* 
   }
   out[6] = outer > 0 ? static_cast<double>(errors[outer - 1]) : 0.0;
   out[7] = sc->eps;
+}
+
+// Outputs of a stopped loop, computed on the device right after the block
+// that stopped it so the host only has to read pinned memory: the float64
+// dual terms (as dual_partial_kernel + dual_final_kernel), g <- g_save, and
+// the SolveMail written last.  Grid (kDualBlocks, 3); every launch before the
+// loop stops, or after its outputs were written, exits at once.  The last
+// workgroup to arrive combines the partials.
+__global__ void solve_finalize_kernel(const float* __restrict__ f, const float* __restrict__ a, int64_t n,
+                                      const float* __restrict__ g_save, float* __restrict__ g_out,
+                                      const float* __restrict__ b, int64_t m, const SolveScalars* __restrict__ sc,
+                                      double tau_a, double tau_b, const float* __restrict__ errors, int outer,
+                                      double* __restrict__ part, SolveState* st, SolveMail* mail, uint32_t seq) {
+  __shared__ double red[2][256];
+  __shared__ int s_last;
+  if (!st->done || st->finalized) return;
+  const int tid = threadIdx.x;
+  const int side = blockIdx.y;
+  double s0 = 0.0, s1 = 0.0;
+  if (side < 2) {
+    const float* pot = side == 0 ? f : g_save;
+    const float* w = side == 0 ? a : b;
+    const int64_t len = side == 0 ? n : m;
+    const double tau = side == 0 ? tau_a : tau_b;
+    const double eps = sc->eps;
+    const double rho = eps * tau / (1.0 - tau);
+    const int64_t per = (len + kDualBlocks - 1) / kDualBlocks;
+    const int64_t i0 = blockIdx.x * per, i1 = min(len, i0 + per);
+    for (int64_t i = i0 + tid; i < i1; i += blockDim.x) {
+      const double wi = w[i];
+      s1 += wi;
+      if (wi > 0.0) {
+        const double h = static_cast<double>(pot[i]) - eps * log(wi);
+        if (tau == 1.0) {
+          s0 += wi * h;
+        } else {
+          s0 -= wi * rho * (exp(-h / rho) - 1.0);
+        }
+      }
+      if (side == 1) g_out[i] = g_save[i];
+    }
+  } else if (blockIdx.x == 0) {
+    for (int i = tid; i < outer; i += blockDim.x) {
+      const float e = errors[i];
+      if (e != e) s0 += 1.0;
+      if (e != -1.f) s1 += 1.0;
+    }
+  }
+  red[0][tid] = s0;
+  red[1][tid] = s1;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (tid < k) {
+      red[0][tid] += red[0][tid + k];
+      red[1][tid] += red[1][tid + k];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    part[(side * kDualBlocks + blockIdx.x) * 2 + 0] = red[0][0];
+    part[(side * kDualBlocks + blockIdx.x) * 2 + 1] = red[1][0];
+    release_to_device();
+    const uint32_t prev = atomicAdd(&st->fin_count, 1u);
+    s_last = (prev == 3u * kDualBlocks - 1u) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last || tid != 0) return;
+  double du[8];
+  for (int sd = 0; sd < 3; ++sd) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int q = 0; q < kDualBlocks; ++q) {
+      t0 += load_f64_coherent(part + (sd * kDualBlocks + q) * 2 + 0);
+      t1 += load_f64_coherent(part + (sd * kDualBlocks + q) * 2 + 1);
+    }
+    du[sd * 2 + 0] = t0;
+    du[sd * 2 + 1] = t1;
+  }
+  du[6] = outer > 0 ? static_cast<double>(errors[outer - 1]) : 0.0;
+  du[7] = sc->eps;
+  for (int q = 0; q < 8; ++q) mail->du[q] = du[q];
+  mail->total_mass = st->total_mass;
+  mail->n_blocks = st->n_blocks;
+  st->finalized = 1;
+  st->fin_count = 0u;
+  __threadfence_system();
+  mail->ready = seq;
 }
 
 // ---------------------------------------------------------------------------
@@ -453,8 +564,10 @@ struct Layout {
   int64_t rows_chunk_x, rows_chunk_y;
 };
 
+// At most 32 chunks: the single-workgroup final pass reads every chunk, and
+// its latency grows with their number.
 void stats_chunks(int64_t np, int* chunks, int64_t* rows) {
-  int64_t r = std::max<int64_t>(64, (np + 255) / 256);
+  int64_t r = std::max<int64_t>(64, (np + 31) / 32);
   *rows = r;
   *chunks = static_cast<int>((np + r - 1) / r);
 }
@@ -666,8 +779,9 @@ int launch_halfstep(const KernelCfg& k, const HalfStepArgs& a, hipStream_t st) {
 // pinned mailbox they land in.
 struct HostHelpers {
   hipStream_t side = nullptr;
-  void* mailbox = nullptr;
+  void* mailbox = nullptr;  // 4 KiB pinned: [0, 1024) copies, [1024, ...) the SolveMail
   int device = -1;
+  uint32_t seq = 0;         // sequence number of the last solve on this thread
 };
 
 int host_helpers(HostHelpers** out) {
@@ -770,7 +884,8 @@ int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const f
   SolveState* state = reinterpret_cast<SolveState*>(P.base + L.state);
   float* dummy_err = reinterpret_cast<float*>(P.base + L.dual);
   hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(256), 0, st, state, dummy_err, 0, counters,
-                     std::max(gp.tiles, fp.tiles));
+                     std::max(gp.tiles, fp.tiles), 0, static_cast<float*>(nullptr), static_cast<const float*>(nullptr),
+                     int64_t{0}, static_cast<float*>(nullptr), static_cast<const float*>(nullptr), int64_t{0});
   OTTX_LAUNCHED();
   const bool g_upd = axis == 0;
   const PassPlan& plan = g_upd ? gp : fp;
@@ -839,9 +954,15 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   uint32_t* counters = reinterpret_cast<uint32_t*>(base + L.counters);
   float* g_save = reinterpret_cast<float*>(base + L.g_save);
   double* dual = reinterpret_cast<double*>(base + L.dual);
-  hipLaunchKernelGGL(init_state_kernel, dim3(4), dim3(256), 0, st, state, out->errors, outer, counters,
-                     std::max(gp.tiles, fp.tiles));
-  OTTX_LAUNCHED();
+  if (o->init_mode < 0 || o->init_mode > 2) return OTTX_EINVAL;
+  {
+    const int64_t span = std::max<int64_t>(std::max<int64_t>(pb->n, pb->m), outer);
+    const int blocks = static_cast<int>(std::min<int64_t>(1024, (span + 255) / 256));
+    hipLaunchKernelGGL(init_state_kernel, dim3(blocks), dim3(256), 0, st, state, out->errors, outer, counters,
+                       std::max(gp.tiles, fp.tiles), o->init_mode == OTTX_INIT_ZEROS ? 1 : 0, out->f, pb->a, pb->n,
+                       out->g, pb->b, pb->m);
+    OTTX_LAUNCHED();
+  }
 
   const double eps = pb->epsilon;
   HalfStepArgs ga = base_args(pb, k, gp, P, 1, out->f, out->g, eps);
@@ -879,7 +1000,7 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     return OTTX_OK;
   };
 
-  if (o->init_from_g) {
+  if (o->init_mode == OTTX_INIT_F_FROM_G) {
     // f <- tau_a (eps log a - eps LSE_j((g_j - C_ij) / eps)), a plain update
     HalfStepArgs f_init = fa;
     f_init.mode = kModeWritePot;
@@ -889,17 +1010,34 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     if (rc != OTTX_OK) return rc;
   }
 
-  SolveState* mail = static_cast<SolveState*>(hh->mailbox);
+  // Results reach the host through pinned memory: the finalizer kernel at the
+  // end of every block writes them (once) after the block that stops the loop.
+  SolveMail* mail = reinterpret_cast<SolveMail*>(static_cast<char*>(hh->mailbox) + 1024);
+  const uint32_t seq = ++hh->seq;
+  mail->ready = 0u;
+  double* dual_part = reinterpret_cast<double*>(base + L.dual_part);
+  auto launch_finalize = [&](hipStream_t s) -> int {
+    hipLaunchKernelGGL(solve_finalize_kernel, dim3(kDualBlocks, 3), dim3(256), 0, s, out->f, pb->a, pb->n, g_save,
+                       out->g, pb->b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors, outer, dual_part, state,
+                       mail, seq);
+    OTTX_LAUNCHED();
+    return OTTX_OK;
+  };
+  auto mail_ready = [&]() { return __atomic_load_n(&mail->ready, __ATOMIC_ACQUIRE) == seq; };
   if (total_blocks > 0) {
     rc = launch_halfstep(k, g_plain, st);  // g-update of iteration 0
     if (rc != OTTX_OK) return rc;
+    auto launch_all = [&](hipStream_t s) -> int {
+      const int r = launch_block(s);
+      return r != OTTX_OK ? r : launch_finalize(s);
+    };
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     hipStream_t cap = nullptr;
     if (o->use_graph) {
       OTTX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
       OTTX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-      rc = launch_block(cap);
+      rc = launch_all(cap);
       hipGraph_t g = nullptr;
       const hipError_t e1 = hipStreamEndCapture(cap, &g);
       if (rc != OTTX_OK || e1 != hipSuccess) {
@@ -922,18 +1060,15 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
       if (exec) {
         if (hipGraphLaunch(exec, st) != hipSuccess) { status = OTTX_EHIP; break; }
       } else {
-        status = launch_block(st);
+        status = launch_all(st);
         if (status != OTTX_OK) break;
       }
       if (hipEventRecord(ev[blk & 1], st) != hipSuccess) { status = OTTX_EHIP; break; }
+      // one block runs ahead: look at the previous block's outcome
       if (blk >= 1) {
+        if (mail_ready()) break;
         if (hipEventSynchronize(ev[(blk - 1) & 1]) != hipSuccess) { status = OTTX_EHIP; break; }
-        if (hipMemcpyAsync(mail, state, sizeof(SolveState), hipMemcpyDeviceToHost, hh->side) != hipSuccess ||
-            hipStreamSynchronize(hh->side) != hipSuccess) {
-          status = OTTX_EHIP;
-          break;
-        }
-        if (mail->done) break;
+        if (mail_ready()) break;
       }
     }
     (void)hipEventDestroy(ev[0]);
@@ -942,47 +1077,50 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     if (graph) (void)hipGraphDestroy(graph);
     if (cap) (void)hipStreamDestroy(cap);
     if (status != OTTX_OK) return status;
-  } else {
-    // No iteration runs: the outputs are the initial potentials.
-    HalfStepArgs mass = ga;
-    mass.mode = kModeSaveOld | kModeErrCol;
-    mass.state = nullptr;
-    rc = launch_halfstep(k, mass, st);
-    if (rc != OTTX_OK) return rc;
+    // The loop stops at the latest in the last block, whose finalizer then
+    // writes the mailbox; wait for it if the speculation has not seen it yet.
+    if (!mail_ready()) {
+      OTTX_HIP(hipStreamSynchronize(st));
+      if (!mail_ready()) return OTTX_EHIP;
+    }
+    const SolveMail* fm = mail;
+    const double* du = fm->du;
+    out->total_mass = fm->total_mass;
+    out->epsilon = du[7];
+    out->reg_ot_cost = du[0] + du[2] + du[7] * (du[1] * du[3] - fm->total_mass);
+    const bool any_nan = du[4] > 0.0;
+    const float last = static_cast<float>(du[6]);
+    out->converged = (!any_nan && outer > 0 && last < static_cast<float>(o->threshold)) ? 1 : 0;
+    out->n_iters = static_cast<int32_t>(du[5]) * inner;
+    out->n_blocks = fm->n_blocks;
+    return OTTX_OK;
   }
 
-  double* dual_part = reinterpret_cast<double*>(base + L.dual_part);
+  // No iteration runs: the outputs are the initial potentials.
+  HalfStepArgs mass = ga;
+  mass.mode = kModeSaveOld | kModeErrCol;
+  mass.state = nullptr;
+  rc = launch_halfstep(k, mass, st);
+  if (rc != OTTX_OK) return rc;
+  double* dpart = reinterpret_cast<double*>(base + L.dual_part);
   hipLaunchKernelGGL(dual_partial_kernel, dim3(kDualBlocks, 3), dim3(256), 0, st, out->f, pb->a, pb->n, g_save,
-                     pb->b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors, outer, dual_part);
-  hipLaunchKernelGGL(dual_final_kernel, dim3(1), dim3(64), 0, st, dual_part, out->errors, outer, P.scal, dual);
+                     pb->b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors, outer, dpart);
+  hipLaunchKernelGGL(dual_final_kernel, dim3(1), dim3(64), 0, st, dpart, out->errors, outer, P.scal, dual);
   OTTX_LAUNCHED();
-  hipLaunchKernelGGL(copy_f32_kernel, dim3(static_cast<int>((pb->m + 255) / 256)), dim3(256), 0, st, g_save,
-                     out->g, pb->m);
+  hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<const double*>(base + L.mass),
+                     gp.tiles, dual + 8);
   OTTX_LAUNCHED();
   char* mb = static_cast<char*>(hh->mailbox);
-  OTTX_HIP(hipMemcpyAsync(mb, state, sizeof(SolveState), hipMemcpyDeviceToHost, st));
-  OTTX_HIP(hipMemcpyAsync(mb + 256, dual, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
-  if (total_blocks == 0) {
-    OTTX_HIP(hipMemcpyAsync(mb + 512, base + L.mass, static_cast<size_t>(gp.tiles) * sizeof(double),
-                            hipMemcpyDeviceToHost, st));
-  }
+  OTTX_HIP(hipMemcpyAsync(mb + 256, dual, 9 * sizeof(double), hipMemcpyDeviceToHost, st));
   OTTX_HIP(hipStreamSynchronize(st));
-  const SolveState* fs = reinterpret_cast<const SolveState*>(mb);
   const double* du = reinterpret_cast<const double*>(mb + 256);
-  double total = fs->total_mass;
-  if (total_blocks == 0) {
-    total = 0.0;
-    const double* mp = reinterpret_cast<const double*>(mb + 512);
-    for (int t = 0; t < gp.tiles && 512 + (t + 1) * 8 <= 4096; ++t) total += mp[t];
-  }
+  const double total = du[8];
   out->total_mass = total;
   out->epsilon = du[7];
   out->reg_ot_cost = du[0] + du[2] + du[7] * (du[1] * du[3] - total);
-  const bool any_nan = du[4] > 0.0;
-  const float last = static_cast<float>(du[6]);
-  out->converged = (!any_nan && outer > 0 && last < static_cast<float>(o->threshold)) ? 1 : 0;
-  out->n_iters = static_cast<int32_t>(du[5]) * inner;
-  out->n_blocks = fs->n_blocks;
+  out->converged = 0;
+  out->n_iters = 0;
+  out->n_blocks = 0;
   return OTTX_OK;
 }
 
@@ -1030,7 +1168,8 @@ extern "C" int ottx_benchmark_halfstep(const ottx_problem* pb, int32_t axis, int
   SolveState* state = reinterpret_cast<SolveState*>(P.base + L.state);
   float* dummy_err = reinterpret_cast<float*>(P.base + L.dual);
   hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(256), 0, st, state, dummy_err, 0, counters,
-                     std::max(gp.tiles, fp.tiles));
+                     std::max(gp.tiles, fp.tiles), 0, static_cast<float*>(nullptr), static_cast<const float*>(nullptr),
+                     int64_t{0}, static_cast<float*>(nullptr), static_cast<const float*>(nullptr), int64_t{0});
   OTTX_LAUNCHED();
   HalfStepArgs a = axis == 0 ? base_args(pb, k, gp, P, 1, f, g, pb->epsilon)
                              : base_args(pb, k, fp, P, 0, g, f, pb->epsilon);
@@ -1201,7 +1340,8 @@ __global__ void batch_norms_kernel(const BatchItem* __restrict__ items, int d, d
 }
 
 // Loop state of every problem (sinkhorn.py:964-972) and the batch counter.
-__global__ void batch_init_kernel(const BatchItem* __restrict__ items, int outer, uint32_t* batch_done) {
+__global__ void batch_init_kernel(const BatchItem* __restrict__ items, int outer, uint32_t* batch_done,
+                                  int zero_init) {
   const BatchItem& it = items[blockIdx.x];
   const int t = threadIdx.x;
   if (t == 0) {
@@ -1209,15 +1349,21 @@ __global__ void batch_init_kernel(const BatchItem* __restrict__ items, int outer
     st->done = 0;
     st->block = 0;
     st->n_blocks = 0;
+    st->finalized = 0;
     st->tiles_done = 0u;
     st->shift_ok[0] = 0;
     st->shift_ok[1] = 0;
+    st->fin_count = 0u;
     st->total_mass = 0.0;
     st->last_err = 0.0;
     if (blockIdx.x == 0) *batch_done = 0u;
   }
   for (int k = t; k < outer; k += blockDim.x) it.errors[k] = -1.f;
   for (int k = t; k < it.n_counters; k += blockDim.x) it.counters[k] = 0u;
+  if (zero_init) {  // DefaultInitializer: 0, or -inf where the weight is 0
+    for (int64_t k = t; k < it.n; k += blockDim.x) it.f[k] = it.a[k] > 0.f ? 0.f : -INFINITY;
+    for (int64_t k = t; k < it.m; k += blockDim.x) it.g[k] = it.b[k] > 0.f ? 0.f : -INFINITY;
+  }
 }
 
 // Stage 1 of the float64 dual terms of every problem: grid (kDualBlocks, 3, K),
@@ -1504,7 +1650,8 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
   if (!out || !out->f || !out->g || !out->errors || !out->reg_ot_cost || !out->converged || !out->n_iters ||
       !workspace)
     return OTTX_EINVAL;
-  if (o->init_from_g) return OTTX_EUNSUPPORTED;
+  if (o->init_mode == OTTX_INIT_F_FROM_G) return OTTX_EUNSUPPORTED;
+  if (o->init_mode < 0 || o->init_mode > 2) return OTTX_EINVAL;
   const int inner = o->inner_iterations;
   const int outer = (o->max_iterations + inner - 1) / inner;
   if (outer < 1) return OTTX_EUNSUPPORTED;
@@ -1615,7 +1762,8 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
                      0, st, d_items, bt->d, chunks, k.prec);
   hipLaunchKernelGGL(batch_norms_kernel, dim3(static_cast<unsigned>((max_pad + 255) / 256), K, 2), dim3(256), 0,
                      st, d_items, bt->d, s);
-  hipLaunchKernelGGL(batch_init_kernel, dim3(K), dim3(256), 0, st, d_items, outer, batch_done);
+  hipLaunchKernelGGL(batch_init_kernel, dim3(K), dim3(256), 0, st, d_items, outer, batch_done,
+                     o->init_mode == OTTX_INIT_ZEROS ? 1 : 0);
   OTTX_LAUNCHED();
 
   const uint32_t g_plain = kModeWritePot | kModeCheckDone;

@@ -186,23 +186,28 @@ class Sinkhorn:
     """sinkhorn.py:782-801."""
     self._check_supported(ot_prob)
     geom = ot_prob.geom
-    init_from_g = False
+    n, m = geom.shape
     if init is None and getattr(self.initializer, "predicts_g", False):
       # A warm start that predicts g only: the solver derives f0 from it with
       # its own first f-update instead of a separate pass over the geometry.
       g = as_device_f32(self.initializer.predict_g(ot_prob), geom.device).clone()
-      f = torch.zeros((geom.shape[0],), dtype=torch.float32, device=geom.device)
-      return run(ot_prob, self, (f, g), init_from_g=True)
+      f = torch.empty((n,), dtype=torch.float32, device=geom.device)
+      return run(ot_prob, self, (f, g), init_mode=_lib.INIT_F_FROM_G)
+    if init is None and type(self.initializer) is init_lib.DefaultInitializer:
+      # zeros, -inf at zero weights (initializers.py:70-99), set by the solve itself
+      f = torch.empty((n,), dtype=torch.float32, device=geom.device)
+      g = torch.empty((m,), dtype=torch.float32, device=geom.device)
+      return run(ot_prob, self, (f, g), init_mode=_lib.INIT_ZEROS)
     if init is None:
       init = self.initializer(ot_prob, lse_mode=self.lse_mode, **kwargs)
     f0, g0 = init
     f = as_device_f32(f0, geom.device).clone()
     g = as_device_f32(g0, geom.device).clone()
-    return run(ot_prob, self, (f, g), init_from_g=init_from_g)
+    return run(ot_prob, self, (f, g))
 
 
 def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
-        init: Tuple[torch.Tensor, torch.Tensor], init_from_g: bool = False) -> SinkhornOutput:
+        init: Tuple[torch.Tensor, torch.Tensor], init_mode: int = _lib.INIT_GIVEN) -> SinkhornOutput:
   """sinkhorn.py:1052-1099: the loop, then ``set_cost`` and ``ot_prob``."""
   lib = _lib.load_library()
   geom: PointCloud = ot_prob.geom
@@ -222,7 +227,7 @@ def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
       momentum=float(solver.momentum.value),
       tile_config=tune.lookup(geom.precision, geom.x.shape[1], n, m),
       use_graph=1 if solver.use_graph else 0,
-      init_from_g=1 if init_from_g else 0)
+      init_mode=int(init_mode))
   errors = torch.empty((solver.outer_iterations,), dtype=torch.float32, device=geom.device)
   out = _lib.SinkhornOut(f=f.data_ptr(), g=g.data_ptr(), errors=errors.data_ptr())
   nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts))

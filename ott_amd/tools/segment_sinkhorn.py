@@ -135,17 +135,15 @@ def segment_sinkhorn_outputs(
       threshold=float(solver.threshold), norm_error=int(solver._norm_error),
       inner_iterations=int(solver.inner_iterations), min_iterations=int(solver.min_iterations),
       max_iterations=int(solver.max_iterations), momentum=float(solver.momentum.value),
-      tile_config=-1, use_graph=1 if solver.use_graph else 0, init_from_g=0)
+      tile_config=-1, use_graph=1 if solver.use_graph else 0, init_mode=_lib.INIT_ZEROS)
   lib = _lib.load_library()
   nbytes = lib.ottx_sinkhorn_batched_workspace_bytes(ctypes.byref(bt), ctypes.byref(opts))
   if nbytes == 0:
     raise ValueError("invalid batched problem")
   ws = _lib.workspace(nbytes, device)
-  f = torch.zeros(xs.shape[0], dtype=torch.float32, device=device)
-  g = torch.zeros(ys.shape[0], dtype=torch.float32, device=device)
-  # zero weights give -inf potentials (initializers.py:95-98)
-  f = torch.where(a > 0, f, torch.full_like(f, -float("inf")))
-  g = torch.where(b > 0, g, torch.full_like(g, -float("inf")))
+  # zeros, -inf at zero weights (initializers.py:95-98), set by the solve itself
+  f = torch.empty(xs.shape[0], dtype=torch.float32, device=device)
+  g = torch.empty(ys.shape[0], dtype=torch.float32, device=device)
   errors = torch.empty((K, solver.outer_iterations), dtype=torch.float32, device=device)
   reg = (ctypes.c_double * K)()
   eps_out = (ctypes.c_double * K)()

@@ -174,7 +174,7 @@ def test_training_reduces_loss():
 @pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
 def test_initializer_fast_path_matches_explicit_init(precision):
   """Sinkhorn(initializer=DeepSetsInitializer) derives f0 inside the solve
-  (init_from_g); it must agree with passing init=(f0, g^) computed outside."""
+  (OTTX_INIT_F_FROM_G); it must agree with passing init=(f0, g^) computed outside."""
   from ott_amd.geometry import pointcloud
   from ott_amd.initializers.neural import deepsets
   from ott_amd.problems.linear import linear_problem
