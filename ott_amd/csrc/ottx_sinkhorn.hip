@@ -67,10 +67,13 @@ __global__ void stats_final_kernel(const double* __restrict__ px, int cx, const 
   double sx1 = 0.0, sx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
   if (t < G * d) {
     const int k = t % d, g = t / d;
+    // unrolled so several independent loads are in flight at once
+#pragma unroll 8
     for (int c = g; c < cx; c += G) {
       sx1 += px[(static_cast<int64_t>(c) * d + k) * 2];
       sx2 += px[(static_cast<int64_t>(c) * d + k) * 2 + 1];
     }
+#pragma unroll 8
     for (int c = g; c < cy; c += G) {
       sy1 += py[(static_cast<int64_t>(c) * d + k) * 2];
       sy2 += py[(static_cast<int64_t>(c) * d + k) * 2 + 1];
@@ -199,6 +202,134 @@ __global__ void norms_kernel(const float* __restrict__ pts, int64_t np, int d, c
     acc += v * v;
   }
   norm[i] = static_cast<float>(s * acc);
+}
+
+// Optional loop-state reset folded into the preparation launch.
+struct PrepInit {
+  SolveState* st = nullptr;
+  float* errors = nullptr;
+  int outer = 0;
+  uint32_t* counters = nullptr;
+  int n_counters = 0;
+  int zero_init = 0;
+  float* f = nullptr;
+  float* g = nullptr;
+};
+
+// Statistics partials of both clouds in one launch: blocks [0, cx) take x,
+// blocks [cx, cx + cy) take y (same per-chunk sums as stats_partial_kernel).
+__global__ void prep_stats_kernel(const float* __restrict__ x, int64_t n, int64_t rows_x, int cx,
+                                  const float* __restrict__ y, int64_t m, int64_t rows_y, int d,
+                                  double* __restrict__ part_x, double* __restrict__ part_y) {
+  __shared__ double s1[256], s2[256];
+  const int t = threadIdx.x;
+  const bool side_y = static_cast<int>(blockIdx.x) >= cx;
+  const int chunk = side_y ? static_cast<int>(blockIdx.x) - cx : static_cast<int>(blockIdx.x);
+  const float* pts = side_y ? y : x;
+  const int64_t np = side_y ? m : n;
+  const int64_t rows = side_y ? rows_y : rows_x;
+  double* part = side_y ? part_y : part_x;
+  const int G = max(1, 256 / d);
+  const int64_t r0 = static_cast<int64_t>(chunk) * rows;
+  const int64_t r1 = min(np, r0 + rows);
+  double a1 = 0.0, a2 = 0.0;
+  if (t < G * d) {
+    const int k = t % d, g = t / d;
+    for (int64_t r = r0 + g; r < r1; r += G) {
+      const double v = pts[r * d + k];
+      a1 += v;
+      a2 += v * v;
+    }
+  }
+  s1[t] = a1;
+  s2[t] = a2;
+  __syncthreads();
+  if (t < d) {
+    double b1 = 0.0, b2 = 0.0;
+    for (int g = 0; g < G; ++g) {
+      b1 += s1[g * d + t];
+      b2 += s2[g * d + t];
+    }
+    part[(static_cast<int64_t>(chunk) * d + t) * 2 + 0] = b1;
+    part[(static_cast<int64_t>(chunk) * d + t) * 2 + 1] = b2;
+  }
+}
+
+// Everything that follows the statistics, in one launch: fragments and norms
+// of both clouds, uniform weights where the caller passed none, and (for a
+// solve) the loop-state reset.  Blocks are split into consecutive segments.
+struct PrepArgs {
+  const float* pts[2];
+  int64_t np[2], np_pad[2];
+  void* frag[2];
+  float* norm[2];
+  float* wfill[2];       // uniform weights to write, or null
+  const float* wuser[2]; // the caller's weights (null: uniform)
+  int64_t pack_threads[2];
+  int seg[7];            // first block of: pack x, pack y, norm x, norm y, w x, w y, init; seg[6] + blocks_init = grid
+  int blocks_init;
+  const float* mu;
+  int d, chunks, prec;
+  double s;
+  PrepInit init;
+};
+
+__global__ void prep_pack_kernel(const PrepArgs p) {
+  const int bid = blockIdx.x;
+  const int t = threadIdx.x;
+  int segi = 0;
+  while (segi < 6 && bid >= p.seg[segi + 1]) ++segi;
+  const int64_t local = static_cast<int64_t>(bid - p.seg[segi]) * blockDim.x + t;
+  if (segi < 2) {  // fragments
+    const int side = segi;
+    if (local >= p.pack_threads[side]) return;
+    if (p.prec == 1)
+      pack_bf16_one(p.pts[side], p.np[side], p.d, p.mu, p.chunks, local, static_cast<uint4*>(p.frag[side]));
+    else
+      pack_f32_one(p.pts[side], p.np[side], p.d, p.mu, p.chunks, local, static_cast<float4*>(p.frag[side]));
+  } else if (segi < 4) {  // norms s |p - mu|^2
+    const int side = segi - 2;
+    if (local >= p.np_pad[side]) return;
+    float* norm = p.norm[side];
+    if (local >= p.np[side]) {
+      norm[local] = 0.f;
+      return;
+    }
+    const float* pt = p.pts[side] + local * p.d;
+    double acc = 0.0;
+    for (int k = 0; k < p.d; ++k) {
+      const double v = static_cast<double>(pt[k] - p.mu[k]);
+      acc += v * v;
+    }
+    norm[local] = static_cast<float>(p.s * acc);
+  } else if (segi < 6) {  // uniform weights
+    const int side = segi - 4;
+    if (local < p.np[side]) p.wfill[side][local] = static_cast<float>(1.0 / static_cast<double>(p.np[side]));
+  } else {  // loop state
+    const PrepInit& in = p.init;
+    const int64_t stride = static_cast<int64_t>(p.blocks_init) * blockDim.x;
+    if (local == 0) {
+      SolveState* st = in.st;
+      st->done = 0;
+      st->block = 0;
+      st->n_blocks = 0;
+      st->finalized = 0;
+      st->tiles_done = 0u;
+      st->shift_ok[0] = 0;
+      st->shift_ok[1] = 0;
+      st->fin_count = 0u;
+      st->total_mass = 0.0;
+      st->last_err = 0.0;
+    }
+    for (int64_t k = local; k < in.outer; k += stride) in.errors[k] = -1.f;
+    for (int64_t k = local; k < in.n_counters; k += stride) in.counters[k] = 0u;
+    if (in.zero_init) {  // DefaultInitializer: 0, -inf where the weight is 0
+      for (int64_t k = local; k < p.np[0]; k += stride)
+        in.f[k] = (p.wuser[0] == nullptr || p.wuser[0][k] > 0.f) ? 0.f : -INFINITY;
+      for (int64_t k = local; k < p.np[1]; k += stride)
+        in.g[k] = (p.wuser[1] == nullptr || p.wuser[1][k] > 0.f) ? 0.f : -INFINITY;
+    }
+  }
 }
 
 // Loop state reset: errors = -1 (sinkhorn.py:964-972), counters = 0.  With
@@ -389,16 +520,22 @@ __global__ void solve_finalize_kernel(const float* __restrict__ f, const float* 
     s_last = (prev == 3u * kDualBlocks - 1u) ? 1 : 0;
   }
   __syncthreads();
-  if (!s_last || tid != 0) return;
+  if (!s_last) return;
+  // the last workgroup: all partials loaded at once (one per thread), then a
+  // fixed-order tree per (side, term)
+  static_assert(3 * kDualBlocks * 2 <= 256, "one partial per thread");
+  const int q = tid % kDualBlocks, sd = (tid / kDualBlocks) % 3, term = tid / (3 * kDualBlocks);
+  red[0][tid] = (tid < 3 * kDualBlocks * 2) ? load_f64_coherent(part + (sd * kDualBlocks + q) * 2 + term) : 0.0;
+  __syncthreads();
+  for (int k = kDualBlocks / 2; k > 0; k >>= 1) {
+    if (tid < 3 * kDualBlocks * 2 && q < k) red[0][tid] += red[0][tid + k];
+    __syncthreads();
+  }
+  if (tid != 0) return;
   double du[8];
-  for (int sd = 0; sd < 3; ++sd) {
-    double t0 = 0.0, t1 = 0.0;
-    for (int q = 0; q < kDualBlocks; ++q) {
-      t0 += load_f64_coherent(part + (sd * kDualBlocks + q) * 2 + 0);
-      t1 += load_f64_coherent(part + (sd * kDualBlocks + q) * 2 + 1);
-    }
-    du[sd * 2 + 0] = t0;
-    du[sd * 2 + 1] = t1;
+  for (int sd2 = 0; sd2 < 3; ++sd2) {
+    du[sd2 * 2 + 0] = red[0][(0 * 3 + sd2) * kDualBlocks];
+    du[sd2 * 2 + 1] = red[0][(1 * 3 + sd2) * kDualBlocks];
   }
   du[6] = outer > 0 ? static_cast<double>(errors[outer - 1]) : 0.0;
   du[7] = sc->eps;
@@ -514,10 +651,13 @@ bool select_kernel(int prec, int d, int tile_config, KernelCfg* out, int* target
 }
 
 int device_cus() {
+  static int cached[64] = {0};  // per device; a racing first query writes the same value
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (dev >= 0 && dev < 64 && cached[dev] > 0) return cached[dev];
   int v = 0;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) return 256;
+  if (dev >= 0 && dev < 64) cached[dev] = v;
   return v;
 }
 
@@ -559,15 +699,15 @@ bool even_tiles(const KernelCfg& k, const PassPlan& p) {
 
 struct Layout {
   size_t frag_x, frag_y, norm_x, norm_y, g_save, lse_x, lse_y, partial, counters, err_col, mass, err_row, state,
-      dual, dual_part, scal, stats_x, stats_y, stats_out, mu, total;
+      dual, dual_part, scal, stats_x, stats_y, stats_out, mu, w_x, w_y, total;
   int chunks_x, chunks_y;
   int64_t rows_chunk_x, rows_chunk_y;
 };
 
-// At most 32 chunks: the single-workgroup final pass reads every chunk, and
+// At most 128 chunks: the single-workgroup final pass reads every chunk, and
 // its latency grows with their number.
 void stats_chunks(int64_t np, int* chunks, int64_t* rows) {
-  int64_t r = std::max<int64_t>(64, (np + 31) / 32);
+  int64_t r = std::max<int64_t>(64, (np + 127) / 128);
   *rows = r;
   *chunks = static_cast<int>((np + r - 1) / r);
 }
@@ -608,6 +748,9 @@ Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& g
   L.stats_y = take(static_cast<size_t>(L.chunks_y) * pb->d * 2 * sizeof(double));
   L.stats_out = take(8 * sizeof(double));
   L.mu = take(static_cast<size_t>(pb->d) * sizeof(float));
+  // uniform weights when the caller passes none (linear_problem.py:65-79)
+  L.w_x = pb->a ? 0 : take(static_cast<size_t>(pb->n) * sizeof(float));
+  L.w_y = pb->b ? 0 : take(static_cast<size_t>(pb->m) * sizeof(float));
   (void)outer;
   L.total = off;
   return L;
@@ -649,7 +792,7 @@ size_t max_workspace(const ottx_problem* pb) {
 int check_problem(const ottx_problem* pb) {
   if (!pb || pb->n <= 0 || pb->m <= 0 || pb->d <= 0 || pb->d > 256) return OTTX_EINVAL;
   if (pb->n > (int64_t{1} << 30) || pb->m > (int64_t{1} << 30)) return OTTX_EINVAL;
-  if (!pb->x || !pb->y || !pb->a || !pb->b) return OTTX_EINVAL;
+  if (!pb->x || !pb->y) return OTTX_EINVAL;  // a or b NULL: uniform weights
   if (!(pb->epsilon > 0.0) && !(pb->relative_epsilon > 0.0)) return OTTX_EINVAL;
   if (!(pb->inv_scale_cost > 0.0) || pb->relative_epsilon < 0.0) return OTTX_EINVAL;
   if (!(pb->tau_a > 0.0 && pb->tau_a <= 1.0) || !(pb->tau_b > 0.0 && pb->tau_b <= 1.0)) return OTTX_EINVAL;
@@ -677,11 +820,15 @@ struct Prepared {
   float* mu;
   double* stats_out;
   SolveScalars* scal;
+  const float* a;  // the caller's weights, or the uniform ones in the workspace
+  const float* b;
 };
 
-// Statistics, centring shift, fragments and norms of both clouds.
+
+// Statistics, centring shift, fragments and norms of both clouds (three
+// launches), plus the loop-state reset of a solve when ``init`` names one.
 int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* ws, hipStream_t st,
-            Prepared* P) {
+            Prepared* P, const PrepInit* init = nullptr) {
   char* base = static_cast<char*>(ws);
   P->base = base;
   P->L = L;
@@ -691,38 +838,63 @@ int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* w
   P->norm_y = reinterpret_cast<float*>(base + L.norm_y);
   P->mu = reinterpret_cast<float*>(base + L.mu);
   P->stats_out = reinterpret_cast<double*>(base + L.stats_out);
+  P->scal = reinterpret_cast<SolveScalars*>(base + L.scal);
+  P->a = pb->a ? pb->a : reinterpret_cast<const float*>(base + L.w_x);
+  P->b = pb->b ? pb->b : reinterpret_cast<const float*>(base + L.w_y);
   double* sx = reinterpret_cast<double*>(base + L.stats_x);
   double* sy = reinterpret_cast<double*>(base + L.stats_y);
-  hipLaunchKernelGGL(stats_partial_kernel, dim3(L.chunks_x), dim3(256), 0, st, pb->x, pb->n, pb->d,
-                     L.rows_chunk_x, sx);
-  hipLaunchKernelGGL(stats_partial_kernel, dim3(L.chunks_y), dim3(256), 0, st, pb->y, pb->m, pb->d,
-                     L.rows_chunk_y, sy);
-  P->scal = reinterpret_cast<SolveScalars*>(base + L.scal);
+  hipLaunchKernelGGL(prep_stats_kernel, dim3(L.chunks_x + L.chunks_y), dim3(256), 0, st, pb->x, pb->n,
+                     L.rows_chunk_x, L.chunks_x, pb->y, pb->m, L.rows_chunk_y, pb->d, sx, sy);
   hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, st, sx, L.chunks_x, sy, L.chunks_y, pb->d,
                      pb->n, pb->m, pb->inv_scale_cost, P->stats_out, P->mu, P->scal, pb->relative_epsilon,
                      pb->epsilon, pb->tau_a, pb->tau_b);
   OTTX_LAUNCHED();
-  const int64_t n_pad = pad_rows(pb->n), m_pad = pad_rows(pb->m);
+  PrepArgs pa{};
   const int chunks = k.frag_chunks();
+  pa.pts[0] = pb->x;
+  pa.pts[1] = pb->y;
+  pa.np[0] = pb->n;
+  pa.np[1] = pb->m;
+  pa.np_pad[0] = pad_rows(pb->n);
+  pa.np_pad[1] = pad_rows(pb->m);
+  pa.frag[0] = P->frag_x;
+  pa.frag[1] = P->frag_y;
+  pa.norm[0] = P->norm_x;
+  pa.norm[1] = P->norm_y;
+  pa.wuser[0] = pb->a;
+  pa.wuser[1] = pb->b;
+  pa.wfill[0] = pb->a ? nullptr : reinterpret_cast<float*>(base + L.w_x);
+  pa.wfill[1] = pb->b ? nullptr : reinterpret_cast<float*>(base + L.w_y);
+  pa.mu = P->mu;
+  pa.d = pb->d;
+  pa.chunks = chunks;
+  pa.prec = k.prec;
+  pa.s = pb->inv_scale_cost;
+  auto blocks = [](int64_t threads) { return static_cast<int>((threads + 255) / 256); };
+  int cur = 0;
   for (int side = 0; side < 2; ++side) {
-    const float* pts = side == 0 ? pb->x : pb->y;
-    const int64_t np = side == 0 ? pb->n : pb->m;
-    const int64_t np_pad = side == 0 ? n_pad : m_pad;
-    void* frag = side == 0 ? P->frag_x : P->frag_y;
-    float* norm = side == 0 ? P->norm_x : P->norm_y;
-    const int64_t n_threads = (np_pad / 16) * chunks * 64;
-    const int blocks = static_cast<int>((n_threads + 255) / 256);
-    if (k.prec == 1) {
-      hipLaunchKernelGGL(pack_bf16_kernel, dim3(blocks), dim3(256), 0, st, pts, np, pb->d, P->mu, chunks,
-                         n_threads, static_cast<uint4*>(frag));
-    } else {
-      hipLaunchKernelGGL(pack_f32_kernel, dim3(blocks), dim3(256), 0, st, pts, np, pb->d, P->mu, chunks,
-                         n_threads, static_cast<float4*>(frag));
-    }
-    hipLaunchKernelGGL(norms_kernel, dim3(static_cast<int>((np_pad + 255) / 256)), dim3(256), 0, st, pts, np,
-                       pb->d, P->mu, pb->inv_scale_cost, np_pad, norm);
-    OTTX_LAUNCHED();
+    pa.pack_threads[side] = (pa.np_pad[side] / 16) * chunks * 64;
+    pa.seg[side] = cur;
+    cur += blocks(pa.pack_threads[side]);
   }
+  for (int side = 0; side < 2; ++side) {
+    pa.seg[2 + side] = cur;
+    cur += blocks(pa.np_pad[side]);
+  }
+  for (int side = 0; side < 2; ++side) {
+    pa.seg[4 + side] = cur;
+    if (pa.wfill[side]) cur += blocks(pa.np[side]);
+  }
+  pa.seg[6] = cur;
+  if (init != nullptr && init->st != nullptr) {
+    pa.init = *init;
+    const int64_t span = std::max<int64_t>(std::max<int64_t>(pb->n, pb->m),
+                                           std::max<int64_t>(init->outer, init->n_counters));
+    pa.blocks_init = static_cast<int>(std::min<int64_t>(256, blocks(span)));
+    cur += pa.blocks_init;
+  }
+  hipLaunchKernelGGL(prep_pack_kernel, dim3(cur), dim3(256), 0, st, pa);
+  OTTX_LAUNCHED();
   return OTTX_OK;
 }
 
@@ -738,7 +910,7 @@ HalfStepArgs base_args(const ottx_problem* pb, const KernelCfg& k, const PassPla
   a.out_frag = g_upd ? P.frag_y : P.frag_x;
   a.out_pot = out_pot;
   a.out_norm = g_upd ? P.norm_y : P.norm_x;
-  a.out_w = g_upd ? pb->b : pb->a;
+  a.out_w = g_upd ? P.b : P.a;
   a.n_in = g_upd ? pb->n : pb->m;
   a.n_out = g_upd ? pb->m : pb->n;
   a.n_in_rb = plan.n_in_rb;
@@ -917,7 +1089,9 @@ int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const f
 
 size_t ottx_sinkhorn_workspace_bytes(const ottx_problem* pb, const ottx_sinkhorn_opts* o) {
   if (check_problem(pb) != OTTX_OK || !o) return 0;
-  return max_workspace(pb);
+  Plan PL;
+  if (!make_plan(pb, o->tile_config, &PL)) return 0;
+  return PL.L.total;
 }
 
 int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ottx_sinkhorn_out* out, void* workspace,
@@ -946,23 +1120,24 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   const bool force_scan = o->min_iterations == o->max_iterations;
   const int total_blocks = force_scan ? o->max_iterations / inner : outer;
 
-  Prepared P;
-  rc = prepare(pb, k, L, workspace, st, &P);
-  if (rc != OTTX_OK) return rc;
-  char* base = P.base;
+  if (o->init_mode < 0 || o->init_mode > 2) return OTTX_EINVAL;
+  char* base = static_cast<char*>(workspace);
   SolveState* state = reinterpret_cast<SolveState*>(base + L.state);
   uint32_t* counters = reinterpret_cast<uint32_t*>(base + L.counters);
   float* g_save = reinterpret_cast<float*>(base + L.g_save);
   double* dual = reinterpret_cast<double*>(base + L.dual);
-  if (o->init_mode < 0 || o->init_mode > 2) return OTTX_EINVAL;
-  {
-    const int64_t span = std::max<int64_t>(std::max<int64_t>(pb->n, pb->m), outer);
-    const int blocks = static_cast<int>(std::min<int64_t>(1024, (span + 255) / 256));
-    hipLaunchKernelGGL(init_state_kernel, dim3(blocks), dim3(256), 0, st, state, out->errors, outer, counters,
-                       std::max(gp.tiles, fp.tiles), o->init_mode == OTTX_INIT_ZEROS ? 1 : 0, out->f, pb->a, pb->n,
-                       out->g, pb->b, pb->m);
-    OTTX_LAUNCHED();
-  }
+  PrepInit pinit;
+  pinit.st = state;
+  pinit.errors = out->errors;
+  pinit.outer = outer;
+  pinit.counters = counters;
+  pinit.n_counters = std::max(gp.tiles, fp.tiles);
+  pinit.zero_init = o->init_mode == OTTX_INIT_ZEROS ? 1 : 0;
+  pinit.f = out->f;
+  pinit.g = out->g;
+  Prepared P;
+  rc = prepare(pb, k, L, workspace, st, &P, &pinit);
+  if (rc != OTTX_OK) return rc;
 
   const double eps = pb->epsilon;
   HalfStepArgs ga = base_args(pb, k, gp, P, 1, out->f, out->g, eps);
@@ -1017,8 +1192,8 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   mail->ready = 0u;
   double* dual_part = reinterpret_cast<double*>(base + L.dual_part);
   auto launch_finalize = [&](hipStream_t s) -> int {
-    hipLaunchKernelGGL(solve_finalize_kernel, dim3(kDualBlocks, 3), dim3(256), 0, s, out->f, pb->a, pb->n, g_save,
-                       out->g, pb->b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors, outer, dual_part, state,
+    hipLaunchKernelGGL(solve_finalize_kernel, dim3(kDualBlocks, 3), dim3(256), 0, s, out->f, P.a, pb->n, g_save,
+                       out->g, P.b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors, outer, dual_part, state,
                        mail, seq);
     OTTX_LAUNCHED();
     return OTTX_OK;
@@ -1034,7 +1209,9 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     hipStream_t cap = nullptr;
-    if (o->use_graph) {
+    // A graph pays for itself only when a block has several launches: with
+    // inner_iterations = 1 (3 launches) direct launches return sooner.
+    if (o->use_graph && inner >= 4) {
       OTTX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
       OTTX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
       rc = launch_all(cap);
@@ -1064,11 +1241,18 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
         if (status != OTTX_OK) break;
       }
       if (hipEventRecord(ev[blk & 1], st) != hipSuccess) { status = OTTX_EHIP; break; }
-      // one block runs ahead: look at the previous block's outcome
+      // One block runs ahead: wait for the previous block, spinning on its
+      // event (a blocking wait can add tens of microseconds of wake-up
+      // latency) and on the mailbox, which the block that stops the loop fills.
       if (blk >= 1) {
-        if (mail_ready()) break;
-        if (hipEventSynchronize(ev[(blk - 1) & 1]) != hipSuccess) { status = OTTX_EHIP; break; }
-        if (mail_ready()) break;
+        bool stop = false;
+        for (;;) {
+          if (mail_ready()) { stop = true; break; }
+          const hipError_t q = hipEventQuery(ev[(blk - 1) & 1]);
+          if (q == hipSuccess) break;
+          if (q != hipErrorNotReady) { status = OTTX_EHIP; break; }
+        }
+        if (status != OTTX_OK || stop || mail_ready()) break;
       }
     }
     (void)hipEventDestroy(ev[0]);
@@ -1103,8 +1287,8 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   rc = launch_halfstep(k, mass, st);
   if (rc != OTTX_OK) return rc;
   double* dpart = reinterpret_cast<double*>(base + L.dual_part);
-  hipLaunchKernelGGL(dual_partial_kernel, dim3(kDualBlocks, 3), dim3(256), 0, st, out->f, pb->a, pb->n, g_save,
-                     pb->b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors, outer, dpart);
+  hipLaunchKernelGGL(dual_partial_kernel, dim3(kDualBlocks, 3), dim3(256), 0, st, out->f, P.a, pb->n, g_save,
+                     P.b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors, outer, dpart);
   hipLaunchKernelGGL(dual_final_kernel, dim3(1), dim3(64), 0, st, dpart, out->errors, outer, P.scal, dual);
   OTTX_LAUNCHED();
   hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<const double*>(base + L.mass),
@@ -1600,8 +1784,10 @@ int make_batch_plan(const ottx_batch* bt, const ottx_sinkhorn_opts* o, BatchPlan
 }
 
 // Pointers of one problem's layout inside the workspace (no launches).
-Prepared prepared_view(char* base, const Layout& L) {
+Prepared prepared_view(char* base, const Layout& L, const ottx_problem& pb) {
   Prepared P;
+  P.a = pb.a;
+  P.b = pb.b;
   P.base = base;
   P.L = L;
   P.frag_x = base + L.frag_x;
@@ -1689,7 +1875,7 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
     const ottx_problem& pb = B.pbs[i];
     const Layout& L = B.L[i];
     char* base = ws + B.base[i];
-    const Prepared P = prepared_view(base, L);
+    const Prepared P = prepared_view(base, L, pb);
     BatchItem& it = items[i];
     it = BatchItem{};
     it.x = pb.x;

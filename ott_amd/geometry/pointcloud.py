@@ -196,13 +196,10 @@ class PointCloud:
     if not self.is_squared_euclidean:
       raise NotImplementedError("ott_amd accelerates the squared-Euclidean cost only")
     n, m = self.shape
-    if a is None:
-      a = torch.full((n,), 1.0 / n, dtype=torch.float32, device=self.device)
-    if b is None:
-      b = torch.full((m,), 1.0 / m, dtype=torch.float32, device=self.device)
+    # None weights are passed as NULL: the library uses uniform 1/n, 1/m
     pb = _lib.Problem(
         n=n, m=m, d=self.x.shape[1], precision=_lib.PRECISIONS[self.precision],
-        x=self.x.data_ptr(), y=self.y.data_ptr(), a=a.data_ptr(), b=b.data_ptr(),
+        x=self.x.data_ptr(), y=self.y.data_ptr(), a=_lib.ptr(a), b=_lib.ptr(b),
         epsilon=float(self.epsilon if epsilon is None else epsilon),
         inv_scale_cost=self.inv_scale_cost, tau_a=float(tau_a), tau_b=float(tau_b),
         relative_epsilon=float(relative_epsilon))

@@ -215,7 +215,7 @@ def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
   n, m = geom.shape
   if f.shape != (n,) or g.shape != (m,):
     raise ValueError(f"init potentials must be [{n}] and [{m}]")
-  a, b = ot_prob.a, ot_prob.b
+  a, b = ot_prob._a, ot_prob._b  # None: uniform, set inside the library
   eps_abs, eps_rel = geom._epsilon_spec()
   pb, keep = geom._problem(a=a, b=b, tau_a=ot_prob.tau_a, tau_b=ot_prob.tau_b,
                            epsilon=eps_abs, relative_epsilon=eps_rel)
