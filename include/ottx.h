@@ -265,10 +265,13 @@ typedef struct ottx_ds_batch {
   const int32_t* x_off;    /* [B + 1] row offsets into x */
   const int32_t* y_off;    /* [B + 1] row offsets into y */
   const int32_t* y_prob;   /* [roundup(m_total, 64)] problem of each head row, 0 on padding */
-  const int32_t* enc_src;  /* [n_total + m_total] row in x (side 0) or y (side 1) of encoder row r */
-  const int32_t* seg_of_row; /* [n_total + m_total] segment 2 b + side of encoder row r */
+  const int32_t* enc_src;  /* [enc_rows] row in x (side 0) or y (side 1) of encoder row r */
+  const int32_t* seg_of_row; /* [enc_rows] segment 2 b + side of encoder row r */
   const int32_t* seg_off;  /* [2 B + 1] encoder row offsets of the segments */
   const float* eps_rel;    /* [B] epsilon / mean cost of each problem, or NULL (then eps / sigma^2 / 2) */
+  int32_t enc_rows;        /* encoder rows = seg_off[2 B]: every point, or a subsample of each cloud
+                              (the pooled embedding is a mean over the set) */
+  int32_t _pad1;
 } ottx_ds_batch;
 
 int64_t ottx_ds_param_count(int32_t d);

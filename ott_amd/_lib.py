@@ -143,6 +143,8 @@ class DsBatch(ctypes.Structure):
       ("seg_of_row", ctypes.c_void_p),
       ("seg_off", ctypes.c_void_p),
       ("eps_rel", ctypes.c_void_p),
+      ("enc_rows", ctypes.c_int32),
+      ("_pad1", ctypes.c_int32),
   ]
 
 

@@ -532,7 +532,7 @@ struct DsLayout {
 DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training) {
   DsLayout L{};
   L.nprob = bt->count;
-  L.Pe = bt->n_total + bt->m_total;
+  L.Pe = bt->enc_rows;
   L.Pe_pad = pad64(L.Pe);
   L.Ph = bt->m_total;
   L.Ph_pad = pad64(L.Ph);
@@ -759,6 +759,10 @@ int64_t ottx_ds_weight_bf16_count(int32_t d) {
 
 size_t ottx_ds_workspace_bytes(const ottx_ds_batch* bt, int32_t training) {
   if (!bt || bt->count < 1 || bt->d < 1 || bt->d > 255 || bt->n_total < 1 || bt->m_total < 1) return 0;
+  // each of the 2 B segments holds at least one encoder row, and no more rows than points
+  if (bt->enc_rows < 2 * bt->count ||
+      static_cast<int64_t>(bt->enc_rows) > static_cast<int64_t>(bt->n_total) + bt->m_total)
+    return 0;
   return ds_layout(bt, param_layout(bt->d), training != 0).total;
 }
 

@@ -81,6 +81,7 @@ class Batch:
   seg_of_row: torch.Tensor
   seg_off: torch.Tensor
   eps_rel: Optional[torch.Tensor] = None
+  enc_rows: int = 0
 
   def struct(self) -> _lib.DsBatch:
     return _lib.DsBatch(
@@ -89,14 +90,19 @@ class Batch:
         x_off=self.x_off.data_ptr(), y_off=self.y_off.data_ptr(), y_prob=self.y_prob.data_ptr(),
         enc_src=self.enc_src.data_ptr(), seg_of_row=self.seg_of_row.data_ptr(),
         seg_off=self.seg_off.data_ptr(),
-        eps_rel=self.eps_rel.data_ptr() if self.eps_rel is not None else None)
+        eps_rel=self.eps_rel.data_ptr() if self.eps_rel is not None else None,
+        enc_rows=self.enc_rows)
 
 
-def make_batch(problems: Sequence[Tuple], device: Optional[torch.device] = None) -> Batch:
+def make_batch(problems: Sequence[Tuple], device: Optional[torch.device] = None,
+               max_pool: Optional[int] = None) -> Batch:
   """Concatenate ``(x, y, eps)`` or ``(x, y, eps, eps_rel)`` problems and build the index arrays.
 
   ``eps_rel`` is epsilon divided by the mean cost; without it the network's
-  scale input is derived from eps and the clouds.
+  scale input is derived from eps and the clouds.  ``max_pool`` caps the
+  points per cloud that go through the encoder: the pooled embedding is a
+  mean over the set, so evenly spaced rows of a large cloud give it at a
+  fraction of the cost (the normalising statistics still use every point).
   """
   device = device or _lib.require_device()
   xs = [torch.as_tensor(p[0], dtype=torch.float32).to(device) for p in problems]
@@ -111,13 +117,19 @@ def make_batch(problems: Sequence[Tuple], device: Optional[torch.device] = None)
   m_pad = (m_total + 63) // 64 * 64
   y_prob = np.zeros(m_pad, dtype=np.int32)
   y_prob[:m_total] = np.repeat(np.arange(B, dtype=np.int32), ms)
-  seg_sizes = np.stack([ns, ms], axis=1).reshape(-1)
+  full = np.stack([ns, ms], axis=1).reshape(-1)
+  seg_sizes = full if max_pool is None else np.minimum(full, int(max_pool))
   seg_off = np.concatenate([[0], np.cumsum(seg_sizes)]).astype(np.int32)
   seg_of_row = np.repeat(np.arange(2 * B, dtype=np.int32), seg_sizes)
-  # encoder row r of segment s = 2 b + side reads row (r - seg_off[s]) + off_side[b]
+  # encoder row r of segment s = 2 b + side reads row off_side[b] + (k * full_s) // size_s,
+  # k = r - seg_off[s]: every row when the segment is not capped, else evenly spaced ones
   side_base = np.stack([x_off[:-1], y_off[:-1]], axis=1).reshape(-1)
-  enc_src = (np.arange(n_total + m_total, dtype=np.int64) - seg_off[seg_of_row] +
-             side_base[seg_of_row]).astype(np.int32)
+  k = np.arange(int(seg_sizes.sum()), dtype=np.int64) - seg_off[seg_of_row]
+  enc_src = (side_base[seg_of_row] + (k * full[seg_of_row]) // seg_sizes[seg_of_row]).astype(np.int32)
+  enc_rows = int(seg_sizes.sum())
+  # every source row lies inside its own cloud
+  side_end = np.stack([x_off[1:], y_off[1:]], axis=1).reshape(-1)
+  assert np.all(enc_src >= side_base[seg_of_row]) and np.all(enc_src < side_end[seg_of_row])
   eps = np.array([float(p[2]) for p in problems], dtype=np.float32)
   has_rel = all(len(p) > 3 and p[3] is not None for p in problems)
   rel = np.array([float(p[3]) for p in problems] if has_rel else [], dtype=np.float32)
@@ -134,7 +146,7 @@ def make_batch(problems: Sequence[Tuple], device: Optional[torch.device] = None)
       count=B, d=d, n_total=n_total, m_total=m_total, x=x, y=y,
       eps=views[6].view(torch.float32), x_off=views[0], y_off=views[1], y_prob=views[2],
       enc_src=views[3], seg_of_row=views[4], seg_off=views[5],
-      eps_rel=views[7].view(torch.float32) if has_rel else None)
+      eps_rel=views[7].view(torch.float32) if has_rel else None, enc_rows=enc_rows)
 
 
 class DeepSets:
@@ -246,19 +258,32 @@ class DeepSets:
 
 
 class DeepSetsInitializer(SinkhornInitializer):
-  """Warm start ``(f0, g^)`` from a trained :class:`DeepSets` model.
+  """Learned warm start from a trained :class:`DeepSets` model.
 
-  Follows SinkhornInitializer.__call__ (initializers.py:70-99): potentials of
-  zero-weight points are set to -inf.
+  The network predicts the potential on the points of its second cloud from
+  both clouds.  With OTT's sequential updates the first g-update discards the
+  initial g (geometry.py:297-299, 396-398), so only f0 matters: by default
+  the initializer runs the network with the clouds swapped, (y, x), to
+  predict f on x, and returns ``(f^, 0)``; the solve then starts with its
+  usual first g-update, with no extra pass over the geometry.  OTT's own
+  learned initializer (initializers/neural/meta_initializer.py:35-213) also
+  predicts f.  ``predict="g"`` predicts g on y instead; Sinkhorn then derives
+  f0 from it with one f-update inside the solve.  Potentials of zero-weight
+  points are -inf (initializers.py:70-99).
   """
 
-  predicts_g = True  # Sinkhorn derives f0 from g^ with its own first f-update
-
-  def __init__(self, model: DeepSets):
+  def __init__(self, model: DeepSets, predict: str = "f", max_pool: Optional[int] = 2048):
+    if predict not in ("f", "g"):
+      raise ValueError("predict must be 'f' or 'g'")
     self.model = model
+    self.predict = predict
+    self.max_pool = max_pool
 
-  def predict_g(self, ot_prob) -> torch.Tensor:
-    """g^ on the target points, -inf where b is zero."""
+  @property
+  def predicts_g(self) -> bool:
+    return self.predict == "g"
+
+  def _potential(self, ot_prob, side: str) -> torch.Tensor:
     geom = ot_prob.geom
     inv_s = geom.inv_scale_cost  # the network works in units of |x - y|^2
     eps_abs, eps_rel = geom._epsilon_spec()
@@ -267,17 +292,31 @@ class DeepSetsInitializer(SinkhornInitializer):
     else:
       # relative epsilon not resolved yet: the network only needs eps / mean cost
       eps, rel = 0.0, eps_rel
-    batch = make_batch([(geom.x, geom.y, eps, rel)], device=geom.device)
-    g_hat = self.model.forward(batch)
+    clouds = (geom.y, geom.x) if side == "f" else (geom.x, geom.y)
+    batch = make_batch([(clouds[0], clouds[1], eps, rel)], device=geom.device, max_pool=self.max_pool)
+    pot = self.model.forward(batch)
     if inv_s != 1.0:
-      g_hat = g_hat * inv_s
-    b = ot_prob.b
-    return torch.where(b > 0.0, g_hat, torch.full_like(g_hat, -math.inf))
+      pot = pot * inv_s
+    w = ot_prob._a if side == "f" else ot_prob._b
+    if w is not None:
+      pot = torch.where(w > 0.0, pot, torch.full_like(pot, -math.inf))
+    return pot
+
+  def predict_f(self, ot_prob) -> torch.Tensor:
+    """f^ on the source points, -inf where a is zero."""
+    return self._potential(ot_prob, "f")
+
+  def predict_g(self, ot_prob) -> torch.Tensor:
+    """g^ on the target points, -inf where b is zero."""
+    return self._potential(ot_prob, "g")
 
   def __call__(self, ot_prob, lse_mode: bool = True, **kwargs):
     if not lse_mode:
       raise NotImplementedError("the learned warm start is for lse_mode=True")
     geom = ot_prob.geom
+    if self.predict == "f":
+      f0 = self.predict_f(ot_prob)
+      return f0, torch.zeros((geom.shape[1],), dtype=torch.float32, device=geom.device)
     g_hat = self.predict_g(ot_prob)
     f0 = geom.update_potential(torch.zeros_like(ot_prob.a), g_hat, torch.log(ot_prob.a), axis=1)
     f0 = ot_prob.tau_a * f0 if ot_prob.tau_a != 1.0 else f0

@@ -2,7 +2,8 @@
 
 Targets are potentials the Sinkhorn solver itself produces: each training
 problem is solved to a tight threshold on the GPU (float32 kernel), and the
-network learns g / eps up to a per-problem constant (centred loss).  Training
+network learns the potential of the second cloud, g, up to a per-problem
+constant (centred loss); the initializer swaps the clouds to predict f.  Training
 uses the project's HIP forward / backward / Adam kernels (``ottx_ds_*``).
 The network's output is in units of sigma^2 = mean(C) / 2, so the target of a
 problem with epsilon = c mean(C) is g* / sigma^2 = 2 c g* / epsilon.
@@ -14,8 +15,8 @@ Problem families (numpy ``default_rng`` seeds):
            delta (|delta| in [0, 0.6]) and scales s in [0.7, 1.4], d = 64.
 epsilon = c * mean(C) with c drawn in [0.03, 0.1] (0.05 for evaluation).
 
-  python -m ott_amd.initializers.neural.train --kind gauss --d 64 --count 4000 \
-      --epochs 20 --out ott_amd/initializers/neural/weights/deepsets_d64.npz
+  python -m ott_amd.initializers.neural.train --kind gauss --d 64 --n 16384 --count 2000 \
+      --epochs 30 --out ott_amd/initializers/neural/weights/deepsets_d64.npz
 """
 
 from __future__ import annotations
@@ -92,13 +93,13 @@ def generate(kind: str, seeds: List[int], n: int, m: int, d: int, threshold: flo
   return dict(X=X, Y=Y, T=T, E=E, R=R, iters=torch.tensor(iters))
 
 
-def _batch_of(data, idx) -> Tuple[deepsets.Batch, torch.Tensor]:
+def _batch_of(data, idx, max_pool=None) -> Tuple[deepsets.Batch, torch.Tensor]:
   probs = [(data["X"][i], data["Y"][i], float(data["E"][i]), float(data["R"][i])) for i in idx]
-  return deepsets.make_batch(probs), data["T"][idx].reshape(-1)
+  return deepsets.make_batch(probs, max_pool=max_pool), data["T"][idx].reshape(-1)
 
 
 def train(model: deepsets.DeepSets, data, epochs: int, batch: int, lr: float, seed: int = 0,
-          log=print) -> List[float]:
+          log=print, max_pool=None) -> List[float]:
   rng = np.random.default_rng(seed)
   count = data["X"].shape[0]
   steps_per_epoch = max(1, count // batch)
@@ -111,7 +112,7 @@ def train(model: deepsets.DeepSets, data, epochs: int, batch: int, lr: float, se
     losses = []
     for s in range(steps_per_epoch):
       idx = torch.as_tensor(perm[s * batch:(s + 1) * batch], device=data["X"].device)
-      bt, target = _batch_of(data, idx.tolist())
+      bt, target = _batch_of(data, idx.tolist(), max_pool)
       loss, grad = model.loss_backward(bt, target)
       # cosine learning-rate decay
       cur = lr * 0.5 * (1.0 + math.cos(math.pi * step / total))
@@ -124,9 +125,9 @@ def train(model: deepsets.DeepSets, data, epochs: int, batch: int, lr: float, se
 
 
 def evaluate(model: deepsets.DeepSets, kind: str, seeds: List[int], n: int, m: int, d: int,
-             precision: str = "bf16_split", inner_iterations: int = 1, log=print) -> Dict:
+             precision: str = "bf16_split", inner_iterations: int = 1, log=print, max_pool=2048) -> Dict:
   """Iterations and time to 1e-3 with zero and learned initialisation."""
-  init = deepsets.DeepSetsInitializer(model)
+  init = deepsets.DeepSetsInitializer(model, max_pool=max_pool)
   rows = []
   for s in seeds:
     x, y, _ = make_problem(kind, s, n, m, d, eps_mult=0.05)
@@ -170,6 +171,7 @@ def main(argv=None) -> int:
   ap.add_argument("--epochs", type=int, default=20)
   ap.add_argument("--batch", type=int, default=16)
   ap.add_argument("--lr", type=float, default=1e-3)
+  ap.add_argument("--max-pool", type=int, default=2048, help="encoder points per cloud (0: all)")
   ap.add_argument("--out", required=True)
   args = ap.parse_args(argv)
   log = lambda s: print(s, flush=True)  # noqa: E731
@@ -178,14 +180,19 @@ def main(argv=None) -> int:
                   args.d, args.threshold, log)
   model = deepsets.DeepSets(args.d, seed=0)
   log("training")
-  hist = train(model, data, args.epochs, args.batch, args.lr, log=log)
+  pool = args.max_pool or None
+  hist = train(model, data, args.epochs, args.batch, args.lr, log=log, max_pool=pool)
   os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
-  model.save(args.out, kind=args.kind, count=args.count, epochs=args.epochs,
-             final_loss=hist[-1], target_iters_mean=float(data["iters"].float().mean()))
+  model.save(args.out, kind=args.kind, count=args.count, epochs=args.epochs, n=args.n,
+             max_pool=args.max_pool, final_loss=hist[-1],
+             target_iters_mean=float(data["iters"].float().mean()))
   log(f"saved {args.out}")
   en = args.eval_n or args.n
   seeds = list(range(args.eval_seed0, args.eval_seed0 + args.eval_count))
-  evaluate(model, args.kind, seeds, en, en, args.d, log=log)
+  # free the training set before evaluating at full size
+  del data
+  torch.cuda.empty_cache()
+  evaluate(model, args.kind, seeds, en, en, args.d, log=log, max_pool=pool)
   return 0
 
 

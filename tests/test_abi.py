@@ -47,6 +47,7 @@ def test_struct_layouts_match_header():
   assert ctypes.sizeof(_lib.SinkhornOut) == 3 * 8 + 3 * 8 + 4 * 4
   assert ctypes.sizeof(_lib.Batch) == 4 * 4 + 4 * 8 + 3 * 8 + 6 * 8
   assert ctypes.sizeof(_lib.BatchOut) == 3 * 8 + 4 * 8 + 2 * 4
+  assert ctypes.sizeof(_lib.DsBatch) == 4 * 4 + 10 * 8 + 2 * 4
 
 
 def test_batched_workspace_query_rejects_bad_batches():

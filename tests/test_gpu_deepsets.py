@@ -21,7 +21,7 @@ def _bf(t):
   return t.to(torch.bfloat16).to(torch.float64)
 
 
-def _reference(params, layout, problems, target, bf16=True):
+def _reference(params, layout, problems, target, bf16=True, pool=None):
   """Float64 forward + loss of the network; returns (g list, loss, named params)."""
   P = {}
   for name, shape in layout.shapes:
@@ -56,7 +56,11 @@ def _reference(params, layout, problems, target, bf16=True):
       a1 = torch.relu(a0 @ W["W1"].T + P["b1"])
       a1 = rnd(a1.detach()) + (a1 - a1.detach())
       return a1 @ W["W2"].T + P["b2"]
-    ex, ey = enc(xt).mean(0), enc(yt).mean(0)
+    def sub(p):  # evenly spaced encoder rows when a cloud exceeds the pool size
+      if pool is None or p.shape[0] <= pool:
+        return p
+      return p[(torch.arange(pool) * p.shape[0]) // pool]
+    ex, ey = enc(sub(xt)).mean(0), enc(sub(yt)).mean(0)
     kappa = torch.tensor(2.0 * rel, dtype=torch.float64) if rel is not None else eps / sig2
     ctx = torch.cat([ex, ey, kappa.reshape(1), torch.zeros(31, dtype=torch.float64)])
     hctx = P["Wc"] @ ctx + P["bh"]
@@ -185,7 +189,7 @@ def test_initializer_fast_path_matches_explicit_init(precision):
   with torch.no_grad():
     model.tensor("w5").copy_(torch.from_numpy(rng.normal(size=256).astype(np.float32) * 0.05))
   model.refresh()
-  init = deepsets.DeepSetsInitializer(model)
+  init = deepsets.DeepSetsInitializer(model, predict="g")
   x = torch.from_numpy(rng.normal(size=(700, d)).astype(np.float32) / 4).cuda()
   y = torch.from_numpy((rng.normal(size=(513, d)) / 4 + 0.1).astype(np.float32)).cuda()
   outs = []
@@ -217,3 +221,33 @@ def test_relative_epsilon_and_absolute_give_same_prediction():
   g_abs = model.forward(deepsets.make_batch([(torch.from_numpy(x), torch.from_numpy(y), c * meanc)]))
   g_rel = model.forward(deepsets.make_batch([(torch.from_numpy(x), torch.from_numpy(y), 0.0, c)]))
   torch.testing.assert_close(g_abs, g_rel, rtol=1e-4, atol=1e-4 * float(g_rel.abs().max()))
+
+
+def test_pooled_subsample_matches_reference():
+  """max_pool caps the encoder rows per cloud (evenly spaced); forward and
+  gradients follow the same float64 function with the subsample."""
+  from ott_amd.initializers.neural import deepsets
+  d = 6
+  model = deepsets.DeepSets(d, seed=5)
+  rng = np.random.default_rng(6)
+  with torch.no_grad():
+    model.tensor("w5").copy_(torch.from_numpy(rng.normal(size=256).astype(np.float32) * 0.05))
+  model.refresh()
+  probs = _problems(d, seed=9)  # clouds of 40 to 130 points
+  pool = 48
+  batch = deepsets.make_batch([(torch.from_numpy(x), torch.from_numpy(y), e) for x, y, e in probs], max_pool=pool)
+  assert batch.enc_rows == sum(min(x.shape[0], pool) + min(y.shape[0], pool) for x, y, _ in probs)
+  m_total = sum(y.shape[0] for _, y, _ in probs)
+  target = rng.normal(size=m_total).astype(np.float32)
+  g = model.forward(batch).cpu().numpy().astype(np.float64)
+  outs, loss_ref, _, P = _reference(model.params.cpu().numpy(), model.layout, probs, target, pool=pool)
+  g_ref = torch.cat(outs).detach().numpy()
+  assert np.max(np.abs(g - g_ref)) < 2e-3 * np.max(np.abs(g_ref))
+  _, grad = model.loss_backward(batch, torch.from_numpy(target))
+  loss_ref.backward()
+  grad = grad.cpu().numpy().astype(np.float64)
+  for name in ("W0", "W2", "Wc", "w5"):
+    off, shape = model.layout.slice(name)
+    got = grad[off:off + int(np.prod(shape))]
+    want = P[name].grad.detach().numpy().reshape(-1)
+    assert np.linalg.norm(got - want) < 3e-2 * np.linalg.norm(want), name
