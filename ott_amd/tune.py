@@ -85,8 +85,9 @@ def benchmark(geom, tile_config: int, axis: int, reps: int = 10) -> float:
   """Mean milliseconds of one half-step of ``geom`` with ``tile_config``."""
   lib = _lib.load_library()
   prob, keep = geom._problem()
+  # the workspace of the configuration being timed
   opts = _lib.SinkhornOpts(threshold=1e-3, norm_error=1, inner_iterations=10, min_iterations=0,
-                           max_iterations=10, momentum=1.0, tile_config=-1, use_graph=0)
+                           max_iterations=10, momentum=1.0, tile_config=tile_config, use_graph=0)
   nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(prob), ctypes.byref(opts))
   ws = _lib.workspace(nbytes, geom.device)
   n, m = geom.shape
