@@ -6,10 +6,17 @@
 // This network instead reads the point clouds themselves:
 //   encoder  phi: R^d -> R^256, 3 layers (ReLU, ReLU, linear), shared by x and y
 //   pooling  e_x = mean_i phi(x~_i),  e_y = mean_j phi(y~_j)
-//   head     g~_j = psi(y~_j, e_x, e_y, kappa), 3 layers (ReLU, ReLU, linear)
+//   context  ctx = [e_x, e_y, kappa, s_x - 1, s_y - 1, mean y~, 0 ...] (CTX wide)
+//   head     g~_j = psi(y~_j, ctx), 3 layers (ReLU, ReLU, linear),
+//            plus a context-conditioned linear + quadratic term z . in_j with
+//            z = Wq ctx and in_j = [y~_j, |y~_j|^2 - 1, 0 ...] the head input
 // with x~ = (x - mu) / sigma, y~ = (y - mu) / sigma per problem
-// (mu = (mean x + mean y) / 2, sigma^2 = mean(C) / 2) and kappa = eps / sigma^2.
-// The prediction is g_j = sigma^2 * g~_j (a potential, up to a constant).
+// (mu = (mean x + mean y) / 2, sigma^2 = mean(C) / 2), kappa = eps / sigma^2,
+// s_x = mean |x~|^2, s_y = mean |y~|^2 (exact moments over every point; the
+// mean of x~ is minus that of y~).  The quadratic term carries the part of the
+// potential that is quadratic in the point (exact for Gaussian clouds); the MLP
+// the rest.  The prediction is g_j = sigma^2 * g~_j (a potential, up to a
+// constant).
 //
 // Every matrix product runs through one bf16 MFMA GEMM, C = A . Bt^T with both
 // operands K-contiguous ("NT"), f32 accumulation, and an epilogue that adds a
@@ -135,7 +142,7 @@ __global__ void sum_splits_kernel(const float* __restrict__ part, int splits, in
 // sigma^2 = mean(C) / 2 with mean(C) = mean|x|^2 + mean|y|^2 - 2 <mean x, mean y>.
 // One workgroup per problem (thread per feature, rows strided).
 // ---------------------------------------------------------------------------
-constexpr int kStatChunks = 64;
+constexpr int kStatChunks = 256;
 
 // Per-(problem, chunk) float64 partial sums of the coordinates and their
 // squares, for x and y: part[((b * kStatChunks + c) * 4 + q) * d + k] with
@@ -173,37 +180,68 @@ __global__ void stats_partial_kernel(const float* __restrict__ x, const float* _
   }
 }
 
-// mu_b = (mean x + mean y) / 2 and sigma_b^2 = mean(C) / 2 with
-// mean(C) = mean |x|^2 + mean |y|^2 - 2 <mean x, mean y>.  Grid B.
+// Context width: [e_x (256), e_y (256), kappa, s_x - 1, s_y - 1, mean y~ (d)]
+// padded to a multiple of 32.
+__host__ __device__ inline int ctx_width(int d) { return 512 + (3 + d + 31) / 32 * 32; }
+
+// mu_b = (mean x + mean y) / 2, sigma_b^2 = mean(C) / 2 with
+// mean(C) = var x + var y + |mean x - mean y|^2 (var = mean |p - mean p|^2), and
+// the moments mom_b = [s_x - 1, s_y - 1, mean y~ (d)] with
+// s_x = (var x + |mean x - mean y|^2 / 4) / sigma^2 (s_y alike) and
+// mean y~ = (mean y - mean x) / (2 sigma).  Grid B; thread (feature k, group g)
+// sums the chunks c = g mod G, then the groups and features are reduced.
 __global__ void stats_final_kernel(const double* __restrict__ part, const int* __restrict__ xoff,
                                    const int* __restrict__ yoff, int d, float* __restrict__ mu,
-                                   float* __restrict__ sig2) {
-  __shared__ double red[3][256];
+                                   float* __restrict__ sig2, float* __restrict__ mom) {
+  __shared__ double red[4][256];
+  __shared__ double fx[256], fy[256];
+  __shared__ double tot[3][256];
   const int b = blockIdx.x, t = threadIdx.x;
   const int nx = xoff[b + 1] - xoff[b], ny = yoff[b + 1] - yoff[b];
-  double nx2 = 0, ny2 = 0, dot = 0;
-  if (t < d) {
-    double q[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int c = 0; c < kStatChunks; ++c)
-      for (int j = 0; j < 4; ++j) q[j] += part[((static_cast<int64_t>(b) * kStatChunks + c) * 4 + j) * d + t];
-    const double mx = q[0] / nx, my = q[2] / ny;
-    mu[b * d + t] = static_cast<float>(0.5 * (mx + my));
-    nx2 = q[1];
-    ny2 = q[3];
-    dot = mx * my;
-  }
-  red[0][t] = nx2;
-  red[1][t] = ny2;
-  red[2][t] = dot;
+  const int G = max(1, 256 / d);
+  const int k = t % d, g = t / d;
+  double q[4] = {0.0, 0.0, 0.0, 0.0};
+  if (t < G * d)
+    for (int c = g; c < kStatChunks; c += G)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) q[j] += part[((static_cast<int64_t>(b) * kStatChunks + c) * 4 + j) * d + k];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) red[j][t] = q[j];
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s)
-      for (int q = 0; q < 3; ++q) red[q][t] += red[q][t + s];
+  double va = 0.0, vb = 0.0, dm = 0.0;
+  if (t < d) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int gg = 0; gg < G; ++gg)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) s[j] += red[j][gg * d + t];
+    const double mx = s[0] / nx, my = s[2] / ny;
+    fx[t] = mx;
+    fy[t] = my;
+    va = s[1] / nx - mx * mx;
+    vb = s[3] / ny - my * my;
+    dm = (mx - my) * (mx - my);
+  }
+  tot[0][t] = va;
+  tot[1][t] = vb;
+  tot[2][t] = dm;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) tot[j][t] += tot[j][t + w];
     __syncthreads();
   }
+  const double A = tot[0][0], Bv = tot[1][0], Cc = tot[2][0];
+  const double s2 = fmax(0.5 * (A + Bv + Cc), 1e-30);
+  const double inv_sig = 1.0 / sqrt(s2);
+  if (t < d) {
+    mu[b * d + t] = static_cast<float>(0.5 * (fx[t] + fy[t]));
+    mom[static_cast<int64_t>(b) * (d + 2) + 2 + t] = static_cast<float>(0.5 * (fy[t] - fx[t]) * inv_sig);
+  }
   if (t == 0) {
-    const double mc = red[0][0] / nx + red[1][0] / ny - 2.0 * red[2][0];
-    sig2[b] = static_cast<float>(fmax(0.5 * mc, 1e-30));
+    sig2[b] = static_cast<float>(s2);
+    mom[static_cast<int64_t>(b) * (d + 2)] = static_cast<float>((A + 0.25 * Cc) / s2 - 1.0);
+    mom[static_cast<int64_t>(b) * (d + 2) + 1] = static_cast<float>((Bv + 0.25 * Cc) / s2 - 1.0);
   }
 }
 
@@ -262,39 +300,77 @@ __global__ void pool_partial_kernel(const float* __restrict__ H, const int* __re
   part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f] = s;
 }
 
-// ctx[b] = [e_x (256), e_y (256), kappa = eps / sigma^2, 0 pad ...] (ld 544).
+// ctx[b] = [e_x (256), e_y (256), kappa = eps / sigma^2, mom_b (d + 2), 0 pad ...]
+// (ld ctx_width(d)).  Grid B.
 __global__ void pool_final_kernel(const float* __restrict__ part, const int* __restrict__ seg_off, int chunks,
                                   const float* __restrict__ eps, const float* __restrict__ eps_rel,
-                                  const float* __restrict__ sig2, float* __restrict__ ctx) {
+                                  const float* __restrict__ sig2, const float* __restrict__ mom, int d,
+                                  float* __restrict__ ctx) {
   const int b = blockIdx.x, f = threadIdx.x;
+  const int W = ctx_width(d);
+  float* cb = ctx + static_cast<int64_t>(b) * W;
   for (int side = 0; side < 2; ++side) {
     const int seg = b * 2 + side;
     float s = 0.f;
     for (int c = 0; c < chunks; ++c) s += part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f];
-    ctx[b * 544 + side * 256 + f] = s / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
+    cb[side * 256 + f] = s / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
   }
   // kappa = eps / sigma^2; with eps = c mean(C) and sigma^2 = mean(C) / 2 it is 2 c.
   const float kappa = (eps_rel && eps_rel[b] > 0.f) ? 2.f * eps_rel[b] : eps[b] / sig2[b];
-  if (f < 32) ctx[b * 544 + 512 + f] = (f == 0) ? kappa : 0.f;
+  for (int i = f; i < W - 512; i += 256) {
+    float v = 0.f;
+    if (i == 0) v = kappa;
+    else if (i <= d + 2) v = mom[static_cast<int64_t>(b) * (d + 2) + i - 1];
+    cb[512 + i] = v;
+  }
 }
 
-// hctx[b, o] = sum_k Wc[o, k] ctx[b, k] + bh[o]   (Wc [256, 544] f32).
+// Context heads, one wave per output row:
+//   rows o < 256:        hctx[b, o] = bh[o] + sum_c Wc[o, c] ctx[b, c]
+//   rows o = 256 + k:    zq[b, k]   = sum_c Wq[k, c] ctx[b, c]   (k < kin)
+// Grid ((256 + kin) / 4, B), block 256.
 __global__ void ctx_forward_kernel(const float* __restrict__ Wc, const float* __restrict__ bh,
-                                   const float* __restrict__ ctx, float* __restrict__ hctx) {
-  const int b = blockIdx.x, o = threadIdx.x;
-  float s = bh[o];
-  for (int k = 0; k < 544; ++k) s = fmaf(Wc[o * 544 + k], ctx[b * 544 + k], s);
-  hctx[b * 256 + o] = s;
+                                   const float* __restrict__ Wq, const float* __restrict__ ctx, int d, int kin,
+                                   float* __restrict__ hctx, float* __restrict__ zq) {
+  const int lane = threadIdx.x & 63;
+  const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int b = blockIdx.y;
+  if (o >= 256 + kin) return;
+  const int W = ctx_width(d);
+  const float* w = o < 256 ? Wc + static_cast<int64_t>(o) * W : Wq + static_cast<int64_t>(o - 256) * W;
+  const float* cb = ctx + static_cast<int64_t>(b) * W;
+  float s = 0.f;
+  for (int c = lane; c < W; c += 64) s = fmaf(w[c], cb[c], s);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) {
+    if (o < 256) hctx[b * 256 + o] = s + bh[o];
+    else zq[b * kin + (o - 256)] = s;
+  }
 }
 
-// Last layer: out[q] = sum_k w5[k] A[q, k] + b5 (A f32 [P, 256]).
+// Head output, one wave per row q of problem b = prob[q]:
+//   out[q] = b5 + sum_k w5[k] A[q, k] + sum_{k < kin} zq[b, k] Y0[q, k]
+// (A f32 [P, 256], Y0 the bf16 head input [P, kin]); g[q] = sigma_b^2 out[q]
+// when g is given.  Grid ceil(rows / 4), block 256.
 __global__ void head_out_kernel(const float* __restrict__ A, const float* __restrict__ w5, const float* __restrict__ b5,
-                                int64_t rows, float* __restrict__ out) {
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+                                const bf16_t* __restrict__ Y0, int kin, const float* __restrict__ zq,
+                                const int* __restrict__ prob, int64_t rows, float* __restrict__ out,
+                                const float* __restrict__ sig2, float* __restrict__ g) {
+  const int lane = threadIdx.x & 63;
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (q >= rows) return;
-  float s = b5[0];
-  for (int k = 0; k < 256; ++k) s = fmaf(w5[k], A[q * 256 + k], s);
-  out[q] = s;
+  const int b = prob[q];
+  const float* a = A + q * 256;
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) s = fmaf(w5[lane + 64 * i], a[lane + 64 * i], s);
+  for (int k = lane; k < kin; k += 64) s = fmaf(zq[b * kin + k], from_bf16(Y0[q * kin + k]), s);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) {
+    const float v = s + b5[0];
+    out[q] = v;
+    if (g) g[q] = sig2[b] * v;
+  }
 }
 
 // Centred MSE per problem on the head points:
@@ -361,6 +437,21 @@ __global__ void colsum_partial_kernel(const bf16_t* __restrict__ Mb, const float
   }
 }
 
+// Gradient of the head's second bias in float32 (the centred loss makes it a
+// sum with heavy cancellation, so it is not taken from the bf16 copy of dZ):
+// part[c][k] = w5[k] sum_{q in chunk c} dout[q] (A[q, k] > 0).
+__global__ void head_bias_grad_partial_kernel(const float* __restrict__ dout, const float* __restrict__ w5,
+                                              const float* __restrict__ A, int64_t rows, int chunks,
+                                              float* __restrict__ part) {
+  const int c = blockIdx.x, k = threadIdx.x;
+  const int64_t per = (rows + chunks - 1) / chunks;
+  const int64_t r0 = c * per, r1 = min(rows, r0 + per);
+  float s = 0.f;
+  for (int64_t q = r0; q < r1; ++q)
+    if (A[q * 256 + k] > 0.f) s += dout[q];
+  part[static_cast<int64_t>(c) * 256 + k] = w5[k] * s;
+}
+
 // Per-group row sums: out[b, :] = sum_{r in [off[b], off[b+1])} M[r, :] (bf16 M).
 __global__ void group_sum_kernel(const bf16_t* __restrict__ M, const int* __restrict__ off, int N,
                                  float* __restrict__ out) {
@@ -387,27 +478,61 @@ __global__ void head_out_wgrad_kernel(const float* __restrict__ dout, const floa
   if (k == 0) part[static_cast<int64_t>(c) * 257 + 256] = sb;
 }
 
-// Context backward: dWc[o, k] += sum_b dh[b, o] ctx[b, k]; dctx[b, k] = sum_o Wc[o, k] dh[b, o].
-__global__ void ctx_backward_kernel(const float* __restrict__ Wc, const float* __restrict__ ctx,
-                                    const float* __restrict__ dh, int nprob, float* __restrict__ dWc,
-                                    float* __restrict__ dctx) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= 544) return;
+// Gradient of the head's quadratic term with respect to z:
+// part[(b * chunks + c) * kin + k] = sum_{q in chunk c of problem b} dout[q] Y0[q, k].
+// Grid (chunks, B), block 256 (k = thread < kin).
+__global__ void quad_grad_partial_kernel(const float* __restrict__ dout, const bf16_t* __restrict__ Y0, int kin,
+                                         const int* __restrict__ yoff, int chunks, float* __restrict__ part) {
+  const int c = blockIdx.x, b = blockIdx.y, k = threadIdx.x;
+  if (k >= kin) return;
+  const int q0 = yoff[b], q1 = yoff[b + 1];
+  const int per = (q1 - q0 + chunks - 1) / chunks;
+  const int lo = q0 + c * per, hi = min(q1, lo + per);
+  float s = 0.f;
+  for (int q = lo; q < hi; ++q) s = fmaf(dout[q], from_bf16(Y0[static_cast<int64_t>(q) * kin + k]), s);
+  part[(static_cast<int64_t>(b) * chunks + c) * kin + k] = s;
+}
+
+// dzq[b, k] = sum_c part[(b * chunks + c) * kin + k].  Grid B, block 256.
+__global__ void quad_grad_final_kernel(const float* __restrict__ part, int kin, int chunks, float* __restrict__ dzq) {
+  const int b = blockIdx.x, k = threadIdx.x;
+  if (k >= kin) return;
+  float s = 0.f;
+  for (int c = 0; c < chunks; ++c) s += part[(static_cast<int64_t>(b) * chunks + c) * kin + k];
+  dzq[b * kin + k] = s;
+}
+
+// Context backward, thread per context column c (W = ctx_width(d)):
+//   dWc[o, c] = sum_b dh[b, o] ctx[b, c],  dWq[k, c] = sum_b dzq[b, k] ctx[b, c],
+//   dctx[b, c] = sum_o Wc[o, c] dh[b, o] + sum_k Wq[k, c] dzq[b, k].
+__global__ void ctx_backward_kernel(const float* __restrict__ Wc, const float* __restrict__ Wq,
+                                    const float* __restrict__ ctx, const float* __restrict__ dh,
+                                    const float* __restrict__ dzq, int nprob, int d, int kin,
+                                    float* __restrict__ dWc, float* __restrict__ dWq, float* __restrict__ dctx) {
+  const int W = ctx_width(d);
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= W) return;
   for (int o = 0; o < 256; ++o) {
     float s = 0.f;
-    for (int b = 0; b < nprob; ++b) s = fmaf(dh[b * 256 + o], ctx[b * 544 + k], s);
-    dWc[o * 544 + k] = s;
+    for (int b = 0; b < nprob; ++b) s = fmaf(dh[b * 256 + o], ctx[static_cast<int64_t>(b) * W + c], s);
+    dWc[static_cast<int64_t>(o) * W + c] = s;
+  }
+  for (int k = 0; k < kin; ++k) {
+    float s = 0.f;
+    for (int b = 0; b < nprob; ++b) s = fmaf(dzq[b * kin + k], ctx[static_cast<int64_t>(b) * W + c], s);
+    dWq[static_cast<int64_t>(k) * W + c] = s;
   }
   for (int b = 0; b < nprob; ++b) {
     float s = 0.f;
-    for (int o = 0; o < 256; ++o) s = fmaf(Wc[o * 544 + k], dh[b * 256 + o], s);
-    dctx[b * 544 + k] = s;
+    for (int o = 0; o < 256; ++o) s = fmaf(Wc[static_cast<int64_t>(o) * W + c], dh[b * 256 + o], s);
+    for (int k = 0; k < kin; ++k) s = fmaf(Wq[static_cast<int64_t>(k) * W + c], dzq[b * kin + k], s);
+    dctx[static_cast<int64_t>(b) * W + c] = s;
   }
 }
 
 // Pool backward: dH[r, f] = dctx[b(r), side(r) * 256 + f] / count(segment),
 // bf16 and transposed bf16, zero on padded rows.
-__global__ void pool_backward_kernel(const float* __restrict__ dctx, const int* __restrict__ seg_of_row,
+__global__ void pool_backward_kernel(const float* __restrict__ dctx, int ctx_w, const int* __restrict__ seg_of_row,
                                      const int* __restrict__ seg_off, int64_t rows, int64_t rows_pad,
                                      bf16_t* __restrict__ dH, bf16_t* __restrict__ dHT) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -417,7 +542,8 @@ __global__ void pool_backward_kernel(const float* __restrict__ dctx, const int* 
   float v = 0.f;
   if (r < rows) {
     const int seg = seg_of_row[r];
-    v = dctx[(seg >> 1) * 544 + (seg & 1) * 256 + f] / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
+    v = dctx[static_cast<int64_t>(seg >> 1) * ctx_w + (seg & 1) * 256 + f] /
+        static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
   }
   const bf16_t h = to_bf16(v);
   dH[i] = h;
@@ -450,13 +576,6 @@ __global__ void to_bf16_kernel(const float* __restrict__ src, int rows, int cols
   if (dstT) dstT[static_cast<int64_t>(c) * ld_dstT + r] = h;
 }
 
-// The head's prediction as a potential: g[q] = sigma_b^2 * out[q].
-__global__ void scale_out_kernel(const float* __restrict__ out, const int* __restrict__ prob,
-                                 const float* __restrict__ sig2, int64_t rows, float* __restrict__ g) {
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q < rows) g[q] = sig2[prob[q]] * out[q];
-}
-
 }  // namespace ottx_ds
 
 using namespace ottx_ds;
@@ -475,12 +594,14 @@ inline size_t al(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 // Parameter layout inside the flat f32 buffer (offsets in floats).
 struct ParamLayout {
   int kin;  // padded input width (multiple of 64: a GEMM N dimension in the gradients)
-  int64_t W0, b0, W1, b1, W2, b2, Wy, Wc, bh, W4, b4, w5, b5, total;
+  int ctx;  // context width
+  int64_t W0, b0, W1, b1, W2, b2, Wy, Wc, bh, W4, b4, w5, b5, Wq, total;
 };
 
 ParamLayout param_layout(int d) {
   ParamLayout L;
   L.kin = (d + 1 + 63) / 64 * 64;  // d coordinates + the squared-norm feature
+  L.ctx = ctx_width(d);
   int64_t o = 0;
   L.W0 = o; o += 256LL * L.kin;
   L.b0 = o; o += 256;
@@ -489,12 +610,13 @@ ParamLayout param_layout(int d) {
   L.W2 = o; o += 256LL * 256;
   L.b2 = o; o += 256;
   L.Wy = o; o += 256LL * L.kin;
-  L.Wc = o; o += 256LL * 544;
+  L.Wc = o; o += 256LL * L.ctx;
   L.bh = o; o += 256;
   L.W4 = o; o += 256LL * 256;
   L.b4 = o; o += 256;
   L.w5 = o; o += 256;
   L.b5 = o; o += 64;
+  L.Wq = o; o += static_cast<int64_t>(L.kin) * L.ctx;
   L.total = o;
   return L;
 }
@@ -525,8 +647,8 @@ WeightCopies weight_copies(const ParamLayout& L) {
 struct DsLayout {
   int64_t Pe, Pe_pad, Ph, Ph_pad;  // encoder / head rows (valid, padded)
   int nprob, chunks, kchunks;
-  size_t mu, sig2, spart, kappa, X0, Y0, A0, A0T, A1, A1T, H2f, part, ctx, hctx, Z1h, Z1hT, A2h, A2hT, A2f, out, dout, loss,
-      dZ, dZT, dA, dAT, dctx, dh, gpart, total;
+  size_t mu, sig2, mom, spart, kappa, X0, Y0, A0, A0T, A1, A1T, H2f, part, ctx, hctx, zq, Z1h, Z1hT, A2h, A2hT, A2f,
+      out, dout, loss, dZ, dZT, dA, dAT, dctx, dh, qpart, dzq, gpart, total;
 };
 
 DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training) {
@@ -536,7 +658,7 @@ DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training)
   L.Pe_pad = pad64(L.Pe);
   L.Ph = bt->m_total;
   L.Ph_pad = pad64(L.Ph);
-  L.chunks = 128;
+  L.chunks = 16;
   L.kchunks = training ? 64 : 0;
   size_t o = 0;
   auto take = [&](size_t b) {
@@ -549,6 +671,7 @@ DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training)
   const int B = L.nprob;
   L.mu = take(static_cast<size_t>(B) * bt->d * 4);
   L.sig2 = take(static_cast<size_t>(B) * 4);
+  L.mom = take(static_cast<size_t>(B) * (bt->d + 2) * 4);
   L.spart = take(static_cast<size_t>(B) * kStatChunks * 4 * bt->d * 8);
   L.kappa = take(static_cast<size_t>(B) * 4);
   L.X0 = take(static_cast<size_t>(Pe) * P.kin * 2 * (training ? 2 : 1));  // X0 (and X0T)
@@ -559,8 +682,9 @@ DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training)
   L.A1T = take_t(static_cast<size_t>(Pe) * 256 * 2);
   L.H2f = take(static_cast<size_t>(Pe) * 256 * 4);
   L.part = take(static_cast<size_t>(2 * B) * L.chunks * 256 * 4);
-  L.ctx = take(static_cast<size_t>(B) * 544 * 4);
+  L.ctx = take(static_cast<size_t>(B) * P.ctx * 4);
   L.hctx = take(static_cast<size_t>(B) * 256 * 4);
+  L.zq = take(static_cast<size_t>(B) * P.kin * 4);
   L.Z1h = take(static_cast<size_t>(Ph) * 256 * 2);
   L.Z1hT = take_t(static_cast<size_t>(Ph) * 256 * 2);
   L.A2h = take(static_cast<size_t>(Ph) * 256 * 2);
@@ -574,9 +698,12 @@ DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training)
   L.dZT = take_t(static_cast<size_t>(Pm) * 256 * 2);
   L.dA = take_t(static_cast<size_t>(Pm) * 256 * 2);
   L.dAT = take_t(static_cast<size_t>(Pm) * 256 * 2);
-  L.dctx = take_t(static_cast<size_t>(B) * 544 * 4);
+  L.dctx = take_t(static_cast<size_t>(B) * P.ctx * 4);
   L.dh = take_t(static_cast<size_t>(B) * 256 * 4);
-  L.gpart = take_t(static_cast<size_t>(L.kchunks) * 256 * 544 * 4);
+  L.qpart = take_t(static_cast<size_t>(B) * L.chunks * P.kin * 4);
+  L.dzq = take_t(static_cast<size_t>(B) * P.kin * 4);
+  // split-K partials of the weight gradients (at most 256 x 256 per chunk)
+  L.gpart = take_t(static_cast<size_t>(L.kchunks) * 256 * 256 * 4);
   L.total = o;
   return L;
 }
@@ -644,8 +771,9 @@ T* at(char* base, size_t off) {
 }
 
 // The forward pass up to the head output (out[Ph]).
+// g_out (optional) receives the prediction as a potential, sigma^2 * out.
 int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, hipStream_t st, bool training,
-            Forward* F) {
+            float* g_out, Forward* F) {
   F->P = param_layout(bt->d);
   F->C = weight_copies(F->P);
   F->L = ds_layout(bt, F->P, training);
@@ -665,7 +793,9 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   double* spart = at<double>(ws, L.spart);
   hipLaunchKernelGGL(stats_partial_kernel, dim3(kStatChunks, B), dim3(256), 0, st, bt->x, bt->y, bt->x_off,
                      bt->y_off, bt->d, spart);
-  hipLaunchKernelGGL(stats_final_kernel, dim3(B), dim3(256), 0, st, spart, bt->x_off, bt->y_off, bt->d, mu, sig2);
+  float* mom = at<float>(ws, L.mom);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(B), dim3(256), 0, st, spart, bt->x_off, bt->y_off, bt->d, mu, sig2,
+                     mom);
   DS_LAUNCHED();
   bf16_t* X0 = at<bf16_t>(ws, L.X0);
   bf16_t* X0T = X0 + Pe * P.kin;
@@ -710,9 +840,9 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   hipLaunchKernelGGL(pool_partial_kernel, dim3(L.chunks, 2 * B), dim3(256), 0, st, at<float>(ws, L.H2f), bt->seg_off,
                      L.chunks, at<float>(ws, L.part));
   hipLaunchKernelGGL(pool_final_kernel, dim3(B), dim3(256), 0, st, at<float>(ws, L.part), bt->seg_off, L.chunks,
-                     bt->eps, bt->eps_rel, sig2, at<float>(ws, L.ctx));
-  hipLaunchKernelGGL(ctx_forward_kernel, dim3(B), dim3(256), 0, st, prm + P.Wc, prm + P.bh, at<float>(ws, L.ctx),
-                     at<float>(ws, L.hctx));
+                     bt->eps, bt->eps_rel, sig2, mom, bt->d, at<float>(ws, L.ctx));
+  hipLaunchKernelGGL(ctx_forward_kernel, dim3((256 + P.kin + 3) / 4, B), dim3(256), 0, st, prm + P.Wc, prm + P.bh,
+                     prm + P.Wq, at<float>(ws, L.ctx), bt->d, P.kin, at<float>(ws, L.hctx), at<float>(ws, L.zq));
   DS_LAUNCHED();
   // head layer 1: Z1 = relu(Y0 Wy^T + hctx[b])
   g = gemm_args(Y0, P.kin, wb + C.Wy, P.kin, static_cast<int>(Ph), 256, P.kin);
@@ -737,8 +867,9 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   g.ldct = Ph;
   g.valid_rows = static_cast<int>(L.Ph);
   if ((rc = gemm(st, g)) != OTTX_OK) return rc;
-  hipLaunchKernelGGL(head_out_kernel, dim3(blocks_for(L.Ph)), dim3(256), 0, st, at<float>(ws, L.A2f), prm + P.w5,
-                     prm + P.b5, L.Ph, at<float>(ws, L.out));
+  hipLaunchKernelGGL(head_out_kernel, dim3(static_cast<unsigned>((L.Ph + 3) / 4)), dim3(256), 0, st,
+                     at<float>(ws, L.A2f), prm + P.w5, prm + P.b5, Y0, P.kin, at<float>(ws, L.zq), bt->y_prob, L.Ph,
+                     at<float>(ws, L.out), sig2, g_out);
   DS_LAUNCHED();
   return OTTX_OK;
 }
@@ -794,12 +925,7 @@ int ottx_ds_forward(const ottx_ds_model* md, const ottx_ds_batch* bt, float* g_o
   if (workspace_bytes < need || (reinterpret_cast<uintptr_t>(workspace) & 255)) return OTTX_EWORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
   Forward F;
-  int rc = forward(bt, md, workspace, st, false, &F);
-  if (rc != OTTX_OK) return rc;
-  hipLaunchKernelGGL(scale_out_kernel, dim3(blocks_for(F.L.Ph)), dim3(256), 0, st, at<float>(F.ws, F.L.out),
-                     bt->y_prob, at<float>(F.ws, F.L.sig2), F.L.Ph, g_out);
-  DS_LAUNCHED();
-  return OTTX_OK;
+  return forward(bt, md, workspace, st, false, g_out, &F);
 }
 
 int ottx_ds_loss_backward(const ottx_ds_model* md, const ottx_ds_batch* bt, const float* target, float* grad,
@@ -810,7 +936,7 @@ int ottx_ds_loss_backward(const ottx_ds_model* md, const ottx_ds_batch* bt, cons
   if (workspace_bytes < need || (reinterpret_cast<uintptr_t>(workspace) & 255)) return OTTX_EWORKSPACE;
   hipStream_t st = static_cast<hipStream_t>(stream);
   Forward F;
-  int rc = forward(bt, md, workspace, st, true, &F);
+  int rc = forward(bt, md, workspace, st, true, nullptr, &F);
   if (rc != OTTX_OK) return rc;
   const ParamLayout& P = F.P;
   const WeightCopies& C = F.C;
@@ -841,7 +967,11 @@ int ottx_ds_loss_backward(const ottx_ds_model* md, const ottx_ds_batch* bt, cons
   // head layer 2: dW4 = dZ^T Z1, db4 = colsum dZ, dZ1 = (dZ W4) * (Z1 > 0)
   if ((rc = weight_grad(st, dZT, Ph, at<bf16_t>(ws, L.Z1hT), 256, 256, gpart, L.kchunks, grad + P.W4)) != OTTX_OK)
     return rc;
-  if ((rc = bias_grad(st, dZ, Ph, 256, gpart, L.kchunks, grad + P.b4)) != OTTX_OK) return rc;
+  hipLaunchKernelGGL(head_bias_grad_partial_kernel, dim3(L.kchunks), dim3(256), 0, st, dout, prm + P.w5,
+                     at<float>(ws, L.A2f), L.Ph, L.kchunks, gpart);
+  hipLaunchKernelGGL(sum_splits_kernel, dim3(1), dim3(256), 0, st, gpart, L.kchunks, static_cast<int64_t>(256),
+                     grad + P.b4, 0);
+  DS_LAUNCHED();
   GemmArgs g = gemm_args(dZ, 256, wb + C.W4T, 256, static_cast<int>(Ph), 256, 256);
   g.mask = at<bf16_t>(ws, L.Z1h);
   g.ldm = 256;
@@ -858,13 +988,19 @@ int ottx_ds_loss_backward(const ottx_ds_model* md, const ottx_ds_batch* bt, cons
   float* dh = at<float>(ws, L.dh);
   hipLaunchKernelGGL(group_sum_kernel, dim3(B), dim3(256), 0, st, dA, bt->y_off, 256, dh);
   hipLaunchKernelGGL(sum_splits_kernel, dim3(1), dim3(256), 0, st, dh, B, static_cast<int64_t>(256), grad + P.bh, 0);
+  // quadratic term: dz[b] = sum_{rows of b} dout Y0
+  float* dzq = at<float>(ws, L.dzq);
+  hipLaunchKernelGGL(quad_grad_partial_kernel, dim3(L.chunks, B), dim3(256), 0, st, dout, at<bf16_t>(ws, L.Y0),
+                     P.kin, bt->y_off, L.chunks, at<float>(ws, L.qpart));
+  hipLaunchKernelGGL(quad_grad_final_kernel, dim3(B), dim3(256), 0, st, at<float>(ws, L.qpart), P.kin, L.chunks,
+                     dzq);
   float* dctx = at<float>(ws, L.dctx);
-  hipLaunchKernelGGL(ctx_backward_kernel, dim3(blocks_for(544)), dim3(256), 0, st, prm + P.Wc, at<float>(ws, L.ctx),
-                     dh, B, grad + P.Wc, dctx);
+  hipLaunchKernelGGL(ctx_backward_kernel, dim3(blocks_for(P.ctx)), dim3(256), 0, st, prm + P.Wc, prm + P.Wq,
+                     at<float>(ws, L.ctx), dh, dzq, B, bt->d, P.kin, grad + P.Wc, grad + P.Wq, dctx);
   DS_LAUNCHED();
   // pool backward -> dH2 (encoder output gradient)
-  hipLaunchKernelGGL(pool_backward_kernel, dim3(blocks_for(Pe * 256)), dim3(256), 0, st, dctx, bt->seg_of_row,
-                     bt->seg_off, L.Pe, Pe, dZ, dZT);
+  hipLaunchKernelGGL(pool_backward_kernel, dim3(blocks_for(Pe * 256)), dim3(256), 0, st, dctx, P.ctx,
+                     bt->seg_of_row, bt->seg_off, L.Pe, Pe, dZ, dZT);
   DS_LAUNCHED();
   // encoder layer 2 (linear): dW2 = dH2^T A1, db2, dA1 = (dH2 W2) * (A1 > 0)
   if ((rc = weight_grad(st, dZT, Pe, at<bf16_t>(ws, L.A1T), 256, 256, gpart, L.kchunks, grad + P.W2)) != OTTX_OK)

@@ -31,7 +31,11 @@ from ott_amd.initializers.linear.initializers import SinkhornInitializer
 __all__ = ["ParamLayout", "param_layout", "DeepSets", "Batch", "make_batch", "DeepSetsInitializer"]
 
 WIDTH = 256
-CTX = 544  # [e_x (256), e_y (256), kappa, zero padding]
+
+
+def ctx_width(d: int) -> int:
+  """Context width: [e_x (256), e_y (256), kappa, s_x - 1, s_y - 1, mean y~ (d)], padded to 32."""
+  return 512 + (3 + d + 31) // 32 * 32
 
 
 @dataclasses.dataclass(frozen=True)
@@ -52,10 +56,12 @@ class ParamLayout:
 def param_layout(d: int) -> ParamLayout:
   """Mirror of ``param_layout`` in ottx_deepsets.hip (inputs: d coordinates + |p~|^2 - 1)."""
   kin = (d + 1 + 63) // 64 * 64
+  ctx = ctx_width(d)
   shapes = (
       ("W0", (WIDTH, kin)), ("b0", (WIDTH,)), ("W1", (WIDTH, WIDTH)), ("b1", (WIDTH,)),
-      ("W2", (WIDTH, WIDTH)), ("b2", (WIDTH,)), ("Wy", (WIDTH, kin)), ("Wc", (WIDTH, CTX)),
+      ("W2", (WIDTH, WIDTH)), ("b2", (WIDTH,)), ("Wy", (WIDTH, kin)), ("Wc", (WIDTH, ctx)),
       ("bh", (WIDTH,)), ("W4", (WIDTH, WIDTH)), ("b4", (WIDTH,)), ("w5", (WIDTH,)), ("b5", (64,)),
+      ("Wq", (kin, ctx)),
   )
   offsets, o = [], 0
   for name, shape in shapes:
@@ -108,9 +114,37 @@ def make_batch(problems: Sequence[Tuple], device: Optional[torch.device] = None,
   xs = [torch.as_tensor(p[0], dtype=torch.float32).to(device) for p in problems]
   ys = [torch.as_tensor(p[1], dtype=torch.float32).to(device) for p in problems]
   d = xs[0].shape[1]
-  ns = np.array([x.shape[0] for x in xs], dtype=np.int64)
-  ms = np.array([y.shape[0] for y in ys], dtype=np.int64)
-  B = len(problems)
+  eps = np.array([float(p[2]) for p in problems], dtype=np.float32)
+  has_rel = all(len(p) > 3 and p[3] is not None for p in problems)
+  rel = np.array([float(p[3]) for p in problems] if has_rel else [], dtype=np.float32)
+  x = xs[0].contiguous() if len(xs) == 1 else torch.cat(xs).contiguous()
+  y = ys[0].contiguous() if len(ys) == 1 else torch.cat(ys).contiguous()
+  key = (str(device), tuple(int(v.shape[0]) for v in xs), tuple(int(v.shape[0]) for v in ys), max_pool,
+         eps.tobytes(), rel.tobytes())
+  hit = _index_cache.get(key)
+  if hit is None:
+    hit = _index_arrays([v.shape[0] for v in xs], [v.shape[0] for v in ys], eps, rel, device, max_pool)
+    if len(_index_cache) >= 16:
+      _index_cache.pop(next(iter(_index_cache)))
+    _index_cache[key] = hit
+  views, enc_rows, n_total, m_total = hit
+  return Batch(
+      count=len(problems), d=d, n_total=n_total, m_total=m_total, x=x, y=y,
+      eps=views[6].view(torch.float32), x_off=views[0], y_off=views[1], y_prob=views[2],
+      enc_src=views[3], seg_of_row=views[4], seg_off=views[5],
+      eps_rel=views[7].view(torch.float32) if has_rel else None, enc_rows=enc_rows)
+
+
+# device index arrays of recent batch shapes (the warm start calls make_batch
+# once per solve, always with the same sizes)
+_index_cache = {}
+
+
+def _index_arrays(n_sizes, m_sizes, eps, rel, device, max_pool):
+  """Offsets, row maps and epsilons of a batch, in one device buffer."""
+  ns = np.array(n_sizes, dtype=np.int64)
+  ms = np.array(m_sizes, dtype=np.int64)
+  B = len(ns)
   x_off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int32)
   y_off = np.concatenate([[0], np.cumsum(ms)]).astype(np.int32)
   m_total, n_total = int(ms.sum()), int(ns.sum())
@@ -130,9 +164,6 @@ def make_batch(problems: Sequence[Tuple], device: Optional[torch.device] = None,
   # every source row lies inside its own cloud
   side_end = np.stack([x_off[1:], y_off[1:]], axis=1).reshape(-1)
   assert np.all(enc_src >= side_base[seg_of_row]) and np.all(enc_src < side_end[seg_of_row])
-  eps = np.array([float(p[2]) for p in problems], dtype=np.float32)
-  has_rel = all(len(p) > 3 and p[3] is not None for p in problems)
-  rel = np.array([float(p[3]) for p in problems] if has_rel else [], dtype=np.float32)
   # one host-to-device copy for every index array
   parts = [x_off, y_off, y_prob, enc_src, seg_of_row, seg_off, eps.view(np.int32), rel.view(np.int32)]
   flat = torch.from_numpy(np.concatenate(parts)).to(device)
@@ -140,13 +171,7 @@ def make_batch(problems: Sequence[Tuple], device: Optional[torch.device] = None,
   for p in parts:
     views.append(flat[o:o + p.size])
     o += p.size
-  x = xs[0].contiguous() if B == 1 else torch.cat(xs).contiguous()
-  y = ys[0].contiguous() if B == 1 else torch.cat(ys).contiguous()
-  return Batch(
-      count=B, d=d, n_total=n_total, m_total=m_total, x=x, y=y,
-      eps=views[6].view(torch.float32), x_off=views[0], y_off=views[1], y_prob=views[2],
-      enc_src=views[3], seg_of_row=views[4], seg_off=views[5],
-      eps_rel=views[7].view(torch.float32) if has_rel else None, enc_rows=enc_rows)
+  return views, enc_rows, n_total, m_total
 
 
 class DeepSets:
@@ -172,11 +197,12 @@ class DeepSets:
     return self.params[off:off + int(np.prod(shape))].view(*shape)
 
   def init_params(self, seed: int) -> None:
-    """He-uniform weights for the ReLU layers; a zero output layer."""
+    """He-uniform weights for the ReLU layers; a zero output layer and quadratic term."""
     gen = torch.Generator().manual_seed(seed)
     host = torch.zeros(self.layout.total, dtype=torch.float32)
-    fan_in = {"W0": self.d + 1, "W1": WIDTH, "W2": WIDTH, "Wy": self.d + 1 + 2 * WIDTH + 1,
-              "Wc": self.d + 1 + 2 * WIDTH + 1, "W4": WIDTH}
+    used = 2 * WIDTH + 3 + self.d  # context entries in use
+    fan_in = {"W0": self.d + 1, "W1": WIDTH, "W2": WIDTH, "Wy": self.d + 1 + used,
+              "Wc": self.d + 1 + used, "W4": WIDTH}
     for name, shape in self.layout.shapes:
       off = dict(self.layout.offsets)[name]
       n = int(np.prod(shape))
@@ -186,7 +212,7 @@ class DeepSets:
         if name in ("W0", "Wy"):
           w[:, self.d + 1:] = 0.0
         if name == "Wc":
-          w[:, 2 * WIDTH + 1:] = 0.0
+          w[:, used:] = 0.0
         host[off:off + n] = w.reshape(-1)
     self.params.copy_(host.to(self.device))
     self.refresh()
