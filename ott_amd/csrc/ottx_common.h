@@ -51,6 +51,7 @@ struct SolveMail {
   double total_mass;
   int32_t n_blocks;
   uint32_t ready;
+  uint64_t progress;   // (seq << 32) | blocks completed, written by every block's finalizer
 };
 
 // Scalars resolved on the device at solve start (relative epsilon).

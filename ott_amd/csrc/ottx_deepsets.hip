@@ -69,7 +69,10 @@ struct GemmArgs {
 };
 
 __global__ __launch_bounds__(256) void gemm_nt_kernel(const GemmArgs g) {
-  constexpr int LDS_STRIDE = 40;  // bf16 per LDS row (32 + 8 pad)
+  // K is staged in chunks of up to KC: every load of a chunk is issued before
+  // the first is used (one exposed memory latency per chunk, not per 32).
+  constexpr int KC = 256;
+  constexpr int LDS_STRIDE = KC + 8;  // bf16 per LDS row (padding staggers the banks)
   __shared__ bf16_t As[64 * LDS_STRIDE];
   __shared__ bf16_t Bs[64 * LDS_STRIDE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -81,26 +84,42 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const GemmArgs g) {
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int lr = tid >> 2, lc = (tid & 3) * 8;  // loader: row, 8-element column group
-  for (int k0 = 0; k0 < g.K; k0 += 32) {
-    const u32x4 av = *reinterpret_cast<const u32x4*>(g.A + (m0 + lr) * g.lda + kbase + k0 + lc);
-    const u32x4 bv = *reinterpret_cast<const u32x4*>(g.Bt + (n0 + lr) * g.ldb + kbase + k0 + lc);
-    __syncthreads();
-    *reinterpret_cast<u32x4*>(As + lr * LDS_STRIDE + lc) = av;
-    *reinterpret_cast<u32x4*>(Bs + lr * LDS_STRIDE + lc) = bv;
-    __syncthreads();
-    bf16x8 af[2], bf[2];
+  for (int kc = 0; kc < g.K; kc += KC) {
+    const int kn = min(KC, g.K - kc);  // multiple of 32
+    const int per_row = kn / 8;         // 8-element groups per row
+    const int groups = kn / 32;         // groups per thread (64 rows * per_row / 256)
+    u32x4 ra[KC / 32], rb[KC / 32];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
-                                             As + (wr + i * 16 + (lane & 15)) * LDS_STRIDE + 8 * (lane >> 4)));
-      bf[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
-                                             Bs + (wc + i * 16 + (lane & 15)) * LDS_STRIDE + 8 * (lane >> 4)));
+    for (int i = 0; i < KC / 32; ++i)
+      if (i < groups) {
+        const int gi = tid + 256 * i, r = gi / per_row, c8 = gi % per_row;
+        ra[i] = *reinterpret_cast<const u32x4*>(g.A + (m0 + r) * g.lda + kbase + kc + c8 * 8);
+        rb[i] = *reinterpret_cast<const u32x4*>(g.Bt + (n0 + r) * g.ldb + kbase + kc + c8 * 8);
+      }
+    __syncthreads();  // the previous chunk's reads of As / Bs are done
+#pragma unroll
+    for (int i = 0; i < KC / 32; ++i)
+      if (i < groups) {
+        const int gi = tid + 256 * i, r = gi / per_row, c8 = gi % per_row;
+        *reinterpret_cast<u32x4*>(As + r * LDS_STRIDE + c8 * 8) = ra[i];
+        *reinterpret_cast<u32x4*>(Bs + r * LDS_STRIDE + c8 * 8) = rb[i];
+      }
+    __syncthreads();
+    for (int k0 = 0; k0 < kn; k0 += 32) {
+      bf16x8 af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                                               As + (wr + i * 16 + (lane & 15)) * LDS_STRIDE + k0 + 8 * (lane >> 4)));
+        bf[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                                               Bs + (wc + i * 16 + (lane & 15)) * LDS_STRIDE + k0 + 8 * (lane >> 4)));
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bf[j], acc[i][j], 0, 0, 0);
   }
   // Epilogue: element (row 4 (lane / 16) + r, column lane % 16) of each block.
 #pragma unroll

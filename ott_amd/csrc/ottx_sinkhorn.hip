@@ -469,6 +469,11 @@ __global__ void solve_finalize_kernel(const float* __restrict__ f, const float* 
                                       double* __restrict__ part, SolveState* st, SolveMail* mail, uint32_t seq) {
   __shared__ double red[2][256];
   __shared__ int s_last;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    // progress of the loop for the host, which keeps one block queued ahead
+    const uint64_t v = (static_cast<uint64_t>(seq) << 32) | static_cast<uint32_t>(st->block);
+    __hip_atomic_store(&mail->progress, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
   if (!st->done || st->finalized) return;
   const int tid = threadIdx.x;
   const int side = blockIdx.y;
@@ -1190,6 +1195,7 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   SolveMail* mail = reinterpret_cast<SolveMail*>(static_cast<char*>(hh->mailbox) + 1024);
   const uint32_t seq = ++hh->seq;
   mail->ready = 0u;
+  __atomic_store_n(&mail->progress, static_cast<uint64_t>(0), __ATOMIC_RELEASE);
   double* dual_part = reinterpret_cast<double*>(base + L.dual_part);
   auto launch_finalize = [&](hipStream_t s) -> int {
     hipLaunchKernelGGL(solve_finalize_kernel, dim3(kDualBlocks, 3), dim3(256), 0, s, out->f, P.a, pb->n, g_save,
@@ -1199,6 +1205,11 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     return OTTX_OK;
   };
   auto mail_ready = [&]() { return __atomic_load_n(&mail->ready, __ATOMIC_ACQUIRE) == seq; };
+  // blocks completed by this solve, as the finalizers report them
+  auto blocks_done = [&]() -> int64_t {
+    const uint64_t v = __atomic_load_n(&mail->progress, __ATOMIC_ACQUIRE);
+    return (v >> 32) == seq ? static_cast<int64_t>(v & 0xffffffffu) : -1;
+  };
   if (total_blocks > 0) {
     rc = launch_halfstep(k, g_plain, st);  // g-update of iteration 0
     if (rc != OTTX_OK) return rc;
@@ -1229,9 +1240,6 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
         return OTTX_EHIP;
       }
     }
-    hipEvent_t ev[2];
-    OTTX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    OTTX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     int status = OTTX_OK;
     for (int blk = 0; blk < total_blocks; ++blk) {
       if (exec) {
@@ -1240,23 +1248,26 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
         status = launch_all(st);
         if (status != OTTX_OK) break;
       }
-      if (hipEventRecord(ev[blk & 1], st) != hipSuccess) { status = OTTX_EHIP; break; }
-      // One block runs ahead: wait for the previous block, spinning on its
-      // event (a blocking wait can add tens of microseconds of wake-up
-      // latency) and on the mailbox, which the block that stops the loop fills.
+      // One block runs ahead: wait until the previous block has finished,
+      // spinning on the pinned mailbox that every block's finalizer updates
+      // (no event records in the stream, whose system-scope release would put
+      // a gap between blocks, and no blocking wait with its wake-up latency).
+      // The stream itself is queried now and then, so an error or a drained
+      // stream ends the wait.
       if (blk >= 1) {
         bool stop = false;
-        for (;;) {
+        for (uint32_t spin = 1;; ++spin) {
           if (mail_ready()) { stop = true; break; }
-          const hipError_t q = hipEventQuery(ev[(blk - 1) & 1]);
-          if (q == hipSuccess) break;
-          if (q != hipErrorNotReady) { status = OTTX_EHIP; break; }
+          if (blocks_done() >= blk) break;
+          if ((spin & 1023u) == 0) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q == hipSuccess) break;
+            if (q != hipErrorNotReady) { status = OTTX_EHIP; break; }
+          }
         }
         if (status != OTTX_OK || stop || mail_ready()) break;
       }
     }
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (cap) (void)hipStreamDestroy(cap);
