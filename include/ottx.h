@@ -53,6 +53,7 @@ enum ottx_init_mode {
   OTTX_INIT_GIVEN = 0,
   OTTX_INIT_F_FROM_G = 1,
   OTTX_INIT_ZEROS = 2,
+  OTTX_INIT_F_GIVEN = 3,
 };
 
 /* A point cloud geometry with squared-Euclidean cost,
@@ -92,7 +93,10 @@ typedef struct ottx_sinkhorn_opts {
                                OTTX_INIT_F_FROM_G: f starts as the f-update of the given g (a warm
                                start that predicts g only);
                                OTTX_INIT_ZEROS: DefaultInitializer (initializers.py:70-99), f and g
-                               start at 0, -inf where the weight is 0 (their contents are ignored) */
+                               start at 0, -inf where the weight is 0 (their contents are ignored);
+                               OTTX_INIT_F_GIVEN: f as given, g as OTTX_INIT_ZEROS sets it (a warm
+                               start that predicts f: the first g-update discards g but for its
+                               -inf entries, geometry.py:297-299) */
   int32_t _pad;
 } ottx_sinkhorn_opts;
 

@@ -27,7 +27,7 @@ OTTX_EWORKSPACE = -4
 PREC_FP32 = 0
 PREC_BF16_SPLIT = 1
 PRECISIONS = {"fp32": PREC_FP32, "bf16_split": PREC_BF16_SPLIT}
-INIT_GIVEN, INIT_F_FROM_G, INIT_ZEROS = 0, 1, 2
+INIT_GIVEN, INIT_F_FROM_G, INIT_ZEROS, INIT_F_GIVEN = 0, 1, 2, 3
 
 OP_APPLY_LSE = 0
 OP_UPDATE = 1

@@ -68,13 +68,15 @@ struct GemmArgs {
   int valid_rows;        // rows >= valid_rows are written as zero
 };
 
+// K is staged through LDS in chunks of KC (NBUF buffers): the loads of a
+// whole chunk are issued before any is used, and with two buffers the next
+// chunk's loads are in flight while the current one is on MFMA.
+template <int KC, int NBUF>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(const GemmArgs g) {
-  // K is staged in chunks of up to KC: every load of a chunk is issued before
-  // the first is used (one exposed memory latency per chunk, not per 32).
-  constexpr int KC = 256;
   constexpr int LDS_STRIDE = KC + 8;  // bf16 per LDS row (padding staggers the banks)
-  __shared__ bf16_t As[64 * LDS_STRIDE];
-  __shared__ bf16_t Bs[64 * LDS_STRIDE];
+  constexpr int PER = KC / 32;        // 16-byte groups per thread per operand and chunk
+  __shared__ bf16_t As[NBUF][64 * LDS_STRIDE];
+  __shared__ bf16_t Bs[NBUF][64 * LDS_STRIDE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const int64_t kbase = static_cast<int64_t>(blockIdx.z) * g.K;
@@ -84,35 +86,41 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const GemmArgs g) {
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int kc = 0; kc < g.K; kc += KC) {
-    const int kn = min(KC, g.K - kc);  // multiple of 32
-    const int per_row = kn / 8;         // 8-element groups per row
-    const int groups = kn / 32;         // groups per thread (64 rows * per_row / 256)
-    u32x4 ra[KC / 32], rb[KC / 32];
+  u32x4 ra[PER], rb[PER];
+  // chunk kc holds kn = min(KC, K - kc) columns (a multiple of 32): kn / 8
+  // groups per row, kn / 32 per thread
+  auto load = [&](int kc) {
+    const int kn = min(KC, g.K - kc), per_row = kn / 8;
 #pragma unroll
-    for (int i = 0; i < KC / 32; ++i)
-      if (i < groups) {
+    for (int i = 0; i < PER; ++i)
+      if (i < kn / 32) {
         const int gi = tid + 256 * i, r = gi / per_row, c8 = gi % per_row;
         ra[i] = *reinterpret_cast<const u32x4*>(g.A + (m0 + r) * g.lda + kbase + kc + c8 * 8);
         rb[i] = *reinterpret_cast<const u32x4*>(g.Bt + (n0 + r) * g.ldb + kbase + kc + c8 * 8);
       }
-    __syncthreads();  // the previous chunk's reads of As / Bs are done
+  };
+  load(0);
+  for (int kc = 0, ci = 0; kc < g.K; kc += KC, ++ci) {
+    const int buf = NBUF == 1 ? 0 : (ci & 1);
+    const int kn = min(KC, g.K - kc), per_row = kn / 8;
+    if (NBUF == 1) __syncthreads();  // the previous chunk's reads of the buffer are done
 #pragma unroll
-    for (int i = 0; i < KC / 32; ++i)
-      if (i < groups) {
+    for (int i = 0; i < PER; ++i)
+      if (i < kn / 32) {
         const int gi = tid + 256 * i, r = gi / per_row, c8 = gi % per_row;
-        *reinterpret_cast<u32x4*>(As + r * LDS_STRIDE + c8 * 8) = ra[i];
-        *reinterpret_cast<u32x4*>(Bs + r * LDS_STRIDE + c8 * 8) = rb[i];
+        *reinterpret_cast<u32x4*>(&As[buf][r * LDS_STRIDE + c8 * 8]) = ra[i];
+        *reinterpret_cast<u32x4*>(&Bs[buf][r * LDS_STRIDE + c8 * 8]) = rb[i];
       }
     __syncthreads();
+    if (kc + KC < g.K) load(kc + KC);
     for (int k0 = 0; k0 < kn; k0 += 32) {
       bf16x8 af[2], bf[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         af[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
-                                               As + (wr + i * 16 + (lane & 15)) * LDS_STRIDE + k0 + 8 * (lane >> 4)));
+            &As[buf][(wr + i * 16 + (lane & 15)) * LDS_STRIDE + k0 + 8 * (lane >> 4)]));
         bf[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
-                                               Bs + (wc + i * 16 + (lane & 15)) * LDS_STRIDE + k0 + 8 * (lane >> 4)));
+            &Bs[buf][(wc + i * 16 + (lane & 15)) * LDS_STRIDE + k0 + 8 * (lane >> 4)]));
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -161,7 +169,7 @@ __global__ void sum_splits_kernel(const float* __restrict__ part, int splits, in
 // sigma^2 = mean(C) / 2 with mean(C) = mean|x|^2 + mean|y|^2 - 2 <mean x, mean y>.
 // One workgroup per problem (thread per feature, rows strided).
 // ---------------------------------------------------------------------------
-constexpr int kStatChunks = 256;
+constexpr int kStatChunks = 128;
 
 // Per-(problem, chunk) float64 partial sums of the coordinates and their
 // squares, for x and y: part[((b * kStatChunks + c) * 4 + q) * d + k] with
@@ -181,11 +189,26 @@ __global__ void stats_partial_kernel(const float* __restrict__ x, const float* _
       const int r0 = off[b], r1 = off[b + 1];
       const int per = (r1 - r0 + kStatChunks - 1) / kStatChunks;
       const int lo = r0 + c * per, hi = min(r1, lo + per);
-      for (int r = lo + gi; r < hi; r += G) {
-        const double v = p[static_cast<int64_t>(r) * d + k];
-        acc[2 * side] += v;
-        acc[2 * side + 1] += v * v;
+      // four rows per trip: independent loads in flight
+      double s1 = 0.0, s2 = 0.0, t1 = 0.0, t2 = 0.0;
+      int r = lo + gi;
+      for (; r + 3 * G < hi; r += 4 * G) {
+        const double v0 = p[static_cast<int64_t>(r) * d + k];
+        const double v1 = p[static_cast<int64_t>(r + G) * d + k];
+        const double v2 = p[static_cast<int64_t>(r + 2 * G) * d + k];
+        const double v3 = p[static_cast<int64_t>(r + 3 * G) * d + k];
+        s1 += v0 + v1;
+        s2 += v2 + v3;
+        t1 += v0 * v0 + v1 * v1;
+        t2 += v2 * v2 + v3 * v3;
       }
+      for (; r < hi; r += G) {
+        const double v = p[static_cast<int64_t>(r) * d + k];
+        s1 += v;
+        t1 += v * v;
+      }
+      acc[2 * side] = s1 + s2;
+      acc[2 * side + 1] = t1 + t2;
     }
   }
   for (int q = 0; q < 4; ++q) fs[q][t] = acc[q];
@@ -220,10 +243,25 @@ __global__ void stats_final_kernel(const double* __restrict__ part, const int* _
   const int G = max(1, 256 / d);
   const int k = t % d, g = t / d;
   double q[4] = {0.0, 0.0, 0.0, 0.0};
-  if (t < G * d)
-    for (int c = g; c < kStatChunks; c += G)
+  if (t < G * d) {
+    // four chunks per trip: sixteen independent loads in flight per thread
+    double q2[4] = {0.0, 0.0, 0.0, 0.0}, q3[4] = {0.0, 0.0, 0.0, 0.0}, q4[4] = {0.0, 0.0, 0.0, 0.0};
+    const double* pb = part + static_cast<int64_t>(b) * kStatChunks * 4 * d + k;
+    int c = g;
+    for (; c + 3 * G < kStatChunks; c += 4 * G)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) q[j] += part[((static_cast<int64_t>(b) * kStatChunks + c) * 4 + j) * d + k];
+      for (int j = 0; j < 4; ++j) {
+        q[j] += pb[(static_cast<int64_t>(c) * 4 + j) * d];
+        q2[j] += pb[(static_cast<int64_t>(c + G) * 4 + j) * d];
+        q3[j] += pb[(static_cast<int64_t>(c + 2 * G) * 4 + j) * d];
+        q4[j] += pb[(static_cast<int64_t>(c + 3 * G) * 4 + j) * d];
+      }
+    for (; c < kStatChunks; c += G)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) q[j] += pb[(static_cast<int64_t>(c) * 4 + j) * d];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) q[j] = (q[j] + q2[j]) + (q3[j] + q4[j]);
+  }
 #pragma unroll
   for (int j = 0; j < 4; ++j) red[j][t] = q[j];
   __syncthreads();
@@ -314,9 +352,17 @@ __global__ void pool_partial_kernel(const float* __restrict__ H, const int* __re
   const int r0 = seg_off[seg], r1 = seg_off[seg + 1];
   const int per = (r1 - r0 + chunks - 1) / chunks;
   const int a = r0 + c * per, e = min(r1, a + per);
-  float s = 0.f;
-  for (int r = a; r < e; ++r) s += H[static_cast<int64_t>(r) * 256 + f];
-  part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f] = s;
+  // four independent sums keep four loads in flight per thread
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int r = a;
+  for (; r + 3 < e; r += 4) {
+    s0 += H[static_cast<int64_t>(r) * 256 + f];
+    s1 += H[static_cast<int64_t>(r + 1) * 256 + f];
+    s2 += H[static_cast<int64_t>(r + 2) * 256 + f];
+    s3 += H[static_cast<int64_t>(r + 3) * 256 + f];
+  }
+  for (; r < e; ++r) s0 += H[static_cast<int64_t>(r) * 256 + f];
+  part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f] = (s0 + s1) + (s2 + s3);
 }
 
 // ctx[b] = [e_x (256), e_y (256), kappa = eps / sigma^2, mom_b (d + 2), 0 pad ...]
@@ -330,9 +376,16 @@ __global__ void pool_final_kernel(const float* __restrict__ part, const int* __r
   float* cb = ctx + static_cast<int64_t>(b) * W;
   for (int side = 0; side < 2; ++side) {
     const int seg = b * 2 + side;
-    float s = 0.f;
-    for (int c = 0; c < chunks; ++c) s += part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f];
-    cb[side * 256 + f] = s / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int c = 0;
+    for (; c + 3 < chunks; c += 4) {
+      s0 += part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f];
+      s1 += part[(static_cast<int64_t>(seg) * chunks + c + 1) * 256 + f];
+      s2 += part[(static_cast<int64_t>(seg) * chunks + c + 2) * 256 + f];
+      s3 += part[(static_cast<int64_t>(seg) * chunks + c + 3) * 256 + f];
+    }
+    for (; c < chunks; ++c) s0 += part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f];
+    cb[side * 256 + f] = ((s0 + s1) + (s2 + s3)) / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
   }
   // kappa = eps / sigma^2; with eps = c mean(C) and sigma^2 = mean(C) / 2 it is 2 c.
   const float kappa = (eps_rel && eps_rel[b] > 0.f) ? 2.f * eps_rel[b] : eps[b] / sig2[b];
@@ -677,7 +730,7 @@ DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training)
   L.Pe_pad = pad64(L.Pe);
   L.Ph = bt->m_total;
   L.Ph_pad = pad64(L.Ph);
-  L.chunks = 16;
+  L.chunks = 32;
   L.kchunks = training ? 64 : 0;
   size_t o = 0;
   auto take = [&](size_t b) {
@@ -729,7 +782,12 @@ DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training)
 
 int gemm(hipStream_t st, const GemmArgs& a) {
   if (a.M % 64 || a.N % 64 || a.K % 32 || a.splits < 1) return OTTX_EINVAL;
-  hipLaunchKernelGGL(gemm_nt_kernel, dim3(a.N / 64, a.M / 64, a.splits), dim3(256), 0, st, a);
+  // one 256-wide chunk for short K and few row tiles; otherwise 64-wide
+  // double-buffered chunks (more workgroups resident, loads overlapped)
+  if (a.K <= 256 && a.M < 8192)
+    hipLaunchKernelGGL((gemm_nt_kernel<256, 1>), dim3(a.N / 64, a.M / 64, a.splits), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((gemm_nt_kernel<64, 2>), dim3(a.N / 64, a.M / 64, a.splits), dim3(256), 0, st, a);
   DS_LAUNCHED();
   return OTTX_OK;
 }

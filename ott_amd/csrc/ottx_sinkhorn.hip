@@ -211,10 +211,15 @@ struct PrepInit {
   int outer = 0;
   uint32_t* counters = nullptr;
   int n_counters = 0;
-  int zero_init = 0;
+  int zero_init = 0;  // bit 0: f, bit 1: g start as DefaultInitializer's (0, -inf at zero weight)
   float* f = nullptr;
   float* g = nullptr;
 };
+
+// Potentials the solve initialises itself for an init mode (PrepInit::zero_init bits).
+inline int zero_init_bits(int init_mode) {
+  return init_mode == OTTX_INIT_ZEROS ? 3 : init_mode == OTTX_INIT_F_GIVEN ? 2 : 0;
+}
 
 // Statistics partials of both clouds in one launch: blocks [0, cx) take x,
 // blocks [cx, cx + cy) take y (same per-chunk sums as stats_partial_kernel).
@@ -323,18 +328,19 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
     }
     for (int64_t k = local; k < in.outer; k += stride) in.errors[k] = -1.f;
     for (int64_t k = local; k < in.n_counters; k += stride) in.counters[k] = 0u;
-    if (in.zero_init) {  // DefaultInitializer: 0, -inf where the weight is 0
+    // DefaultInitializer: 0, -inf where the weight is 0
+    if (in.zero_init & 1)
       for (int64_t k = local; k < p.np[0]; k += stride)
         in.f[k] = (p.wuser[0] == nullptr || p.wuser[0][k] > 0.f) ? 0.f : -INFINITY;
+    if (in.zero_init & 2)
       for (int64_t k = local; k < p.np[1]; k += stride)
         in.g[k] = (p.wuser[1] == nullptr || p.wuser[1][k] > 0.f) ? 0.f : -INFINITY;
-    }
   }
 }
 
 // Loop state reset: errors = -1 (sinkhorn.py:964-972), counters = 0.  With
-// ``zero_init`` the potentials start as DefaultInitializer's: 0, or -inf where
-// the weight is 0 (initializers.py:70-99).
+// ``zero_init`` bit 0 (f) / bit 1 (g) the potentials start as
+// DefaultInitializer's: 0, or -inf where the weight is 0 (initializers.py:70-99).
 __global__ void init_state_kernel(SolveState* st, float* errors, int outer, uint32_t* counters, int n_counters,
                                   int zero_init, float* f, const float* a, int64_t n, float* g, const float* b,
                                   int64_t m) {
@@ -354,10 +360,10 @@ __global__ void init_state_kernel(SolveState* st, float* errors, int outer, uint
   }
   for (int64_t k = i; k < outer; k += stride) errors[k] = -1.f;
   for (int64_t k = i; k < n_counters; k += stride) counters[k] = 0u;
-  if (zero_init) {
+  if (zero_init & 1)
     for (int64_t k = i; k < n; k += stride) f[k] = a[k] > 0.f ? 0.f : -INFINITY;
+  if (zero_init & 2)
     for (int64_t k = i; k < m; k += stride) g[k] = b[k] > 0.f ? 0.f : -INFINITY;
-  }
 }
 
 // out[0] = sum of v[0, n) (one workgroup, fixed order).
@@ -1125,7 +1131,7 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   const bool force_scan = o->min_iterations == o->max_iterations;
   const int total_blocks = force_scan ? o->max_iterations / inner : outer;
 
-  if (o->init_mode < 0 || o->init_mode > 2) return OTTX_EINVAL;
+  if (o->init_mode < 0 || o->init_mode > 3) return OTTX_EINVAL;
   char* base = static_cast<char*>(workspace);
   SolveState* state = reinterpret_cast<SolveState*>(base + L.state);
   uint32_t* counters = reinterpret_cast<uint32_t*>(base + L.counters);
@@ -1137,7 +1143,7 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   pinit.outer = outer;
   pinit.counters = counters;
   pinit.n_counters = std::max(gp.tiles, fp.tiles);
-  pinit.zero_init = o->init_mode == OTTX_INIT_ZEROS ? 1 : 0;
+  pinit.zero_init = zero_init_bits(o->init_mode);
   pinit.f = out->f;
   pinit.g = out->g;
   Prepared P;
@@ -1555,10 +1561,11 @@ __global__ void batch_init_kernel(const BatchItem* __restrict__ items, int outer
   }
   for (int k = t; k < outer; k += blockDim.x) it.errors[k] = -1.f;
   for (int k = t; k < it.n_counters; k += blockDim.x) it.counters[k] = 0u;
-  if (zero_init) {  // DefaultInitializer: 0, or -inf where the weight is 0
+  // DefaultInitializer: 0, or -inf where the weight is 0 (bit 0: f, bit 1: g)
+  if (zero_init & 1)
     for (int64_t k = t; k < it.n; k += blockDim.x) it.f[k] = it.a[k] > 0.f ? 0.f : -INFINITY;
+  if (zero_init & 2)
     for (int64_t k = t; k < it.m; k += blockDim.x) it.g[k] = it.b[k] > 0.f ? 0.f : -INFINITY;
-  }
 }
 
 // Stage 1 of the float64 dual terms of every problem: grid (kDualBlocks, 3, K),
@@ -1848,7 +1855,7 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
       !workspace)
     return OTTX_EINVAL;
   if (o->init_mode == OTTX_INIT_F_FROM_G) return OTTX_EUNSUPPORTED;
-  if (o->init_mode < 0 || o->init_mode > 2) return OTTX_EINVAL;
+  if (o->init_mode < 0 || o->init_mode > 3) return OTTX_EINVAL;
   const int inner = o->inner_iterations;
   const int outer = (o->max_iterations + inner - 1) / inner;
   if (outer < 1) return OTTX_EUNSUPPORTED;
@@ -1960,7 +1967,7 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
   hipLaunchKernelGGL(batch_norms_kernel, dim3(static_cast<unsigned>((max_pad + 255) / 256), K, 2), dim3(256), 0,
                      st, d_items, bt->d, s);
   hipLaunchKernelGGL(batch_init_kernel, dim3(K), dim3(256), 0, st, d_items, outer, batch_done,
-                     o->init_mode == OTTX_INIT_ZEROS ? 1 : 0);
+                     zero_init_bits(o->init_mode));
   OTTX_LAUNCHED();
 
   const uint32_t g_plain = kModeWritePot | kModeCheckDone;

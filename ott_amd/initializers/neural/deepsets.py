@@ -251,11 +251,19 @@ class DeepSets:
       self._ws = ws = _lib.workspace(nbytes, self.device)
     return ws, bs
 
-  def forward(self, batch: Batch) -> torch.Tensor:
-    """Predicted potentials g (per head row, batch order), up to a constant per problem."""
+  def forward(self, batch: Batch, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Predicted potentials g (per head row, batch order), up to a constant per problem.
+
+    ``out`` (float32, contiguous, ``m_total`` values on the model's device)
+    receives the prediction when given.
+    """
     lib = _lib.load_library()
     ws, bs = self._workspace(batch, training=False)
-    out = torch.empty(batch.m_total, dtype=torch.float32, device=self.device)
+    if out is None:
+      out = torch.empty(batch.m_total, dtype=torch.float32, device=self.device)
+    elif (out.dtype != torch.float32 or not out.is_contiguous() or out.numel() != batch.m_total
+          or out.device != self.device):
+      raise ValueError("out must be a contiguous float32 tensor of m_total values on the model's device")
     _lib.check("ottx_ds_forward", lib.ottx_ds_forward(
         ctypes.byref(self.struct()), ctypes.byref(bs), out.data_ptr(), ws.data_ptr(), ws.numel(),
         _lib.stream_ptr()))
@@ -309,7 +317,11 @@ class DeepSetsInitializer(SinkhornInitializer):
   def predicts_g(self) -> bool:
     return self.predict == "g"
 
-  def _potential(self, ot_prob, side: str) -> torch.Tensor:
+  @property
+  def predicts_f(self) -> bool:
+    return self.predict == "f"
+
+  def _potential(self, ot_prob, side: str, out: Optional[torch.Tensor] = None) -> torch.Tensor:
     geom = ot_prob.geom
     inv_s = geom.inv_scale_cost  # the network works in units of |x - y|^2
     eps_abs, eps_rel = geom._epsilon_spec()
@@ -320,21 +332,26 @@ class DeepSetsInitializer(SinkhornInitializer):
       eps, rel = 0.0, eps_rel
     clouds = (geom.y, geom.x) if side == "f" else (geom.x, geom.y)
     batch = make_batch([(clouds[0], clouds[1], eps, rel)], device=geom.device, max_pool=self.max_pool)
+    w = ot_prob._a if side == "f" else ot_prob._b
+    if inv_s == 1.0 and w is None:
+      return self.model.forward(batch, out=out)
     pot = self.model.forward(batch)
     if inv_s != 1.0:
       pot = pot * inv_s
-    w = ot_prob._a if side == "f" else ot_prob._b
     if w is not None:
       pot = torch.where(w > 0.0, pot, torch.full_like(pot, -math.inf))
+    if out is not None:
+      out.copy_(pot)
+      return out
     return pot
 
-  def predict_f(self, ot_prob) -> torch.Tensor:
-    """f^ on the source points, -inf where a is zero."""
-    return self._potential(ot_prob, "f")
+  def predict_f(self, ot_prob, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """f^ on the source points, -inf where a is zero (written into ``out`` if given)."""
+    return self._potential(ot_prob, "f", out)
 
-  def predict_g(self, ot_prob) -> torch.Tensor:
-    """g^ on the target points, -inf where b is zero."""
-    return self._potential(ot_prob, "g")
+  def predict_g(self, ot_prob, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """g^ on the target points, -inf where b is zero (written into ``out`` if given)."""
+    return self._potential(ot_prob, "g", out)
 
   def __call__(self, ot_prob, lse_mode: bool = True, **kwargs):
     if not lse_mode:

@@ -190,9 +190,21 @@ class Sinkhorn:
     if init is None and getattr(self.initializer, "predicts_g", False):
       # A warm start that predicts g only: the solver derives f0 from it with
       # its own first f-update instead of a separate pass over the geometry.
-      g = as_device_f32(self.initializer.predict_g(ot_prob), geom.device).clone()
+      g = as_device_f32(self.initializer.predict_g(ot_prob), geom.device)
       f = torch.empty((n,), dtype=torch.float32, device=geom.device)
       return run(ot_prob, self, (f, g), init_mode=_lib.INIT_F_FROM_G)
+    if init is None and getattr(self.initializer, "predicts_f", False):
+      # A warm start that predicts f only (OTT's learned initializer predicts
+      # f too, meta_initializer.py:192-207): its prediction is a fresh tensor
+      # that becomes the solve's f; g starts as the DefaultInitializer's, which
+      # the first g-update discards but for its -inf entries (geometry.py:297-299).
+      # The solve is set up first so that its launches follow the network's
+      # directly; the network writes its prediction into the solve's f.
+      f = torch.empty((n,), dtype=torch.float32, device=geom.device)
+      g = torch.empty((m,), dtype=torch.float32, device=geom.device)
+      launch = _prepare(ot_prob, self, (f, g), init_mode=_lib.INIT_F_GIVEN)
+      self.initializer.predict_f(ot_prob, out=f)
+      return launch()
     if init is None and type(self.initializer) is init_lib.DefaultInitializer:
       # zeros, -inf at zero weights (initializers.py:70-99), set by the solve itself
       f = torch.empty((n,), dtype=torch.float32, device=geom.device)
@@ -209,6 +221,12 @@ class Sinkhorn:
 def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
         init: Tuple[torch.Tensor, torch.Tensor], init_mode: int = _lib.INIT_GIVEN) -> SinkhornOutput:
   """sinkhorn.py:1052-1099: the loop, then ``set_cost`` and ``ot_prob``."""
+  return _prepare(ot_prob, solver, init, init_mode)()
+
+
+def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
+             init: Tuple[torch.Tensor, torch.Tensor], init_mode: int):
+  """Everything of :func:`run` up to the solve call; returns the call."""
   lib = _lib.load_library()
   geom: PointCloud = ot_prob.geom
   f, g = init
@@ -234,12 +252,17 @@ def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
   if nbytes == 0:
     raise NotImplementedError(f"problem not supported by the HIP kernels: {geom!r}")
   ws = _lib.workspace(nbytes, geom.device)
-  _lib.check("ottx_sinkhorn_solve", lib.ottx_sinkhorn_solve(
-      ctypes.byref(pb), ctypes.byref(opts), ctypes.byref(out), ws.data_ptr(),
-      ws.numel(), _lib.stream_ptr()))
-  del keep, ws
-  geom._set_resolved_epsilon(out.epsilon)
-  return SinkhornOutput(
-      potentials=(f, g), errors=errors, reg_ot_cost=float(out.reg_ot_cost),
-      ot_prob=ot_prob, threshold=float(solver.threshold),
-      converged=bool(out.converged), inner_iterations=int(solver.inner_iterations))
+  stream = _lib.stream_ptr()
+
+  def launch() -> SinkhornOutput:
+    _lib.check("ottx_sinkhorn_solve", lib.ottx_sinkhorn_solve(
+        ctypes.byref(pb), ctypes.byref(opts), ctypes.byref(out), ws.data_ptr(),
+        ws.numel(), stream))
+    geom._set_resolved_epsilon(out.epsilon)
+    return SinkhornOutput(
+        potentials=(f, g), errors=errors, reg_ot_cost=float(out.reg_ot_cost),
+        ot_prob=ot_prob, threshold=float(solver.threshold),
+        converged=bool(out.converged), inner_iterations=int(solver.inner_iterations))
+
+  launch.keep = keep  # weights referenced by pb stay alive until the call
+  return launch

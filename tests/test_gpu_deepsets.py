@@ -189,10 +189,13 @@ def test_training_reduces_loss():
   assert last < 0.2 * first, (first, last)
 
 
+@pytest.mark.parametrize("predict", ["g", "f"])
 @pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
-def test_initializer_fast_path_matches_explicit_init(precision):
-  """Sinkhorn(initializer=DeepSetsInitializer) derives f0 inside the solve
-  (OTTX_INIT_F_FROM_G); it must agree with passing init=(f0, g^) computed outside."""
+def test_initializer_fast_path_matches_explicit_init(precision, predict):
+  """Sinkhorn(initializer=DeepSetsInitializer) hands the prediction to the solve
+  (predict="g": f0 derived inside the solve, OTTX_INIT_F_FROM_G; predict="f":
+  g set inside the solve, OTTX_INIT_F_GIVEN); it must agree with passing
+  init=initializer(prob) computed outside."""
   from ott_amd.geometry import pointcloud
   from ott_amd.initializers.neural import deepsets
   from ott_amd.problems.linear import linear_problem
@@ -203,7 +206,7 @@ def test_initializer_fast_path_matches_explicit_init(precision):
   with torch.no_grad():
     model.tensor("w5").copy_(torch.from_numpy(rng.normal(size=256).astype(np.float32) * 0.05))
   model.refresh()
-  init = deepsets.DeepSetsInitializer(model, predict="g")
+  init = deepsets.DeepSetsInitializer(model, predict=predict)
   x = torch.from_numpy(rng.normal(size=(700, d)).astype(np.float32) / 4).cuda()
   y = torch.from_numpy((rng.normal(size=(513, d)) / 4 + 0.1).astype(np.float32)).cuda()
   outs = []
