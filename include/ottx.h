@@ -97,7 +97,11 @@ typedef struct ottx_sinkhorn_opts {
                                OTTX_INIT_F_GIVEN: f as given, g as OTTX_INIT_ZEROS sets it (a warm
                                start that predicts f: the first g-update discards g but for its
                                -inf entries, geometry.py:297-299) */
-  int32_t _pad;
+  int32_t recenter_potentials; /* 1: SinkhornState.recenter on the outputs (sinkhorn.py:76-114,
+                               1008-1009): f -= mean of its finite entries, g += the same, for a
+                               balanced problem; no change when tau_a or tau_b is 1 (the errors
+                               are unchanged in both cases).  Unbalanced problems with tau_a < 1
+                               and tau_b < 1 return OTTX_EUNSUPPORTED. */
 } ottx_sinkhorn_opts;
 
 /* Outputs of one solve (SinkhornOutput fields, sinkhorn.py:280-323). */

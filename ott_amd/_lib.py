@@ -63,7 +63,7 @@ class SinkhornOpts(ctypes.Structure):
       ("tile_config", ctypes.c_int32),
       ("use_graph", ctypes.c_int32),
       ("init_mode", ctypes.c_int32),
-      ("_pad", ctypes.c_int32),
+      ("recenter_potentials", ctypes.c_int32),
   ]
 
 

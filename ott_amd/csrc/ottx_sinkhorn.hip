@@ -468,12 +468,14 @@ __global__ void dual_final_kernel(const double* __restrict__ part, const float* 
 // the SolveMail written last.  Grid (kDualBlocks, 3); every launch before the
 // loop stops, or after its outputs were written, exits at once.  The last
 // workgroup to arrive combines the partials.
-__global__ void solve_finalize_kernel(const float* __restrict__ f, const float* __restrict__ a, int64_t n,
+__global__ void solve_finalize_kernel(const float* f, const float* __restrict__ a, int64_t n,
                                       const float* __restrict__ g_save, float* __restrict__ g_out,
                                       const float* __restrict__ b, int64_t m, const SolveScalars* __restrict__ sc,
                                       double tau_a, double tau_b, const float* __restrict__ errors, int outer,
-                                      double* __restrict__ part, SolveState* st, SolveMail* mail, uint32_t seq) {
+                                      double* __restrict__ part, SolveState* st, SolveMail* mail, uint32_t seq,
+                                      int recenter, float* __restrict__ f_out) {
   __shared__ double red[2][256];
+  __shared__ double cen[2][kDualBlocks];
   __shared__ int s_last;
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
     // progress of the loop for the host, which keeps one block queued ahead
@@ -484,6 +486,7 @@ __global__ void solve_finalize_kernel(const float* __restrict__ f, const float* 
   const int tid = threadIdx.x;
   const int side = blockIdx.y;
   double s0 = 0.0, s1 = 0.0;
+  double c0 = 0.0, c1 = 0.0;  // recentering: sum and count of the finite f
   if (side < 2) {
     const float* pot = side == 0 ? f : g_save;
     const float* w = side == 0 ? a : b;
@@ -495,6 +498,10 @@ __global__ void solve_finalize_kernel(const float* __restrict__ f, const float* 
     const int64_t i0 = blockIdx.x * per, i1 = min(len, i0 + per);
     for (int64_t i = i0 + tid; i < i1; i += blockDim.x) {
       const double wi = w[i];
+      if (side == 0 && recenter && isfinite(pot[i])) {
+        c0 += pot[i];
+        c1 += 1.0;
+      }
       s1 += wi;
       if (wi > 0.0) {
         const double h = static_cast<double>(pot[i]) - eps * log(wi);
@@ -526,6 +533,25 @@ __global__ void solve_finalize_kernel(const float* __restrict__ f, const float* 
   if (tid == 0) {
     part[(side * kDualBlocks + blockIdx.x) * 2 + 0] = red[0][0];
     part[(side * kDualBlocks + blockIdx.x) * 2 + 1] = red[1][0];
+  }
+  if (recenter && side == 0) {
+    __syncthreads();
+    red[0][tid] = c0;
+    red[1][tid] = c1;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if (tid < k) {
+        red[0][tid] += red[0][tid + k];
+        red[1][tid] += red[1][tid + k];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      part[(3 * kDualBlocks + blockIdx.x) * 2 + 0] = red[0][0];
+      part[(3 * kDualBlocks + blockIdx.x) * 2 + 1] = red[1][0];
+    }
+  }
+  if (tid == 0) {
     release_to_device();
     const uint32_t prev = atomicAdd(&st->fin_count, 1u);
     s_last = (prev == 3u * kDualBlocks - 1u) ? 1 : 0;
@@ -538,9 +564,26 @@ __global__ void solve_finalize_kernel(const float* __restrict__ f, const float* 
   const int q = tid % kDualBlocks, sd = (tid / kDualBlocks) % 3, term = tid / (3 * kDualBlocks);
   red[0][tid] = (tid < 3 * kDualBlocks * 2) ? load_f64_coherent(part + (sd * kDualBlocks + q) * 2 + term) : 0.0;
   __syncthreads();
+  if (recenter && tid < kDualBlocks) {
+    cen[0][tid] = load_f64_coherent(part + (3 * kDualBlocks + tid) * 2 + 0);
+    cen[1][tid] = load_f64_coherent(part + (3 * kDualBlocks + tid) * 2 + 1);
+  }
+  __syncthreads();
   for (int k = kDualBlocks / 2; k > 0; k >>= 1) {
     if (tid < 3 * kDualBlocks * 2 && q < k) red[0][tid] += red[0][tid + k];
+    if (recenter && tid < k) {
+      cen[0][tid] += cen[0][tid + k];
+      cen[1][tid] += cen[1][tid + k];
+    }
     __syncthreads();
+  }
+  // SinkhornState.recenter, balanced branch (sinkhorn.py:98-102): f - c and
+  // g + c with c the mean of the finite f; the dual terms follow the shift.
+  const double shift = (recenter && cen[1][0] > 0.0) ? cen[0][0] / cen[1][0] : 0.0;
+  if (recenter) {
+    const float cf = static_cast<float>(shift);
+    for (int64_t i = tid; i < n; i += blockDim.x) f_out[i] -= cf;
+    for (int64_t j = tid; j < m; j += blockDim.x) g_out[j] = g_save[j] + cf;
   }
   if (tid != 0) return;
   double du[8];
@@ -548,6 +591,9 @@ __global__ void solve_finalize_kernel(const float* __restrict__ f, const float* 
     du[sd2 * 2 + 0] = red[0][(0 * 3 + sd2) * kDualBlocks];
     du[sd2 * 2 + 1] = red[0][(1 * 3 + sd2) * kDualBlocks];
   }
+  // sum_a a (f - c - eps log a) = div_a - c sum a; likewise + c sum b for g
+  du[0] -= shift * du[1];
+  du[2] += shift * du[3];
   du[6] = outer > 0 ? static_cast<double>(errors[outer - 1]) : 0.0;
   du[7] = sc->eps;
   for (int q = 0; q < 8; ++q) mail->du[q] = du[q];
@@ -751,7 +797,7 @@ Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& g
   L.err_row = take(static_cast<size_t>(max_tiles) * sizeof(double));
   L.state = take(sizeof(SolveState));
   L.dual = take(16 * sizeof(double));
-  L.dual_part = take(3 * kDualBlocks * 2 * sizeof(double));
+  L.dual_part = take(4 * kDualBlocks * 2 * sizeof(double));  // 3 dual sides + the recentering sums
   L.scal = take(sizeof(SolveScalars));
   stats_chunks(pb->n, &L.chunks_x, &L.rows_chunk_x);
   stats_chunks(pb->m, &L.chunks_y, &L.rows_chunk_y);
@@ -1132,6 +1178,13 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   const int total_blocks = force_scan ? o->max_iterations / inner : outer;
 
   if (o->init_mode < 0 || o->init_mode > 3) return OTTX_EINVAL;
+  // recenter_potentials: balanced problems are recentred by the finalizer;
+  // with one tau = 1 the reference leaves the potentials as they are
+  // (sinkhorn.py:104-106); both taus < 1 need the recentring terms of
+  // lse_step (sinkhorn.py:809-854), which are not implemented.
+  const bool both_unbalanced = pb->tau_a < 1.0 && pb->tau_b < 1.0;
+  if (o->recenter_potentials && (both_unbalanced || total_blocks == 0)) return OTTX_EUNSUPPORTED;
+  const int recenter = (o->recenter_potentials && pb->tau_a == 1.0 && pb->tau_b == 1.0) ? 1 : 0;
   char* base = static_cast<char*>(workspace);
   SolveState* state = reinterpret_cast<SolveState*>(base + L.state);
   uint32_t* counters = reinterpret_cast<uint32_t*>(base + L.counters);
@@ -1206,7 +1259,7 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   auto launch_finalize = [&](hipStream_t s) -> int {
     hipLaunchKernelGGL(solve_finalize_kernel, dim3(kDualBlocks, 3), dim3(256), 0, s, out->f, P.a, pb->n, g_save,
                        out->g, P.b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors, outer, dual_part, state,
-                       mail, seq);
+                       mail, seq, recenter, out->f);
     OTTX_LAUNCHED();
     return OTTX_OK;
   };
@@ -1854,7 +1907,7 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
   if (!out || !out->f || !out->g || !out->errors || !out->reg_ot_cost || !out->converged || !out->n_iters ||
       !workspace)
     return OTTX_EINVAL;
-  if (o->init_mode == OTTX_INIT_F_FROM_G) return OTTX_EUNSUPPORTED;
+  if (o->init_mode == OTTX_INIT_F_FROM_G || o->recenter_potentials) return OTTX_EUNSUPPORTED;
   if (o->init_mode < 0 || o->init_mode > 3) return OTTX_EINVAL;
   const int inner = o->inner_iterations;
   const int outer = (o->max_iterations + inner - 1) / inner;

@@ -163,6 +163,12 @@ class Sinkhorn:
       raise NotImplementedError("ott_amd accelerates PointCloud geometries only")
     if not ot_prob.geom._schedule_is_target_only():
       raise NotImplementedError("epsilon schedules with init > 1 are not supported yet")
+    if self.recenter_potentials and ot_prob.tau_a < 1.0 and ot_prob.tau_b < 1.0:
+      # the recentring terms of lse_step (sinkhorn.py:809-854) are not implemented;
+      # balanced problems and those with one tau = 1 are supported
+      raise NotImplementedError("recenter_potentials with tau_a < 1 and tau_b < 1 is not supported yet")
+    if self.recenter_potentials and self.max_iterations == 0:
+      raise NotImplementedError("recenter_potentials with max_iterations = 0 is not supported")
 
   def _check_options(self) -> None:
     """Solver options the HIP path runs; the others raise NotImplementedError."""
@@ -172,8 +178,6 @@ class Sinkhorn:
       raise NotImplementedError("Anderson acceleration is not supported by ott_amd")
     if self.parallel_dual_updates:
       raise NotImplementedError("parallel_dual_updates is not supported by ott_amd yet")
-    if self.recenter_potentials:
-      raise NotImplementedError("recenter_potentials is not supported by ott_amd yet")
     if not self.momentum.is_constant:
       raise NotImplementedError("adaptive momentum is not supported by ott_amd yet")
     if self.progress_fn is not None:
@@ -245,7 +249,8 @@ def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
       momentum=float(solver.momentum.value),
       tile_config=tune.lookup(geom.precision, geom.x.shape[1], n, m),
       use_graph=1 if solver.use_graph else 0,
-      init_mode=int(init_mode))
+      init_mode=int(init_mode),
+      recenter_potentials=1 if solver.recenter_potentials else 0)
   errors = torch.empty((solver.outer_iterations,), dtype=torch.float32, device=geom.device)
   out = _lib.SinkhornOut(f=f.data_ptr(), g=g.data_ptr(), errors=errors.data_ptr())
   nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts))

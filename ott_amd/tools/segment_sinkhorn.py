@@ -88,6 +88,8 @@ def segment_sinkhorn_outputs(
     raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
   solver = sinkhorn.Sinkhorn(**dict(sinkhorn_kwargs))
   solver._check_options()
+  if solver.recenter_potentials:
+    raise NotImplementedError("recenter_potentials is not supported by the batched solve")
   if solver.initializer is not None and not isinstance(solver.initializer, sinkhorn.init_lib.DefaultInitializer):
     raise NotImplementedError("segment_sinkhorn starts every problem from zero potentials")
   eps_abs, eps_rel = _epsilon_args(epsilon, relative_epsilon)

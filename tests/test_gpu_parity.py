@@ -201,3 +201,47 @@ def test_determinism():
   o2 = _solve(x, y, a, b, "bf16_split", dict(epsilon=0.05, relative_epsilon="mean"))
   assert torch.equal(o1.f, o2.f) and torch.equal(o1.g, o2.g)
   assert o1.reg_ot_cost == o2.reg_ot_cost
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("masses", ["equal", "unequal"])
+def test_recenter_potentials_balanced(precision, masses):
+  """recenter_potentials=True (sinkhorn.py:76-114, 1008-1009) on a balanced
+  problem: f has zero mean over its finite entries, f + g is that of the
+  plain solve, and outputs and reg_ot_cost match the oracle's.  With unequal
+  masses (sum b = 1.5 sum a, a fixed number of iterations) the shift moves the
+  dual terms of reg_ot_cost by c (sum b - sum a)."""
+  x, y, a, b = _cloud(173, 291, 4, seed=5, zero_a=[2], zero_b=[7])
+  kw = dict(inner_iterations=1, threshold=1e-4)
+  if masses == "unequal":
+    b = (b * 1.5).astype(np.float32)
+    kw = dict(inner_iterations=1, min_iterations=7, max_iterations=7)
+  out = _solve(x, y, a, b, precision, dict(epsilon=0.1), recenter_potentials=True, **kw)
+  plain = _solve(x, y, a, b, precision, dict(epsilon=0.1), **kw)
+  ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.1, recenter_potentials=True, **kw)
+  _check_solve(out, ref, precision)
+  f, g = out.f.cpu().numpy().astype(np.float64), out.g.cpu().numpy().astype(np.float64)
+  fp, gp = plain.f.cpu().numpy().astype(np.float64), plain.g.cpu().numpy().astype(np.float64)
+  fin = np.isfinite(f)
+  assert np.array_equal(fin, np.isfinite(fp)) and np.isneginf(f[2]) and np.isneginf(g[7])
+  scale = np.abs(fp[fin]).max()
+  assert abs(f[fin].mean()) <= 1e-5 * scale
+  c = fp[fin].mean()
+  np.testing.assert_allclose(f[fin], fp[fin] - c, atol=1e-5 * scale)
+  gfin = np.isfinite(g)
+  np.testing.assert_allclose(g[gfin], gp[gfin] + c, atol=1e-5 * scale)
+  shift = c * (float(b.astype(np.float64).sum()) - float(a.astype(np.float64).sum()))
+  assert abs(out.reg_ot_cost - (plain.reg_ot_cost + shift)) <= 1e-5 * max(1.0, abs(plain.reg_ot_cost))
+  # the errors do not depend on the gauge
+  np.testing.assert_array_equal(out.errors.cpu().numpy(), plain.errors.cpu().numpy())
+
+
+def test_recenter_potentials_unbalanced_one_side_is_a_no_op():
+  """With tau_a = 1 and tau_b < 1 the reference leaves the potentials as they are (sinkhorn.py:104-106)."""
+  x, y, a, b = _cloud(150, 120, 3, seed=6)
+  kw = dict(inner_iterations=1, tau_b=0.9)
+  out = _solve(x, y, a, b, "fp32", dict(epsilon=0.1), recenter_potentials=True, **dict(kw))
+  plain = _solve(x, y, a, b, "fp32", dict(epsilon=0.1), **dict(kw))
+  np.testing.assert_array_equal(out.f.cpu().numpy(), plain.f.cpu().numpy())
+  np.testing.assert_array_equal(out.g.cpu().numpy(), plain.g.cpu().numpy())
+  assert out.reg_ot_cost == plain.reg_ot_cost

@@ -244,3 +244,23 @@ def test_fp32_twin_agrees():
   assert o32.n_iters == o64.n_iters
   np.testing.assert_allclose(o32.f, o64.f, atol=2e-5)
   assert math.isclose(o32.reg_ot_cost, o64.reg_ot_cost, rel_tol=1e-5)
+
+
+def test_oracle_recenter_balanced_closed_form():
+  """SinkhornState.recenter, balanced (sinkhorn.py:98-102): zero mean over the
+  finite f, f_i + g_j unchanged, -inf entries kept."""
+  from oracle import sinkhorn64 as o
+  rng = np.random.default_rng(0)
+  f = rng.normal(size=9)
+  g = rng.normal(size=5)
+  f[3] = -np.inf
+  a = np.full(9, 1 / 9.0)
+  b = np.full(5, 0.2)
+  fr, gr = o.recenter(f, g, a, b, 0.1, 1.0, 1.0)
+  fin = np.isfinite(f)
+  assert abs(fr[fin].mean()) < 1e-14 and np.isneginf(fr[3])
+  np.testing.assert_allclose((fr[:, None] + gr[None])[fin], (f[:, None] + g[None])[fin], atol=1e-14)
+  np.testing.assert_allclose(fr[fin], f[fin] - f[fin].mean(), atol=1e-14)
+  # one tau = 1: unchanged (sinkhorn.py:104-106)
+  f2, g2 = o.recenter(f, g, a, b, 0.1, 1.0, 0.9)
+  assert f2 is f and g2 is g
