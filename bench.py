@@ -30,6 +30,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -54,7 +56,8 @@ def workload(name: str):
 
 def golden(name: str):
   path = {"gaussian_d64": "gaussian_shifted_n16384_d64.json",
-          "uniform_d2": "uniform_shifted_n1024_d2.json"}[name]
+          "uniform_d2": "uniform_shifted_n1024_d2.json",
+          "gmm_d16": "gmm_pair_n2048_d16_heldout.json"}[name]
   path = os.path.join(ROOT, "tests", "golden", path)
   if os.path.exists(path):
     with open(path) as fh:
@@ -157,6 +160,74 @@ def batched_cfg4(precision, steps, warmup, sequential=True):
       linear.solve(geom, **kw)
     torch.cuda.synchronize()
     res["sequential_single_solves_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+  return res
+
+
+def learned_gmm_d16(count=256, seed0=100000):
+  """The d=16 warm start on held-out 4-component mixture problems (n=m=2048).
+
+  Each problem is solved from zeros and from the shipped DeepSets start
+  (bf16 split, eps = 0.05 mean(C), 1e-3 checked every iteration); the
+  network's forward is inside the learned timings.  Seeds 200000.. are also
+  compared with the FP64 oracle values in tests/golden.
+  """
+  import torch
+  from ott_amd import datasets
+  from ott_amd.geometry import pointcloud
+  from ott_amd.initializers.neural import learned
+  from ott_amd.problems.linear import linear_problem
+  from ott_amd.solvers.linear import sinkhorn
+  init, name = learned.bench_initializer(16)
+  if init is None:
+    return None
+
+  def solve(x, y, ini):
+    geom = pointcloud.PointCloud(x, y, epsilon=0.05, relative_epsilon="mean", precision="bf16_split")
+    return sinkhorn.Sinkhorn(threshold=1e-3, inner_iterations=1, max_iterations=5000,
+                             initializer=ini)(linear_problem.LinearProblem(geom))
+
+  def timed(x, y, ini):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = solve(x, y, ini)
+    torch.cuda.synchronize()
+    return out, (time.perf_counter() - t0) * 1e3
+
+  rows = []
+  seeds = list(range(seed0, seed0 + count))
+  gold = golden("gmm_d16")
+  gold_seeds = [r["seed"] for r in gold["problems"]] if gold else []
+  for seed in seeds + gold_seeds:
+    pb = datasets.gmm_pair(seed)
+    x, y = torch.from_numpy(pb.x).cuda(), torch.from_numpy(pb.y).cuda()
+    if not rows:  # warm up both paths once
+      solve(x, y, None)
+      solve(x, y, init)
+    oz, tz = timed(x, y, None)
+    ol, tl = timed(x, y, init)
+    rows.append(dict(seed=seed, iz=oz.n_iters, il=ol.n_iters, tz=tz, tl=tl, rz=oz.reg_ot_cost,
+                     rl=ol.reg_ot_cost, conv=oz.converged and ol.converged))
+  held = [r for r in rows if r["seed"] in seeds]
+  res = {
+      "workload": f"gmm_pair n=m=2048 d=16 (4-component mixtures), {len(held)} held-out seeds "
+                  f"{seed0}..{seed0 + count - 1}, bf16 split, threshold 1e-3 every iteration",
+      "init": name,
+      "zeros": {"mean_iters": float(np.mean([r["iz"] for r in held])),
+                "mean_ms": round(float(np.mean([r["tz"] for r in held])), 3)},
+      "learned": {"mean_iters": float(np.mean([r["il"] for r in held])),
+                  "mean_ms": round(float(np.mean([r["tl"] for r in held])), 3)},
+      "all_converged": all(r["conv"] for r in held),
+      "learned_vs_zero_reg_rel_diff_max": float(max(abs(r["rl"] - r["rz"]) / abs(r["rz"]) for r in held)),
+  }
+  res["iteration_ratio"] = round(res["learned"]["mean_iters"] / res["zeros"]["mean_iters"], 3)
+  if gold:
+    ref = {r["seed"]: r["reg_ot_cost"] for r in gold["problems"]}
+    gr = [r for r in rows if r["seed"] in ref]
+    res["fp64_oracle_rel_err_max"] = {
+        "seeds": len(gr),
+        "zeros": float(max(abs(r["rz"] - ref[r["seed"]]) / abs(ref[r["seed"]]) for r in gr)),
+        "learned": float(max(abs(r["rl"] - ref[r["seed"]]) / abs(ref[r["seed"]]) for r in gr)),
+    }
   return res
 
 
@@ -276,6 +347,8 @@ def main() -> int:
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
   ap.add_argument("--cpu-budget", type=float, default=15.0)
   ap.add_argument("--no-batched", action="store_true", help="skip the 512-problem batched solve")
+  ap.add_argument("--no-learned-eval", action="store_true",
+                  help="skip the d=16 warm-start evaluation on 256 held-out problems")
   ap.add_argument("--batched-only", action="store_true", help="only the 512-problem batched solve")
   ap.add_argument("--halfstep-only", action="store_true",
                   help="only launch the half-step kernels and a calibration copy (for profiler passes)")
@@ -329,6 +402,11 @@ def main() -> int:
     batched = batched_cfg4("bf16_split", max(1, args.steps), 1)
     log(f"batched 512 problems: {batched}")
 
+  learned16 = None
+  if not args.no_learned_eval:
+    learned16 = learned_gmm_d16()
+    log(f"d=16 warm start: {learned16}")
+
   ms_a = sum(t_a) / len(t_a)
   ms_b = sum(t_b) / len(t_b)
   gold = golden(args.workload)
@@ -375,6 +453,7 @@ def main() -> int:
       "roofline": roof,
       "roofline_fp32_variant": roof_b,
       "batched_512_problems": batched,
+      "learned_warm_start_gmm_d16": learned16,
       "cpu_baseline": cpu,
   }
   print(json.dumps(line), flush=True)

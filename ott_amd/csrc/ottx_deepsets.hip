@@ -445,6 +445,127 @@ __global__ void head_out_kernel(const float* __restrict__ A, const float* __rest
   }
 }
 
+// The whole head for inference, one workgroup per 64 head rows:
+//   Z1 = relu(Y0 Wy^T + hctx[b])  (bf16, as the unfused GEMM rounds it)
+//   A2 = relu(Z1 W4^T + b4)       (float32)
+//   out = A2 w5 + b5 + zq[b] . Y0,  g = sigma_b^2 out
+// The row tile and Z1 stay in LDS; wave w owns output columns [64 w, 64 w + 64)
+// and reads its weight rows straight from L2 (every workgroup shares them).
+// KIN = the padded input width (64, 128, 192 or 256).  Grid rows_pad / 64.
+template <int KIN>
+__global__ __launch_bounds__(256) void head_fused_kernel(const bf16_t* __restrict__ Y0, const bf16_t* __restrict__ Wy,
+                                                         const float* __restrict__ hctx,
+                                                         const bf16_t* __restrict__ W4, const float* __restrict__ b4,
+                                                         const float* __restrict__ w5, const float* __restrict__ b5,
+                                                         const float* __restrict__ zq, const int* __restrict__ prob,
+                                                         int64_t rows, const float* __restrict__ sig2,
+                                                         float* __restrict__ out, float* __restrict__ g) {
+  constexpr int LY = KIN + 8, LZ = 256 + 8;  // LDS row strides (bf16)
+  __shared__ bf16_t Ys[64 * LY];
+  __shared__ bf16_t Zs[64 * LZ];
+  __shared__ float rsum[4][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
+  // the row tile of Y0 (padded rows are zero)
+  constexpr int G = 64 * KIN / 8;  // 16-byte groups
+  for (int i = tid; i < G; i += 256) {
+    const int r = i / (KIN / 8), c8 = i % (KIN / 8);
+    *reinterpret_cast<u32x4*>(&Ys[r * LY + c8 * 8]) =
+        *reinterpret_cast<const u32x4*>(Y0 + (r0 + r) * KIN + c8 * 8);
+  }
+  __syncthreads();
+  const int cw = wave * 64;         // first output column of this wave
+  const int lr = lane & 15, lk = 8 * (lane >> 4);
+  f32x4 acc[4][4];
+  // layer 1
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int k0 = 0; k0 < KIN; k0 += 32) {
+    bf16x8 bw[4], ay[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      bw[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(Wy + (cw + 16 * j + lr) * KIN + k0 + lk));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      ay[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Ys[(16 * i + lr) * LY + k0 + lk]));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ay[i], bw[j], acc[i][j], 0, 0, 0);
+  }
+  // element (row 16 i + 4 (lane / 16) + r, column cw + 16 j + lane % 16)
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * i + 4 * (lane >> 4) + r;
+      const int64_t q = r0 + row;
+      const float* hb = hctx + static_cast<int64_t>(q < rows ? prob[q] : 0) * 256;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int col = cw + 16 * j + lr;
+        const float v = q < rows ? fmaxf(acc[i][j][r] + hb[col], 0.f) : 0.f;
+        Zs[row * LZ + col] = to_bf16(v);
+      }
+    }
+  __syncthreads();
+  // layer 2
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int k0 = 0; k0 < 256; k0 += 32) {
+    bf16x8 bw[4], az[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      bw[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(W4 + (cw + 16 * j + lr) * 256 + k0 + lk));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      az[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Zs[(16 * i + lr) * LZ + k0 + lk]));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(az[i], bw[j], acc[i][j], 0, 0, 0);
+  }
+  // w5 . relu(. + b4) over this wave's 64 columns, per row
+  float wv[4], bv[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    wv[j] = w5[cw + 16 * j + lr];
+    bv[j] = b4[cw + 16 * j + lr];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float sacc = 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sacc = fmaf(wv[j], fmaxf(acc[i][j][r] + bv[j], 0.f), sacc);
+      // sum over the 16 lanes that share the row (lane bits 0..3)
+      sacc += __shfl_xor(sacc, 1);
+      sacc += __shfl_xor(sacc, 2);
+      sacc += __shfl_xor(sacc, 4);
+      sacc += __shfl_xor(sacc, 8);
+      if (lr == 0) rsum[wave][16 * i + 4 * (lane >> 4) + r] = sacc;
+    }
+  __syncthreads();
+  if (tid < 64) {
+    const int64_t q = r0 + tid;
+    if (q < rows) {
+      const int b = prob[q];
+      float v = ((rsum[0][tid] + rsum[1][tid]) + (rsum[2][tid] + rsum[3][tid])) + b5[0];
+      const float* z = zq + static_cast<int64_t>(b) * KIN;
+      for (int k = 0; k < KIN; ++k) v = fmaf(z[k], from_bf16(Ys[tid * LY + k]), v);
+      out[q] = v;
+      if (g) g[q] = sig2[b] * v;
+    }
+  }
+}
+
 // Centred MSE per problem on the head points:
 // L = (1/B) sum_b (1/m_b) sum_j (r_j - mean_b r)^2,  r = out - target;
 // dout_j = 2 (r_j - mean_b r) / (B m_b).  loss[b] receives L_b.
@@ -921,6 +1042,23 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   hipLaunchKernelGGL(ctx_forward_kernel, dim3((256 + P.kin + 3) / 4, B), dim3(256), 0, st, prm + P.Wc, prm + P.bh,
                      prm + P.Wq, at<float>(ws, L.ctx), bt->d, P.kin, at<float>(ws, L.hctx), at<float>(ws, L.zq));
   DS_LAUNCHED();
+  if (!training) {
+    // inference: the head in one launch
+    const unsigned grid = static_cast<unsigned>(Ph / 64);
+    auto launch = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), 0, st, Y0, wb + C.Wy, at<float>(ws, L.hctx), wb + C.W4,
+                         prm + P.b4, prm + P.w5, prm + P.b5, at<float>(ws, L.zq), bt->y_prob, L.Ph, sig2,
+                         at<float>(ws, L.out), g_out);
+    };
+    switch (P.kin) {
+      case 64: launch(head_fused_kernel<64>); break;
+      case 128: launch(head_fused_kernel<128>); break;
+      case 192: launch(head_fused_kernel<192>); break;
+      default: launch(head_fused_kernel<256>); break;
+    }
+    DS_LAUNCHED();
+    return OTTX_OK;
+  }
   // head layer 1: Z1 = relu(Y0 Wy^T + hctx[b])
   g = gemm_args(Y0, P.kin, wb + C.Wy, P.kin, static_cast<int>(Ph), 256, P.kin);
   g.gbias = at<float>(ws, L.hctx);

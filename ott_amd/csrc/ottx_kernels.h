@@ -68,21 +68,13 @@ __device__ __forceinline__ float2 load_f2_coherent(const float2* p) {
 // ---------------------------------------------------------------------------
 // Block finalizer: sinkhorn.py:914-931 (errors layout), :940-953 and
 // fixed_point_loop.py:63-70 (continue test), run by one thread of the last
-// column tile of an error pass.
+// column tile of an error pass with the pass's sums: col = sum |colmarg -
+// target|^p, row (unbalanced: the row term) and the total mass.
 // ---------------------------------------------------------------------------
-__device__ inline void finalize_block(const HalfStepArgs& p) {
-  double col = 0.0, mass = 0.0;
-  for (int t = 0; t < p.tiles; ++t) {
-    col += load_f64_coherent(p.err_partial + t);
-    mass += load_f64_coherent(p.mass_partial + t);
-  }
+__device__ inline void finalize_block(const HalfStepArgs& p, double col, double row, double mass) {
   const double inv_p = 1.0 / static_cast<double>(p.norm_p);
   double err = (p.norm_p == 1) ? col : pow(col, inv_p);
-  if (!p.balanced) {
-    double row = 0.0;
-    for (int t = 0; t < p.row_tiles; ++t) row += p.row_err_partial[t];
-    err += (p.norm_p == 1) ? row : pow(row, inv_p);
-  }
+  if (!p.balanced) err += (p.norm_p == 1) ? row : pow(row, inv_p);
   SolveState* st = p.state;
   const int k = st->block;
   const int t_last = k * p.inner + p.inner - 1;
@@ -626,8 +618,29 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
       if (tid == 0) {
         release_to_device();
         const uint32_t prev = atomicAdd(&p.state->tiles_done, 1u);
-        if (prev == static_cast<uint32_t>(p.tiles - 1)) {
-          finalize_block(p);
+        s_last = (prev == static_cast<uint32_t>(p.tiles - 1)) ? 1 : 0;
+      }
+      __syncthreads();
+      if (s_last) {
+        // the last tile: every partial of the pass loaded at once (a few per
+        // thread) and summed in a fixed-order tree, then the block's finalizer
+        double col = 0.0, mass = 0.0, row = 0.0;
+        for (int t = tid; t < p.tiles; t += NT) {
+          col += load_f64_coherent(p.err_partial + t);
+          if (p.mass_partial) mass += load_f64_coherent(p.mass_partial + t);
+        }
+        if (!p.balanced)
+          for (int t = tid; t < p.row_tiles; t += NT) row += p.row_err_partial[t];
+        block_sum2<NT>(s_red, tid, col, mass);
+        col = s_red[0][0];
+        mass = s_red[1][0];
+        __syncthreads();
+        if (!p.balanced) {
+          block_sum2<NT>(s_red, tid, row, 0.0);
+          row = s_red[0][0];
+        }
+        if (tid == 0) {
+          finalize_block(p, col, row, mass);
           p.state->tiles_done = 0u;
         }
       }

@@ -136,13 +136,24 @@ __device__ __forceinline__ void pack_bf16_one(const float* __restrict__ pts, int
   const int64_t row = rb * 16 + (lane & 15);
   const int k0 = kb * 32 + 8 * (lane >> 4);
   uint32_t hw[4], lw[4];
+  // the 8 features as two 16-byte loads when the row holds all of them
+  float in[8];
+  if (row < np && k0 + 8 <= d && (d & 3) == 0 && (reinterpret_cast<uintptr_t>(pts) & 15) == 0) {
+    const float4 p0 = *reinterpret_cast<const float4*>(pts + row * d + k0);
+    const float4 p1 = *reinterpret_cast<const float4*>(pts + row * d + k0 + 4);
+    in[0] = p0.x; in[1] = p0.y; in[2] = p0.z; in[3] = p0.w;
+    in[4] = p1.x; in[5] = p1.y; in[6] = p1.z; in[7] = p1.w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) in[e] = (row < np && k0 + e < d) ? pts[row * d + k0 + e] : 0.f;
+  }
 #pragma unroll
   for (int e = 0; e < 8; e += 2) {
     uint32_t h2[2], l2[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int k = k0 + e + q;
-      const float v = (row < np && k < d) ? pts[row * d + k] - mu[k] : 0.f;
+      const float v = (row < np && k < d) ? in[e + q] - mu[k] : 0.f;
       const uint32_t h = f32_to_bf16_rne(v);
       const uint32_t l = f32_to_bf16_rne(v - bf16_to_f32(h));
       h2[q] = h;
@@ -301,12 +312,26 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
       return;
     }
     const float* pt = p.pts[side] + local * p.d;
-    double acc = 0.0;
-    for (int k = 0; k < p.d; ++k) {
-      const double v = static_cast<double>(pt[k] - p.mu[k]);
-      acc += v * v;
+    // four independent sums; 16-byte loads when the rows allow them
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = 0;
+    if ((p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[side]) & 15) == 0) {
+      for (; k < p.d; k += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(pt + k);
+        const float4 u = *reinterpret_cast<const float4*>(p.mu + k);
+        const double d0 = static_cast<double>(v.x - u.x), d1 = static_cast<double>(v.y - u.y);
+        const double d2 = static_cast<double>(v.z - u.z), d3 = static_cast<double>(v.w - u.w);
+        a0 += d0 * d0;
+        a1 += d1 * d1;
+        a2 += d2 * d2;
+        a3 += d3 * d3;
+      }
     }
-    norm[local] = static_cast<float>(p.s * acc);
+    for (; k < p.d; ++k) {
+      const double v = static_cast<double>(pt[k] - p.mu[k]);
+      a0 += v * v;
+    }
+    norm[local] = static_cast<float>(p.s * ((a0 + a1) + (a2 + a3)));
   } else if (segi < 6) {  // uniform weights
     const int side = segi - 4;
     if (local < p.np[side]) p.wfill[side][local] = static_cast<float>(1.0 / static_cast<double>(p.np[side]));

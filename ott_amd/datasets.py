@@ -61,6 +61,20 @@ def gaussian_mixture(rng: np.random.Generator, n: int, d: int, k: int) -> np.nda
   return means[comp] + rng.normal(size=(n, d)) / np.sqrt(d)
 
 
+def gmm_pair(seed: int, n: int = 2048, m: int = 2048, d: int = 16, k: int = 4,
+             eps_mult: float = 0.05) -> Problem:
+  """x and y: k-component mixtures drawn in turn from one generator (seed); uniform weights.
+
+  The learned warm start's d=16 family: its training problems use seeds 1000..20999
+  (with eps_mult drawn per problem), held-out evaluation seeds start at 100000.
+  """
+  rng = np.random.default_rng(seed)
+  x = gaussian_mixture(rng, n, d, k)
+  y = gaussian_mixture(rng, m, d, k)
+  return Problem(x.astype(np.float32), y.astype(np.float32), None, None, eps_mult,
+                 name=f"gmm_pair_seed{seed}_n{n}_m{m}_d{d}")
+
+
 def mixture_pair(n: int = 32768, m: int = 8192, d: int = 128, k: int = 8,
                  seed_x: int = 2, seed_y: int = 3, tau: float = 0.95,
                  eps_mult: float = 0.02, weights: str = "uniform_noise") -> Problem:

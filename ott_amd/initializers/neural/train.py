@@ -43,6 +43,7 @@ def make_problem(kind: str, seed: int, n: int, m: int, d: int, eps_mult: float =
                  ) -> Tuple[np.ndarray, np.ndarray, float]:
   rng = np.random.default_rng(seed)
   if kind == "gmm":
+    # the same draws as datasets.gmm_pair(seed, n, m, d)
     x = datasets.gaussian_mixture(rng, n, d, 4)
     y = datasets.gaussian_mixture(rng, m, d, 4)
   elif kind == "gauss":

@@ -5,7 +5,7 @@ golden values for this path, so these fixtures are oracle outputs on seeded
 inputs (regenerated from ``ott_amd.datasets``): they let GPU tests and the
 benchmark check full-size results without re-running a minutes-long FP64 solve.
 
-  python tests/golden/make_golden.py [uniform_d2] [gaussian_d64] [mixture_d128] [batched_d32]
+  python tests/golden/make_golden.py [uniform_d2] [gaussian_d64] [mixture_d128] [batched_d32] [gmm_d16]
 """
 
 from __future__ import annotations
@@ -80,6 +80,18 @@ def main(which):
                    oracle_wall_s=time.time() - t0),
               open(os.path.join(HERE, "batched_uneven_512_d32.json"), "w"))
     print("batched_uneven_512_d32", sum(r["n_iters"] for r in recs), time.time() - t0, flush=True)
+  if "gmm_d16" in which:
+    # held-out problems of the d=16 warm-start family: zero-start oracle solves,
+    # error checked every iteration as in the benchmark
+    recs = []
+    for seed in range(200000, 200008):
+      pb = datasets.gmm_pair(seed)
+      rec, _ = _run(pb, 5000, inner_iterations=1)
+      recs.append(dict(seed=seed, epsilon=rec["epsilon"], reg_ot_cost=rec["reg_ot_cost"],
+                       n_iters=rec["n_iters"], converged=rec["converged"]))
+      print(pb.name, rec["n_iters"], rec["reg_ot_cost"], flush=True)
+    json.dump(dict(name="gmm_pair_n2048_d16_heldout", threshold=1e-3, inner_iterations=1, problems=recs),
+              open(os.path.join(HERE, "gmm_pair_n2048_d16_heldout.json"), "w"), indent=1)
 
 
 if __name__ == "__main__":
