@@ -250,7 +250,7 @@ __device__ __forceinline__ void tile_mfma(const OutOperand<PREC, KK, NB>& B, con
 }
 
 // exp2 of one tile's accumulators against the fixed per-column shift mx[],
-// summed into sm[] (the SHIFT half of compute_tile).
+// summed into sm[] (fixed-shift mode).
 template <int NB, int MT>
 __device__ __forceinline__ void tile_exp_shift(const f32x4 (&acc)[MT][NB], float c, const float (&mx)[NB],
                                                float (&sm)[NB]) {
@@ -268,42 +268,47 @@ __device__ __forceinline__ void tile_exp_shift(const f32x4 (&acc)[MT][NB], float
   }
 }
 
-template <int PREC, int KK, int NB, int MT, bool SHIFT>
-__device__ __forceinline__ void compute_tile(const OutOperand<PREC, KK, NB>& B, const u32x4* L, const float* LB,
-                                             int lane, float c, float (&mx)[NB], float (&sm)[NB]) {
-  f32x4 acc[MT][NB];
-  tile_mfma<PREC, KK, NB, MT>(B, L, LB, lane, acc);
-  if constexpr (SHIFT) {
-    tile_exp_shift<NB, MT>(acc, c, mx, sm);
-  } else {
-    // Streaming log-sum-exp, base 2: one rescale per column per tile.  The
-    // tile max is a tree and the sum uses four partial accumulators, so no
-    // VALU chain is longer than MT + 2 dependent operations.
+// Streaming log-sum-exp of one tile's accumulators, base 2, with a running
+// per-column max (the mode without a fixed shift): one rescale per column per
+// tile.  The tile max is a tree and the sum uses four
+// partial accumulators, so no VALU chain is longer than MT + 2 dependent
+// operations.
+template <int NB, int MT>
+__device__ __forceinline__ void tile_exp_max(const f32x4 (&acc)[MT][NB], float c, float (&mx)[NB],
+                                           float (&sm)[NB]) {
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-      float m4[MT];
+  for (int nb = 0; nb < NB; ++nb) {
+    float m4[MT];
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-        m4[mt] = fmaxf(fmaxf(acc[mt][nb][0], acc[mt][nb][1]), fmaxf(acc[mt][nb][2], acc[mt][nb][3]));
+    for (int mt = 0; mt < MT; ++mt)
+      m4[mt] = fmaxf(fmaxf(acc[mt][nb][0], acc[mt][nb][1]), fmaxf(acc[mt][nb][2], acc[mt][nb][3]));
 #pragma unroll
-      for (int w = 1; w < MT; w <<= 1)
+    for (int w = 1; w < MT; w <<= 1)
 #pragma unroll
-        for (int mt = 0; mt + w < MT; mt += 2 * w) m4[mt] = fmaxf(m4[mt], m4[mt + w]);
-      const float mnew = fmaxf(mx[nb], m4[0]);
-      if (mnew != -INFINITY) {
-        const float nmc = -c * mnew;
-        float part[4];
+      for (int mt = 0; mt + w < MT; mt += 2 * w) m4[mt] = fmaxf(m4[mt], m4[mt + w]);
+    const float mnew = fmaxf(mx[nb], m4[0]);
+    if (mnew != -INFINITY) {
+      const float nmc = -c * mnew;
+      float part[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) part[r] = exp2_fast(fmaf(acc[0][nb][r], c, nmc));
+      for (int r = 0; r < 4; ++r) part[r] = exp2_fast(fmaf(acc[0][nb][r], c, nmc));
 #pragma unroll
-        for (int mt = 1; mt < MT; ++mt)
+      for (int mt = 1; mt < MT; ++mt)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) part[r] += exp2_fast(fmaf(acc[mt][nb][r], c, nmc));
-        sm[nb] = fmaf(sm[nb], exp2_fast(fmaf(mx[nb], c, nmc)), (part[0] + part[1]) + (part[2] + part[3]));
-        mx[nb] = mnew;
-      }
+        for (int r = 0; r < 4; ++r) part[r] += exp2_fast(fmaf(acc[mt][nb][r], c, nmc));
+      sm[nb] = fmaf(sm[nb], exp2_fast(fmaf(mx[nb], c, nmc)), (part[0] + part[1]) + (part[2] + part[3]));
+      mx[nb] = mnew;
     }
   }
+}
+
+// exp / sum of one tile's accumulators in either mode.
+template <int NB, int MT, bool SHIFT>
+__device__ __forceinline__ void tile_exp(const f32x4 (&acc)[MT][NB], float c, float (&mx)[NB], float (&sm)[NB]) {
+  if constexpr (SHIFT)
+    tile_exp_shift<NB, MT>(acc, c, mx, sm);
+  else
+    tile_exp_max<NB, MT>(acc, c, mx, sm);
 }
 
 // Streams in-side row blocks [rb_begin, rb_end) (multiples of MT) against the
@@ -343,65 +348,50 @@ __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, c
   tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb_begin, tid, pre0, pb0);
   tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb_begin + MT, rb_last), tid, pre1,
                                               pb1);
-  if constexpr (SHIFT) {
-    // Software pipeline across tiles: the exp2 / sum of tile t runs while the
-    // MFMAs of tile t + 1 are in flight (independent instructions for the
-    // scheduler to interleave), so one wave keeps both the matrix core and
-    // the VALU busy.
-    f32x4 acc0[MT][NB], acc1[MT][NB];
+  // Software pipeline across tiles: the exp2 / sum of tile t runs after the
+  // MFMAs of tile t + 1 are issued.  The MFMAs run at raised wave priority, so
+  // of the two waves on a SIMD the one with matrix work keeps the matrix core
+  // fed while the other's exp2 / sum fills the VALU.
+  f32x4 acc0[MT][NB], acc1[MT][NB];
 #pragma unroll
-    for (int q = 0; q < PER_T; ++q)
-      if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
-    if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb_begin) * 16 + tid, n_in, inv2s);
-    __syncthreads();
-    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb_begin + 2 * MT, rb_last), tid,
-                                                pre0, pb0);
-    tile_mfma<PREC, KK, NB, MT>(B, L0, B0, lane, acc0);
-    for (int rb = rb_begin; rb < rb_end; rb += 2 * MT) {
-      // tile rb + MT: stage, MFMA, and finish tile rb
-#pragma unroll
-      for (int q = 0; q < PER_T; ++q)
-        if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L1[tid + q * NT] = pre1[q];
-      if (tid < ROWS) B1[tid] = row_bias(pb1, static_cast<int64_t>(rb + MT) * 16 + tid, n_in, inv2s);
-      __syncthreads();
-      tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 3 * MT, rb_last), tid,
-                                                  pre1, pb1);
-      tile_mfma<PREC, KK, NB, MT>(B, L1, B1, lane, acc1);
-      tile_exp_shift<NB, MT>(acc0, c, mx, sm);
-      if (rb + 2 * MT >= rb_end) {
-        tile_exp_shift<NB, MT>(acc1, c, mx, sm);
-        break;
-      }
-      // tile rb + 2 MT: stage, MFMA, and finish tile rb + MT
-#pragma unroll
-      for (int q = 0; q < PER_T; ++q)
-        if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
-      if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb + 2 * MT) * 16 + tid, n_in, inv2s);
-      __syncthreads();
-      tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 4 * MT, rb_last), tid,
-                                                  pre0, pb0);
-      tile_mfma<PREC, KK, NB, MT>(B, L0, B0, lane, acc0);
-      tile_exp_shift<NB, MT>(acc1, c, mx, sm);
-    }
-    return;
-  }
+  for (int q = 0; q < PER_T; ++q)
+    if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
+  if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb_begin) * 16 + tid, n_in, inv2s);
+  __syncthreads();
+  tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb_begin + 2 * MT, rb_last), tid,
+                                              pre0, pb0);
+  __builtin_amdgcn_s_setprio(1);
+  tile_mfma<PREC, KK, NB, MT>(B, L0, B0, lane, acc0);
+  __builtin_amdgcn_s_setprio(0);
   for (int rb = rb_begin; rb < rb_end; rb += 2 * MT) {
-#pragma unroll
-    for (int q = 0; q < PER_T; ++q)
-      if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
-    if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb) * 16 + tid, n_in, inv2s);
-    __syncthreads();
-    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 2 * MT, rb_last), tid, pre0,
-                                                pb0);
-    compute_tile<PREC, KK, NB, MT, SHIFT>(B, L0, B0, lane, c, mx, sm);
+    // tile rb + MT: stage, MFMA, and finish tile rb
 #pragma unroll
     for (int q = 0; q < PER_T; ++q)
       if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L1[tid + q * NT] = pre1[q];
     if (tid < ROWS) B1[tid] = row_bias(pb1, static_cast<int64_t>(rb + MT) * 16 + tid, n_in, inv2s);
     __syncthreads();
-    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 3 * MT, rb_last), tid, pre1,
-                                                pb1);
-    compute_tile<PREC, KK, NB, MT, SHIFT>(B, L1, B1, lane, c, mx, sm);
+    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 3 * MT, rb_last), tid,
+                                                pre1, pb1);
+    __builtin_amdgcn_s_setprio(1);
+    tile_mfma<PREC, KK, NB, MT>(B, L1, B1, lane, acc1);
+    __builtin_amdgcn_s_setprio(0);
+    tile_exp<NB, MT, SHIFT>(acc0, c, mx, sm);
+    if (rb + 2 * MT >= rb_end) {
+      tile_exp<NB, MT, SHIFT>(acc1, c, mx, sm);
+      break;
+    }
+    // tile rb + 2 MT: stage, MFMA, and finish tile rb + MT
+#pragma unroll
+    for (int q = 0; q < PER_T; ++q)
+      if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
+    if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb + 2 * MT) * 16 + tid, n_in, inv2s);
+    __syncthreads();
+    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 4 * MT, rb_last), tid,
+                                                pre0, pb0);
+    __builtin_amdgcn_s_setprio(1);
+    tile_mfma<PREC, KK, NB, MT>(B, L0, B0, lane, acc0);
+    __builtin_amdgcn_s_setprio(0);
+    tile_exp<NB, MT, SHIFT>(acc1, c, mx, sm);
   }
 }
 
