@@ -445,6 +445,106 @@ __global__ void head_out_kernel(const float* __restrict__ A, const float* __rest
   }
 }
 
+// One layer over a tile of 16 RB rows for the fused inference kernels:
+// acc = In . W^T over K, In a [16 RB x K] bf16 tile in LDS (row stride LI),
+// W [256 x K] bf16 rows in global memory (read by every workgroup, so from
+// L2).  Wave cw / 64 owns the output columns [cw, cw + 64): acc[i][j] holds
+// rows 16 i + 4 (lane / 16) + r and column cw + 16 j + lane % 16.
+template <int RB, int K, int LI>
+__device__ __forceinline__ void layer_tile(const bf16_t* In, const bf16_t* __restrict__ W, int cw, int lane,
+                                           f32x4 (&acc)[RB][4]) {
+  const int lr = lane & 15, lk = 8 * (lane >> 4);
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    bf16x8 bw[4], ai[RB];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      bw[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(W + (cw + 16 * j + lr) * K + k0 + lk));
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      ai[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&In[(16 * i + lr) * LI + k0 + lk]));
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai[i], bw[j], acc[i][j], 0, 0, 0);
+  }
+}
+
+// A [ROWS x K] bf16 row tile of a row-major matrix into LDS (row stride LI).
+template <int ROWS, int K, int LI>
+__device__ __forceinline__ void load_tile(const bf16_t* __restrict__ src, int64_t r0, bf16_t* dst, int tid) {
+  for (int i = tid; i < ROWS * K / 8; i += 256) {
+    const int r = i / (K / 8), c8 = i % (K / 8);
+    *reinterpret_cast<u32x4*>(&dst[r * LI + c8 * 8]) = *reinterpret_cast<const u32x4*>(src + (r0 + r) * K + c8 * 8);
+  }
+}
+
+// relu(acc + bias) rounded to bf16 into an LDS tile; rows >= valid are zero
+// (as the unfused GEMM epilogue writes them).
+template <int RB, int LO>
+__device__ __forceinline__ void relu_bf16_tile(const f32x4 (&acc)[RB][4], const float* __restrict__ bias, int cw,
+                                               int lane, int64_t r0, int64_t valid, bf16_t* dst) {
+  const int lr = lane & 15;
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * i + 4 * (lane >> 4) + r;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int col = cw + 16 * j + lr;
+        const float v = r0 + row < valid ? fmaxf(acc[i][j][r] + bias[col], 0.f) : 0.f;
+        dst[row * LO + col] = to_bf16(v);
+      }
+    }
+}
+
+// The encoder for inference, one workgroup per 32 encoder rows (the encoder
+// sees a few thousand rows: small tiles spread them over more CUs):
+//   A0 = relu(X0 W0^T + b0), A1 = relu(A0 W1^T + b1) (bf16, as the unfused
+//   GEMMs round them), H2 = A1 W2^T + b2 (float32, for the pooling).
+// Grid rows_pad / 32.
+template <int KIN>
+__global__ __launch_bounds__(256) void encoder_fused_kernel(const bf16_t* __restrict__ X0,
+                                                            const bf16_t* __restrict__ W0, const float* __restrict__ b0,
+                                                            const bf16_t* __restrict__ W1, const float* __restrict__ b1,
+                                                            const bf16_t* __restrict__ W2, const float* __restrict__ b2,
+                                                            int64_t rows, float* __restrict__ H2) {
+  constexpr int RB = 2, LX = KIN + 8, LZ = 256 + 8;
+  __shared__ bf16_t Xs[16 * RB * LX];
+  __shared__ bf16_t Za[16 * RB * LZ];
+  __shared__ bf16_t Zb[16 * RB * LZ];
+  const int tid = threadIdx.x, lane = tid & 63, cw = (tid >> 6) * 64;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 16 * RB;
+  load_tile<16 * RB, KIN, LX>(X0, r0, Xs, tid);
+  __syncthreads();
+  f32x4 acc[RB][4];
+  layer_tile<RB, KIN, LX>(Xs, W0, cw, lane, acc);
+  relu_bf16_tile<RB, LZ>(acc, b0, cw, lane, r0, rows, Za);
+  __syncthreads();
+  layer_tile<RB, 256, LZ>(Za, W1, cw, lane, acc);
+  relu_bf16_tile<RB, LZ>(acc, b1, cw, lane, r0, rows, Zb);
+  __syncthreads();
+  layer_tile<RB, 256, LZ>(Zb, W2, cw, lane, acc);
+  const int lr = lane & 15;
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 16 * i + 4 * (lane >> 4) + r;
+      const int64_t q = r0 + row;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int col = cw + 16 * j + lr;
+        H2[q * 256 + col] = q < rows ? acc[i][j][r] + b2[col] : 0.f;
+      }
+    }
+}
+
 // The whole head for inference, one workgroup per 64 head rows:
 //   Z1 = relu(Y0 Wy^T + hctx[b])  (bf16, as the unfused GEMM rounds it)
 //   A2 = relu(Z1 W4^T + b4)       (float32)
@@ -466,37 +566,13 @@ __global__ __launch_bounds__(256) void head_fused_kernel(const bf16_t* __restric
   __shared__ float rsum[4][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
-  // the row tile of Y0 (padded rows are zero)
-  constexpr int G = 64 * KIN / 8;  // 16-byte groups
-  for (int i = tid; i < G; i += 256) {
-    const int r = i / (KIN / 8), c8 = i % (KIN / 8);
-    *reinterpret_cast<u32x4*>(&Ys[r * LY + c8 * 8]) =
-        *reinterpret_cast<const u32x4*>(Y0 + (r0 + r) * KIN + c8 * 8);
-  }
+  load_tile<64, KIN, LY>(Y0, r0, Ys, tid);  // padded rows are zero
   __syncthreads();
-  const int cw = wave * 64;         // first output column of this wave
-  const int lr = lane & 15, lk = 8 * (lane >> 4);
+  const int cw = wave * 64;  // first output column of this wave
+  const int lr = lane & 15;
   f32x4 acc[4][4];
-  // layer 1
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int k0 = 0; k0 < KIN; k0 += 32) {
-    bf16x8 bw[4], ay[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      bw[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(Wy + (cw + 16 * j + lr) * KIN + k0 + lk));
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      ay[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Ys[(16 * i + lr) * LY + k0 + lk]));
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ay[i], bw[j], acc[i][j], 0, 0, 0);
-  }
-  // element (row 16 i + 4 (lane / 16) + r, column cw + 16 j + lane % 16)
+  // layer 1, with the per-problem bias of each row
+  layer_tile<4, KIN, LY>(Ys, Wy, cw, lane, acc);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -513,24 +589,7 @@ __global__ __launch_bounds__(256) void head_fused_kernel(const bf16_t* __restric
     }
   __syncthreads();
   // layer 2
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int k0 = 0; k0 < 256; k0 += 32) {
-    bf16x8 bw[4], az[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      bw[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(W4 + (cw + 16 * j + lr) * 256 + k0 + lk));
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      az[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&Zs[(16 * i + lr) * LZ + k0 + lk]));
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(az[i], bw[j], acc[i][j], 0, 0, 0);
-  }
+  layer_tile<4, 256, LZ>(Zs, W4, cw, lane, acc);
   // w5 . relu(. + b4) over this wave's 64 columns, per row
   float wv[4], bv[4];
 #pragma unroll
@@ -1007,33 +1066,49 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
                      P.kin, mu, sig2, Y0, training ? Y0T : nullptr);
   DS_LAUNCHED();
   int rc;
-  // encoder layer 0: A0 = relu(X0 W0^T + b0)
-  GemmArgs g = gemm_args(X0, P.kin, wb + C.W0, P.kin, static_cast<int>(Pe), 256, P.kin);
-  g.bias = prm + P.b0;
-  g.relu = 1;
-  g.Cb = at<bf16_t>(ws, L.A0);
-  g.ldcb = 256;
-  g.CbT = training ? at<bf16_t>(ws, L.A0T) : nullptr;
-  g.ldct = Pe;
-  g.valid_rows = static_cast<int>(L.Pe);
-  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
-  // encoder layer 1: A1 = relu(A0 W1^T + b1)
-  g = gemm_args(at<bf16_t>(ws, L.A0), 256, wb + C.W1, 256, static_cast<int>(Pe), 256, 256);
-  g.bias = prm + P.b1;
-  g.relu = 1;
-  g.Cb = at<bf16_t>(ws, L.A1);
-  g.ldcb = 256;
-  g.CbT = training ? at<bf16_t>(ws, L.A1T) : nullptr;
-  g.ldct = Pe;
-  g.valid_rows = static_cast<int>(L.Pe);
-  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
-  // encoder layer 2 (linear): H2 = A1 W2^T + b2 (f32 for the pooling)
-  g = gemm_args(at<bf16_t>(ws, L.A1), 256, wb + C.W2, 256, static_cast<int>(Pe), 256, 256);
-  g.bias = prm + P.b2;
-  g.Cf = at<float>(ws, L.H2f);
-  g.ldc = 256;
-  g.valid_rows = static_cast<int>(L.Pe);
-  if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+  GemmArgs g;
+  if (!training) {
+    // inference: the encoder in one launch
+    auto launch = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(Pe / 32)), dim3(256), 0, st, X0, wb + C.W0, prm + P.b0,
+                         wb + C.W1, prm + P.b1, wb + C.W2, prm + P.b2, L.Pe, at<float>(ws, L.H2f));
+    };
+    switch (P.kin) {
+      case 64: launch(encoder_fused_kernel<64>); break;
+      case 128: launch(encoder_fused_kernel<128>); break;
+      case 192: launch(encoder_fused_kernel<192>); break;
+      default: launch(encoder_fused_kernel<256>); break;
+    }
+    DS_LAUNCHED();
+  } else {
+    // encoder layer 0: A0 = relu(X0 W0^T + b0)
+    g = gemm_args(X0, P.kin, wb + C.W0, P.kin, static_cast<int>(Pe), 256, P.kin);
+    g.bias = prm + P.b0;
+    g.relu = 1;
+    g.Cb = at<bf16_t>(ws, L.A0);
+    g.ldcb = 256;
+    g.CbT = training ? at<bf16_t>(ws, L.A0T) : nullptr;
+    g.ldct = Pe;
+    g.valid_rows = static_cast<int>(L.Pe);
+    if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+    // encoder layer 1: A1 = relu(A0 W1^T + b1)
+    g = gemm_args(at<bf16_t>(ws, L.A0), 256, wb + C.W1, 256, static_cast<int>(Pe), 256, 256);
+    g.bias = prm + P.b1;
+    g.relu = 1;
+    g.Cb = at<bf16_t>(ws, L.A1);
+    g.ldcb = 256;
+    g.CbT = training ? at<bf16_t>(ws, L.A1T) : nullptr;
+    g.ldct = Pe;
+    g.valid_rows = static_cast<int>(L.Pe);
+    if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+    // encoder layer 2 (linear): H2 = A1 W2^T + b2 (f32 for the pooling)
+    g = gemm_args(at<bf16_t>(ws, L.A1), 256, wb + C.W2, 256, static_cast<int>(Pe), 256, 256);
+    g.bias = prm + P.b2;
+    g.Cf = at<float>(ws, L.H2f);
+    g.ldc = 256;
+    g.valid_rows = static_cast<int>(L.Pe);
+    if ((rc = gemm(st, g)) != OTTX_OK) return rc;
+  }
   // pooling per (problem, side) and the context vector
   hipLaunchKernelGGL(pool_partial_kernel, dim3(L.chunks, 2 * B), dim3(256), 0, st, at<float>(ws, L.H2f), bt->seg_off,
                      L.chunks, at<float>(ws, L.part));
