@@ -395,6 +395,75 @@ __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, c
   }
 }
 
+// Ping-pong variant of stream_rows for 8-wave workgroups.  Waves 0-3 (group 0)
+// and 4-7 (group 1) sit two per SIMD and work half a tile apart, locked by
+// one barrier per phase: in phase k, group g runs the MFMAs of tile
+// (k - g) / 2 when k - g is even and the exp2 / sum of tile (k - g - 1) / 2
+// when it is odd.  So in every phase each SIMD has one wave on the matrix
+// core and its partner on the VALU, and each wave needs one accumulator set.
+// Tile t + 1 is staged into LDS buffer (t + 1) & 1 at the start of phase
+// 2t + 1, when the buffer's previous tile (t - 1) has been read by both
+// groups; its global prefetch was issued two phases earlier.
+template <int PREC, int KK, int NB, int MT, bool SHIFT>
+__device__ __forceinline__ void stream_rows_pp(const OutOperand<PREC, KK, NB>& B, const void* in_frag,
+                                               const float* __restrict__ in_pot, const float* __restrict__ in_norm,
+                                               int64_t n_in, int rb_begin, int rb_end, float inv2s, float c,
+                                               int tid, float (&mx)[NB], float (&sm)[NB], u32x4* s_tile,
+                                               float* s_bias) {
+  constexpr int W = 8;
+  constexpr int CH = FragChunks<PREC, KK>::value;
+  constexpr int NT = W * 64;
+  constexpr int TILE_U4 = MT * CH * 64;
+  constexpr int PER_T = (TILE_U4 + NT - 1) / NT;
+  constexpr int ROWS = MT * 16;
+  static_assert((ROWS & (ROWS - 1)) == 0 && ROWS <= NT, "MT * 16 rows must be a power of two");
+  const int lane = tid & 63;
+  const int group = tid >> 8;  // waves 0-3 / 4-7
+  const u32x4* __restrict__ src = static_cast<const u32x4*>(in_frag);
+  const int T = (rb_end - rb_begin) / MT;  // tiles of this split
+  const int rb_last = rb_end - MT;
+  u32x4 pre[PER_T];
+  float2 pb;
+  auto stage = [&](int t) {
+    u32x4* L = s_tile + (t & 1) * TILE_U4;
+#pragma unroll
+    for (int q = 0; q < PER_T; ++q)
+      if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L[tid + q * NT] = pre[q];
+    if (tid < ROWS) s_bias[(t & 1) * ROWS + tid] = row_bias(pb, static_cast<int64_t>(rb_begin + t * MT) * 16 + tid, n_in, inv2s);
+  };
+  tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb_begin, tid, pre, pb);
+  stage(0);
+  tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb_begin + MT, rb_last), tid, pre,
+                                              pb);
+  __syncthreads();
+  f32x4 acc[MT][NB];
+  for (int k = 0; k <= 2 * T; ++k) {
+    if (k & 1) {
+      // stage tile (k + 1) / 2, prefetch the one after it
+      const int tn = (k >> 1) + 1;
+      if (tn < T) {
+        stage(tn);
+        tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb_begin + (tn + 1) * MT, rb_last),
+                                                    tid, pre, pb);
+      }
+    }
+    const int kg = k - group;
+    if (kg >= 0) {
+      const int t = kg >> 1;
+      if (t < T) {
+        if ((kg & 1) == 0) {
+          __builtin_amdgcn_s_setprio(1);
+          tile_mfma<PREC, KK, NB, MT>(B, s_tile + (t & 1) * TILE_U4, s_bias + (t & 1) * ROWS, lane, acc);
+          __builtin_amdgcn_s_setprio(0);
+        } else {
+          tile_exp<NB, MT, SHIFT>(acc, c, mx, sm);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Combine the four 16-lane row groups of each column; lanes < 16 store the
 // column's (max, sum) to s_ms[(wave * NB + nb) * 16 + lane].
 template <int NB>
@@ -530,8 +599,12 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
     __syncthreads();
   }
   if (shifted) {
-    stream_rows<PREC, KK, NB, MT, W, true>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s, c,
-                                           tid, mx, sm, s_tile, s_bias);
+    if constexpr (W == 8)
+      stream_rows_pp<PREC, KK, NB, MT, true>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s,
+                                             c, tid, mx, sm, s_tile, s_bias);
+    else
+      stream_rows<PREC, KK, NB, MT, W, true>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s,
+                                             c, tid, mx, sm, s_tile, s_bias);
     combine_lanes<NB>(mx, sm, c, lane, wave, s_ms);
     if (tid == 0) s_last = 1;
     __syncthreads();
@@ -550,8 +623,12 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
       mx[nb] = -INFINITY;
       sm[nb] = 0.f;
     }
-    stream_rows<PREC, KK, NB, MT, W, false>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s,
-                                            c, tid, mx, sm, s_tile, s_bias);
+    if constexpr (W == 8)
+      stream_rows_pp<PREC, KK, NB, MT, false>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s,
+                                              c, tid, mx, sm, s_tile, s_bias);
+    else
+      stream_rows<PREC, KK, NB, MT, W, false>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s,
+                                              c, tid, mx, sm, s_tile, s_bias);
     combine_lanes<NB>(mx, sm, c, lane, wave, s_ms);
   }
   __syncthreads();

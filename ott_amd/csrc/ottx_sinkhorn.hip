@@ -671,6 +671,7 @@ int kernel_variants(int prec, int d, KernelCfg* out, int max_out) {
       case 2:
         add(make_cfg<1, 2, 4, 4, 4>()); add(make_cfg<1, 2, 4, 2, 4>());
         add(make_cfg<1, 2, 2, 4, 4>()); add(make_cfg<1, 2, 8, 2, 4>());
+        add(make_cfg<1, 2, 4, 2, 8>()); add(make_cfg<1, 2, 2, 4, 8>()); add(make_cfg<1, 2, 4, 4, 8>());
         break;
       case 3: add(make_cfg<1, 3, 2, 4, 4>()); add(make_cfg<1, 3, 2, 2, 4>()); add(make_cfg<1, 3, 1, 4, 4>()); break;
       case 4: add(make_cfg<1, 4, 2, 4, 4>()); add(make_cfg<1, 4, 2, 2, 4>()); add(make_cfg<1, 4, 1, 4, 4>()); break;
@@ -696,6 +697,7 @@ int kernel_variants(int prec, int d, KernelCfg* out, int max_out) {
       add(make_cfg<0, 12, 4, 4, 4>()); add(make_cfg<0, 12, 4, 2, 4>()); add(make_cfg<0, 12, 2, 4, 4>());
     } else if (ks <= 16) {
       add(make_cfg<0, 16, 4, 4, 4>()); add(make_cfg<0, 16, 4, 2, 4>()); add(make_cfg<0, 16, 2, 4, 4>());
+      add(make_cfg<0, 16, 2, 4, 8>()); add(make_cfg<0, 16, 4, 2, 8>());
     } else if (ks <= 24) {
       add(make_cfg<0, 24, 2, 4, 4>()); add(make_cfg<0, 24, 2, 2, 4>()); add(make_cfg<0, 24, 1, 4, 4>());
     } else if (ks <= 32) {
