@@ -225,6 +225,10 @@ struct PrepInit {
   int zero_init = 0;  // bit 0: f, bit 1: g start as DefaultInitializer's (0, -inf at zero weight)
   float* f = nullptr;
   float* g = nullptr;
+  // fixed shift of the first f-update estimated from the given f (or null)
+  float* lse_f = nullptr;
+  double tau_a = 1.0;
+  const SolveScalars* scal = nullptr;
 };
 
 // Potentials the solve initialises itself for an init mode (PrepInit::zero_init bits).
@@ -290,6 +294,31 @@ struct PrepArgs {
   PrepInit init;
 };
 
+// s |p - mu|^2 of one point in float64: four independent sums, 16-byte loads
+// when the rows allow them.
+__device__ __forceinline__ float point_norm(const float* __restrict__ pt, bool vec16, const float* __restrict__ mu,
+                                            int d, double s) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int k = 0;
+  if (vec16) {
+    for (; k < d; k += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(pt + k);
+      const float4 u = *reinterpret_cast<const float4*>(mu + k);
+      const double d0 = static_cast<double>(v.x - u.x), d1 = static_cast<double>(v.y - u.y);
+      const double d2 = static_cast<double>(v.z - u.z), d3 = static_cast<double>(v.w - u.w);
+      a0 += d0 * d0;
+      a1 += d1 * d1;
+      a2 += d2 * d2;
+      a3 += d3 * d3;
+    }
+  }
+  for (; k < d; ++k) {
+    const double v = static_cast<double>(pt[k] - mu[k]);
+    a0 += v * v;
+  }
+  return static_cast<float>(s * ((a0 + a1) + (a2 + a3)));
+}
+
 __global__ void prep_pack_kernel(const PrepArgs p) {
   const int bid = blockIdx.x;
   const int t = threadIdx.x;
@@ -311,27 +340,8 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
       norm[local] = 0.f;
       return;
     }
-    const float* pt = p.pts[side] + local * p.d;
-    // four independent sums; 16-byte loads when the rows allow them
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int k = 0;
-    if ((p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[side]) & 15) == 0) {
-      for (; k < p.d; k += 4) {
-        const float4 v = *reinterpret_cast<const float4*>(pt + k);
-        const float4 u = *reinterpret_cast<const float4*>(p.mu + k);
-        const double d0 = static_cast<double>(v.x - u.x), d1 = static_cast<double>(v.y - u.y);
-        const double d2 = static_cast<double>(v.z - u.z), d3 = static_cast<double>(v.w - u.w);
-        a0 += d0 * d0;
-        a1 += d1 * d1;
-        a2 += d2 * d2;
-        a3 += d3 * d3;
-      }
-    }
-    for (; k < p.d; ++k) {
-      const double v = static_cast<double>(pt[k] - p.mu[k]);
-      a0 += v * v;
-    }
-    norm[local] = static_cast<float>(p.s * ((a0 + a1) + (a2 + a3)));
+    const bool vec16 = (p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[side]) & 15) == 0;
+    norm[local] = point_norm(p.pts[side] + local * p.d, vec16, p.mu, p.d, p.s);
   } else if (segi < 6) {  // uniform weights
     const int side = segi - 4;
     if (local < p.np[side]) p.wfill[side][local] = static_cast<float>(1.0 / static_cast<double>(p.np[side]));
@@ -345,13 +355,33 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
       st->n_blocks = 0;
       st->finalized = 0;
       st->tiles_done = 0u;
-      st->shift_ok[0] = 0;
+      st->shift_ok[0] = in.lse_f != nullptr ? 1 : 0;
       st->shift_ok[1] = 0;
       st->fin_count = 0u;
       st->total_mass = 0.0;
       st->last_err = 0.0;
     }
     for (int64_t k = local; k < in.outer; k += stride) in.errors[k] = -1.f;
+    if (in.lse_f != nullptr) {
+      // The first f-update's fixed shift from the given f: the column
+      // log-sum-exp (in the kernel's units) that the f-update's epilogue
+      // would map back to f itself, L = (eps log a - f / tau_a + s|x - mu|^2) / (2 s).
+      // A warm start puts the first update's result close to it; rows past n
+      // and non-finite values get -inf, so their tiles use the running max,
+      // and a tile whose sums leave range falls back to it as well.
+      const double eps = in.scal->eps;
+      const bool vec16 = (p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[0]) & 15) == 0;
+      for (int64_t k = local; k < p.np_pad[0]; k += stride) {
+        float L = -INFINITY;
+        if (k < p.np[0]) {
+          const double w = p.wuser[0] ? static_cast<double>(p.wuser[0][k]) : 1.0 / static_cast<double>(p.np[0]);
+          const double nrm = point_norm(p.pts[0] + k * p.d, vec16, p.mu, p.d, p.s);
+          const double v = (eps * log(w) - static_cast<double>(in.f[k]) / in.tau_a + nrm) / (2.0 * p.s);
+          if (isfinite(v)) L = static_cast<float>(v);
+        }
+        in.lse_f[k] = L;
+      }
+    }
     for (int64_t k = local; k < in.n_counters; k += stride) in.counters[k] = 0u;
     // DefaultInitializer: 0, -inf where the weight is 0
     if (in.zero_init & 1)
@@ -972,6 +1002,7 @@ int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* w
   pa.seg[6] = cur;
   if (init != nullptr && init->st != nullptr) {
     pa.init = *init;
+    pa.init.scal = P->scal;
     const int64_t span = std::max<int64_t>(std::max<int64_t>(pb->n, pb->m),
                                            std::max<int64_t>(init->outer, init->n_counters));
     pa.blocks_init = static_cast<int>(std::min<int64_t>(256, blocks(span)));
@@ -1226,6 +1257,11 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   pinit.zero_init = zero_init_bits(o->init_mode);
   pinit.f = out->f;
   pinit.g = out->g;
+  // a given f (warm start or restart) estimates the first f-update's shift
+  if (o->init_mode == OTTX_INIT_GIVEN || o->init_mode == OTTX_INIT_F_GIVEN) {
+    pinit.lse_f = reinterpret_cast<float*>(base + L.lse_x);
+    pinit.tau_a = pb->tau_a;
+  }
   Prepared P;
   rc = prepare(pb, k, L, workspace, st, &P, &pinit);
   if (rc != OTTX_OK) return rc;
