@@ -52,6 +52,7 @@ struct SolveMail {
   int32_t n_blocks;
   uint32_t ready;
   uint64_t progress;   // (seq << 32) | blocks completed, written by every block's finalizer
+  double last_err;     // error of the last completed block, written just before progress
 };
 
 // Scalars resolved on the device at solve start (relative epsilon).

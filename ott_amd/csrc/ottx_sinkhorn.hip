@@ -533,9 +533,12 @@ __global__ void solve_finalize_kernel(const float* f, const float* __restrict__ 
   __shared__ double cen[2][kDualBlocks];
   __shared__ int s_last;
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-    // progress of the loop for the host, which keeps one block queued ahead
-    const uint64_t v = (static_cast<uint64_t>(seq) << 32) | static_cast<uint32_t>(st->block);
-    __hip_atomic_store(&mail->progress, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    // progress of the loop (and the last error) for the host, which keeps
+    // at most one block queued ahead
+    __hip_atomic_store(&mail->last_err, st->last_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const uint32_t done_bit = st->done ? 0x80000000u : 0u;  // the loop has stopped: results follow
+    const uint64_t v = (static_cast<uint64_t>(seq) << 32) | done_bit | static_cast<uint32_t>(st->block);
+    __hip_atomic_store(&mail->progress, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   if (!st->done || st->finalized) return;
   const int tid = threadIdx.x;
@@ -1327,10 +1330,13 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     return OTTX_OK;
   };
   auto mail_ready = [&]() { return __atomic_load_n(&mail->ready, __ATOMIC_ACQUIRE) == seq; };
-  // blocks completed by this solve, as the finalizers report them
-  auto blocks_done = [&]() -> int64_t {
+  // blocks completed by this solve, as the finalizers report them, and
+  // whether the loop has stopped (the results then follow in the mailbox)
+  auto progress = [&](bool* stopped) -> int64_t {
     const uint64_t v = __atomic_load_n(&mail->progress, __ATOMIC_ACQUIRE);
-    return (v >> 32) == seq ? static_cast<int64_t>(v & 0xffffffffu) : -1;
+    if ((v >> 32) != seq) return -1;
+    *stopped = (v & 0x80000000u) != 0;
+    return static_cast<int64_t>(v & 0x7fffffffu);
   };
   if (total_blocks > 0) {
     rc = launch_halfstep(k, g_plain, st);  // g-update of iteration 0
@@ -1363,6 +1369,40 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
       }
     }
     int status = OTTX_OK;
+    // Waits until `blocks` blocks have finished (or the loop has stopped),
+    // spinning on the pinned mailbox that every block's finalizer updates (no
+    // event records in the stream, whose system-scope release would put a gap
+    // between blocks, and no blocking wait with its wake-up latency).  The
+    // stream itself is queried now and then, so an error or a drained stream
+    // ends the wait.  Returns true when the loop has stopped.
+    auto wait_blocks = [&](int blocks) -> bool {
+      for (uint32_t spin = 1;; ++spin) {
+        if (mail_ready()) return true;
+        bool stopped = false;
+        if (progress(&stopped) >= blocks) {
+          if (!stopped) return mail_ready();
+          // the loop has stopped: its finalizer is writing the results
+          while (!mail_ready()) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q != hipSuccess && q != hipErrorNotReady) {
+              status = OTTX_EHIP;
+              return true;
+            }
+            if (q == hipSuccess) return true;
+          }
+          return true;
+        }
+        if ((spin & 1023u) == 0) {
+          const hipError_t q = hipStreamQuery(st);
+          if (q == hipSuccess) return mail_ready();
+          if (q != hipErrorNotReady) {
+            status = OTTX_EHIP;
+            return true;
+          }
+        }
+      }
+    };
+    double err_prev = -1.0, err_prev2 = -1.0;
     for (int blk = 0; blk < total_blocks; ++blk) {
       if (exec) {
         if (hipGraphLaunch(exec, st) != hipSuccess) { status = OTTX_EHIP; break; }
@@ -1370,24 +1410,19 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
         status = launch_all(st);
         if (status != OTTX_OK) break;
       }
-      // One block runs ahead: wait until the previous block has finished,
-      // spinning on the pinned mailbox that every block's finalizer updates
-      // (no event records in the stream, whose system-scope release would put
-      // a gap between blocks, and no blocking wait with its wake-up latency).
-      // The stream itself is queried now and then, so an error or a drained
-      // stream ends the wait.
-      if (blk >= 1) {
-        bool stop = false;
-        for (uint32_t spin = 1;; ++spin) {
-          if (mail_ready()) { stop = true; break; }
-          if (blocks_done() >= blk) break;
-          if ((spin & 1023u) == 0) {
-            const hipError_t q = hipStreamQuery(st);
-            if (q == hipSuccess) break;
-            if (q != hipErrorNotReady) { status = OTTX_EHIP; break; }
-          }
-        }
-        if (status != OTTX_OK || stop || mail_ready()) break;
+      if (blk == 0) continue;
+      // One block runs ahead: wait for the previous block.
+      if (wait_blocks(blk)) break;
+      // Its error predicts the running block's (a geometric decrease, or a
+      // sixteenfold one from a single error).  When that is below the threshold
+      // the running block is likely the last: wait for it instead of queuing
+      // a speculative block that would only exit early.
+      err_prev2 = err_prev;
+      __atomic_load(&mail->last_err, &err_prev, __ATOMIC_ACQUIRE);
+      if (!force_scan && blk + 1 < total_blocks && blk + 1 >= (o->min_iterations + inner - 1) / inner &&
+          std::isfinite(err_prev) && err_prev >= 0.0) {
+        const double pred = (err_prev2 > 0.0) ? err_prev * (err_prev / err_prev2) : err_prev / 16.0;
+        if (pred < o->threshold && wait_blocks(blk + 1)) break;
       }
     }
     if (exec) (void)hipGraphExecDestroy(exec);
