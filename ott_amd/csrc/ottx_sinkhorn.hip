@@ -228,21 +228,168 @@ struct PrepInit {
   // fixed shift of the first f-update estimated from the given f (or null)
   float* lse_f = nullptr;
   double tau_a = 1.0;
+  // fixed shift of the first g-update (or null): kShiftFromPot estimates it
+  // from the given g, kShiftProbe from a row sample with the given f
+  float* lse_g = nullptr;
+  int lse_g_mode = 0;
+  double tau_b = 1.0;
   const SolveScalars* scal = nullptr;
 };
+
+constexpr int kShiftFromPot = 1;
+constexpr int kShiftProbe = 2;
+// Rows of x sampled by the first g-update's shift probe.
+constexpr int kProbeRows = 64;
 
 // Potentials the solve initialises itself for an init mode (PrepInit::zero_init bits).
 inline int zero_init_bits(int init_mode) {
   return init_mode == OTTX_INIT_ZEROS ? 3 : init_mode == OTTX_INIT_F_GIVEN ? 2 : 0;
 }
 
+// Row r < K of the shift probe's sample: evenly strided over the n rows.
+__device__ __forceinline__ int64_t probe_row(int r, int64_t n, int K) { return (static_cast<int64_t>(r) * n) / K; }
+
+constexpr int kProbeCols = 64;
+constexpr int kProbeChunk = 64;
+
+// First g-update's fixed shift from a row sample, for a solve that starts
+// from a given f and no g.  For column j this stores the largest of
+//   <x_r - x_0, y_j - x_0> + f_r / (2 s) - |x_r - x_0|^2 / 2
+// over the K = min(64, n) evenly strided rows r: the kernel's acc_rj with the
+// points centred on x_0 instead of mu, which differs from it by a term of j
+// alone; prep_pack_kernel adds that term and half of log2(n) / c, since the
+// log-sum-exp over all rows lies between the max and the max + log2(n) / c.
+// Needing neither mu nor eps, the probe runs beside the statistics.  The value
+// only sets the streaming shift of the first g-update: a tile whose sums
+// leave range falls back to the running max.
+//
+// A workgroup of four waves takes 64 columns, one per lane, and wave w the
+// sampled rows w, w + 4, ...  The probe is short, so it is bound by memory
+// latency: each chunk of 64 features takes one round trip, with the sampled
+// rows, x_0 and f staged in LDS and the lane's column in registers, every load
+// issued before the first wait.  The waves merge their maxima through LDS.
+__device__ void probe_max_block(int blk, const float* __restrict__ x, int64_t n, const float* __restrict__ y,
+                                int64_t m, int64_t m_pad, int d, const float* __restrict__ f, double s,
+                                float* __restrict__ out) {
+  constexpr int R = kProbeRows / 4;
+  constexpr int KC = kProbeChunk;
+  __shared__ float4 s_x[kProbeRows][KC / 4];
+  __shared__ float4 s_c[KC / 4];
+  __shared__ float s_f[kProbeRows], s_nrm[kProbeRows];
+  __shared__ float s_mx[4][kProbeCols];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int K = static_cast<int>(min<int64_t>(kProbeRows, n));
+  const int64_t col0 = static_cast<int64_t>(blk) * kProbeCols;
+  const float* yj = y + min(col0 + lane, m - 1) * d;
+  const bool vec16 = (d & 3) == 0 && (reinterpret_cast<uintptr_t>(y) & 15) == 0;
+  if (tid < kProbeRows) {
+    s_f[tid] = tid < K ? f[probe_row(tid, n, K)] : 0.f;
+    s_nrm[tid] = 0.f;
+  }
+  float acc[R], yc = 0.f;
+#pragma unroll
+  for (int t = 0; t < R; ++t) acc[t] = 0.f;
+  for (int k0 = 0; k0 < d; k0 += KC) {
+    const int kc = min(KC, d - k0);
+    // the lane's column, the chunk of the sampled rows and of x_0, zero past d
+    float yv[KC];
+    if (vec16) {
+#pragma unroll
+      for (int k4 = 0; k4 < KC / 4; ++k4) {
+        const float4 v = 4 * k4 < kc ? *reinterpret_cast<const float4*>(yj + k0 + 4 * k4) : make_float4(0.f, 0.f, 0.f, 0.f);
+        yv[4 * k4] = v.x;
+        yv[4 * k4 + 1] = v.y;
+        yv[4 * k4 + 2] = v.z;
+        yv[4 * k4 + 3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < KC; ++k) yv[k] = k < kc ? yj[k0 + k] : 0.f;
+    }
+    float xs[kProbeRows * KC / 256];
+#pragma unroll
+    for (int i = 0; i < kProbeRows * KC / 256; ++i) {
+      const int e = tid + 256 * i, r = e / KC, k = e % KC;
+      xs[i] = (r < K && k < kc) ? x[probe_row(r, n, K) * d + k0 + k] : 0.f;
+    }
+    const float ck = (tid < KC && tid < kc) ? x[k0 + tid] : 0.f;
+    __syncthreads();  // the previous chunk's readers are done
+    float* sx = reinterpret_cast<float*>(s_x);
+#pragma unroll
+    for (int i = 0; i < kProbeRows * KC / 256; ++i) sx[tid + 256 * i] = xs[i];
+    if (tid < KC) reinterpret_cast<float*>(s_c)[tid] = ck;
+    __syncthreads();
+    // |x_r - x_0|^2 over the chunk: four threads a row
+    {
+      const int r = tid >> 2, q = tid & 3;
+      float nr = 0.f;
+#pragma unroll
+      for (int i = 0; i < KC / 16; ++i) {
+        const float4 v = s_x[r][q * (KC / 16) + i], u = s_c[q * (KC / 16) + i];
+        const float a = v.x - u.x, b = v.y - u.y, c = v.z - u.z, e = v.w - u.w;
+        nr += a * a + b * b + c * c + e * e;
+      }
+      nr += __shfl_xor(nr, 1);
+      nr += __shfl_xor(nr, 2);
+      if (q == 0) s_nrm[r] += nr;
+    }
+    // <x_r, y_j - x_0> for the wave's rows, and <x_0, y_j - x_0>
+#pragma unroll
+    for (int k4 = 0; k4 < KC / 4; ++k4) {
+      const float4 u = s_c[k4];
+      yv[4 * k4 + 0] -= u.x;
+      yv[4 * k4 + 1] -= u.y;
+      yv[4 * k4 + 2] -= u.z;
+      yv[4 * k4 + 3] -= u.w;
+      yc += u.x * yv[4 * k4] + u.y * yv[4 * k4 + 1] + u.z * yv[4 * k4 + 2] + u.w * yv[4 * k4 + 3];
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      const int r = w + 4 * t;
+      float a = 0.f;
+#pragma unroll
+      for (int k4 = 0; k4 < KC / 4; ++k4) {
+        const float4 xv = s_x[r][k4];
+        a += xv.x * yv[4 * k4] + xv.y * yv[4 * k4 + 1] + xv.z * yv[4 * k4 + 2] + xv.w * yv[4 * k4 + 3];
+      }
+      acc[t] += a;
+    }
+  }
+  __syncthreads();
+  const float inv2s = static_cast<float>(0.5 / s);
+  float M = -INFINITY;
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int r = w + 4 * t;
+    if (r >= K) continue;
+    const float a = acc[t] - yc + s_f[r] * inv2s - 0.5f * s_nrm[r];
+    if (isfinite(a)) M = fmaxf(M, a);
+  }
+  s_mx[w][lane] = M;
+  __syncthreads();
+  if (tid < kProbeCols) {
+    const int64_t j = col0 + tid;
+    if (j >= m_pad) return;
+    M = fmaxf(fmaxf(s_mx[0][tid], s_mx[1][tid]), fmaxf(s_mx[2][tid], s_mx[3][tid]));
+    out[j] = j < m ? M : -INFINITY;
+  }
+}
+
 // Statistics partials of both clouds in one launch: blocks [0, cx) take x,
 // blocks [cx, cx + cy) take y (same per-chunk sums as stats_partial_kernel).
-__global__ void prep_stats_kernel(const float* __restrict__ x, int64_t n, int64_t rows_x, int cx,
-                                  const float* __restrict__ y, int64_t m, int64_t rows_y, int d,
-                                  double* __restrict__ part_x, double* __restrict__ part_y) {
+// With ``probe`` set, the blocks past them run probe_max_block with the
+// given f.
+__global__ __launch_bounds__(256) void prep_stats_kernel(const float* __restrict__ x, int64_t n, int64_t rows_x, int cx,
+                                  const float* __restrict__ y, int64_t m, int64_t rows_y, int cy, int d,
+                                  double* __restrict__ part_x, double* __restrict__ part_y,
+                                  const float* __restrict__ f, double s, int64_t m_pad, float* __restrict__ probe) {
   __shared__ double s1[256], s2[256];
   const int t = threadIdx.x;
+  if (static_cast<int>(blockIdx.x) >= cx + cy) {
+    probe_max_block(static_cast<int>(blockIdx.x) - cx - cy, x, n, y, m, m_pad, d, f, s, probe);
+    return;
+  }
   const bool side_y = static_cast<int>(blockIdx.x) >= cx;
   const int chunk = side_y ? static_cast<int>(blockIdx.x) - cx : static_cast<int>(blockIdx.x);
   const float* pts = side_y ? y : x;
@@ -342,6 +489,36 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
     }
     const bool vec16 = (p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[side]) & 15) == 0;
     norm[local] = point_norm(p.pts[side] + local * p.d, vec16, p.mu, p.d, p.s);
+    if (side == 1 && p.init.lse_g != nullptr && p.init.lse_g_mode == kShiftProbe) {
+      // the probe's max, centred on x_0, to the kernel's centring on mu:
+      // + <mu - x_0, (mu + x_0) / 2 - y_j>, plus half of log2(n) / c
+      const float M = p.init.lse_g[local];
+      if (isfinite(M)) {
+        const float* yj = p.pts[1] + local * p.d;
+        const float* x0 = p.pts[0];
+        double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+        int k = 0;
+        if (vec16 && (reinterpret_cast<uintptr_t>(x0) & 15) == 0) {
+          for (; k < p.d; k += 4) {
+            const float4 v = *reinterpret_cast<const float4*>(yj + k);
+            const float4 u = *reinterpret_cast<const float4*>(p.mu + k);
+            const float4 z = *reinterpret_cast<const float4*>(x0 + k);
+            c0 += (static_cast<double>(u.x) - z.x) * (0.5 * (static_cast<double>(u.x) + z.x) - v.x);
+            c1 += (static_cast<double>(u.y) - z.y) * (0.5 * (static_cast<double>(u.y) + z.y) - v.y);
+            c2 += (static_cast<double>(u.z) - z.z) * (0.5 * (static_cast<double>(u.z) + z.z) - v.z);
+            c3 += (static_cast<double>(u.w) - z.w) * (0.5 * (static_cast<double>(u.w) + z.w) - v.w);
+          }
+        }
+        for (; k < p.d; ++k) {
+          const double u = p.mu[k], z = x0[k];
+          c0 += (u - z) * (0.5 * (u + z) - static_cast<double>(yj[k]));
+        }
+        const double corr = (c0 + c1) + (c2 + c3);
+        const double L = static_cast<double>(M) + corr +
+                         0.5 * log2(static_cast<double>(p.np[0])) / static_cast<double>(p.init.scal->c);
+        p.init.lse_g[local] = isfinite(L) ? static_cast<float>(L) : -INFINITY;
+      }
+    }
   } else if (segi < 6) {  // uniform weights
     const int side = segi - 4;
     if (local < p.np[side]) p.wfill[side][local] = static_cast<float>(1.0 / static_cast<double>(p.np[side]));
@@ -356,7 +533,7 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
       st->finalized = 0;
       st->tiles_done = 0u;
       st->shift_ok[0] = in.lse_f != nullptr ? 1 : 0;
-      st->shift_ok[1] = 0;
+      st->shift_ok[1] = in.lse_g != nullptr ? 1 : 0;
       st->fin_count = 0u;
       st->total_mass = 0.0;
       st->last_err = 0.0;
@@ -380,6 +557,21 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
           if (isfinite(v)) L = static_cast<float>(v);
         }
         in.lse_f[k] = L;
+      }
+    }
+    if (in.lse_g != nullptr && in.lse_g_mode == kShiftFromPot) {
+      // the same estimate for the first g-update from the given g
+      const double eps = in.scal->eps;
+      const bool vec16 = (p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[1]) & 15) == 0;
+      for (int64_t k = local; k < p.np_pad[1]; k += stride) {
+        float L = -INFINITY;
+        if (k < p.np[1]) {
+          const double w = p.wuser[1] ? static_cast<double>(p.wuser[1][k]) : 1.0 / static_cast<double>(p.np[1]);
+          const double nrm = point_norm(p.pts[1] + k * p.d, vec16, p.mu, p.d, p.s);
+          const double v = (eps * log(w) - static_cast<double>(in.g[k]) / in.tau_b + nrm) / (2.0 * p.s);
+          if (isfinite(v)) L = static_cast<float>(v);
+        }
+        in.lse_g[k] = L;
       }
     }
     for (int64_t k = local; k < in.n_counters; k += stride) in.counters[k] = 0u;
@@ -960,8 +1152,14 @@ int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* w
   P->b = pb->b ? pb->b : reinterpret_cast<const float*>(base + L.w_y);
   double* sx = reinterpret_cast<double*>(base + L.stats_x);
   double* sy = reinterpret_cast<double*>(base + L.stats_y);
-  hipLaunchKernelGGL(prep_stats_kernel, dim3(L.chunks_x + L.chunks_y), dim3(256), 0, st, pb->x, pb->n,
-                     L.rows_chunk_x, L.chunks_x, pb->y, pb->m, L.rows_chunk_y, pb->d, sx, sy);
+  // the first g-update's shift probe runs beside the statistics
+  const bool probe = init != nullptr && init->st != nullptr && init->lse_g != nullptr &&
+                     init->lse_g_mode == kShiftProbe;
+  const int64_t m_pad = pad_rows(pb->m);
+  const int probe_blocks = probe ? static_cast<int>((m_pad + kProbeCols - 1) / kProbeCols) : 0;
+  hipLaunchKernelGGL(prep_stats_kernel, dim3(L.chunks_x + L.chunks_y + probe_blocks), dim3(256), 0, st, pb->x,
+                     pb->n, L.rows_chunk_x, L.chunks_x, pb->y, pb->m, L.rows_chunk_y, L.chunks_y, pb->d, sx, sy,
+                     probe ? init->f : nullptr, pb->inv_scale_cost, m_pad, probe ? init->lse_g : nullptr);
   hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, st, sx, L.chunks_x, sy, L.chunks_y, pb->d,
                      pb->n, pb->m, pb->inv_scale_cost, P->stats_out, P->mu, P->scal, pb->relative_epsilon,
                      pb->epsilon, pb->tau_a, pb->tau_b);
@@ -1260,10 +1458,15 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   pinit.zero_init = zero_init_bits(o->init_mode);
   pinit.f = out->f;
   pinit.g = out->g;
-  // a given f (warm start or restart) estimates the first f-update's shift
+  // a given f (warm start or restart) estimates the first f-update's shift;
+  // the first g-update's comes from the given g, or from a row sample of the
+  // log-sum-exp with the given f when g is not given
   if (o->init_mode == OTTX_INIT_GIVEN || o->init_mode == OTTX_INIT_F_GIVEN) {
     pinit.lse_f = reinterpret_cast<float*>(base + L.lse_x);
     pinit.tau_a = pb->tau_a;
+    pinit.lse_g = reinterpret_cast<float*>(base + L.lse_y);
+    pinit.lse_g_mode = o->init_mode == OTTX_INIT_GIVEN ? kShiftFromPot : kShiftProbe;
+    pinit.tau_b = pb->tau_b;
   }
   Prepared P;
   rc = prepare(pb, k, L, workspace, st, &P, &pinit);
