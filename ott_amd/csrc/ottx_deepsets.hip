@@ -183,32 +183,40 @@ __global__ void stats_partial_kernel(const float* __restrict__ x, const float* _
   const int k = t % d, gi = t / d;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (t < G * d) {
+    // the chunk's rows of both clouds, U rows of each per trip: the sums are
+    // short, so they are bound by latency, and 2 U loads are in flight
+    constexpr int U = 8;
+    int lo[2], hi[2];
+#pragma unroll
     for (int side = 0; side < 2; ++side) {
       const int* off = side ? yoff : xoff;
-      const float* p = side ? y : x;
       const int r0 = off[b], r1 = off[b + 1];
       const int per = (r1 - r0 + kStatChunks - 1) / kStatChunks;
-      const int lo = r0 + c * per, hi = min(r1, lo + per);
-      // four rows per trip: independent loads in flight
-      double s1 = 0.0, s2 = 0.0, t1 = 0.0, t2 = 0.0;
-      int r = lo + gi;
-      for (; r + 3 * G < hi; r += 4 * G) {
-        const double v0 = p[static_cast<int64_t>(r) * d + k];
-        const double v1 = p[static_cast<int64_t>(r + G) * d + k];
-        const double v2 = p[static_cast<int64_t>(r + 2 * G) * d + k];
-        const double v3 = p[static_cast<int64_t>(r + 3 * G) * d + k];
-        s1 += v0 + v1;
-        s2 += v2 + v3;
-        t1 += v0 * v0 + v1 * v1;
-        t2 += v2 * v2 + v3 * v3;
+      lo[side] = r0 + c * per + gi;
+      hi[side] = min(r1, r0 + c * per + per);
+    }
+    for (int i = 0; lo[0] + i * G < hi[0] || lo[1] + i * G < hi[1]; i += U) {
+      float v[2][U];
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        const float* p = side ? y : x;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          // rows past the chunk load its last row and are zeroed after the load,
+          // so every load is unconditional (hi > lo when the chunk has rows)
+          const int r = lo[side] + (i + u) * G;
+          const float w = p[static_cast<int64_t>(min(r, max(hi[side] - 1, 0))) * d + k];
+          v[side][u] = r < hi[side] ? w : 0.f;
+        }
       }
-      for (; r < hi; r += G) {
-        const double v = p[static_cast<int64_t>(r) * d + k];
-        s1 += v;
-        t1 += v * v;
-      }
-      acc[2 * side] = s1 + s2;
-      acc[2 * side + 1] = t1 + t2;
+#pragma unroll
+      for (int side = 0; side < 2; ++side)
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const double w = v[side][u];
+          acc[2 * side] += w;
+          acc[2 * side + 1] += w * w;
+        }
     }
   }
   for (int q = 0; q < 4; ++q) fs[q][t] = acc[q];
@@ -230,57 +238,55 @@ __host__ __device__ inline int ctx_width(int d) { return 512 + (3 + d + 31) / 32
 // mean(C) = var x + var y + |mean x - mean y|^2 (var = mean |p - mean p|^2), and
 // the moments mom_b = [s_x - 1, s_y - 1, mean y~ (d)] with
 // s_x = (var x + |mean x - mean y|^2 / 4) / sigma^2 (s_y alike) and
-// mean y~ = (mean y - mean x) / (2 sigma).  Grid B; thread (feature k, group g)
-// sums the chunks c = g mod G, then the groups and features are reduced.
-__global__ void stats_final_kernel(const double* __restrict__ part, const int* __restrict__ xoff,
-                                   const int* __restrict__ yoff, int d, float* __restrict__ mu,
-                                   float* __restrict__ sig2, float* __restrict__ mom) {
-  __shared__ double red[4][256];
-  __shared__ double fx[256], fy[256];
+// mean y~ = (mean y - mean x) / (2 sigma).  Grid B of kFinalThreads threads:
+// thread (value v = (q, k), group g) sums the chunks c = g mod G of one of the
+// 4 d values, all its loads in flight at once, then the groups are reduced.
+constexpr int kFinalThreads = 1024;
+
+__global__ __launch_bounds__(kFinalThreads) void stats_final_kernel(
+    const double* __restrict__ part, const int* __restrict__ xoff, const int* __restrict__ yoff, int d,
+    float* __restrict__ mu, float* __restrict__ sig2, float* __restrict__ mom) {
+  __shared__ double red[kFinalThreads];
+  __shared__ double sums[4][256];
   __shared__ double tot[3][256];
   const int b = blockIdx.x, t = threadIdx.x;
   const int nx = xoff[b + 1] - xoff[b], ny = yoff[b + 1] - yoff[b];
-  const int G = max(1, 256 / d);
-  const int k = t % d, g = t / d;
-  double q[4] = {0.0, 0.0, 0.0, 0.0};
-  if (t < G * d) {
-    // four chunks per trip: sixteen independent loads in flight per thread
-    double q2[4] = {0.0, 0.0, 0.0, 0.0}, q3[4] = {0.0, 0.0, 0.0, 0.0}, q4[4] = {0.0, 0.0, 0.0, 0.0};
-    const double* pb = part + static_cast<int64_t>(b) * kStatChunks * 4 * d + k;
-    int c = g;
-    for (; c + 3 * G < kStatChunks; c += 4 * G)
+  const int V = 4 * d;  // values per chunk, in part's order (q, k)
+  const int G = max(1, kFinalThreads / V);
+  const int v = t % V, g = t / V;
+  double a = 0.0;
+  if (t < G * V) {
+    const double* pb = part + static_cast<int64_t>(b) * kStatChunks * V + v;
+    constexpr int U = 16;
+    for (int c = g; c < kStatChunks; c += U * G) {
+      double w[U];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        q[j] += pb[(static_cast<int64_t>(c) * 4 + j) * d];
-        q2[j] += pb[(static_cast<int64_t>(c + G) * 4 + j) * d];
-        q3[j] += pb[(static_cast<int64_t>(c + 2 * G) * 4 + j) * d];
-        q4[j] += pb[(static_cast<int64_t>(c + 3 * G) * 4 + j) * d];
-      }
-    for (; c < kStatChunks; c += G)
+      for (int u = 0; u < U; ++u) w[u] = c + u * G < kStatChunks ? pb[static_cast<int64_t>(c + u * G) * V] : 0.0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) q[j] += pb[(static_cast<int64_t>(c) * 4 + j) * d];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) q[j] = (q[j] + q2[j]) + (q3[j] + q4[j]);
+      for (int u = 0; u < U; ++u) a += w[u];
+    }
   }
-#pragma unroll
-  for (int j = 0; j < 4; ++j) red[j][t] = q[j];
+  red[t] = a;
   __syncthreads();
-  double va = 0.0, vb = 0.0, dm = 0.0;
-  if (t < d) {
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int gg = 0; gg < G; ++gg)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) s[j] += red[j][gg * d + t];
-    const double mx = s[0] / nx, my = s[2] / ny;
-    fx[t] = mx;
-    fy[t] = my;
-    va = s[1] / nx - mx * mx;
-    vb = s[3] / ny - my * my;
-    dm = (mx - my) * (mx - my);
+  for (int e = t; e < V; e += kFinalThreads) {
+    double s2 = 0.0;
+    for (int gg = 0; gg < G; ++gg) s2 += red[gg * V + e];
+    sums[e / d][e % d] = s2;
   }
-  tot[0][t] = va;
-  tot[1][t] = vb;
-  tot[2][t] = dm;
+  __syncthreads();
+  double va = 0.0, vb = 0.0, dm = 0.0, mxk = 0.0, myk = 0.0;
+  if (t < d) {
+    mxk = sums[0][t] / nx;
+    myk = sums[2][t] / ny;
+    va = sums[1][t] / nx - mxk * mxk;
+    vb = sums[3][t] / ny - myk * myk;
+    dm = (mxk - myk) * (mxk - myk);
+  }
+  if (t < 256) {
+    tot[0][t] = va;
+    tot[1][t] = vb;
+    tot[2][t] = dm;
+  }
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (t < w)
@@ -292,8 +298,8 @@ __global__ void stats_final_kernel(const double* __restrict__ part, const int* _
   const double s2 = fmax(0.5 * (A + Bv + Cc), 1e-30);
   const double inv_sig = 1.0 / sqrt(s2);
   if (t < d) {
-    mu[b * d + t] = static_cast<float>(0.5 * (fx[t] + fy[t]));
-    mom[static_cast<int64_t>(b) * (d + 2) + 2 + t] = static_cast<float>(0.5 * (fy[t] - fx[t]) * inv_sig);
+    mu[b * d + t] = static_cast<float>(0.5 * (mxk + myk));
+    mom[static_cast<int64_t>(b) * (d + 2) + 2 + t] = static_cast<float>(0.5 * (myk - mxk) * inv_sig);
   }
   if (t == 0) {
     sig2[b] = static_cast<float>(s2);
@@ -302,24 +308,38 @@ __global__ void stats_final_kernel(const double* __restrict__ part, const int* _
   }
 }
 
-__global__ void normalize_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                 const int* __restrict__ src, const int* __restrict__ seg_of_row,
-                                 const int* __restrict__ prob, int64_t rows, int64_t rows_pad, int d, int kpad,
-                                 const float* __restrict__ mu, const float* __restrict__ sig2, bf16_t* __restrict__ X,
-                                 bf16_t* __restrict__ XT) {
+// Rows to normalise into bf16 network inputs: encoder rows (src / seg_of_row
+// give the point and its problem and side) or head rows (prob gives the
+// problem of target point r).
+struct NormJob {
+  const int* src;
+  const int* seg_of_row;
+  const int* prob;
+  int64_t rows, rows_pad;
+  bf16_t* X;
+  bf16_t* XT;  // transposed copy for the weight gradients, or null
+};
+
+// Both jobs in one launch: blocks [0, blocks0) take job 0, the rest job 1.
+__global__ void normalize_kernel(const float* __restrict__ x, const float* __restrict__ y, NormJob j0, NormJob j1,
+                                 int blocks0, int d, int kpad, const float* __restrict__ mu,
+                                 const float* __restrict__ sig2) {
   // one 64-lane wave per row: coalesced loads and stores, wave-reduced |p~|^2
+  const bool second = static_cast<int>(blockIdx.x) >= blocks0;
+  const NormJob J = second ? j1 : j0;
   const int lane = threadIdx.x & 63;
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (r >= rows_pad) return;
+  const int64_t blk = second ? blockIdx.x - blocks0 : blockIdx.x;
+  const int64_t r = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (r >= J.rows_pad) return;
   const float* p = nullptr;
   int b = 0;
-  if (r < rows) {
-    if (src) {
-      const int seg = seg_of_row[r];
+  if (r < J.rows) {
+    if (J.src) {
+      const int seg = J.seg_of_row[r];
       b = seg >> 1;
-      p = ((seg & 1) ? y : x) + static_cast<int64_t>(src[r]) * d;
+      p = ((seg & 1) ? y : x) + static_cast<int64_t>(J.src[r]) * d;
     } else {
-      b = prob[r];
+      b = J.prob[r];
       p = y + r * d;
     }
   }
@@ -339,8 +359,8 @@ __global__ void normalize_kernel(const float* __restrict__ x, const float* __res
       else if (k == d) v = part - 1.f;
     }
     const bf16_t h = to_bf16(v);
-    X[r * kpad + k] = h;
-    if (XT) XT[static_cast<int64_t>(k) * rows_pad + r] = h;
+    J.X[r * kpad + k] = h;
+    if (J.XT) J.XT[static_cast<int64_t>(k) * J.rows_pad + r] = h;
   }
 }
 
@@ -1051,19 +1071,19 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   hipLaunchKernelGGL(stats_partial_kernel, dim3(kStatChunks, B), dim3(256), 0, st, bt->x, bt->y, bt->x_off,
                      bt->y_off, bt->d, spart);
   float* mom = at<float>(ws, L.mom);
-  hipLaunchKernelGGL(stats_final_kernel, dim3(B), dim3(256), 0, st, spart, bt->x_off, bt->y_off, bt->d, mu, sig2,
-                     mom);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(B), dim3(kFinalThreads), 0, st, spart, bt->x_off, bt->y_off, bt->d,
+                     mu, sig2, mom);
   DS_LAUNCHED();
   bf16_t* X0 = at<bf16_t>(ws, L.X0);
   bf16_t* X0T = X0 + Pe * P.kin;
   bf16_t* Y0 = at<bf16_t>(ws, L.Y0);
   bf16_t* Y0T = Y0 + Ph * P.kin;
-  hipLaunchKernelGGL(normalize_kernel, dim3(static_cast<unsigned>((Pe + 3) / 4)), dim3(256), 0, st, bt->x, bt->y, bt->enc_src,
-                     bt->seg_of_row, static_cast<const int*>(nullptr), L.Pe, Pe, bt->d, P.kin, mu, sig2, X0,
-                     training ? X0T : nullptr);
-  hipLaunchKernelGGL(normalize_kernel, dim3(static_cast<unsigned>((Ph + 3) / 4)), dim3(256), 0, st, bt->x, bt->y,
-                     static_cast<const int*>(nullptr), static_cast<const int*>(nullptr), bt->y_prob, L.Ph, Ph, bt->d,
-                     P.kin, mu, sig2, Y0, training ? Y0T : nullptr);
+  // encoder and head inputs in one launch
+  const NormJob enc{bt->enc_src, bt->seg_of_row, nullptr, L.Pe, Pe, X0, training ? X0T : nullptr};
+  const NormJob head{nullptr, nullptr, bt->y_prob, L.Ph, Ph, Y0, training ? Y0T : nullptr};
+  const int enc_blocks = static_cast<int>((Pe + 3) / 4);
+  hipLaunchKernelGGL(normalize_kernel, dim3(static_cast<unsigned>(enc_blocks + (Ph + 3) / 4)), dim3(256), 0, st,
+                     bt->x, bt->y, enc, head, enc_blocks, bt->d, P.kin, mu, sig2);
   DS_LAUNCHED();
   int rc;
   GemmArgs g;

@@ -202,13 +202,12 @@ class Sinkhorn:
       # f too, meta_initializer.py:192-207): its prediction is a fresh tensor
       # that becomes the solve's f; g starts as the DefaultInitializer's, which
       # the first g-update discards but for its -inf entries (geometry.py:297-299).
-      # The solve is set up first so that its launches follow the network's
-      # directly; the network writes its prediction into the solve's f.
+      # The network is launched first and writes its prediction into the
+      # solve's f; the host sets the solve up while the GPU runs the network.
       f = torch.empty((n,), dtype=torch.float32, device=geom.device)
       g = torch.empty((m,), dtype=torch.float32, device=geom.device)
-      launch = _prepare(ot_prob, self, (f, g), init_mode=_lib.INIT_F_GIVEN)
       self.initializer.predict_f(ot_prob, out=f)
-      return launch()
+      return _prepare(ot_prob, self, (f, g), init_mode=_lib.INIT_F_GIVEN)()
     if init is None and type(self.initializer) is init_lib.DefaultInitializer:
       # zeros, -inf at zero weights (initializers.py:70-99), set by the solve itself
       f = torch.empty((n,), dtype=torch.float32, device=geom.device)
