@@ -19,6 +19,32 @@ namespace ottx {
 // Small kernels
 // ---------------------------------------------------------------------------
 
+// Thread t's float64 sums of feature t % d and its square over rows
+// r0 + t / d, r0 + t / d + G, ... < r1.  U rows per trip with every load
+// unconditional (rows past the chunk reload its last row and count zero):
+// the sums are short, so they are bound by latency.
+__device__ __forceinline__ void chunk_sums(const float* __restrict__ pts, int64_t r0, int64_t r1, int d, int G,
+                                           int t, double& a1, double& a2) {
+  if (r1 <= r0) return;
+  constexpr int U = 16;
+  const int k = t % d, g = t / d;
+  for (int64_t r = r0 + g; r < r1; r += static_cast<int64_t>(U) * G) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t ru = r + static_cast<int64_t>(u) * G;
+      const float w = pts[min(ru, r1 - 1) * d + k];
+      v[u] = ru < r1 ? w : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double w = v[u];
+      a1 += w;
+      a2 += w * w;
+    }
+  }
+}
+
 // Per-chunk, per-feature float64 sums of x and x^2 (deterministic two-stage
 // reduction for mean(C), pointcloud.py:170-186, and for the centring shift).
 // Thread t < G*d (G = 256 / d row groups) owns feature t % d and rows
@@ -31,14 +57,7 @@ __global__ void stats_partial_kernel(const float* __restrict__ pts, int64_t np, 
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_chunk;
   const int64_t r1 = min(np, r0 + rows_per_chunk);
   double a1 = 0.0, a2 = 0.0;
-  if (t < G * d) {
-    const int k = t % d, g = t / d;
-    for (int64_t r = r0 + g; r < r1; r += G) {
-      const double v = pts[r * d + k];
-      a1 += v;
-      a2 += v * v;
-    }
-  }
+  if (t < G * d) chunk_sums(pts, r0, r1, d, G, t, a1, a2);
   s1[t] = a1;
   s2[t] = a2;
   __syncthreads();
@@ -53,56 +72,62 @@ __global__ void stats_partial_kernel(const float* __restrict__ pts, int64_t np, 
   }
 }
 
-// One workgroup: combine the chunks, write mean(C) (out[0]), the common
+// One workgroup of kFinalThreads: combine the chunks, write mean(C) (out[0]), the common
 // centring shift mu = (mean x + mean y) / 2 (float, [d]) and, when ``sc`` is
 // given, the solve's scalars: eps = rel * mean(C) if rel > 0 else eps_abs.
-__global__ void stats_final_kernel(const double* __restrict__ px, int cx, const double* __restrict__ py,
-                                   int cy, int d, int64_t n, int64_t m, double s,
-                                   double* __restrict__ out, float* __restrict__ mu, SolveScalars* sc,
-                                   double rel, double eps_abs, double tau_a, double tau_b) {
-  __shared__ double red[4][256];
+constexpr int kFinalThreads = 1024;
+
+__global__ __launch_bounds__(kFinalThreads) void stats_final_kernel(
+    const double* __restrict__ px, int cx, const double* __restrict__ py, int cy, int d, int64_t n, int64_t m,
+    double s, double* __restrict__ out, float* __restrict__ mu, SolveScalars* sc, double rel, double eps_abs,
+    double tau_a, double tau_b) {
+  __shared__ double red[kFinalThreads];
+  __shared__ double sums[4][256];
   __shared__ double acc3[3][256];
   const int t = threadIdx.x;
-  const int G = max(1, 256 / d);
-  double sx1 = 0.0, sx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
-  if (t < G * d) {
-    const int k = t % d, g = t / d;
-    // unrolled so several independent loads are in flight at once
-#pragma unroll 8
-    for (int c = g; c < cx; c += G) {
-      sx1 += px[(static_cast<int64_t>(c) * d + k) * 2];
-      sx2 += px[(static_cast<int64_t>(c) * d + k) * 2 + 1];
-    }
-#pragma unroll 8
-    for (int c = g; c < cy; c += G) {
-      sy1 += py[(static_cast<int64_t>(c) * d + k) * 2];
-      sy2 += py[(static_cast<int64_t>(c) * d + k) * 2 + 1];
+  // thread (value v, group g): values v < 2 d are x's (feature v / 2, sum or
+  // sum of squares), the rest y's, in the partials' order; group g sums the
+  // chunks g, g + G, ... with all the loads of a trip in flight
+  const int V = 4 * d;
+  const int G = max(1, kFinalThreads / V);
+  const int v = t % V, g = t / V;
+  double a = 0.0;
+  if (t < G * V) {
+    const bool side_y = v >= 2 * d;
+    const double* p = (side_y ? py : px) + (side_y ? v - 2 * d : v);
+    const int C = side_y ? cy : cx;
+    constexpr int U = 16;
+    for (int c = g; c < C; c += U * G) {
+      double w[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) w[u] = c + u * G < C ? p[static_cast<int64_t>(c + u * G) * 2 * d] : 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) a += w[u];
     }
   }
-  red[0][t] = sx1;
-  red[1][t] = sx2;
-  red[2][t] = sy1;
-  red[3][t] = sy2;
+  red[t] = a;
+  __syncthreads();
+  for (int e = t; e < V; e += kFinalThreads) {
+    double s2 = 0.0;
+    for (int gg = 0; gg < G; ++gg) s2 += red[gg * V + e];
+    // e = side * 2 d + 2 k + j -> sums[2 side + j][k]
+    sums[2 * (e / (2 * d)) + (e & 1)][(e % (2 * d)) >> 1] = s2;
+  }
   __syncthreads();
   double nx2 = 0.0, ny2 = 0.0, dot = 0.0;
   if (t < d) {
-    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
-    for (int g = 0; g < G; ++g) {
-      a1 += red[0][g * d + t];
-      a2 += red[1][g * d + t];
-      b1 += red[2][g * d + t];
-      b2 += red[3][g * d + t];
-    }
-    const double mxk = a1 / static_cast<double>(n);
-    const double myk = b1 / static_cast<double>(m);
+    const double mxk = sums[0][t] / static_cast<double>(n);
+    const double myk = sums[2][t] / static_cast<double>(m);
     if (mu) mu[t] = static_cast<float>(0.5 * (mxk + myk));
-    nx2 = a2;
-    ny2 = b2;
+    nx2 = sums[1][t];
+    ny2 = sums[3][t];
     dot = mxk * myk;
   }
-  acc3[0][t] = nx2;
-  acc3[1][t] = ny2;
-  acc3[2][t] = dot;
+  if (t < 256) {
+    acc3[0][t] = nx2;
+    acc3[1][t] = ny2;
+    acc3[2][t] = dot;
+  }
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if (t < st)
@@ -400,14 +425,7 @@ __global__ __launch_bounds__(256) void prep_stats_kernel(const float* __restrict
   const int64_t r0 = static_cast<int64_t>(chunk) * rows;
   const int64_t r1 = min(np, r0 + rows);
   double a1 = 0.0, a2 = 0.0;
-  if (t < G * d) {
-    const int k = t % d, g = t / d;
-    for (int64_t r = r0 + g; r < r1; r += G) {
-      const double v = pts[r * d + k];
-      a1 += v;
-      a2 += v * v;
-    }
-  }
+  if (t < G * d) chunk_sums(pts, r0, r1, d, G, t, a1, a2);
   s1[t] = a1;
   s2[t] = a2;
   __syncthreads();
@@ -1160,7 +1178,7 @@ int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* w
   hipLaunchKernelGGL(prep_stats_kernel, dim3(L.chunks_x + L.chunks_y + probe_blocks), dim3(256), 0, st, pb->x,
                      pb->n, L.rows_chunk_x, L.chunks_x, pb->y, pb->m, L.rows_chunk_y, L.chunks_y, pb->d, sx, sy,
                      probe ? init->f : nullptr, pb->inv_scale_cost, m_pad, probe ? init->lse_g : nullptr);
-  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, st, sx, L.chunks_x, sy, L.chunks_y, pb->d,
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(kFinalThreads), 0, st, sx, L.chunks_x, sy, L.chunks_y, pb->d,
                      pb->n, pb->m, pb->inv_scale_cost, P->stats_out, P->mu, P->scal, pb->relative_epsilon,
                      pb->epsilon, pb->tau_a, pb->tau_b);
   OTTX_LAUNCHED();
@@ -1335,7 +1353,7 @@ int ottx_mean_cost(int64_t n, int64_t m, int32_t d, const float* x, const float*
                                         align_up(static_cast<size_t>(cy) * d * 16));
   hipLaunchKernelGGL(stats_partial_kernel, dim3(cx), dim3(256), 0, st, x, n, d, rx, sx);
   hipLaunchKernelGGL(stats_partial_kernel, dim3(cy), dim3(256), 0, st, y, m, d, ry, sy);
-  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(256), 0, st, sx, cx, sy, cy, d, n, m, inv_scale_cost, o,
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(kFinalThreads), 0, st, sx, cx, sy, cy, d, n, m, inv_scale_cost, o,
                      static_cast<float*>(nullptr), static_cast<SolveScalars*>(nullptr), 0.0, 0.0, 1.0, 1.0);
   OTTX_LAUNCHED();
   OTTX_HIP(hipMemcpyAsync(out_host, o, sizeof(double), hipMemcpyDeviceToHost, st));
