@@ -372,40 +372,63 @@ __global__ void pool_partial_kernel(const float* __restrict__ H, const int* __re
   const int r0 = seg_off[seg], r1 = seg_off[seg + 1];
   const int per = (r1 - r0 + chunks - 1) / chunks;
   const int a = r0 + c * per, e = min(r1, a + per);
-  // four independent sums keep four loads in flight per thread
-  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-  int r = a;
-  for (; r + 3 < e; r += 4) {
-    s0 += H[static_cast<int64_t>(r) * 256 + f];
-    s1 += H[static_cast<int64_t>(r + 1) * 256 + f];
-    s2 += H[static_cast<int64_t>(r + 2) * 256 + f];
-    s3 += H[static_cast<int64_t>(r + 3) * 256 + f];
+  // U rows per trip, every load unconditional (rows past the chunk reload its
+  // last row and count zero): the sum is short and bound by latency
+  constexpr int U = 16;
+  float sum = 0.f;
+  for (int r = a; r < e; r += U) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float w = H[static_cast<int64_t>(min(r + u, e - 1)) * 256 + f];
+      v[u] = r + u < e ? w : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u += 2) sum += v[u] + v[u + 1];
   }
-  for (; r < e; ++r) s0 += H[static_cast<int64_t>(r) * 256 + f];
-  part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f] = (s0 + s1) + (s2 + s3);
+  part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f] = sum;
 }
 
 // ctx[b] = [e_x (256), e_y (256), kappa = eps / sigma^2, mom_b (d + 2), 0 pad ...]
 // (ld ctx_width(d)).  Grid B.
-__global__ void pool_final_kernel(const float* __restrict__ part, const int* __restrict__ seg_off, int chunks,
+constexpr int kPoolGroups = 4;
+
+__global__ __launch_bounds__(256 * kPoolGroups) void pool_final_kernel(const float* __restrict__ part, const int* __restrict__ seg_off, int chunks,
                                   const float* __restrict__ eps, const float* __restrict__ eps_rel,
                                   const float* __restrict__ sig2, const float* __restrict__ mom, int d,
                                   float* __restrict__ ctx) {
-  const int b = blockIdx.x, f = threadIdx.x;
+  // thread (feature f, group g) sums the chunks g, g + kPoolGroups, ... of
+  // both sides, all its loads in flight at once; the groups meet in LDS
+  __shared__ float red[2][kPoolGroups][256];
+  const int b = blockIdx.x, f = threadIdx.x & 255, g = threadIdx.x >> 8;
   const int W = ctx_width(d);
   float* cb = ctx + static_cast<int64_t>(b) * W;
+#pragma unroll
   for (int side = 0; side < 2; ++side) {
     const int seg = b * 2 + side;
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    int c = 0;
-    for (; c + 3 < chunks; c += 4) {
-      s0 += part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f];
-      s1 += part[(static_cast<int64_t>(seg) * chunks + c + 1) * 256 + f];
-      s2 += part[(static_cast<int64_t>(seg) * chunks + c + 2) * 256 + f];
-      s3 += part[(static_cast<int64_t>(seg) * chunks + c + 3) * 256 + f];
+    constexpr int U = 16;
+    float sum = 0.f;
+    for (int c = g; c < chunks; c += U * kPoolGroups) {
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int cu = c + u * kPoolGroups;
+        v[u] = cu < chunks ? part[(static_cast<int64_t>(seg) * chunks + cu) * 256 + f] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) sum += v[u];
     }
-    for (; c < chunks; ++c) s0 += part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f];
-    cb[side * 256 + f] = ((s0 + s1) + (s2 + s3)) / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
+    red[side][g][f] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x >= 256) return;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int seg = b * 2 + side;
+    float sum = 0.f;
+#pragma unroll
+    for (int q = 0; q < kPoolGroups; ++q) sum += red[side][q][f];
+    cb[side * 256 + f] = sum / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
   }
   // kappa = eps / sigma^2; with eps = c mean(C) and sigma^2 = mean(C) / 2 it is 2 c.
   const float kappa = (eps_rel && eps_rel[b] > 0.f) ? 2.f * eps_rel[b] : eps[b] / sig2[b];
@@ -930,7 +953,7 @@ DsLayout ds_layout(const ottx_ds_batch* bt, const ParamLayout& P, bool training)
   L.Pe_pad = pad64(L.Pe);
   L.Ph = bt->m_total;
   L.Ph_pad = pad64(L.Ph);
-  L.chunks = 32;
+  L.chunks = 64;
   L.kchunks = training ? 64 : 0;
   size_t o = 0;
   auto take = [&](size_t b) {
@@ -1132,8 +1155,8 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   // pooling per (problem, side) and the context vector
   hipLaunchKernelGGL(pool_partial_kernel, dim3(L.chunks, 2 * B), dim3(256), 0, st, at<float>(ws, L.H2f), bt->seg_off,
                      L.chunks, at<float>(ws, L.part));
-  hipLaunchKernelGGL(pool_final_kernel, dim3(B), dim3(256), 0, st, at<float>(ws, L.part), bt->seg_off, L.chunks,
-                     bt->eps, bt->eps_rel, sig2, mom, bt->d, at<float>(ws, L.ctx));
+  hipLaunchKernelGGL(pool_final_kernel, dim3(B), dim3(256 * kPoolGroups), 0, st, at<float>(ws, L.part), bt->seg_off,
+                     L.chunks, bt->eps, bt->eps_rel, sig2, mom, bt->d, at<float>(ws, L.ctx));
   hipLaunchKernelGGL(ctx_forward_kernel, dim3((256 + P.kin + 3) / 4, B), dim3(256), 0, st, prm + P.Wc, prm + P.bh,
                      prm + P.Wq, at<float>(ws, L.ctx), bt->d, P.kin, at<float>(ws, L.hctx), at<float>(ws, L.zq));
   DS_LAUNCHED();

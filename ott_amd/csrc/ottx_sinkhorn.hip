@@ -503,10 +503,32 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
     float* norm = p.norm[side];
     if (local >= p.np[side]) {
       norm[local] = 0.f;
+      if (side == 0 && p.init.lse_f != nullptr) p.init.lse_f[local] = -INFINITY;
+      if (side == 1 && p.init.lse_g != nullptr && p.init.lse_g_mode == kShiftFromPot) p.init.lse_g[local] = -INFINITY;
       return;
     }
     const bool vec16 = (p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[side]) & 15) == 0;
-    norm[local] = point_norm(p.pts[side] + local * p.d, vec16, p.mu, p.d, p.s);
+    const float nrm = point_norm(p.pts[side] + local * p.d, vec16, p.mu, p.d, p.s);
+    norm[local] = nrm;
+    if (side == 0 && p.init.lse_f != nullptr) {
+      // The first f-update's fixed shift from the given f: the column
+      // log-sum-exp (in the kernel's units) that the f-update's epilogue
+      // would map back to f itself, L = (eps log a - f / tau_a + s|x - mu|^2) / (2 s).
+      // A warm start puts the first update's result close to it; non-finite
+      // values get -inf, so their tiles use the running max, and a tile whose
+      // sums leave range falls back to it as well.
+      const PrepInit& in = p.init;
+      const double w = p.wuser[0] ? static_cast<double>(p.wuser[0][local]) : 1.0 / static_cast<double>(p.np[0]);
+      const double v = (in.scal->eps * log(w) - static_cast<double>(in.f[local]) / in.tau_a + nrm) / (2.0 * p.s);
+      in.lse_f[local] = isfinite(v) ? static_cast<float>(v) : -INFINITY;
+    }
+    if (side == 1 && p.init.lse_g != nullptr && p.init.lse_g_mode == kShiftFromPot) {
+      // the same estimate for the first g-update from the given g
+      const PrepInit& in = p.init;
+      const double w = p.wuser[1] ? static_cast<double>(p.wuser[1][local]) : 1.0 / static_cast<double>(p.np[1]);
+      const double v = (in.scal->eps * log(w) - static_cast<double>(in.g[local]) / in.tau_b + nrm) / (2.0 * p.s);
+      in.lse_g[local] = isfinite(v) ? static_cast<float>(v) : -INFINITY;
+    }
     if (side == 1 && p.init.lse_g != nullptr && p.init.lse_g_mode == kShiftProbe) {
       // the probe's max, centred on x_0, to the kernel's centring on mu:
       // + <mu - x_0, (mu + x_0) / 2 - y_j>, plus half of log2(n) / c
@@ -557,41 +579,6 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
       st->last_err = 0.0;
     }
     for (int64_t k = local; k < in.outer; k += stride) in.errors[k] = -1.f;
-    if (in.lse_f != nullptr) {
-      // The first f-update's fixed shift from the given f: the column
-      // log-sum-exp (in the kernel's units) that the f-update's epilogue
-      // would map back to f itself, L = (eps log a - f / tau_a + s|x - mu|^2) / (2 s).
-      // A warm start puts the first update's result close to it; rows past n
-      // and non-finite values get -inf, so their tiles use the running max,
-      // and a tile whose sums leave range falls back to it as well.
-      const double eps = in.scal->eps;
-      const bool vec16 = (p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[0]) & 15) == 0;
-      for (int64_t k = local; k < p.np_pad[0]; k += stride) {
-        float L = -INFINITY;
-        if (k < p.np[0]) {
-          const double w = p.wuser[0] ? static_cast<double>(p.wuser[0][k]) : 1.0 / static_cast<double>(p.np[0]);
-          const double nrm = point_norm(p.pts[0] + k * p.d, vec16, p.mu, p.d, p.s);
-          const double v = (eps * log(w) - static_cast<double>(in.f[k]) / in.tau_a + nrm) / (2.0 * p.s);
-          if (isfinite(v)) L = static_cast<float>(v);
-        }
-        in.lse_f[k] = L;
-      }
-    }
-    if (in.lse_g != nullptr && in.lse_g_mode == kShiftFromPot) {
-      // the same estimate for the first g-update from the given g
-      const double eps = in.scal->eps;
-      const bool vec16 = (p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[1]) & 15) == 0;
-      for (int64_t k = local; k < p.np_pad[1]; k += stride) {
-        float L = -INFINITY;
-        if (k < p.np[1]) {
-          const double w = p.wuser[1] ? static_cast<double>(p.wuser[1][k]) : 1.0 / static_cast<double>(p.np[1]);
-          const double nrm = point_norm(p.pts[1] + k * p.d, vec16, p.mu, p.d, p.s);
-          const double v = (eps * log(w) - static_cast<double>(in.g[k]) / in.tau_b + nrm) / (2.0 * p.s);
-          if (isfinite(v)) L = static_cast<float>(v);
-        }
-        in.lse_g[k] = L;
-      }
-    }
     for (int64_t k = local; k < in.n_counters; k += stride) in.counters[k] = 0u;
     // DefaultInitializer: 0, -inf where the weight is 0
     if (in.zero_init & 1)
