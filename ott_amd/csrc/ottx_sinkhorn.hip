@@ -262,6 +262,8 @@ struct PrepInit {
 };
 
 constexpr int kShiftFromPot = 1;
+// Threads sharing one row in the preparation's norm segments (d <= 256).
+constexpr int kNormLanes = 16;
 constexpr int kShiftProbe = 2;
 // Rows of x sampled by the first g-update's shift probe.
 constexpr int kProbeRows = 64;
@@ -459,31 +461,6 @@ struct PrepArgs {
   PrepInit init;
 };
 
-// s |p - mu|^2 of one point in float64: four independent sums, 16-byte loads
-// when the rows allow them.
-__device__ __forceinline__ float point_norm(const float* __restrict__ pt, bool vec16, const float* __restrict__ mu,
-                                            int d, double s) {
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int k = 0;
-  if (vec16) {
-    for (; k < d; k += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(pt + k);
-      const float4 u = *reinterpret_cast<const float4*>(mu + k);
-      const double d0 = static_cast<double>(v.x - u.x), d1 = static_cast<double>(v.y - u.y);
-      const double d2 = static_cast<double>(v.z - u.z), d3 = static_cast<double>(v.w - u.w);
-      a0 += d0 * d0;
-      a1 += d1 * d1;
-      a2 += d2 * d2;
-      a3 += d3 * d3;
-    }
-  }
-  for (; k < d; ++k) {
-    const double v = static_cast<double>(pt[k] - mu[k]);
-    a0 += v * v;
-  }
-  return static_cast<float>(s * ((a0 + a1) + (a2 + a3)));
-}
-
 __global__ void prep_pack_kernel(const PrepArgs p) {
   const int bid = blockIdx.x;
   const int t = threadIdx.x;
@@ -497,66 +474,94 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
       pack_bf16_one(p.pts[side], p.np[side], p.d, p.mu, p.chunks, local, static_cast<uint4*>(p.frag[side]));
     else
       pack_f32_one(p.pts[side], p.np[side], p.d, p.mu, p.chunks, local, static_cast<float4*>(p.frag[side]));
-  } else if (segi < 4) {  // norms s |p - mu|^2
+  } else if (segi < 4) {  // norms s |p - mu|^2, kNormLanes threads a row
     const int side = segi - 2;
-    if (local >= p.np_pad[side]) return;
+    const int64_t row = local / kNormLanes;
+    const int ln = static_cast<int>(local % kNormLanes);
+    if (row >= p.np_pad[side]) return;  // whole rows leave together
     float* norm = p.norm[side];
-    if (local >= p.np[side]) {
-      norm[local] = 0.f;
-      if (side == 0 && p.init.lse_f != nullptr) p.init.lse_f[local] = -INFINITY;
-      if (side == 1 && p.init.lse_g != nullptr && p.init.lse_g_mode == kShiftFromPot) p.init.lse_g[local] = -INFINITY;
+    const PrepInit& in = p.init;
+    const bool est_f = side == 0 && in.lse_f != nullptr;
+    const bool est_g = side == 1 && in.lse_g != nullptr && in.lse_g_mode == kShiftFromPot;
+    const bool probe = side == 1 && in.lse_g != nullptr && in.lse_g_mode == kShiftProbe;
+    if (row >= p.np[side]) {
+      if (ln == 0) {
+        norm[row] = 0.f;
+        if (est_f) in.lse_f[row] = -INFINITY;
+        if (est_g) in.lse_g[row] = -INFINITY;
+      }
       return;
     }
-    const bool vec16 = (p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[side]) & 15) == 0;
-    const float nrm = point_norm(p.pts[side] + local * p.d, vec16, p.mu, p.d, p.s);
-    norm[local] = nrm;
-    if (side == 0 && p.init.lse_f != nullptr) {
+    // each lane's features, all loads of the row in flight at once; with the
+    // probe also the term that moves its centring from x_0 to mu,
+    // <mu - x_0, (mu + x_0) / 2 - y_j>
+    const float* pt = p.pts[side] + row * p.d;
+    const float* x0 = p.pts[0];
+    const bool vec16 = (p.d & 3) == 0 && (reinterpret_cast<uintptr_t>(p.pts[side]) & 15) == 0 &&
+                       (reinterpret_cast<uintptr_t>(x0) & 15) == 0;
+    double a = 0.0, cr = 0.0;
+    if (vec16) {
+#pragma unroll
+      for (int i = 0; i < 256 / (4 * kNormLanes); ++i) {
+        const int k = 4 * (ln + kNormLanes * i);
+        if (k < p.d) {
+          const float4 v = *reinterpret_cast<const float4*>(pt + k);
+          const float4 u = *reinterpret_cast<const float4*>(p.mu + k);
+          const double e0 = static_cast<double>(v.x - u.x), e1 = static_cast<double>(v.y - u.y);
+          const double e2 = static_cast<double>(v.z - u.z), e3 = static_cast<double>(v.w - u.w);
+          a += (e0 * e0 + e1 * e1) + (e2 * e2 + e3 * e3);
+          if (probe) {
+            const float4 z = *reinterpret_cast<const float4*>(x0 + k);
+            cr += (static_cast<double>(u.x) - z.x) * (0.5 * (static_cast<double>(u.x) + z.x) - v.x) +
+                  (static_cast<double>(u.y) - z.y) * (0.5 * (static_cast<double>(u.y) + z.y) - v.y) +
+                  (static_cast<double>(u.z) - z.z) * (0.5 * (static_cast<double>(u.z) + z.z) - v.z) +
+                  (static_cast<double>(u.w) - z.w) * (0.5 * (static_cast<double>(u.w) + z.w) - v.w);
+          }
+        }
+      }
+    } else {
+      for (int k = ln; k < p.d; k += kNormLanes) {
+        const double e = static_cast<double>(pt[k] - p.mu[k]);
+        a += e * e;
+        if (probe) {
+          const double u = p.mu[k], z = x0[k];
+          cr += (u - z) * (0.5 * (u + z) - static_cast<double>(pt[k]));
+        }
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < kNormLanes; off <<= 1) {
+      a += __shfl_xor(a, off, kNormLanes);
+      cr += __shfl_xor(cr, off, kNormLanes);
+    }
+    if (ln != 0) return;
+    const float nrm = static_cast<float>(p.s * a);
+    norm[row] = nrm;
+    if (est_f) {
       // The first f-update's fixed shift from the given f: the column
       // log-sum-exp (in the kernel's units) that the f-update's epilogue
       // would map back to f itself, L = (eps log a - f / tau_a + s|x - mu|^2) / (2 s).
       // A warm start puts the first update's result close to it; non-finite
       // values get -inf, so their tiles use the running max, and a tile whose
       // sums leave range falls back to it as well.
-      const PrepInit& in = p.init;
-      const double w = p.wuser[0] ? static_cast<double>(p.wuser[0][local]) : 1.0 / static_cast<double>(p.np[0]);
-      const double v = (in.scal->eps * log(w) - static_cast<double>(in.f[local]) / in.tau_a + nrm) / (2.0 * p.s);
-      in.lse_f[local] = isfinite(v) ? static_cast<float>(v) : -INFINITY;
+      const double w = p.wuser[0] ? static_cast<double>(p.wuser[0][row]) : 1.0 / static_cast<double>(p.np[0]);
+      const double v = (in.scal->eps * log(w) - static_cast<double>(in.f[row]) / in.tau_a + nrm) / (2.0 * p.s);
+      in.lse_f[row] = isfinite(v) ? static_cast<float>(v) : -INFINITY;
     }
-    if (side == 1 && p.init.lse_g != nullptr && p.init.lse_g_mode == kShiftFromPot) {
+    if (est_g) {
       // the same estimate for the first g-update from the given g
-      const PrepInit& in = p.init;
-      const double w = p.wuser[1] ? static_cast<double>(p.wuser[1][local]) : 1.0 / static_cast<double>(p.np[1]);
-      const double v = (in.scal->eps * log(w) - static_cast<double>(in.g[local]) / in.tau_b + nrm) / (2.0 * p.s);
-      in.lse_g[local] = isfinite(v) ? static_cast<float>(v) : -INFINITY;
+      const double w = p.wuser[1] ? static_cast<double>(p.wuser[1][row]) : 1.0 / static_cast<double>(p.np[1]);
+      const double v = (in.scal->eps * log(w) - static_cast<double>(in.g[row]) / in.tau_b + nrm) / (2.0 * p.s);
+      in.lse_g[row] = isfinite(v) ? static_cast<float>(v) : -INFINITY;
     }
-    if (side == 1 && p.init.lse_g != nullptr && p.init.lse_g_mode == kShiftProbe) {
-      // the probe's max, centred on x_0, to the kernel's centring on mu:
-      // + <mu - x_0, (mu + x_0) / 2 - y_j>, plus half of log2(n) / c
-      const float M = p.init.lse_g[local];
+    if (probe) {
+      // the probe's max, centred on x_0, moved to the centring on mu, plus
+      // half of log2(n) / c
+      const float M = in.lse_g[row];
       if (isfinite(M)) {
-        const float* yj = p.pts[1] + local * p.d;
-        const float* x0 = p.pts[0];
-        double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
-        int k = 0;
-        if (vec16 && (reinterpret_cast<uintptr_t>(x0) & 15) == 0) {
-          for (; k < p.d; k += 4) {
-            const float4 v = *reinterpret_cast<const float4*>(yj + k);
-            const float4 u = *reinterpret_cast<const float4*>(p.mu + k);
-            const float4 z = *reinterpret_cast<const float4*>(x0 + k);
-            c0 += (static_cast<double>(u.x) - z.x) * (0.5 * (static_cast<double>(u.x) + z.x) - v.x);
-            c1 += (static_cast<double>(u.y) - z.y) * (0.5 * (static_cast<double>(u.y) + z.y) - v.y);
-            c2 += (static_cast<double>(u.z) - z.z) * (0.5 * (static_cast<double>(u.z) + z.z) - v.z);
-            c3 += (static_cast<double>(u.w) - z.w) * (0.5 * (static_cast<double>(u.w) + z.w) - v.w);
-          }
-        }
-        for (; k < p.d; ++k) {
-          const double u = p.mu[k], z = x0[k];
-          c0 += (u - z) * (0.5 * (u + z) - static_cast<double>(yj[k]));
-        }
-        const double corr = (c0 + c1) + (c2 + c3);
-        const double L = static_cast<double>(M) + corr +
-                         0.5 * log2(static_cast<double>(p.np[0])) / static_cast<double>(p.init.scal->c);
-        p.init.lse_g[local] = isfinite(L) ? static_cast<float>(L) : -INFINITY;
+        const double L = static_cast<double>(M) + cr +
+                         0.5 * log2(static_cast<double>(p.np[0])) / static_cast<double>(in.scal->c);
+        in.lse_g[row] = isfinite(L) ? static_cast<float>(L) : -INFINITY;
       }
     }
   } else if (segi < 6) {  // uniform weights
@@ -1199,7 +1204,7 @@ int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* w
   }
   for (int side = 0; side < 2; ++side) {
     pa.seg[2 + side] = cur;
-    cur += blocks(pa.np_pad[side]);
+    cur += blocks(pa.np_pad[side] * kNormLanes);
   }
   for (int side = 0; side < 2; ++side) {
     pa.seg[4 + side] = cur;
