@@ -195,6 +195,54 @@ def test_solve_warm_start_restart():
   assert rel < 1e-5
 
 
+class _GivenF:
+  """An initializer that predicts f only: the solve takes f as given and sets g
+  itself (OTTX_INIT_F_GIVEN), as with the learned warm start."""
+  predicts_f = True
+
+  def __init__(self, f):
+    self.f = f
+
+  def predict_f(self, ot_prob, out=None):
+    out.copy_(self.f)
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("start", ["converged", "off_sample"])
+def test_first_pass_shift_estimates_leave_the_solve_unchanged(precision, start):
+  """A start from f alone estimates the first g-update's log-sum-exp from 64
+  sampled rows and streams that pass with a fixed shift; a tile whose sums
+  leave range re-streams with the running max.  Either way the solve must be
+  the one of the same start given as (f, g) explicitly (first g-update from
+  f, sinkhorn.py:794-807).  "off_sample" raises f by 100 eps on every row the
+  sample misses, so the estimate is far outside the window and the fallback
+  runs."""
+  from ott_amd.problems.linear import linear_problem
+  from ott_amd.solvers.linear import sinkhorn
+  n, m = 700, 513
+  x, y, a, b = _cloud(n, m, 16, seed=12)
+  eps = 0.05
+  geom = _geom(x, y, precision, epsilon=eps)
+  ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+  solver = sinkhorn.Sinkhorn(threshold=1e-4, inner_iterations=1)
+  f0 = solver(linear_problem.LinearProblem(geom, ta, tb)).f.clone()
+  if start == "off_sample":
+    sampled = (np.arange(64) * n) // 64  # the probe's rows
+    off = np.ones(n, dtype=bool)
+    off[sampled] = False
+    f0[torch.from_numpy(off).cuda()] += 100.0 * eps
+  fast = sinkhorn.Sinkhorn(threshold=1e-4, inner_iterations=1, initializer=_GivenF(f0))(
+      linear_problem.LinearProblem(_geom(x, y, precision, epsilon=eps), ta, tb))
+  explicit = solver(linear_problem.LinearProblem(_geom(x, y, precision, epsilon=eps), ta, tb),
+                    init=(f0, torch.zeros(m, device=f0.device)))
+  assert fast.converged and explicit.converged
+  assert fast.n_iters == explicit.n_iters
+  assert abs(fast.reg_ot_cost - explicit.reg_ot_cost) <= 1e-6 * abs(explicit.reg_ot_cost)
+  torch.testing.assert_close(fast.f, explicit.f, rtol=1e-5, atol=1e-5 * float(explicit.f.abs().max()))
+  torch.testing.assert_close(fast.g, explicit.g, rtol=1e-5, atol=1e-5 * float(explicit.g.abs().max()))
+
+
 def test_determinism():
   x, y, a, b = _cloud(700, 900, 64, seed=8)
   o1 = _solve(x, y, a, b, "bf16_split", dict(epsilon=0.05, relative_epsilon="mean"))
