@@ -341,8 +341,8 @@ def cpu_baseline(pb, n_iters_needed, budget_s=15.0):
 def main() -> int:
   ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
   ap.add_argument("--gpus", type=int, default=1)
-  ap.add_argument("--steps", type=int, default=3)
-  ap.add_argument("--warmup", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=20)
+  ap.add_argument("--warmup", type=int, default=5)
   ap.add_argument("--workload", default="gaussian_d64", choices=["gaussian_d64", "uniform_d2"])
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
   ap.add_argument("--cpu-budget", type=float, default=15.0)
