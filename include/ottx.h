@@ -102,6 +102,15 @@ typedef struct ottx_sinkhorn_opts {
                                balanced problem; no change when tau_a or tau_b is 1 (the errors
                                are unchanged in both cases).  Unbalanced problems with tau_a < 1
                                and tau_b < 1 return OTTX_EUNSUPPORTED. */
+  int32_t parallel_dual_updates; /* 1: Sinkhorn(parallel_dual_updates=True) (sinkhorn.py:840-846):
+                               the f-update reads the g of the previous iteration.  Supported for
+                               the symmetric problems of a Sinkhorn divergence
+                               (sinkhorn_divergence.py:298-318): x and y the same buffer, a and b
+                               the same buffer (or both NULL), tau_a = tau_b = 1, init_mode
+                               OTTX_INIT_ZEROS, no recentring.  There f and g stay equal, so each
+                               iteration is one pass: g <- (1 - w) g + w g^(f), then f <- g; the
+                               error, row plus column marginal (sinkhorn.py:160-189), is twice the
+                               column one.  Other problems return OTTX_EUNSUPPORTED. */
 } ottx_sinkhorn_opts;
 
 /* Outputs of one solve (SinkhornOutput fields, sinkhorn.py:280-323). */

@@ -64,6 +64,7 @@ class SinkhornOpts(ctypes.Structure):
       ("use_graph", ctypes.c_int32),
       ("init_mode", ctypes.c_int32),
       ("recenter_potentials", ctypes.c_int32),
+      ("parallel_dual_updates", ctypes.c_int32),
   ]
 
 

@@ -94,6 +94,7 @@ struct HalfStepArgs {
   float inv_rho;        // 1 / rho_out (0 when tau == 1)
   float momentum;       // constant momentum weight
   int32_t balanced;
+  int32_t symmetric;    // symmetric problem with parallel updates: error = 2 x column error
   int32_t side;         // 1: out = y (tau_b), 0: out = x (tau_a)
   const SolveScalars* scal;  // when set, overrides c, eps, eps_marg and inv_rho
   const float* lse_shift;    // per-column log-sum-exp (acc units) of the previous same-side pass

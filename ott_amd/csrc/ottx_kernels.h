@@ -75,6 +75,9 @@ __device__ inline void finalize_block(const HalfStepArgs& p, double col, double 
   const double inv_p = 1.0 / static_cast<double>(p.norm_p);
   double err = (p.norm_p == 1) ? col : pow(col, inv_p);
   if (!p.balanced) err += (p.norm_p == 1) ? row : pow(row, inv_p);
+  // parallel updates of a symmetric problem: the row marginal's error equals
+  // the column one (sinkhorn.py:160-189 adds both)
+  if (p.symmetric) err *= 2.0;
   SolveState* st = p.state;
   const int k = st->block;
   const int t_last = k * p.inner + p.inner - 1;

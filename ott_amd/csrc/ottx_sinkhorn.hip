@@ -719,6 +719,15 @@ __global__ void dual_final_kernel(const double* __restrict__ part, const float* 
   out[7] = sc->eps;
 }
 
+// f <- g for the parallel updates of a symmetric problem (f = g throughout);
+// launches after the loop has stopped return at once.
+__global__ void copy_potential_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n,
+                                      const SolveState* __restrict__ st) {
+  if (st->done) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 // Outputs of a stopped loop, computed on the device right after the block
 // that stopped it so the host only has to read pinned memory: the float64
 // dual terms (as dual_partial_kernel + dual_final_kernel), g <- g_save, and
@@ -1454,6 +1463,12 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   const bool both_unbalanced = pb->tau_a < 1.0 && pb->tau_b < 1.0;
   if (o->recenter_potentials && (both_unbalanced || total_blocks == 0)) return OTTX_EUNSUPPORTED;
   const int recenter = (o->recenter_potentials && pb->tau_a == 1.0 && pb->tau_b == 1.0) ? 1 : 0;
+  // parallel dual updates: the symmetric problems of a Sinkhorn divergence,
+  // where f = g throughout and one pass per iteration updates both
+  const bool symmetric = o->parallel_dual_updates != 0;
+  if (symmetric && (pb->x != pb->y || pb->n != pb->m || pb->a != pb->b || pb->tau_a != 1.0 ||
+                    pb->tau_b != 1.0 || o->init_mode != OTTX_INIT_ZEROS || o->recenter_potentials))
+    return OTTX_EUNSUPPORTED;
   char* base = static_cast<char*>(workspace);
   SolveState* state = reinterpret_cast<SolveState*>(base + L.state);
   uint32_t* counters = reinterpret_cast<uint32_t*>(base + L.counters);
@@ -1508,10 +1523,19 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   f_plain.mode = kModeWritePot | kModeCheckDone;
   f_err.mode = kModeWritePot | kModeCheckDone | (balanced ? 0u : kModeErrRow);
 
+  if (symmetric) g_err.symmetric = 1;
   auto launch_block = [&](hipStream_t s) -> int {
     for (int it = 0; it < inner; ++it) {
-      int r = launch_halfstep(k, it == inner - 1 ? f_err : f_plain, s);
-      if (r != OTTX_OK) return r;
+      int r;
+      if (symmetric) {
+        // f <- g: the f-update of the parallel step is the g-update by symmetry
+        hipLaunchKernelGGL(copy_potential_kernel, dim3(static_cast<unsigned>((pb->n + 255) / 256)), dim3(256), 0, s,
+                           static_cast<const float*>(out->g), out->f, pb->n, static_cast<const SolveState*>(state));
+        OTTX_LAUNCHED();
+      } else {
+        r = launch_halfstep(k, it == inner - 1 ? f_err : f_plain, s);
+        if (r != OTTX_OK) return r;
+      }
       r = launch_halfstep(k, it == inner - 1 ? g_err : g_plain, s);
       if (r != OTTX_OK) return r;
     }

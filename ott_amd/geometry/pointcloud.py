@@ -189,6 +189,24 @@ class PointCloud:
     init = self._epsilon_init
     return not isinstance(init, eps_scheduler.Epsilon) or init.is_target_only
 
+  # -- divergences ------------------------------------------------------------
+  @classmethod
+  def prepare_divergences(cls, x, y, static_b: bool = False, **kwargs: Any) -> Tuple["PointCloud", ...]:
+    """pointcloud.py:226-237: the geometries (x, y), (x, x) and, unless
+    ``static_b``, (y, y).  The symmetric ones hold one cloud on both sides (the
+    same device buffer), which the solver's parallel updates require."""
+    geoms = [cls(x, y, **kwargs), cls(x, None, **kwargs)]
+    if not static_b:
+      geoms.append(cls(y, None, **kwargs))
+    return tuple(geoms)
+
+  def copy_epsilon(self, other: "PointCloud") -> "PointCloud":
+    """geometry.py:248-254: this geometry with ``other``'s epsilon (resolved)."""
+    return type(self)(self.x, None if self.y is self.x else self.y, cost_fn=self.cost_fn,
+                      batch_size=self._batch_size, scale_cost=self._scale_cost,
+                      epsilon=other.epsilon_scheduler, relative_epsilon=other._relative_epsilon,
+                      precision=self.precision)
+
   # -- the operator protocol -------------------------------------------------
   def _problem(self, a: Optional[torch.Tensor] = None, b: Optional[torch.Tensor] = None,
                tau_a: float = 1.0, tau_b: float = 1.0, epsilon: Optional[float] = None,

@@ -101,7 +101,9 @@ class Sinkhorn:
   Supported on the GPU path: ``lse_mode=True``; any ``threshold``,
   ``norm_error`` (p >= 1), ``inner_iterations``, ``min_iterations``,
   ``max_iterations``; constant ``Momentum`` (``start == 0``); sequential
-  updates; balanced or unbalanced problems; warm starts through ``init`` or an
+  updates, and parallel updates for the symmetric problems of a Sinkhorn
+  divergence (the same cloud and weights on both sides, from the default
+  start); balanced or unbalanced problems; warm starts through ``init`` or an
   ``initializer``.  Options outside that set raise ``NotImplementedError``;
   there is no silent fallback.  ``implicit_diff`` and ``use_danskin`` only
   affect reverse mode, which this package does not provide.
@@ -169,6 +171,13 @@ class Sinkhorn:
       raise NotImplementedError("recenter_potentials with tau_a < 1 and tau_b < 1 is not supported yet")
     if self.recenter_potentials and self.max_iterations == 0:
       raise NotImplementedError("recenter_potentials with max_iterations = 0 is not supported")
+    if self.parallel_dual_updates and not _is_symmetric(ot_prob):
+      # sinkhorn_divergence.py:298-318 runs parallel updates on x against x
+      # with a = b; there f = g throughout and one pass per iteration does both
+      raise NotImplementedError("parallel_dual_updates is supported for symmetric problems only "
+                                "(the same point cloud and weights on both sides, balanced)")
+    if self.parallel_dual_updates and self.recenter_potentials:
+      raise NotImplementedError("parallel_dual_updates with recenter_potentials is not supported")
 
   def _check_options(self) -> None:
     """Solver options the HIP path runs; the others raise NotImplementedError."""
@@ -176,8 +185,6 @@ class Sinkhorn:
       raise NotImplementedError("ott_amd runs the log-domain (lse_mode=True) solver only")
     if self.anderson is not None:
       raise NotImplementedError("Anderson acceleration is not supported by ott_amd")
-    if self.parallel_dual_updates:
-      raise NotImplementedError("parallel_dual_updates is not supported by ott_amd yet")
     if not self.momentum.is_constant:
       raise NotImplementedError("adaptive momentum is not supported by ott_amd yet")
     if self.progress_fn is not None:
@@ -191,6 +198,8 @@ class Sinkhorn:
     self._check_supported(ot_prob)
     geom = ot_prob.geom
     n, m = geom.shape
+    if self.parallel_dual_updates and (init is not None or type(self.initializer) is not init_lib.DefaultInitializer):
+      raise NotImplementedError("parallel_dual_updates runs from the default (zero) start only")
     if init is None and getattr(self.initializer, "predicts_g", False):
       # A warm start that predicts g only: the solver derives f0 from it with
       # its own first f-update instead of a separate pass over the geometry.
@@ -221,6 +230,16 @@ class Sinkhorn:
     return run(ot_prob, self, (f, g))
 
 
+def _is_symmetric(ot_prob: linear_problem.LinearProblem) -> bool:
+  """The same points and weights on both sides of a balanced problem."""
+  geom = ot_prob.geom
+  same_points = geom.x is geom.y or (geom.x.data_ptr() == geom.y.data_ptr() and geom.x.shape == geom.y.shape)
+  a, b = ot_prob._a, ot_prob._b
+  same_weights = (a is None and b is None) or (a is not None and b is not None and a.data_ptr() == b.data_ptr()
+                                                 and a.shape == b.shape)
+  return same_points and same_weights and ot_prob.tau_a == 1.0 and ot_prob.tau_b == 1.0
+
+
 def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
         init: Tuple[torch.Tensor, torch.Tensor], init_mode: int = _lib.INIT_GIVEN) -> SinkhornOutput:
   """sinkhorn.py:1052-1099: the loop, then ``set_cost`` and ``ot_prob``."""
@@ -249,7 +268,8 @@ def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
       tile_config=tune.lookup(geom.precision, geom.x.shape[1], n, m),
       use_graph=1 if solver.use_graph else 0,
       init_mode=int(init_mode),
-      recenter_potentials=1 if solver.recenter_potentials else 0)
+      recenter_potentials=1 if solver.recenter_potentials else 0,
+      parallel_dual_updates=1 if solver.parallel_dual_updates else 0)
   errors = torch.empty((solver.outer_iterations,), dtype=torch.float32, device=geom.device)
   out = _lib.SinkhornOut(f=f.data_ptr(), g=g.data_ptr(), errors=errors.data_ptr())
   nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts))

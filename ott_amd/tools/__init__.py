@@ -1,3 +1,3 @@
-from ott_amd.tools import segment_sinkhorn
+from ott_amd.tools import segment_sinkhorn, sinkhorn_divergence
 
-__all__ = ["segment_sinkhorn"]
+__all__ = ["segment_sinkhorn", "sinkhorn_divergence"]

@@ -88,6 +88,8 @@ def segment_sinkhorn_outputs(
     raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
   solver = sinkhorn.Sinkhorn(**dict(sinkhorn_kwargs))
   solver._check_options()
+  if solver.parallel_dual_updates:
+    raise NotImplementedError("parallel_dual_updates is not supported by the batched solve")
   if solver.recenter_potentials:
     raise NotImplementedError("recenter_potentials is not supported by the batched solve")
   if solver.initializer is not None and not isinstance(solver.initializer, sinkhorn.init_lib.DefaultInitializer):

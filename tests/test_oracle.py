@@ -264,3 +264,33 @@ def test_oracle_recenter_balanced_closed_form():
   # one tau = 1: unchanged (sinkhorn.py:104-106)
   f2, g2 = o.recenter(f, g, a, b, 0.1, 1.0, 0.9)
   assert f2 is f and g2 is g
+
+
+def test_oracle_parallel_updates_on_a_symmetric_problem():
+  """Parallel updates with momentum 0.5 on x against x with a = b (the
+  symmetric terms of sinkhorn_divergence.py:298-318): f = g at every
+  iteration (equal starts, a symmetric cost), the same fixed point as the
+  sequential solver, and the parallel error adds the row and the column
+  marginal errors (sinkhorn.py:160-189), which are equal here."""
+  rng = np.random.default_rng(11)
+  x = rng.normal(size=(40, 3))
+  a = rng.uniform(0.2, 1.0, size=40)
+  a /= a.sum()
+  kw = dict(epsilon=0.1, threshold=1e-9, inner_iterations=1, max_iterations=5000)
+  par = oracle.solve_pointcloud(x, x, a, a, parallel_dual_updates=True, momentum=0.5, **kw)
+  # sequential updates oscillate on a symmetric problem and crawl: 5000
+  # iterations leave them near, not at, the fixed point the parallel ones
+  # reach in a few dozen
+  seq = oracle.solve_pointcloud(x, x, a, a, **kw)
+  assert par.converged and par.n_iters < 100
+  np.testing.assert_allclose(par.f, par.g, atol=1e-12)  # equal up to rounding
+  assert math.isclose(par.reg_ot_cost, seq.reg_ot_cost, rel_tol=1e-7)
+  # the symmetric fixed point: f = eps log a - eps LSE_j((f_j - C_ij) / eps)
+  cost = ((x[:, None, :] - x[None]) ** 2).sum(-1)
+  t = 0.1 * np.log(a) - 0.1 * special.logsumexp((par.f[None, :] - cost) / 0.1, axis=1)
+  np.testing.assert_allclose(par.f, t, atol=1e-8)
+  # the error of the first iteration: twice the L1 distance of the column marginal to a
+  one = oracle.solve_pointcloud(x, x, a, a, parallel_dual_updates=True, momentum=0.5, epsilon=0.1,
+                                threshold=0.0, inner_iterations=1, max_iterations=1)
+  plan = np.exp((one.f[:, None] + one.g[None] - cost) / 0.1)
+  np.testing.assert_allclose(one.errors[0], 2.0 * np.abs(plan.sum(0) - a).sum(), rtol=1e-10)
