@@ -163,6 +163,40 @@ def batched_cfg4(precision, steps, warmup, sequential=True):
   return res
 
 
+def divergence_f1(pb, steps, warmup):
+  """Sinkhorn divergence of the workload's clouds (tools/sinkhorn_divergence.py).
+
+  Three solves per call: (x, y) sequential, (x, x) and (y, y) with parallel
+  updates and momentum 0.5 (one pass per iteration), eps = 0.05 mean(C) of
+  (x, y) shared by all three, threshold 1e-3 checked every iteration.  Timed
+  from the call to the host-visible value; zero starts, as in the reference.
+  """
+  import torch
+  from ott_amd.tools import sinkhorn_divergence as sd
+  x = torch.from_numpy(pb.x).cuda()
+  y = torch.from_numpy(pb.y).cuda()
+  res = {"workload": "sinkdiv(x, y) on the benchmark clouds, eps = 0.05 mean C(x, y) shared, "
+                     "threshold 1e-3 every iteration, zero starts"}
+  for prec in ("bf16_split", "fp32"):
+    def one():
+      return sd.sinkdiv(x, y, epsilon=pb.eps_mult, relative_epsilon="mean", precision=prec,
+                        solve_kwargs=dict(threshold=1e-3, inner_iterations=1))
+    for _ in range(warmup):
+      one()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(steps):
+      torch.cuda.synchronize()
+      t0 = time.perf_counter()
+      div, out = one()
+      torch.cuda.synchronize()
+      times.append((time.perf_counter() - t0) * 1e3)
+    res[prec] = {"ms": round(sum(times) / len(times), 3), "ms_min": round(min(times), 3),
+                 "divergence": div, "n_iters_xy_xx_yy": list(out.n_iters),
+                 "converged": list(out.converged)}
+  return res
+
+
 def learned_gmm_d16(count=256, seed0=100000):
   """The d=16 warm start on held-out 4-component mixture problems (n=m=2048).
 
@@ -349,6 +383,7 @@ def main() -> int:
   ap.add_argument("--no-batched", action="store_true", help="skip the 512-problem batched solve")
   ap.add_argument("--no-learned-eval", action="store_true",
                   help="skip the d=16 warm-start evaluation on 256 held-out problems")
+  ap.add_argument("--no-divergence", action="store_true", help="skip the Sinkhorn divergence line")
   ap.add_argument("--batched-only", action="store_true", help="only the 512-problem batched solve")
   ap.add_argument("--halfstep-only", action="store_true",
                   help="only launch the half-step kernels and a calibration copy (for profiler passes)")
@@ -402,6 +437,11 @@ def main() -> int:
     batched = batched_cfg4("bf16_split", max(1, args.steps), 1)
     log(f"batched 512 problems: {batched}")
 
+  divergence = None
+  if not args.no_divergence:
+    divergence = divergence_f1(pb, max(3, args.steps // 4), 1)
+    log(f"sinkhorn divergence: {divergence}")
+
   learned16 = None
   if not args.no_learned_eval:
     learned16 = learned_gmm_d16()
@@ -454,6 +494,7 @@ def main() -> int:
       "roofline_fp32_variant": roof_b,
       "batched_512_problems": batched,
       "learned_warm_start_gmm_d16": learned16,
+      "sinkhorn_divergence": divergence,
       "cpu_baseline": cpu,
   }
   print(json.dumps(line), flush=True)
