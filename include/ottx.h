@@ -215,6 +215,14 @@ int ottx_pointcloud_op(const ottx_problem* problem, int32_t op, int32_t axis,
                        double eps, float* out, void* workspace,
                        size_t workspace_bytes, void* stream);
 
+/* Dense transport matrix from potentials, geometry.py:361-365
+ * (transport_from_potentials): out[i * m + j] = exp((f_i + g_j - C_ij) / eps)
+ * with C_ij = inv_scale_cost * |x_i - y_j|^2 summed from the differences in
+ * float32, row-major [n, m].  For SinkhornOutput.matrix (sinkhorn.py:445-451);
+ * the caller owns out (n * m floats). */
+int ottx_transport_matrix(const ottx_problem* problem, const float* f, const float* g, double eps, float* out,
+                          void* stream);
+
 /* Full log-domain Sinkhorn solve: Sinkhorn.__call__ -> run -> iterations
  * (sinkhorn.py:782-801, 1052-1099) with the fixed-point loop of
  * math/fixed_point_loop.py:23-93, the error of sinkhorn.py:127-221 fused into

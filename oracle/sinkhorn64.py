@@ -48,6 +48,9 @@ __all__ = [
     "dual_cost",
     "segment_point_cloud",
     "segment_sinkhorn",
+    "transport_from_potentials",
+    "apply_transport_from_potentials",
+    "transport_cost_at_cost",
 ]
 
 DEFAULT_EPSILON_SCALE = 0.05  # geometry/epsilon_scheduler.py:22
@@ -499,3 +502,31 @@ def segment_sinkhorn(x, y, num_per_segment_x, num_per_segment_y, max_measure_siz
   for k in range(sx.shape[0]):
     outs.append(solve_pointcloud(sx[k], sy[k], sa[k], sb[k], **pc_kwargs, **(sinkhorn_kwargs or {})))
   return outs
+
+
+# --------------------------------------------------------------------------
+# Output side: the transport matrix and its applications
+# --------------------------------------------------------------------------
+
+def transport_from_potentials(cost: np.ndarray, f, g, eps: float) -> np.ndarray:
+  """geometry.py:361-365 with _center (geometry.py:351-352): exp((f_i + g_j - C_ij) / eps)."""
+  f = np.asarray(f, dtype=np.float64)
+  g = np.asarray(g, dtype=np.float64)
+  with np.errstate(invalid="ignore"):
+    return np.exp((f[:, None] + g[None, :] - cost) / eps)
+
+
+def apply_transport_from_potentials(cost: np.ndarray, f, g, vec, eps: float, axis: int = 0) -> np.ndarray:
+  """geometry.py:445-486 (lse_mode): axis 0 sums over the rows, P^T vec; axis 1
+  over the columns, P vec; vec is one vector or a batch [k, .].  The reference
+  takes a signed, weighted log-sum-exp; this restatement forms the sum
+  directly in float64."""
+  p = transport_from_potentials(cost, f, g, eps)
+  v = np.asarray(vec, dtype=np.float64)
+  return (v @ p) if axis == 0 else (v @ p.T)
+
+
+def transport_cost_at_cost(cost_p: np.ndarray, f, g, eps: float, other_cost: np.ndarray) -> float:
+  """sinkhorn.py:386-411 (dense branch): sum(P * C'), P from the solve's cost."""
+  return float((transport_from_potentials(cost_p, f, g, eps) * other_cost).sum())
+

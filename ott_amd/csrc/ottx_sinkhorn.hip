@@ -719,6 +719,31 @@ __global__ void dual_final_kernel(const double* __restrict__ part, const float* 
   out[7] = sc->eps;
 }
 
+// Dense transport matrix exp((f_i + g_j - s |x_i - y_j|^2) / eps), row-major
+// [n, m]: four rows by 64 columns per workgroup, one entry per thread, the
+// squared distance summed from the differences.
+__global__ void transport_matrix_kernel(const float* __restrict__ x, const float* __restrict__ y, int64_t n,
+                                        int64_t m, int d, float s, const float* __restrict__ f,
+                                        const float* __restrict__ g, float inv_eps, float* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.y) * 4 + (threadIdx.x >> 6);
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * 64 + (threadIdx.x & 63);
+  if (i >= n || j >= m) return;
+  const float* xi = x + i * d;
+  const float* yj = y + j * d;
+  float c0 = 0.f, c1 = 0.f;
+  int k = 0;
+  for (; k + 1 < d; k += 2) {
+    const float u = xi[k] - yj[k], v = xi[k + 1] - yj[k + 1];
+    c0 = fmaf(u, u, c0);
+    c1 = fmaf(v, v, c1);
+  }
+  if (k < d) {
+    const float u = xi[k] - yj[k];
+    c0 = fmaf(u, u, c0);
+  }
+  out[i * m + j] = expf((f[i] + g[j] - s * (c0 + c1)) * inv_eps);
+}
+
 // f <- g for the parallel updates of a symmetric problem (f = g throughout);
 // launches after the loop has stopped return at once.
 __global__ void copy_potential_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n,
@@ -1365,6 +1390,19 @@ int ottx_mean_cost(int64_t n, int64_t m, int32_t d, const float* x, const float*
 size_t ottx_op_workspace_bytes(const ottx_problem* pb) {
   if (check_problem(pb) != OTTX_OK) return 0;
   return max_workspace(pb);
+}
+
+int ottx_transport_matrix(const ottx_problem* pb, const float* f, const float* g, double eps, float* out,
+                          void* stream) {
+  int rc = check_problem(pb);
+  if (rc != OTTX_OK) return rc;
+  if (!f || !g || !out || !(eps > 0.0)) return OTTX_EINVAL;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const dim3 grid(static_cast<unsigned>((pb->m + 63) / 64), static_cast<unsigned>((pb->n + 3) / 4));
+  hipLaunchKernelGGL(transport_matrix_kernel, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d,
+                     static_cast<float>(pb->inv_scale_cost), f, g, static_cast<float>(1.0 / eps), out);
+  OTTX_LAUNCHED();
+  return OTTX_OK;
 }
 
 int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const float* f, const float* g,

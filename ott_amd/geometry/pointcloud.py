@@ -261,6 +261,67 @@ class PointCloud:
     """geometry.py:326-349 (target epsilon)."""
     return self._op(_lib.OP_MARGINAL, f, g, self.epsilon, axis)
 
+  # -- transport from potentials (output side) --------------------------------
+  def transport_from_potentials(self, f, g) -> torch.Tensor:
+    """geometry.py:361-365: the dense [n, m] transport matrix (target epsilon)."""
+    lib = _lib.load_library()
+    n, m = self.shape
+    f = as_device_f32(f, self.device)
+    g = as_device_f32(g, self.device)
+    out = torch.empty((n, m), dtype=torch.float32, device=self.device)
+    pb, keep = self._problem(epsilon=self.epsilon)
+    _lib.check("ottx_transport_matrix", lib.ottx_transport_matrix(
+        ctypes.byref(pb), f.data_ptr(), g.data_ptr(), float(self.epsilon), out.data_ptr(), _lib.stream_ptr()))
+    del keep
+    return out
+
+  def apply_transport_from_potentials(self, f, g, vec, axis: int = 0) -> torch.Tensor:
+    """geometry.py:445-486: P^T vec (axis 0, vec over the n rows) or P vec
+    (axis 1, vec over the m columns), P = exp((f + g - C) / eps), for one
+    vector or a batch [k, n or m].
+
+    The reference's signed log-sum-exp (logsumexp with weights and
+    return_sign) is run as two fused log-sum-exp passes per vector, one with
+    the positive and one with the negative part of vec folded into the
+    potential as eps log(part).
+    """
+    f = as_device_f32(f, self.device)
+    g = as_device_f32(g, self.device)
+    vec = as_device_f32(vec, self.device)
+    single = vec.ndim == 1
+    vecs = vec[None] if single else vec
+    n, m = self.shape
+    if vecs.ndim != 2 or vecs.shape[1] != (n if axis == 0 else m):
+      raise ValueError(f"vec must be [{n if axis == 0 else m}] or [k, {n if axis == 0 else m}] for axis={axis}")
+    eps = self.epsilon
+    keep_pot = g if axis == 0 else f  # the potential of the output side
+    outs = []
+    for v in vecs:
+      parts = []
+      for part in (torch.clamp(v, min=0.0), torch.clamp(-v, min=0.0)):
+        with torch.no_grad():
+          shifted = (f if axis == 0 else g) + eps * torch.log(part)
+        res = self._op(_lib.OP_APPLY_LSE, shifted if axis == 0 else f, g if axis == 0 else shifted, eps, axis)
+        parts.append(torch.exp((res.double() + keep_pot.double()) / eps))
+      outs.append((parts[0] - parts[1]).float())
+    out = torch.stack(outs)
+    return out[0] if single else out
+
+  def low_rank_cost_factors(self) -> Tuple[torch.Tensor, torch.Tensor]:
+    """pointcloud.py:298-319 (``to_LRCGeometry`` of the squared-Euclidean cost):
+    C = cost_1 cost_2^T with cost_1 = [|x|^2, 1, -2 x] s and cost_2 = [1, |y|^2, y],
+    here on points centred by the mean of both clouds (C is unchanged by a
+    common shift, and the terms cancel less)."""
+    mu = 0.5 * (self.x.double().mean(0) + self.y.double().mean(0))
+    x = self.x.double() - mu
+    y = self.y.double() - mu
+    s = self.inv_scale_cost
+    ones_x = torch.ones((x.shape[0], 1), dtype=torch.float64, device=self.device)
+    ones_y = torch.ones((y.shape[0], 1), dtype=torch.float64, device=self.device)
+    cost_1 = torch.cat([(x * x).sum(1, keepdim=True), ones_x, -2.0 * x], dim=1) * s
+    cost_2 = torch.cat([ones_y, (y * y).sum(1, keepdim=True), y], dim=1)
+    return cost_1, cost_2
+
   def __repr__(self) -> str:
     n, m = self.shape
     return (f"PointCloud(n={n}, m={m}, d={self.x.shape[1]}, precision={self.precision!r})")

@@ -94,6 +94,48 @@ class SinkhornOutput(NamedTuple):
   def transport_mass(self) -> float:
     return float(self.marginal(0).double().sum())
 
+  @property
+  def scalings(self) -> Tuple[torch.Tensor, torch.Tensor]:
+    """sinkhorn.py:437-441: exp(f / eps), exp(g / eps)."""
+    eps = self.geom.epsilon
+    return torch.exp(self.f / eps), torch.exp(self.g / eps)
+
+  @property
+  def matrix(self) -> torch.Tensor:
+    """sinkhorn.py:443-449: the dense transport matrix (geometry.py:361-365)."""
+    return self.geom.transport_from_potentials(self.f, self.g)
+
+  def apply(self, inputs, axis: int = 0, lse_mode: bool = True) -> torch.Tensor:
+    """sinkhorn.py:456-470: the transport applied to ``inputs`` (axis 1: its transpose)."""
+    if not lse_mode:
+      raise NotImplementedError("apply with lse_mode=False is not supported by ott_amd")
+    return self.geom.apply_transport_from_potentials(self.f, self.g, inputs, axis=axis)
+
+  def transport_cost_at_geom(self, other_geom: PointCloud) -> float:
+    """sinkhorn.py:386-411: <P, C'> for another PointCloud's cost C'.
+
+    As the reference does for costs with a low-rank form, C' = cost_1 cost_2^T
+    (pointcloud.py:298-319) and the sum is sum(apply(cost_1^T) * cost_2^T):
+    the two constant columns by one marginal pass each, the d coordinate
+    columns by signed applies, never materialising P or C'.
+    """
+    if not isinstance(other_geom, PointCloud) or not other_geom.is_squared_euclidean:
+      raise NotImplementedError("transport_cost_at_geom needs a squared-Euclidean PointCloud")
+    if tuple(other_geom.shape) != tuple(self.geom.shape):
+      raise ValueError(f"other_geom must be {tuple(self.geom.shape)}, got {tuple(other_geom.shape)}")
+    cost_1, cost_2 = other_geom.low_rank_cost_factors()
+    col = self.marginal(0).double()  # 1^T P, over the m columns
+    row = self.marginal(1).double()  # P 1, over the n rows
+    total = (row * cost_1[:, 0]).sum() + (col * cost_2[:, 1]).sum() * other_geom.inv_scale_cost
+    cross = self.apply(cost_1[:, 2:].T.float(), axis=0).double()  # [d, m]: P^T (-2 s x_k)
+    total = total + (cross * cost_2[:, 2:].T).sum()
+    return float(total)
+
+  @property
+  def primal_cost(self) -> float:
+    """sinkhorn.py:345-348."""
+    return self.transport_cost_at_geom(other_geom=self.geom)
+
 
 class Sinkhorn:
   """Sinkhorn solver with OTT's constructor (sinkhorn.py:718-780).

@@ -1,0 +1,91 @@
+"""Output-side transport operations of SinkhornOutput (sinkhorn.py:337-470)
+against the dense FP64 oracle, evaluated at the potentials the GPU solve
+returned: ``matrix`` (geometry.py:361-365), ``apply`` both ways with signed and
+batched inputs (geometry.py:445-486), ``primal_cost`` and
+``transport_cost_at_geom`` through the low-rank form of the squared-Euclidean
+cost (sinkhorn.py:386-411, pointcloud.py:298-319)."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import sinkhorn64 as oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _solved(precision):
+  from ott_amd.geometry import pointcloud
+  from ott_amd.problems.linear import linear_problem
+  from ott_amd.solvers.linear import sinkhorn
+  rng = np.random.default_rng(31)
+  x = (rng.normal(size=(300, 8)) / np.sqrt(8)).astype(np.float32)
+  y = (rng.normal(size=(200, 8)) / np.sqrt(8) + 0.25).astype(np.float32)
+  a = rng.uniform(0.2, 1.2, 300)
+  a[7] = 0.0
+  a = (a / a.sum()).astype(np.float32)
+  b = rng.uniform(0.2, 1.2, 200)
+  b = (b / b.sum()).astype(np.float32)
+  geom = pointcloud.PointCloud(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), epsilon=0.05,
+                               relative_epsilon="mean", precision=precision)
+  prob = linear_problem.LinearProblem(geom, torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+  out = sinkhorn.Sinkhorn(threshold=1e-5, inner_iterations=1)(prob)
+  assert out.converged
+  f = out.f.double().cpu().numpy()
+  g = out.g.double().cpu().numpy()
+  cost = oracle.sqeuclidean_cost(x.astype(np.float64), y.astype(np.float64))
+  return out, x, y, f, g, cost, geom.epsilon
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+def test_matrix(precision):
+  out, x, y, f, g, cost, eps = _solved(precision)
+  p_ref = oracle.transport_from_potentials(cost, f, g, eps)
+  p = out.matrix.double().cpu().numpy()
+  assert p.shape == (300, 200)
+  assert np.all(p[7] == 0.0)  # zero weight: f = -inf
+  np.testing.assert_allclose(p, p_ref, rtol=1e-4, atol=1e-6 * p_ref.max())
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_apply_signed_batched(precision, axis):
+  out, x, y, f, g, cost, eps = _solved(precision)
+  rng = np.random.default_rng(2)
+  vec = rng.normal(size=(3, 300 if axis == 0 else 200)).astype(np.float32)
+  vec[1] = np.abs(vec[1])  # one vector of a single sign
+  got = out.apply(torch.from_numpy(vec).cuda(), axis=axis).double().cpu().numpy()
+  ref = oracle.apply_transport_from_potentials(cost, f, g, vec.astype(np.float64), eps, axis=axis)
+  # the scale of the sum of absolute terms bounds the cancellation
+  mag = oracle.apply_transport_from_potentials(cost, f, g, np.abs(vec).astype(np.float64), eps, axis=axis)
+  tol = 2e-5 if precision == "fp32" else 1e-4
+  np.testing.assert_allclose(got, ref, atol=tol * float(mag.max()))
+  one = out.apply(torch.from_numpy(vec[0]).cuda(), axis=axis).double().cpu().numpy()
+  np.testing.assert_allclose(one, got[0], rtol=0, atol=1e-12 * float(mag.max()) + 1e-30)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+def test_primal_cost_and_transport_cost_at_geom(precision):
+  from ott_amd.geometry import pointcloud
+  out, x, y, f, g, cost, eps = _solved(precision)
+  ref = oracle.transport_cost_at_cost(cost, f, g, eps, cost)
+  tol = 1e-5 if precision == "fp32" else 1e-4
+  assert math.isclose(out.primal_cost, ref, rel_tol=tol), (out.primal_cost, ref)
+  rng = np.random.default_rng(4)
+  x2 = (rng.normal(size=(300, 5)) + 1.0).astype(np.float32)
+  y2 = rng.normal(size=(200, 5)).astype(np.float32)
+  other = pointcloud.PointCloud(torch.from_numpy(x2).cuda(), torch.from_numpy(y2).cuda(), epsilon=1.0,
+                                scale_cost=2.0)
+  other_cost = oracle.sqeuclidean_cost(x2.astype(np.float64), y2.astype(np.float64), 0.5)
+  ref2 = oracle.transport_cost_at_cost(cost, f, g, eps, other_cost)
+  assert math.isclose(out.transport_cost_at_geom(other), ref2, rel_tol=tol), (out.transport_cost_at_geom(other), ref2)
+  # at convergence the primal cost and eps KL(P | a b^T) add up to reg_ot_cost (sinkhorn.py:224-277, :350-384)
+  p = oracle.transport_from_potentials(cost, f, g, eps)
+  a = out.a.double().cpu().numpy()
+  b = out.b.double().cpu().numpy()
+  q = a[:, None] * b[None, :]
+  with np.errstate(divide="ignore", invalid="ignore"):
+    kl = float(np.where(p > 0, p * np.log(p / q), 0.0).sum() - p.sum() + q.sum())
+  assert math.isclose(out.primal_cost + eps * kl, out.reg_ot_cost, rel_tol=1e-3)

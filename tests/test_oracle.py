@@ -294,3 +294,35 @@ def test_oracle_parallel_updates_on_a_symmetric_problem():
                                 threshold=0.0, inner_iterations=1, max_iterations=1)
   plan = np.exp((one.f[:, None] + one.g[None] - cost) / 0.1)
   np.testing.assert_allclose(one.errors[0], 2.0 * np.abs(plan.sum(0) - a).sum(), rtol=1e-10)
+
+
+def test_oracle_transport_ops_against_reg_ot_cost():
+  """At convergence the dual value equals the primal one:
+  reg_ot_cost = <P, C> + eps KL(P | a b^T) with KL(P|Q) = sum P log(P/Q) - P + Q
+  (sinkhorn.py:224-277 against :350-384), which pins the transport matrix and
+  the transport cost of the oracle; apply is the matrix product both ways."""
+  rng = np.random.default_rng(5)
+  x = rng.normal(size=(30, 3))
+  y = rng.normal(size=(25, 3)) + 0.4
+  a = rng.uniform(0.3, 1.0, 30)
+  a /= a.sum()
+  b = rng.uniform(0.3, 1.0, 25)
+  b /= b.sum()
+  eps = 0.2
+  out = oracle.solve_pointcloud(x, y, a, b, epsilon=eps, threshold=1e-12, inner_iterations=1,
+                                max_iterations=20000)
+  assert out.converged
+  cost = oracle.sqeuclidean_cost(x, y)
+  p = oracle.transport_from_potentials(cost, out.f, out.g, eps)
+  q = a[:, None] * b[None, :]
+  kl = float((p * np.log(p / q) - p + q).sum())
+  primal = oracle.transport_cost_at_cost(cost, out.f, out.g, eps, cost)
+  assert math.isclose(primal + eps * kl, out.reg_ot_cost, rel_tol=1e-9)
+  np.testing.assert_allclose(p.sum(1), a, rtol=1e-9)
+  np.testing.assert_allclose(p.sum(0), b, rtol=1e-9)
+  v = rng.normal(size=(2, 30))
+  np.testing.assert_allclose(oracle.apply_transport_from_potentials(cost, out.f, out.g, v, eps, axis=0),
+                             v @ p, rtol=1e-12)
+  w = rng.normal(size=25)
+  np.testing.assert_allclose(oracle.apply_transport_from_potentials(cost, out.f, out.g, w, eps, axis=1),
+                             p @ w, rtol=1e-12)
