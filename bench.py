@@ -197,6 +197,38 @@ def divergence_f1(pb, steps, warmup):
   return res
 
 
+def transport_ops_f2(pb, steps):
+  """Output-side transport ops on the solved benchmark problem (bf16 split):
+  one signed apply (two fused log-sum-exp passes) and primal_cost (two
+  marginal passes and 2 d signed passes), timed from call to host value."""
+  import torch
+  from ott_amd.geometry import pointcloud
+  from ott_amd.problems.linear import linear_problem
+  from ott_amd.solvers.linear import sinkhorn
+  x = torch.from_numpy(pb.x).cuda()
+  y = torch.from_numpy(pb.y).cuda()
+  geom = pointcloud.PointCloud(x, y, epsilon=pb.eps_mult, relative_epsilon="mean", precision="bf16_split")
+  out = sinkhorn.Sinkhorn(threshold=1e-3, inner_iterations=1)(linear_problem.LinearProblem(geom))
+  vec = torch.randn(x.shape[0], device="cuda", generator=torch.Generator("cuda").manual_seed(0))
+
+  def timed(fn):
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(steps):
+      t0 = time.perf_counter()
+      r = fn()
+      torch.cuda.synchronize()
+      ts.append((time.perf_counter() - t0) * 1e3)
+    return round(sum(ts) / len(ts), 3), r
+
+  ms_apply, _ = timed(lambda: out.apply(vec, axis=0))
+  ms_primal, primal = timed(lambda: out.primal_cost)
+  return {"workload": "solved benchmark problem (bf16 split, threshold 1e-3)",
+          "apply_one_signed_vector_ms": ms_apply, "primal_cost_ms": ms_primal,
+          "primal_cost": primal, "reg_ot_cost": out.reg_ot_cost}
+
+
 def learned_gmm_d16(count=256, seed0=100000):
   """The d=16 warm start on held-out 4-component mixture problems (n=m=2048).
 
@@ -384,6 +416,7 @@ def main() -> int:
   ap.add_argument("--no-learned-eval", action="store_true",
                   help="skip the d=16 warm-start evaluation on 256 held-out problems")
   ap.add_argument("--no-divergence", action="store_true", help="skip the Sinkhorn divergence line")
+  ap.add_argument("--no-transport-ops", action="store_true", help="skip the output-side transport ops line")
   ap.add_argument("--batched-only", action="store_true", help="only the 512-problem batched solve")
   ap.add_argument("--halfstep-only", action="store_true",
                   help="only launch the half-step kernels and a calibration copy (for profiler passes)")
@@ -442,6 +475,11 @@ def main() -> int:
     divergence = divergence_f1(pb, max(3, args.steps // 4), 1)
     log(f"sinkhorn divergence: {divergence}")
 
+  ops = None
+  if not args.no_transport_ops:
+    ops = transport_ops_f2(pb, 3)
+    log(f"transport ops: {ops}")
+
   learned16 = None
   if not args.no_learned_eval:
     learned16 = learned_gmm_d16()
@@ -495,6 +533,7 @@ def main() -> int:
       "batched_512_problems": batched,
       "learned_warm_start_gmm_d16": learned16,
       "sinkhorn_divergence": divergence,
+      "transport_ops": ops,
       "cpu_baseline": cpu,
   }
   print(json.dumps(line), flush=True)
