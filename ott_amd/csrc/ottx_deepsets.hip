@@ -389,47 +389,43 @@ __global__ void pool_partial_kernel(const float* __restrict__ H, const int* __re
   part[(static_cast<int64_t>(seg) * chunks + c) * 256 + f] = sum;
 }
 
-// ctx[b] = [e_x (256), e_y (256), kappa = eps / sigma^2, mom_b (d + 2), 0 pad ...]
-// (ld ctx_width(d)).  Grid B.
-constexpr int kPoolGroups = 4;
+// The context vector of problem b and its heads, in one launch:
+//   ctx[b] = [e_x (256), e_y (256), kappa = eps / sigma^2, mom_b (d + 2), 0 pad ...]
+//   (ld ctx_width(d)), e the means of the pooled encoder outputs;
+//   rows o < 256:        hctx[b, o] = bh[o] + sum_c Wc[o, c] ctx[b, c]
+//   rows o = 256 + k:    zq[b, k]   = sum_c Wq[k, c] ctx[b, c]   (k < kin)
+// Every workgroup forms ctx[b] in LDS from the pooling partials (thread f
+// sums feature f's chunks of both sides, kCtxTrip of each side per trip, all
+// in flight), workgroup 0 also stores it for the backward pass, and each wave
+// then takes one output row.  Grid ((256 + kin) / 4, B), block 256.
+constexpr int kCtxTrip = 32;
+constexpr int kCtxMax = 512 + (3 + 256 + 31) / 32 * 32;  // ctx_width(256)
 
-__global__ __launch_bounds__(256 * kPoolGroups) void pool_final_kernel(const float* __restrict__ part, const int* __restrict__ seg_off, int chunks,
-                                  const float* __restrict__ eps, const float* __restrict__ eps_rel,
-                                  const float* __restrict__ sig2, const float* __restrict__ mom, int d,
-                                  float* __restrict__ ctx) {
-  // thread (feature f, group g) sums the chunks g, g + kPoolGroups, ... of
-  // both sides, all its loads in flight at once; the groups meet in LDS
-  __shared__ float red[2][kPoolGroups][256];
-  const int b = blockIdx.x, f = threadIdx.x & 255, g = threadIdx.x >> 8;
+__global__ __launch_bounds__(256) void ctx_forward_kernel(
+    const float* __restrict__ part, const int* __restrict__ seg_off, int chunks, const float* __restrict__ eps,
+    const float* __restrict__ eps_rel, const float* __restrict__ sig2, const float* __restrict__ mom, int d,
+    const float* __restrict__ Wc, const float* __restrict__ bh, const float* __restrict__ Wq, int kin,
+    float* __restrict__ ctx, float* __restrict__ hctx, float* __restrict__ zq) {
+  __shared__ float cb[kCtxMax];
+  const int b = blockIdx.y, f = threadIdx.x;
   const int W = ctx_width(d);
-  float* cb = ctx + static_cast<int64_t>(b) * W;
+  const int seg0 = 2 * b;
+  float sum[2] = {0.f, 0.f};
+  for (int c = 0; c < chunks; c += kCtxTrip) {
+    float v[2][kCtxTrip];
 #pragma unroll
-  for (int side = 0; side < 2; ++side) {
-    const int seg = b * 2 + side;
-    constexpr int U = 16;
-    float sum = 0.f;
-    for (int c = g; c < chunks; c += U * kPoolGroups) {
-      float v[U];
+    for (int side = 0; side < 2; ++side)
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int cu = c + u * kPoolGroups;
-        v[u] = cu < chunks ? part[(static_cast<int64_t>(seg) * chunks + cu) * 256 + f] : 0.f;
-      }
+      for (int u = 0; u < kCtxTrip; ++u)
+        v[side][u] = c + u < chunks ? part[(static_cast<int64_t>(seg0 + side) * chunks + c + u) * 256 + f] : 0.f;
 #pragma unroll
-      for (int u = 0; u < U; ++u) sum += v[u];
-    }
-    red[side][g][f] = sum;
+    for (int side = 0; side < 2; ++side)
+#pragma unroll
+      for (int u = 0; u < kCtxTrip; ++u) sum[side] += v[side][u];
   }
-  __syncthreads();
-  if (threadIdx.x >= 256) return;
 #pragma unroll
-  for (int side = 0; side < 2; ++side) {
-    const int seg = b * 2 + side;
-    float sum = 0.f;
-#pragma unroll
-    for (int q = 0; q < kPoolGroups; ++q) sum += red[side][q][f];
-    cb[side * 256 + f] = sum / static_cast<float>(seg_off[seg + 1] - seg_off[seg]);
-  }
+  for (int side = 0; side < 2; ++side)
+    cb[side * 256 + f] = sum[side] / static_cast<float>(seg_off[seg0 + side + 1] - seg_off[seg0 + side]);
   // kappa = eps / sigma^2; with eps = c mean(C) and sigma^2 = mean(C) / 2 it is 2 c.
   const float kappa = (eps_rel && eps_rel[b] > 0.f) ? 2.f * eps_rel[b] : eps[b] / sig2[b];
   for (int i = f; i < W - 512; i += 256) {
@@ -438,22 +434,13 @@ __global__ __launch_bounds__(256 * kPoolGroups) void pool_final_kernel(const flo
     else if (i <= d + 2) v = mom[static_cast<int64_t>(b) * (d + 2) + i - 1];
     cb[512 + i] = v;
   }
-}
-
-// Context heads, one wave per output row:
-//   rows o < 256:        hctx[b, o] = bh[o] + sum_c Wc[o, c] ctx[b, c]
-//   rows o = 256 + k:    zq[b, k]   = sum_c Wq[k, c] ctx[b, c]   (k < kin)
-// Grid ((256 + kin) / 4, B), block 256.
-__global__ void ctx_forward_kernel(const float* __restrict__ Wc, const float* __restrict__ bh,
-                                   const float* __restrict__ Wq, const float* __restrict__ ctx, int d, int kin,
-                                   float* __restrict__ hctx, float* __restrict__ zq) {
+  __syncthreads();
+  if (blockIdx.x == 0)
+    for (int i = f; i < W; i += 256) ctx[static_cast<int64_t>(b) * W + i] = cb[i];
   const int lane = threadIdx.x & 63;
   const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int b = blockIdx.y;
   if (o >= 256 + kin) return;
-  const int W = ctx_width(d);
   const float* w = o < 256 ? Wc + static_cast<int64_t>(o) * W : Wq + static_cast<int64_t>(o - 256) * W;
-  const float* cb = ctx + static_cast<int64_t>(b) * W;
   float s = 0.f;
   for (int c = lane; c < W; c += 64) s = fmaf(w[c], cb[c], s);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
@@ -1155,10 +1142,9 @@ int forward(const ottx_ds_batch* bt, const ottx_ds_model* md, void* workspace, h
   // pooling per (problem, side) and the context vector
   hipLaunchKernelGGL(pool_partial_kernel, dim3(L.chunks, 2 * B), dim3(256), 0, st, at<float>(ws, L.H2f), bt->seg_off,
                      L.chunks, at<float>(ws, L.part));
-  hipLaunchKernelGGL(pool_final_kernel, dim3(B), dim3(256 * kPoolGroups), 0, st, at<float>(ws, L.part), bt->seg_off,
-                     L.chunks, bt->eps, bt->eps_rel, sig2, mom, bt->d, at<float>(ws, L.ctx));
-  hipLaunchKernelGGL(ctx_forward_kernel, dim3((256 + P.kin + 3) / 4, B), dim3(256), 0, st, prm + P.Wc, prm + P.bh,
-                     prm + P.Wq, at<float>(ws, L.ctx), bt->d, P.kin, at<float>(ws, L.hctx), at<float>(ws, L.zq));
+  hipLaunchKernelGGL(ctx_forward_kernel, dim3((256 + P.kin + 3) / 4, B), dim3(256), 0, st, at<float>(ws, L.part),
+                     bt->seg_off, L.chunks, bt->eps, bt->eps_rel, sig2, mom, bt->d, prm + P.Wc, prm + P.bh,
+                     prm + P.Wq, P.kin, at<float>(ws, L.ctx), at<float>(ws, L.hctx), at<float>(ws, L.zq));
   DS_LAUNCHED();
   if (!training) {
     // inference: the head in one launch
