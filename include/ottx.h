@@ -215,6 +215,15 @@ int ottx_pointcloud_op(const ottx_problem* problem, int32_t op, int32_t axis,
                        double eps, float* out, void* workspace,
                        size_t workspace_bytes, void* stream);
 
+/* Cost maxima behind PointCloud's scale_cost modes (pointcloud.py:342-383):
+ * out_host[0] = max_i |x_i|^2, out_host[1] = max_j |y_j|^2 (cost_fn.norm of the
+ * squared-Euclidean cost, for "max_norm" and "max_bound") and, when
+ * with_max_cost, out_host[2] = max_ij |x_i - y_j|^2 (the unscaled cost, for
+ * "max_cost"; one pass over the pairs).  workspace: at least 256 bytes,
+ * 256-byte aligned.  Blocks until the values are on the host. */
+int ottx_cost_maxima(int64_t n, int64_t m, int32_t d, const float* x, const float* y, int32_t with_max_cost,
+                     double* out_host, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Dense transport matrix from potentials, geometry.py:361-365
  * (transport_from_potentials): out[i * m + j] = exp((f_i + g_j - C_ij) / eps)
  * with C_ij = inv_scale_cost * |x_i - y_j|^2 summed from the differences in

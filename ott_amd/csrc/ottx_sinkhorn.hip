@@ -719,6 +719,63 @@ __global__ void dual_final_kernel(const double* __restrict__ part, const float* 
   out[7] = sc->eps;
 }
 
+// Cost maxima for the scale_cost modes: the largest squared norm of a cloud
+// and the largest squared distance of a pair, as float bits (non-negative
+// floats order as their bits) combined with one atomicMax per workgroup.
+__global__ void zero_words_kernel(uint32_t* p, int count) {
+  if (static_cast<int>(threadIdx.x) < count) p[threadIdx.x] = 0u;
+}
+
+__device__ __forceinline__ void block_max_store(float v, uint32_t* out) {
+  __shared__ float red[4];
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(out, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+}
+
+__global__ __launch_bounds__(256) void max_sq_norm_kernel(const float* __restrict__ p, int64_t np, int d,
+                                                          uint32_t* out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  float v = 0.f;
+  if (i < np) {
+    const float* r = p + i * d;
+    float a0 = 0.f, a1 = 0.f;
+    int k = 0;
+    for (; k + 1 < d; k += 2) {
+      a0 = fmaf(r[k], r[k], a0);
+      a1 = fmaf(r[k + 1], r[k + 1], a1);
+    }
+    if (k < d) a0 = fmaf(r[k], r[k], a0);
+    v = a0 + a1;
+  }
+  block_max_store(v, out);
+}
+
+__global__ __launch_bounds__(256) void max_cost_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                       int64_t n, int64_t m, int d, uint32_t* out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.y) * 4 + (threadIdx.x >> 6);
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * 64 + (threadIdx.x & 63);
+  float v = 0.f;
+  if (i < n && j < m) {
+    const float* xi = x + i * d;
+    const float* yj = y + j * d;
+    float c0 = 0.f, c1 = 0.f;
+    int k = 0;
+    for (; k + 1 < d; k += 2) {
+      const float u = xi[k] - yj[k], w = xi[k + 1] - yj[k + 1];
+      c0 = fmaf(u, u, c0);
+      c1 = fmaf(w, w, c1);
+    }
+    if (k < d) {
+      const float u = xi[k] - yj[k];
+      c0 = fmaf(u, u, c0);
+    }
+    v = c0 + c1;
+  }
+  block_max_store(v, out);
+}
+
 // Dense transport matrix exp((f_i + g_j - s |x_i - y_j|^2) / eps), row-major
 // [n, m]: four rows by 64 columns per workgroup, one entry per thread, the
 // squared distance summed from the differences.
@@ -1379,11 +1436,40 @@ int ottx_mean_cost(int64_t n, int64_t m, int32_t d, const float* x, const float*
                                         align_up(static_cast<size_t>(cy) * d * 16));
   hipLaunchKernelGGL(stats_partial_kernel, dim3(cx), dim3(256), 0, st, x, n, d, rx, sx);
   hipLaunchKernelGGL(stats_partial_kernel, dim3(cy), dim3(256), 0, st, y, m, d, ry, sy);
-  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(kFinalThreads), 0, st, sx, cx, sy, cy, d, n, m, inv_scale_cost, o,
-                     static_cast<float*>(nullptr), static_cast<SolveScalars*>(nullptr), 0.0, 0.0, 1.0, 1.0);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(kFinalThreads), 0, st, sx, cx, sy, cy, d, n, m,
+                     inv_scale_cost, o, static_cast<float*>(nullptr), static_cast<SolveScalars*>(nullptr), 0.0, 0.0,
+                     1.0, 1.0);
   OTTX_LAUNCHED();
   OTTX_HIP(hipMemcpyAsync(out_host, o, sizeof(double), hipMemcpyDeviceToHost, st));
   OTTX_HIP(hipStreamSynchronize(st));
+  return OTTX_OK;
+}
+
+int ottx_cost_maxima(int64_t n, int64_t m, int32_t d, const float* x, const float* y, int32_t with_max_cost,
+                     double* out_host, void* workspace, size_t workspace_bytes, void* stream) {
+  if (n <= 0 || m <= 0 || d <= 0 || !x || !y || !out_host || !workspace) return OTTX_EINVAL;
+  if (workspace_bytes < 256) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  uint32_t* bits = static_cast<uint32_t*>(workspace);
+  hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, st, bits, 4);
+  hipLaunchKernelGGL(max_sq_norm_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, x, n, d,
+                     bits + 0);
+  hipLaunchKernelGGL(max_sq_norm_kernel, dim3(static_cast<unsigned>((m + 255) / 256)), dim3(256), 0, st, y, m, d,
+                     bits + 1);
+  if (with_max_cost) {
+    const dim3 grid(static_cast<unsigned>((m + 63) / 64), static_cast<unsigned>((n + 3) / 4));
+    hipLaunchKernelGGL(max_cost_kernel, grid, dim3(256), 0, st, x, y, n, m, d, bits + 2);
+  }
+  OTTX_LAUNCHED();
+  uint32_t host[4];
+  OTTX_HIP(hipMemcpyAsync(host, bits, sizeof(host), hipMemcpyDeviceToHost, st));
+  OTTX_HIP(hipStreamSynchronize(st));
+  float v[3];
+  std::memcpy(v, host, sizeof(v));
+  out_host[0] = v[0];
+  out_host[1] = v[1];
+  out_host[2] = with_max_cost ? static_cast<double>(v[2]) : 0.0;
   return OTTX_OK;
 }
 

@@ -11,6 +11,7 @@ materialised: every operator is one fused half-step launch.
 from __future__ import annotations
 
 import ctypes
+import math
 from typing import Any, Literal, Optional, Tuple, Union
 
 import numpy as np
@@ -82,6 +83,7 @@ class PointCloud:
     self.precision = precision
     self._eps_cache: Optional[eps_scheduler.Epsilon] = None
     self._mean_cache: Optional[float] = None
+    self._inv_scale_cache: Optional[float] = None
 
   # -- shape / dtype ---------------------------------------------------------
   @property
@@ -114,15 +116,53 @@ class PointCloud:
   # -- cost scaling and epsilon ---------------------------------------------
   @property
   def inv_scale_cost(self) -> float:
-    """pointcloud.py:343-383; float scales only."""
+    """pointcloud.py:342-383.
+
+    Floats scale directly.  "mean" uses the closed-form mean of the unscaled
+    cost, "max_norm" / "max_bound" the largest squared norms and "max_cost" the
+    largest pair cost, each computed once on the GPU and kept.  "median" needs
+    the materialised cost matrix, which the reference refuses in online mode
+    too, and raises.
+    """
     s = self._scale_cost
     if isinstance(s, (int, float, np.floating)) and not isinstance(s, bool):
       if not float(s) > 0.0:
         raise ValueError(f"scale_cost must be positive, got {s}")
       return 1.0 / float(s)
-    if isinstance(s, str) and s in ("mean", "max_cost", "median", "max_norm", "max_bound"):
-      raise NotImplementedError(f"scale_cost={s!r} is not supported by ott_amd yet")
+    if isinstance(s, str) and s in ("mean", "max_cost", "max_norm", "max_bound"):
+      if self._inv_scale_cache is None:
+        self._inv_scale_cache = 1.0 / self._cost_summary(s)
+      return self._inv_scale_cache
+    if s == "median":
+      raise NotImplementedError("scale_cost='median' needs the materialised cost matrix "
+                                "(the reference does not run it online either)")
     raise ValueError(f"Scaling {s} not implemented.")
+
+  def _cost_summary(self, mode: str) -> float:
+    """The unscaled quantity a scale_cost mode divides by (pointcloud.py:343-377)."""
+    lib = _lib.load_library()
+    n, m = self.shape
+    d = self.x.shape[1]
+    if mode == "mean":
+      nbytes = lib.ottx_mean_cost_workspace_bytes(n, m, d)
+      ws = _lib.workspace(nbytes, self.device)
+      out = ctypes.c_double(0.0)
+      _lib.check("ottx_mean_cost", lib.ottx_mean_cost(
+          n, m, d, self.x.data_ptr(), self.y.data_ptr(), 1.0, ctypes.byref(out), ws.data_ptr(), ws.numel(),
+          _lib.stream_ptr()))
+      return float(out.value)
+    ws = _lib.workspace(256, self.device)
+    out = (ctypes.c_double * 3)()
+    _lib.check("ottx_cost_maxima", lib.ottx_cost_maxima(
+        n, m, d, self.x.data_ptr(), self.y.data_ptr(), 1 if mode == "max_cost" else 0, out, ws.data_ptr(),
+        ws.numel(), _lib.stream_ptr()))
+    x_max, y_max, c_max = float(out[0]), float(out[1]), float(out[2])
+    if mode == "max_cost":
+      return c_max
+    if mode == "max_norm":
+      return max(x_max, y_max)
+    # max_bound for the squared-Euclidean cost
+    return x_max + y_max + 2.0 * math.sqrt(x_max * y_max)
 
   @property
   def mean_cost_matrix(self) -> float:

@@ -6,6 +6,8 @@ bf16-split path rel 1e-3 (on potentials, relative to their scale, and on
 reg_ot_cost).
 """
 
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -293,3 +295,31 @@ def test_recenter_potentials_unbalanced_one_side_is_a_no_op():
   np.testing.assert_array_equal(out.f.cpu().numpy(), plain.f.cpu().numpy())
   np.testing.assert_array_equal(out.g.cpu().numpy(), plain.g.cpu().numpy())
   assert out.reg_ot_cost == plain.reg_ot_cost
+
+
+@pytest.mark.parametrize("mode", ["mean", "max_norm", "max_bound", "max_cost"])
+def test_scale_cost_modes(mode):
+  """PointCloud(scale_cost=...) string modes (pointcloud.py:342-383): the
+  inverse scale against numpy in float64, then a solve on the scaled cost
+  against the oracle given the same scale as a float."""
+  x, y, a, b = _cloud(333, 257, 6, seed=17, shift=0.5)
+  geom = _geom(x, y, "fp32", epsilon=0.05, relative_epsilon="mean", scale_cost=mode)
+  x64, y64 = x.astype(np.float64), y.astype(np.float64)
+  nx, ny = (x64 ** 2).sum(1).max(), (y64 ** 2).sum(1).max()
+  ref = {
+      "mean": ((x64[:, None, :] - y64[None]) ** 2).sum(-1).mean(),
+      "max_norm": max(nx, ny),
+      "max_bound": nx + ny + 2.0 * np.sqrt(nx * ny),
+      "max_cost": ((x64[:, None, :] - y64[None]) ** 2).sum(-1).max(),
+  }[mode]
+  assert math.isclose(1.0 / geom.inv_scale_cost, ref, rel_tol=1e-5), (1.0 / geom.inv_scale_cost, ref)
+  out = _solve(x, y, a, b, "fp32", dict(epsilon=0.05, relative_epsilon="mean", scale_cost=mode))
+  ref_out = oracle.solve_pointcloud(x, y, a, b, epsilon=0.05, relative_epsilon="mean",
+                                    scale_cost=1.0 / geom.inv_scale_cost)
+  _check_solve(out, ref_out, "fp32")
+
+
+def test_scale_cost_median_is_refused():
+  x, y, _, _ = _cloud(20, 30, 3, seed=1)
+  with pytest.raises(NotImplementedError):
+    _ = _geom(x, y, "fp32", epsilon=0.1, scale_cost="median").inv_scale_cost
