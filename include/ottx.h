@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define OTTX_ABI_VERSION 1
+#define OTTX_ABI_VERSION 2
 
 enum ottx_status {
   OTTX_OK = 0,
@@ -111,6 +111,13 @@ typedef struct ottx_sinkhorn_opts {
                                iteration is one pass: g <- (1 - w) g + w g^(f), then f <- g; the
                                error, row plus column marginal (sinkhorn.py:160-189), is twice the
                                column one.  Other problems return OTTX_EUNSUPPORTED. */
+  double epsilon_init;      /* epsilon schedule (epsilon_scheduler.py:24-60): iteration t uses
+                               max(epsilon_init epsilon_decay^t, 1) epsilon; the marginal error
+                               and the outputs use epsilon itself.  Values <= 1 (0 included) run
+                               at epsilon throughout.  A schedule (epsilon_init > 1) is supported
+                               for balanced problems without parallel updates; others return
+                               OTTX_EUNSUPPORTED, and so does the batched solve. */
+  double epsilon_decay;     /* in (0, 1]; 0 is read as 1 */
 } ottx_sinkhorn_opts;
 
 /* Outputs of one solve (SinkhornOutput fields, sinkhorn.py:280-323). */

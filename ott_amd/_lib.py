@@ -52,6 +52,10 @@ class Problem(ctypes.Structure):
   ]
 
 
+# OTTX_ABI_VERSION of include/ottx.h that these structures mirror
+ABI_VERSION = 2
+
+
 class SinkhornOpts(ctypes.Structure):
   _fields_ = [
       ("threshold", ctypes.c_double),
@@ -65,6 +69,8 @@ class SinkhornOpts(ctypes.Structure):
       ("init_mode", ctypes.c_int32),
       ("recenter_potentials", ctypes.c_int32),
       ("parallel_dual_updates", ctypes.c_int32),
+      ("epsilon_init", ctypes.c_double),
+      ("epsilon_decay", ctypes.c_double),
   ]
 
 
@@ -223,6 +229,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
       fn = getattr(lib, name)
       fn.restype = restype
       fn.argtypes = argtypes
+    if lib.ottx_abi_version() != ABI_VERSION:
+      raise RuntimeError(f"{path} has ABI version {lib.ottx_abi_version()}, these bindings expect "
+                         f"{ABI_VERSION}; rebuild it")
     _lib = lib
     return lib
 

@@ -95,6 +95,11 @@ struct HalfStepArgs {
   float momentum;       // constant momentum weight
   int32_t balanced;
   int32_t symmetric;    // symmetric problem with parallel updates: error = 2 x column error
+  int32_t eps_sched;    // epsilon schedule: eps_t = max(eps_init eps_decay^t, 1) eps
+  int32_t err_pass;     // the schedule's error pass (target eps, after the g-update)
+  int32_t it_offset;    // iteration of this launch = state->block * inner + it_offset
+  double eps_init;
+  double eps_decay;
   int32_t side;         // 1: out = y (tau_b), 0: out = x (tau_a)
   const SolveScalars* scal;  // when set, overrides c, eps, eps_marg and inv_rho
   const float* lse_shift;    // per-column log-sum-exp (acc units) of the previous same-side pass

@@ -575,6 +575,22 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
     eps = eps_marg = p.scal->eps_f;
     inv_rho = p.side ? p.scal->inv_rho_b : p.scal->inv_rho_a;
   }
+  uint32_t mode = p.mode;
+  if (p.eps_sched) {
+    // epsilon schedule (epsilon_scheduler.py:47-60): this launch's iteration
+    // t from the block index, eps_t = max(init decay^t, 1) eps.  While eps_t >
+    // eps the marginal error, taken at the target eps, cannot come from this
+    // pass: the error pass that follows the g-update measures it instead.
+    const int t = p.state->block * p.inner + p.it_offset;
+    const double scale = fmax(p.eps_init * pow(p.eps_decay, t), 1.0);
+    if (p.err_pass) {
+      if (!(scale > 1.0)) return;  // the g-update measured it
+    } else {
+      if (scale > 1.0) mode &= ~(kModeErrCol | kModeErrRow);
+      eps = static_cast<float>(static_cast<double>(eps) * scale);
+      c = static_cast<float>(static_cast<double>(c) / scale);
+    }
+  }
 
   OutOperand<PREC, KK, NB> B;
   B.load(p.out_frag, static_cast<int64_t>(tile * W + wave) * NB, lane);
@@ -668,8 +684,8 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
     }
     if (tid == 0 && p.state != nullptr) p.state->shift_ok[p.side] = 1;
   }
-  const EpiScalars es{p.mode, p.norm_p, p.two_s, eps, eps_marg, p.tau, inv_rho, p.momentum};
-  const bool want_err = (p.mode & (kModeErrCol | kModeErrRow)) != 0;
+  const EpiScalars es{mode, p.norm_p, p.two_s, eps, eps_marg, p.tau, inv_rho, p.momentum};
+  const bool want_err = (mode & (kModeErrCol | kModeErrRow)) != 0;
   double e_acc = 0.0, m_acc = 0.0;
   for (int t = tid; t < COLS; t += NT) {
     const int64_t j = col0 + t;
@@ -684,7 +700,7 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
       p.err_partial[tile] = s_red[0][0];
       if (p.mass_partial) p.mass_partial[tile] = s_red[1][0];
     }
-    if ((p.mode & kModeErrCol) && p.state != nullptr) {
+    if ((mode & kModeErrCol) && p.state != nullptr) {
       if (tid == 0) {
         release_to_device();
         const uint32_t prev = atomicAdd(&p.state->tiles_done, 1u);

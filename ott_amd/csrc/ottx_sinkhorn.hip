@@ -1593,6 +1593,13 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   if (symmetric && (pb->x != pb->y || pb->n != pb->m || pb->a != pb->b || pb->tau_a != 1.0 ||
                     pb->tau_b != 1.0 || o->init_mode != OTTX_INIT_ZEROS || o->recenter_potentials))
     return OTTX_EUNSUPPORTED;
+  // epsilon schedule (epsilon_scheduler.py:47-60): balanced problems with
+  // sequential updates
+  const double eps_init = o->epsilon_init > 1.0 ? o->epsilon_init : 1.0;
+  const double eps_decay = o->epsilon_decay > 0.0 ? o->epsilon_decay : 1.0;
+  if (!(eps_decay <= 1.0) || !(eps_init < 1e30)) return OTTX_EINVAL;
+  const bool sched = eps_init > 1.0;
+  if (sched && (symmetric || pb->tau_a != 1.0 || pb->tau_b != 1.0)) return OTTX_EUNSUPPORTED;
   char* base = static_cast<char*>(workspace);
   SolveState* state = reinterpret_cast<SolveState*>(base + L.state);
   uint32_t* counters = reinterpret_cast<uint32_t*>(base + L.counters);
@@ -1648,9 +1655,34 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   f_err.mode = kModeWritePot | kModeCheckDone | (balanced ? 0u : kModeErrRow);
 
   if (symmetric) g_err.symmetric = 1;
+  // With a schedule every launch works out its iteration t = block * inner +
+  // it_offset on the device, and the g-update that ends a block measures the
+  // error only once eps_t is the target; before that a pass at the target eps
+  // (no potential written: h_old is g_t, saved by that g-update) measures it.
+  // The g-update after the f-update of iteration it belongs to iteration it + 1.
+  HalfStepArgs g_meas = ga;
+  g_meas.mode = kModeCheckDone | kModeErrCol;
+  g_meas.out_pot = g_save;
+  g_meas.lse_shift = g_meas.lse_store = nullptr;
+  g_meas.err_pass = 1;
+  g_meas.it_offset = inner;
+  for (HalfStepArgs* a : {&g_plain, &g_err, &f_plain, &f_err, &g_meas}) {
+    a->eps_sched = sched ? 1 : 0;
+    a->eps_init = eps_init;
+    a->eps_decay = eps_decay;
+  }
   auto launch_block = [&](hipStream_t s) -> int {
     for (int it = 0; it < inner; ++it) {
       int r;
+      if (sched) {
+        HalfStepArgs fa_it = it == inner - 1 ? f_err : f_plain;
+        HalfStepArgs ga_it = it == inner - 1 ? g_err : g_plain;
+        fa_it.it_offset = it;
+        ga_it.it_offset = it + 1;
+        if ((r = launch_halfstep(k, fa_it, s)) != OTTX_OK) return r;
+        if ((r = launch_halfstep(k, ga_it, s)) != OTTX_OK) return r;
+        continue;
+      }
       if (symmetric) {
         // f <- g: the f-update of the parallel step is the g-update by symmetry
         hipLaunchKernelGGL(copy_potential_kernel, dim3(static_cast<unsigned>((pb->n + 255) / 256)), dim3(256), 0, s,
@@ -1663,7 +1695,7 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
       r = launch_halfstep(k, it == inner - 1 ? g_err : g_plain, s);
       if (r != OTTX_OK) return r;
     }
-    return OTTX_OK;
+    return sched ? launch_halfstep(k, g_meas, s) : OTTX_OK;
   };
 
   if (o->init_mode == OTTX_INIT_F_FROM_G) {
@@ -2366,7 +2398,9 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
   if (!out || !out->f || !out->g || !out->errors || !out->reg_ot_cost || !out->converged || !out->n_iters ||
       !workspace)
     return OTTX_EINVAL;
-  if (o->init_mode == OTTX_INIT_F_FROM_G || o->recenter_potentials) return OTTX_EUNSUPPORTED;
+  if (o->init_mode == OTTX_INIT_F_FROM_G || o->recenter_potentials || o->parallel_dual_updates ||
+      o->epsilon_init > 1.0)
+    return OTTX_EUNSUPPORTED;
   if (o->init_mode < 0 || o->init_mode > 3) return OTTX_EINVAL;
   const int inner = o->inner_iterations;
   const int outer = (o->max_iterations + inner - 1) / inner;

@@ -229,6 +229,13 @@ class PointCloud:
     init = self._epsilon_init
     return not isinstance(init, eps_scheduler.Epsilon) or init.is_target_only
 
+  def _schedule(self) -> Tuple[float, float]:
+    """(init, decay) of the epsilon schedule, (1, 1) for a fixed epsilon."""
+    init = self._epsilon_init
+    if isinstance(init, eps_scheduler.Epsilon) and not init.is_target_only:
+      return float(init.init), float(init.decay)
+    return 1.0, 1.0
+
   # -- divergences ------------------------------------------------------------
   @classmethod
   def prepare_divergences(cls, x, y, static_b: bool = False, **kwargs: Any) -> Tuple["PointCloud", ...]:

@@ -205,8 +205,11 @@ class Sinkhorn:
     self._check_options()
     if not isinstance(ot_prob.geom, PointCloud):
       raise NotImplementedError("ott_amd accelerates PointCloud geometries only")
-    if not ot_prob.geom._schedule_is_target_only():
-      raise NotImplementedError("epsilon schedules with init > 1 are not supported yet")
+    if not ot_prob.geom._schedule_is_target_only() and (
+        ot_prob.tau_a != 1.0 or ot_prob.tau_b != 1.0 or self.parallel_dual_updates):
+      # the schedule runs on balanced problems with sequential updates
+      raise NotImplementedError("epsilon schedules (init > 1) are supported for balanced problems "
+                                "without parallel_dual_updates only")
     if self.recenter_potentials and ot_prob.tau_a < 1.0 and ot_prob.tau_b < 1.0:
       # the recentring terms of lse_step (sinkhorn.py:809-854) are not implemented;
       # balanced problems and those with one tau = 1 are supported
@@ -299,6 +302,7 @@ def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
     raise ValueError(f"init potentials must be [{n}] and [{m}]")
   a, b = ot_prob._a, ot_prob._b  # None: uniform, set inside the library
   eps_abs, eps_rel = geom._epsilon_spec()
+  sched_init, sched_decay = geom._schedule()
   pb, keep = geom._problem(a=a, b=b, tau_a=ot_prob.tau_a, tau_b=ot_prob.tau_b,
                            epsilon=eps_abs, relative_epsilon=eps_rel)
   opts = _lib.SinkhornOpts(
@@ -311,7 +315,8 @@ def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
       use_graph=1 if solver.use_graph else 0,
       init_mode=int(init_mode),
       recenter_potentials=1 if solver.recenter_potentials else 0,
-      parallel_dual_updates=1 if solver.parallel_dual_updates else 0)
+      parallel_dual_updates=1 if solver.parallel_dual_updates else 0,
+      epsilon_init=sched_init, epsilon_decay=sched_decay)
   errors = torch.empty((solver.outer_iterations,), dtype=torch.float32, device=geom.device)
   out = _lib.SinkhornOut(f=f.data_ptr(), g=g.data_ptr(), errors=errors.data_ptr())
   nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts))

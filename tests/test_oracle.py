@@ -326,3 +326,33 @@ def test_oracle_transport_ops_against_reg_ot_cost():
   w = rng.normal(size=25)
   np.testing.assert_allclose(oracle.apply_transport_from_potentials(cost, out.f, out.g, w, eps, axis=1),
                              p @ w, rtol=1e-12)
+
+
+def test_oracle_epsilon_schedule():
+  """Iteration t of a scheduled solve runs at max(init decay^t, 1) eps
+  (epsilon_scheduler.py:47-60) and errors are taken at eps: one iteration
+  equals an unscheduled iteration at 4 eps, two equal that followed by one at
+  2 eps, and once the schedule reaches eps the solve ends at the unscheduled
+  fixed point."""
+  x, y, a, b = _problem(40, 30, 3, seed=11)
+  cost = oracle.sqeuclidean_cost(x, y)
+  ops = oracle.DenseLSE(cost)
+  eps = 0.05
+  kw = dict(inner_iterations=1, threshold=0.0)
+  one = oracle.sinkhorn(ops, a, b, eps, min_iterations=1, max_iterations=1, eps_init=4.0, eps_decay=0.5, **kw)
+  big = oracle.sinkhorn(ops, a, b, 4 * eps, min_iterations=1, max_iterations=1, **kw)
+  np.testing.assert_array_equal(one.f, big.f)
+  np.testing.assert_array_equal(one.g, big.g)
+  want_err = oracle.solution_error(ops, one.f, one.g, a, b, eps, 1.0, 1.0, (1,), False)[0]
+  assert one.errors[0] == want_err
+  assert one.errors[0] != big.errors[0]
+  two = oracle.sinkhorn(ops, a, b, eps, min_iterations=2, max_iterations=2, eps_init=4.0, eps_decay=0.5, **kw)
+  step = oracle.sinkhorn(ops, a, b, 2 * eps, min_iterations=1, max_iterations=1, init=(big.f, big.g), **kw)
+  np.testing.assert_allclose(two.f, step.f, rtol=0, atol=1e-14)
+  np.testing.assert_allclose(two.g, step.g, rtol=0, atol=1e-14)
+  sched = oracle.sinkhorn(ops, a, b, eps, threshold=1e-10, inner_iterations=5, max_iterations=5000,
+                          eps_init=20.0, eps_decay=0.7)
+  plain = oracle.sinkhorn(ops, a, b, eps, threshold=1e-10, inner_iterations=5, max_iterations=5000)
+  assert sched.converged and plain.converged
+  assert math.isclose(sched.reg_ot_cost, plain.reg_ot_cost, rel_tol=1e-9)
+  np.testing.assert_allclose(sched.f - sched.f[0], plain.f - plain.f[0], atol=1e-7)
