@@ -118,6 +118,15 @@ typedef struct ottx_sinkhorn_opts {
                                for balanced problems without parallel updates; others return
                                OTTX_EUNSUPPORTED, and so does the batched solve. */
   double epsilon_decay;     /* in (0, 1]; 0 is read as 1 */
+  int32_t momentum_start;   /* 0: constant momentum (the weight above).  > 0: Momentum(start)
+                               (acceleration.py:138-160): iterations from momentum_start on use
+                               Lehmann's weight 2 / (1 + sqrt(1 - r^(1 / inner))),
+                               r = min(err[k-1] / err[k-2], 0.99), k = momentum_start /
+                               inner_iterations, if err[k-1] < momentum_error_threshold.  Needs
+                               norm_error = 1 and momentum_start >= 2 inner_iterations; not with
+                               parallel updates or in the batched solve (OTTX_EUNSUPPORTED). */
+  int32_t _pad0;
+  double momentum_error_threshold; /* Momentum.error_threshold (inf when unused) */
 } ottx_sinkhorn_opts;
 
 /* Outputs of one solve (SinkhornOutput fields, sinkhorn.py:280-323). */

@@ -33,6 +33,7 @@ __all__ = [
     "std_cost",
     "resolve_epsilon",
     "epsilon_at",
+    "momentum_weight",
     "DenseLSE",
     "apply_lse_kernel",
     "update_potential",
@@ -331,6 +332,23 @@ class OracleOutput:
   inner_iterations: int
 
 
+def momentum_weight(errors, it, value, start, error_threshold, inner_iterations):
+  """Momentum.weight and Momentum.lehmann (acceleration.py:138-160): ``value``
+  before iteration ``start`` (or when start = 0), then, if the error of block
+  start // inner - 1 is below ``error_threshold``, Lehmann's weight
+  2 / (1 + sqrt(1 - r^(1 / inner))) with r = min(err[idx-1] / err[idx-2], 0.99)
+  in float32, as the reference evaluates it on its float32 errors."""
+  if start == 0 or it < start:
+    return value
+  idx = start // inner_iterations
+  e1 = np.float32(errors[idx - 1])
+  if not e1 < error_threshold:
+    return value
+  ratio = np.minimum(e1 / np.float32(errors[idx - 2]), np.float32(0.99))
+  power = np.float32(1.0 / inner_iterations)
+  return float(np.float32(2.0) / (np.float32(1.0) + np.sqrt(np.float32(1.0) - ratio ** power)))
+
+
 def sinkhorn(
     ops,
     a: np.ndarray,
@@ -350,6 +368,8 @@ def sinkhorn(
     init: Optional[Tuple[np.ndarray, np.ndarray]] = None,
     eps_init: float = 1.0,
     eps_decay: float = 1.0,
+    momentum_start: int = 0,
+    momentum_error_threshold: float = np.inf,
     progress_fn: Optional[Callable] = None,
     dtype=np.float64,
 ) -> OracleOutput:
@@ -361,8 +381,8 @@ def sinkhorn(
     stop test between blocks, ``lax.scan`` when min == max);
   - one iteration: sinkhorn.py:879-938 with ``lse_step`` sinkhorn.py:803-857
     (recentering terms omitted: only used when ``recenter_potentials`` and both
-    taus < 1, which this oracle rejects) and constant ``Momentum``
-    (acceleration.py:161-172);
+    taus < 1, which this oracle rejects) and ``Momentum``, constant or
+    adaptive (acceleration.py:138-172);
   - termination: sinkhorn.py:940-962; outputs: sinkhorn.py:974-1029, n_iters
     sinkhorn.py:431-434; cost: sinkhorn.py:224-277 through ``run``
     sinkhorn.py:1052-1062.
@@ -388,7 +408,8 @@ def sinkhorn(
 
   def one_iteration(it, f, g, compute_error):
     eps_it = dtype(epsilon_at(float(epsilon), it, eps_init, eps_decay))
-    w = dtype(momentum)
+    w = dtype(momentum_weight(errors, it, momentum, momentum_start, momentum_error_threshold,
+                              inner_iterations))
     old_f, old_g = f, g
     new_g = tau_b * update_potential(ops, old_f, old_g, log_b, eps_it, axis=0)
     with np.errstate(invalid="ignore"):

@@ -71,6 +71,9 @@ class SinkhornOpts(ctypes.Structure):
       ("parallel_dual_updates", ctypes.c_int32),
       ("epsilon_init", ctypes.c_double),
       ("epsilon_decay", ctypes.c_double),
+      ("momentum_start", ctypes.c_int32),
+      ("_pad0", ctypes.c_int32),
+      ("momentum_error_threshold", ctypes.c_double),
   ]
 
 

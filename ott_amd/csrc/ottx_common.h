@@ -100,6 +100,8 @@ struct HalfStepArgs {
   int32_t it_offset;    // iteration of this launch = state->block * inner + it_offset
   double eps_init;
   double eps_decay;
+  int32_t mom_start;    // adaptive momentum from this iteration on (0: constant momentum)
+  float mom_threshold;  // ... if the error of block mom_start / inner - 1 is below this
   int32_t side;         // 1: out = y (tau_b), 0: out = x (tau_a)
   const SolveScalars* scal;  // when set, overrides c, eps, eps_marg and inv_rho
   const float* lse_shift;    // per-column log-sum-exp (acc units) of the previous same-side pass

@@ -65,6 +65,21 @@ __device__ __forceinline__ float2 load_f2_coherent(const float2* p) {
   return r;
 }
 
+// Momentum.weight / Momentum.lehmann (acceleration.py:138-160) once iteration
+// start is reached: the constant value unless the error of block
+// idx - 1 (idx = start / inner) is below the threshold, then
+// 2 / (1 + sqrt(1 - r^(1 / inner))), r = min(err[idx - 1] / err[idx - 2], 0.99),
+// in float32 like the reference's errors.  start >= 2 inner (checked by the host).
+__device__ inline float adaptive_momentum(const float* errors, int start, int inner, float value,
+                                          float threshold) {
+  const int idx = start / inner;
+  const float e1 = errors[idx - 1];
+  if (!(e1 < threshold)) return value;
+  float r = e1 / errors[idx - 2];
+  if (r > 0.99f) r = 0.99f;  // NaN stays NaN, as with jnp.minimum
+  return 2.f / (1.f + sqrtf(1.f - powf(r, 1.f / static_cast<float>(inner))));
+}
+
 // ---------------------------------------------------------------------------
 // Block finalizer: sinkhorn.py:914-931 (errors layout), :940-953 and
 // fixed_point_loop.py:63-70 (continue test), run by one thread of the last
@@ -591,6 +606,20 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
       c = static_cast<float>(static_cast<double>(c) / scale);
     }
   }
+  float momentum = p.momentum;
+  if (p.mom_start > 0) {
+    // adaptive momentum: iterations from mom_start on use Lehmann's weight,
+    // taken from the errors of blocks idx - 2 and idx - 1 (idx = mom_start /
+    // inner).  The g-update ending block idx - 1 is itself iteration
+    // mom_start when inner divides it; its block's error is not final yet, so
+    // it writes the plain update and momentum_fixup_kernel blends it after
+    // the block has been finalized.
+    const int t = p.state->block * p.inner + p.it_offset;
+    if (t >= p.mom_start)
+      momentum = (p.mom_start / p.inner - 1 == p.state->block)
+                     ? 1.f
+                     : adaptive_momentum(p.errors, p.mom_start, p.inner, p.momentum, p.mom_threshold);
+  }
 
   OutOperand<PREC, KK, NB> B;
   B.load(p.out_frag, static_cast<int64_t>(tile * W + wave) * NB, lane);
@@ -684,7 +713,7 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
     }
     if (tid == 0 && p.state != nullptr) p.state->shift_ok[p.side] = 1;
   }
-  const EpiScalars es{mode, p.norm_p, p.two_s, eps, eps_marg, p.tau, inv_rho, p.momentum};
+  const EpiScalars es{mode, p.norm_p, p.two_s, eps, eps_marg, p.tau, inv_rho, momentum};
   const bool want_err = (mode & (kModeErrCol | kModeErrRow)) != 0;
   double e_acc = 0.0, m_acc = 0.0;
   for (int t = tid; t < COLS; t += NT) {

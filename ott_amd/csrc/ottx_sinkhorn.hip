@@ -810,6 +810,24 @@ __global__ void copy_potential_kernel(const float* __restrict__ src, float* __re
   if (i < n) dst[i] = src[i];
 }
 
+// Adaptive momentum, the iteration ``start`` that a block-ending g-update
+// computes when inner divides start: that launch measures the error its
+// weight depends on, so it writes the plain update (and g_save the previous
+// g); once the block has been finalized, this kernel blends them with the
+// weight (acceleration.py:161-172).  Every other launch exits at once.
+__global__ void momentum_fixup_kernel(float* __restrict__ g, const float* __restrict__ g_save, int64_t m,
+                                      const SolveState* __restrict__ st, const float* __restrict__ errors,
+                                      int start, int inner, float value, float threshold) {
+  if (st->done || st->block * inner != start) return;
+  const float w = adaptive_momentum(errors, start, inner, value, threshold);
+  if (w == 1.f) return;
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j < m) {
+    const float h_old = g_save[j];
+    g[j] = (1.f - w) * (isfinite(h_old) ? h_old : 0.f) + w * g[j];
+  }
+}
+
 // Outputs of a stopped loop, computed on the device right after the block
 // that stopped it so the host only has to read pinned memory: the float64
 // dual terms (as dual_partial_kernel + dual_final_kernel), g <- g_save, and
@@ -1600,6 +1618,13 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   if (!(eps_decay <= 1.0) || !(eps_init < 1e30)) return OTTX_EINVAL;
   const bool sched = eps_init > 1.0;
   if (sched && (symmetric || pb->tau_a != 1.0 || pb->tau_b != 1.0)) return OTTX_EUNSUPPORTED;
+  // adaptive momentum (acceleration.py:138-160) on L1 errors, from an
+  // iteration that leaves two recorded blocks before it
+  const int mom_start = o->momentum_start;
+  if (mom_start < 0) return OTTX_EINVAL;
+  const bool adaptive = mom_start > 0;
+  if (adaptive && (symmetric || o->norm_error != 1 || mom_start < 2 * inner)) return OTTX_EUNSUPPORTED;
+  const bool per_launch = sched || adaptive;
   char* base = static_cast<char*>(workspace);
   SolveState* state = reinterpret_cast<SolveState*>(base + L.state);
   uint32_t* counters = reinterpret_cast<uint32_t*>(base + L.counters);
@@ -1670,11 +1695,13 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     a->eps_sched = sched ? 1 : 0;
     a->eps_init = eps_init;
     a->eps_decay = eps_decay;
+    a->mom_start = mom_start;
+    a->mom_threshold = static_cast<float>(o->momentum_error_threshold);
   }
   auto launch_block = [&](hipStream_t s) -> int {
     for (int it = 0; it < inner; ++it) {
       int r;
-      if (sched) {
+      if (per_launch) {
         HalfStepArgs fa_it = it == inner - 1 ? f_err : f_plain;
         HalfStepArgs ga_it = it == inner - 1 ? g_err : g_plain;
         fa_it.it_offset = it;
@@ -1695,7 +1722,18 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
       r = launch_halfstep(k, it == inner - 1 ? g_err : g_plain, s);
       if (r != OTTX_OK) return r;
     }
-    return sched ? launch_halfstep(k, g_meas, s) : OTTX_OK;
+    if (sched) {
+      const int r = launch_halfstep(k, g_meas, s);
+      if (r != OTTX_OK) return r;
+    }
+    if (adaptive && mom_start % inner == 0) {
+      hipLaunchKernelGGL(momentum_fixup_kernel, dim3(static_cast<unsigned>((pb->m + 255) / 256)), dim3(256), 0, s,
+                         out->g, static_cast<const float*>(g_save), pb->m, static_cast<const SolveState*>(state),
+                         static_cast<const float*>(out->errors), mom_start, inner, static_cast<float>(o->momentum),
+                         static_cast<float>(o->momentum_error_threshold));
+      OTTX_LAUNCHED();
+    }
+    return OTTX_OK;
   };
 
   if (o->init_mode == OTTX_INIT_F_FROM_G) {
@@ -2399,7 +2437,7 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
       !workspace)
     return OTTX_EINVAL;
   if (o->init_mode == OTTX_INIT_F_FROM_G || o->recenter_potentials || o->parallel_dual_updates ||
-      o->epsilon_init > 1.0)
+      o->epsilon_init > 1.0 || o->momentum_start != 0)
     return OTTX_EUNSUPPORTED;
   if (o->init_mode < 0 || o->init_mode > 3) return OTTX_EINVAL;
   const int inner = o->inner_iterations;

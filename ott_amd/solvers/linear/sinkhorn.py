@@ -231,7 +231,12 @@ class Sinkhorn:
     if self.anderson is not None:
       raise NotImplementedError("Anderson acceleration is not supported by ott_amd")
     if not self.momentum.is_constant:
-      raise NotImplementedError("adaptive momentum is not supported by ott_amd yet")
+      # Momentum(start > 0) (acceleration.py:138-160) reads the L1 errors of the
+      # two blocks before ``start``
+      if int(self._norm_error) != 1 or self.parallel_dual_updates:
+        raise NotImplementedError("adaptive momentum is supported with norm_error=1 and sequential updates")
+      if self.momentum.start < 2 * self.inner_iterations:
+        raise NotImplementedError("adaptive momentum needs start >= 2 * inner_iterations")
     if self.progress_fn is not None:
       raise NotImplementedError("progress_fn is not supported by ott_amd yet")
     if int(self._norm_error) < 1:
@@ -316,7 +321,9 @@ def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
       init_mode=int(init_mode),
       recenter_potentials=1 if solver.recenter_potentials else 0,
       parallel_dual_updates=1 if solver.parallel_dual_updates else 0,
-      epsilon_init=sched_init, epsilon_decay=sched_decay)
+      epsilon_init=sched_init, epsilon_decay=sched_decay,
+      momentum_start=int(solver.momentum.start),
+      momentum_error_threshold=float(solver.momentum.error_threshold))
   errors = torch.empty((solver.outer_iterations,), dtype=torch.float32, device=geom.device)
   out = _lib.SinkhornOut(f=f.data_ptr(), g=g.data_ptr(), errors=errors.data_ptr())
   nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts))

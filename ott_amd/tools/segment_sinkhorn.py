@@ -92,6 +92,8 @@ def segment_sinkhorn_outputs(
     raise NotImplementedError("parallel_dual_updates is not supported by the batched solve")
   if solver.recenter_potentials:
     raise NotImplementedError("recenter_potentials is not supported by the batched solve")
+  if not solver.momentum.is_constant:
+    raise NotImplementedError("adaptive momentum is not supported by the batched solve")
   if solver.initializer is not None and not isinstance(solver.initializer, sinkhorn.init_lib.DefaultInitializer):
     raise NotImplementedError("segment_sinkhorn starts every problem from zero potentials")
   eps_abs, eps_rel = _epsilon_args(epsilon, relative_epsilon)

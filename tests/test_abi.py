@@ -43,7 +43,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
   from ott_amd import _lib
   assert ctypes.sizeof(_lib.Problem) == 8 + 8 + 4 + 4 + 4 * 8 + 5 * 8
-  assert ctypes.sizeof(_lib.SinkhornOpts) == 72
+  assert ctypes.sizeof(_lib.SinkhornOpts) == 88
   assert ctypes.sizeof(_lib.SinkhornOut) == 3 * 8 + 3 * 8 + 4 * 4
   assert ctypes.sizeof(_lib.Batch) == 4 * 4 + 4 * 8 + 3 * 8 + 6 * 8
   assert ctypes.sizeof(_lib.BatchOut) == 3 * 8 + 4 * 8 + 2 * 4

@@ -1,0 +1,46 @@
+"""Adaptive momentum, ``Momentum(start > 0)`` (acceleration.py:138-172), on the
+GPU solve against the FP64 oracle.
+
+From iteration ``start`` on, both updates use Lehmann's weight from the errors
+of blocks start // inner - 2 and - 1 when the latter is below
+``error_threshold``.  The cases cover ``start`` a multiple of
+``inner_iterations`` (the block-ending g-update of iteration ``start`` is
+blended after its block's error is final), a start inside a block, the graph
+path (inner 10), and a threshold that holds and one that does not.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import sinkhorn64 as oracle
+from tests.test_gpu_parity import _check_solve, _cloud, _solve
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("inner,start,error_threshold", [(2, 6, math.inf), (3, 7, math.inf), (10, 30, math.inf),
+                                                         (5, 10, 0.5), (5, 10, 0.05)])
+def test_adaptive_momentum_matches_oracle(precision, inner, start, error_threshold):
+  from ott_amd.solvers.linear import acceleration
+  x, y, a, b = _cloud(300, 257, 6, seed=12, zero_a=[4])
+  kw = dict(threshold=1e-4, inner_iterations=inner, max_iterations=2000)
+  mom = acceleration.Momentum(start=start, error_threshold=error_threshold)
+  out = _solve(x, y, a, b, precision, dict(epsilon=0.02), momentum=mom, **kw)
+  ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.02, momentum_start=start,
+                                momentum_error_threshold=error_threshold, **kw)
+  w = oracle.momentum_weight(ref.errors, start, 1.0, start, error_threshold, inner)
+  assert (w > 1.0) == (error_threshold > 0.1)
+  _check_solve(out, ref, precision)
+
+
+def test_adaptive_momentum_limits():
+  from ott_amd.solvers.linear import acceleration
+  x, y, a, b = _cloud(64, 48, 3, seed=2)
+  with pytest.raises(NotImplementedError):  # needs two recorded blocks before start
+    _solve(x, y, a, b, "fp32", dict(epsilon=0.05), inner_iterations=5, momentum=acceleration.Momentum(start=5))
+  with pytest.raises(NotImplementedError):  # the weight reads L1 errors
+    _solve(x, y, a, b, "fp32", dict(epsilon=0.05), inner_iterations=1, norm_error=2,
+           momentum=acceleration.Momentum(start=5))

@@ -356,3 +356,34 @@ def test_oracle_epsilon_schedule():
   assert sched.converged and plain.converged
   assert math.isclose(sched.reg_ot_cost, plain.reg_ot_cost, rel_tol=1e-9)
   np.testing.assert_allclose(sched.f - sched.f[0], plain.f - plain.f[0], atol=1e-7)
+
+
+def test_oracle_adaptive_momentum():
+  """Momentum(start > 0) (acceleration.py:138-160): Lehmann's weight for a
+  known error ratio; a threshold that never holds keeps the constant value;
+  otherwise the run equals the constant one up to ``start`` and, from there,
+  a constant-momentum run restarted with the weight taken from the errors of
+  blocks start // inner - 2 and - 1."""
+  w = oracle.momentum_weight(np.array([0.4, 0.1, -1.0]), 6, 1.0, 6, np.inf, 3)
+  assert math.isclose(w, 2.0 / (1.0 + math.sqrt(1.0 - 0.25 ** (1.0 / 3.0))), rel_tol=1e-6)
+  assert oracle.momentum_weight(np.array([0.4, 0.1]), 5, 1.0, 6, np.inf, 3) == 1.0
+  assert oracle.momentum_weight(np.array([0.4, 0.39999]), 6, 1.0, 6, np.inf, 3) == pytest.approx(
+      2.0 / (1.0 + math.sqrt(1.0 - 0.99 ** (1.0 / 3.0))), rel=1e-6)  # ratio capped at 0.99
+  x, y, a, b = _problem(40, 30, 3, seed=5)
+  ops = oracle.DenseLSE(oracle.sqeuclidean_cost(x, y))
+  eps, inner, start = 0.02, 2, 6
+  kw = dict(threshold=1e-13, inner_iterations=inner, max_iterations=40)
+  const = oracle.sinkhorn(ops, a, b, eps, **kw)
+  never = oracle.sinkhorn(ops, a, b, eps, momentum_start=start, momentum_error_threshold=-1.0, **kw)
+  np.testing.assert_array_equal(never.f, const.f)
+  adapt = oracle.sinkhorn(ops, a, b, eps, momentum_start=start, **kw)
+  head = oracle.sinkhorn(ops, a, b, eps, **dict(kw, max_iterations=start))
+  np.testing.assert_array_equal(adapt.errors[:start // inner], head.errors)
+  e1, e2 = np.float32(head.errors[start // inner - 1]), np.float32(head.errors[start // inner - 2])
+  w = 2.0 / (1.0 + math.sqrt(1.0 - float(min(e1 / e2, np.float32(0.99))) ** (1.0 / inner)))
+  assert 1.0 < w < 2.0
+  tail = oracle.sinkhorn(ops, a, b, eps, momentum=w, init=(head.f, head.g),
+                         **dict(kw, max_iterations=40 - start))
+  np.testing.assert_allclose(adapt.f, tail.f, rtol=0, atol=1e-9)
+  np.testing.assert_allclose(adapt.errors[start // inner:], tail.errors, rtol=1e-5)
+  assert adapt.errors[-1] < const.errors[-1]
