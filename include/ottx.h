@@ -248,6 +248,20 @@ int ottx_cost_maxima(int64_t n, int64_t m, int32_t d, const float* x, const floa
 int ottx_transport_matrix(const ottx_problem* problem, const float* f, const float* g, double eps, float* out,
                           void* stream);
 
+/* <P, C'> = sum_ij P_ij C'_ij, P_ij = exp((f_i + g_j - C_ij) / eps) of the
+ * problem's clouds and C'_ij = other_inv_scale_cost |x'_i - y'_j|^2 of another
+ * pair of clouds of the same sizes (x_other [n, d_other], y_other [m, d_other]
+ * row-major float32), or C' = C when both are NULL.  Replaces
+ * SinkhornOutput.transport_cost_at_geom and primal_cost (sinkhorn.py:345-411),
+ * which the reference evaluates through the cost's low-rank factors
+ * (pointcloud.py:298-319); here one pass over the pairs, neither matrix stored,
+ * float32 pair terms summed in float64.  Blocks until *out_host is written. */
+size_t ottx_transport_cost_workspace_bytes(int64_t n, int64_t m);
+int ottx_transport_cost(const ottx_problem* problem, const float* f, const float* g, double eps,
+                        const float* x_other, const float* y_other, int32_t d_other,
+                        double other_inv_scale_cost, double* out_host, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
 /* Full log-domain Sinkhorn solve: Sinkhorn.__call__ -> run -> iterations
  * (sinkhorn.py:782-801, 1052-1099) with the fixed-point loop of
  * math/fixed_point_loop.py:23-93, the error of sinkhorn.py:127-221 fused into

@@ -179,6 +179,11 @@ _SIGNATURES = [
     ("ottx_transport_matrix", ctypes.c_int,
      [ctypes.POINTER(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
       ctypes.c_void_p]),
+    ("ottx_transport_cost_workspace_bytes", ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int64]),
+    ("ottx_transport_cost", ctypes.c_int,
+     [ctypes.POINTER(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_int32, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t,
+      ctypes.c_void_p]),
     ("ottx_sinkhorn_workspace_bytes", ctypes.c_size_t,
      [ctypes.POINTER(Problem), ctypes.POINTER(SinkhornOpts)]),
     ("ottx_sinkhorn_batched_workspace_bytes", ctypes.c_size_t,

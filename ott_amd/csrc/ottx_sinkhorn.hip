@@ -801,6 +801,101 @@ __global__ void transport_matrix_kernel(const float* __restrict__ x, const float
   out[i * m + j] = expf((f[i] + g[j] - s * (c0 + c1)) * inv_eps);
 }
 
+// <P, C'> in one pass over the pairs (SinkhornOutput.primal_cost and
+// transport_cost_at_geom, sinkhorn.py:345-411): P_ij = exp((f_i + g_j - s
+// |x_i - y_j|^2) / eps) and C'_ij = s' |x'_i - y'_j|^2, both squared distances
+// summed from the differences in float32.  A workgroup takes 64 rows by 64
+// columns; each thread 4 x 4 pairs, with the features staged through LDS,
+// feature-major, 32 at a time.  One float64 partial per workgroup.
+constexpr int kTcTile = 64;
+constexpr int kTcChunk = 32;
+constexpr int kTcPad = kTcTile + 4;  // keeps float4 rows aligned, spreads the staging stores
+
+__device__ __forceinline__ void tc_sq_dist(const float* __restrict__ x, const float* __restrict__ y, int64_t n,
+                                           int64_t m, int d, int64_t i0, int64_t j0, float (*xs)[kTcPad],
+                                           float (*ys)[kTcPad], int tid, float (&acc)[4][4]) {
+  const int r0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
+  for (int k0 = 0; k0 < d; k0 += kTcChunk) {
+    const int kc = min(kTcChunk, d - k0);
+    __syncthreads();
+    for (int e = tid; e < kTcTile * kTcChunk; e += 256) {
+      const int r = e / kTcChunk, k = e % kTcChunk;
+      const int64_t i = i0 + r, j = j0 + r;
+      xs[k][r] = (k < kc && i < n) ? x[i * d + k0 + k] : 0.f;
+      ys[k][r] = (k < kc && j < m) ? y[j * d + k0 + k] : 0.f;
+    }
+    __syncthreads();
+    for (int k = 0; k < kc; ++k) {
+      const float4 xv = *reinterpret_cast<const float4*>(&xs[k][r0]);
+      const float4 yv = *reinterpret_cast<const float4*>(&ys[k][c0]);
+      const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
+      const float ya[4] = {yv.x, yv.y, yv.z, yv.w};
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const float u = xa[a] - ya[b];
+          acc[a][b] = fmaf(u, u, acc[a][b]);
+        }
+    }
+  }
+}
+
+template <bool SAME>
+__global__ __launch_bounds__(256) void transport_cost_kernel(
+    const float* __restrict__ x, const float* __restrict__ y, int64_t n, int64_t m, int d, float s,
+    const float* __restrict__ f, const float* __restrict__ g, float inv_eps, const float* __restrict__ xo,
+    const float* __restrict__ yo, int d_o, float s_o, double* __restrict__ partial) {
+  __shared__ __align__(16) float xs[kTcChunk][kTcPad];
+  __shared__ __align__(16) float ys[kTcChunk][kTcPad];
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  const int64_t i0 = static_cast<int64_t>(blockIdx.y) * kTcTile;
+  const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kTcTile;
+  float acc[4][4] = {}, acc_o[4][4] = {};
+  tc_sq_dist(x, y, n, m, d, i0, j0, xs, ys, tid, acc);
+  if constexpr (!SAME) tc_sq_dist(xo, yo, n, m, d_o, i0, j0, xs, ys, tid, acc_o);
+  const int r0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
+  float part = 0.f;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t i = i0 + r0 + a;
+    if (i >= n) continue;
+    const float fi = f[i];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int64_t j = j0 + c0 + b;
+      if (j >= m) continue;
+      const float p = expf((fi + g[j] - s * acc[a][b]) * inv_eps);
+      const float c = SAME ? s * acc[a][b] : s_o * acc_o[a][b];
+      if (p > 0.f) part = fmaf(p, c, part);
+    }
+  }
+  red[tid] = static_cast<double>(part);
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) partial[static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// Sum of the partials in a fixed order (one workgroup of 1024 threads).
+__global__ __launch_bounds__(1024) void sum_partials_kernel(const double* __restrict__ partial, int64_t count,
+                                                            double* __restrict__ out) {
+  __shared__ double red[1024];
+  const int tid = threadIdx.x;
+  double v = 0.0;
+  for (int64_t k = tid; k < count; k += 1024) v += partial[k];
+  red[tid] = v;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) out[0] = red[0];
+}
+
 // f <- g for the parallel updates of a symmetric problem (f = g throughout);
 // launches after the loop has stopped return at once.
 __global__ void copy_potential_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n,
@@ -1506,6 +1601,46 @@ int ottx_transport_matrix(const ottx_problem* pb, const float* f, const float* g
   hipLaunchKernelGGL(transport_matrix_kernel, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d,
                      static_cast<float>(pb->inv_scale_cost), f, g, static_cast<float>(1.0 / eps), out);
   OTTX_LAUNCHED();
+  return OTTX_OK;
+}
+
+size_t ottx_transport_cost_workspace_bytes(int64_t n, int64_t m) {
+  if (n <= 0 || m <= 0) return 0;
+  const int64_t blocks = ((n + kTcTile - 1) / kTcTile) * ((m + kTcTile - 1) / kTcTile);
+  return static_cast<size_t>(256 + ((blocks * 8 + 255) / 256) * 256);
+}
+
+int ottx_transport_cost(const ottx_problem* pb, const float* f, const float* g, double eps, const float* x_other,
+                        const float* y_other, int32_t d_other, double other_inv_scale_cost, double* out_host,
+                        void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_problem(pb);
+  if (rc != OTTX_OK) return rc;
+  if (!f || !g || !out_host || !workspace || !(eps > 0.0)) return OTTX_EINVAL;
+  const bool same = x_other == nullptr && y_other == nullptr;
+  if (!same && (!x_other || !y_other || d_other <= 0)) return OTTX_EINVAL;
+  if (workspace_bytes < ottx_transport_cost_workspace_bytes(pb->n, pb->m)) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  const int64_t gx = (pb->m + kTcTile - 1) / kTcTile, gy = (pb->n + kTcTile - 1) / kTcTile;
+  if (gy > 65535) return OTTX_EUNSUPPORTED;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double* result = static_cast<double*>(workspace);
+  double* partial = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
+  const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>(gy));
+  const float s = static_cast<float>(pb->inv_scale_cost);
+  const float inv_eps = static_cast<float>(1.0 / eps);
+  if (same)
+    hipLaunchKernelGGL(transport_cost_kernel<true>, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d, s, f,
+                       g, inv_eps, pb->x, pb->y, pb->d, s, partial);
+  else
+    hipLaunchKernelGGL(transport_cost_kernel<false>, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d, s, f,
+                       g, inv_eps, x_other, y_other, d_other, static_cast<float>(other_inv_scale_cost), partial);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(1024), 0, st, static_cast<const double*>(partial), gx * gy,
+                     result);
+  OTTX_LAUNCHED();
+  double host = 0.0;
+  OTTX_HIP(hipMemcpyAsync(&host, result, sizeof(double), hipMemcpyDeviceToHost, st));
+  OTTX_HIP(hipStreamSynchronize(st));
+  *out_host = host;
   return OTTX_OK;
 }
 

@@ -322,6 +322,32 @@ class PointCloud:
     del keep
     return out
 
+  def transport_cost(self, f, g, other: Optional["PointCloud"] = None) -> float:
+    """sum_ij P_ij C'_ij with P = exp((f + g - C) / eps) (target epsilon) and C'
+    the cost of ``other`` (this geometry's own cost when None): the quantity
+    of sinkhorn.py:345-411, in one pass over the pairs."""
+    lib = _lib.load_library()
+    n, m = self.shape
+    f = as_device_f32(f, self.device)
+    g = as_device_f32(g, self.device)
+    if f.shape != (n,) or g.shape != (m,):
+      raise ValueError(f"potentials must be [{n}] and [{m}]")
+    pb, keep = self._problem(epsilon=self.epsilon)
+    nbytes = lib.ottx_transport_cost_workspace_bytes(n, m)
+    ws = _lib.workspace(nbytes, self.device)
+    out = ctypes.c_double(0.0)
+    if other is None:
+      xo = yo = None
+      d_o, s_o = 0, 0.0
+    else:
+      xo, yo = other.x, other.y
+      d_o, s_o = int(xo.shape[1]), float(other.inv_scale_cost)
+    _lib.check("ottx_transport_cost", lib.ottx_transport_cost(
+        ctypes.byref(pb), f.data_ptr(), g.data_ptr(), float(self.epsilon), _lib.ptr(xo), _lib.ptr(yo), d_o, s_o,
+        ctypes.byref(out), ws.data_ptr(), nbytes, _lib.stream_ptr()))
+    del keep, ws
+    return float(out.value)
+
   def apply_transport_from_potentials(self, f, g, vec, axis: int = 0) -> torch.Tensor:
     """geometry.py:445-486: P^T vec (axis 0, vec over the n rows) or P vec
     (axis 1, vec over the m columns), P = exp((f + g - C) / eps), for one

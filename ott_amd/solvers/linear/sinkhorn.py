@@ -114,22 +114,16 @@ class SinkhornOutput(NamedTuple):
   def transport_cost_at_geom(self, other_geom: PointCloud) -> float:
     """sinkhorn.py:386-411: <P, C'> for another PointCloud's cost C'.
 
-    As the reference does for costs with a low-rank form, C' = cost_1 cost_2^T
-    (pointcloud.py:298-319) and the sum is sum(apply(cost_1^T) * cost_2^T):
-    the two constant columns by one marginal pass each, the d coordinate
-    columns by signed applies, never materialising P or C'.
+    The reference sums P against the cost's low-rank factors
+    (pointcloud.py:298-319); here one fused pass over the pairs forms P_ij and
+    C'_ij from the two pairs of clouds and sums P_ij C'_ij in float64, never
+    materialising P or C' (``ottx_transport_cost``).
     """
     if not isinstance(other_geom, PointCloud) or not other_geom.is_squared_euclidean:
       raise NotImplementedError("transport_cost_at_geom needs a squared-Euclidean PointCloud")
     if tuple(other_geom.shape) != tuple(self.geom.shape):
       raise ValueError(f"other_geom must be {tuple(self.geom.shape)}, got {tuple(other_geom.shape)}")
-    cost_1, cost_2 = other_geom.low_rank_cost_factors()
-    col = self.marginal(0).double()  # 1^T P, over the m columns
-    row = self.marginal(1).double()  # P 1, over the n rows
-    total = (row * cost_1[:, 0]).sum() + (col * cost_2[:, 1]).sum() * other_geom.inv_scale_cost
-    cross = self.apply(cost_1[:, 2:].T.float(), axis=0).double()  # [d, m]: P^T (-2 s x_k)
-    total = total + (cross * cost_2[:, 2:].T).sum()
-    return float(total)
+    return self.geom.transport_cost(self.f, self.g, None if other_geom is self.geom else other_geom)
 
   @property
   def primal_cost(self) -> float:

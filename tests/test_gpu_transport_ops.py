@@ -2,8 +2,8 @@
 against the dense FP64 oracle, evaluated at the potentials the GPU solve
 returned: ``matrix`` (geometry.py:361-365), ``apply`` both ways with signed and
 batched inputs (geometry.py:445-486), ``primal_cost`` and
-``transport_cost_at_geom`` through the low-rank form of the squared-Euclidean
-cost (sinkhorn.py:386-411, pointcloud.py:298-319)."""
+``transport_cost_at_geom`` (sinkhorn.py:345-411; one fused pass over the pairs,
+here with another cloud pair of 37 features, more than one staging chunk)."""
 
 import math
 
@@ -74,8 +74,8 @@ def test_primal_cost_and_transport_cost_at_geom(precision):
   tol = 1e-5 if precision == "fp32" else 1e-4
   assert math.isclose(out.primal_cost, ref, rel_tol=tol), (out.primal_cost, ref)
   rng = np.random.default_rng(4)
-  x2 = (rng.normal(size=(300, 5)) + 1.0).astype(np.float32)
-  y2 = rng.normal(size=(200, 5)).astype(np.float32)
+  x2 = (rng.normal(size=(300, 37)) + 1.0).astype(np.float32)
+  y2 = rng.normal(size=(200, 37)).astype(np.float32)
   other = pointcloud.PointCloud(torch.from_numpy(x2).cuda(), torch.from_numpy(y2).cuda(), epsilon=1.0,
                                 scale_cost=2.0)
   other_cost = oracle.sqeuclidean_cost(x2.astype(np.float64), y2.astype(np.float64), 0.5)
