@@ -130,6 +130,43 @@ class SinkhornOutput(NamedTuple):
     """sinkhorn.py:345-348."""
     return self.transport_cost_at_geom(other_geom=self.geom)
 
+  def _potential_marginal_terms(self) -> float:
+    """<f, P 1> + <g, 1^T P> with non-finite potentials read as 0."""
+    f = self.f.double()
+    g = self.g.double()
+    safe_f = torch.where(torch.isfinite(f), f, torch.zeros_like(f))
+    safe_g = torch.where(torch.isfinite(g), g, torch.zeros_like(g))
+    return float((safe_f * self.marginal(1).double()).sum() + (safe_g * self.marginal(0).double()).sum())
+
+  def cost_at_geom(self, other_geom: PointCloud) -> float:
+    """sinkhorn.py:474-479: <P, C'> - eps sum(entr(P)).
+
+    With log P_ij = (f_i + g_j - C_ij) / eps, -eps sum(entr(P)) =
+    <f, P 1> + <g, 1^T P> - <P, C>, so no matrix is formed.
+    """
+    return self.transport_cost_at_geom(other_geom) + self._potential_marginal_terms() - self.primal_cost
+
+  @property
+  def entropy(self) -> float:
+    """sinkhorn.py:512-522."""
+    return (self.primal_cost - self._potential_marginal_terms()) / self.geom.epsilon
+
+  @property
+  def normalized_entropy(self) -> float:
+    """sinkhorn.py:524-529."""
+    assert self.ot_prob.is_assignment, "Normalized entropy only valid for assignment problem."
+    return self.entropy / math.log(self.geom.shape[0]) - 1.0
+
+  @property
+  def diag(self) -> torch.Tensor:
+    """sinkhorn.py:531-538: P_ii = exp((f_i + g_i - C_ii) / eps), C_ii the cost
+    of x_i against y_i (pointcloud diag_cost)."""
+    n, m = self.geom.shape
+    assert n == m, "Problem must be square for transport matrix to have a diag."
+    diff = self.geom.x.double() - self.geom.y.double()
+    diag_cost = self.geom.inv_scale_cost * (diff * diff).sum(1)
+    return torch.exp((self.f.double() + self.g.double() - diag_cost) / self.geom.epsilon).float()
+
 
 class Sinkhorn:
   """Sinkhorn solver with OTT's constructor (sinkhorn.py:718-780).

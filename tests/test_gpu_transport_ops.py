@@ -89,3 +89,46 @@ def test_primal_cost_and_transport_cost_at_geom(precision):
   with np.errstate(divide="ignore", invalid="ignore"):
     kl = float(np.where(p > 0, p * np.log(p / q), 0.0).sum() - p.sum() + q.sum())
   assert math.isclose(out.primal_cost + eps * kl, out.reg_ot_cost, rel_tol=1e-3)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+def test_entropy_and_cost_at_geom(precision):
+  """entropy (sinkhorn.py:512-522) is sum(entr(P)) and cost_at_geom
+  (sinkhorn.py:474-479) is <P, C'> - eps sum(entr(P)), both against the dense
+  matrix of the oracle."""
+  from ott_amd.geometry import pointcloud
+  from scipy import special
+  out, x, y, f, g, cost, eps = _solved(precision)
+  p = oracle.transport_from_potentials(cost, f, g, eps)
+  ent = float(special.entr(p).sum())
+  tol = 1e-5 if precision == "fp32" else 1e-4
+  assert math.isclose(out.entropy, ent, rel_tol=tol), (out.entropy, ent)
+  rng = np.random.default_rng(8)
+  x2 = rng.normal(size=(300, 3)).astype(np.float32)
+  y2 = rng.normal(size=(200, 3)).astype(np.float32)
+  other = pointcloud.PointCloud(torch.from_numpy(x2).cuda(), torch.from_numpy(y2).cuda(), epsilon=1.0)
+  other_cost = oracle.sqeuclidean_cost(x2.astype(np.float64), y2.astype(np.float64))
+  ref = float((p * other_cost).sum()) - eps * ent
+  got = out.cost_at_geom(other)
+  assert abs(got - ref) <= tol * float((p * other_cost).sum()), (got, ref)
+
+
+def test_diag_and_normalized_entropy():
+  """diag (sinkhorn.py:531-538) and normalized_entropy (:524-529) of an
+  assignment problem (n = m, uniform weights)."""
+  from ott_amd.geometry import pointcloud
+  from ott_amd.problems.linear import linear_problem
+  from ott_amd.solvers.linear import sinkhorn
+  from scipy import special
+  rng = np.random.default_rng(17)
+  x = rng.normal(size=(128, 4)).astype(np.float32)
+  y = (rng.normal(size=(128, 4)) + 0.1).astype(np.float32)
+  geom = pointcloud.PointCloud(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), epsilon=0.5)
+  out = sinkhorn.Sinkhorn(threshold=1e-5)(linear_problem.LinearProblem(geom))
+  assert out.converged
+  f = out.f.double().cpu().numpy()
+  g = out.g.double().cpu().numpy()
+  p = oracle.transport_from_potentials(oracle.sqeuclidean_cost(x.astype(np.float64), y.astype(np.float64)), f, g, 0.5)
+  np.testing.assert_allclose(out.diag.double().cpu().numpy(), np.diag(p), rtol=1e-5, atol=1e-12)
+  ent = float(special.entr(p).sum())
+  assert math.isclose(out.normalized_entropy, ent / math.log(128) - 1.0, rel_tol=1e-5, abs_tol=1e-6)
