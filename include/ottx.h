@@ -262,6 +262,21 @@ int ottx_transport_cost(const ottx_problem* problem, const float* f, const float
                         double other_inv_scale_cost, double* out_host, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/* Gradient of reg_ot_cost with respect to the points of one cloud, with the
+ * potentials held fixed: the Danskin gradient that jax.grad takes through
+ * SinkhornOutput.reg_ot_cost, whose f and g are stop_gradient'ed
+ * (sinkhorn.py:329-335, compute_kl_reg_cost :224-277).  Only the coupling's
+ * total mass depends on the points, so
+ *   side 0: grad[i * d + k] = 2 s sum_j P_ij (x_ik - y_jk)   ([n, d])
+ *   side 1: grad[j * d + k] = 2 s sum_i P_ij (y_jk - x_ik)   ([m, d])
+ * with P_ij = exp((f_i + g_j - s |x_i - y_j|^2) / eps), s = inv_scale_cost,
+ * held constant (a relative epsilon or a data-dependent scale_cost adds terms
+ * this does not include).  d <= 64 (else OTTX_EUNSUPPORTED).  Asynchronous on
+ * the stream. */
+size_t ottx_cost_gradient_workspace_bytes(const ottx_problem* problem, int32_t side);
+int ottx_cost_gradient(const ottx_problem* problem, const float* f, const float* g, double eps, int32_t side,
+                       float* grad, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Full log-domain Sinkhorn solve: Sinkhorn.__call__ -> run -> iterations
  * (sinkhorn.py:782-801, 1052-1099) with the fixed-point loop of
  * math/fixed_point_loop.py:23-93, the error of sinkhorn.py:127-221 fused into

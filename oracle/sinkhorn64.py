@@ -34,6 +34,7 @@ __all__ = [
     "resolve_epsilon",
     "epsilon_at",
     "momentum_weight",
+    "reg_ot_cost_grad",
     "DenseLSE",
     "apply_lse_kernel",
     "update_potential",
@@ -551,3 +552,23 @@ def transport_cost_at_cost(cost_p: np.ndarray, f, g, eps: float, other_cost: np.
   """sinkhorn.py:386-411 (dense branch): sum(P * C'), P from the solve's cost."""
   return float((transport_from_potentials(cost_p, f, g, eps) * other_cost).sum())
 
+
+
+def reg_ot_cost_grad(x, y, a, b, f, g, eps, scale=1.0):
+  """Gradient of reg_ot_cost with f and g held fixed (the Danskin gradient of
+  sinkhorn.py:329-335 through compute_kl_reg_cost, sinkhorn.py:224-277):
+  with C = scale |x_i - y_j|^2 and P = exp((f + g - C) / eps),
+    d/dx_i = 2 scale sum_j P_ij (x_i - y_j),  d/dy_j = 2 scale sum_i P_ij (y_j - x_i),
+    d/da_i = f_i - eps log a_i - eps + eps sum(b)  (a_i > 0; 0 otherwise),
+  and d/db likewise.  Balanced problems, eps and scale held constant."""
+  x = np.asarray(x, np.float64)
+  y = np.asarray(y, np.float64)
+  a = np.asarray(a, np.float64)
+  b = np.asarray(b, np.float64)
+  p = transport_from_potentials(sqeuclidean_cost(x, y, scale), f, g, eps)
+  gx = 2.0 * scale * (p.sum(1)[:, None] * x - p @ y)
+  gy = 2.0 * scale * (p.sum(0)[:, None] * y - p.T @ x)
+  with np.errstate(divide="ignore", invalid="ignore"):
+    ga = np.where(a > 0, f - eps * np.log(a) - eps, 0.0) + eps * b.sum()
+    gb = np.where(b > 0, g - eps * np.log(b) - eps, 0.0) + eps * a.sum()
+  return gx, gy, ga, gb

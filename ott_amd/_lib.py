@@ -184,6 +184,10 @@ _SIGNATURES = [
      [ctypes.POINTER(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_int32, ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t,
       ctypes.c_void_p]),
+    ("ottx_cost_gradient_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(Problem), ctypes.c_int32]),
+    ("ottx_cost_gradient", ctypes.c_int,
+     [ctypes.POINTER(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     ("ottx_sinkhorn_workspace_bytes", ctypes.c_size_t,
      [ctypes.POINTER(Problem), ctypes.POINTER(SinkhornOpts)]),
     ("ottx_sinkhorn_batched_workspace_bytes", ctypes.c_size_t,

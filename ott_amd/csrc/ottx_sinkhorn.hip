@@ -896,6 +896,119 @@ __global__ __launch_bounds__(1024) void sum_partials_kernel(const double* __rest
   if (tid == 0) out[0] = red[0];
 }
 
+// Danskin gradient of reg_ot_cost with respect to one cloud (sinkhorn.py:
+// 329-335 with compute_kl_reg_cost, :224-277): with f and g held fixed only
+// the coupling's total mass depends on the points, and
+//   d reg / d p_i = 2 s sum_q P_iq (p_i - q_q)
+// for the cloud p against the other cloud q.  A workgroup owns 64 points p
+// and streams a slice of the q points 64 at a time: the 64 x 64 block of P
+// goes through LDS (pair terms from the differences, as above), then each
+// thread accumulates sum_q P_iq (p_ik - q_qk) for 4 points by 4 features.
+// Partials per slice, summed in float64 by grad_reduce_kernel.  d <= 64.
+constexpr int kGdMaxD = 64;
+constexpr int kGdRow = kGdMaxD + 4;
+
+__global__ __launch_bounds__(256) void grad_partial_kernel(
+    const float* __restrict__ p, const float* __restrict__ pot_p, int64_t np, const float* __restrict__ q,
+    const float* __restrict__ pot_q, int64_t nq, int d, float s, float inv_eps, int64_t slice,
+    float* __restrict__ partial) {
+  __shared__ __align__(16) float ps[kGdMaxD][kTcPad];  // own points, feature-major
+  __shared__ __align__(16) float qs[kGdMaxD][kTcPad];  // streamed points, feature-major
+  __shared__ __align__(16) float qr[kTcTile][kGdRow];  // streamed points, point-major
+  __shared__ __align__(16) float pt[kTcTile][kTcPad];  // P block, [q][p]
+  const int tid = threadIdx.x;
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * kTcTile;
+  const int64_t q_begin = static_cast<int64_t>(blockIdx.y) * slice;
+  const int64_t q_end = min(nq, q_begin + slice);
+  for (int e = tid; e < kTcTile * kGdMaxD; e += 256) {
+    const int r = e / kGdMaxD, k = e % kGdMaxD;
+    const int64_t i = i0 + r;
+    ps[k][r] = (k < d && i < np) ? p[i * d + k] : 0.f;
+  }
+  const int r0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
+  float fi[4];
+  float own[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t i = i0 + r0 + a;
+    fi[a] = i < np ? pot_p[i] : -INFINITY;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) own[a][b] = (c0 + b < d) ? ps[c0 + b][r0 + a] : 0.f;
+  float acc[4][4] = {};
+  for (int64_t j0 = q_begin; j0 < q_end; j0 += kTcTile) {
+    __syncthreads();
+    for (int e = tid; e < kTcTile * kGdMaxD; e += 256) {
+      const int r = e / kGdMaxD, k = e % kGdMaxD;
+      const int64_t j = j0 + r;
+      const float v = (k < d && j < q_end) ? q[j * d + k] : 0.f;
+      qs[k][r] = v;
+      qr[r][k] = v;
+    }
+    __syncthreads();
+    // P block: rows r0..r0+3 (own points) x columns c0..c0+3 (streamed points)
+    float dist[4][4] = {};
+    for (int k = 0; k < d; ++k) {
+      const float4 xv = *reinterpret_cast<const float4*>(&ps[k][r0]);
+      const float4 yv = *reinterpret_cast<const float4*>(&qs[k][c0]);
+      const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
+      const float ya[4] = {yv.x, yv.y, yv.z, yv.w};
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const float u = xa[a] - ya[b];
+          dist[a][b] = fmaf(u, u, dist[a][b]);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int64_t j = j0 + c0 + b;
+      const float gj = j < q_end ? pot_q[j] : -INFINITY;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const float e = (fi[a] + gj - s * dist[a][b]) * inv_eps;
+        pt[c0 + b][r0 + a] = (e == e && e > -INFINITY) ? expf(e) : 0.f;
+      }
+    }
+    __syncthreads();
+    // acc[a][b] += sum_q P(r0 + a, q) (own[a][b] - q_(c0 + b)): rows r0.., features c0..
+    const int jn = static_cast<int>(min<int64_t>(kTcTile, q_end - j0));
+    for (int jj = 0; jj < jn; ++jj) {
+      const float4 pv = *reinterpret_cast<const float4*>(&pt[jj][r0]);
+      const float4 qv = *reinterpret_cast<const float4*>(&qr[jj][c0]);
+      const float pa[4] = {pv.x, pv.y, pv.z, pv.w};
+      const float qa[4] = {qv.x, qv.y, qv.z, qv.w};
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(pa[a], own[a][b] - qa[b], acc[a][b]);
+    }
+  }
+  // partial[slice][i][k] for the thread's 4 points x 4 features
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int64_t i = i0 + r0 + a;
+    if (i >= np) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (c0 + b < d) partial[(static_cast<int64_t>(blockIdx.y) * np + i) * d + c0 + b] = acc[a][b];
+  }
+}
+
+// grad[i][k] = 2 s sum over slices of partial[slice][i][k], summed in float64.
+__global__ void grad_reduce_kernel(const float* __restrict__ partial, int64_t count, int slices, double two_s,
+                                   float* __restrict__ grad) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  double v = 0.0;
+  for (int t = 0; t < slices; ++t) v += static_cast<double>(partial[static_cast<int64_t>(t) * count + e]);
+  grad[e] = static_cast<float>(two_s * v);
+}
+
 // f <- g for the parallel updates of a symmetric problem (f = g throughout);
 // launches after the loop has stopped return at once.
 __global__ void copy_potential_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n,
@@ -1641,6 +1754,52 @@ int ottx_transport_cost(const ottx_problem* pb, const float* f, const float* g, 
   OTTX_HIP(hipMemcpyAsync(&host, result, sizeof(double), hipMemcpyDeviceToHost, st));
   OTTX_HIP(hipStreamSynchronize(st));
   *out_host = host;
+  return OTTX_OK;
+}
+
+// Slices of the streamed cloud for the gradient: enough workgroups for the
+// chip (about 2048), each slice a multiple of 64 points.
+static void grad_plan(int64_t np, int64_t nq, int64_t* slice, int* slices) {
+  const int64_t tiles = (np + kTcTile - 1) / kTcTile;
+  const int64_t q_tiles = (nq + kTcTile - 1) / kTcTile;
+  int64_t want = std::max<int64_t>(1, std::min<int64_t>(q_tiles, (2048 + tiles - 1) / tiles));
+  const int64_t per = (q_tiles + want - 1) / want;
+  *slice = per * kTcTile;
+  *slices = static_cast<int>((q_tiles + per - 1) / per);
+}
+
+size_t ottx_cost_gradient_workspace_bytes(const ottx_problem* pb, int32_t side) {
+  if (check_problem(pb) != OTTX_OK || (side != 0 && side != 1) || pb->d > kGdMaxD) return 0;
+  const int64_t np = side == 0 ? pb->n : pb->m, nq = side == 0 ? pb->m : pb->n;
+  int64_t slice;
+  int slices;
+  grad_plan(np, nq, &slice, &slices);
+  return static_cast<size_t>(((static_cast<int64_t>(slices) * np * pb->d * 4 + 255) / 256) * 256);
+}
+
+int ottx_cost_gradient(const ottx_problem* pb, const float* f, const float* g, double eps, int32_t side, float* grad,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_problem(pb);
+  if (rc != OTTX_OK) return rc;
+  if (!f || !g || !grad || !workspace || !(eps > 0.0) || (side != 0 && side != 1)) return OTTX_EINVAL;
+  if (pb->d > kGdMaxD) return OTTX_EUNSUPPORTED;
+  if (workspace_bytes < ottx_cost_gradient_workspace_bytes(pb, side)) return OTTX_EWORKSPACE;
+  const bool rows = side == 0;
+  const int64_t np = rows ? pb->n : pb->m, nq = rows ? pb->m : pb->n;
+  int64_t slice;
+  int slices;
+  grad_plan(np, nq, &slice, &slices);
+  const int64_t tiles = (np + kTcTile - 1) / kTcTile;
+  if (tiles > 0x7fffffff || slices > 65535) return OTTX_EUNSUPPORTED;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  float* partial = static_cast<float*>(workspace);
+  hipLaunchKernelGGL(grad_partial_kernel, dim3(static_cast<unsigned>(tiles), static_cast<unsigned>(slices)), dim3(256),
+                     0, st, rows ? pb->x : pb->y, rows ? f : g, np, rows ? pb->y : pb->x, rows ? g : f, nq, pb->d,
+                     static_cast<float>(pb->inv_scale_cost), static_cast<float>(1.0 / eps), slice, partial);
+  const int64_t count = np * pb->d;
+  hipLaunchKernelGGL(grad_reduce_kernel, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, st,
+                     static_cast<const float*>(partial), count, slices, 2.0 * pb->inv_scale_cost, grad);
+  OTTX_LAUNCHED();
   return OTTX_OK;
 }
 

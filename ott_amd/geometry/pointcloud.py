@@ -348,6 +348,29 @@ class PointCloud:
     del keep, ws
     return float(out.value)
 
+  def cost_gradient(self, f, g, side: int = 0) -> torch.Tensor:
+    """sum_j P_ij dC_ij/dx_i (side 0, [n, d]) or sum_i P_ij dC_ij/dy_j (side 1,
+    [m, d]) with P = exp((f + g - C) / eps) at the target epsilon: the gradient
+    of the coupling's total mass term of reg_ot_cost (sinkhorn.py:224-277)
+    with the potentials held fixed."""
+    lib = _lib.load_library()
+    n, m = self.shape
+    f = as_device_f32(f, self.device)
+    g = as_device_f32(g, self.device)
+    if f.shape != (n,) or g.shape != (m,):
+      raise ValueError(f"potentials must be [{n}] and [{m}]")
+    pb, keep = self._problem(epsilon=self.epsilon)
+    nbytes = lib.ottx_cost_gradient_workspace_bytes(ctypes.byref(pb), int(side))
+    if nbytes == 0:
+      raise NotImplementedError("cost_gradient supports d <= 64 (side 0 or 1)")
+    ws = _lib.workspace(nbytes, self.device)
+    out = torch.empty(((n if side == 0 else m), self.x.shape[1]), dtype=torch.float32, device=self.device)
+    _lib.check("ottx_cost_gradient", lib.ottx_cost_gradient(
+        ctypes.byref(pb), f.data_ptr(), g.data_ptr(), float(self.epsilon), int(side), out.data_ptr(),
+        ws.data_ptr(), nbytes, _lib.stream_ptr()))
+    del keep, ws  # stream-ordered reuse by the caching allocator
+    return out
+
   def apply_transport_from_potentials(self, f, g, vec, axis: int = 0) -> torch.Tensor:
     """geometry.py:445-486: P^T vec (axis 0, vec over the n rows) or P vec
     (axis 1, vec over the m columns), P = exp((f + g - C) / eps), for one

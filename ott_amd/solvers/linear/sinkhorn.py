@@ -130,6 +130,41 @@ class SinkhornOutput(NamedTuple):
     """sinkhorn.py:345-348."""
     return self.transport_cost_at_geom(other_geom=self.geom)
 
+  def reg_ot_cost_grad(self) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Gradients of ``reg_ot_cost`` with respect to x, y, a and b with the
+    potentials held fixed: what jax.grad returns through the reference's
+    reg_ot_cost, whose f and g are stop_gradient'ed under the default
+    ``use_danskin`` (sinkhorn.py:329-335, compute_kl_reg_cost :224-277).
+
+    - x: 2 s sum_j P_ij (x_i - y_j), y: 2 s sum_i P_ij (y_j - x_i), on the GPU
+      (``ottx_cost_gradient``, d <= 64);
+    - a: f - eps log a - eps + eps sum(b) where a > 0 (0 at zero weights), b
+      likewise.
+
+    Balanced problems with an absolute epsilon and a float ``scale_cost``: a
+    relative epsilon or a data-dependent scale would add terms through eps or
+    s, which this does not include.
+    """
+    geom = self.geom
+    if self.ot_prob.tau_a != 1.0 or self.ot_prob.tau_b != 1.0:
+      raise NotImplementedError("reg_ot_cost_grad supports balanced problems only")
+    if geom._relative_epsilon is not None or not isinstance(geom._scale_cost, (int, float)):
+      raise NotImplementedError("reg_ot_cost_grad needs an absolute epsilon and a float scale_cost")
+    if not geom._schedule_is_target_only():
+      raise NotImplementedError("reg_ot_cost_grad needs a fixed epsilon")
+    return (geom.cost_gradient(self.f, self.g, side=0), geom.cost_gradient(self.f, self.g, side=1),
+            *self._weight_grads())
+
+  def _weight_grads(self) -> Tuple[torch.Tensor, torch.Tensor]:
+    eps = self.geom.epsilon
+    a, b = self.a.double(), self.b.double()
+    f, g = self.f.double(), self.g.double()
+    zero_a = torch.zeros_like(a)
+    zero_b = torch.zeros_like(b)
+    ga = torch.where(a > 0, f - eps * torch.log(torch.where(a > 0, a, 1.0)) - eps, zero_a) + eps * b.sum()
+    gb = torch.where(b > 0, g - eps * torch.log(torch.where(b > 0, b, 1.0)) - eps, zero_b) + eps * a.sum()
+    return ga, gb
+
   def _potential_marginal_terms(self) -> float:
     """<f, P 1> + <g, 1^T P> with non-finite potentials read as 0."""
     f = self.f.double()

@@ -387,3 +387,42 @@ def test_oracle_adaptive_momentum():
   np.testing.assert_allclose(adapt.f, tail.f, rtol=0, atol=1e-9)
   np.testing.assert_allclose(adapt.errors[start // inner:], tail.errors, rtol=1e-5)
   assert adapt.errors[-1] < const.errors[-1]
+
+
+def test_oracle_reg_ot_cost_grad_against_finite_differences():
+  """The gradient at fixed potentials equals the derivative of the converged
+  reg_ot_cost (envelope theorem): central differences of full solves in the
+  points and, keeping the total mass, in the weights."""
+  x, y, a, b = _problem(7, 5, 2, seed=21)
+  eps, scale = 0.3, 0.7
+  kw = dict(epsilon=eps, scale_cost=1.0 / scale, threshold=1e-14, inner_iterations=1, max_iterations=20000)
+  out = oracle.solve_pointcloud(x, y, a, b, **kw)
+  assert out.converged
+  gx, gy, ga, gb = oracle.reg_ot_cost_grad(x, y, a, b, out.f, out.g, eps, scale)
+  h = 1e-5
+
+  def cost(x_, y_, a_, b_):
+    return oracle.solve_pointcloud(x_, y_, a_, b_, **kw).reg_ot_cost
+
+  for (i, k) in [(0, 0), (3, 1), (6, 0)]:
+    xp, xm = x.copy(), x.copy()
+    xp[i, k] += h
+    xm[i, k] -= h
+    assert abs((cost(xp, y, a, b) - cost(xm, y, a, b)) / (2 * h) - gx[i, k]) < 1e-7
+  for (j, k) in [(1, 1), (4, 0)]:
+    yp, ym = y.copy(), y.copy()
+    yp[j, k] += h
+    ym[j, k] -= h
+    assert abs((cost(x, yp, a, b) - cost(x, ym, a, b)) / (2 * h) - gy[j, k]) < 1e-7
+  ap, am = a.copy(), a.copy()
+  ap[2] += h
+  ap[5] -= h
+  am[2] -= h
+  am[5] += h
+  assert abs((cost(x, y, ap, b) - cost(x, y, am, b)) / (2 * h) - (ga[2] - ga[5])) < 1e-6
+  bp, bm = b.copy(), b.copy()
+  bp[0] += h
+  bp[3] -= h
+  bm[0] -= h
+  bm[3] += h
+  assert abs((cost(x, y, a, bp) - cost(x, y, a, bm)) / (2 * h) - (gb[0] - gb[3])) < 1e-6
