@@ -199,8 +199,9 @@ def divergence_f1(pb, steps, warmup):
 
 def transport_ops_f2(pb, steps):
   """Output-side transport ops on the solved benchmark problem (bf16 split):
-  one signed apply (two fused log-sum-exp passes) and primal_cost (two
-  marginal passes and 2 d signed passes), timed from call to host value."""
+  one signed apply (two fused log-sum-exp passes), primal_cost (one fused
+  pass over the pairs) and the fixed-potential gradient of reg_ot_cost with
+  respect to x and to y (one pass each), timed from call to result."""
   import torch
   from ott_amd.geometry import pointcloud
   from ott_amd.problems.linear import linear_problem
@@ -224,8 +225,11 @@ def transport_ops_f2(pb, steps):
 
   ms_apply, _ = timed(lambda: out.apply(vec, axis=0))
   ms_primal, primal = timed(lambda: out.primal_cost)
+  ms_gx, _ = timed(lambda: geom.cost_gradient(out.f, out.g, side=0))
+  ms_gy, _ = timed(lambda: geom.cost_gradient(out.f, out.g, side=1))
   return {"workload": "solved benchmark problem (bf16 split, threshold 1e-3)",
           "apply_one_signed_vector_ms": ms_apply, "primal_cost_ms": ms_primal,
+          "cost_gradient_x_ms": ms_gx, "cost_gradient_y_ms": ms_gy,
           "primal_cost": primal, "reg_ot_cost": out.reg_ot_cost}
 
 
