@@ -44,3 +44,24 @@ def test_adaptive_momentum_limits():
   with pytest.raises(NotImplementedError):  # the weight reads L1 errors
     _solve(x, y, a, b, "fp32", dict(epsilon=0.05), inner_iterations=1, norm_error=2,
            momentum=acceleration.Momentum(start=5))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("inner,start,init,decay", [(5, 10, 4.0, 0.9), (10, 30, 10.0, 0.8)])
+def test_adaptive_momentum_with_epsilon_schedule(precision, inner, start, init, decay):
+  """Both per-iteration options together: the schedule's error pass and the
+  momentum blend after it run in the same blocks."""
+  import torch
+  from ott_amd.geometry import epsilon_scheduler, pointcloud
+  from ott_amd.solvers import linear
+  from ott_amd.solvers.linear import acceleration
+  x, y, a, b = _cloud(300, 257, 6, seed=12, zero_a=[4])
+  kw = dict(threshold=1e-4, inner_iterations=inner, max_iterations=2000)
+  geom = pointcloud.PointCloud(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(),
+                               epsilon=epsilon_scheduler.Epsilon(0.02, init=init, decay=decay), precision=precision)
+  out = linear.solve(geom, a=torch.from_numpy(a).cuda(), b=torch.from_numpy(b).cuda(),
+                     momentum=acceleration.Momentum(start=start), **kw)
+  ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.02, momentum_start=start, eps_init=init, eps_decay=decay,
+                                **kw)
+  assert ref.converged
+  _check_solve(out, ref, precision)
