@@ -47,7 +47,7 @@ def _epsilon_args(epsilon, relative_epsilon) -> Tuple[float, float]:
   """(absolute epsilon, relative multiplier) of PointCloud(epsilon, relative_epsilon)."""
   if isinstance(epsilon, eps_scheduler.Epsilon):
     if not epsilon.is_target_only:
-      raise NotImplementedError("epsilon schedules with init > 1 are not supported yet")
+      raise NotImplementedError("epsilon schedules with init > 1 are not supported by the batched solve")
     epsilon = epsilon.target
   if relative_epsilon == "mean":
     return 0.0, (eps_scheduler.DEFAULT_EPSILON_SCALE if epsilon is None else float(epsilon))
