@@ -208,11 +208,12 @@ class Sinkhorn:
 
   Supported on the GPU path: ``lse_mode=True``; any ``threshold``,
   ``norm_error`` (p >= 1), ``inner_iterations``, ``min_iterations``,
-  ``max_iterations``; constant ``Momentum`` (``start == 0``); sequential
-  updates, and parallel updates for the symmetric problems of a Sinkhorn
-  divergence (the same cloud and weights on both sides, from the default
-  start); balanced or unbalanced problems; warm starts through ``init`` or an
-  ``initializer``.  Options outside that set raise ``NotImplementedError``;
+  ``max_iterations``; constant ``Momentum`` and adaptive ``Momentum(start)``
+  (norm_error 1, start >= 2 inner_iterations); epsilon schedules
+  ``Epsilon(target, init, decay)`` on balanced problems; sequential updates,
+  and parallel updates for the symmetric problems of a Sinkhorn divergence
+  (the same cloud and weights on both sides, from the default start); balanced
+  or unbalanced problems; warm starts through ``init`` or an ``initializer``.  Options outside that set raise ``NotImplementedError``;
   there is no silent fallback.  ``implicit_diff`` and ``use_danskin`` only
   affect reverse mode, which this package does not provide.
 
