@@ -392,14 +392,27 @@ class PointCloud:
     eps = self.epsilon
     keep_pot = g if axis == 0 else f  # the potential of the output side
     outs = []
-    for v in vecs:
-      parts = []
-      for part in (torch.clamp(v, min=0.0), torch.clamp(-v, min=0.0)):
+    # which vectors have positive / negative entries: a part that is all zero
+    # contributes nothing and skips its pass (one host read for the batch)
+    lo, hi = torch.aminmax(vecs, dim=1)
+    signs = torch.stack([hi > 0, lo < 0], dim=1).tolist()
+    n_out = m if axis == 0 else n
+    for v, (has_pos, has_neg) in zip(vecs, signs):
+      total = None
+      for part, sign, present in ((torch.clamp(v, min=0.0), 1.0, has_pos), (torch.clamp(-v, min=0.0), -1.0, has_neg)):
+        if not present:
+          continue
         with torch.no_grad():
           shifted = (f if axis == 0 else g) + eps * torch.log(part)
         res = self._op(_lib.OP_APPLY_LSE, shifted if axis == 0 else f, g if axis == 0 else shifted, eps, axis)
-        parts.append(torch.exp((res.double() + keep_pot.double()) / eps))
-      outs.append((parts[0] - parts[1]).float())
+        term = torch.exp((res.double() + keep_pot.double()) / eps)
+        if total is None:
+          total = term if sign > 0 else -term
+        else:  # the negative part, after the positive one
+          total = total - term
+      if total is None:
+        total = torch.zeros((n_out,), dtype=torch.float64, device=self.device)
+      outs.append(total.float())
     out = torch.stack(outs)
     return out[0] if single else out
 
