@@ -41,3 +41,16 @@ def test_abi_struct_offsets():
   assert o.epsilon_init.offset == 56 and o.epsilon_decay.offset == 64
   assert o.momentum_start.offset == 72 and o.momentum_error_threshold.offset == 80
   assert np.isclose(_lib.SinkhornOpts(momentum_error_threshold=np.inf).momentum_error_threshold, np.inf)
+
+
+def test_reg_ot_cost_gradient_limits_on_the_host():
+  from ott_amd.solvers.linear import danskin
+  x = torch.zeros((8, 3))
+  with pytest.raises(NotImplementedError):  # a relative epsilon would add terms through eps
+    danskin.reg_ot_cost(x, x, epsilon=None)
+  with pytest.raises(NotImplementedError):
+    danskin.reg_ot_cost(x, x, epsilon=0.1, scale_cost="mean")
+  with pytest.raises(NotImplementedError):
+    danskin.reg_ot_cost(x, x, epsilon=0.1, tau_a=0.5)
+  with pytest.raises(NotImplementedError):
+    danskin.reg_ot_cost(torch.zeros((8, 65)), torch.zeros((8, 65)), epsilon=0.1)
