@@ -16,7 +16,20 @@ Pinning (what checks this oracle independently of its own formulas) lives in
 ``tests/test_oracle.py``: brute-force costs, ``scipy.special.logsumexp``, the
 m=1 closed form of a half-step, the exact marginal after a half-step, an
 L-BFGS solve of the smooth semi-dual, the Hungarian limit at small epsilon, the
-epsilon -> infinity limit, scale invariance and gauge invariance.
+epsilon -> infinity limit, scale invariance and gauge invariance; and in
+``tests/test_oracle_costs.py`` for the unbalanced case: the fixed point against
+an L-BFGS minimisation of its primal, ``compute_kl_reg_cost`` against a
+numerical infimum of its Lagrangian, closed forms for one point per side,
+continuity at tau -> 1, planted slips that must fail, and ``ent_reg_cost`` /
+``dual_cost`` against the coupling's own entropy.
+
+One property of the reference is kept as it is: with tau < 1 its iteration
+converges to the optimum of the problem whose entropy is taken against the
+counting measure, while ``compute_kl_reg_cost`` is the dual function of the
+problem whose entropy is taken against a b^T, evaluated at f - eps log a and
+g - eps log b.  So in the unbalanced case reg_ot_cost is not the primal value
+of the returned coupling (it is in the balanced case), and this oracle, like
+the HIP path, reproduces the reference's number rather than that primal.
 """
 
 from __future__ import annotations
@@ -275,7 +288,12 @@ def solution_error(ops, f, g, a, b, eps, tau_a, tau_b, norm_error=(1,),
 
 
 def compute_kl_reg_cost(ops, f, g, a, b, eps, tau_a=1.0, tau_b=1.0):
-  """solvers/linear/sinkhorn.py:224-277 (lse_mode=True)."""
+  """solvers/linear/sinkhorn.py:224-277 (lse_mode=True).
+
+  For tau < 1 this is the Lagrange dual function of the a b^T-referenced
+  unbalanced problem at (f - eps log a, g - eps log b), whatever (f, g) is;
+  at the solver's fixed point it differs from the primal value of the
+  coupling (see the module docstring)."""
   supp_a, supp_b = a > 0, b > 0
   with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
     fa = eps * np.log(a)  # geometry.py:524-534 potential_from_scaling
@@ -560,7 +578,13 @@ def reg_ot_cost_grad(x, y, a, b, f, g, eps, scale=1.0):
   with C = scale |x_i - y_j|^2 and P = exp((f + g - C) / eps),
     d/dx_i = 2 scale sum_j P_ij (x_i - y_j),  d/dy_j = 2 scale sum_i P_ij (y_j - x_i),
     d/da_i = f_i - eps log a_i - eps + eps sum(b)  (a_i > 0; 0 otherwise),
-  and d/db likewise.  Balanced problems, eps and scale held constant."""
+  and d/db likewise.  Balanced problems, eps and scale held constant.
+
+  At a zero weight a_i = 0 the a-gradient here is eps sum(b) (the where()
+  branch), a value this library chose: jax.grad through the reference's
+  compute_kl_reg_cost gives NaN there (a (f - eps log a) with f = -inf and
+  1 / a from log a pass a NaN through jnp.where), so parity at zero weights is
+  a documented deviation, not a pinned value."""
   x = np.asarray(x, np.float64)
   y = np.asarray(y, np.float64)
   a = np.asarray(a, np.float64)
