@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define OTTX_ABI_VERSION 2
+#define OTTX_ABI_VERSION 3
 
 enum ottx_status {
   OTTX_OK = 0,
@@ -74,10 +74,29 @@ typedef struct ottx_problem {
   double inv_scale_cost;   /* s in C = s * ||x - y||^2, > 0 (pointcloud.py:334-340) */
   double tau_a;            /* in (0, 1] (linear_problem.py:51-63) */
   double tau_b;
-  double relative_epsilon; /* > 0: epsilon = relative_epsilon * mean(C), resolved on the
-                              device by ottx_sinkhorn_solve (relative_epsilon="mean",
-                              geometry.py:147-171); 0: use epsilon */
+  double relative_epsilon; /* > 0: epsilon = relative_epsilon * mean(C) or * std(C) (see
+                              epsilon_statistic), resolved on the device by ottx_sinkhorn_solve
+                              (relative_epsilon="mean" / "std" and the default epsilon
+                              0.05 * std(C), geometry.py:147-171); 0: use epsilon */
+  int32_t epsilon_statistic; /* enum ottx_eps_statistic: the statistic relative_epsilon
+                              multiplies */
+  int32_t _pad_problem;
 } ottx_problem;
+
+/* The statistic of the cost matrix a relative epsilon multiplies
+ * (geometry/geometry.py:112-136, both in closed form, never forming C). */
+enum ottx_eps_statistic {
+  OTTX_EPS_STAT_MEAN = 0,  /* mean(C) */
+  OTTX_EPS_STAT_STD = 1,   /* std(C) = sqrt(relu(mean(C^2) - mean(C)^2)) */
+};
+
+/* Per-block progress callback of a solve (Sinkhorn(progress_fn=...),
+ * sinkhorn.py:933-937): called on the host thread of ottx_sinkhorn_solve once
+ * each block of inner_iterations has completed, in order, with the block's
+ * index and its raw error (before the +inf rule of min_iterations).  The
+ * solve waits for every block before queuing the next while a callback is
+ * set. */
+typedef void (*ottx_progress_fn)(void* user, int32_t block, double error);
 
 /* Options of the Sinkhorn solver: solvers/linear/sinkhorn.py:718-780. */
 typedef struct ottx_sinkhorn_opts {
@@ -127,6 +146,8 @@ typedef struct ottx_sinkhorn_opts {
                                parallel updates or in the batched solve (OTTX_EUNSUPPORTED). */
   int32_t _pad0;
   double momentum_error_threshold; /* Momentum.error_threshold (inf when unused) */
+  ottx_progress_fn progress_fn; /* NULL: none */
+  void* progress_user;      /* passed back to progress_fn */
 } ottx_sinkhorn_opts;
 
 /* Outputs of one solve (SinkhornOutput fields, sinkhorn.py:280-323). */
@@ -210,6 +231,17 @@ size_t ottx_mean_cost_workspace_bytes(int64_t n, int64_t m, int32_t d);
 int ottx_mean_cost(int64_t n, int64_t m, int32_t d, const float* x,
                    const float* y, double inv_scale_cost, double* out_host,
                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* Mean and standard deviation of the cost matrix, float64, in closed form
+ * from per-cloud moments (row sums of u, |u|^2 u, |u|^2, |u|^4 and the Gram
+ * matrix U^T U of the centred points): out_host[0] = mean(C), out_host[1] =
+ * std(C) = sqrt(relu(mean(C^2) - mean(C)^2)), both over the n m entries with
+ * C = inv_scale_cost |x_i - y_j|^2.  Replaces geometry/geometry.py:112-136
+ * (mean_cost_matrix, std_cost_matrix), which forms C^2 for the standard
+ * deviation.  Blocks until both values are on the host. */
+size_t ottx_cost_moments_workspace_bytes(int64_t n, int64_t m, int32_t d);
+int ottx_cost_moments(int64_t n, int64_t m, int32_t d, const float* x, const float* y, double inv_scale_cost,
+                      double* out_host, void* workspace, size_t workspace_bytes, void* stream);
 
 /* The Geometry operator protocol of the reference, for one PointCloud.
  * mode selects the output written to out[k] (k over rows if axis == 1,

@@ -49,11 +49,19 @@ class Problem(ctypes.Structure):
       ("tau_a", ctypes.c_double),
       ("tau_b", ctypes.c_double),
       ("relative_epsilon", ctypes.c_double),
+      ("epsilon_statistic", ctypes.c_int32),
+      ("_pad_problem", ctypes.c_int32),
   ]
 
 
 # OTTX_ABI_VERSION of include/ottx.h that these structures mirror
-ABI_VERSION = 2
+ABI_VERSION = 3
+
+EPS_STAT_MEAN = 0
+EPS_STAT_STD = 1
+
+# void (*)(void* user, int32_t block, double error)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_double)
 
 
 class SinkhornOpts(ctypes.Structure):
@@ -74,6 +82,8 @@ class SinkhornOpts(ctypes.Structure):
       ("momentum_start", ctypes.c_int32),
       ("_pad0", ctypes.c_int32),
       ("momentum_error_threshold", ctypes.c_double),
+      ("progress_fn", PROGRESS_FN),
+      ("progress_user", ctypes.c_void_p),
   ]
 
 
@@ -165,6 +175,12 @@ _SIGNATURES = [
     ("ottx_mean_cost_workspace_bytes", ctypes.c_size_t,
      [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32]),
     ("ottx_mean_cost", ctypes.c_int,
+     [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+      ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    ("ottx_cost_moments_workspace_bytes", ctypes.c_size_t,
+     [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32]),
+    ("ottx_cost_moments", ctypes.c_int,
      [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),

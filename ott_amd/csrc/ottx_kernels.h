@@ -26,15 +26,22 @@ __device__ __forceinline__ float exp2_fast(float x) { return __builtin_amdgcn_ex
 
 __device__ __forceinline__ uint32_t f32_to_bf16_rne(float v) {
   uint32_t u = __float_as_uint(v);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;  // NaN stays a (quiet) NaN
   u += 0x7fffu + ((u >> 16) & 1u);
   return u >> 16;
 }
 
 __device__ __forceinline__ float bf16_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
 
+// NaN-propagating maximum (IEEE 754-2019 maximum; v_maximum3_f32 on gfx950).
+// fmaxf drops a NaN operand, which would let a NaN row vanish from a column's
+// log-sum-exp; jax.scipy.special.logsumexp (math/utils.py:132-138) returns NaN
+// there, and so must the kernels.
+__device__ __forceinline__ float nan_max(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+
 // Combine two (max, sum) pairs of base-2 scaled log-sum-exp accumulators.
 __device__ __forceinline__ void lse_merge(float& mx, float& sm, float om, float os, float c) {
-  const float mn = fmaxf(mx, om);
+  const float mn = nan_max(mx, om);
   float s = 0.f;
   if (mn != -INFINITY) s = sm * exp2_fast((mx - mn) * c) + os * exp2_fast((om - mn) * c);
   mx = mn;
@@ -296,15 +303,16 @@ __device__ __forceinline__ void tile_exp_max(const f32x4 (&acc)[MT][NB], float c
                                            float (&sm)[NB]) {
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
+    // NaN-propagating maxima: a NaN accumulator makes the column's sum NaN
     float m4[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
-      m4[mt] = fmaxf(fmaxf(acc[mt][nb][0], acc[mt][nb][1]), fmaxf(acc[mt][nb][2], acc[mt][nb][3]));
+      m4[mt] = nan_max(nan_max(acc[mt][nb][0], acc[mt][nb][1]), nan_max(acc[mt][nb][2], acc[mt][nb][3]));
 #pragma unroll
     for (int w = 1; w < MT; w <<= 1)
 #pragma unroll
-      for (int mt = 0; mt + w < MT; mt += 2 * w) m4[mt] = fmaxf(m4[mt], m4[mt + w]);
-    const float mnew = fmaxf(mx[nb], m4[0]);
+      for (int mt = 0; mt + w < MT; mt += 2 * w) m4[mt] = nan_max(m4[mt], m4[mt + w]);
+    const float mnew = nan_max(mx[nb], m4[0]);
     if (mnew != -INFINITY) {
       const float nmc = -c * mnew;
       float part[4];

@@ -118,7 +118,11 @@ __global__ __launch_bounds__(kFinalThreads) void stats_final_kernel(
   if (t < d) {
     const double mxk = sums[0][t] / static_cast<double>(n);
     const double myk = sums[2][t] / static_cast<double>(m);
-    if (mu) mu[t] = static_cast<float>(0.5 * (mxk + myk));
+    // any common shift leaves the cost unchanged: a feature whose mean is not
+    // finite (a NaN or inf input) is not shifted, so a bad row stays confined
+    // to its own costs, as in the reference, instead of reaching every point
+    const double shift = 0.5 * (mxk + myk);
+    if (mu) mu[t] = isfinite(shift) ? static_cast<float>(shift) : 0.f;
     nx2 = sums[1][t];
     ny2 = sums[3][t];
     dot = mxk * myk;
@@ -140,6 +144,256 @@ __global__ __launch_bounds__(kFinalThreads) void stats_final_kernel(
     out[0] = mean_c;
     if (sc) {
       const double eps = rel > 0.0 ? rel * mean_c : eps_abs;
+      sc->eps = eps;
+      sc->eps_f = static_cast<float>(eps);
+      sc->c = static_cast<float>(2.0 * s * 1.4426950408889634 / eps);
+      sc->inv_rho_a = tau_a == 1.0 ? 0.f : static_cast<float>((1.0 - tau_a) / (eps * tau_a));
+      sc->inv_rho_b = tau_b == 1.0 ? 0.f : static_cast<float>((1.0 - tau_b) / (eps * tau_b));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Standard deviation of the cost matrix without forming it
+// (geometry/geometry.py:119-136: sqrt(relu(mean(C^2) - mean(C)^2)), the means
+// uniform over the n m entries).  With u = x - mu, v = y - mu (the cost is
+// invariant under a common shift) and D_ij = |u_i - v_j|^2 = C_ij / s:
+//   E[D]   = E|u|^2 + E|v|^2 - 2 <E u, E v>
+//   E[D^2] = E|u|^4 + E|v|^4 + 2 E|u|^2 E|v|^2 + 4 tr(S_u S_v)
+//            - 4 <E[|u|^2 u], E v> - 4 <E u, E[|v|^2 v]>
+// with S_u = E[u u^T].  Per cloud the terms are row sums (vector segment) and
+// the Gram matrix U^T U (Gram segment, 32 x 32 tiles of the upper triangle);
+// everything is float64, so the cancellation in E[D^2] - E[D]^2 costs no
+// accuracy that float32 inputs have.
+// ---------------------------------------------------------------------------
+constexpr int kMomGramTile = 32;
+
+struct MomentArgs {
+  const float* pts[2];
+  int64_t np[2];
+  int d;
+  int nkb;                // feature blocks of 32
+  int tiles;              // Gram tiles (kb <= lb pairs)
+  const float* mu;
+  int vchunks[2];
+  int64_t vrows[2];
+  double* vpart[2];       // [chunk][2 d + 2]: sum u_k, sum |u|^2 u_k, sum |u|^2, sum |u|^4
+  int gchunks[2];
+  int64_t grows[2];
+  double* gpart[2];       // [chunk][tile][32 * 32]
+  int seg[5];             // first block of: vec x, vec y, Gram x, Gram y; seg[4] = grid
+};
+
+// Rows of each moment chunk: a vector chunk holds at most 32 rows per wave,
+// a Gram chunk a share of the rows that keeps about 8 chunks per cloud.
+inline void moment_plan(int64_t np, int d, int* vchunks, int64_t* vrows, int* gchunks, int64_t* grows) {
+  *vrows = std::max<int64_t>(128, (np + 127) / 128);
+  *vchunks = static_cast<int>((np + *vrows - 1) / *vrows);
+  *grows = std::max<int64_t>(256, ((np + 7) / 8 + 63) / 64 * 64);
+  *gchunks = static_cast<int>((np + *grows - 1) / *grows);
+  (void)d;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Vector moments of rows [r0, r1): wave w takes rows r0 + w, r0 + w + 4, ...,
+// eight at a time with every load issued first; lane l holds features
+// l, l + 64, l + 128, l + 192.
+__device__ void moment_vec_block(const float* __restrict__ pts, int64_t r0, int64_t r1, int d,
+                                 const float* __restrict__ mu, double* __restrict__ out) {
+  constexpr int Q = 4, U = 8;
+  __shared__ double s_red[4][2 * 256 + 2];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  double mk[Q], au[Q], awu[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int k = lane + 64 * q;
+    mk[q] = k < d ? static_cast<double>(mu[k]) : 0.0;
+    au[q] = awu[q] = 0.0;
+  }
+  double aw = 0.0, aw2 = 0.0;
+  for (int64_t rb = r0 + w; rb < r1; rb += 4 * U) {
+    float v[U][Q];
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) {
+      const int64_t r = min(rb + 4 * uu, r1 - 1);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int k = lane + 64 * q;
+        v[uu][q] = k < d ? pts[r * d + k] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) {
+      if (rb + 4 * uu >= r1) break;  // uniform over the wave
+      double u[Q], sq = 0.0;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        u[q] = (lane + 64 * q < d) ? static_cast<double>(v[uu][q]) - mk[q] : 0.0;
+        sq += u[q] * u[q];
+      }
+      sq = wave_sum_f64(sq);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        au[q] += u[q];
+        awu[q] += sq * u[q];
+      }
+      aw += sq;
+      aw2 += sq * sq;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int k = lane + 64 * q;
+    if (k < d) {
+      s_red[w][k] = au[q];
+      s_red[w][d + k] = awu[q];
+    }
+  }
+  if (lane == 0) {
+    s_red[w][2 * d] = aw;
+    s_red[w][2 * d + 1] = aw2;
+  }
+  __syncthreads();
+  for (int e = tid; e < 2 * d + 2; e += 256) out[e] = (s_red[0][e] + s_red[1][e]) + (s_red[2][e] + s_red[3][e]);
+}
+
+// Gram tile (kb, lb) of rows [r0, r1): sum_i u_ik u_il for k in block kb, l in
+// block lb; 64 rows at a time through LDS, each thread four entries.
+__device__ void moment_gram_block(const float* __restrict__ pts, int64_t r0, int64_t r1, int d,
+                                  const float* __restrict__ mu, int kb, int lb, double* __restrict__ out) {
+  constexpr int TT = kMomGramTile;
+  __shared__ double s_k[64][TT + 1], s_l[64][TT + 1];
+  const int tid = threadIdx.x, kk = tid & (TT - 1), l4 = (tid >> 5) * 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int c = tid & (TT - 1);
+  const int kq = kb * TT + c, lq = lb * TT + c;
+  const double mkq = kq < d ? static_cast<double>(mu[kq]) : 0.0;
+  const double mlq = lq < d ? static_cast<double>(mu[lq]) : 0.0;
+  for (int64_t rs = r0; rs < r1; rs += 64) {
+    float vk[8], vl[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = (tid >> 5) + 8 * i;
+      const int64_t row = min(rs + r, r1 - 1);
+      vk[i] = kq < d ? pts[row * d + kq] : 0.f;
+      vl[i] = lq < d ? pts[row * d + lq] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = (tid >> 5) + 8 * i;
+      const bool in = rs + r < r1;
+      s_k[r][c] = (in && kq < d) ? static_cast<double>(vk[i]) - mkq : 0.0;
+      s_l[r][c] = (in && lq < d) ? static_cast<double>(vl[i]) - mlq : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int r = 0; r < 64; ++r) {
+      const double uk = s_k[r][kk];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[e] = fma(uk, s_l[r][l4 + e], acc[e]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) out[kk * TT + l4 + e] = acc[e];
+}
+
+// Gram tile index t -> (kb, lb), kb <= lb, row-major over the upper triangle.
+__device__ __forceinline__ void gram_tile_blocks(int t, int nkb, int* kb, int* lb) {
+  int k = 0;
+  while (t >= nkb - k) {
+    t -= nkb - k;
+    ++k;
+  }
+  *kb = k;
+  *lb = k + t;
+}
+
+__global__ __launch_bounds__(256) void moments_kernel(const MomentArgs a) {
+  const int bid = blockIdx.x;
+  if (bid < a.seg[2]) {
+    const int side = bid >= a.seg[1] ? 1 : 0;
+    const int chunk = bid - a.seg[side];
+    const int64_t r0 = static_cast<int64_t>(chunk) * a.vrows[side];
+    const int64_t r1 = min(a.np[side], r0 + a.vrows[side]);
+    moment_vec_block(a.pts[side], r0, r1, a.d, a.mu, a.vpart[side] + static_cast<int64_t>(chunk) * (2 * a.d + 2));
+    return;
+  }
+  const int side = bid >= a.seg[3] ? 1 : 0;
+  const int local = bid - a.seg[2 + side];
+  const int chunk = local / a.tiles, tile = local % a.tiles;
+  int kb, lb;
+  gram_tile_blocks(tile, a.nkb, &kb, &lb);
+  const int64_t r0 = static_cast<int64_t>(chunk) * a.grows[side];
+  const int64_t r1 = min(a.np[side], r0 + a.grows[side]);
+  moment_gram_block(a.pts[side], r0, r1, a.d, a.mu, kb, lb,
+                    a.gpart[side] + (static_cast<int64_t>(chunk) * a.tiles + tile) * kMomGramTile * kMomGramTile);
+}
+
+// One workgroup: sums the moment partials, writes std(C) to out[1] and, when
+// ``sc`` is given and rel > 0, the solve's scalars with eps = rel * std(C).
+__global__ __launch_bounds__(1024) void std_final_kernel(const MomentArgs a, double s, double* __restrict__ out,
+                                                        SolveScalars* sc, double rel, double tau_a, double tau_b) {
+  constexpr int NT = 1024, TT2 = kMomGramTile * kMomGramTile;
+  __shared__ double s_vec[2][2 * 256 + 2];
+  __shared__ double s_red[4][NT];
+  const int t = threadIdx.x;
+  const int d = a.d, V = 2 * d + 2;
+  for (int e = t; e < 2 * V; e += NT) {
+    const int side = e / V, v = e % V;
+    const double* p = a.vpart[side] + v;
+    double acc = 0.0;
+    for (int c = 0; c < a.vchunks[side]; ++c) acc += p[static_cast<int64_t>(c) * V];
+    s_vec[side][v] = acc;
+  }
+  // tr(G_u G_v): both Gram matrices summed over their chunks, entry by entry;
+  // an off-diagonal tile stands for its mirror image too
+  double tr = 0.0;
+  for (int e = t; e < a.tiles * TT2; e += NT) {
+    const int tile = e / TT2;
+    int kb, lb;
+    gram_tile_blocks(tile, a.nkb, &kb, &lb);
+    double gx = 0.0, gy = 0.0;
+    for (int c = 0; c < a.gchunks[0]; ++c) gx += a.gpart[0][static_cast<int64_t>(c) * a.tiles * TT2 + e];
+    for (int c = 0; c < a.gchunks[1]; ++c) gy += a.gpart[1][static_cast<int64_t>(c) * a.tiles * TT2 + e];
+    tr += (kb == lb ? 1.0 : 2.0) * gx * gy;
+  }
+  __syncthreads();
+  const double n = static_cast<double>(a.np[0]), m = static_cast<double>(a.np[1]);
+  double d_uv = 0.0, d_wu_v = 0.0, d_u_wv = 0.0;
+  if (t < d) {
+    const double mu_u = s_vec[0][t] / n, mu_v = s_vec[1][t] / m;
+    d_uv = mu_u * mu_v;
+    d_wu_v = (s_vec[0][d + t] / n) * mu_v;
+    d_u_wv = mu_u * (s_vec[1][d + t] / m);
+  }
+  s_red[0][t] = tr;
+  s_red[1][t] = d_uv;
+  s_red[2][t] = d_wu_v;
+  s_red[3][t] = d_u_wv;
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if (t < st)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) s_red[q][t] += s_red[q][t + st];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double e2u = s_vec[0][2 * d] / n, e4u = s_vec[0][2 * d + 1] / n;
+    const double e2v = s_vec[1][2 * d] / m, e4v = s_vec[1][2 * d + 1] / m;
+    const double trs = s_red[0][0] / (n * m);
+    const double ed = e2u + e2v - 2.0 * s_red[1][0];
+    const double ed2 = e4u + e4v + 2.0 * e2u * e2v + 4.0 * trs - 4.0 * s_red[2][0] - 4.0 * s_red[3][0];
+    const double var = ed2 - ed * ed;
+    const double sd = s * sqrt(var > 0.0 ? var : 0.0);
+    out[1] = sd;
+    if (sc && rel > 0.0) {
+      const double eps = rel * sd;
       sc->eps = eps;
       sc->eps_f = static_cast<float>(eps);
       sc->c = static_cast<float>(2.0 * s * 1.4426950408889634 / eps);
@@ -728,10 +982,12 @@ __global__ void zero_words_kernel(uint32_t* p, int count) {
 
 __device__ __forceinline__ void block_max_store(float v, uint32_t* out) {
   __shared__ float red[4];
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  // NaN-propagating (jnp.max returns NaN); a NaN's bits exceed every
+  // non-negative float's, so the unsigned atomicMax keeps it too
+  for (int off = 32; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) atomicMax(out, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+  if (threadIdx.x == 0) atomicMax(out, __float_as_uint(nan_max(nan_max(red[0], red[1]), nan_max(red[2], red[3]))));
 }
 
 __global__ __launch_bounds__(256) void max_sq_norm_kernel(const float* __restrict__ p, int64_t np, int d,
@@ -1333,12 +1589,82 @@ bool even_tiles(const KernelCfg& k, const PassPlan& p) {
   return p.rb_per_split > 0 && p.rb_per_split % (2 * k.mt) == 0 && p.n_in_rb % (2 * k.mt) == 0;
 }
 
+// Partial buffers of the std(C) moments (offsets relative to their region).
+struct MomentLayout {
+  int vchunks[2], gchunks[2];
+  int64_t vrows[2], grows[2];
+  int nkb, tiles;
+  size_t vpart[2], gpart[2], total;
+};
+
+MomentLayout moment_layout(int64_t n, int64_t m, int d) {
+  MomentLayout M{};
+  moment_plan(n, d, &M.vchunks[0], &M.vrows[0], &M.gchunks[0], &M.grows[0]);
+  moment_plan(m, d, &M.vchunks[1], &M.vrows[1], &M.gchunks[1], &M.grows[1]);
+  M.nkb = (d + kMomGramTile - 1) / kMomGramTile;
+  M.tiles = M.nkb * (M.nkb + 1) / 2;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = (off + bytes + 255) & ~static_cast<size_t>(255);
+    return at;
+  };
+  for (int side = 0; side < 2; ++side)
+    M.vpart[side] = take(static_cast<size_t>(M.vchunks[side]) * (2 * d + 2) * sizeof(double));
+  for (int side = 0; side < 2; ++side)
+    M.gpart[side] = take(static_cast<size_t>(M.gchunks[side]) * M.tiles * kMomGramTile * kMomGramTile *
+                         sizeof(double));
+  M.total = off;
+  return M;
+}
+
+// Moments of both clouds and std(C) (two launches after the statistics, which
+// wrote mu): out[1] = std(C); with sc and rel > 0 the solve's scalars follow it.
+int launch_std(const float* x, int64_t n, const float* y, int64_t m, int d, const float* mu, double s,
+               char* region, const MomentLayout& M, double* out, SolveScalars* sc, double rel, double tau_a,
+               double tau_b, hipStream_t st) {
+  MomentArgs a{};
+  a.pts[0] = x;
+  a.pts[1] = y;
+  a.np[0] = n;
+  a.np[1] = m;
+  a.d = d;
+  a.nkb = M.nkb;
+  a.tiles = M.tiles;
+  a.mu = mu;
+  int cur = 0;
+  for (int side = 0; side < 2; ++side) {
+    a.vchunks[side] = M.vchunks[side];
+    a.vrows[side] = M.vrows[side];
+    a.vpart[side] = reinterpret_cast<double*>(region + M.vpart[side]);
+    a.gchunks[side] = M.gchunks[side];
+    a.grows[side] = M.grows[side];
+    a.gpart[side] = reinterpret_cast<double*>(region + M.gpart[side]);
+    a.seg[side] = cur;
+    cur += M.vchunks[side];
+  }
+  for (int side = 0; side < 2; ++side) {
+    a.seg[2 + side] = cur;
+    cur += M.gchunks[side] * M.tiles;
+  }
+  a.seg[4] = cur;
+  hipLaunchKernelGGL(moments_kernel, dim3(cur), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(std_final_kernel, dim3(1), dim3(1024), 0, st, a, s, out, sc, rel, tau_a, tau_b);
+  return hipGetLastError() == hipSuccess ? OTTX_OK : OTTX_EHIP;
+}
+
 struct Layout {
   size_t frag_x, frag_y, norm_x, norm_y, g_save, lse_x, lse_y, partial, counters, err_col, mass, err_row, state,
-      dual, dual_part, scal, stats_x, stats_y, stats_out, mu, w_x, w_y, total;
+      dual, dual_part, scal, stats_x, stats_y, stats_out, mu, w_x, w_y, moments, total;
   int chunks_x, chunks_y;
   int64_t rows_chunk_x, rows_chunk_y;
+  MomentLayout mom;
 };
+
+// A relative epsilon on std(C) needs the moment partials.
+inline bool wants_std(const ottx_problem* pb) {
+  return pb->relative_epsilon > 0.0 && pb->epsilon_statistic == OTTX_EPS_STAT_STD;
+}
 
 // At most 128 chunks: the single-workgroup final pass reads every chunk, and
 // its latency grows with their number.
@@ -1387,6 +1713,10 @@ Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& g
   // uniform weights when the caller passes none (linear_problem.py:65-79)
   L.w_x = pb->a ? 0 : take(static_cast<size_t>(pb->n) * sizeof(float));
   L.w_y = pb->b ? 0 : take(static_cast<size_t>(pb->m) * sizeof(float));
+  if (wants_std(pb)) {
+    L.mom = moment_layout(pb->n, pb->m, pb->d);
+    L.moments = take(L.mom.total);
+  }
   (void)outer;
   L.total = off;
   return L;
@@ -1433,6 +1763,7 @@ int check_problem(const ottx_problem* pb) {
   if (!(pb->inv_scale_cost > 0.0) || pb->relative_epsilon < 0.0) return OTTX_EINVAL;
   if (!(pb->tau_a > 0.0 && pb->tau_a <= 1.0) || !(pb->tau_b > 0.0 && pb->tau_b <= 1.0)) return OTTX_EINVAL;
   if (pb->precision != OTTX_PREC_FP32 && pb->precision != OTTX_PREC_BF16_SPLIT) return OTTX_EINVAL;
+  if (pb->epsilon_statistic != OTTX_EPS_STAT_MEAN && pb->epsilon_statistic != OTTX_EPS_STAT_STD) return OTTX_EINVAL;
   return OTTX_OK;
 }
 
@@ -1488,9 +1819,15 @@ int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* w
                      pb->n, L.rows_chunk_x, L.chunks_x, pb->y, pb->m, L.rows_chunk_y, L.chunks_y, pb->d, sx, sy,
                      probe ? init->f : nullptr, pb->inv_scale_cost, m_pad, probe ? init->lse_g : nullptr);
   hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(kFinalThreads), 0, st, sx, L.chunks_x, sy, L.chunks_y, pb->d,
-                     pb->n, pb->m, pb->inv_scale_cost, P->stats_out, P->mu, P->scal, pb->relative_epsilon,
-                     pb->epsilon, pb->tau_a, pb->tau_b);
+                     pb->n, pb->m, pb->inv_scale_cost, P->stats_out, P->mu, P->scal,
+                     wants_std(pb) ? 0.0 : pb->relative_epsilon, pb->epsilon, pb->tau_a, pb->tau_b);
   OTTX_LAUNCHED();
+  if (wants_std(pb)) {
+    // epsilon = relative_epsilon * std(C), resolved on the device like the mean
+    const int rc = launch_std(pb->x, pb->n, pb->y, pb->m, pb->d, P->mu, pb->inv_scale_cost, base + L.moments, L.mom,
+                              P->stats_out, P->scal, pb->relative_epsilon, pb->tau_a, pb->tau_b, st);
+    if (rc != OTTX_OK) return rc;
+  }
   PrepArgs pa{};
   const int chunks = k.frag_chunks();
   pa.pts[0] = pb->x;
@@ -1668,6 +2005,46 @@ int ottx_mean_cost(int64_t n, int64_t m, int32_t d, const float* x, const float*
   OTTX_LAUNCHED();
   OTTX_HIP(hipMemcpyAsync(out_host, o, sizeof(double), hipMemcpyDeviceToHost, st));
   OTTX_HIP(hipStreamSynchronize(st));
+  return OTTX_OK;
+}
+
+size_t ottx_cost_moments_workspace_bytes(int64_t n, int64_t m, int32_t d) {
+  if (n <= 0 || m <= 0 || d <= 0 || d > 256) return 0;
+  return ottx_mean_cost_workspace_bytes(n, m, d) + align_up(static_cast<size_t>(d) * sizeof(float)) +
+         moment_layout(n, m, d).total;
+}
+
+int ottx_cost_moments(int64_t n, int64_t m, int32_t d, const float* x, const float* y, double inv_scale_cost,
+                      double* out_host, void* workspace, size_t workspace_bytes, void* stream) {
+  if (n <= 0 || m <= 0 || d <= 0 || d > 256 || !x || !y || !out_host || !workspace) return OTTX_EINVAL;
+  if (!(inv_scale_cost > 0.0)) return OTTX_EINVAL;
+  if (workspace_bytes < ottx_cost_moments_workspace_bytes(n, m, d)) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int cx, cy;
+  int64_t rx, ry;
+  stats_chunks(n, &cx, &rx);
+  stats_chunks(m, &cy, &ry);
+  char* base = static_cast<char*>(workspace);
+  double* sx = reinterpret_cast<double*>(base);
+  double* sy = reinterpret_cast<double*>(base + align_up(static_cast<size_t>(cx) * d * 16));
+  const size_t stats_bytes = ottx_mean_cost_workspace_bytes(n, m, d);
+  double* o = reinterpret_cast<double*>(base + stats_bytes - align_up(8 * sizeof(double)));
+  float* mu = reinterpret_cast<float*>(base + stats_bytes);
+  char* region = base + stats_bytes + align_up(static_cast<size_t>(d) * sizeof(float));
+  hipLaunchKernelGGL(stats_partial_kernel, dim3(cx), dim3(256), 0, st, x, n, d, rx, sx);
+  hipLaunchKernelGGL(stats_partial_kernel, dim3(cy), dim3(256), 0, st, y, m, d, ry, sy);
+  hipLaunchKernelGGL(stats_final_kernel, dim3(1), dim3(kFinalThreads), 0, st, sx, cx, sy, cy, d, n, m,
+                     inv_scale_cost, o, mu, static_cast<SolveScalars*>(nullptr), 0.0, 0.0, 1.0, 1.0);
+  OTTX_LAUNCHED();
+  const int rc = launch_std(x, n, y, m, d, mu, inv_scale_cost, region, moment_layout(n, m, d), o, nullptr, 0.0, 1.0,
+                            1.0, st);
+  if (rc != OTTX_OK) return rc;
+  double host[2];
+  OTTX_HIP(hipMemcpyAsync(host, o, sizeof(host), hipMemcpyDeviceToHost, st));
+  OTTX_HIP(hipStreamSynchronize(st));
+  out_host[0] = host[0];
+  out_host[1] = host[1];
   return OTTX_OK;
 }
 
@@ -2135,6 +2512,20 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
         status = launch_all(st);
         if (status != OTTX_OK) break;
       }
+      if (o->progress_fn != nullptr) {
+        // progress reporting: every block is waited for, so the mailbox's
+        // error is this block's when the callback reads it
+        const bool stopped = wait_blocks(blk + 1);
+        if (status != OTTX_OK) break;
+        bool st_bit = false;
+        if (progress(&st_bit) >= blk + 1) {
+          double e = 0.0;
+          __atomic_load(&mail->last_err, &e, __ATOMIC_ACQUIRE);
+          o->progress_fn(o->progress_user, blk, e);
+        }
+        if (stopped) break;
+        continue;
+      }
       if (blk == 0) continue;
       // One block runs ahead: wait for the previous block.
       if (wait_blocks(blk)) break;
@@ -2350,7 +2741,8 @@ __global__ void batch_stats_kernel(const BatchItem* __restrict__ items, int d, d
       b1 += red[2][g * d + t];
       b2 += red[3][g * d + t];
     }
-    it.mu[t] = static_cast<float>(0.5 * (a1 / static_cast<double>(it.n) + b1 / static_cast<double>(it.m)));
+    const double shift = 0.5 * (a1 / static_cast<double>(it.n) + b1 / static_cast<double>(it.m));
+    it.mu[t] = isfinite(shift) ? static_cast<float>(shift) : 0.f;  // as stats_final_kernel
     nx2 = a2;
     ny2 = b2;
     dot = (a1 / it.n_div) * (b1 / it.m_div);

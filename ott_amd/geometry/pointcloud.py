@@ -46,7 +46,9 @@ class PointCloud:
     scale_cost: a positive float ``s`` so that ``C = ||x - y||^2 / s``.
     epsilon: float, :class:`Epsilon`, or the multiplier when
       ``relative_epsilon`` is set.
-    relative_epsilon: ``"mean"`` (closed form on the GPU) or ``None``.
+    relative_epsilon: ``"mean"`` or ``"std"`` (both in closed form on the
+      GPU, the cost matrix is never formed) or ``None``; with ``epsilon`` also
+      ``None`` the epsilon is 0.05 * std of the cost, the reference's default.
     precision: ``"fp32"`` (f32 MFMA cross term) or ``"bf16_split"`` (hi/lo
       bfloat16 operands, f32 accumulation).  Not part of OTT's signature.
   """
@@ -83,6 +85,7 @@ class PointCloud:
     self.precision = precision
     self._eps_cache: Optional[eps_scheduler.Epsilon] = None
     self._mean_cache: Optional[float] = None
+    self._std_cache: Optional[float] = None
     self._inv_scale_cache: Optional[float] = None
 
   # -- shape / dtype ---------------------------------------------------------
@@ -181,26 +184,55 @@ class PointCloud:
     return self._mean_cache
 
   @property
+  def std_cost_matrix(self) -> float:
+    """geometry.py:119-136: sqrt(relu(mean(C^2) - mean(C)^2)), in closed form
+    on the GPU from the clouds' moments (the reference forms C^2)."""
+    if self._std_cache is None:
+      lib = _lib.load_library()
+      n, m = self.shape
+      d = self.x.shape[1]
+      nbytes = lib.ottx_cost_moments_workspace_bytes(n, m, d)
+      if nbytes == 0:
+        raise NotImplementedError(f"std_cost_matrix supports d <= 256, got d={d}")
+      ws = _lib.workspace(nbytes, self.device)
+      out = (ctypes.c_double * 2)()
+      _lib.check("ottx_cost_moments", lib.ottx_cost_moments(
+          n, m, d, self.x.data_ptr(), self.y.data_ptr(), self.inv_scale_cost, out, ws.data_ptr(), ws.numel(),
+          _lib.stream_ptr()))
+      if self._mean_cache is None:
+        self._mean_cache = float(out[0])
+      self._std_cache = float(out[1])
+    return self._std_cache
+
+  def _relative_statistic(self) -> Optional[int]:
+    """The cost statistic the epsilon multiplies (None: an absolute epsilon)
+    and, through it, geometry.py:147-171's branches."""
+    init = self._epsilon_init
+    if isinstance(init, eps_scheduler.Epsilon):
+      return None
+    if self._relative_epsilon is None:
+      return None if init is not None else _lib.EPS_STAT_STD
+    if self._relative_epsilon == "std":
+      return _lib.EPS_STAT_STD
+    if self._relative_epsilon == "mean":
+      return _lib.EPS_STAT_MEAN
+    raise ValueError(f"Invalid relative epsilon: {self._relative_epsilon}.")
+
+  @property
   def epsilon_scheduler(self) -> eps_scheduler.Epsilon:
     """geometry.py:147-171."""
     if self._eps_cache is not None:
       return self._eps_cache
     init = self._epsilon_init
+    stat = self._relative_statistic()
     if isinstance(init, eps_scheduler.Epsilon):
       sched = init
-    elif self._relative_epsilon is None:
-      if init is None:
-        raise NotImplementedError(
-            "the default epsilon (0.05 * std of the cost) is not supported yet; "
-            "pass epsilon=... or relative_epsilon='mean'")
+    elif stat is None:
       sched = eps_scheduler.Epsilon(float(init))
-    elif self._relative_epsilon == "mean":
-      mult = eps_scheduler.DEFAULT_EPSILON_SCALE if init is None else float(init)
-      sched = eps_scheduler.Epsilon(mult * self.mean_cost_matrix)
-    elif self._relative_epsilon == "std":
-      raise NotImplementedError("relative_epsilon='std' is not supported yet")
     else:
-      raise ValueError(f"Invalid relative epsilon: {self._relative_epsilon}.")
+      mult = eps_scheduler.DEFAULT_EPSILON_SCALE if init is None else float(init)
+      scale = self.mean_cost_matrix if stat == _lib.EPS_STAT_MEAN else self.std_cost_matrix
+      sched = eps_scheduler.Epsilon(mult * scale)
     self._eps_cache = sched
     return sched
 
@@ -208,18 +240,19 @@ class PointCloud:
   def epsilon(self) -> float:
     return float(self.epsilon_scheduler.target)
 
-  def _epsilon_spec(self) -> Tuple[float, float]:
-    """(absolute epsilon, relative multiplier) for the C ABI.
+  def _epsilon_spec(self) -> Tuple[float, float, int]:
+    """(absolute epsilon, relative multiplier, statistic) for the C ABI.
 
-    A not-yet-resolved ``relative_epsilon="mean"`` is passed as a multiplier so
-    the solve resolves it on the device without a host round trip.
+    A not-yet-resolved relative epsilon (``"mean"``, ``"std"`` or the default
+    0.05 * std) is passed as a multiplier so the solve resolves it on the
+    device without a host round trip.
     """
-    if (self._eps_cache is None and self._relative_epsilon == "mean" and
-        not isinstance(self._epsilon_init, eps_scheduler.Epsilon)):
+    stat = self._relative_statistic()
+    if self._eps_cache is None and stat is not None:
       mult = (eps_scheduler.DEFAULT_EPSILON_SCALE if self._epsilon_init is None
               else float(self._epsilon_init))
-      return 0.0, mult
-    return self.epsilon, 0.0
+      return 0.0, mult, stat
+    return self.epsilon, 0.0, _lib.EPS_STAT_MEAN
 
   def _set_resolved_epsilon(self, eps: float) -> None:
     if self._eps_cache is None:
@@ -257,7 +290,8 @@ class PointCloud:
   # -- the operator protocol -------------------------------------------------
   def _problem(self, a: Optional[torch.Tensor] = None, b: Optional[torch.Tensor] = None,
                tau_a: float = 1.0, tau_b: float = 1.0, epsilon: Optional[float] = None,
-               relative_epsilon: float = 0.0) -> Tuple[_lib.Problem, Tuple]:
+               relative_epsilon: float = 0.0,
+               epsilon_statistic: int = _lib.EPS_STAT_MEAN) -> Tuple[_lib.Problem, Tuple]:
     if not self.is_squared_euclidean:
       raise NotImplementedError("ott_amd accelerates the squared-Euclidean cost only")
     n, m = self.shape
@@ -267,7 +301,7 @@ class PointCloud:
         x=self.x.data_ptr(), y=self.y.data_ptr(), a=_lib.ptr(a), b=_lib.ptr(b),
         epsilon=float(self.epsilon if epsilon is None else epsilon),
         inv_scale_cost=self.inv_scale_cost, tau_a=float(tau_a), tau_b=float(tau_b),
-        relative_epsilon=float(relative_epsilon))
+        relative_epsilon=float(relative_epsilon), epsilon_statistic=int(epsilon_statistic))
     return pb, (a, b)
 
   def _op(self, op: int, f, g, eps: float, axis: int,

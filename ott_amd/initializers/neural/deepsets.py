@@ -330,7 +330,9 @@ class DeepSetsInitializer(SinkhornInitializer):
   def _potential(self, ot_prob, side: str, out: Optional[torch.Tensor] = None) -> torch.Tensor:
     geom = ot_prob.geom
     inv_s = geom.inv_scale_cost  # the network works in units of |x - y|^2
-    eps_abs, eps_rel = geom._epsilon_spec()
+    eps_abs, eps_rel, stat = geom._epsilon_spec()
+    if eps_abs == 0.0 and stat != _lib.EPS_STAT_MEAN:
+      eps_abs = geom.epsilon  # a multiple of std(C): resolved first
     if eps_abs > 0.0:
       eps, rel = eps_abs / inv_s, None
     else:

@@ -1,8 +1,11 @@
 """Momentum (solvers/linear/acceleration.py:127-172 in OTT).
 
-Only the constant form (``start == 0``) runs on the GPU path; the adaptive
-Lehmann momentum and Anderson acceleration raise ``NotImplementedError`` in
-:class:`~ott_amd.solvers.linear.sinkhorn.Sinkhorn`.
+Constant momentum (``start == 0``) and the adaptive Lehmann momentum
+(``start > 0``, acceleration.py:138-160) run on the GPU path; the adaptive
+form needs ``norm_error=1``, ``start >= 2 inner_iterations`` and sequential
+updates (:class:`~ott_amd.solvers.linear.sinkhorn.Sinkhorn` raises
+``NotImplementedError`` otherwise).  Anderson acceleration is not run; its
+class exists so that passing it raises the same error.
 """
 
 from __future__ import annotations

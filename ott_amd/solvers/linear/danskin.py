@@ -29,12 +29,14 @@ class _RegOTCost(torch.autograd.Function):
     out = solve(geom, a=None if a is None else a.detach(), b=None if b is None else b.detach(), **solve_kwargs)
     ctx.out = out
     ctx.has_a, ctx.has_b = a is not None, b is not None
+    # gradients go back to each input's own device
+    ctx.devices = tuple(None if t is None else t.device for t in (x, y, a, b))
     return torch.tensor(out.reg_ot_cost, dtype=torch.float64, device=x.device)
 
   @staticmethod
   def backward(ctx, grad_out):
     out = ctx.out
-    w = grad_out.to(torch.float64)
+    w = grad_out.to(device=out.f.device, dtype=torch.float64)
     need = ctx.needs_input_grad
     gx = gy = ga = gb = None
     if need[0] or need[1]:
@@ -47,7 +49,8 @@ class _RegOTCost(torch.autograd.Function):
       wa, wb = out._weight_grads()
       ga = (w * wa).float() if need[2] and ctx.has_a else None
       gb = (w * wb).float() if need[3] and ctx.has_b else None
-    return gx, gy, ga, gb, None, None
+    grads = [None if gr is None else gr.to(dev) for gr, dev in zip((gx, gy, ga, gb), ctx.devices)]
+    return (*grads, None, None)
 
 
 def reg_ot_cost(x: torch.Tensor, y: torch.Tensor, a: Optional[torch.Tensor] = None,
@@ -63,5 +66,12 @@ def reg_ot_cost(x: torch.Tensor, y: torch.Tensor, a: Optional[torch.Tensor] = No
     raise NotImplementedError("reg_ot_cost gradients support balanced problems only")
   if x.shape[1] > 64:
     raise NotImplementedError("reg_ot_cost gradients support at most 64 features")
+  # the gradient here holds f and g fixed, which is the reference's gradient
+  # only under use_danskin (the default, sinkhorn.py:329-335); without it the
+  # reference differentiates through the potentials as well
+  if solve_kwargs.get("use_danskin", True) is False or (
+      "implicit_diff" in solve_kwargs and solve_kwargs["implicit_diff"] is None and
+      solve_kwargs.get("use_danskin") is None):
+    raise NotImplementedError("reg_ot_cost gradients are the use_danskin ones (f and g held fixed)")
   geom_kwargs = dict(epsilon=float(epsilon), scale_cost=float(scale_cost), precision=precision)
   return _RegOTCost.apply(x, y, a, b, geom_kwargs, solve_kwargs)

@@ -15,6 +15,7 @@ import ctypes
 import math
 from typing import Any, Callable, NamedTuple, Optional, Tuple
 
+import numpy as np
 import torch
 
 from ott_amd import _lib, tune
@@ -213,7 +214,9 @@ class Sinkhorn:
   ``Epsilon(target, init, decay)`` on balanced problems; sequential updates,
   and parallel updates for the symmetric problems of a Sinkhorn divergence
   (the same cloud and weights on both sides, from the default start); balanced
-  or unbalanced problems; warm starts through ``init`` or an ``initializer``.  Options outside that set raise ``NotImplementedError``;
+  or unbalanced problems; warm starts through ``init`` or an ``initializer``;
+  ``progress_fn``, called per iteration once each block has run (the solve
+  then waits for every block).  Options outside that set raise ``NotImplementedError``;
   there is no silent fallback.  ``implicit_diff`` and ``use_danskin`` only
   affect reverse mode, which this package does not provide.
 
@@ -304,8 +307,6 @@ class Sinkhorn:
         raise NotImplementedError("adaptive momentum is supported with norm_error=1 and sequential updates")
       if self.momentum.start < 2 * self.inner_iterations:
         raise NotImplementedError("adaptive momentum needs start >= 2 * inner_iterations")
-    if self.progress_fn is not None:
-      raise NotImplementedError("progress_fn is not supported by ott_amd yet")
     if int(self._norm_error) < 1:
       raise ValueError("norm_error must be >= 1")
 
@@ -373,10 +374,10 @@ def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
   if f.shape != (n,) or g.shape != (m,):
     raise ValueError(f"init potentials must be [{n}] and [{m}]")
   a, b = ot_prob._a, ot_prob._b  # None: uniform, set inside the library
-  eps_abs, eps_rel = geom._epsilon_spec()
+  eps_abs, eps_rel, eps_stat = geom._epsilon_spec()
   sched_init, sched_decay = geom._schedule()
   pb, keep = geom._problem(a=a, b=b, tau_a=ot_prob.tau_a, tau_b=ot_prob.tau_b,
-                           epsilon=eps_abs, relative_epsilon=eps_rel)
+                           epsilon=eps_abs, relative_epsilon=eps_rel, epsilon_statistic=eps_stat)
   opts = _lib.SinkhornOpts(
       threshold=float(solver.threshold), norm_error=int(solver._norm_error),
       inner_iterations=int(solver.inner_iterations),
@@ -391,6 +392,9 @@ def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
       epsilon_init=sched_init, epsilon_decay=sched_decay,
       momentum_start=int(solver.momentum.start),
       momentum_error_threshold=float(solver.momentum.error_threshold))
+  progress = _progress_callback(solver)
+  if progress is not None:
+    opts.progress_fn = progress.cfunc
   errors = torch.empty((solver.outer_iterations,), dtype=torch.float32, device=geom.device)
   out = _lib.SinkhornOut(f=f.data_ptr(), g=g.data_ptr(), errors=errors.data_ptr())
   nbytes = lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts))
@@ -403,11 +407,56 @@ def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
     _lib.check("ottx_sinkhorn_solve", lib.ottx_sinkhorn_solve(
         ctypes.byref(pb), ctypes.byref(opts), ctypes.byref(out), ws.data_ptr(),
         ws.numel(), stream))
+    if progress is not None and progress.error is not None:
+      raise progress.error
     geom._set_resolved_epsilon(out.epsilon)
     return SinkhornOutput(
         potentials=(f, g), errors=errors, reg_ot_cost=float(out.reg_ot_cost),
         ot_prob=ot_prob, threshold=float(solver.threshold),
         converged=bool(out.converged), inner_iterations=int(solver.inner_iterations))
 
-  launch.keep = keep  # weights referenced by pb stay alive until the call
+  launch.keep = (keep, progress)  # weights referenced by pb, the callback, alive until the call
   return launch
+
+
+class ProgressState(NamedTuple):
+  """What a ``progress_fn`` receives as the solver state: ``errors`` as the
+  reference's ``SinkhornState.errors`` reads at that iteration ([outer, 1],
+  host float32; -1 for blocks not run, +inf for iterations without an
+  error).  The potentials stay on the GPU and are not part of it."""
+
+  errors: Any
+
+
+class _Progress:
+  """Per-iteration ``progress_fn`` calls (sinkhorn.py:933-937) from the
+  solve's per-block callback: block k stands for iterations k inner .. k inner
+  + inner - 1, whose state shows +inf in row k until its last iteration, which
+  records the block's error (+inf before ``min_iterations``,
+  sinkhorn.py:914-931)."""
+
+  def __init__(self, solver: "Sinkhorn"):
+    self.fn = solver.progress_fn
+    self.inner = int(solver.inner_iterations)
+    self.min_it = int(solver.min_iterations)
+    self.max_it = int(solver.max_iterations)
+    self.errors = np.full((solver.outer_iterations, 1), -1.0, dtype=np.float32)
+    self.error: Optional[BaseException] = None
+    self.cfunc = _lib.PROGRESS_FN(self._on_block)
+
+  def _on_block(self, user, block: int, err: float) -> None:
+    if self.error is not None:
+      return
+    try:
+      for j in range(self.inner):
+        it = block * self.inner + j
+        last = j == self.inner - 1
+        record = last and (it >= self.min_it or it == self.max_it - 1)
+        self.errors[block, 0] = np.float32(err) if record else np.inf
+        self.fn((it, self.inner, self.max_it, ProgressState(errors=self.errors.copy())))
+    except BaseException as e:  # re-raised by the solve's caller
+      self.error = e
+
+
+def _progress_callback(solver: "Sinkhorn") -> Optional[_Progress]:
+  return None if solver.progress_fn is None else _Progress(solver)

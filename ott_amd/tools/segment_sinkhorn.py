@@ -63,12 +63,13 @@ def segment_sinkhorn_outputs(
     indices_are_sorted: bool = False, num_per_segment_x: Optional[Sequence[int]] = None,
     num_per_segment_y: Optional[Sequence[int]] = None, weights_x=None, weights_y=None,
     sinkhorn_kwargs: Mapping[str, Any] = MappingProxyType({}), tau_a: float = 1.0, tau_b: float = 1.0,
-    padded_statistics: bool = True, **kwargs: Any) -> SegmentSinkhornOutput:
+    padded_statistics: bool = True, tile_config: int = -1, **kwargs: Any) -> SegmentSinkhornOutput:
   """All outputs of the batched solve (see ``segment_sinkhorn``).
 
   ``padded_statistics=True`` (OTT's behaviour) computes a relative epsilon
   over each segment padded to ``max_measure_size`` rows; ``False`` uses each
   segment's own points, i.e. what one ``PointCloud`` per segment would give.
+  ``tile_config`` (beyond OTT) forces one of ``ottx_tile_configs``' variants.
   """
   if cost_fn is not None and not isinstance(cost_fn, costs.SqEuclidean):
     raise NotImplementedError("ott_amd accelerates the squared-Euclidean cost only")
@@ -141,7 +142,7 @@ def segment_sinkhorn_outputs(
       threshold=float(solver.threshold), norm_error=int(solver._norm_error),
       inner_iterations=int(solver.inner_iterations), min_iterations=int(solver.min_iterations),
       max_iterations=int(solver.max_iterations), momentum=float(solver.momentum.value),
-      tile_config=-1, use_graph=1 if solver.use_graph else 0, init_mode=_lib.INIT_ZEROS)
+      tile_config=int(tile_config), use_graph=1 if solver.use_graph else 0, init_mode=_lib.INIT_ZEROS)
   lib = _lib.load_library()
   nbytes = lib.ottx_sinkhorn_batched_workspace_bytes(ctypes.byref(bt), ctypes.byref(opts))
   if nbytes == 0:

@@ -9,6 +9,13 @@ cache keyed by device, precision, feature dimension and the power-of-two size
 class of n and m.  ``lookup`` returns the cached choice, falling back to the
 defaults shipped in ``tuned_configs.json``, then to -1 (the kernel default).
 
+A tile_config is a position in the library's variant table, so every entry
+also records the variant it names ({tile_config, nb, mt, w, target}) and the
+ABI version; ``lookup`` accepts an entry only if the loaded library still lists
+exactly that variant at that position.  A cache written by another build
+(whose table differs) therefore falls back to the default instead of sending
+solves to a variant the entry did not measure.
+
 The cache path is ``$OTTX_TUNE_CACHE`` or ``~/.cache/ott_amd/tune.json``.
 """
 
@@ -28,7 +35,7 @@ from ott_amd import _lib
 _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULTS_PATH = os.path.join(_HERE, "tuned_configs.json")
 _lock = threading.Lock()
-_cache: Optional[Dict[str, int]] = None
+_cache: Optional[Dict[str, dict]] = None
 
 
 def cache_path() -> str:
@@ -61,24 +68,48 @@ def tile_configs(precision: str, d: int) -> List[dict]:
                target=buf[5 * i + 4]) for i in range(count)]
 
 
-def _load() -> Dict[str, int]:
+_FIELDS = ("tile_config", "nb", "mt", "w", "target")
+
+
+def entry(cfg: dict) -> dict:
+  """The cache record of a variant (as ``tile_configs`` lists it)."""
+  return dict({f: int(cfg[f]) for f in _FIELDS}, abi=_lib.ABI_VERSION)
+
+
+def _load() -> Dict[str, dict]:
   global _cache
   if _cache is None:
-    merged: Dict[str, int] = {}
+    merged: Dict[str, dict] = {}
     for path in (DEFAULTS_PATH, cache_path()):
       if os.path.exists(path):
         try:
           with open(path) as fh:
-            merged.update({k: int(v) for k, v in json.load(fh).items()})
+            merged.update({k: v for k, v in json.load(fh).items() if isinstance(v, dict)})
         except (OSError, ValueError):
           pass
     _cache = merged
   return _cache
 
 
+def _valid(rec: dict, precision: str, d: int) -> bool:
+  """The record names a variant this library lists at the same position."""
+  try:
+    if int(rec.get("abi", -1)) != _lib.ABI_VERSION:
+      return False
+    want = {f: int(rec[f]) for f in _FIELDS}
+  except (KeyError, TypeError, ValueError):
+    return False
+  return any(all(c[f] == want[f] for f in _FIELDS) for c in tile_configs(precision, d))
+
+
 def lookup(precision: str, d: int, n: int, m: int) -> int:
+  """The tuned tile_config for the shape, or -1 when none is recorded or the
+  record does not name a variant of the loaded library."""
   with _lock:
-    return _load().get(key(precision, d, n, m), -1)
+    rec = _load().get(key(precision, d, n, m))
+  if rec is None or not _valid(rec, precision, d):
+    return -1
+  return int(rec["tile_config"])
 
 
 def benchmark(geom, tile_config: int, axis: int, reps: int = 10) -> float:
@@ -114,7 +145,7 @@ def autotune(geom, reps: int = 10, save: bool = True, verbose: bool = False) -> 
   best = min(results, key=lambda r: r["ms_per_iteration"])
   k = key(geom.precision, d, n, m)
   with _lock:
-    _load()[k] = int(best["tile_config"])
+    _load()[k] = entry(best)
     if save:
       path = cache_path()
       os.makedirs(os.path.dirname(path), exist_ok=True)
@@ -125,7 +156,7 @@ def autotune(geom, reps: int = 10, save: bool = True, verbose: bool = False) -> 
             existing = json.load(fh)
         except (OSError, ValueError):
           existing = {}
-      existing[k] = int(best["tile_config"])
+      existing[k] = entry(best)
       with open(path, "w") as fh:
         json.dump(existing, fh, indent=1, sort_keys=True)
   return dict(best=best, key=k, results=results)

@@ -36,14 +36,14 @@ def test_library_exports_every_declared_symbol():
   missing = [n for n in sorted(_declared()) if not hasattr(lib, n)]
   assert not missing, f"missing exports: {missing}"
   assert set(_lib.EXPORTED_SYMBOLS) == _declared()
-  assert lib.ottx_abi_version() == 2
+  assert lib.ottx_abi_version() == 3
   assert lib.ottx_strerror(-1) == b"invalid argument"
 
 
 def test_struct_layouts_match_header():
   from ott_amd import _lib
-  assert ctypes.sizeof(_lib.Problem) == 8 + 8 + 4 + 4 + 4 * 8 + 5 * 8
-  assert ctypes.sizeof(_lib.SinkhornOpts) == 88
+  assert ctypes.sizeof(_lib.Problem) == 8 + 8 + 4 + 4 + 4 * 8 + 5 * 8 + 2 * 4
+  assert ctypes.sizeof(_lib.SinkhornOpts) == 88 + 2 * 8
   assert ctypes.sizeof(_lib.SinkhornOut) == 3 * 8 + 3 * 8 + 4 * 4
   assert ctypes.sizeof(_lib.Batch) == 4 * 4 + 4 * 8 + 3 * 8 + 6 * 8
   assert ctypes.sizeof(_lib.BatchOut) == 3 * 8 + 4 * 8 + 2 * 4
@@ -85,6 +85,13 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
   assert lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), None) == 0
   assert lib.ottx_op_workspace_bytes(ctypes.byref(pb)) == 0
   assert lib.ottx_mean_cost_workspace_bytes(0, 3, 2) == 0
+  assert lib.ottx_cost_moments_workspace_bytes(0, 3, 2) == 0
+  assert lib.ottx_cost_moments_workspace_bytes(3, 3, 257) == 0
+  assert lib.ottx_cost_moments_workspace_bytes(3, 3, 2) > 0
+  # an unknown epsilon statistic is rejected at planning
+  bad = _lib.Problem(n=8, m=8, d=2, precision=0, x=1, y=1, epsilon=0.1, inv_scale_cost=1.0, tau_a=1.0,
+                     tau_b=1.0, relative_epsilon=0.05, epsilon_statistic=2)
+  assert lib.ottx_op_workspace_bytes(ctypes.byref(bad)) == 0
   rc = lib.ottx_sinkhorn_solve(ctypes.byref(pb), None, None, None, 0, None)
   assert rc == _lib.OTTX_EINVAL
 
@@ -104,3 +111,9 @@ def test_workspace_planning_all_shapes(n, m, d, precision):
   assert lib.ottx_op_workspace_bytes(ctypes.byref(pb)) > 0
   assert lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts)) > 0
   assert lib.ottx_tile_configs(precision, d, None, 0) > 0
+  # a relative epsilon on std(C) adds the moment partials to the workspace
+  std = _lib.Problem(n=n, m=m, d=d, precision=precision, x=1, y=1, a=1, b=1, epsilon=0.0,
+                     inv_scale_cost=1.0, tau_a=1.0, tau_b=1.0, relative_epsilon=0.05,
+                     epsilon_statistic=_lib.EPS_STAT_STD)
+  assert (lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(std), ctypes.byref(opts)) >
+          lib.ottx_sinkhorn_workspace_bytes(ctypes.byref(pb), ctypes.byref(opts)))
