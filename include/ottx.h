@@ -42,6 +42,7 @@ enum ottx_status {
   OTTX_EUNSUPPORTED = -2,  /* valid OTT option this library does not run */
   OTTX_EHIP = -3,          /* a HIP runtime call failed */
   OTTX_EWORKSPACE = -4,    /* workspace too small or misaligned */
+  OTTX_ESCRATCH = -5,      /* problem larger than the no-allocation scratch */
 };
 
 enum ottx_precision {
@@ -262,6 +263,25 @@ int ottx_pointcloud_op(const ottx_problem* problem, int32_t op, int32_t axis,
                        const float* f, const float* g, const float* log_marginal,
                        double eps, float* out, void* workspace,
                        size_t workspace_bytes, void* stream);
+
+/* Size of the library's static scratch for ottx_pointcloud_op_noalloc. */
+#define OTTX_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* ottx_pointcloud_op without a workspace argument and without any allocation.
+ * No reference counterpart: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The scratch is a file-scope __device__ array of
+ * OTTX_NOALLOC_SCRATCH_BYTES per device, held until the process ends; a
+ * problem whose scratch need (ottx_pointcloud_op_noalloc_bytes) exceeds it
+ * returns OTTX_ESCRATCH.  Every call on a device shares that device's array
+ * (each call re-initialises what must start at zero): calls that overlap on
+ * one device, from several streams or threads, race.  Results are those of
+ * ottx_pointcloud_op on the same inputs. */
+int ottx_pointcloud_op_noalloc(const ottx_problem* problem, int32_t op, int32_t axis,
+                               const float* f, const float* g, const float* log_marginal,
+                               double eps, float* out, void* stream);
+/* Scratch bytes ottx_pointcloud_op_noalloc needs for the problem (0: invalid). */
+size_t ottx_pointcloud_op_noalloc_bytes(const ottx_problem* problem);
 
 /* Cost maxima behind PointCloud's scale_cost modes (pointcloud.py:342-383):
  * out_host[0] = max_i |x_i|^2, out_host[1] = max_j |y_j|^2 (cost_fn.norm of the

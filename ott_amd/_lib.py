@@ -23,6 +23,8 @@ OTTX_EINVAL = -1
 OTTX_EUNSUPPORTED = -2
 OTTX_EHIP = -3
 OTTX_EWORKSPACE = -4
+OTTX_ESCRATCH = -5
+NOALLOC_SCRATCH_BYTES = 32 << 20  # OTTX_NOALLOC_SCRATCH_BYTES
 
 PREC_FP32 = 0
 PREC_BF16_SPLIT = 1
@@ -189,6 +191,10 @@ _SIGNATURES = [
      [ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    ("ottx_pointcloud_op_noalloc", ctypes.c_int,
+     [ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    ("ottx_pointcloud_op_noalloc_bytes", ctypes.c_size_t, [ctypes.POINTER(Problem)]),
     ("ottx_cost_maxima", ctypes.c_int,
      [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
       ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),

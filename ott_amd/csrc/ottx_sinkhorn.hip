@@ -1968,6 +1968,7 @@ const char* ottx_strerror(int status) {
     case OTTX_EUNSUPPORTED: return "unsupported option";
     case OTTX_EHIP: return "HIP runtime error";
     case OTTX_EWORKSPACE: return "workspace too small or misaligned";
+    case OTTX_ESCRATCH: return "problem larger than the no-allocation scratch (OTTX_NOALLOC_SCRATCH_BYTES)";
     default: return "unknown status";
   }
 }
@@ -2180,23 +2181,34 @@ int ottx_cost_gradient(const ottx_problem* pb, const float* f, const float* g, d
   return OTTX_OK;
 }
 
-int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const float* f, const float* g,
-                       const float* log_marginal, double eps, float* out, void* workspace, size_t workspace_bytes,
-                       void* stream) {
+}  // extern "C"
+
+namespace ottx {
+
+// Arguments of ottx_pointcloud_op checked, the default plan made: OTTX_OK
+// and *PL, or the status to return.
+int pointcloud_op_plan(const ottx_problem* pb, int32_t op, int32_t axis, const float* f, const float* g,
+                       const float* log_marginal, double eps, float* out, Plan* PL) {
   int rc = check_problem(pb);
   if (rc != OTTX_OK) return rc;
-  if (!f || !g || !out || !workspace || !(eps > 0.0)) return OTTX_EINVAL;
+  if (!f || !g || !out || !(eps > 0.0)) return OTTX_EINVAL;
   if (axis != 0 && axis != 1) return OTTX_EINVAL;
   if (op != OTTX_OP_APPLY_LSE && op != OTTX_OP_UPDATE && op != OTTX_OP_MARGINAL) return OTTX_EINVAL;
-  Plan PL;
-  if (!make_plan(pb, -1, &PL)) return OTTX_EUNSUPPORTED;
+  if (op == OTTX_OP_UPDATE && !log_marginal) return OTTX_EINVAL;
+  if (!make_plan(pb, -1, PL)) return OTTX_EUNSUPPORTED;
+  return OTTX_OK;
+}
+
+// The operator call on a checked plan with its workspace (caller memory or
+// the no-allocation scratch).
+int pointcloud_op_run(const ottx_problem* pb, const Plan& PL, int32_t op, int32_t axis, const float* f,
+                      const float* g, const float* log_marginal, double eps, float* out, void* workspace,
+                      hipStream_t st) {
+  int rc;
   const KernelCfg& k = PL.k;
   const PassPlan& gp = PL.gp;
   const PassPlan& fp = PL.fp;
   const Layout& L = PL.L;
-  if (workspace_bytes < L.total) return OTTX_EWORKSPACE;
-  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
-  hipStream_t st = static_cast<hipStream_t>(stream);
   Prepared P;
   rc = prepare(pb, k, L, workspace, st, &P);
   if (rc != OTTX_OK) return rc;
@@ -2219,7 +2231,6 @@ int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const f
     hipLaunchKernelGGL(copy_f32_kernel, dim3(static_cast<int>((n_out + 255) / 256)), dim3(256), 0, st, h, out,
                        n_out);
     OTTX_LAUNCHED();
-    if (!log_marginal) return OTTX_EINVAL;
     a.out_pot = out;
     a.out_logw = log_marginal;
     a.tau = 1.f;
@@ -2233,6 +2244,68 @@ int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const f
     a.mode = kModeMarginal;
   }
   return launch_halfstep(k, a, st);
+}
+
+// The no-allocation scratch of ottx_pointcloud_op_noalloc: one array per
+// device, held until the process ends.  In host code ``scratch`` names a
+// host-side stand-in, never device memory: only the address that
+// hipGetSymbolAddress returns for the current device (kept per device in
+// scratch_ptr) may reach a kernel or a copy.
+static __device__ __align__(256) unsigned char scratch[OTTX_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch (resolved once per device,
+// with that device current).
+int scratch_ptr(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return OTTX_EHIP;
+  if (dev < 0 || dev >= kMaxDevices) return OTTX_EUNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return OTTX_EHIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return OTTX_OK;
+}
+
+}  // namespace ottx
+
+extern "C" {
+
+int ottx_pointcloud_op(const ottx_problem* pb, int32_t op, int32_t axis, const float* f, const float* g,
+                       const float* log_marginal, double eps, float* out, void* workspace, size_t workspace_bytes,
+                       void* stream) {
+  Plan PL;
+  const int rc = pointcloud_op_plan(pb, op, axis, f, g, log_marginal, eps, out, &PL);
+  if (rc != OTTX_OK) return rc;
+  if (!workspace) return OTTX_EINVAL;
+  if (workspace_bytes < PL.L.total) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  return pointcloud_op_run(pb, PL, op, axis, f, g, log_marginal, eps, out, workspace,
+                           static_cast<hipStream_t>(stream));
+}
+
+int ottx_pointcloud_op_noalloc(const ottx_problem* pb, int32_t op, int32_t axis, const float* f, const float* g,
+                               const float* log_marginal, double eps, float* out, void* stream) {
+  Plan PL;
+  int rc = pointcloud_op_plan(pb, op, axis, f, g, log_marginal, eps, out, &PL);
+  if (rc != OTTX_OK) return rc;
+  if (PL.L.total > static_cast<size_t>(OTTX_NOALLOC_SCRATCH_BYTES)) return OTTX_ESCRATCH;
+  void* ws = nullptr;
+  rc = scratch_ptr(&ws);
+  if (rc != OTTX_OK) return rc;
+  return pointcloud_op_run(pb, PL, op, axis, f, g, log_marginal, eps, out, ws, static_cast<hipStream_t>(stream));
+}
+
+size_t ottx_pointcloud_op_noalloc_bytes(const ottx_problem* pb) {
+  Plan PL;
+  if (check_problem(pb) != OTTX_OK || !make_plan(pb, -1, &PL)) return 0;
+  return PL.L.total;
 }
 
 size_t ottx_sinkhorn_workspace_bytes(const ottx_problem* pb, const ottx_sinkhorn_opts* o) {
