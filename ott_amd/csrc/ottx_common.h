@@ -86,8 +86,8 @@ struct HalfStepArgs {
   uint32_t mode;
   int32_t norm_p;
   float inv2s;          // 1 / (2 s)
-  float two_s;          // 2 s
-  float c;              // 2 s log2(e) / eps
+  double two_s;         // 2 s
+  float c;              // 2 s log2(e) / eps (the launch's; the y side of the fragments carries the packing c)
   float eps;            // epsilon of this launch's exponent
   float eps_marg;       // epsilon of the marginal (target epsilon)
   float tau;            // tau of the out side
@@ -104,7 +104,8 @@ struct HalfStepArgs {
   float mom_threshold;  // ... if the error of block mom_start / inner - 1 is below this
   int32_t side;         // 1: out = y (tau_b), 0: out = x (tau_a)
   const SolveScalars* scal;  // when set, overrides c, eps, eps_marg and inv_rho
-  const float* lse_shift;    // per-column log-sum-exp (acc units) of the previous same-side pass
+  const SolveScalars* pack_scal;  // c the out fragments were packed with (null: c above)
+  const float* lse_shift;    // per-column log-sum-exp (exponent units at the packing c) of the previous same-side pass
   float* lse_store;          // where this pass stores its per-column log-sum-exp
   float2* partial;      // [splits][n_out_pad] running (max, sum) per column
   uint32_t* tile_counter;   // [tiles]

@@ -7,10 +7,15 @@
 //
 //   LSE_i((f_i - C_ij) / eps),   C_ij = s (|x_i|^2 + |y_j|^2 - 2 <x_i, y_j>)
 //
-// without materialising C: the cross term runs on MFMA with the row bias
-// beta_i = (f_i - s|x_i|^2) / (2 s) loaded as the accumulator input, so the
-// MFMA result is acc_ij = <x_i, y_j> + beta_i and the exponent in base 2 is
-// c * acc_ij with c = 2 s log2(e) / eps.  A streaming max / sum-of-exp2 per
+// without materialising C: the cross term runs on MFMA.  The y fragments are
+// packed already multiplied by c = 2 s log2(e) / eps (the target epsilon),
+// and the row bias c beta_i, beta_i = (f_i - s|x_i|^2) / (2 s), is loaded as
+// the accumulator input, so the MFMA result acc_ij = c (<x_i, y_j> + beta_i)
+// is the base-2 exponent itself ("exponent units"; a launch at another
+// epsilon, an epsilon schedule, multiplies it by k = eps / eps_t).  With a
+// fixed per-column shift m_j one VOP2 subtraction per pair remains, and none
+// when a workgroup's shifts are close enough to share one, folded into the
+// bias.  A streaming max / sum-of-exp2 per
 // column reduces the tile, wave shuffles combine lane groups, and a split-K
 // combine (last-arriving workgroup per column tile) merges row splits.  The
 // epilogue turns (max, sum) into the new potential, the marginal of the old
@@ -39,23 +44,44 @@ __device__ __forceinline__ float bf16_to_f32(uint32_t h) { return __uint_as_floa
 // there, and so must the kernels.
 __device__ __forceinline__ float nan_max(float a, float b) { return __builtin_elementwise_maximum(a, b); }
 
-// Combine two (max, sum) pairs of base-2 scaled log-sum-exp accumulators.
-__device__ __forceinline__ void lse_merge(float& mx, float& sm, float om, float os, float c) {
+// Combine two (max, sum) pairs of base-2 log-sum-exp accumulators (max in
+// exponent units).
+__device__ __forceinline__ void lse_merge(float& mx, float& sm, float om, float os) {
   const float mn = nan_max(mx, om);
   float s = 0.f;
-  if (mn != -INFINITY) s = sm * exp2_fast((mx - mn) * c) + os * exp2_fast((om - mn) * c);
+  if (mn != -INFINITY) s = sm * exp2_fast(mx - mn) + os * exp2_fast(om - mn);
   mx = mn;
   sm = s;
 }
 
-// Cross-workgroup hand-off without L2 invalidation.  On gfx950 an agent-scope
-// __threadfence() is buffer_wbl2 + buffer_inv: the invalidate drops every
-// cached line of the XCD's L2, which the other workgroups streaming through
-// that L2 then refetch.  Writers here only need a release (write back their
-// dirty lines before the arrival counter's atomic); readers take the data with
+// Cross-workgroup hand-off without touching the L2 as a whole.  On gfx950 an
+// agent-scope release fence is buffer_wbl2 (write back every dirty line of
+// the XCD's L2) and __threadfence() adds buffer_inv; issued by every
+// workgroup of a launch, the write-backs alone cost about 16 us per half-step
+// at n = m = 16384.  Instead, the data handed to another workgroup is stored
+// write-through at agent scope (global_store ... sc1), each thread waits for
+// its own stores to complete (s_waitcnt 0) before the workgroup barrier that
+// precedes the arrival counter's atomic, and readers take the data with
 // agent-coherent (sc1) loads, which bypass a stale L2, after the counter's
 // atomic returned.
-__device__ __forceinline__ void release_to_device() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void store_f2_through(float2* p, float2 v) {
+  const uint64_t u = static_cast<uint64_t>(__float_as_uint(v.x)) |
+                     (static_cast<uint64_t>(__float_as_uint(v.y)) << 32);
+  __hip_atomic_store(reinterpret_cast<uint64_t*>(p), u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_f64_through(double* p, double v) {
+  __hip_atomic_store(reinterpret_cast<uint64_t*>(p), static_cast<uint64_t>(__double_as_longlong(v)),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// This thread's stores (and loads) have completed; the compiler keeps memory
+// operations on their side of it.
+__device__ __forceinline__ void wait_own_stores() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 __device__ __forceinline__ double load_f64_coherent(const double* p) {
   const uint64_t v = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED,
@@ -193,27 +219,21 @@ __device__ __forceinline__ void tile_prefetch(const u32x4* __restrict__ src, con
 }
 
 // Bias of row rb * 16 + tid (tid < ROWS) from its prefetched potential and
-// norm; rows past n_in get -inf so they add nothing.
-__device__ __forceinline__ float row_bias(float2 r, int64_t row, int64_t n_in, float inv2s) {
-  return row < n_in ? (r.x - r.y) * inv2s : -INFINITY;
+// norm, in exponent units (kb = c / (2 s)), less a workgroup-uniform shift;
+// rows past n_in get -inf so they add nothing.
+__device__ __forceinline__ float row_bias(float2 r, int64_t row, int64_t n_in, float kb, float bshift) {
+  return row < n_in ? fmaf(r.x - r.y, kb, -bshift) : -INFINITY;
 }
 
-// NB column blocks held in B, updating each lane's running (max, sum) of
-// exp2(c * acc) per column.  in_frag / in_pot / in_norm are relative to the
-// first row of the in side; rows i >= n_in get a -inf bias.
-//
-// SHIFT: mx[] holds a fixed per-column shift (the column's log-sum-exp from
-// the previous half-step of the same side) and only the sum is accumulated;
-// otherwise mx[] is the running max, rescaled once per tile.
-//
-// The W waves of the workgroup stream the same rows: each tile of MT row
-// blocks (contiguous in the fragment layout) and its row biases are copied
-// once into LDS (double-buffered, next tile prefetched into registers while
-// the current one is on MFMA), and every wave reads its A operands from LDS.
+// How the exponent of a pair comes from its accumulator in fixed-shift mode:
+// fmaf(acc, k, -m_j) at a launch epsilon other than the packing one,
+// acc - m_j otherwise, or acc alone when the shift is workgroup-uniform (in
+// the bias).
+enum ExpMode : int { kExpFma = 0, kExpSub = 1, kExpNone = 2 };
+
 // One streamed tile (in LDS at L, row biases at LB) against the NB column
-// blocks in B: MFMA cross terms into acc, then the log-sum-exp update of
-// (mx, sm).  SHIFT: mx[] is a fixed per-column shift and only sm is summed;
-// otherwise mx[] is the running max, rescaled once per tile.
+// blocks in B: the exponents of MT x NB 16 x 16 pairs into acc, the bias
+// entering as the accumulator input.
 template <int PREC, int KK, int NB, int MT>
 __device__ __forceinline__ void tile_mfma(const OutOperand<PREC, KK, NB>& B, const u32x4* L, const float* LB,
                                           int lane, f32x4 (&acc)[MT][NB]) {
@@ -274,32 +294,38 @@ __device__ __forceinline__ void tile_mfma(const OutOperand<PREC, KK, NB>& B, con
   }
 }
 
-// exp2 of one tile's accumulators against the fixed per-column shift mx[],
-// summed into sm[] (fixed-shift mode).
-template <int NB, int MT>
-__device__ __forceinline__ void tile_exp_shift(const f32x4 (&acc)[MT][NB], float c, const float (&mx)[NB],
+// exp2 of one tile's exponents against the fixed per-column shift mx[],
+// summed into sm[] (fixed-shift mode); EM: how the exponent is formed
+// (ExpMode).  k multiplies the accumulator (kExpFma only).
+template <int NB, int MT, int EM>
+__device__ __forceinline__ void tile_exp_shift(const f32x4 (&acc)[MT][NB], float k, const float (&mx)[NB],
                                                float (&sm)[NB]) {
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
-    const float nmc = -c * mx[nb];
+    const float m = mx[nb];
+    auto ex = [&](float v) {
+      if constexpr (EM == kExpFma) return exp2_fast(fmaf(v, k, -m));
+      else if constexpr (EM == kExpSub) return exp2_fast(v - m);
+      else return exp2_fast(v);
+    };
     float part[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) part[r] = exp2_fast(fmaf(acc[0][nb][r], c, nmc));
+    for (int r = 0; r < 4; ++r) part[r] = ex(acc[0][nb][r]);
 #pragma unroll
     for (int mt = 1; mt < MT; ++mt)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) part[r] += exp2_fast(fmaf(acc[mt][nb][r], c, nmc));
+      for (int r = 0; r < 4; ++r) part[r] += ex(acc[mt][nb][r]);
     sm[nb] += (part[0] + part[1]) + (part[2] + part[3]);
   }
 }
 
 // Streaming log-sum-exp of one tile's accumulators, base 2, with a running
 // per-column max (the mode without a fixed shift): one rescale per column per
-// tile.  The tile max is a tree and the sum uses four
-// partial accumulators, so no VALU chain is longer than MT + 2 dependent
-// operations.
+// tile.  The exponent is k acc (k > 0), mx[] in exponent units.  The tile max
+// is a tree and the sum uses four partial accumulators, so no VALU chain is
+// longer than MT + 2 dependent operations.
 template <int NB, int MT>
-__device__ __forceinline__ void tile_exp_max(const f32x4 (&acc)[MT][NB], float c, float (&mx)[NB],
+__device__ __forceinline__ void tile_exp_max(const f32x4 (&acc)[MT][NB], float k, float (&mx)[NB],
                                            float (&sm)[NB]) {
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
@@ -312,35 +338,38 @@ __device__ __forceinline__ void tile_exp_max(const f32x4 (&acc)[MT][NB], float c
     for (int w = 1; w < MT; w <<= 1)
 #pragma unroll
       for (int mt = 0; mt + w < MT; mt += 2 * w) m4[mt] = nan_max(m4[mt], m4[mt + w]);
-    const float mnew = nan_max(mx[nb], m4[0]);
+    const float mnew = nan_max(mx[nb], k * m4[0]);
     if (mnew != -INFINITY) {
-      const float nmc = -c * mnew;
       float part[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) part[r] = exp2_fast(fmaf(acc[0][nb][r], c, nmc));
+      for (int r = 0; r < 4; ++r) part[r] = exp2_fast(fmaf(acc[0][nb][r], k, -mnew));
 #pragma unroll
       for (int mt = 1; mt < MT; ++mt)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) part[r] += exp2_fast(fmaf(acc[mt][nb][r], c, nmc));
-      sm[nb] = fmaf(sm[nb], exp2_fast(fmaf(mx[nb], c, nmc)), (part[0] + part[1]) + (part[2] + part[3]));
+        for (int r = 0; r < 4; ++r) part[r] += exp2_fast(fmaf(acc[mt][nb][r], k, -mnew));
+      sm[nb] = fmaf(sm[nb], exp2_fast(mx[nb] - mnew), (part[0] + part[1]) + (part[2] + part[3]));
       mx[nb] = mnew;
     }
   }
 }
 
 // exp / sum of one tile's accumulators in either mode.
-template <int NB, int MT, bool SHIFT>
-__device__ __forceinline__ void tile_exp(const f32x4 (&acc)[MT][NB], float c, float (&mx)[NB], float (&sm)[NB]) {
+template <int NB, int MT, bool SHIFT, int EM>
+__device__ __forceinline__ void tile_exp(const f32x4 (&acc)[MT][NB], float k, float (&mx)[NB], float (&sm)[NB]) {
   if constexpr (SHIFT)
-    tile_exp_shift<NB, MT>(acc, c, mx, sm);
+    tile_exp_shift<NB, MT, EM>(acc, k, mx, sm);
   else
-    tile_exp_max<NB, MT>(acc, c, mx, sm);
+    tile_exp_max<NB, MT>(acc, k, mx, sm);
 }
 
 // Streams in-side row blocks [rb_begin, rb_end) (multiples of MT) against the
 // NB column blocks held in B, updating each lane's running (max, sum) of
-// exp2(c * acc) per column.  in_frag / in_pot / in_norm are relative to the
+// exp2(k acc) per column (EM).  in_frag / in_pot / in_norm are relative to the
 // first row of the in side; rows i >= n_in get a -inf bias.
+//
+// SHIFT: mx[] holds a fixed per-column shift (the column's log-sum-exp from
+// the previous half-step of the same side) and only the sum is accumulated;
+// otherwise mx[] is the running max, rescaled once per tile.
 //
 // The W waves of the workgroup stream the same rows: each tile of MT row
 // blocks (contiguous in the fragment layout) and its row biases are copied
@@ -349,11 +378,11 @@ __device__ __forceinline__ void tile_exp(const f32x4 (&acc)[MT][NB], float c, fl
 // LDS is double-buffered, with one barrier per tile.  [rb_begin, rb_end)
 // holds an even number of tiles (the planners guarantee it), so each loop
 // trip computes two tiles without a branch in between.
-template <int PREC, int KK, int NB, int MT, int W, bool SHIFT>
+template <int PREC, int KK, int NB, int MT, int W, bool SHIFT, int EM>
 __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, const void* in_frag,
                                             const float* __restrict__ in_pot, const float* __restrict__ in_norm,
-                                            int64_t n_in, int rb_begin, int rb_end, float inv2s, float c,
-                                            int tid, float (&mx)[NB], float (&sm)[NB], u32x4* s_tile,
+                                            int64_t n_in, int rb_begin, int rb_end, float kb, float bshift,
+                                            float k, int tid, float (&mx)[NB], float (&sm)[NB], u32x4* s_tile,
                                             float* s_bias) {
   constexpr int CH = FragChunks<PREC, KK>::value;
   constexpr int NT = W * 64;
@@ -382,7 +411,7 @@ __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, c
 #pragma unroll
   for (int q = 0; q < PER_T; ++q)
     if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
-  if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb_begin) * 16 + tid, n_in, inv2s);
+  if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb_begin) * 16 + tid, n_in, kb, bshift);
   __syncthreads();
   tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb_begin + 2 * MT, rb_last), tid,
                                               pre0, pb0);
@@ -394,30 +423,30 @@ __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, c
 #pragma unroll
     for (int q = 0; q < PER_T; ++q)
       if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L1[tid + q * NT] = pre1[q];
-    if (tid < ROWS) B1[tid] = row_bias(pb1, static_cast<int64_t>(rb + MT) * 16 + tid, n_in, inv2s);
+    if (tid < ROWS) B1[tid] = row_bias(pb1, static_cast<int64_t>(rb + MT) * 16 + tid, n_in, kb, bshift);
     __syncthreads();
     tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 3 * MT, rb_last), tid,
                                                 pre1, pb1);
     __builtin_amdgcn_s_setprio(1);
     tile_mfma<PREC, KK, NB, MT>(B, L1, B1, lane, acc1);
     __builtin_amdgcn_s_setprio(0);
-    tile_exp<NB, MT, SHIFT>(acc0, c, mx, sm);
+    tile_exp<NB, MT, SHIFT, EM>(acc0, k, mx, sm);
     if (rb + 2 * MT >= rb_end) {
-      tile_exp<NB, MT, SHIFT>(acc1, c, mx, sm);
+      tile_exp<NB, MT, SHIFT, EM>(acc1, k, mx, sm);
       break;
     }
     // tile rb + 2 MT: stage, MFMA, and finish tile rb + MT
 #pragma unroll
     for (int q = 0; q < PER_T; ++q)
       if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
-    if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb + 2 * MT) * 16 + tid, n_in, inv2s);
+    if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb + 2 * MT) * 16 + tid, n_in, kb, bshift);
     __syncthreads();
     tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 4 * MT, rb_last), tid,
                                                 pre0, pb0);
     __builtin_amdgcn_s_setprio(1);
     tile_mfma<PREC, KK, NB, MT>(B, L0, B0, lane, acc0);
     __builtin_amdgcn_s_setprio(0);
-    tile_exp<NB, MT, SHIFT>(acc1, c, mx, sm);
+    tile_exp<NB, MT, SHIFT, EM>(acc1, k, mx, sm);
   }
 }
 
@@ -430,11 +459,11 @@ __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, c
 // Tile t + 1 is staged into LDS buffer (t + 1) & 1 at the start of phase
 // 2t + 1, when the buffer's previous tile (t - 1) has been read by both
 // groups; its global prefetch was issued two phases earlier.
-template <int PREC, int KK, int NB, int MT, bool SHIFT>
+template <int PREC, int KK, int NB, int MT, bool SHIFT, int EM>
 __device__ __forceinline__ void stream_rows_pp(const OutOperand<PREC, KK, NB>& B, const void* in_frag,
                                                const float* __restrict__ in_pot, const float* __restrict__ in_norm,
-                                               int64_t n_in, int rb_begin, int rb_end, float inv2s, float c,
-                                               int tid, float (&mx)[NB], float (&sm)[NB], u32x4* s_tile,
+                                               int64_t n_in, int rb_begin, int rb_end, float kb, float bshift,
+                                               float k, int tid, float (&mx)[NB], float (&sm)[NB], u32x4* s_tile,
                                                float* s_bias) {
   constexpr int W = 8;
   constexpr int CH = FragChunks<PREC, KK>::value;
@@ -455,7 +484,7 @@ __device__ __forceinline__ void stream_rows_pp(const OutOperand<PREC, KK, NB>& B
 #pragma unroll
     for (int q = 0; q < PER_T; ++q)
       if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L[tid + q * NT] = pre[q];
-    if (tid < ROWS) s_bias[(t & 1) * ROWS + tid] = row_bias(pb, static_cast<int64_t>(rb_begin + t * MT) * 16 + tid, n_in, inv2s);
+    if (tid < ROWS) s_bias[(t & 1) * ROWS + tid] = row_bias(pb, static_cast<int64_t>(rb_begin + t * MT) * 16 + tid, n_in, kb, bshift);
   };
   tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, rb_begin, tid, pre, pb);
   stage(0);
@@ -463,17 +492,17 @@ __device__ __forceinline__ void stream_rows_pp(const OutOperand<PREC, KK, NB>& B
                                               pb);
   __syncthreads();
   f32x4 acc[MT][NB];
-  for (int k = 0; k <= 2 * T; ++k) {
-    if (k & 1) {
-      // stage tile (k + 1) / 2, prefetch the one after it
-      const int tn = (k >> 1) + 1;
+  for (int ph = 0; ph <= 2 * T; ++ph) {
+    if (ph & 1) {
+      // stage tile (ph + 1) / 2, prefetch the one after it
+      const int tn = (ph >> 1) + 1;
       if (tn < T) {
         stage(tn);
         tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb_begin + (tn + 1) * MT, rb_last),
                                                     tid, pre, pb);
       }
     }
-    const int kg = k - group;
+    const int kg = ph - group;
     if (kg >= 0) {
       const int t = kg >> 1;
       if (t < T) {
@@ -482,7 +511,7 @@ __device__ __forceinline__ void stream_rows_pp(const OutOperand<PREC, KK, NB>& B
           tile_mfma<PREC, KK, NB, MT>(B, s_tile + (t & 1) * TILE_U4, s_bias + (t & 1) * ROWS, lane, acc);
           __builtin_amdgcn_s_setprio(0);
         } else {
-          tile_exp<NB, MT, SHIFT>(acc, c, mx, sm);
+          tile_exp<NB, MT, SHIFT, EM>(acc, k, mx, sm);
         }
       }
     }
@@ -490,10 +519,26 @@ __device__ __forceinline__ void stream_rows_pp(const OutOperand<PREC, KK, NB>& B
   }
 }
 
+// The streaming loop of a W-wave workgroup: the ping-pong schedule for eight
+// waves, the double-buffered one otherwise.
+template <int PREC, int KK, int NB, int MT, int W, bool SHIFT, ExpMode EM>
+__device__ __forceinline__ void stream_any(const OutOperand<PREC, KK, NB>& B, const void* in_frag,
+                                           const float* __restrict__ in_pot, const float* __restrict__ in_norm,
+                                           int64_t n_in, int rb_begin, int rb_end, float kb, float bshift, float k,
+                                           int tid, float (&mx)[NB], float (&sm)[NB], u32x4* s_tile,
+                                           float* s_bias) {
+  if constexpr (W == 8)
+    stream_rows_pp<PREC, KK, NB, MT, SHIFT, EM>(B, in_frag, in_pot, in_norm, n_in, rb_begin, rb_end, kb, bshift, k,
+                                                tid, mx, sm, s_tile, s_bias);
+  else
+    stream_rows<PREC, KK, NB, MT, W, SHIFT, EM>(B, in_frag, in_pot, in_norm, n_in, rb_begin, rb_end, kb, bshift, k,
+                                                tid, mx, sm, s_tile, s_bias);
+}
+
 // Combine the four 16-lane row groups of each column; lanes < 16 store the
 // column's (max, sum) to s_ms[(wave * NB + nb) * 16 + lane].
 template <int NB>
-__device__ __forceinline__ void combine_lanes(float (&mx)[NB], float (&sm)[NB], float c, int lane, int wave,
+__device__ __forceinline__ void combine_lanes(float (&mx)[NB], float (&sm)[NB], int lane, int wave,
                                               float2* s_ms) {
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb) {
@@ -501,7 +546,7 @@ __device__ __forceinline__ void combine_lanes(float (&mx)[NB], float (&sm)[NB], 
     for (int off = 16; off <= 32; off <<= 1) {
       const float om = __shfl_xor(mx[nb], off, 64);
       const float os = __shfl_xor(sm[nb], off, 64);
-      lse_merge(mx[nb], sm[nb], om, os, c);
+      lse_merge(mx[nb], sm[nb], om, os);
     }
     if (lane < 16) s_ms[(wave * NB + nb) * 16 + lane] = make_float2(mx[nb], sm[nb]);
   }
@@ -511,7 +556,8 @@ __device__ __forceinline__ void combine_lanes(float (&mx)[NB], float (&sm)[NB], 
 struct EpiScalars {
   uint32_t mode;
   int32_t norm_p;
-  float two_s, eps, eps_marg, tau, inv_rho, momentum;
+  float eps, eps_marg, tau, inv_rho, momentum;
+  double conv;  // exponent units of the launch to cost units: 2 s / (k c_pack)
 };
 
 // Epilogue of one out-side point j (pointers already offset to the problem):
@@ -528,7 +574,9 @@ __device__ __forceinline__ void column_epilogue(const EpiScalars& e, float2 ms, 
   if (ms.x == -INFINITY || !(ms.y > 0.f)) {
     app = (ms.y != ms.y) ? ms.y : -INFINITY;
   } else {
-    app = fmaf(e.two_s, ms.x, -out_norm[j]) + e.eps * logf(ms.y);
+    // in float64: the conversion factor is the exact inverse of the packing
+    // scale, so the large term carries no systematic relative error
+    app = static_cast<float>(e.conv * (static_cast<double>(ms.x) + static_cast<double>(log2f(ms.y)))) - out_norm[j];
   }
   const float h_old = out_pot[j];
   float h_marg = h_old;
@@ -589,6 +637,8 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
   __shared__ double s_red[2][NT];
   __shared__ int s_last;
 
+  __shared__ float s_mm[W][2];
+
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -598,6 +648,8 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
     eps = eps_marg = p.scal->eps_f;
     inv_rho = p.side ? p.scal->inv_rho_b : p.scal->inv_rho_a;
   }
+  // the exponent constant the y fragments were packed with
+  const float c_pack = p.pack_scal != nullptr ? p.pack_scal->c : p.c;
   uint32_t mode = p.mode;
   if (p.eps_sched) {
     // epsilon schedule (epsilon_scheduler.py:47-60): this launch's iteration
@@ -629,6 +681,12 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
                      : adaptive_momentum(p.errors, p.mom_start, p.inner, p.momentum, p.mom_threshold);
   }
 
+  // Exponent units: acc = c_pack (<x, y> + beta); this launch's exponent is
+  // k acc, k = 1 exactly at the packing epsilon; the row bias is
+  // c_pack beta = (f - norm) kb.
+  const float k = c / c_pack;
+  const float kb = static_cast<float>(static_cast<double>(c_pack) / p.two_s);
+
   OutOperand<PREC, KK, NB> B;
   B.load(p.out_frag, static_cast<int64_t>(tile * W + wave) * NB, lane);
   float mx[NB], sm[NB];
@@ -637,37 +695,77 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
   const int64_t cbase = static_cast<int64_t>(tile * W + wave) * NB * 16 + (lane & 15);
   // Fixed-shift streaming when the previous half-step of this side left a
   // finite log-sum-exp for every column of the tile; the workgroup falls back
-  // to the running max if the resulting sums leave [2^-60, 2^60].
+  // to the running max if the resulting sums leave [2^-60, 2^60].  When the
+  // workgroup's shifts all lie within kUniformRange of their largest S (and
+  // k = 1), every column is summed relative to S itself: S goes into the row
+  // bias and a pair costs no operation before its exp2; the sums then only
+  // need to stay above 2^-100.  The exponents are then rounded at magnitudes
+  // up to the range, and that rounding changes from pass to pass with S: it
+  // sets a floor under the marginal error of about ln 2 ulp(range) / 2, 3.3e-7
+  // for a range of 8 (at 64 the floor rises to 2.6e-6, visible against the
+  // oracle's errors).  The fp32 path keeps per-column shifts.
+  constexpr float kUniformRange = PREC == 1 ? 8.f : -1.f;
   bool shifted = p.lse_shift != nullptr && p.state != nullptr && p.state->shift_ok[p.side] != 0;
+  bool uniform = false;
+  float wg_shift = 0.f;
   if (shifted) {
     if (tid == 0) s_last = 1;
     __syncthreads();
     bool ok = true;
+    float lmax = -INFINITY, lmin = INFINITY;
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-      mx[nb] = p.lse_shift[cbase + nb * 16];
+      mx[nb] = k * p.lse_shift[cbase + nb * 16];
       sm[nb] = 0.f;
       ok = ok && isfinite(mx[nb]);
+      lmax = fmaxf(lmax, mx[nb]);
+      lmin = fminf(lmin, mx[nb]);
     }
     if (!ok) s_last = 0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      lmax = fmaxf(lmax, __shfl_xor(lmax, off, 64));
+      lmin = fminf(lmin, __shfl_xor(lmin, off, 64));
+    }
+    if (lane == 0) {
+      s_mm[wave][0] = lmax;
+      s_mm[wave][1] = lmin;
+    }
     __syncthreads();
     shifted = s_last != 0;
+    if (shifted && k == 1.f) {
+      float M = s_mm[0][0], m = s_mm[0][1];
+#pragma unroll
+      for (int w = 1; w < W; ++w) {
+        M = fmaxf(M, s_mm[w][0]);
+        m = fminf(m, s_mm[w][1]);
+      }
+      uniform = M - m <= kUniformRange;
+      wg_shift = M;
+    }
     __syncthreads();
   }
   if (shifted) {
-    if constexpr (W == 8)
-      stream_rows_pp<PREC, KK, NB, MT, true>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s,
-                                             c, tid, mx, sm, s_tile, s_bias);
-    else
-      stream_rows<PREC, KK, NB, MT, W, true>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s,
-                                             c, tid, mx, sm, s_tile, s_bias);
-    combine_lanes<NB>(mx, sm, c, lane, wave, s_ms);
+    if (uniform) {
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) mx[nb] = wg_shift;
+      stream_any<PREC, KK, NB, MT, W, true, kExpNone>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end,
+                                                      kb, wg_shift, k, tid, mx, sm, s_tile, s_bias);
+    } else if (k == 1.f) {
+      stream_any<PREC, KK, NB, MT, W, true, kExpSub>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end,
+                                                     kb, 0.f, k, tid, mx, sm, s_tile, s_bias);
+    } else {
+      stream_any<PREC, KK, NB, MT, W, true, kExpFma>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end,
+                                                     kb, 0.f, k, tid, mx, sm, s_tile, s_bias);
+    }
+    combine_lanes<NB>(mx, sm, lane, wave, s_ms);
     if (tid == 0) s_last = 1;
     __syncthreads();
     const bool has_rows = static_cast<int64_t>(rb_begin) * 16 < p.n_in;
+    const float lo = uniform ? 7.888609e-31f : 8.673617e-19f;  // 2^-100 : 2^-60
     for (int t = tid; t < COLS; t += NT) {
       const float S = s_ms[t].y;
-      if (has_rows && !(S >= 8.673617e-19f && S <= 1.1529215e18f)) s_last = 0;
+      if (has_rows && !(S >= lo && S <= 1.1529215e18f)) s_last = 0;
     }
     __syncthreads();
     shifted = s_last != 0;
@@ -679,13 +777,9 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
       mx[nb] = -INFINITY;
       sm[nb] = 0.f;
     }
-    if constexpr (W == 8)
-      stream_rows_pp<PREC, KK, NB, MT, false>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s,
-                                              c, tid, mx, sm, s_tile, s_bias);
-    else
-      stream_rows<PREC, KK, NB, MT, W, false>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end, p.inv2s,
-                                              c, tid, mx, sm, s_tile, s_bias);
-    combine_lanes<NB>(mx, sm, c, lane, wave, s_ms);
+    stream_any<PREC, KK, NB, MT, W, false, kExpFma>(B, p.in_frag, p.in_pot, p.in_norm, p.n_in, rb_begin, rb_end,
+                                                    kb, 0.f, k, tid, mx, sm, s_tile, s_bias);
+    combine_lanes<NB>(mx, sm, lane, wave, s_ms);
   }
   __syncthreads();
 
@@ -693,8 +787,8 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
   if (p.splits > 1) {
     const int64_t stride = static_cast<int64_t>(p.tiles) * COLS;
     float2* part = p.partial + static_cast<int64_t>(split) * stride;
-    for (int t = tid; t < COLS; t += NT) part[col0 + t] = s_ms[t];
-    release_to_device();
+    for (int t = tid; t < COLS; t += NT) store_f2_through(part + col0 + t, s_ms[t]);
+    wait_own_stores();
     __syncthreads();
     if (tid == 0) {
       const uint32_t prev = atomicAdd(p.tile_counter + tile, 1u);
@@ -703,10 +797,16 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
     __syncthreads();
     if (!s_last) return;
     for (int t = tid; t < COLS; t += NT) {
+      // the splits' pairs loaded eight at a time, one round trip per eight
       float M = -INFINITY, S = 0.f;
-      for (int sp = 0; sp < p.splits; ++sp) {
-        const float2 v = load_f2_coherent(p.partial + sp * stride + col0 + t);
-        lse_merge(M, S, v.x, v.y, c);
+      for (int sp0 = 0; sp0 < p.splits; sp0 += 8) {
+        float2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          v[u] = (sp0 + u < p.splits) ? load_f2_coherent(p.partial + (sp0 + u) * stride + col0 + t)
+                                      : make_float2(-INFINITY, 0.f);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) lse_merge(M, S, v[u].x, v[u].y);
       }
       s_ms[t] = make_float2(M, S);
     }
@@ -715,13 +815,16 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
   }
 
   if (p.lse_store != nullptr) {
+    // stored in packing units (independent of this launch's k)
+    const float inv_k = 1.f / k;
     for (int t = tid; t < COLS; t += NT) {
       const float2 ms = s_ms[t];
-      p.lse_store[col0 + t] = (ms.y > 0.f) ? ms.x + log2f(ms.y) / c : -INFINITY;
+      p.lse_store[col0 + t] = (ms.y > 0.f) ? (ms.x + log2f(ms.y)) * inv_k : -INFINITY;
     }
     if (tid == 0 && p.state != nullptr) p.state->shift_ok[p.side] = 1;
   }
-  const EpiScalars es{mode, p.norm_p, p.two_s, eps, eps_marg, p.tau, inv_rho, momentum};
+  const EpiScalars es{mode,     p.norm_p, eps, eps_marg, p.tau, inv_rho, momentum,
+                      p.two_s / (static_cast<double>(k) * static_cast<double>(c_pack))};
   const bool want_err = (mode & (kModeErrCol | kModeErrRow)) != 0;
   double e_acc = 0.0, m_acc = 0.0;
   for (int t = tid; t < COLS; t += NT) {
@@ -734,12 +837,12 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
   if (want_err) {
     block_sum2<NT>(s_red, tid, e_acc, m_acc);
     if (tid == 0) {
-      p.err_partial[tile] = s_red[0][0];
-      if (p.mass_partial) p.mass_partial[tile] = s_red[1][0];
+      store_f64_through(p.err_partial + tile, s_red[0][0]);
+      if (p.mass_partial) store_f64_through(p.mass_partial + tile, s_red[1][0]);
     }
     if ((mode & kModeErrCol) && p.state != nullptr) {
       if (tid == 0) {
-        release_to_device();
+        wait_own_stores();  // the partials above are stored by this thread
         const uint32_t prev = atomicAdd(&p.state->tiles_done, 1u);
         s_last = (prev == static_cast<uint32_t>(p.tiles - 1)) ? 1 : 0;
       }

@@ -403,10 +403,12 @@ __global__ __launch_bounds__(1024) void std_final_kernel(const MomentArgs a, dou
   }
 }
 
-// bf16 hi/lo fragments: [rb][kb][hi|lo][lane][8], lane -> row rb*16 + lane%16,
-// features kb*32 + 8*(lane/16) + e.
+// bf16 hi/lo fragments of scale (p - mu): [rb][kb][hi|lo][lane][8], lane ->
+// row rb*16 + lane%16, features kb*32 + 8*(lane/16) + e.  The y cloud is
+// packed with scale = c (2 s log2(e) / eps), the x cloud with 1, so the
+// half-step's MFMA products come out as base-2 exponents.
 __device__ __forceinline__ void pack_bf16_one(const float* __restrict__ pts, int64_t np, int d,
-                                              const float* __restrict__ mu, int KB, int64_t gid,
+                                              const float* __restrict__ mu, float scale, int KB, int64_t gid,
                                               uint4* __restrict__ frag) {
   const int lane = static_cast<int>(gid & 63);
   const int64_t chunk = gid >> 6;
@@ -432,7 +434,7 @@ __device__ __forceinline__ void pack_bf16_one(const float* __restrict__ pts, int
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int k = k0 + e + q;
-      const float v = (row < np && k < d) ? in[e + q] - mu[k] : 0.f;
+      const float v = (row < np && k < d) ? (in[e + q] - mu[k]) * scale : 0.f;
       const uint32_t h = f32_to_bf16_rne(v);
       const uint32_t l = f32_to_bf16_rne(v - bf16_to_f32(h));
       h2[q] = h;
@@ -445,16 +447,10 @@ __device__ __forceinline__ void pack_bf16_one(const float* __restrict__ pts, int
   frag[(chunk * 2 + 1) * 64 + lane] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
 }
 
-__global__ void pack_bf16_kernel(const float* __restrict__ pts, int64_t np, int d,
-                                 const float* __restrict__ mu, int KB, int64_t n_threads,
-                                 uint4* __restrict__ frag) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid < n_threads) pack_bf16_one(pts, np, d, mu, KB, gid, frag);
-}
-
-// fp32 fragments: [rb][kq][lane][4], element r -> feature (4 kq + r) * 4 + lane/16.
+// fp32 fragments of scale (p - mu): [rb][kq][lane][4], element r -> feature
+// (4 kq + r) * 4 + lane/16.
 __device__ __forceinline__ void pack_f32_one(const float* __restrict__ pts, int64_t np, int d,
-                                             const float* __restrict__ mu, int KQ, int64_t gid,
+                                             const float* __restrict__ mu, float scale, int KQ, int64_t gid,
                                              float4* __restrict__ frag) {
   const int lane = static_cast<int>(gid & 63);
   const int64_t chunk = gid >> 6;
@@ -465,16 +461,9 @@ __device__ __forceinline__ void pack_f32_one(const float* __restrict__ pts, int6
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int k = (4 * kq + r) * 4 + (lane >> 4);
-    v[r] = (row < np && k < d) ? pts[row * d + k] - mu[k] : 0.f;
+    v[r] = (row < np && k < d) ? (pts[row * d + k] - mu[k]) * scale : 0.f;
   }
   frag[chunk * 64 + lane] = make_float4(v[0], v[1], v[2], v[3]);
-}
-
-__global__ void pack_f32_kernel(const float* __restrict__ pts, int64_t np, int d,
-                                const float* __restrict__ mu, int KQ, int64_t n_threads,
-                                float4* __restrict__ frag) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid < n_threads) pack_f32_one(pts, np, d, mu, KQ, gid, frag);
 }
 
 // s * |p - mu|^2 per point (float64 accumulation), zero on padded rows.
@@ -710,6 +699,7 @@ struct PrepArgs {
   int seg[7];            // first block of: pack x, pack y, norm x, norm y, w x, w y, init; seg[6] + blocks_init = grid
   int blocks_init;
   const float* mu;
+  const SolveScalars* scal;  // c of the y fragments
   int d, chunks, prec;
   double s;
   PrepInit init;
@@ -724,10 +714,11 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
   if (segi < 2) {  // fragments
     const int side = segi;
     if (local >= p.pack_threads[side]) return;
+    const float scale = side == 1 ? p.scal->c : 1.f;
     if (p.prec == 1)
-      pack_bf16_one(p.pts[side], p.np[side], p.d, p.mu, p.chunks, local, static_cast<uint4*>(p.frag[side]));
+      pack_bf16_one(p.pts[side], p.np[side], p.d, p.mu, scale, p.chunks, local, static_cast<uint4*>(p.frag[side]));
     else
-      pack_f32_one(p.pts[side], p.np[side], p.d, p.mu, p.chunks, local, static_cast<float4*>(p.frag[side]));
+      pack_f32_one(p.pts[side], p.np[side], p.d, p.mu, scale, p.chunks, local, static_cast<float4*>(p.frag[side]));
   } else if (segi < 4) {  // norms s |p - mu|^2, kNormLanes threads a row
     const int side = segi - 2;
     const int64_t row = local / kNormLanes;
@@ -793,28 +784,30 @@ __global__ void prep_pack_kernel(const PrepArgs p) {
     norm[row] = nrm;
     if (est_f) {
       // The first f-update's fixed shift from the given f: the column
-      // log-sum-exp (in the kernel's units) that the f-update's epilogue
-      // would map back to f itself, L = (eps log a - f / tau_a + s|x - mu|^2) / (2 s).
+      // log-sum-exp (base 2, in exponent units) that the f-update's epilogue
+      // would map back to f itself, L = c (eps log a - f / tau_a + s|x - mu|^2) / (2 s).
       // A warm start puts the first update's result close to it; non-finite
       // values get -inf, so their tiles use the running max, and a tile whose
       // sums leave range falls back to it as well.
       const double w = p.wuser[0] ? static_cast<double>(p.wuser[0][row]) : 1.0 / static_cast<double>(p.np[0]);
-      const double v = (in.scal->eps * log(w) - static_cast<double>(in.f[row]) / in.tau_a + nrm) / (2.0 * p.s);
+      const double v = static_cast<double>(in.scal->c) *
+                       (in.scal->eps * log(w) - static_cast<double>(in.f[row]) / in.tau_a + nrm) / (2.0 * p.s);
       in.lse_f[row] = isfinite(v) ? static_cast<float>(v) : -INFINITY;
     }
     if (est_g) {
       // the same estimate for the first g-update from the given g
       const double w = p.wuser[1] ? static_cast<double>(p.wuser[1][row]) : 1.0 / static_cast<double>(p.np[1]);
-      const double v = (in.scal->eps * log(w) - static_cast<double>(in.g[row]) / in.tau_b + nrm) / (2.0 * p.s);
+      const double v = static_cast<double>(in.scal->c) *
+                       (in.scal->eps * log(w) - static_cast<double>(in.g[row]) / in.tau_b + nrm) / (2.0 * p.s);
       in.lse_g[row] = isfinite(v) ? static_cast<float>(v) : -INFINITY;
     }
     if (probe) {
-      // the probe's max, centred on x_0, moved to the centring on mu, plus
-      // half of log2(n) / c
+      // the probe's max, centred on x_0, moved to the centring on mu, in
+      // exponent units, plus half of log2(n)
       const float M = in.lse_g[row];
       if (isfinite(M)) {
-        const double L = static_cast<double>(M) + cr +
-                         0.5 * log2(static_cast<double>(p.np[0])) / static_cast<double>(in.scal->c);
+        const double L = (static_cast<double>(M) + cr) * static_cast<double>(in.scal->c) +
+                         0.5 * log2(static_cast<double>(p.np[0]));
         in.lse_g[row] = isfinite(L) ? static_cast<float>(L) : -INFINITY;
       }
     }
@@ -1364,8 +1357,8 @@ __global__ void solve_finalize_kernel(const float* f, const float* __restrict__ 
     __syncthreads();
   }
   if (tid == 0) {
-    part[(side * kDualBlocks + blockIdx.x) * 2 + 0] = red[0][0];
-    part[(side * kDualBlocks + blockIdx.x) * 2 + 1] = red[1][0];
+    store_f64_through(part + (side * kDualBlocks + blockIdx.x) * 2 + 0, red[0][0]);
+    store_f64_through(part + (side * kDualBlocks + blockIdx.x) * 2 + 1, red[1][0]);
   }
   if (recenter && side == 0) {
     __syncthreads();
@@ -1380,12 +1373,12 @@ __global__ void solve_finalize_kernel(const float* f, const float* __restrict__ 
       __syncthreads();
     }
     if (tid == 0) {
-      part[(3 * kDualBlocks + blockIdx.x) * 2 + 0] = red[0][0];
-      part[(3 * kDualBlocks + blockIdx.x) * 2 + 1] = red[1][0];
+      store_f64_through(part + (3 * kDualBlocks + blockIdx.x) * 2 + 0, red[0][0]);
+      store_f64_through(part + (3 * kDualBlocks + blockIdx.x) * 2 + 1, red[1][0]);
     }
   }
   if (tid == 0) {
-    release_to_device();
+    wait_own_stores();  // this thread stored every partial of the workgroup
     const uint32_t prev = atomicAdd(&st->fin_count, 1u);
     s_last = (prev == 3u * kDualBlocks - 1u) ? 1 : 0;
   }
@@ -1845,6 +1838,7 @@ int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* w
   pa.wfill[0] = pb->a ? nullptr : reinterpret_cast<float*>(base + L.w_x);
   pa.wfill[1] = pb->b ? nullptr : reinterpret_cast<float*>(base + L.w_y);
   pa.mu = P->mu;
+  pa.scal = P->scal;
   pa.d = pb->d;
   pa.chunks = chunks;
   pa.prec = k.prec;
@@ -1900,7 +1894,7 @@ HalfStepArgs base_args(const ottx_problem* pb, const KernelCfg& k, const PassPla
   a.norm_p = 1;
   const double s = pb->inv_scale_cost;
   a.inv2s = static_cast<float>(1.0 / (2.0 * s));
-  a.two_s = static_cast<float>(2.0 * s);
+  a.two_s = 2.0 * s;
   a.c = static_cast<float>(2.0 * s * 1.4426950408889634 / (eps > 0.0 ? eps : 1.0));
   a.eps = static_cast<float>(eps);
   a.eps_marg = static_cast<float>(pb->epsilon);
@@ -1916,6 +1910,7 @@ HalfStepArgs base_args(const ottx_problem* pb, const KernelCfg& k, const PassPla
   a.mass_partial = g_upd ? reinterpret_cast<double*>(base + P.L.mass) : nullptr;
   a.state = reinterpret_cast<SolveState*>(base + P.L.state);
   a.scal = P.scal;
+  a.pack_scal = P.scal;
   a.side = side_out;
   (void)k;
   return a;
@@ -2852,10 +2847,11 @@ __global__ void batch_pack_kernel(const BatchItem* __restrict__ items, int d, in
   if (gid >= n_threads) return;
   const float* pts = side ? it.y : it.x;
   void* frag = side ? it.frag_y : it.frag_x;
+  const float scale = side ? it.scal->c : 1.f;
   if (prec == 1) {
-    pack_bf16_one(pts, np, d, it.mu, chunks, gid, static_cast<uint4*>(frag));
+    pack_bf16_one(pts, np, d, it.mu, scale, chunks, gid, static_cast<uint4*>(frag));
   } else {
-    pack_f32_one(pts, np, d, it.mu, chunks, gid, static_cast<float4*>(frag));
+    pack_f32_one(pts, np, d, it.mu, scale, chunks, gid, static_cast<float4*>(frag));
   }
 }
 
