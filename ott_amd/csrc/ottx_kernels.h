@@ -234,64 +234,89 @@ enum ExpMode : int { kExpFma = 0, kExpSub = 1, kExpNone = 2 };
 // One streamed tile (in LDS at L, row biases at LB) against the NB column
 // blocks in B: the exponents of MT x NB 16 x 16 pairs into acc, the bias
 // entering as the accumulator input.
+// The A operands (streamed rows) and row biases of one tile, read from LDS
+// into registers.
+template <int PREC, int KK, int MT>
+struct InTile {
+  static constexpr int KQ = (KK + 3) / 4;
+  bf16x8 ah[PREC == 1 ? MT : 1][PREC == 1 ? KK : 1];
+  bf16x8 al[PREC == 1 ? MT : 1][PREC == 1 ? KK : 1];
+  float4 aq[PREC == 0 ? MT : 1][PREC == 0 ? KQ : 1];
+  f32x4 bias[MT];
+
+  __device__ __forceinline__ void load(const u32x4* L, const float* LB, int lane) {
+    const int rsub = (lane >> 4) * 4;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const float4 bv = *reinterpret_cast<const float4*>(LB + mt * 16 + rsub);
+      bias[mt][0] = bv.x;
+      bias[mt][1] = bv.y;
+      bias[mt][2] = bv.z;
+      bias[mt][3] = bv.w;
+      if constexpr (PREC == 1) {
+#pragma unroll
+        for (int kb = 0; kb < KK; ++kb) {
+          ah[mt][kb] = __builtin_bit_cast(bf16x8, L[((mt * KK + kb) * 2) * 64 + lane]);
+          al[mt][kb] = __builtin_bit_cast(bf16x8, L[((mt * KK + kb) * 2 + 1) * 64 + lane]);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < KQ; ++q) aq[mt][q] = __builtin_bit_cast(float4, L[(mt * KQ + q) * 64 + lane]);
+      }
+    }
+  }
+};
+
 template <int PREC, int KK, int NB, int MT>
-__device__ __forceinline__ void tile_mfma(const OutOperand<PREC, KK, NB>& B, const u32x4* L, const float* LB,
-                                          int lane, f32x4 (&acc)[MT][NB]) {
-  constexpr int KQ = (KK + 3) / 4;
-  const int rsub = (lane >> 4) * 4;
+__device__ __forceinline__ void tile_mfma(const OutOperand<PREC, KK, NB>& B, const InTile<PREC, KK, MT>& A,
+                                          f32x4 (&acc)[MT][NB]) {
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
-    const float4 bv = *reinterpret_cast<const float4*>(LB + mt * 16 + rsub);
-    f32x4 bias;
-    bias[0] = bv.x;
-    bias[1] = bv.y;
-    bias[2] = bv.z;
-    bias[3] = bv.w;
     if constexpr (PREC == 1) {
-      bf16x8 ah[KK], al[KK];
-#pragma unroll
-      for (int kb = 0; kb < KK; ++kb) {
-        ah[kb] = __builtin_bit_cast(bf16x8, L[((mt * KK + kb) * 2) * 64 + lane]);
-        al[kb] = __builtin_bit_cast(bf16x8, L[((mt * KK + kb) * 2 + 1) * 64 + lane]);
-      }
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb)
-        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], B.bh[nb][0], bias, 0, 0, 0);
+        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.ah[mt][0], B.bh[nb][0], A.bias[mt], 0, 0, 0);
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb)
-        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[0], B.bl[nb][0], acc[mt][nb], 0, 0, 0);
+        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.ah[mt][0], B.bl[nb][0], acc[mt][nb], 0, 0, 0);
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb)
-        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[0], B.bh[nb][0], acc[mt][nb], 0, 0, 0);
+        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.al[mt][0], B.bh[nb][0], acc[mt][nb], 0, 0, 0);
 #pragma unroll
       for (int kb = 1; kb < KK; ++kb) {
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], B.bh[nb][kb], acc[mt][nb], 0, 0, 0);
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.ah[mt][kb], B.bh[nb][kb], acc[mt][nb], 0, 0, 0);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[kb], B.bl[nb][kb], acc[mt][nb], 0, 0, 0);
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.ah[mt][kb], B.bl[nb][kb], acc[mt][nb], 0, 0, 0);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[kb], B.bh[nb][kb], acc[mt][nb], 0, 0, 0);
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.al[mt][kb], B.bh[nb][kb], acc[mt][nb], 0, 0, 0);
       }
     } else {
-      float4 aq[KQ];
-#pragma unroll
-      for (int q = 0; q < KQ; ++q) aq[q] = __builtin_bit_cast(float4, L[(mt * KQ + q) * 64 + lane]);
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb)
-        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aq[0].x, B.bq[nb][0].x, bias, 0, 0, 0);
+        acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A.aq[mt][0].x, B.bq[nb][0].x, A.bias[mt], 0, 0, 0);
 #pragma unroll
       for (int ks = 1; ks < KK; ++ks) {
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(f4_get(aq[ks >> 2], ks & 3),
+          acc[mt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(f4_get(A.aq[mt][ks >> 2], ks & 3),
                                                              f4_get(B.bq[nb][ks >> 2], ks & 3), acc[mt][nb], 0,
                                                              0, 0);
       }
     }
   }
+}
+
+// Reads a tile from LDS and runs its MFMAs.
+template <int PREC, int KK, int NB, int MT>
+__device__ __forceinline__ void tile_mfma(const OutOperand<PREC, KK, NB>& B, const u32x4* L, const float* LB,
+                                          int lane, f32x4 (&acc)[MT][NB]) {
+  InTile<PREC, KK, MT> A;
+  A.load(L, LB, lane);
+  tile_mfma<PREC, KK, NB, MT>(B, A, acc);
 }
 
 // exp2 of one tile's exponents against the fixed per-column shift mx[],
@@ -418,6 +443,18 @@ __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, c
   __builtin_amdgcn_s_setprio(1);
   tile_mfma<PREC, KK, NB, MT>(B, L0, B0, lane, acc0);
   __builtin_amdgcn_s_setprio(0);
+  // One half trip: stage tile rb_stage (prefetched into pre) into buffer L,
+  // prefetch tile rb_pref into pre, MFMA the staged tile into acc_new and
+  // finish the previous tile in acc_old.  When the tile's A operands fit in
+  // about 40 registers (PRE_A), they are read from LDS right after the
+  // barrier and the previous tile's exp2 / sum runs while the reads are in
+  // flight, so the MFMAs issue with every operand in registers; the
+  // scheduling barriers keep the compiler from sinking the reads to their
+  // uses, which would expose the LDS latency inside the MFMA phase.  Larger
+  // tiles read their operands per row block (holding all of them would cost
+  // occupancy) and finish the previous tile after the MFMAs.
+  constexpr int A_REGS = MT * 4 + (PREC == 1 ? MT * KK * 8 : MT * ((KK + 3) / 4) * 4);
+  constexpr bool PRE_A = A_REGS <= 40;
   for (int rb = rb_begin; rb < rb_end; rb += 2 * MT) {
     // tile rb + MT: stage, MFMA, and finish tile rb
 #pragma unroll
@@ -425,12 +462,25 @@ __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, c
       if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L1[tid + q * NT] = pre1[q];
     if (tid < ROWS) B1[tid] = row_bias(pb1, static_cast<int64_t>(rb + MT) * 16 + tid, n_in, kb, bshift);
     __syncthreads();
-    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 3 * MT, rb_last), tid,
-                                                pre1, pb1);
-    __builtin_amdgcn_s_setprio(1);
-    tile_mfma<PREC, KK, NB, MT>(B, L1, B1, lane, acc1);
-    __builtin_amdgcn_s_setprio(0);
-    tile_exp<NB, MT, SHIFT, EM>(acc0, k, mx, sm);
+    if constexpr (PRE_A) {
+      InTile<PREC, KK, MT> A;
+      A.load(L1, B1, lane);
+      tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 3 * MT, rb_last), tid,
+                                                  pre1, pb1);
+      __builtin_amdgcn_sched_barrier(0);
+      tile_exp<NB, MT, SHIFT, EM>(acc0, k, mx, sm);
+      __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_s_setprio(1);
+      tile_mfma<PREC, KK, NB, MT>(B, A, acc1);
+      __builtin_amdgcn_s_setprio(0);
+    } else {
+      tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 3 * MT, rb_last), tid,
+                                                  pre1, pb1);
+      __builtin_amdgcn_s_setprio(1);
+      tile_mfma<PREC, KK, NB, MT>(B, L1, B1, lane, acc1);
+      __builtin_amdgcn_s_setprio(0);
+      tile_exp<NB, MT, SHIFT, EM>(acc0, k, mx, sm);
+    }
     if (rb + 2 * MT >= rb_end) {
       tile_exp<NB, MT, SHIFT, EM>(acc1, k, mx, sm);
       break;
@@ -441,12 +491,25 @@ __device__ __forceinline__ void stream_rows(const OutOperand<PREC, KK, NB>& B, c
       if (TILE_U4 % NT == 0 || tid + q * NT < TILE_U4) L0[tid + q * NT] = pre0[q];
     if (tid < ROWS) B0[tid] = row_bias(pb0, static_cast<int64_t>(rb + 2 * MT) * 16 + tid, n_in, kb, bshift);
     __syncthreads();
-    tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 4 * MT, rb_last), tid,
-                                                pre0, pb0);
-    __builtin_amdgcn_s_setprio(1);
-    tile_mfma<PREC, KK, NB, MT>(B, L0, B0, lane, acc0);
-    __builtin_amdgcn_s_setprio(0);
-    tile_exp<NB, MT, SHIFT, EM>(acc1, k, mx, sm);
+    if constexpr (PRE_A) {
+      InTile<PREC, KK, MT> A;
+      A.load(L0, B0, lane);
+      tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 4 * MT, rb_last), tid,
+                                                  pre0, pb0);
+      __builtin_amdgcn_sched_barrier(0);
+      tile_exp<NB, MT, SHIFT, EM>(acc1, k, mx, sm);
+      __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_s_setprio(1);
+      tile_mfma<PREC, KK, NB, MT>(B, A, acc0);
+      __builtin_amdgcn_s_setprio(0);
+    } else {
+      tile_prefetch<TILE_U4, NT, PER_T, ROWS, CH>(src, in_pot, in_norm, n_in, min(rb + 4 * MT, rb_last), tid,
+                                                  pre0, pb0);
+      __builtin_amdgcn_s_setprio(1);
+      tile_mfma<PREC, KK, NB, MT>(B, L0, B0, lane, acc0);
+      __builtin_amdgcn_s_setprio(0);
+      tile_exp<NB, MT, SHIFT, EM>(acc1, k, mx, sm);
+    }
   }
 }
 
