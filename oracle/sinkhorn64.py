@@ -21,7 +21,11 @@ epsilon -> infinity limit, scale invariance and gauge invariance; and in
 an L-BFGS minimisation of its primal, ``compute_kl_reg_cost`` against a
 numerical infimum of its Lagrangian, closed forms for one point per side,
 continuity at tau -> 1, planted slips that must fail, and ``ent_reg_cost`` /
-``dual_cost`` against the coupling's own entropy.
+``dual_cost`` against the coupling's own entropy; for recentering (both taus
+< 1), ``recenter`` mapping any translate of the optimum back to it, each
+recentred half-step returning the translation-optimal potential (an identity
+that uses neither k, xi nor smin), the recentred iteration reaching the plain
+iteration's optimum, and planted slips in k, xi and smin.
 
 One property of the reference is kept as it is: with tau < 1 its iteration
 converges to the optimum of the problem whose entropy is taken against the
@@ -57,6 +61,9 @@ __all__ = [
     "solution_error",
     "compute_kl_reg_cost",
     "recenter",
+    "recenter_k",
+    "recenter_xi",
+    "smin",
     "OracleOutput",
     "sinkhorn",
     "ent_reg_cost",
@@ -335,6 +342,26 @@ def recenter(f, g, a, b, eps, tau_a, tau_b):
   return f + shift, g - shift
 
 
+def recenter_k(tau_i, tau_j, tau_a, tau_b):
+  """``k`` of Sinkhorn.lse_step (sinkhorn.py:809-812)."""
+  num = -tau_j * (tau_a - 1.0) * (tau_b - 1.0) * (tau_i - 1.0)
+  denom = (tau_j - 1.0) * (tau_a * (tau_b - 1.0) + tau_b * (tau_a - 1.0))
+  return num / denom
+
+
+def recenter_xi(tau_i, tau_j, tau_a, tau_b):
+  """``xi`` of Sinkhorn.lse_step (sinkhorn.py:814-816)."""
+  k_ij = recenter_k(tau_i, tau_j, tau_a, tau_b)
+  return k_ij / (1.0 - k_ij)
+
+
+def smin(potential, marginal, eps, tau):
+  """``smin`` of Sinkhorn.lse_step (sinkhorn.py:818-822): the soft minimum
+  -rho log sum_i marginal_i exp(-potential_i / rho)."""
+  r = rho(eps, tau)
+  return -r * sps.logsumexp(-potential / r, b=marginal)
+
+
 # --------------------------------------------------------------------------
 # The solver loop
 # --------------------------------------------------------------------------
@@ -399,15 +426,15 @@ def sinkhorn(
   - loop: math/fixed_point_loop.py:23-93 (blocks of ``inner_iterations``,
     stop test between blocks, ``lax.scan`` when min == max);
   - one iteration: sinkhorn.py:879-938 with ``lse_step`` sinkhorn.py:803-857
-    (recentering terms omitted: only used when ``recenter_potentials`` and both
-    taus < 1, which this oracle rejects) and ``Momentum``, constant or
+    (with ``recenter_potentials`` and both taus < 1, the translation-invariant
+    terms -k smin(other) + xi smin(new) of sinkhorn.py:824-854, rho at the
+    target epsilon as ``ot_prob.epsilon`` is) and ``Momentum``, constant or
     adaptive (acceleration.py:138-172);
   - termination: sinkhorn.py:940-962; outputs: sinkhorn.py:974-1029, n_iters
     sinkhorn.py:431-434; cost: sinkhorn.py:224-277 through ``run``
     sinkhorn.py:1052-1062.
   """
-  if recenter_potentials and tau_a < 1.0 and tau_b < 1.0:
-    raise NotImplementedError("unbalanced recentering is not restated")
+  rc_terms = recenter_potentials and tau_a < 1.0 and tau_b < 1.0
   a = np.asarray(a, dtype=dtype)
   b = np.asarray(b, dtype=dtype)
   eps_t = np.asarray(epsilon, dtype=dtype)
@@ -431,10 +458,16 @@ def sinkhorn(
                               inner_iterations))
     old_f, old_g = f, g
     new_g = tau_b * update_potential(ops, old_f, old_g, log_b, eps_it, axis=0)
+    if rc_terms:
+      new_g = new_g - recenter_k(tau_b, tau_b, tau_a, tau_b) * smin(old_f, a, eps_t, tau_a)
+      new_g = new_g + recenter_xi(tau_b, tau_a, tau_a, tau_b) * smin(new_g, b, eps_t, tau_b)
     with np.errstate(invalid="ignore"):
       g = (1.0 - w) * np.where(np.isfinite(old_g), old_g, 0.0) + w * new_g
     g_for_f = old_g if parallel_dual_updates else g
     new_f = tau_a * update_potential(ops, old_f, g_for_f, log_a, eps_it, axis=1)
+    if rc_terms:
+      new_f = new_f - recenter_k(tau_a, tau_a, tau_a, tau_b) * smin(g_for_f, b, eps_t, tau_b)
+      new_f = new_f + recenter_xi(tau_a, tau_b, tau_a, tau_b) * smin(new_f, a, eps_t, tau_a)
     with np.errstate(invalid="ignore"):
       f = (1.0 - w) * np.where(np.isfinite(old_f), old_f, 0.0) + w * new_f
     if it == max_iterations - 1 or (compute_error and it >= min_iterations):

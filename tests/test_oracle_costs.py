@@ -292,3 +292,115 @@ def test_planted_slips_in_the_unbalanced_cost_fail(monkeypatch, slip):
       return kl(ops, f, g, a, b, eps, tau_a, tau_b) - eps * (np.sum(a) * np.sum(b) - total)
     monkeypatch.setattr(oracle, "compute_kl_reg_cost", dropped)
   assert _dual_gap() > 1e-4
+
+
+# --------------------------------------------------------------------------
+# Recentering (sinkhorn.py:76-114, 803-857) with both taus < 1.
+#
+# Along a translation (f + l, g - l) the coupling does not change and the
+# marginal terms of the unbalanced dual, -rho_a sum a exp(-(f + l) / rho_a) -
+# rho_b sum b exp(-(g - l) / rho_b), are maximised where
+# exp(-l / rho_a) A = exp(l / rho_b) B (A, B the sums at l = 0).  At the
+# optimum of the dual that fixes l = 0, so the shift that ``recenter`` applies
+# to any translate of the optimum must map it back exactly; and the
+# translation-invariant iteration (the k / xi / smin terms) must converge to
+# the same potentials as the plain unbalanced iteration, whose fixed point is
+# pinned above by a direct minimisation of the primal.
+# --------------------------------------------------------------------------
+RC_TAUS = [(0.8, 0.9), (0.95, 0.95), (0.6, 0.85)]
+
+
+@functools.lru_cache(maxsize=None)
+def _rc_case(ta, tb):
+  x, y, a, b = _clouds(10, 7, 3, seed=41, mass_b=1.2)
+  eps = 0.4
+  return x, y, a, b, eps, _solve(x, y, a, b, eps, ta, tb)
+
+
+@pytest.mark.parametrize("ta,tb", RC_TAUS)
+@pytest.mark.parametrize("lam", [0.0, 0.7, -1.3])
+def test_recenter_maps_translates_of_the_optimum_back(ta, tb, lam):
+  x, y, a, b, eps, ref = _rc_case(ta, tb)
+  f, g = oracle.recenter(ref.f + lam, ref.g - lam, a, b, eps, ta, tb)
+  np.testing.assert_allclose(f, ref.f, rtol=0, atol=1e-9)
+  np.testing.assert_allclose(g, ref.g, rtol=0, atol=1e-9)
+
+
+def _rc_solve(x, y, a, b, eps, ta, tb, max_iterations=20000):
+  return oracle.solve_pointcloud(x, y, a, b, epsilon=eps, tau_a=ta, tau_b=tb, threshold=1e-12,
+                                 inner_iterations=1, max_iterations=max_iterations,
+                                 recenter_potentials=True)
+
+
+@pytest.mark.parametrize("ta,tb", RC_TAUS)
+def test_translation_invariant_iteration_reaches_the_unbalanced_optimum(ta, tb):
+  x, y, a, b, eps, ref = _rc_case(ta, tb)
+  out = _rc_solve(x, y, a, b, eps, ta, tb)
+  assert out.converged
+  np.testing.assert_allclose(out.f, ref.f, rtol=0, atol=1e-8)
+  np.testing.assert_allclose(out.g, ref.g, rtol=0, atol=1e-8)
+  assert abs(out.reg_ot_cost - ref.reg_ot_cost) <= 1e-9 * abs(ref.reg_ot_cost)
+
+
+def _tiu_g(ops, a, b, eps, ta, tb, f, g):
+  new_g = tb * oracle.update_potential(ops, f, g, np.log(b), eps, axis=0)
+  new_g = new_g - oracle.recenter_k(tb, tb, ta, tb) * oracle.smin(f, a, eps, ta)
+  return new_g + oracle.recenter_xi(tb, ta, ta, tb) * oracle.smin(new_g, b, eps, tb)
+
+
+def _tiu_f(ops, a, b, eps, ta, tb, f, g):
+  new_f = ta * oracle.update_potential(ops, f, g, np.log(a), eps, axis=1)
+  new_f = new_f - oracle.recenter_k(ta, ta, ta, tb) * oracle.smin(g, b, eps, tb)
+  return new_f + oracle.recenter_xi(ta, tb, ta, tb) * oracle.smin(new_f, a, eps, ta)
+
+
+def _identity_residual(ta, tb, seed=3):
+  """max |g_new - tau_b U(f) - (1 - tau_b) s| and its f twin after one
+  recentred iteration of the oracle from random potentials (momentum 1, so
+  the iteration's potentials are the updates themselves)."""
+  x, y, a, b, eps, _ = _rc_case(ta, tb)
+  ops = oracle.DenseLSE(oracle.sqeuclidean_cost(x, y))
+  rng = np.random.default_rng(seed)
+  f0, g0 = rng.normal(size=len(a)), rng.normal(size=len(b))
+  ra, rb = _rho(eps, ta), _rho(eps, tb)
+  tau = ra * rb / (ra + rb)
+
+  def shift(f, g):
+    return tau * (special.logsumexp(-f / ra, b=a) - special.logsumexp(-g / rb, b=b))
+
+  # the oracle's iteration, read through its two halves
+  g1 = _tiu_g(ops, a, b, eps, ta, tb, f0, g0)
+  f1 = _tiu_f(ops, a, b, eps, ta, tb, f0, g1)
+  out = oracle.sinkhorn(ops, a, b, eps, tau_a=ta, tau_b=tb, init=(f0, g0), inner_iterations=1,
+                        min_iterations=1, max_iterations=1, recenter_potentials=True)
+  # the iteration's state is (f1, g1); its output is that state recentred
+  s1 = shift(f1, g1)
+  np.testing.assert_allclose(out.f, f1 + s1, rtol=0, atol=1e-12)
+  np.testing.assert_allclose(out.g, g1 - s1, rtol=0, atol=1e-12)
+  ug = tb * oracle.update_potential(ops, f0, g0, np.log(b), eps, axis=0)
+  uf = ta * oracle.update_potential(ops, f0, g1, np.log(a), eps, axis=1)
+  rg = np.max(np.abs(g1 - (ug + (1.0 - tb) * shift(f0, g1))))
+  rf = np.max(np.abs(f1 - (uf - (1.0 - ta) * shift(f1, g1))))
+  return rg, rf
+
+
+@pytest.mark.parametrize("ta,tb", RC_TAUS)
+def test_recentred_updates_are_translation_optimal(ta, tb):
+  rg, rf = _identity_residual(ta, tb)
+  assert rg < 1e-12 and rf < 1e-12
+
+
+@pytest.mark.parametrize("slip", ["k_sign", "xi_swapped", "xi_is_k", "smin_sign"])
+def test_planted_slips_in_the_recentering_terms_fail(monkeypatch, slip):
+  ta, tb = 0.8, 0.9
+  k, xi, sm = oracle.recenter_k, oracle.recenter_xi, oracle.smin
+  if slip == "k_sign":
+    monkeypatch.setattr(oracle, "recenter_k", lambda ti, tj, t1, t2: -k(ti, tj, t1, t2))
+  elif slip == "xi_swapped":
+    monkeypatch.setattr(oracle, "recenter_xi", lambda ti, tj, t1, t2: xi(tj, ti, t1, t2))
+  elif slip == "xi_is_k":
+    monkeypatch.setattr(oracle, "recenter_xi", lambda ti, tj, t1, t2: k(ti, tj, t1, t2))
+  else:
+    monkeypatch.setattr(oracle, "smin", lambda p, w, e, t: -sm(p, w, e, t))
+  rg, rf = _identity_residual(ta, tb)
+  assert max(rg, rf) > 1e-6
