@@ -29,6 +29,7 @@ enum : uint32_t {
   kModeApply = 1u << 4,      // out_vec = apply_lse_kernel
   kModeMarginal = 1u << 5,   // out_vec = marginal_from_potentials
   kModeCheckDone = 1u << 6,  // return at once when state->done is set
+  kModeWriteApp = 1u << 7,   // out_vec = eps LSE_i((h_i - C_ij) / eps) as it is (recentred updates)
 };
 
 // Loop state shared by the kernels of one solve (device memory).
@@ -62,6 +63,12 @@ struct SolveScalars {
   float c;           // 2 s log2(e) / eps
   float inv_rho_a;   // (1 - tau_a) / (eps tau_a), 0 when tau_a == 1
   float inv_rho_b;
+};
+
+// Recentred unbalanced iterations: log sum_i w_i exp(-pot_i / rho) of f (0,
+// weights a, rho_a) and of g (1, weights b, rho_b) at their latest update.
+struct RcScalars {
+  double lse[2];
 };
 
 // Arguments of one half-step launch: the out side gets the new potential, the

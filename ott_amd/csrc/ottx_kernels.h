@@ -663,6 +663,7 @@ __device__ __forceinline__ void column_epilogue(const EpiScalars& e, float2 ms, 
   }
   if (e.mode & kModeApply) out_vec[j] = (h_old == -INFINITY || h_old != h_old) ? h_old : app;
   if (e.mode & kModeMarginal) out_vec[j] = expf((h_old + app) / e.eps_marg);
+  if (e.mode & kModeWriteApp) out_vec[j] = app;
 }
 
 // Deterministic tree sum of two float64 values over the workgroup; the

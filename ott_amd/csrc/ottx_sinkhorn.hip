@@ -1432,6 +1432,195 @@ __global__ void solve_finalize_kernel(const float* f, const float* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Recentred unbalanced iterations (both taus < 1 with recenter_potentials):
+// Sinkhorn.lse_step's translation-invariant terms (sinkhorn.py:803-857) and
+// the recentring of SinkhornState (sinkhorn.py:76-114).  The half-steps write
+// their log-sum-exp terms (kModeWriteApp); one workgroup per side then forms
+//   t = tau (eps log w - app) - k smin(other),  new = t + xi smin(t),
+// blends it with momentum and records log sum w exp(-pot / rho) of the new
+// potential, from which the next update's smin and the error's shift follow.
+// smin(p, w) = -rho log sum_i w_i exp(-p_i / rho) (sinkhorn.py:818-822);
+// entries with w_i = 0 are left out, as jax.scipy.special.logsumexp with b
+// does.  These passes are O(n) against the O(n m d) half-steps.
+// ---------------------------------------------------------------------------
+constexpr int kRcThreads = 1024;
+
+// Running log-sum-exp (max, sum) in float64.
+__device__ __forceinline__ void lse_push(double& mx, double& sm, double z) {
+  if (!(z > -INFINITY)) {
+    if (z != z) mx = sm = NAN;
+    return;
+  }
+  if (z > mx) {
+    sm = sm * exp(mx - z) + 1.0;
+    mx = z;
+  } else {
+    sm += exp(z - mx);
+  }
+}
+
+// Workgroup combine of the threads' (max, sum); the result in every thread.
+__device__ double block_lse(double mx, double sm, double (*s_red)[kRcThreads]) {
+  const int tid = threadIdx.x;
+  __syncthreads();
+  s_red[0][tid] = mx;
+  s_red[1][tid] = sm;
+  __syncthreads();
+  for (int k = kRcThreads / 2; k > 0; k >>= 1) {
+    if (tid < k) {
+      const double m1 = s_red[0][tid], s1 = s_red[1][tid];
+      const double m2 = s_red[0][tid + k], s2 = s_red[1][tid + k];
+      if (m1 != m1 || m2 != m2 || s1 != s1 || s2 != s2) {
+        s_red[0][tid] = s_red[1][tid] = NAN;
+      } else {
+        const double m = fmax(m1, m2);
+        s_red[0][tid] = m;
+        s_red[1][tid] = (m == -INFINITY) ? 0.0 : s1 * exp(m1 - m) + s2 * exp(m2 - m);
+      }
+    }
+    __syncthreads();
+  }
+  const double m = s_red[0][0], s = s_red[1][0];
+  return (s > 0.0) ? m + log(s) : (s != s ? NAN : -INFINITY);
+}
+
+// log sum_i w_i exp(-pot_i / rho) of the starting f.
+__device__ __forceinline__ double rc_rho(const SolveScalars* sc, double tau) { return sc->eps * tau / (1.0 - tau); }
+
+__global__ __launch_bounds__(kRcThreads) void rc_lse_kernel(const float* __restrict__ pot,
+                                                            const float* __restrict__ w, int64_t n, double tau,
+                                                            const SolveScalars* __restrict__ sc, double* out) {
+  __shared__ double s_red[2][kRcThreads];
+  const double rho = rc_rho(sc, tau);
+  double mx = -INFINITY, sm = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kRcThreads) {
+    const double wi = w[i];
+    if (wi > 0.0) lse_push(mx, sm, log(wi) - static_cast<double>(pot[i]) / rho);
+  }
+  const double r = block_lse(mx, sm, s_red);
+  if (threadIdx.x == 0) *out = r;
+}
+
+struct RcFinishArgs {
+  const float* app;   // eps LSE terms of the side's half-step
+  float* pot;         // old potential in, new potential out
+  const float* w;     // the side's weights
+  int64_t n;
+  double tau, tau_other;
+  double kc, xc;      // k(tau_i, tau_i) and xi(tau_i, tau_j) of the side
+  const SolveScalars* sc;
+  const double* lse_other;  // log sum w exp(-pot / rho) of the other side, for its smin
+  double* lse_self;         // written: the same sum for this side's new potential
+  float momentum;
+  int32_t mom_start, inner, it_offset;
+  float mom_threshold;
+  const float* errors;
+  const SolveState* st;
+};
+
+__global__ __launch_bounds__(kRcThreads) void rc_finish_kernel(const RcFinishArgs p) {
+  __shared__ double s_red[2][kRcThreads];
+  if (p.st->done) return;
+  float wm = p.momentum;
+  if (p.mom_start > 0 && p.st->block * p.inner + p.it_offset >= p.mom_start)
+    wm = adaptive_momentum(p.errors, p.mom_start, p.inner, p.momentum, p.mom_threshold);
+  const double eps = p.sc->eps, rho_self = rc_rho(p.sc, p.tau);
+  const double smin_other = -rc_rho(p.sc, p.tau_other) * *p.lse_other;
+  // t_i as the plain update forms it (the kernel epilogue's order), minus k smin
+  auto t_of = [&](int64_t i) -> double {
+    const double wi = p.w[i];
+    if (!(wi > 0.0)) return -INFINITY;
+    const float a = p.app[i];
+    const float upd = static_cast<float>(eps) * logf(static_cast<float>(wi)) - (isfinite(a) ? a : 0.f);
+    return static_cast<double>(upd) * p.tau - p.kc * smin_other;
+  };
+  double mx = -INFINITY, sm = 0.0;
+  for (int64_t i = threadIdx.x; i < p.n; i += kRcThreads) {
+    const double wi = p.w[i];
+    if (wi > 0.0) lse_push(mx, sm, log(wi) - t_of(i) / rho_self);
+  }
+  const double smin_t = -rho_self * block_lse(mx, sm, s_red);
+  mx = -INFINITY;
+  sm = 0.0;
+  for (int64_t i = threadIdx.x; i < p.n; i += kRcThreads) {
+    const double nw = t_of(i) + p.xc * smin_t;
+    const float old = p.pot[i];
+    float v = static_cast<float>(nw);
+    if (wm != 1.f) v = (1.f - wm) * (isfinite(old) ? old : 0.f) + wm * v;
+    p.pot[i] = v;
+    const double wi = p.w[i];
+    if (wi > 0.0) lse_push(mx, sm, log(wi) - static_cast<double>(v) / rho_self);
+  }
+  const double r = block_lse(mx, sm, s_red);
+  if (threadIdx.x == 0) *p.lse_self = r;
+}
+
+// Errors of the state (f, g) recentred (sinkhorn.py:57-68, 160-221): the
+// coupling, and so both marginals, do not change under the translation; the
+// targets a exp(-(f + s) / rho_a) and b exp(-(g - s) / rho_b) do.  Column
+// marginals come from a marginal half-step, row marginals from the f-update's
+// log-sum-exp terms.  Thread 0 then runs the block finalizer.
+__global__ __launch_bounds__(kRcThreads) void rc_error_kernel(const HalfStepArgs fin, const float* __restrict__ colmarg,
+                                                              const float* __restrict__ app_f,
+                                                              const float* __restrict__ f, const float* __restrict__ g,
+                                                              const float* __restrict__ a, const float* __restrict__ b,
+                                                              int64_t n, int64_t m, double tau_a, double tau_b,
+                                                              const SolveScalars* __restrict__ sc,
+                                                              const RcScalars* __restrict__ rc) {
+  __shared__ double s_red[2][kRcThreads];
+  __shared__ double s_m[kRcThreads];
+  const SolveState* st = fin.state;
+  if (st->done) return;
+  const double eps = sc->eps, rho_a = rc_rho(sc, tau_a), rho_b = rc_rho(sc, tau_b);
+  const int tid = threadIdx.x;
+  const double shift = rho_a * rho_b / (rho_a + rho_b) * (rc->lse[0] - rc->lse[1]);
+  const double pn = static_cast<double>(fin.norm_p);
+  double col = 0.0, row = 0.0, mass = 0.0;
+  for (int64_t j = tid; j < m; j += kRcThreads) {
+    const float marg = colmarg[j];
+    const float w = b[j];
+    const double target = (w > 0.f) ? static_cast<double>(w) * exp(-(static_cast<double>(g[j]) - shift) / rho_b) : 0.0;
+    const double diff = fabs(static_cast<double>(marg) - target);
+    col += (fin.norm_p == 1) ? diff : pow(diff, pn);
+    mass += static_cast<double>(marg);
+  }
+  for (int64_t i = tid; i < n; i += kRcThreads) {
+    const float marg = expf((f[i] + app_f[i]) / static_cast<float>(eps));
+    const float w = a[i];
+    const double target = (w > 0.f) ? static_cast<double>(w) * exp(-(static_cast<double>(f[i]) + shift) / rho_a) : 0.0;
+    const double diff = fabs(static_cast<double>(marg) - target);
+    row += (fin.norm_p == 1) ? diff : pow(diff, pn);
+  }
+  s_red[0][tid] = col;
+  s_red[1][tid] = row;
+  s_m[tid] = mass;
+  __syncthreads();
+  for (int k = kRcThreads / 2; k > 0; k >>= 1) {
+    if (tid < k) {
+      s_red[0][tid] += s_red[0][tid + k];
+      s_red[1][tid] += s_red[1][tid + k];
+      s_m[tid] += s_m[tid + k];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) finalize_block(fin, s_red[0][0], s_red[1][0], s_m[0]);
+}
+
+// Outputs of the stopped loop, recentred (sinkhorn.py:1008-1009): f + s in
+// place and g - s into g_out, before solve_finalize_kernel reads them.
+__global__ __launch_bounds__(kRcThreads) void rc_output_kernel(float* f, const float* __restrict__ g,
+                                                               float* __restrict__ g_out, int64_t n, int64_t m,
+                                                               double tau_a, double tau_b,
+                                                               const SolveScalars* __restrict__ sc,
+                                                               const RcScalars* __restrict__ rc, const SolveState* st) {
+  if (!st->done || st->finalized) return;
+  const double rho_a = rc_rho(sc, tau_a), rho_b = rc_rho(sc, tau_b);
+  const float s = static_cast<float>(rho_a * rho_b / (rho_a + rho_b) * (rc->lse[0] - rc->lse[1]));
+  for (int64_t i = threadIdx.x; i < n; i += kRcThreads) f[i] += s;
+  for (int64_t j = threadIdx.x; j < m; j += kRcThreads) g_out[j] = g[j] - s;
+}
+
+// ---------------------------------------------------------------------------
 // Kernel selection
 // ---------------------------------------------------------------------------
 typedef void (*HalfStepFn)(const HalfStepArgs);
@@ -1648,7 +1837,7 @@ int launch_std(const float* x, int64_t n, const float* y, int64_t m, int d, cons
 
 struct Layout {
   size_t frag_x, frag_y, norm_x, norm_y, g_save, lse_x, lse_y, partial, counters, err_col, mass, err_row, state,
-      dual, dual_part, scal, stats_x, stats_y, stats_out, mu, w_x, w_y, moments, total;
+      dual, dual_part, scal, stats_x, stats_y, stats_out, mu, w_x, w_y, moments, rc_app, rc_col, rc, total;
   int chunks_x, chunks_y;
   int64_t rows_chunk_x, rows_chunk_y;
   MomentLayout mom;
@@ -1697,6 +1886,13 @@ Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& g
   L.dual = take(16 * sizeof(double));
   L.dual_part = take(4 * kDualBlocks * 2 * sizeof(double));  // 3 dual sides + the recentering sums
   L.scal = take(sizeof(SolveScalars));
+  if (pb->tau_a < 1.0 && pb->tau_b < 1.0) {
+    // recentred unbalanced iterations: the f-update's log-sum-exp terms, the
+    // column marginals (then the shifted output g) and the soft minima
+    L.rc_app = take(n_pad * sizeof(float));
+    L.rc_col = take(m_pad * sizeof(float));
+    L.rc = take(sizeof(RcScalars));
+  }
   stats_chunks(pb->n, &L.chunks_x, &L.rows_chunk_x);
   stats_chunks(pb->m, &L.chunks_y, &L.rows_chunk_y);
   L.stats_x = take(static_cast<size_t>(L.chunks_x) * pb->d * 2 * sizeof(double));
@@ -2339,11 +2535,12 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   if (o->init_mode < 0 || o->init_mode > 3) return OTTX_EINVAL;
   // recenter_potentials: balanced problems are recentred by the finalizer;
   // with one tau = 1 the reference leaves the potentials as they are
-  // (sinkhorn.py:104-106); both taus < 1 need the recentring terms of
-  // lse_step (sinkhorn.py:809-854), which are not implemented.
+  // (sinkhorn.py:104-106); both taus < 1 run the translation-invariant
+  // iteration of lse_step (sinkhorn.py:809-854), recentred errors and outputs.
   const bool both_unbalanced = pb->tau_a < 1.0 && pb->tau_b < 1.0;
-  if (o->recenter_potentials && (both_unbalanced || total_blocks == 0)) return OTTX_EUNSUPPORTED;
+  if (o->recenter_potentials && total_blocks == 0) return OTTX_EUNSUPPORTED;
   const int recenter = (o->recenter_potentials && pb->tau_a == 1.0 && pb->tau_b == 1.0) ? 1 : 0;
+  const bool rc_mode = o->recenter_potentials && both_unbalanced;
   // parallel dual updates: the symmetric problems of a Sinkhorn divergence,
   // where f = g throughout and one pass per iteration updates both
   const bool symmetric = o->parallel_dual_updates != 0;
@@ -2437,7 +2634,80 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     a->mom_start = mom_start;
     a->mom_threshold = static_cast<float>(o->momentum_error_threshold);
   }
+  // Recentred unbalanced iterations, in plain order: g-update terms, g
+  // finished; f-update terms, f finished; at the block's last iteration the
+  // column marginals of the new state and the recentred errors.
+  RcScalars* rcs = rc_mode ? reinterpret_cast<RcScalars*>(base + L.rc) : nullptr;
+  float* rc_app = rc_mode ? reinterpret_cast<float*>(base + L.rc_app) : nullptr;
+  float* rc_col = rc_mode ? reinterpret_cast<float*>(base + L.rc_col) : nullptr;
+  HalfStepArgs g_rc = ga, f_rc = fa, m_rc = ga;
+  g_rc.mode = kModeWriteApp | kModeCheckDone;
+  g_rc.out_vec = g_save;
+  f_rc.mode = kModeWriteApp | kModeCheckDone;
+  f_rc.out_vec = rc_app;
+  m_rc.mode = kModeMarginal | kModeCheckDone;
+  m_rc.out_vec = rc_col;
+  RcFinishArgs rg{}, rf{};
+  if (rc_mode) {
+    const double ta = pb->tau_a, tb = pb->tau_b;
+    auto kk = [&](double ti, double tj) {
+      return -tj * (ta - 1.0) * (tb - 1.0) * (ti - 1.0) / ((tj - 1.0) * (ta * (tb - 1.0) + tb * (ta - 1.0)));
+    };
+    auto xi = [&](double ti, double tj) { return kk(ti, tj) / (1.0 - kk(ti, tj)); };
+    for (RcFinishArgs* r : {&rg, &rf}) {
+      r->sc = P.scal;
+      r->momentum = static_cast<float>(o->momentum);
+      r->mom_start = mom_start;
+      r->inner = inner;
+      r->mom_threshold = static_cast<float>(o->momentum_error_threshold);
+      r->errors = out->errors;
+      r->st = state;
+    }
+    rg.app = g_save;
+    rg.pot = out->g;
+    rg.w = P.b;
+    rg.n = pb->m;
+    rg.tau = tb;
+    rg.tau_other = ta;
+    rg.kc = kk(tb, tb);
+    rg.xc = xi(tb, ta);
+    rg.lse_other = &rcs->lse[0];
+    rg.lse_self = &rcs->lse[1];
+    rf.app = rc_app;
+    rf.pot = out->f;
+    rf.w = P.a;
+    rf.n = pb->n;
+    rf.tau = ta;
+    rf.tau_other = tb;
+    rf.kc = kk(ta, ta);
+    rf.xc = xi(ta, tb);
+    rf.lse_other = &rcs->lse[1];
+    rf.lse_self = &rcs->lse[0];
+  }
+  auto launch_rc_block = [&](hipStream_t s) -> int {
+    for (int it = 0; it < inner; ++it) {
+      int r;
+      RcFinishArgs rg_it = rg, rf_it = rf;
+      rg_it.it_offset = rf_it.it_offset = it;
+      if ((r = launch_halfstep(k, g_rc, s)) != OTTX_OK) return r;
+      hipLaunchKernelGGL(rc_finish_kernel, dim3(1), dim3(kRcThreads), 0, s, rg_it);
+      OTTX_LAUNCHED();
+      if ((r = launch_halfstep(k, f_rc, s)) != OTTX_OK) return r;
+      hipLaunchKernelGGL(rc_finish_kernel, dim3(1), dim3(kRcThreads), 0, s, rf_it);
+      OTTX_LAUNCHED();
+      if (it == inner - 1) {
+        if ((r = launch_halfstep(k, m_rc, s)) != OTTX_OK) return r;
+        hipLaunchKernelGGL(rc_error_kernel, dim3(1), dim3(kRcThreads), 0, s, ga, static_cast<const float*>(rc_col),
+                           static_cast<const float*>(rc_app), static_cast<const float*>(out->f),
+                           static_cast<const float*>(out->g), P.a, P.b, pb->n, pb->m, pb->tau_a, pb->tau_b,
+                           static_cast<const SolveScalars*>(P.scal), static_cast<const RcScalars*>(rcs));
+        OTTX_LAUNCHED();
+      }
+    }
+    return OTTX_OK;
+  };
   auto launch_block = [&](hipStream_t s) -> int {
+    if (rc_mode) return launch_rc_block(s);
     for (int it = 0; it < inner; ++it) {
       int r;
       if (per_launch) {
@@ -2493,9 +2763,18 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   __atomic_store_n(&mail->progress, static_cast<uint64_t>(0), __ATOMIC_RELEASE);
   double* dual_part = reinterpret_cast<double*>(base + L.dual_part);
   auto launch_finalize = [&](hipStream_t s) -> int {
-    hipLaunchKernelGGL(solve_finalize_kernel, dim3(kDualBlocks, 3), dim3(256), 0, s, out->f, P.a, pb->n, g_save,
-                       out->g, P.b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors, outer, dual_part, state,
-                       mail, seq, recenter, out->f);
+    if (rc_mode) {
+      // the recentred outputs: f in place, g into the column-marginal buffer,
+      // which the finalizer then copies to g
+      hipLaunchKernelGGL(rc_output_kernel, dim3(1), dim3(kRcThreads), 0, s, out->f,
+                         static_cast<const float*>(out->g), rc_col, pb->n, pb->m, pb->tau_a, pb->tau_b,
+                         static_cast<const SolveScalars*>(P.scal), static_cast<const RcScalars*>(rcs),
+                         static_cast<const SolveState*>(state));
+      OTTX_LAUNCHED();
+    }
+    hipLaunchKernelGGL(solve_finalize_kernel, dim3(kDualBlocks, 3), dim3(256), 0, s, out->f, P.a, pb->n,
+                       rc_mode ? rc_col : g_save, out->g, P.b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors,
+                       outer, dual_part, state, mail, seq, recenter, out->f);
     OTTX_LAUNCHED();
     return OTTX_OK;
   };
@@ -2509,8 +2788,15 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     return static_cast<int64_t>(v & 0x7fffffffu);
   };
   if (total_blocks > 0) {
-    rc = launch_halfstep(k, g_plain, st);  // g-update of iteration 0
-    if (rc != OTTX_OK) return rc;
+    if (rc_mode) {
+      // log sum a exp(-f / rho_a) of the starting f, for the first g-update
+      hipLaunchKernelGGL(rc_lse_kernel, dim3(1), dim3(kRcThreads), 0, st, static_cast<const float*>(out->f), P.a,
+                         pb->n, pb->tau_a, static_cast<const SolveScalars*>(P.scal), &rcs->lse[0]);
+      OTTX_LAUNCHED();
+    } else {
+      rc = launch_halfstep(k, g_plain, st);  // g-update of iteration 0
+      if (rc != OTTX_OK) return rc;
+    }
     auto launch_all = [&](hipStream_t s) -> int {
       const int r = launch_block(s);
       return r != OTTX_OK ? r : launch_finalize(s);

@@ -280,10 +280,6 @@ class Sinkhorn:
       # the schedule runs on balanced problems with sequential updates
       raise NotImplementedError("epsilon schedules (init > 1) are supported for balanced problems "
                                 "without parallel_dual_updates only")
-    if self.recenter_potentials and ot_prob.tau_a < 1.0 and ot_prob.tau_b < 1.0:
-      # the recentring terms of lse_step (sinkhorn.py:809-854) are not implemented;
-      # balanced problems and those with one tau = 1 are supported
-      raise NotImplementedError("recenter_potentials with tau_a < 1 and tau_b < 1 is not supported yet")
     if self.recenter_potentials and self.max_iterations == 0:
       raise NotImplementedError("recenter_potentials with max_iterations = 0 is not supported")
     if self.parallel_dual_updates and not _is_symmetric(ot_prob):

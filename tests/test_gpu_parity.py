@@ -297,6 +297,35 @@ def test_recenter_potentials_unbalanced_one_side_is_a_no_op():
   assert out.reg_ot_cost == plain.reg_ot_cost
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("taus,inner,momentum,zeros", [((0.8, 0.9), 1, 1.0, False),
+                                                       ((0.95, 0.95), 10, 1.0, True),
+                                                       ((0.7, 0.85), 5, 0.8, False)])
+def test_recenter_potentials_unbalanced_both_sides(precision, taus, inner, momentum, zeros):
+  """recenter_potentials=True with tau_a, tau_b < 1: the translation-invariant
+  iteration of lse_step (sinkhorn.py:803-857), errors of the recentred state
+  (sinkhorn.py:57-68) and recentred outputs (sinkhorn.py:1008-1009), against
+  the oracle; the outputs are translation-optimal (a zero Sejourne shift)."""
+  from ott_amd.solvers.linear import acceleration
+  ta, tb = taus
+  x, y, a, b = _cloud(211, 157, 5, seed=23, zero_a=[4] if zeros else None, zero_b=[9] if zeros else None)
+  b = (b * 1.3).astype(np.float32)
+  kw = dict(inner_iterations=inner, threshold=1e-4, max_iterations=400)
+  out = _solve(x, y, a, b, precision, dict(epsilon=0.1), recenter_potentials=True, tau_a=ta, tau_b=tb,
+               momentum=acceleration.Momentum(value=momentum), **kw)
+  ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.1, recenter_potentials=True, tau_a=ta, tau_b=tb,
+                                momentum=momentum, **kw)
+  assert ref.converged
+  _check_solve(out, ref, precision)
+  f = out.f.cpu().numpy().astype(np.float64)
+  g = out.g.cpu().numpy().astype(np.float64)
+  if zeros:
+    assert np.isneginf(f[4]) and np.isneginf(g[9])
+  fr, _ = oracle.recenter(f, g, a.astype(np.float64), b.astype(np.float64), 0.1, ta, tb)
+  fin = np.isfinite(f)
+  assert np.max(np.abs(fr[fin] - f[fin])) <= 1e-4 * np.abs(f[fin]).max()
+
+
 @pytest.mark.parametrize("mode", ["mean", "max_norm", "max_bound", "max_cost"])
 def test_scale_cost_modes(mode):
   """PointCloud(scale_cost=...) string modes (pointcloud.py:342-383): the
