@@ -2547,13 +2547,13 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   if (symmetric && (pb->x != pb->y || pb->n != pb->m || pb->a != pb->b || pb->tau_a != 1.0 ||
                     pb->tau_b != 1.0 || o->init_mode != OTTX_INIT_ZEROS || o->recenter_potentials))
     return OTTX_EUNSUPPORTED;
-  // epsilon schedule (epsilon_scheduler.py:47-60): balanced problems with
-  // sequential updates
+  // epsilon schedule (epsilon_scheduler.py:47-60) with sequential updates, not
+  // with the recentred iteration (whose soft minima take rho at the target)
   const double eps_init = o->epsilon_init > 1.0 ? o->epsilon_init : 1.0;
   const double eps_decay = o->epsilon_decay > 0.0 ? o->epsilon_decay : 1.0;
   if (!(eps_decay <= 1.0) || !(eps_init < 1e30)) return OTTX_EINVAL;
   const bool sched = eps_init > 1.0;
-  if (sched && (symmetric || pb->tau_a != 1.0 || pb->tau_b != 1.0)) return OTTX_EUNSUPPORTED;
+  if (sched && (symmetric || rc_mode)) return OTTX_EUNSUPPORTED;
   // adaptive momentum (acceleration.py:138-160) on L1 errors, from an
   // iteration that leaves two recorded blocks before it
   const int mom_start = o->momentum_start;
@@ -2627,7 +2627,15 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   g_meas.lse_shift = g_meas.lse_store = nullptr;
   g_meas.err_pass = 1;
   g_meas.it_offset = inner;
-  for (HalfStepArgs* a : {&g_plain, &g_err, &f_plain, &f_err, &g_meas}) {
+  // Unbalanced problems also need the row error at the target eps: a pass over
+  // the rows of the state after the f-update of iteration it (f not written),
+  // before the g-update that finalizes the block reads its partials.
+  HalfStepArgs f_meas = fa;
+  f_meas.mode = kModeCheckDone | kModeErrRow;
+  f_meas.lse_shift = f_meas.lse_store = nullptr;
+  f_meas.err_pass = 1;
+  f_meas.it_offset = inner - 1;
+  for (HalfStepArgs* a : {&g_plain, &g_err, &f_plain, &f_err, &g_meas, &f_meas}) {
     a->eps_sched = sched ? 1 : 0;
     a->eps_init = eps_init;
     a->eps_decay = eps_decay;
@@ -2716,6 +2724,7 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
         fa_it.it_offset = it;
         ga_it.it_offset = it + 1;
         if ((r = launch_halfstep(k, fa_it, s)) != OTTX_OK) return r;
+        if (sched && !balanced && it == inner - 1 && (r = launch_halfstep(k, f_meas, s)) != OTTX_OK) return r;
         if ((r = launch_halfstep(k, ga_it, s)) != OTTX_OK) return r;
         continue;
       }

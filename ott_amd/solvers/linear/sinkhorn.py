@@ -276,10 +276,12 @@ class Sinkhorn:
     if not isinstance(ot_prob.geom, PointCloud):
       raise NotImplementedError("ott_amd accelerates PointCloud geometries only")
     if not ot_prob.geom._schedule_is_target_only() and (
-        ot_prob.tau_a != 1.0 or ot_prob.tau_b != 1.0 or self.parallel_dual_updates):
-      # the schedule runs on balanced problems with sequential updates
-      raise NotImplementedError("epsilon schedules (init > 1) are supported for balanced problems "
-                                "without parallel_dual_updates only")
+        self.parallel_dual_updates or
+        (self.recenter_potentials and ot_prob.tau_a < 1.0 and ot_prob.tau_b < 1.0)):
+      # the schedule runs with sequential updates, and not with the recentred
+      # unbalanced iteration
+      raise NotImplementedError("epsilon schedules (init > 1) are not supported with parallel_dual_updates "
+                                "or with recenter_potentials on problems with both taus < 1")
     if self.recenter_potentials and self.max_iterations == 0:
       raise NotImplementedError("recenter_potentials with max_iterations = 0 is not supported")
     if self.parallel_dual_updates and not _is_symmetric(ot_prob):
