@@ -283,6 +283,20 @@ int ottx_pointcloud_op_noalloc(const ottx_problem* problem, int32_t op, int32_t 
 /* Scratch bytes ottx_pointcloud_op_noalloc needs for the problem (0: invalid). */
 size_t ottx_pointcloud_op_noalloc_bytes(const ottx_problem* problem);
 
+/* Transport applied to vectors, geometry.py:445-486
+ * (apply_transport_from_potentials, vmapped over a batch):
+ *   axis 0: out[v * m + j] = sum_i P_ij vecs[v * n + i]   (vecs [k, n], out [k, m])
+ *   axis 1: out[v * n + i] = sum_j P_ij vecs[v * m + j]   (vecs [k, m], out [k, n])
+ * with P_ij = exp((f_i + g_j - C_ij) / eps), signed vectors included (the
+ * reference's signed log-sum-exp with b=vec, return_sign=True).  One
+ * log-sum-exp pass, then one launch per 16 vectors: the cross term on MFMA as
+ * in the half-step, the products with the vectors on fp32 MFMA.  Backs
+ * SinkhornOutput.apply (sinkhorn.py:453-467). */
+size_t ottx_apply_workspace_bytes(const ottx_problem* problem, int32_t axis);
+int ottx_pointcloud_apply(const ottx_problem* problem, int32_t axis, const float* f, const float* g,
+                          const float* vecs, int32_t k, double eps, float* out, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
 /* Cost maxima behind PointCloud's scale_cost modes (pointcloud.py:342-383):
  * out_host[0] = max_i |x_i|^2, out_host[1] = max_j |y_j|^2 (cost_fn.norm of the
  * squared-Euclidean cost, for "max_norm" and "max_bound") and, when

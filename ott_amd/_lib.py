@@ -195,6 +195,10 @@ _SIGNATURES = [
      [ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     ("ottx_pointcloud_op_noalloc_bytes", ctypes.c_size_t, [ctypes.POINTER(Problem)]),
+    ("ottx_apply_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(Problem), ctypes.c_int32]),
+    ("ottx_pointcloud_apply", ctypes.c_int,
+     [ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+      ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     ("ottx_cost_maxima", ctypes.c_int,
      [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
       ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),

@@ -944,6 +944,134 @@ __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) 
   halfstep_block<PREC, KK, NB, MT, W>(p, blockIdx.x, blockIdx.y);
 }
 
+// ---------------------------------------------------------------------------
+// Transport applied to vectors (geometry.py:445-486): out[v][j] = sum_i P_ij
+// V[i][v] with P_ij = exp((h_i + h_j - C_ij) / eps), h_i the in side's and h_j
+// the out side's potential, for 16 vectors per launch, signs and all.  The
+// exponents come from the half-step's MFMA cross term; with the column's
+// log-sum-exp L_j (exponent units, from a half-step pass) the weights
+// exp2(acc_ij - L_j) lie in [0, 1] and sum to one over i, the products with
+// V run on fp32 MFMA (16x16x4: for r < 4, lane l gives P[4 (l / 16) + r][l %
+// 16] as A and V[4 (l / 16) + r][l % 16] as B), and the column factor
+// exp((L_j 2 s / c + h_j - s|y_j|^2) / eps) is applied once at the end.
+// V arrives in fragments of 16 rows: [rb][lane = 16 q + v][r] = V[16 rb + 4 q + r][v].
+// ---------------------------------------------------------------------------
+struct ApplyArgs {
+  HalfStepArgs h;      // in / out sides, potentials, norms, plan; lse_shift = the column log-sum-exp
+  const void* v_frag;  // the 16 vectors' fragments over the in side's rows
+  float* o_part;       // [splits][n_out_pad][16] partial sums
+  float* out;          // out[v * out_stride + j]
+  int64_t out_stride;
+  int32_t kvec;        // vectors of this launch (<= 16)
+};
+
+template <int PREC, int KK, int NB, int MT, int W>
+__global__ __launch_bounds__(W * 64) void apply_pv_kernel(const ApplyArgs A) {
+  const HalfStepArgs& p = A.h;
+  constexpr int NT = W * 64;
+  constexpr int COLS = W * NB * 16;
+  constexpr int CH = FragChunks<PREC, KK>::value;
+  constexpr int TILE_U4 = MT * CH * 64;
+  constexpr int VT_U4 = MT * 64;
+  constexpr int ROWS = MT * 16;
+  __shared__ u32x4 s_tile[TILE_U4];
+  __shared__ u32x4 s_v[VT_U4];
+  __shared__ float s_bias[ROWS];
+  __shared__ int s_last;
+  const int tile = blockIdx.x, split = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const float c_pack = p.pack_scal != nullptr ? p.pack_scal->c : p.c;
+  const float c = p.scal != nullptr ? p.scal->c : p.c;
+  const float eps = p.scal != nullptr ? p.scal->eps_f : p.eps;
+  const float k = c / c_pack;
+  const float kb = static_cast<float>(static_cast<double>(c_pack) / p.two_s);
+
+  OutOperand<PREC, KK, NB> B;
+  B.load(p.out_frag, static_cast<int64_t>(tile * W + wave) * NB, lane);
+  const int64_t cbase = static_cast<int64_t>(tile * W + wave) * NB * 16 + (lane & 15);
+  float m[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const float L = p.lse_shift[cbase + nb * 16] * k;
+    m[nb] = isfinite(L) ? L : 0.f;  // -inf: every weight of the column is 0 already
+  }
+  f32x4 O[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) O[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const u32x4* __restrict__ src = static_cast<const u32x4*>(p.in_frag);
+  const u32x4* __restrict__ vsrc = static_cast<const u32x4*>(A.v_frag);
+  const int rb_begin = split * p.rb_per_split;
+  const int rb_end = min(rb_begin + p.rb_per_split, p.n_in_rb);
+  for (int rb = rb_begin; rb < rb_end; rb += MT) {
+    for (int q = tid; q < TILE_U4; q += NT) s_tile[q] = src[static_cast<size_t>(rb) * CH * 64 + q];
+    for (int q = tid; q < VT_U4; q += NT) s_v[q] = vsrc[static_cast<size_t>(rb) * 64 + q];
+    if (tid < ROWS) {
+      const int64_t row = static_cast<int64_t>(rb) * 16 + tid;
+      const int64_t i = min(row, p.n_in - 1);
+      s_bias[tid] = row_bias(make_float2(p.in_pot[i], p.in_norm[i]), row, p.n_in, kb, 0.f);
+    }
+    __syncthreads();
+    InTile<PREC, KK, MT> Ain;
+    Ain.load(s_tile, s_bias, lane);
+    f32x4 acc[MT][NB];
+    tile_mfma<PREC, KK, NB, MT>(B, Ain, acc);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const float4 vv = __builtin_bit_cast(float4, s_v[mt * 64 + lane]);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        O[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(exp2_fast(fmaf(acc[mt][nb][0], k, -m[nb])), vv.x, O[nb], 0, 0, 0);
+        O[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(exp2_fast(fmaf(acc[mt][nb][1], k, -m[nb])), vv.y, O[nb], 0, 0, 0);
+        O[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(exp2_fast(fmaf(acc[mt][nb][2], k, -m[nb])), vv.z, O[nb], 0, 0, 0);
+        O[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(exp2_fast(fmaf(acc[mt][nb][3], k, -m[nb])), vv.w, O[nb], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+
+  // Split-K: partials stored write-through, the last split to arrive sums them.
+  const int64_t n_out_pad = static_cast<int64_t>(p.tiles) * COLS;
+  const int64_t col0 = static_cast<int64_t>(tile) * COLS;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t j = col0 + (wave * NB + nb) * 16 + 4 * (lane >> 4) + t;
+      __hip_atomic_store(reinterpret_cast<uint32_t*>(A.o_part + (split * n_out_pad + j) * 16 + (lane & 15)),
+                         __float_as_uint(O[nb][t]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  wait_own_stores();
+  __syncthreads();
+  if (tid == 0) {
+    const uint32_t prev = atomicAdd(p.tile_counter + tile, 1u);
+    s_last = (prev == static_cast<uint32_t>(p.splits - 1)) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  for (int e = tid; e < COLS * 16; e += NT) {
+    const int64_t j = col0 + e / 16;
+    const int v = e % 16;
+    float s = 0.f;
+    for (int sp = 0; sp < p.splits; ++sp)
+      s += __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(A.o_part + (sp * n_out_pad + j) * 16 + v),
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (j >= p.n_out || v >= A.kvec) continue;
+    const float L = p.lse_shift[j];
+    const float h = p.out_pot[j];
+    float r;
+    if (L == -INFINITY || h == -INFINITY) {
+      r = (s != s) ? s : 0.f;
+    } else {
+      const double ex = (static_cast<double>(L) * p.two_s / static_cast<double>(c_pack) + static_cast<double>(h) -
+                         static_cast<double>(p.out_norm[j])) / static_cast<double>(eps);
+      r = static_cast<float>(static_cast<double>(s) * exp(ex));
+    }
+    A.out[static_cast<int64_t>(v) * A.out_stride + j] = r;
+  }
+  if (tid == 0) p.tile_counter[tile] = 0u;
+}
+
 // Batched half-step over many independent problems in one launch.  Workgroup
 // b runs (problem, column tile, row split) = work[b].x, .y, .z with that
 // problem's arguments; ``mode`` replaces the per-problem mode.  Problems whose

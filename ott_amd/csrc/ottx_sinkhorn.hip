@@ -2957,6 +2957,177 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// Transport applied to vectors (geometry.py:445-486): a column log-sum-exp
+// pass of the half-step, then apply_pv_kernel for each block of 16 vectors.
+// ---------------------------------------------------------------------------
+typedef void (*ApplyFn)(const ApplyArgs);
+struct ApplyCfg {
+  ApplyFn fn;
+  int nb, mt, w;
+};
+
+template <int PREC, int KK, int NB, int MT>
+ApplyCfg apply_cfg() {
+  return ApplyCfg{&apply_pv_kernel<PREC, KK, NB, MT, 4>, NB, MT, 4};
+}
+
+// One tile shape per fragment layout; wide features keep fewer column blocks
+// and rows in registers.
+bool select_apply(int prec, int kk, ApplyCfg* out) {
+  if (prec == OTTX_PREC_BF16_SPLIT) {
+    switch (kk) {
+      case 1: *out = apply_cfg<1, 1, 2, 2>(); return true;
+      case 2: *out = apply_cfg<1, 2, 2, 2>(); return true;
+      case 3: *out = apply_cfg<1, 3, 2, 2>(); return true;
+      case 4: *out = apply_cfg<1, 4, 2, 2>(); return true;
+      case 5: *out = apply_cfg<1, 5, 1, 1>(); return true;
+      case 6: *out = apply_cfg<1, 6, 1, 1>(); return true;
+      case 7: *out = apply_cfg<1, 7, 1, 1>(); return true;
+      case 8: *out = apply_cfg<1, 8, 1, 1>(); return true;
+      default: return false;
+    }
+  }
+  switch (kk) {
+    case 1: *out = apply_cfg<0, 1, 2, 2>(); return true;
+    case 2: *out = apply_cfg<0, 2, 2, 2>(); return true;
+    case 4: *out = apply_cfg<0, 4, 2, 2>(); return true;
+    case 8: *out = apply_cfg<0, 8, 2, 2>(); return true;
+    case 12: *out = apply_cfg<0, 12, 2, 2>(); return true;
+    case 16: *out = apply_cfg<0, 16, 2, 2>(); return true;
+    case 24: *out = apply_cfg<0, 24, 1, 2>(); return true;
+    case 32: *out = apply_cfg<0, 32, 1, 2>(); return true;
+    case 48: *out = apply_cfg<0, 48, 1, 1>(); return true;
+    case 64: *out = apply_cfg<0, 64, 1, 1>(); return true;
+    default: return false;
+  }
+}
+
+// Fragments of up to 16 vectors over n rows (zero past n and past kvec):
+// frag[(rb * 64 + 16 q + v) * 4 + r] = vecs[v][16 rb + 4 q + r].
+__global__ void pack_vectors_kernel(const float* __restrict__ vecs, int64_t n, int kvec, int64_t n_pad,
+                                    float* __restrict__ frag) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_pad * 16) return;
+  const int r = static_cast<int>(e & 3);
+  const int l = static_cast<int>((e >> 2) & 63);
+  const int64_t rb = e >> 8;
+  const int v = l & 15;
+  const int64_t i = rb * 16 + 4 * (l >> 4) + r;
+  frag[e] = (v < kvec && i < n) ? vecs[static_cast<int64_t>(v) * n + i] : 0.f;
+}
+
+__global__ void zero_u32_kernel(uint32_t* __restrict__ p, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+struct ApplyLayout {
+  ApplyCfg ac;
+  PassPlan pv;
+  size_t vfrag, opart, counters, total;
+};
+
+bool apply_layout(const ottx_problem* pb, int axis, const Plan& PL, ApplyLayout* AL) {
+  if (!select_apply(PL.k.prec, PL.k.kk, &AL->ac)) return false;
+  KernelCfg kpv = PL.k;
+  kpv.nb = AL->ac.nb;
+  kpv.mt = AL->ac.mt;
+  kpv.w = AL->ac.w;
+  const int64_t n_in = axis == 0 ? pb->n : pb->m, n_out = axis == 0 ? pb->m : pb->n;
+  AL->pv = plan_pass(kpv, n_in, n_out, device_cus(), 2);
+  size_t off = PL.L.total;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align_up(off + bytes);
+    return at;
+  };
+  const int64_t n_in_pad = pad_rows(n_in), n_out_pad = pad_rows(n_out);
+  AL->vfrag = take(static_cast<size_t>(n_in_pad) * 16 * sizeof(float));
+  AL->opart = take(static_cast<size_t>(AL->pv.splits) * n_out_pad * 16 * sizeof(float));
+  AL->counters = take(static_cast<size_t>(AL->pv.tiles) * sizeof(uint32_t));
+  AL->total = off;
+  return static_cast<int64_t>(AL->pv.tiles) * kpv.cols() == n_out_pad && AL->pv.n_in_rb * 16 == n_in_pad;
+}
+
+extern "C" size_t ottx_apply_workspace_bytes(const ottx_problem* pb, int32_t axis) {
+  if (check_problem(pb) != OTTX_OK || (axis != 0 && axis != 1)) return 0;
+  Plan PL;
+  if (!make_plan(pb, -1, &PL)) return 0;
+  ApplyLayout AL;
+  if (!apply_layout(pb, axis, PL, &AL)) return 0;
+  return AL.total;
+}
+
+extern "C" int ottx_pointcloud_apply(const ottx_problem* pb, int32_t axis, const float* f, const float* g,
+                                     const float* vecs, int32_t kvec, double eps, float* out, void* workspace,
+                                     size_t workspace_bytes, void* stream) {
+  int rc = check_problem(pb);
+  if (rc != OTTX_OK) return rc;
+  if (!f || !g || !vecs || !out || kvec < 1 || !(eps > 0.0) || (axis != 0 && axis != 1)) return OTTX_EINVAL;
+  if (!workspace || (reinterpret_cast<uintptr_t>(workspace) & 255)) return OTTX_EWORKSPACE;
+  Plan PL;
+  if (!make_plan(pb, -1, &PL)) return OTTX_EUNSUPPORTED;
+  ApplyLayout AL;
+  if (!apply_layout(pb, axis, PL, &AL)) return OTTX_EUNSUPPORTED;
+  if (workspace_bytes < AL.total) return OTTX_EWORKSPACE;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const KernelCfg& k = PL.k;
+  const Layout& L = PL.L;
+  Prepared P;
+  rc = prepare(pb, k, L, workspace, st, &P);
+  if (rc != OTTX_OK) return rc;
+  char* base = P.base;
+  uint32_t* counters = reinterpret_cast<uint32_t*>(base + L.counters);
+  uint32_t* pv_counters = reinterpret_cast<uint32_t*>(base + AL.counters);
+  const int max_tiles = std::max(PL.gp.tiles, PL.fp.tiles);
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(static_cast<unsigned>((max_tiles + 255) / 256)), dim3(256), 0, st,
+                     counters, static_cast<int64_t>(max_tiles));
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(static_cast<unsigned>((AL.pv.tiles + 255) / 256)), dim3(256), 0, st,
+                     pv_counters, static_cast<int64_t>(AL.pv.tiles));
+  OTTX_LAUNCHED();
+  // the column log-sum-exp in exponent units (no epilogue output)
+  const bool g_upd = axis == 0;
+  const float* h = g_upd ? g : f;
+  const float* in_pot = g_upd ? f : g;
+  float* lse = reinterpret_cast<float*>(base + (g_upd ? L.lse_y : L.lse_x));
+  HalfStepArgs a = base_args(pb, k, g_upd ? PL.gp : PL.fp, P, g_upd ? 1 : 0, const_cast<float*>(in_pot),
+                             const_cast<float*>(h), eps);
+  a.state = nullptr;
+  a.scal = nullptr;
+  a.mode = 0;
+  a.lse_shift = nullptr;
+  a.lse_store = lse;
+  rc = launch_halfstep(k, a, st);
+  if (rc != OTTX_OK) return rc;
+  // P V, 16 vectors per launch
+  const int64_t n_in = g_upd ? pb->n : pb->m, n_out = g_upd ? pb->m : pb->n;
+  const int64_t n_in_pad = pad_rows(n_in);
+  ApplyArgs A{};
+  A.h = a;
+  A.h.tiles = AL.pv.tiles;
+  A.h.splits = AL.pv.splits;
+  A.h.rb_per_split = AL.pv.rb_per_split;
+  A.h.n_in_rb = AL.pv.n_in_rb;
+  A.h.lse_shift = lse;
+  A.h.lse_store = nullptr;
+  A.h.tile_counter = pv_counters;
+  float* vfrag = reinterpret_cast<float*>(base + AL.vfrag);
+  A.v_frag = vfrag;
+  A.o_part = reinterpret_cast<float*>(base + AL.opart);
+  A.out_stride = n_out;
+  for (int v0 = 0; v0 < kvec; v0 += 16) {
+    const int kb = std::min(16, kvec - v0);
+    hipLaunchKernelGGL(pack_vectors_kernel, dim3(static_cast<unsigned>((n_in_pad * 16 + 255) / 256)), dim3(256), 0,
+                       st, vecs + static_cast<int64_t>(v0) * n_in, n_in, kb, n_in_pad, vfrag);
+    A.out = out + static_cast<int64_t>(v0) * n_out;
+    A.kvec = kb;
+    hipLaunchKernelGGL(AL.ac.fn, dim3(AL.pv.tiles, AL.pv.splits), dim3(64 * AL.ac.w), 0, st, A);
+    OTTX_LAUNCHED();
+  }
+  return OTTX_OK;
+}
+
 extern "C" int ottx_tile_configs(int32_t precision, int32_t d, int32_t* desc, int32_t max_configs) {
   KernelCfg v[8];
   const int n = kernel_variants(precision, d, v, 8);

@@ -408,46 +408,39 @@ class PointCloud:
   def apply_transport_from_potentials(self, f, g, vec, axis: int = 0) -> torch.Tensor:
     """geometry.py:445-486: P^T vec (axis 0, vec over the n rows) or P vec
     (axis 1, vec over the m columns), P = exp((f + g - C) / eps), for one
-    vector or a batch [k, n or m].
+    vector or a batch [k, n or m], signs included.
 
-    The reference's signed log-sum-exp (logsumexp with weights and
-    return_sign) is run as two fused log-sum-exp passes per vector, one with
-    the positive and one with the negative part of vec folded into the
-    potential as eps log(part).
+    ``ottx_pointcloud_apply``: one log-sum-exp pass over the pairs, then one
+    launch per 16 vectors that forms the weights on MFMA and multiplies them
+    with the vectors on fp32 MFMA (no host synchronisation).
     """
+    if axis not in (0, 1):
+      raise ValueError(f"axis must be 0 or 1, got {axis}")
+    lib = _lib.load_library()
     f = as_device_f32(f, self.device)
     g = as_device_f32(g, self.device)
     vec = as_device_f32(vec, self.device)
     single = vec.ndim == 1
-    vecs = vec[None] if single else vec
+    vecs = (vec[None] if single else vec).contiguous()
     n, m = self.shape
-    if vecs.ndim != 2 or vecs.shape[1] != (n if axis == 0 else m):
-      raise ValueError(f"vec must be [{n if axis == 0 else m}] or [k, {n if axis == 0 else m}] for axis={axis}")
-    eps = self.epsilon
-    keep_pot = g if axis == 0 else f  # the potential of the output side
-    outs = []
-    # which vectors have positive / negative entries: a part that is all zero
-    # contributes nothing and skips its pass (one host read for the batch)
-    lo, hi = torch.aminmax(vecs, dim=1)
-    signs = torch.stack([hi > 0, lo < 0], dim=1).tolist()
-    n_out = m if axis == 0 else n
-    for v, (has_pos, has_neg) in zip(vecs, signs):
-      total = None
-      for part, sign, present in ((torch.clamp(v, min=0.0), 1.0, has_pos), (torch.clamp(-v, min=0.0), -1.0, has_neg)):
-        if not present:
-          continue
-        with torch.no_grad():
-          shifted = (f if axis == 0 else g) + eps * torch.log(part)
-        res = self._op(_lib.OP_APPLY_LSE, shifted if axis == 0 else f, g if axis == 0 else shifted, eps, axis)
-        term = torch.exp((res.double() + keep_pot.double()) / eps)
-        if total is None:
-          total = term if sign > 0 else -term
-        else:  # the negative part, after the positive one
-          total = total - term
-      if total is None:
-        total = torch.zeros((n_out,), dtype=torch.float64, device=self.device)
-      outs.append(total.float())
-    out = torch.stack(outs)
+    if f.shape != (n,) or g.shape != (m,):
+      raise ValueError(f"potentials must be [{n}] and [{m}]")
+    n_in, n_out = (n, m) if axis == 0 else (m, n)
+    if vecs.ndim != 2 or vecs.shape[1] != n_in:
+      raise ValueError(f"vec must be [{n_in}] or [k, {n_in}] for axis={axis}")
+    k = vecs.shape[0]
+    out = torch.empty((k, n_out), dtype=torch.float32, device=self.device)
+    if k == 0:
+      return out[0] if single else out
+    pb, keep = self._problem(epsilon=self.epsilon)
+    nbytes = lib.ottx_apply_workspace_bytes(ctypes.byref(pb), int(axis))
+    if nbytes == 0:
+      raise NotImplementedError("apply_transport_from_potentials: unsupported problem shape")
+    ws = _lib.workspace(nbytes, self.device)
+    _lib.check("ottx_pointcloud_apply", lib.ottx_pointcloud_apply(
+        ctypes.byref(pb), int(axis), f.data_ptr(), g.data_ptr(), vecs.data_ptr(), int(k), float(self.epsilon),
+        out.data_ptr(), ws.data_ptr(), nbytes, _lib.stream_ptr()))
+    del keep, ws  # stream-ordered reuse by the caching allocator
     return out[0] if single else out
 
   def low_rank_cost_factors(self) -> Tuple[torch.Tensor, torch.Tensor]:

@@ -67,6 +67,46 @@ def test_apply_signed_batched(precision, axis):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("d", [8, 100])
+def test_apply_many_vectors(precision, axis, d):
+  """37 signed vectors (launches of 16, 16 and 5) on a problem with several
+  row splits, wide features, a zero weight, an all-zero vector and a NaN
+  entry, which makes its vector's output NaN as the reference's signed
+  log-sum-exp does."""
+  from ott_amd.geometry import pointcloud
+  from ott_amd.problems.linear import linear_problem
+  from ott_amd.solvers.linear import sinkhorn
+  rng = np.random.default_rng(40 + d)
+  n, m = 1500, 1100
+  x = (rng.normal(size=(n, d)) / np.sqrt(d)).astype(np.float32)
+  y = (rng.normal(size=(m, d)) / np.sqrt(d) + 0.2 / np.sqrt(d)).astype(np.float32)
+  a = rng.uniform(0.2, 1.2, n)
+  a[11] = 0.0
+  a = (a / a.sum()).astype(np.float32)
+  geom = pointcloud.PointCloud(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), epsilon=0.05,
+                               relative_epsilon="mean", precision=precision)
+  out = sinkhorn.Sinkhorn(threshold=1e-4)(linear_problem.LinearProblem(geom, torch.from_numpy(a).cuda()))
+  assert out.converged
+  f, g = out.f.double().cpu().numpy(), out.g.double().cpu().numpy()
+  cost = oracle.sqeuclidean_cost(x.astype(np.float64), y.astype(np.float64))
+  vec = rng.normal(size=(37, n if axis == 0 else m)).astype(np.float32)
+  vec[9] = 0.0
+  vec[20, 3] = np.nan
+  got = out.apply(torch.from_numpy(vec).cuda(), axis=axis).double().cpu().numpy()
+  ok = np.ones(37, bool)
+  ok[20] = False
+  ref = oracle.apply_transport_from_potentials(cost, f, g, vec[ok].astype(np.float64), geom.epsilon, axis=axis)
+  mag = oracle.apply_transport_from_potentials(cost, f, g, np.abs(vec[ok]).astype(np.float64), geom.epsilon,
+                                               axis=axis)
+  # bf16 split: the cross term's relative error times the exponents' size
+  tol = 2e-5 if precision == "fp32" else 1e-3
+  np.testing.assert_allclose(got[ok], ref, atol=tol * float(mag.max()))
+  assert np.all(got[9] == 0.0)
+  assert np.all(np.isnan(got[20]))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
 def test_primal_cost_and_transport_cost_at_geom(precision):
   from ott_amd.geometry import pointcloud
   out, x, y, f, g, cost, eps = _solved(precision)
