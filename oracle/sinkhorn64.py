@@ -73,6 +73,8 @@ __all__ = [
     "transport_from_potentials",
     "apply_transport_from_potentials",
     "transport_cost_at_cost",
+    "soft_c_transform",
+    "barycentric_map",
 ]
 
 DEFAULT_EPSILON_SCALE = 0.05  # geometry/epsilon_scheduler.py:22
@@ -597,6 +599,27 @@ def apply_transport_from_potentials(cost: np.ndarray, f, g, vec, eps: float, axi
   p = transport_from_potentials(cost, f, g, eps)
   v = np.asarray(vec, dtype=np.float64)
   return (v @ p) if axis == 0 else (v @ p.T)
+
+
+def soft_c_transform(cost: np.ndarray, fg, weights, eps: float, axis: int) -> np.ndarray:
+  """LinearProblem._c_transform, soft branch (linear_problem.py:149-171):
+  fg expanded on axis 1 - axis, z = (fg - C) / eps, -eps logsumexp(z, b=weights,
+  axis=axis) with ``weights`` broadcast against z as the reference's does."""
+  fg = np.expand_dims(np.asarray(fg, np.float64), 1 - axis)
+  z = (fg - cost) / eps
+  w = np.broadcast_to(np.asarray(weights, np.float64), z.shape)
+  return -eps * sps.logsumexp(z, b=w, axis=axis)
+
+
+def barycentric_map(cost: np.ndarray, h, weights, eps: float, support) -> np.ndarray:
+  """x - grad f(x) / 2 for f(x) = -eps log sum_k w_k exp((h_k - |x - z_k|^2) / eps)
+  (potentials.py:52-84, costs.py:220-226): the pi-weighted mean of the support,
+  pi_k(x) proportional to w_k exp((h_k - C(x, z_k)) / eps); ``cost`` is [k_new, K]."""
+  z = (np.asarray(h, np.float64)[None, :] - cost) / eps
+  with np.errstate(divide="ignore"):
+    z = z + np.log(np.broadcast_to(np.asarray(weights, np.float64), z.shape))
+  pi = np.exp(z - sps.logsumexp(z, axis=1, keepdims=True))
+  return pi @ np.asarray(support, np.float64)
 
 
 def transport_cost_at_cost(cost_p: np.ndarray, f, g, eps: float, other_cost: np.ndarray) -> float:

@@ -1,3 +1,3 @@
-from ott_amd.problems.linear import linear_problem
+from ott_amd.problems.linear import linear_problem, potentials
 
-__all__ = ["linear_problem"]
+__all__ = ["linear_problem", "potentials"]

@@ -112,6 +112,12 @@ class SinkhornOutput(NamedTuple):
       raise NotImplementedError("apply with lse_mode=False is not supported by ott_amd")
     return self.geom.apply_transport_from_potentials(self.f, self.g, inputs, axis=axis)
 
+  def to_dual_potentials(self, epsilon: Optional[float] = None):
+    """sinkhorn.py:480-499: the dual potential functions, soft c-transforms of
+    g against (y, b) and of f against (x, a) (linear_problem.py:111-171)."""
+    from ott_amd.problems.linear import potentials
+    return potentials.from_solution(self.geom, self.f, self.g, self.a, self.b, epsilon=epsilon)
+
   def transport_cost_at_geom(self, other_geom: PointCloud) -> float:
     """sinkhorn.py:386-411: <P, C'> for another PointCloud's cost C'.
 

@@ -172,3 +172,38 @@ def test_diag_and_normalized_entropy():
   np.testing.assert_allclose(out.diag.double().cpu().numpy(), np.diag(p), rtol=1e-5, atol=1e-12)
   ent = float(special.entr(p).sum())
   assert math.isclose(out.normalized_entropy, ent / math.log(128) - 1.0, rel_tol=1e-5, abs_tol=1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+def test_to_dual_potentials(precision):
+  """SinkhornOutput.to_dual_potentials (sinkhorn.py:480-499): f and g as soft
+  c-transforms at new points (batched and single, g with the reference's
+  eps log k for a batch of k), the forward and backward transport maps
+  (potentials.py:52-84) and the distance (potentials.py:86-101), against the
+  oracle."""
+  out, x, y, f, g, cost, eps = _solved(precision)
+  a = out.a.double().cpu().numpy()
+  b = out.b.double().cpu().numpy()
+  dp = out.to_dual_potentials()
+  rng = np.random.default_rng(12)
+  xs = (rng.normal(size=(50, 8)) / np.sqrt(8)).astype(np.float32)
+  ys = (rng.normal(size=(40, 8)) / np.sqrt(8) + 0.25).astype(np.float32)
+  cx = oracle.sqeuclidean_cost(xs.astype(np.float64), y.astype(np.float64))
+  cy = oracle.sqeuclidean_cost(x.astype(np.float64), ys.astype(np.float64))
+  f_ref = oracle.soft_c_transform(cx, g, b, eps, axis=1)
+  g_ref = oracle.soft_c_transform(cy, f, np.full(40, 1.0 / 40), eps, axis=0)
+  tol = 1e-4 if precision == "fp32" else 1e-3
+  scale = max(np.abs(f_ref).max(), np.abs(g_ref).max(), eps)
+  np.testing.assert_allclose(dp.f(torch.from_numpy(xs).cuda()).double().cpu().numpy(), f_ref, atol=tol * scale)
+  np.testing.assert_allclose(dp.g(torch.from_numpy(ys).cuda()).double().cpu().numpy(), g_ref, atol=tol * scale)
+  g_one = oracle.soft_c_transform(cy[:, :1], f, np.ones(1), eps, axis=0)[0]
+  assert abs(float(dp.g(torch.from_numpy(ys[0]).cuda())) - g_one) <= tol * scale
+  t_fwd = oracle.barycentric_map(cx, g, b, eps, y)
+  t_bwd = oracle.barycentric_map(cy.T, f, np.ones(len(f)), eps, x)
+  np.testing.assert_allclose(dp.transport(torch.from_numpy(xs).cuda()).double().cpu().numpy(), t_fwd,
+                             atol=tol * np.abs(y).max())
+  np.testing.assert_allclose(dp.transport(torch.from_numpy(ys).cuda(), forward=False).double().cpu().numpy(), t_bwd,
+                             atol=tol * np.abs(x).max())
+  g_each = np.array([oracle.soft_c_transform(cy[:, j:j + 1], f, np.ones(1), eps, axis=0)[0] for j in range(40)])
+  dist_ref = f_ref.mean() + g_each.mean()
+  assert abs(dp.distance(torch.from_numpy(xs).cuda(), torch.from_numpy(ys).cuda()) - dist_ref) <= tol * scale

@@ -426,3 +426,35 @@ def test_oracle_reg_ot_cost_grad_against_finite_differences():
   bm[0] -= h
   bm[3] += h
   assert abs((cost(x, y, a, bp) - cost(x, y, a, bm)) / (2 * h) - (gb[0] - gb[3])) < 1e-6
+
+
+def test_soft_c_transform_and_barycentric_map():
+  """soft_c_transform (linear_problem.py:149-171) and barycentric_map
+  (potentials.py:52-84): a one-point support's closed form c(x, y) - g -
+  eps log b, the hard c-transform -max_j (g_j - C_ij) as eps -> 0, the
+  broadcasting of the new points' uniform weights on axis 0 (+ eps log k),
+  and the map against central differences of f: T(x) = x - grad f(x) / 2."""
+  rng = np.random.default_rng(8)
+  x = rng.normal(size=(5, 3))
+  y = rng.normal(size=(7, 3))
+  g = rng.normal(size=7)
+  b = rng.uniform(0.1, 1.0, 7)
+  cost = oracle.sqeuclidean_cost(x, y)
+  one = oracle.soft_c_transform(cost[:, :1], g[:1], b[:1], 0.3, axis=1)
+  np.testing.assert_allclose(one, cost[:, 0] - g[0] - 0.3 * np.log(b[0]), rtol=1e-13)
+  hard = -np.max(g[None, :] - cost, axis=1)
+  np.testing.assert_allclose(oracle.soft_c_transform(cost, g, b, 1e-4, axis=1), hard, atol=2e-3)
+  f = rng.normal(size=5)
+  k = cost.shape[1]
+  ga = oracle.soft_c_transform(cost, f, np.full(k, 1.0 / k), 0.3, axis=0)
+  brute = np.array([-0.3 * np.log(np.sum(np.exp((f - cost[:, j]) / 0.3)) / k) for j in range(k)])
+  np.testing.assert_allclose(ga, brute, rtol=1e-12)
+
+  def f_of(pt):
+    return oracle.soft_c_transform(oracle.sqeuclidean_cost(pt[None, :], y), g, b, 0.3, axis=1)[0]
+
+  t = oracle.barycentric_map(oracle.sqeuclidean_cost(x, y), g, b, 0.3, y)
+  h = 1e-6
+  for i in range(len(x)):
+    grad = np.array([(f_of(x[i] + h * e) - f_of(x[i] - h * e)) / (2 * h) for e in np.eye(3)])
+    np.testing.assert_allclose(t[i], x[i] - grad / 2, atol=1e-7)
