@@ -120,8 +120,10 @@ typedef struct ottx_sinkhorn_opts {
   int32_t recenter_potentials; /* 1: SinkhornState.recenter on the outputs (sinkhorn.py:76-114,
                                1008-1009): f -= mean of its finite entries, g += the same, for a
                                balanced problem; no change when tau_a or tau_b is 1 (the errors
-                               are unchanged in both cases).  Unbalanced problems with tau_a < 1
-                               and tau_b < 1 return OTTX_EUNSUPPORTED. */
+                               are unchanged in both cases).  With tau_a < 1 and tau_b < 1 the
+                               translation-invariant iteration of lse_step (sinkhorn.py:809-854)
+                               runs, the errors are those of the recentred state and the outputs
+                               are recentred (the Sejourne shift). */
   int32_t parallel_dual_updates; /* 1: Sinkhorn(parallel_dual_updates=True) (sinkhorn.py:840-846):
                                the f-update reads the g of the previous iteration.  Supported for
                                the symmetric problems of a Sinkhorn divergence
@@ -135,8 +137,9 @@ typedef struct ottx_sinkhorn_opts {
                                max(epsilon_init epsilon_decay^t, 1) epsilon; the marginal error
                                and the outputs use epsilon itself.  Values <= 1 (0 included) run
                                at epsilon throughout.  A schedule (epsilon_init > 1) is supported
-                               for balanced problems without parallel updates; others return
-                               OTTX_EUNSUPPORTED, and so does the batched solve. */
+                               without parallel updates and without the recentred unbalanced
+                               iteration; those return OTTX_EUNSUPPORTED, and so does the batched
+                               solve. */
   double epsilon_decay;     /* in (0, 1]; 0 is read as 1 */
   int32_t momentum_start;   /* 0: constant momentum (the weight above).  > 0: Momentum(start)
                                (acceleration.py:138-160): iterations from momentum_start on use
@@ -293,8 +296,18 @@ size_t ottx_pointcloud_op_noalloc_bytes(const ottx_problem* problem);
  * in the half-step, the products with the vectors on fp32 MFMA.  Backs
  * SinkhornOutput.apply (sinkhorn.py:453-467). */
 size_t ottx_apply_workspace_bytes(const ottx_problem* problem, int32_t axis);
+/* flags: repeated applies of the same transport (iterative solvers) may keep
+ * the workspace between calls and skip the preparation of the clouds
+ * (OTTX_APPLY_PREPARED: a previous call on this workspace prepared this
+ * problem) and the column log-sum-exp of this axis (OTTX_APPLY_LSE_READY: a
+ * previous call on this workspace computed it for the same f, g, eps and
+ * axis).  A workspace for both axes: the larger of the two sizes. */
+enum ottx_apply_flags {
+  OTTX_APPLY_PREPARED = 1,
+  OTTX_APPLY_LSE_READY = 2,
+};
 int ottx_pointcloud_apply(const ottx_problem* problem, int32_t axis, const float* f, const float* g,
-                          const float* vecs, int32_t k, double eps, float* out, void* workspace,
+                          const float* vecs, int32_t k, double eps, int32_t flags, float* out, void* workspace,
                           size_t workspace_bytes, void* stream);
 
 /* Cost maxima behind PointCloud's scale_cost modes (pointcloud.py:342-383):

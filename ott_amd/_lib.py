@@ -24,6 +24,8 @@ OTTX_EUNSUPPORTED = -2
 OTTX_EHIP = -3
 OTTX_EWORKSPACE = -4
 OTTX_ESCRATCH = -5
+APPLY_PREPARED = 1
+APPLY_LSE_READY = 2
 NOALLOC_SCRATCH_BYTES = 32 << 20  # OTTX_NOALLOC_SCRATCH_BYTES
 
 PREC_FP32 = 0
@@ -198,7 +200,7 @@ _SIGNATURES = [
     ("ottx_apply_workspace_bytes", ctypes.c_size_t, [ctypes.POINTER(Problem), ctypes.c_int32]),
     ("ottx_pointcloud_apply", ctypes.c_int,
      [ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
-      ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+      ctypes.c_double, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     ("ottx_cost_maxima", ctypes.c_int,
      [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
       ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),

@@ -1981,10 +1981,8 @@ struct Prepared {
 };
 
 
-// Statistics, centring shift, fragments and norms of both clouds (three
-// launches), plus the loop-state reset of a solve when ``init`` names one.
-int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* ws, hipStream_t st,
-            Prepared* P, const PrepInit* init = nullptr) {
+// The prepared buffers of a workspace (no launch).
+void prepared_view(const ottx_problem* pb, const Layout& L, void* ws, Prepared* P) {
   char* base = static_cast<char*>(ws);
   P->base = base;
   P->L = L;
@@ -1997,6 +1995,14 @@ int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* w
   P->scal = reinterpret_cast<SolveScalars*>(base + L.scal);
   P->a = pb->a ? pb->a : reinterpret_cast<const float*>(base + L.w_x);
   P->b = pb->b ? pb->b : reinterpret_cast<const float*>(base + L.w_y);
+}
+
+// Statistics, centring shift, fragments and norms of both clouds (three
+// launches), plus the loop-state reset of a solve when ``init`` names one.
+int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* ws, hipStream_t st,
+            Prepared* P, const PrepInit* init = nullptr) {
+  prepared_view(pb, L, ws, P);
+  char* base = P->base;
   double* sx = reinterpret_cast<double*>(base + L.stats_x);
   double* sy = reinterpret_cast<double*>(base + L.stats_y);
   // the first g-update's shift probe runs beside the statistics
@@ -3060,8 +3066,8 @@ extern "C" size_t ottx_apply_workspace_bytes(const ottx_problem* pb, int32_t axi
 }
 
 extern "C" int ottx_pointcloud_apply(const ottx_problem* pb, int32_t axis, const float* f, const float* g,
-                                     const float* vecs, int32_t kvec, double eps, float* out, void* workspace,
-                                     size_t workspace_bytes, void* stream) {
+                                     const float* vecs, int32_t kvec, double eps, int32_t flags, float* out,
+                                     void* workspace, size_t workspace_bytes, void* stream) {
   int rc = check_problem(pb);
   if (rc != OTTX_OK) return rc;
   if (!f || !g || !vecs || !out || kvec < 1 || !(eps > 0.0) || (axis != 0 && axis != 1)) return OTTX_EINVAL;
@@ -3075,8 +3081,12 @@ extern "C" int ottx_pointcloud_apply(const ottx_problem* pb, int32_t axis, const
   const KernelCfg& k = PL.k;
   const Layout& L = PL.L;
   Prepared P;
-  rc = prepare(pb, k, L, workspace, st, &P);
-  if (rc != OTTX_OK) return rc;
+  if (flags & OTTX_APPLY_PREPARED) {
+    prepared_view(pb, L, workspace, &P);
+  } else {
+    rc = prepare(pb, k, L, workspace, st, &P);
+    if (rc != OTTX_OK) return rc;
+  }
   char* base = P.base;
   uint32_t* counters = reinterpret_cast<uint32_t*>(base + L.counters);
   uint32_t* pv_counters = reinterpret_cast<uint32_t*>(base + AL.counters);
@@ -3098,8 +3108,10 @@ extern "C" int ottx_pointcloud_apply(const ottx_problem* pb, int32_t axis, const
   a.mode = 0;
   a.lse_shift = nullptr;
   a.lse_store = lse;
-  rc = launch_halfstep(k, a, st);
-  if (rc != OTTX_OK) return rc;
+  if (!(flags & OTTX_APPLY_LSE_READY)) {
+    rc = launch_halfstep(k, a, st);
+    if (rc != OTTX_OK) return rc;
+  }
   // P V, 16 vectors per launch
   const int64_t n_in = g_upd ? pb->n : pb->m, n_out = g_upd ? pb->m : pb->n;
   const int64_t n_in_pad = pad_rows(n_in);

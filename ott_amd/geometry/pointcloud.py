@@ -438,10 +438,15 @@ class PointCloud:
       raise NotImplementedError("apply_transport_from_potentials: unsupported problem shape")
     ws = _lib.workspace(nbytes, self.device)
     _lib.check("ottx_pointcloud_apply", lib.ottx_pointcloud_apply(
-        ctypes.byref(pb), int(axis), f.data_ptr(), g.data_ptr(), vecs.data_ptr(), int(k), float(self.epsilon),
+        ctypes.byref(pb), int(axis), f.data_ptr(), g.data_ptr(), vecs.data_ptr(), int(k), float(self.epsilon), 0,
         out.data_ptr(), ws.data_ptr(), nbytes, _lib.stream_ptr()))
     del keep, ws  # stream-ordered reuse by the caching allocator
     return out[0] if single else out
+
+  def transport_operator(self, f, g) -> "TransportOperator":
+    """P and P^T as repeatable operators at fixed potentials (iterative
+    solvers: the clouds are prepared and each axis' log-sum-exp computed once)."""
+    return TransportOperator(self, f, g)
 
   def low_rank_cost_factors(self) -> Tuple[torch.Tensor, torch.Tensor]:
     """pointcloud.py:298-319 (``to_LRCGeometry`` of the squared-Euclidean cost):
@@ -461,3 +466,46 @@ class PointCloud:
   def __repr__(self) -> str:
     n, m = self.shape
     return (f"PointCloud(n={n}, m={m}, d={self.x.shape[1]}, precision={self.precision!r})")
+
+
+class TransportOperator:
+  """v -> P^T v (axis 0) and v -> P v (axis 1), P = exp((f + g - C) / eps) at
+  the geometry's epsilon, for fixed potentials: the workspace keeps the
+  prepared clouds and each axis' column log-sum-exp between calls
+  (``ottx_pointcloud_apply`` with OTTX_APPLY_PREPARED / OTTX_APPLY_LSE_READY)."""
+
+  def __init__(self, geom: PointCloud, f, g):
+    self.geom = geom
+    self.f = as_device_f32(f, geom.device).contiguous()
+    self.g = as_device_f32(g, geom.device).contiguous()
+    n, m = geom.shape
+    if self.f.shape != (n,) or self.g.shape != (m,):
+      raise ValueError(f"potentials must be [{n}] and [{m}]")
+    lib = _lib.load_library()
+    self._pb, self._keep = geom._problem(epsilon=geom.epsilon)
+    self._nbytes = max(lib.ottx_apply_workspace_bytes(ctypes.byref(self._pb), 0),
+                       lib.ottx_apply_workspace_bytes(ctypes.byref(self._pb), 1))
+    if self._nbytes == 0:
+      raise NotImplementedError("transport operator: unsupported problem shape")
+    self._ws = _lib.workspace(self._nbytes, geom.device)
+    self._prepared = False
+    self._lse_ready = [False, False]
+
+  def __call__(self, vec, axis: int) -> torch.Tensor:
+    lib = _lib.load_library()
+    vec = as_device_f32(vec, self.geom.device)
+    single = vec.ndim == 1
+    vecs = (vec[None] if single else vec).contiguous()
+    n, m = self.geom.shape
+    n_in, n_out = (n, m) if axis == 0 else (m, n)
+    if vecs.ndim != 2 or vecs.shape[1] != n_in:
+      raise ValueError(f"vec must be [{n_in}] or [k, {n_in}] for axis={axis}")
+    out = torch.empty((vecs.shape[0], n_out), dtype=torch.float32, device=self.geom.device)
+    flags = (_lib.APPLY_PREPARED if self._prepared else 0) | (_lib.APPLY_LSE_READY if self._lse_ready[axis] else 0)
+    _lib.check("ottx_pointcloud_apply", lib.ottx_pointcloud_apply(
+        ctypes.byref(self._pb), int(axis), self.f.data_ptr(), self.g.data_ptr(), vecs.data_ptr(),
+        int(vecs.shape[0]), float(self.geom.epsilon), flags, out.data_ptr(), self._ws.data_ptr(), self._nbytes,
+        _lib.stream_ptr()))
+    self._prepared = True
+    self._lse_ready[axis] = True
+    return out[0] if single else out

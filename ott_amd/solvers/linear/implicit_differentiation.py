@@ -1,0 +1,215 @@
+"""Implicit differentiation of the Sinkhorn potentials.
+
+Mirrors ``ott.solvers.linear.implicit_differentiation.ImplicitDiff``
+(solvers/linear/implicit_differentiation.py:31-289) with its default
+preconditioner h = eps log and the default non-symmetric solver, normal-
+equation conjugate gradients (``lineax.NormalCG`` through
+``lineax_implicit.solve_lineax``, rtol = atol = 1e-6, lineax_implicit.py:56-95).
+
+``solve`` applies minus the inverse of the Jacobian of the preconditioned
+first-order conditions to the cotangents (gr_f, gr_g) on the Schur complement
+of the smaller side; every product with the transport runs on the GPU as the
+fused transport apply at fixed potentials (``PointCloud.transport_operator``).
+``gradient`` pulls the result back through the first-order conditions
+(:212-251) to the inputs, in closed form for the squared-Euclidean point
+cloud: with (v_a, v_b) the solve's output, alpha = v_a / marginal_a and
+beta = v_b / marginal_b,
+
+  dC_ij = -P_ij (alpha_i + beta_j),
+  dx_i  = -2 s [(v_a_i + (P beta)_i) x_i - alpha_i (P y)_i - (P (beta y))_i],
+  dy_j  = -2 s [((P^T alpha)_j + v_b_j) y_j - (P^T (alpha x))_j - beta_j (P^T x)_j],
+  da_i  = -eps v_a_i / a_i,   db_j = -eps v_b_j / b_j,
+
+s the cost's inverse scale and eps held fixed (an absolute epsilon; the
+reference also carries gradients through a relative epsilon, which this
+module refuses).  Entries with zero weight are decoupled (their potential is
+-inf): their rows of the system are the identity and their gradients zero,
+where the reference's arithmetic produces NaN.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+from typing import Any, Dict, Optional, Tuple
+
+import torch
+
+from ott_amd.geometry.pointcloud import PointCloud
+
+__all__ = ["ImplicitDiff", "potentials"]
+
+
+def _normal_cg(op, op_t, rhs: torch.Tensor, rtol: float, atol: float, max_iter: int,
+               ridge_kernel: float = 0.0, ridge_identity: float = 0.0) -> torch.Tensor:
+  """x with op(x) = rhs by conjugate gradients on op_t(op(x)) = op_t(rhs)
+  (lineax.NormalCG), the ridge terms of solve_lineax added to both operators;
+  float64 vectors, the operator products in float32 on the GPU."""
+  def reg(fn):
+    if ridge_kernel == 0.0 and ridge_identity == 0.0:
+      return fn
+    return lambda z: fn(z) + ridge_kernel * z.sum() + ridge_identity * z
+  a, at = reg(op), reg(op_t)
+  b = at(rhs)
+  x = torch.zeros_like(rhs)
+  r = b.clone()
+  p = r.clone()
+  rs = torch.dot(r, r)
+  bound = (atol + rtol * torch.linalg.vector_norm(b)) ** 2
+  for it in range(max_iter):
+    if it % 8 == 0 and bool(rs <= bound):  # one host read every few steps
+      break
+    ap = at(a(p))
+    alpha = rs / torch.dot(p, ap)
+    x = x + alpha * p
+    r = r - alpha * ap
+    rs_new = torch.dot(r, r)
+    p = r + (rs_new / rs) * p
+    rs = rs_new
+  return x
+
+
+@dataclasses.dataclass
+class ImplicitDiff:
+  """implicit_differentiation.py:31-80: ``solver_kwargs`` (rtol, atol,
+  max_iter, ridge_kernel, ridge_identity) and ``symmetric``; the default
+  preconditioner only (``precondition_fun`` None)."""
+
+  solver_kwargs: Optional[Dict[str, Any]] = None
+  symmetric: bool = False
+  precondition_fun: Optional[Any] = None
+
+  def _check(self):
+    if self.precondition_fun is not None:
+      raise NotImplementedError("ImplicitDiff supports the default preconditioner (eps log) only")
+
+  def solve(self, gr: Tuple[torch.Tensor, torch.Tensor], geom: PointCloud, f: torch.Tensor, g: torch.Tensor,
+            a: torch.Tensor, b: torch.Tensor, tau_a: float = 1.0, tau_b: float = 1.0,
+            op=None) -> Tuple[torch.Tensor, torch.Tensor]:
+    """implicit_differentiation.py:81-209: returns (-vjp_gr_f, -vjp_gr_g)."""
+    self._check()
+    kw = dict(self.solver_kwargs or {})
+    rtol = float(kw.pop("rtol", 1e-6))
+    atol = float(kw.pop("atol", 1e-6))
+    ridge_kernel = float(kw.pop("ridge_kernel", 0.0))
+    ridge_identity = float(kw.pop("ridge_identity", 0.0))
+    n, m = geom.shape
+    max_iter = int(kw.pop("max_iter", 10 * min(n, m)))
+    if kw:
+      raise TypeError(f"unknown solver_kwargs: {sorted(kw)}")
+    op = geom.transport_operator(f, g) if op is None else op
+    dev = geom.device
+    one_n = torch.ones((n,), dtype=torch.float32, device=dev)
+    one_m = torch.ones((m,), dtype=torch.float32, device=dev)
+    marg_a = op(one_m, axis=1).double()  # P 1 (rows)
+    marg_b = op(one_n, axis=0).double()  # P^T 1 (columns)
+    live_a = (a > 0) & (marg_a > 0)
+    live_b = (b > 0) & (marg_b > 0)
+    inv_ma = torch.where(live_a, 1.0 / marg_a, torch.zeros_like(marg_a))
+    inv_mb = torch.where(live_b, 1.0 / marg_b, torch.zeros_like(marg_b))
+    # diagonal blocks: marginal h'(marginal) / eps = 1, plus eps / rho = (1 - tau) / tau
+    # from the marginal fit when tau < 1 (unbalanced_functions.py:58-86); 1 where decoupled
+    da = torch.where(live_a, torch.full_like(marg_a, 1.0 / tau_a), torch.ones_like(marg_a))
+    db = torch.where(live_b, torch.full_like(marg_b, 1.0 / tau_b), torch.ones_like(marg_b))
+
+    def pv(z, axis):
+      return op(z.float(), axis=axis).double()
+
+    vjp_fg = lambda z: pv(z * inv_mb, 1)       # [m] -> [n]
+    vjp_gf = lambda z: pv(z * inv_ma, 0)       # [n] -> [m]
+    vjp_fgt = lambda z: pv(z, 0) * inv_mb      # [n] -> [m]
+    vjp_gft = lambda z: pv(z, 1) * inv_ma      # [m] -> [n]
+    gr_f = torch.where(live_a, gr[0].double(), torch.zeros_like(marg_a))
+    gr_g = torch.where(live_b, gr[1].double(), torch.zeros_like(marg_b))
+    sym = self.symmetric
+    if n > m:
+      schur = lambda z: db * z - vjp_gf(vjp_fg(z) / da)
+      schur_t = schur if sym else (lambda z: db * z - vjp_fgt(vjp_gft(z) / da))
+      res = gr_g - vjp_gf(gr_f / da)
+      sch = _normal_cg(schur, schur_t, res, rtol, atol, max_iter, ridge_kernel, ridge_identity)
+      vjp_gr_f = (gr_f - vjp_fg(sch)) / da
+      vjp_gr_g = sch
+    else:
+      schur = lambda z: da * z - vjp_fg(vjp_gf(z) / db)
+      schur_t = schur if sym else (lambda z: da * z - vjp_gft(vjp_fgt(z) / db))
+      res = gr_f - vjp_fg(gr_g / db)
+      sch = _normal_cg(schur, schur_t, res, rtol, atol, max_iter, ridge_kernel, ridge_identity)
+      vjp_gr_g = (gr_g - vjp_gf(sch)) / db
+      vjp_gr_f = sch
+    vjp_gr_f = torch.where(live_a, vjp_gr_f, torch.zeros_like(vjp_gr_f))
+    vjp_gr_g = torch.where(live_b, vjp_gr_g, torch.zeros_like(vjp_gr_g))
+    return -vjp_gr_f, -vjp_gr_g
+
+  def gradient(self, gr, geom: PointCloud, f: torch.Tensor, g: torch.Tensor, a: torch.Tensor, b: torch.Tensor,
+               tau_a: float = 1.0, tau_b: float = 1.0):
+    """implicit_differentiation.py:253-269: (dx, dy, da, db) in float64."""
+    op = geom.transport_operator(f, g)
+    v_a, v_b = self.solve(gr, geom, f, g, a, b, tau_a, tau_b, op=op)
+    dev = geom.device
+    n, m = geom.shape
+    x, y = geom.x.double(), geom.y.double()
+    s = float(geom.inv_scale_cost)
+    eps = float(geom.epsilon)
+    one_n = torch.ones((n,), dtype=torch.float32, device=dev)
+    one_m = torch.ones((m,), dtype=torch.float32, device=dev)
+    marg_a = op(one_m, axis=1).double()
+    marg_b = op(one_n, axis=0).double()
+    alpha = torch.where(marg_a > 0, v_a / marg_a, torch.zeros_like(v_a))
+    beta = torch.where(marg_b > 0, v_b / marg_b, torch.zeros_like(v_b))
+    # P applied to [beta, y^T, (beta y)^T] over the columns and P^T to [alpha, x^T, (alpha x)^T] over the rows
+    right = torch.cat([beta[None], y.t(), (beta[:, None] * y).t()]).float()
+    left = torch.cat([alpha[None], x.t(), (alpha[:, None] * x).t()]).float()
+    pr = op(right, axis=1).double()  # [1 + 2d, n]
+    pl = op(left, axis=0).double()   # [1 + 2d, m]
+    d = x.shape[1]
+    p_beta, p_y, p_by = pr[0], pr[1:1 + d].t(), pr[1 + d:].t()
+    pt_alpha, pt_x, pt_ax = pl[0], pl[1:1 + d].t(), pl[1 + d:].t()
+    dx = -2.0 * s * ((v_a + p_beta)[:, None] * x - alpha[:, None] * p_y - p_by)
+    dy = -2.0 * s * ((pt_alpha + v_b)[:, None] * y - pt_ax - beta[:, None] * pt_x)
+    aa, bb = a.double(), b.double()
+    da = torch.where(aa > 0, -eps * v_a / aa, torch.zeros_like(aa))
+    db = torch.where(bb > 0, -eps * v_b / bb, torch.zeros_like(bb))
+    return dx, dy, da, db
+
+
+class _Potentials(torch.autograd.Function):
+
+  @staticmethod
+  def forward(ctx, x, y, a, b, geom_kwargs, solve_kwargs, implicit):
+    from ott_amd.solvers.linear._solve import solve
+    geom = PointCloud(x.detach(), y.detach(), **geom_kwargs)
+    out = solve(geom, a=None if a is None else a.detach(), b=None if b is None else b.detach(), **solve_kwargs)
+    ctx.out = out
+    ctx.implicit = implicit
+    ctx.devices = tuple(None if t is None else t.device for t in (x, y, a, b))
+    ctx.has = (a is not None, b is not None)
+    return out.f.clone(), out.g.clone()
+
+  @staticmethod
+  def backward(ctx, gr_f, gr_g):
+    out = ctx.out
+    dev = out.f.device
+    zf = torch.zeros_like(out.f) if gr_f is None else gr_f.to(dev)
+    zg = torch.zeros_like(out.g) if gr_g is None else gr_g.to(dev)
+    prob = out.ot_prob
+    dx, dy, da, db = ctx.implicit.gradient((zf, zg), out.geom, out.f, out.g, prob.a, prob.b,
+                                           prob.tau_a, prob.tau_b)
+    need = ctx.needs_input_grad
+    grads = [dx if need[0] else None, dy if need[1] else None,
+             da if (need[2] and ctx.has[0]) else None, db if (need[3] and ctx.has[1]) else None]
+    grads = [None if gr is None else gr.float().to(dv) for gr, dv in zip(grads, ctx.devices)]
+    return (*grads, None, None, None)
+
+
+def potentials(x: torch.Tensor, y: torch.Tensor, a: Optional[torch.Tensor] = None,
+               b: Optional[torch.Tensor] = None, *, epsilon: float, scale_cost: float = 1.0,
+               precision: str = "fp32", implicit_diff: Optional[ImplicitDiff] = None,
+               **solve_kwargs: Any) -> Tuple[torch.Tensor, torch.Tensor]:
+  """``out = linear.solve(PointCloud(x, y, epsilon, scale_cost), a, b,
+  **solve_kwargs); (out.f, out.g)`` as tensors whose backward is the implicit
+  VJP (sinkhorn.py:1102-1136, ``_iterations_implicit``) with respect to x, y,
+  a and b.  An absolute epsilon and a float scale_cost."""
+  if not isinstance(epsilon, (int, float)) or not isinstance(scale_cost, (int, float)):
+    raise NotImplementedError("implicit gradients need an absolute epsilon and a float scale_cost")
+  implicit = ImplicitDiff() if implicit_diff is None else implicit_diff
+  geom_kwargs = dict(epsilon=float(epsilon), scale_cost=float(scale_cost), precision=precision)
+  return _Potentials.apply(x, y, a, b, geom_kwargs, solve_kwargs, implicit)
