@@ -3249,43 +3249,62 @@ struct BatchItem {
   int32_t pad_;
 };
 
-// One workgroup per problem: float64 feature sums of x and y, the centring
-// shift, mean(C) over (n_div, m_div) rows and the solve scalars
+// Statistics of every problem in two stages.  Stage 1, grid (chunks, K, 2):
+// float64 per-feature sums of x and x^2 over 256-row chunks of one cloud
+// (chunk_sums' fixed order), so that the largest problem is spread over many
+// workgroups.  part[((k * 2 + side) * chunks + c) * d + feature][2].
+constexpr int kBatchStatRows = 256;
+
+__global__ __launch_bounds__(256) void batch_stats_partial_kernel(const BatchItem* __restrict__ items, int d,
+                                                                  int chunks, double* __restrict__ part) {
+  __shared__ double s1[256], s2[256];
+  const BatchItem& it = items[blockIdx.y];
+  const int side = blockIdx.z, c = blockIdx.x, t = threadIdx.x;
+  const int64_t np = side ? it.m : it.n;
+  const int64_t r0 = static_cast<int64_t>(c) * kBatchStatRows;
+  if (r0 >= np) return;  // the whole workgroup
+  const int64_t r1 = min(np, r0 + kBatchStatRows);
+  const int G = max(1, 256 / d);
+  double a1 = 0.0, a2 = 0.0;
+  if (t < G * d) chunk_sums(side ? it.y : it.x, r0, r1, d, G, t, a1, a2);
+  s1[t] = a1;
+  s2[t] = a2;
+  __syncthreads();
+  if (t < d) {
+    double b1 = 0.0, b2 = 0.0;
+    for (int g = 0; g < G; ++g) {
+      b1 += s1[g * d + t];
+      b2 += s2[g * d + t];
+    }
+    double* q = part + ((static_cast<int64_t>(blockIdx.y) * 2 + side) * chunks + c) * d * 2 + t * 2;
+    q[0] = b1;
+    q[1] = b2;
+  }
+}
+
+// Stage 2, one workgroup per problem: the chunk partials summed in order, the
+// centring shift, mean(C) over (n_div, m_div) rows and the solve scalars
 // (pointcloud.py:170-186, geometry.py:147-171).
-__global__ void batch_stats_kernel(const BatchItem* __restrict__ items, int d, double s, double rel, double tau_a,
-                                   double tau_b) {
-  __shared__ double red[4][256];
+__global__ __launch_bounds__(256) void batch_stats_final_kernel(const BatchItem* __restrict__ items, int d,
+                                                                int chunks, const double* __restrict__ part,
+                                                                double s, double rel, double tau_a, double tau_b) {
   __shared__ double acc3[3][256];
   const BatchItem& it = items[blockIdx.x];
   const int t = threadIdx.x;
-  const int G = max(1, 256 / d);
-  double sx1 = 0.0, sx2 = 0.0, sy1 = 0.0, sy2 = 0.0;
-  if (t < G * d) {
-    const int k = t % d, g = t / d;
-    for (int64_t r = g; r < it.n; r += G) {
-      const double v = it.x[r * d + k];
-      sx1 += v;
-      sx2 += v * v;
-    }
-    for (int64_t r = g; r < it.m; r += G) {
-      const double v = it.y[r * d + k];
-      sy1 += v;
-      sy2 += v * v;
-    }
-  }
-  red[0][t] = sx1;
-  red[1][t] = sx2;
-  red[2][t] = sy1;
-  red[3][t] = sy2;
-  __syncthreads();
   double nx2 = 0.0, ny2 = 0.0, dot = 0.0;
   if (t < d) {
     double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
-    for (int g = 0; g < G; ++g) {
-      a1 += red[0][g * d + t];
-      a2 += red[1][g * d + t];
-      b1 += red[2][g * d + t];
-      b2 += red[3][g * d + t];
+    const int cx = static_cast<int>((it.n + kBatchStatRows - 1) / kBatchStatRows);
+    const int cy = static_cast<int>((it.m + kBatchStatRows - 1) / kBatchStatRows);
+    const double* px = part + (static_cast<int64_t>(blockIdx.x) * 2 + 0) * chunks * d * 2;
+    const double* py = part + (static_cast<int64_t>(blockIdx.x) * 2 + 1) * chunks * d * 2;
+    for (int c = 0; c < cx; ++c) {
+      a1 += px[(c * d + t) * 2 + 0];
+      a2 += px[(c * d + t) * 2 + 1];
+    }
+    for (int c = 0; c < cy; ++c) {
+      b1 += py[(c * d + t) * 2 + 0];
+      b2 += py[(c * d + t) * 2 + 1];
     }
     const double shift = 0.5 * (a1 / static_cast<double>(it.n) + b1 / static_cast<double>(it.m));
     it.mu[t] = isfinite(shift) ? static_cast<float>(shift) : 0.f;  // as stats_final_kernel
@@ -3314,48 +3333,83 @@ __global__ void batch_stats_kernel(const BatchItem* __restrict__ items, int d, d
   }
 }
 
-// Fragments of both clouds of every problem: grid (x blocks, K, side).
-__global__ void batch_pack_kernel(const BatchItem* __restrict__ items, int d, int chunks, int prec) {
-  const BatchItem& it = items[blockIdx.y];
-  const int side = blockIdx.z;
+// Fragments and norms of every problem in one launch over a work table of
+// (problem, side, block, kind): kind 0 packs 256 fragment threads, kind 1
+// computes 256 norms (only the blocks a problem needs are launched).
+__global__ __launch_bounds__(256) void batch_prep_kernel(const BatchItem* __restrict__ items,
+                                                         const int4* __restrict__ table, int d, int chunks,
+                                                         int prec, double s) {
+  __shared__ float s_mu[256];
+  const int4 w = table[blockIdx.x];
+  const BatchItem& it = items[w.x];
+  const int side = w.y;
   const int64_t np = side ? it.m : it.n;
   const int64_t np_pad = side ? it.m_pad : it.n_pad;
-  const int64_t n_threads = (np_pad / 16) * chunks * 64;
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= n_threads) return;
   const float* pts = side ? it.y : it.x;
-  void* frag = side ? it.frag_y : it.frag_x;
-  const float scale = side ? it.scal->c : 1.f;
-  if (prec == 1) {
-    pack_bf16_one(pts, np, d, it.mu, scale, chunks, gid, static_cast<uint4*>(frag));
-  } else {
-    pack_f32_one(pts, np, d, it.mu, scale, chunks, gid, static_cast<float4*>(frag));
+  if (w.w == 0) {
+    const int64_t gid = static_cast<int64_t>(w.z) * 256 + threadIdx.x;
+    if (gid >= (np_pad / 16) * chunks * 64) return;
+    void* frag = side ? it.frag_y : it.frag_x;
+    const float scale = side ? it.scal->c : 1.f;
+    if (prec == 1)
+      pack_bf16_one(pts, np, d, it.mu, scale, chunks, gid, static_cast<uint4*>(frag));
+    else
+      pack_f32_one(pts, np, d, it.mu, scale, chunks, gid, static_cast<float4*>(frag));
+    return;
   }
-}
-
-// s |p - mu|^2 of every point, zero on padded rows: grid (x blocks, K, side).
-__global__ void batch_norms_kernel(const BatchItem* __restrict__ items, int d, double s) {
-  const BatchItem& it = items[blockIdx.y];
-  const int side = blockIdx.z;
-  const int64_t np = side ? it.m : it.n;
-  const int64_t np_pad = side ? it.m_pad : it.n_pad;
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (int k = threadIdx.x; k < d; k += 256) s_mu[k] = it.mu[k];
+  __syncthreads();
+  const int64_t i = static_cast<int64_t>(w.z) * 256 + threadIdx.x;
   if (i >= np_pad) return;
   float* norm = side ? it.norm_y : it.norm_x;
   if (i >= np) {
     norm[i] = 0.f;
     return;
   }
-  const float* pts = side ? it.y : it.x;
+  const float* row = pts + i * d;
   double acc = 0.0;
-  for (int k = 0; k < d; ++k) {
-    const double v = static_cast<double>(pts[i * d + k] - it.mu[k]);
-    acc += v * v;
+  if ((d & 3) == 0 && (reinterpret_cast<uintptr_t>(row) & 15) == 0) {
+    for (int k = 0; k < d; k += 16) {
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        v[u] = (k + 4 * u < d) ? *reinterpret_cast<const float4*>(row + k + 4 * u) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k + 4 * u >= d) continue;
+        const double e0 = v[u].x - s_mu[k + 4 * u], e1 = v[u].y - s_mu[k + 4 * u + 1];
+        const double e2 = v[u].z - s_mu[k + 4 * u + 2], e3 = v[u].w - s_mu[k + 4 * u + 3];
+        acc += (e0 * e0 + e1 * e1) + (e2 * e2 + e3 * e3);
+      }
+    }
+  } else {
+    for (int k = 0; k < d; ++k) {
+      const double v = static_cast<double>(row[k] - s_mu[k]);
+      acc += v * v;
+    }
   }
   norm[i] = static_cast<float>(s * acc);
 }
 
-// Loop state of every problem (sinkhorn.py:964-972) and the batch counter.
+// Progress of a batched solve for the host, written into pinned memory after
+// every block: the number of finished problems, then (seq << 32) | blocks.
+struct BatchMail {
+  uint64_t progress;
+  uint32_t done;
+  uint32_t pad;
+};
+
+__global__ void batch_progress_kernel(uint32_t* counters, BatchMail* mail, uint32_t seq) {
+  if (threadIdx.x != 0) return;
+  const uint32_t dn = __hip_atomic_load(&counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t blocks = counters[1] + 1u;
+  counters[1] = blocks;
+  __hip_atomic_store(&mail->done, dn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&mail->progress, (static_cast<uint64_t>(seq) << 32) | blocks, __ATOMIC_RELEASE,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Loop state of every problem (sinkhorn.py:964-972) and the batch counters.
 __global__ void batch_init_kernel(const BatchItem* __restrict__ items, int outer, uint32_t* batch_done,
                                   int zero_init) {
   const BatchItem& it = items[blockIdx.x];
@@ -3372,7 +3426,10 @@ __global__ void batch_init_kernel(const BatchItem* __restrict__ items, int outer
     st->fin_count = 0u;
     st->total_mass = 0.0;
     st->last_err = 0.0;
-    if (blockIdx.x == 0) *batch_done = 0u;
+    if (blockIdx.x == 0) {
+      batch_done[0] = 0u;  // finished problems
+      batch_done[1] = 0u;  // completed blocks
+    }
   }
   for (int k = t; k < outer; k += blockDim.x) it.errors[k] = -1.f;
   for (int k = t; k < it.n_counters; k += blockDim.x) it.counters[k] = 0u;
@@ -3476,7 +3533,9 @@ struct BatchPlan {
   std::vector<Layout> L;
   std::vector<size_t> base;  // offset of each problem's layout
   size_t off_done = 0, off_items = 0, off_args_g = 0, off_args_f = 0, off_work_g = 0, off_work_f = 0,
-         off_out = 0, total = 0;
+         off_out = 0, off_stats = 0, off_prep = 0, total = 0;
+  int stat_chunks = 0;
+  int64_t n_prep = 0;  // (problem, side, block, kind) items of the fragment / norm preparation
   int64_t n_work_g = 0, n_work_f = 0;
   int64_t max_np = 0;
 };
@@ -3607,7 +3666,17 @@ int make_batch_plan(const ottx_batch* bt, const ottx_sinkhorn_opts* o, BatchPlan
   if (B->n_work_g > (int64_t{1} << 31) - 1 || B->n_work_f > (int64_t{1} << 31) - 1) return OTTX_EINVAL;
   B->off_work_g = take(sizeof(int4) * B->n_work_g);
   B->off_work_f = take(sizeof(int4) * B->n_work_f);
+  B->n_prep = 0;
+  for (int i = 0; i < B->K; ++i)
+    for (int side = 0; side < 2; ++side) {
+      const int64_t np_pad = pad_rows(side ? B->pbs[i].m : B->pbs[i].n, B->pad);
+      B->n_prep += (np_pad / 16 * k.frag_chunks() * 64 + 255) / 256 + (np_pad + 255) / 256;
+    }
+  if (B->n_prep > (int64_t{1} << 31) - 1) return OTTX_EINVAL;
+  B->off_prep = take(sizeof(int4) * B->n_prep);
   B->off_out = take(sizeof(double) * 16 * B->K);
+  B->stat_chunks = static_cast<int>((B->max_np + kBatchStatRows - 1) / kBatchStatRows);
+  B->off_stats = take(sizeof(double) * 2 * static_cast<size_t>(B->K) * 2 * B->stat_chunks * bt->d);
   for (int i = 0; i < B->K; ++i) {
     B->L[i] = make_layout(&B->pbs[i], k, B->gp[i], B->fp[i], 1, B->pad);
     B->base[i] = take(B->L[i].total);
@@ -3705,7 +3774,8 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
   HalfStepArgs* args_f = reinterpret_cast<HalfStepArgs*>(sb + (B.off_args_f - B.off_items));
   int4* work_g = reinterpret_cast<int4*>(sb + (B.off_work_g - B.off_items));
   int4* work_f = reinterpret_cast<int4*>(sb + (B.off_work_f - B.off_items));
-  int64_t wg = 0, wf = 0;
+  int4* prep = reinterpret_cast<int4*>(sb + (B.off_prep - B.off_items));
+  int64_t wg = 0, wf = 0, wp = 0;
   for (int i = 0; i < K; ++i) {
     const ottx_problem& pb = B.pbs[i];
     const Layout& L = B.L[i];
@@ -3764,6 +3834,13 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
       for (int sp = 0; sp < B.gp[i].splits; ++sp) work_g[wg++] = make_int4(i, t, sp, 0);
     for (int t = 0; t < B.fp[i].tiles; ++t)
       for (int sp = 0; sp < B.fp[i].splits; ++sp) work_f[wf++] = make_int4(i, t, sp, 0);
+    for (int side = 0; side < 2; ++side) {
+      const int64_t np_pad = side ? it.m_pad : it.n_pad;
+      const int pack_blocks = static_cast<int>((np_pad / 16 * k.frag_chunks() * 64 + 255) / 256);
+      const int norm_blocks = static_cast<int>((np_pad + 255) / 256);
+      for (int q = 0; q < pack_blocks; ++q) prep[wp++] = make_int4(i, side, q, 0);
+      for (int q = 0; q < norm_blocks; ++q) prep[wp++] = make_int4(i, side, q, 1);
+    }
   }
   BatchItem* d_items = reinterpret_cast<BatchItem*>(ws + B.off_items);
   HalfStepArgs* d_args_g = reinterpret_cast<HalfStepArgs*>(ws + B.off_args_g);
@@ -3774,15 +3851,13 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
 
   // Preparation of every problem.
   const double s = bt->inv_scale_cost;
-  hipLaunchKernelGGL(batch_stats_kernel, dim3(K), dim3(256), 0, st, d_items, bt->d, s, bt->relative_epsilon,
-                     bt->tau_a, bt->tau_b);
-  const int64_t max_pad = pad_rows(B.max_np, B.pad);
-  const int chunks = k.frag_chunks();
-  const int64_t pack_threads = (max_pad / 16) * chunks * 64;
-  hipLaunchKernelGGL(batch_pack_kernel, dim3(static_cast<unsigned>((pack_threads + 255) / 256), K, 2), dim3(256),
-                     0, st, d_items, bt->d, chunks, k.prec);
-  hipLaunchKernelGGL(batch_norms_kernel, dim3(static_cast<unsigned>((max_pad + 255) / 256), K, 2), dim3(256), 0,
-                     st, d_items, bt->d, s);
+  double* d_stats = reinterpret_cast<double*>(ws + B.off_stats);
+  hipLaunchKernelGGL(batch_stats_partial_kernel, dim3(B.stat_chunks, K, 2), dim3(256), 0, st, d_items, bt->d,
+                     B.stat_chunks, d_stats);
+  hipLaunchKernelGGL(batch_stats_final_kernel, dim3(K), dim3(256), 0, st, d_items, bt->d, B.stat_chunks,
+                     static_cast<const double*>(d_stats), s, bt->relative_epsilon, bt->tau_a, bt->tau_b);
+  hipLaunchKernelGGL(batch_prep_kernel, dim3(static_cast<unsigned>(B.n_prep)), dim3(256), 0, st, d_items,
+                     reinterpret_cast<const int4*>(ws + B.off_prep), bt->d, k.frag_chunks(), k.prec, s);
   hipLaunchKernelGGL(batch_init_kernel, dim3(K), dim3(256), 0, st, d_items, outer, batch_done,
                      zero_init_bits(o->init_mode));
   OTTX_LAUNCHED();
@@ -3798,6 +3873,9 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
     OTTX_LAUNCHED();
     return OTTX_OK;
   };
+  BatchMail* bmail = reinterpret_cast<BatchMail*>(static_cast<char*>(hh->mailbox) + 2048);
+  const uint32_t seq = ++hh->seq;
+  __atomic_store_n(&bmail->progress, static_cast<uint64_t>(0), __ATOMIC_RELEASE);
   auto launch_block = [&](hipStream_t s2) -> int {
     for (int itn = 0; itn < inner; ++itn) {
       int r = launch(false, itn == inner - 1 ? f_err : f_plain, s2);
@@ -3805,6 +3883,8 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
       r = launch(true, itn == inner - 1 ? g_err : g_plain, s2);
       if (r != OTTX_OK) return r;
     }
+    hipLaunchKernelGGL(batch_progress_kernel, dim3(1), dim3(64), 0, s2, batch_done, bmail, seq);
+    OTTX_LAUNCHED();
     return OTTX_OK;
   };
 
@@ -3830,12 +3910,26 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
       return OTTX_EHIP;
     }
   }
-  hipEvent_t ev[2];
-  OTTX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  OTTX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  uint32_t* mail = static_cast<uint32_t*>(hh->mailbox);
   int status = OTTX_OK;
   int launched = 0;
+  // One block runs ahead of the host, which spins on the pinned mailbox that
+  // every block's progress kernel writes (no event in the stream, no blocking
+  // wait); the stream is queried now and then so that an error or a drained
+  // stream ends the wait.  Returns false on an error.
+  auto wait_block = [&](uint32_t blocks) -> bool {
+    for (uint32_t spin = 1;; ++spin) {
+      const uint64_t v = __atomic_load_n(&bmail->progress, __ATOMIC_ACQUIRE);
+      if ((v >> 32) == seq && static_cast<uint32_t>(v) >= blocks) return true;
+      if ((spin & 1023u) == 0) {
+        const hipError_t q = hipStreamQuery(st);
+        if (q == hipSuccess) {
+          const uint64_t w = __atomic_load_n(&bmail->progress, __ATOMIC_ACQUIRE);
+          return (w >> 32) == seq && static_cast<uint32_t>(w) >= blocks;
+        }
+        if (q != hipErrorNotReady) return false;
+      }
+    }
+  };
   for (int blk = 0; blk < total_blocks; ++blk) {
     if (exec) {
       if (hipGraphLaunch(exec, st) != hipSuccess) { status = OTTX_EHIP; break; }
@@ -3844,19 +3938,11 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
       if (status != OTTX_OK) break;
     }
     ++launched;
-    if (hipEventRecord(ev[blk & 1], st) != hipSuccess) { status = OTTX_EHIP; break; }
     if (blk >= 1) {
-      if (hipEventSynchronize(ev[(blk - 1) & 1]) != hipSuccess) { status = OTTX_EHIP; break; }
-      if (hipMemcpyAsync(mail, batch_done, sizeof(uint32_t), hipMemcpyDeviceToHost, hh->side) != hipSuccess ||
-          hipStreamSynchronize(hh->side) != hipSuccess) {
-        status = OTTX_EHIP;
-        break;
-      }
-      if (mail[0] >= static_cast<uint32_t>(K)) break;
+      if (!wait_block(static_cast<uint32_t>(blk))) { status = OTTX_EHIP; break; }
+      if (__atomic_load_n(&bmail->done, __ATOMIC_ACQUIRE) >= static_cast<uint32_t>(K)) break;
     }
   }
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   if (cap) (void)hipStreamDestroy(cap);
