@@ -3892,7 +3892,9 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t cap = nullptr;
-  if (o->use_graph) {
+  // As for one problem, a graph pays for its capture and instantiation only
+  // when a block has several launches.
+  if (o->use_graph && inner >= 4) {
     OTTX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     OTTX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
     rc = launch_block(cap);
