@@ -303,6 +303,9 @@ def learned_gmm_d16(count=256, seed0=100000):
   return res
 
 
+PEAK_EXP_PER_S = 1024 * 2.4e9 * 16 / 4  # 1024 SIMDs, 16 lanes a cycle, transcendentals at quarter rate
+
+
 def halfstep_roofline(pb, precision, reps=20):
   """Mean duration of the fused g-update half-step, from hipEvents on the launch stream."""
   import torch
@@ -341,8 +344,68 @@ def halfstep_roofline(pb, precision, reps=20):
       "avg_launch_ms": round(ms, 5), "flops_per_launch": flops,
       "pairs_per_s": pairs / (ms * 1e-3),
       "exp_per_s": pairs / (ms * 1e-3),
+      "exp_frac": round(pairs / (ms * 1e-3) / PEAK_EXP_PER_S, 4),
       "algorithmic_bytes_per_launch": hbm,
+      "hbm_gbs": round(hbm / (ms * 1e-3) / 1e9, 2),
+      "hbm_frac": round(hbm / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 6),
   }
+
+
+def config_line(name, steps, warmup):
+  """Time to 1e-3 and the half-step roofline of another configuration: the
+  small uniform d = 2 problem (fp32; its roofline is memory and latency, so
+  HBM GB/s and the exp fraction are the figures) or the uneven unbalanced
+  d = 128 mixture (bf16 split, tau_a = tau_b = 0.95, eps = 0.02 mean C)."""
+  from ott_amd import datasets
+  if name == "uniform_d2":
+    pb, max_it, prec = datasets.uniform_shifted(1024, 1024, 2, seed=0), 2000, "fp32"
+    gold = golden("uniform_d2")
+  else:
+    pb, max_it, prec = datasets.mixture_pair(), 5000, "bf16_split"
+    path = os.path.join(ROOT, "tests", "golden", "mixture_pair_n32768_m8192_d128.json")
+    gold = json.load(open(path)) if os.path.exists(path) else None
+  t, out = time_solves(pb, max_it, prec, None, steps, warmup)
+  roof = halfstep_roofline(pb, prec)
+  res = {"workload": pb.name, "precision": prec, "n": int(pb.x.shape[0]), "m": int(pb.y.shape[0]),
+         "d": int(pb.x.shape[1]), "tau_a": pb.tau_a, "tau_b": pb.tau_b,
+         "ms_median": round(float(np.median(t)), 3), "ms_min": round(min(t), 3), "steps": steps,
+         "warmup": warmup, "n_iters": out.n_iters, "converged": out.converged, "reg_ot_cost": out.reg_ot_cost,
+         "roofline": roof}
+  if gold is not None:
+    res["rel_err_vs_fp64_oracle"] = abs(out.reg_ot_cost - gold["reg_ot_cost"]) / abs(gold["reg_ot_cost"])
+    res["oracle_n_iters"] = gold["n_iters"]
+  return res
+
+
+def cpu_fp32_restatement(pb, max_iterations, budget_s=30.0):
+  """OTT's float32 operation order on the CPU (the oracle's float32 twin: dense
+  expanded cost, (f + g - C) / eps, max-shifted log-sum-exp), run to 1e-3 or
+  until the budget is spent, single-threaded numpy."""
+  from oracle import sinkhorn64 as oracle
+  x = pb.x.astype(np.float32)
+  y = pb.y.astype(np.float32)
+  n, m = x.shape[0], y.shape[0]
+  a = np.full(n, 1.0 / n, np.float32) if pb.a is None else pb.a.astype(np.float32)
+  b = np.full(m, 1.0 / m, np.float32) if pb.b is None else pb.b.astype(np.float32)
+  eps = oracle.resolve_epsilon(pb.x, pb.y, pb.eps_mult, "mean")
+  t0 = time.perf_counter()
+  cost = oracle.sqeuclidean_cost(x, y, expanded=True, dtype=np.float32)
+  ops = oracle.DenseLSE(cost)
+  done = {"iters": 0}
+
+  def progress(status):
+    done["iters"] = status[0] + 1
+    if time.perf_counter() - t0 > budget_s:
+      raise TimeoutError
+
+  try:
+    out = oracle.sinkhorn(ops, a, b, eps, tau_a=pb.tau_a, tau_b=pb.tau_b, threshold=1e-3, inner_iterations=1,
+                          max_iterations=max_iterations, progress_fn=progress, dtype=np.float32)
+    finished, iters = True, out.n_iters
+  except TimeoutError:
+    finished, iters = False, done["iters"]
+  ms = (time.perf_counter() - t0) * 1e3
+  return {"ms": round(ms, 1), "finished": finished, "iterations": iters, "threads": 1}
 
 
 def copy_calibration(nbytes=1 << 28, reps=5):
@@ -423,6 +486,8 @@ def main() -> int:
                   help="skip the d=16 warm-start evaluation on 256 held-out problems")
   ap.add_argument("--no-divergence", action="store_true", help="skip the Sinkhorn divergence line")
   ap.add_argument("--no-transport-ops", action="store_true", help="skip the output-side transport ops line")
+  ap.add_argument("--no-configs", action="store_true",
+                  help="skip the uniform d=2 and unbalanced d=128 configuration lines")
   ap.add_argument("--batched-only", action="store_true", help="only the 512-problem batched solve")
   ap.add_argument("--halfstep-only", action="store_true",
                   help="only launch the half-step kernels and a calibration copy (for profiler passes)")
@@ -463,10 +528,10 @@ def main() -> int:
   for name, prec, ini, inner in (("bf16_split_zero_init", "bf16_split", None, 1),
                                  ("A_inner10", "bf16_split", init, 10),
                                  ("B_inner10", "fp32", None, 10)):
-    t, o = time_solves(pb, max_it, prec, ini, args.steps, 1, inner_iterations=inner)
-    extra[name] = {"ms": round(sum(t) / len(t), 3), "ms_min": round(min(t), 3),
-                   "n_iters": o.n_iters, "inner_iterations": inner, "converged": o.converged,
-                   "reg_ot_cost": o.reg_ot_cost}
+    t, o = time_solves(pb, max_it, prec, ini, args.steps, max(3, args.warmup), inner_iterations=inner)
+    extra[name] = {"ms": round(sum(t) / len(t), 3), "ms_median": round(float(np.median(t)), 3),
+                   "ms_min": round(min(t), 3), "n_iters": o.n_iters, "inner_iterations": inner,
+                   "converged": o.converged, "reg_ot_cost": o.reg_ot_cost}
     log(f"{name}: {extra[name]}")
   roof = halfstep_roofline(pb, "bf16_split")
   roof_b = halfstep_roofline(pb, "fp32")
@@ -485,6 +550,12 @@ def main() -> int:
   if not args.no_transport_ops:
     ops = transport_ops_f2(pb, 3)
     log(f"transport ops: {ops}")
+
+  configs = None
+  if not args.no_configs:
+    configs = {name: config_line(name, max(5, args.steps // 2), max(3, args.warmup))
+               for name in ("uniform_d2", "mixture_d128_unbalanced")}
+    log(f"other configurations: {configs}")
 
   learned16 = None
   if not args.no_learned_eval:
@@ -505,8 +576,21 @@ def main() -> int:
   cpu = None
   if not args.no_cpu:
     log("timing the CPU oracle sample ...")
-    cpu = cpu_baseline(pb, out_b.n_iters, args.cpu_budget)
-    log(f"cpu per iteration {cpu['per_iteration_ms']} ms on {cpu['cores']} threads")
+    f64 = cpu_baseline(pb, out_b.n_iters, args.cpu_budget)
+    log(f"cpu fp64 per iteration {f64['per_iteration_ms']} ms on {f64['cores']} threads")
+    f32 = cpu_fp32_restatement(pb, max_it, budget_s=max(30.0, args.cpu_budget))
+    log(f"cpu fp32 restatement: {f32}")
+    from ott_amd import datasets
+    small = cpu_fp32_restatement(datasets.uniform_shifted(1024, 1024, 2, seed=0), 2000, budget_s=30.0)
+    log(f"cpu fp32 restatement, uniform d=2: {small}")
+    if f32["finished"]:
+      cpu = {"value": f32["ms"], "unit": "ms", "cores": 1, "kind": "port",
+             "sample": (f"the whole solve to 1e-3 ({f32['iterations']} iterations) in OTT's float32 operation "
+                        f"order on the host (oracle/sinkhorn64.py float32 twin: dense expanded cost, "
+                        f"max-shifted log-sum-exp; single-threaded numpy)"),
+             "fp64_oracle": f64, "uniform_d2_fp32_restatement": small}
+    else:
+      cpu = dict(f64, fp32_restatement=f32, uniform_d2_fp32_restatement=small)
 
   line = {
       "metric": METRIC, "value": round(ms_a, 3), "unit": "ms", "n_gpus": 1,
@@ -537,6 +621,7 @@ def main() -> int:
       "roofline": roof,
       "roofline_fp32_variant": roof_b,
       "batched_512_problems": batched,
+      "other_configurations": configs,
       "learned_warm_start_gmm_d16": learned16,
       "sinkhorn_divergence": divergence,
       "transport_ops": ops,

@@ -2819,9 +2819,12 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     hipStream_t cap = nullptr;
-    // A graph pays for itself only when a block has several launches: with
-    // inner_iterations = 1 (3 launches) direct launches return sooner.
-    if (o->use_graph && inner >= 4) {
+    // A graph pays for its capture and instantiation (about a millisecond)
+    // only when a block has several launches that are short against the
+    // host's launch cost: with inner_iterations = 1 (3 launches), or with
+    // half-steps of more than 2^24 pairs (the device then runs behind the
+    // host anyway), direct launches return sooner.
+    if (o->use_graph && inner >= 4 && static_cast<double>(pb->n) * static_cast<double>(pb->m) <= 16777216.0) {
       OTTX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
       OTTX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
       rc = launch_all(cap);
