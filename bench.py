@@ -51,6 +51,8 @@ def workload(name: str):
     return datasets.gaussian_shifted(16384, 16384, 64, seed=1), 5000
   if name == "uniform_d2":
     return datasets.uniform_shifted(1024, 1024, 2, seed=0), 2000
+  if name == "mixture_d128":
+    return datasets.mixture_pair(), 5000
   raise SystemExit(f"unknown workload {name}")
 
 
@@ -335,7 +337,7 @@ def halfstep_roofline(pb, precision, reps=20):
   hbm = (n + m) * d * 4 + 3 * (n + m) * 4
   peak = PEAK_BF16_TFLOPS if precision == "bf16_split" else PEAK_F32_MFMA_TFLOPS
   achieved = flops / (ms * 1e-3) / 1e12
-  traffic = load_traffic(precision, tile)
+  traffic = load_traffic(pb.name, precision, tile)
   return {
       "kernel": f"halfstep_kernel ({precision}, g-update, tile_config {tile})",
       "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
@@ -423,14 +425,16 @@ def copy_calibration(nbytes=1 << 28, reps=5):
   return {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}
 
 
-def load_traffic(precision, tile):
-  """Per-launch HBM bytes of the half-step measured with rocprofv3 --pmc, if recorded."""
+def load_traffic(workload_name, precision, tile):
+  """Per-launch HBM bytes of the half-step on one workload, measured with
+  rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE, scaled by a copy's readings), if
+  recorded for the same tile configuration."""
   path = os.path.join(ROOT, "profiles", "halfstep_traffic.json")
   if not os.path.exists(path):
     return None
   with open(path) as fh:
     rec = json.load(fh)
-  ent = rec.get(precision)
+  ent = rec.get("workloads", {}).get(workload_name, {}).get(precision)
   if not ent or ent.get("tile_config") != tile:
     return None
   return ent
@@ -478,7 +482,8 @@ def main() -> int:
   ap.add_argument("--gpus", type=int, default=1)
   ap.add_argument("--steps", type=int, default=20)
   ap.add_argument("--warmup", type=int, default=5)
-  ap.add_argument("--workload", default="gaussian_d64", choices=["gaussian_d64", "uniform_d2"])
+  ap.add_argument("--workload", default="gaussian_d64", choices=["gaussian_d64", "uniform_d2", "mixture_d128"],
+                  help="mixture_d128 (n=32768, m=8192, unbalanced) is for --halfstep-only profiler passes")
   ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
   ap.add_argument("--cpu-budget", type=float, default=15.0)
   ap.add_argument("--no-batched", action="store_true", help="skip the 512-problem batched solve")
@@ -505,6 +510,10 @@ def main() -> int:
   _lib.load_library()
   from ott_amd.initializers.neural import learned
   pb, max_it = workload(args.workload)
+  if args.workload == "mixture_d128" and not args.halfstep_only:
+    print("--workload mixture_d128 runs with --halfstep-only; its timing line is part of the default run",
+          file=sys.stderr)
+    return 2
   log(f"workload {pb.name}: n={pb.x.shape[0]} m={pb.y.shape[0]} d={pb.x.shape[1]}")
   if args.batched_only:
     print(json.dumps({p: batched_cfg4(p, args.steps, args.warmup, sequential=False)
