@@ -305,7 +305,9 @@ def learned_gmm_d16(count=256, seed0=100000):
   return res
 
 
-PEAK_EXP_PER_S = 1024 * 2.4e9 * 16 / 4  # 1024 SIMDs, 16 lanes a cycle, transcendentals at quarter rate
+# 1024 SIMDs x 16 lanes x 2.4 GHz, v_exp_f32 at half rate on gfx950 (a probe
+# kernel of independent v_exp_f32 measured 17.9 T/s)
+PEAK_EXP_PER_S = 1024 * 2.4e9 * 16 / 2
 
 
 def halfstep_roofline(pb, precision, reps=20):

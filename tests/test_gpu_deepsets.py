@@ -165,6 +165,19 @@ def test_adam_step_and_save_load(tmp_path):
   v = 0.001 * g * g / (1 - 0.999)
   want = p0.double() - 1e-3 * m / (torch.sqrt(v) + 1e-8)
   torch.testing.assert_close(model.params.double(), want, rtol=1e-5, atol=1e-7)
+  # steps 2 and 3 with unrelated gradients, where the bias corrections
+  # 1 - 0.9^t and 1 - 0.999^t no longer cancel (optax.adam: moments
+  # m <- 0.9 m + 0.1 g, v <- 0.999 v + 0.001 g^2, update m_hat / (sqrt(v_hat) + eps))
+  rng = np.random.default_rng(5)
+  m_s, v_s, p = 0.1 * g, 0.001 * g * g, want
+  for t in (2, 3):
+    gt = torch.from_numpy(rng.normal(scale=float(g.abs().mean()) * 3, size=g.numel())).to(grad.dtype).to(grad.device)
+    model.adam_step(gt, lr=1e-3)
+    gd = gt.double()
+    m_s = 0.9 * m_s + 0.1 * gd
+    v_s = 0.999 * v_s + 0.001 * gd * gd
+    p = p - 1e-3 * (m_s / (1 - 0.9 ** t)) / (torch.sqrt(v_s / (1 - 0.999 ** t)) + 1e-8)
+    torch.testing.assert_close(model.params.double(), p, rtol=1e-5, atol=1e-7)
   path = str(tmp_path / "m.npz")
   model.save(path)
   other = deepsets.DeepSets.load(path)
