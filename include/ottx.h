@@ -319,6 +319,18 @@ int ottx_pointcloud_apply(const ottx_problem* problem, int32_t axis, const float
 int ottx_cost_maxima(int64_t n, int64_t m, int32_t d, const float* x, const float* y, int32_t with_max_cost,
                      double* out_host, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Median of the unscaled cost matrix |x_i - y_j|^2 over its n m entries,
+ * the value behind scale_cost="median" (pointcloud.py:351-353, geometry.py:
+ * 106-109 median_cost_matrix: jnp.median, the mean of the two middle entries
+ * when n m is even; NaN when an entry is NaN).  An exact radix select over
+ * the entries' float bits, computed from the differences in float32, four
+ * passes over the pairs; the matrix is never formed.  n m <= 2^40.
+ * workspace: ottx_cost_median_workspace_bytes(), 256-byte aligned.  Blocks
+ * until the value is on the host. */
+size_t ottx_cost_median_workspace_bytes(void);
+int ottx_cost_median(int64_t n, int64_t m, int32_t d, const float* x, const float* y, double* out_host,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
 /* Dense transport matrix from potentials, geometry.py:361-365
  * (transport_from_potentials): out[i * m + j] = exp((f_i + g_j - C_ij) / eps)
  * with C_ij = inv_scale_cost * |x_i - y_j|^2 summed from the differences in

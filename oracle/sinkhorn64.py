@@ -48,6 +48,7 @@ __all__ = [
     "sqeuclidean_cost",
     "mean_cost",
     "std_cost",
+    "median_cost",
     "resolve_epsilon",
     "epsilon_at",
     "momentum_weight",
@@ -127,6 +128,13 @@ def std_cost(x, y, scale: float = 1.0) -> float:
   n, m = c.shape
   tmp = np.sum((1.0 / m) * np.sum(c ** 2, axis=0) / n) - mean_cost(x, y, scale) ** 2
   return float(np.sqrt(max(tmp, 0.0)))
+
+
+def median_cost(x, y, scale: float = 1.0) -> float:
+  """Median of the cost matrix (geometry/geometry.py:106-109, jnp.median:
+  the mean of the two middle entries when n m is even), the value that
+  scale_cost="median" divides by (geometry/pointcloud.py:351-353)."""
+  return float(np.median(sqeuclidean_cost(x, y, scale)))
 
 
 def resolve_epsilon(x, y, epsilon=None, relative_epsilon=None,

@@ -204,6 +204,10 @@ _SIGNATURES = [
     ("ottx_cost_maxima", ctypes.c_int,
      [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
       ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    ("ottx_cost_median_workspace_bytes", ctypes.c_size_t, []),
+    ("ottx_cost_median", ctypes.c_int,
+     [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     ("ottx_transport_matrix", ctypes.c_int,
      [ctypes.POINTER(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
       ctypes.c_void_p]),

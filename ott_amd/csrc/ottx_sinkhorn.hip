@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <vector>
 
@@ -1127,6 +1128,151 @@ __global__ __launch_bounds__(256) void transport_cost_kernel(
     __syncthreads();
   }
   if (tid == 0) partial[static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Median of the unscaled cost matrix (geometry.py:106-109 median_cost_matrix,
+// used by scale_cost="median", pointcloud.py:351-353: jnp.median over the
+// n m entries, the mean of the two middle ones when n m is even) without
+// forming the matrix: an exact radix select over the float bits of the costs.
+// The costs are summed from the differences, so they are >= 0 and order like
+// their bit patterns.  Three passes bin bits [31:21], [20:10] and [9:0] of the
+// entries whose higher bits equal the prefix selected so far; after each, one
+// workgroup picks the bin that holds the wanted rank.  For an even count a
+// fourth pass takes the smallest entry above the lower middle one, unless
+// the lower middle value also holds the next rank.  Any NaN entry makes the
+// median NaN, as in jnp.median.
+// ---------------------------------------------------------------------------
+struct MedianState {
+  unsigned long long rank;      // rank wanted among the entries matching the prefix
+  unsigned long long total;     // n m
+  unsigned long long eq_count;  // entries equal to the lower middle value
+  uint32_t prefix;              // selected high bits
+  uint32_t mask;                // which bits of prefix are selected
+  uint32_t lo_bits;             // the lower middle entry
+  uint32_t hi_bits;             // the smallest entry above it (fourth pass)
+  uint32_t need_hi;             // the fourth pass is needed
+  uint32_t nan_seen;
+};
+constexpr int kMedBins = 2048;
+
+__global__ void median_init_kernel(MedianState* st, unsigned long long* hist, unsigned long long total) {
+  for (int e = threadIdx.x; e < kMedBins; e += blockDim.x) hist[e] = 0ull;
+  if (threadIdx.x == 0) {
+    st->rank = (total - 1) / 2;
+    st->total = total;
+    st->eq_count = 0ull;
+    st->prefix = 0u;
+    st->mask = 0u;
+    st->lo_bits = 0u;
+    st->hi_bits = 0xffffffffu;
+    st->need_hi = 0u;
+    st->nan_seen = 0u;
+  }
+}
+
+// One 64 x 64 block of costs per workgroup (tc_sq_dist); PASS 0-2 add the
+// block's matching entries to the global histogram, PASS 3 takes the minimum
+// above the lower middle entry.
+template <int PASS>
+__global__ __launch_bounds__(256) void median_pass_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                          int64_t n, int64_t m, int d, MedianState* st,
+                                                          unsigned long long* __restrict__ hist) {
+  __shared__ __align__(16) float xs[kTcChunk][kTcPad];
+  __shared__ __align__(16) float ys[kTcChunk][kTcPad];
+  __shared__ uint32_t h[PASS < 3 ? kMedBins : 1];
+  __shared__ uint32_t s_flag;
+  if (PASS == 3 && st->need_hi == 0u) return;  // the whole grid
+  const int tid = threadIdx.x;
+  constexpr int kShift = PASS == 0 ? 21 : (PASS == 1 ? 10 : 0);
+  constexpr uint32_t kBinMask = PASS == 2 ? 1023u : 2047u;
+  if (PASS < 3)
+    for (int e = tid; e < kMedBins; e += 256) h[e] = 0u;
+  if (tid == 0) s_flag = PASS < 3 ? 0u : 0xffffffffu;
+  const int64_t i0 = static_cast<int64_t>(blockIdx.y) * kTcTile;
+  const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kTcTile;
+  float acc[4][4] = {};
+  tc_sq_dist(x, y, n, m, d, i0, j0, xs, ys, tid, acc);  // begins with a barrier
+  const uint32_t prefix = st->prefix, mask = st->mask, lo = st->lo_bits;
+  const int r0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
+  bool nan = false;
+  uint32_t mn = 0xffffffffu;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (i0 + r0 + a >= n || j0 + c0 + b >= m) continue;
+      const uint32_t u = __float_as_uint(acc[a][b]);
+      if (PASS == 3) {
+        if (u > lo && u < mn) mn = u;
+      } else {
+        if (PASS == 0 && acc[a][b] != acc[a][b]) nan = true;
+        if ((u & mask) == prefix) atomicAdd(&h[(u >> kShift) & kBinMask], 1u);
+      }
+    }
+  }
+  if (PASS == 3) {
+    if (mn != 0xffffffffu) atomicMin(&s_flag, mn);
+    __syncthreads();
+    if (tid == 0 && s_flag != 0xffffffffu) atomicMin(&st->hi_bits, s_flag);
+    return;
+  }
+  if (nan) s_flag = 1u;
+  __syncthreads();
+  for (int e = tid; e <= static_cast<int>(kBinMask); e += 256)
+    if (h[e]) atomicAdd(&hist[e], static_cast<unsigned long long>(h[e]));
+  if (PASS == 0 && tid == 0 && s_flag) atomicOr(&st->nan_seen, 1u);
+}
+
+// One workgroup: the bin of the histogram that holds the wanted rank extends
+// the prefix; the histogram is cleared for the next pass.
+template <int PASS>
+__global__ __launch_bounds__(1024) void median_select_kernel(MedianState* st, unsigned long long* hist) {
+  constexpr int kShift = PASS == 0 ? 21 : (PASS == 1 ? 10 : 0);
+  constexpr int kBins = PASS == 2 ? 1024 : 2048;
+  constexpr int kPer = kBins / 1024;
+  __shared__ unsigned long long s_sum[1024];
+  const int t = threadIdx.x;
+  unsigned long long c[kPer];
+  unsigned long long own = 0ull;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    c[q] = hist[t * kPer + q];
+    hist[t * kPer + q] = 0ull;
+    own += c[q];
+  }
+  s_sum[t] = own;
+  __syncthreads();
+  // inclusive scan of the per-thread sums
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned long long v = t >= off ? s_sum[t - off] : 0ull;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  const unsigned long long rank = st->rank;
+  unsigned long long before = s_sum[t] - own;
+  __syncthreads();
+  if (rank >= before && rank < before + own) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      if (rank < before + c[q]) {
+        const uint32_t bin = static_cast<uint32_t>(t * kPer + q);
+        st->prefix |= bin << kShift;
+        st->mask |= static_cast<uint32_t>(kBins - 1) << kShift;
+        st->rank = rank - before;
+        if (PASS == 2) {
+          st->lo_bits = st->prefix;
+          st->eq_count = c[q];
+          // an even count also wants the next rank: the same value if it
+          // repeats beyond this rank, else the smallest larger entry
+          st->need_hi = (st->total % 2ull == 0ull && rank - before + 1ull >= c[q]) ? 1u : 0u;
+        }
+        break;
+      }
+      before += c[q];
+    }
+  }
 }
 
 // Sum of the partials in a fixed order (one workgroup of 1024 threads).
@@ -2271,6 +2417,48 @@ int ottx_cost_maxima(int64_t n, int64_t m, int32_t d, const float* x, const floa
   out_host[0] = v[0];
   out_host[1] = v[1];
   out_host[2] = with_max_cost ? static_cast<double>(v[2]) : 0.0;
+  return OTTX_OK;
+}
+
+size_t ottx_cost_median_workspace_bytes(void) {
+  return align_up(sizeof(MedianState)) + align_up(sizeof(unsigned long long) * kMedBins);
+}
+
+int ottx_cost_median(int64_t n, int64_t m, int32_t d, const float* x, const float* y, double* out_host,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+  if (n <= 0 || m <= 0 || d <= 0 || !x || !y || !out_host || !workspace) return OTTX_EINVAL;
+  if (n > (int64_t{1} << 40) / m) return OTTX_EINVAL;  // n m <= 2^40 entries
+  if (workspace_bytes < ottx_cost_median_workspace_bytes()) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  if ((n + kTcTile - 1) / kTcTile > 65535) return OTTX_EINVAL;  // row blocks index the grid's y dimension
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  MedianState* ms = static_cast<MedianState*>(workspace);
+  unsigned long long* hist =
+      reinterpret_cast<unsigned long long*>(static_cast<char*>(workspace) + align_up(sizeof(MedianState)));
+  const dim3 grid(static_cast<unsigned>((m + kTcTile - 1) / kTcTile), static_cast<unsigned>((n + kTcTile - 1) / kTcTile));
+  hipLaunchKernelGGL(median_init_kernel, dim3(1), dim3(256), 0, st, ms, hist,
+                     static_cast<unsigned long long>(n) * static_cast<unsigned long long>(m));
+  hipLaunchKernelGGL(median_pass_kernel<0>, grid, dim3(256), 0, st, x, y, n, m, d, ms, hist);
+  hipLaunchKernelGGL(median_select_kernel<0>, dim3(1), dim3(1024), 0, st, ms, hist);
+  hipLaunchKernelGGL(median_pass_kernel<1>, grid, dim3(256), 0, st, x, y, n, m, d, ms, hist);
+  hipLaunchKernelGGL(median_select_kernel<1>, dim3(1), dim3(1024), 0, st, ms, hist);
+  hipLaunchKernelGGL(median_pass_kernel<2>, grid, dim3(256), 0, st, x, y, n, m, d, ms, hist);
+  hipLaunchKernelGGL(median_select_kernel<2>, dim3(1), dim3(1024), 0, st, ms, hist);
+  hipLaunchKernelGGL(median_pass_kernel<3>, grid, dim3(256), 0, st, x, y, n, m, d, ms, hist);
+  OTTX_LAUNCHED();
+  MedianState host;
+  OTTX_HIP(hipMemcpyAsync(&host, ms, sizeof(host), hipMemcpyDeviceToHost, st));
+  OTTX_HIP(hipStreamSynchronize(st));
+  float lo, hi;
+  std::memcpy(&lo, &host.lo_bits, sizeof(lo));
+  std::memcpy(&hi, &host.hi_bits, sizeof(hi));
+  if (host.nan_seen) {
+    out_host[0] = std::numeric_limits<double>::quiet_NaN();
+  } else if (host.total % 2ull == 1ull) {
+    out_host[0] = lo;
+  } else {
+    out_host[0] = 0.5 * (static_cast<double>(lo) + static_cast<double>(host.need_hi ? hi : lo));
+  }
   return OTTX_OK;
 }
 

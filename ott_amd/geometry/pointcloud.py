@@ -87,6 +87,7 @@ class PointCloud:
     self._mean_cache: Optional[float] = None
     self._std_cache: Optional[float] = None
     self._inv_scale_cache: Optional[float] = None
+    self._median_cache: Optional[float] = None
 
   # -- shape / dtype ---------------------------------------------------------
   @property
@@ -122,23 +123,22 @@ class PointCloud:
     """pointcloud.py:342-383.
 
     Floats scale directly.  "mean" uses the closed-form mean of the unscaled
-    cost, "max_norm" / "max_bound" the largest squared norms and "max_cost" the
-    largest pair cost, each computed once on the GPU and kept.  "median" needs
-    the materialised cost matrix, which the reference refuses in online mode
-    too, and raises.
+    cost, "max_norm" / "max_bound" the largest squared norms, "max_cost" the
+    largest pair cost and "median" the median entry (jnp.median of the
+    unscaled matrix, as the reference computes for clouds that are not online;
+    an exact selection over the pairs on the GPU, ``ottx_cost_median``), each
+    computed once and kept.
     """
     s = self._scale_cost
     if isinstance(s, (int, float, np.floating)) and not isinstance(s, bool):
       if not float(s) > 0.0:
         raise ValueError(f"scale_cost must be positive, got {s}")
       return 1.0 / float(s)
-    if isinstance(s, str) and s in ("mean", "max_cost", "max_norm", "max_bound"):
+    if isinstance(s, str) and s in ("mean", "max_cost", "max_norm", "max_bound", "median"):
       if self._inv_scale_cache is None:
-        self._inv_scale_cache = 1.0 / self._cost_summary(s)
+        v = self._cost_summary(s)
+        self._inv_scale_cache = math.inf if v == 0.0 else 1.0 / v
       return self._inv_scale_cache
-    if s == "median":
-      raise NotImplementedError("scale_cost='median' needs the materialised cost matrix "
-                                "(the reference does not run it online either)")
     raise ValueError(f"Scaling {s} not implemented.")
 
   def _cost_summary(self, mode: str) -> float:
@@ -154,6 +154,8 @@ class PointCloud:
           n, m, d, self.x.data_ptr(), self.y.data_ptr(), 1.0, ctypes.byref(out), ws.data_ptr(), ws.numel(),
           _lib.stream_ptr()))
       return float(out.value)
+    if mode == "median":
+      return self._unscaled_median()
     ws = _lib.workspace(256, self.device)
     out = (ctypes.c_double * 3)()
     _lib.check("ottx_cost_maxima", lib.ottx_cost_maxima(
@@ -166,6 +168,24 @@ class PointCloud:
       return max(x_max, y_max)
     # max_bound for the squared-Euclidean cost
     return x_max + y_max + 2.0 * math.sqrt(x_max * y_max)
+
+  def _unscaled_median(self) -> float:
+    if self._median_cache is None:
+      lib = _lib.load_library()
+      n, m = self.shape
+      ws = _lib.workspace(lib.ottx_cost_median_workspace_bytes(), self.device)
+      out = ctypes.c_double(0.0)
+      _lib.check("ottx_cost_median", lib.ottx_cost_median(
+          n, m, self.x.shape[1], self.x.data_ptr(), self.y.data_ptr(), ctypes.byref(out), ws.data_ptr(),
+          ws.numel(), _lib.stream_ptr()))
+      self._median_cache = float(out.value)
+    return self._median_cache
+
+  @property
+  def median_cost_matrix(self) -> float:
+    """geometry.py:106-109: the median of the (scaled) cost matrix, exact, on
+    the GPU without forming the matrix."""
+    return self.inv_scale_cost * self._unscaled_median()
 
   @property
   def mean_cost_matrix(self) -> float:

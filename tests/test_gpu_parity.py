@@ -348,7 +348,55 @@ def test_scale_cost_modes(mode):
   _check_solve(out, ref_out, "fp32")
 
 
-def test_scale_cost_median_is_refused():
-  x, y, _, _ = _cloud(20, 30, 3, seed=1)
-  with pytest.raises(NotImplementedError):
-    _ = _geom(x, y, "fp32", epsilon=0.1, scale_cost="median").inv_scale_cost
+def _median_f32(x, y):
+  """np.median of the float32 costs summed from the differences (the entries
+  the kernel selects from); float64 where the inputs make the sums exact."""
+  c = ((x[:, None, :].astype(np.float64) - y[None].astype(np.float64)) ** 2).sum(-1)
+  return float(np.median(c))
+
+
+@pytest.mark.parametrize("n,m,d,kind", [(333, 257, 6, "normal"), (300, 258, 6, "normal"), (1, 1, 3, "normal"),
+                                        (1, 2, 3, "normal"), (65, 64, 130, "normal"), (2048, 3001, 17, "normal"),
+                                        (97, 96, 3, "integer"), (120, 80, 2, "duplicates")])
+def test_median_cost_matches_numpy(n, m, d, kind):
+  """ottx_cost_median (jnp.median of the unscaled cost matrix) against
+  np.median of every entry: odd and even counts, one entry, d above the
+  staging chunk, ties (integer coordinates give exact, repeated costs;
+  duplicated points give repeated zeros)."""
+  rng = np.random.default_rng(n + m + d)
+  if kind == "integer":
+    x = rng.integers(-3, 4, size=(n, d)).astype(np.float32)
+    y = rng.integers(-3, 4, size=(m, d)).astype(np.float32)
+  elif kind == "duplicates":
+    base = rng.normal(size=(10, d)).astype(np.float32)
+    x = base[rng.integers(0, 10, size=n)]
+    y = base[rng.integers(0, 10, size=m)]
+  else:
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    y = (rng.normal(size=(m, d)) + 0.3).astype(np.float32)
+  print(f"median n={n} m={m} d={d} {kind}", flush=True)
+  geom = _geom(x, y, "fp32", epsilon=0.1, scale_cost="median")
+  want = _median_f32(x, y)
+  got = 1.0 / geom.inv_scale_cost
+  if kind == "integer":
+    assert got == want, (got, want)
+  else:
+    assert math.isclose(got, want, rel_tol=2e-6, abs_tol=1e-30), (got, want)
+  assert math.isclose(geom.median_cost_matrix, 1.0, rel_tol=1e-12) or want == 0.0
+
+
+def test_median_cost_nan_entry():
+  x, y, _, _ = _cloud(50, 40, 3, seed=2)
+  x[7, 1] = np.nan
+  assert math.isnan(_geom(x, y, "fp32", epsilon=0.1, scale_cost=2.0).median_cost_matrix)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+def test_solve_with_median_scale(precision):
+  """scale_cost="median" through the solve against the oracle given the
+  oracle's own median as the scale."""
+  x, y, a, b = _cloud(333, 257, 6, seed=17, shift=0.5)
+  out = _solve(x, y, a, b, precision, dict(epsilon=0.05, relative_epsilon="mean", scale_cost="median"))
+  med = oracle.median_cost(x, y)
+  ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.05, relative_epsilon="mean", scale_cost=med)
+  _check_solve(out, ref, precision)

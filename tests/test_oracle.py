@@ -458,3 +458,17 @@ def test_soft_c_transform_and_barycentric_map():
   for i in range(len(x)):
     grad = np.array([(f_of(x[i] + h * e) - f_of(x[i] - h * e)) / (2 * h) for e in np.eye(3)])
     np.testing.assert_allclose(t[i], x[i] - grad / 2, atol=1e-7)
+
+
+def test_median_cost_against_sorted_entries():
+  """median_cost = the middle entry of the sorted cost matrix (odd n m) or the
+  mean of the two middle ones (even n m), on brute-force costs."""
+  rng = np.random.default_rng(31)
+  for n, m in ((7, 5), (6, 5), (1, 1), (1, 2), (40, 33)):
+    x = rng.normal(size=(n, 3))
+    y = rng.normal(size=(m, 3))
+    c = np.sort(np.array([[np.sum((xi - yj) ** 2) for yj in y] for xi in x]).ravel())
+    k = c.size
+    want = c[k // 2] if k % 2 else 0.5 * (c[k // 2 - 1] + c[k // 2])
+    assert math.isclose(oracle.median_cost(x, y), want, rel_tol=1e-12)
+    assert math.isclose(oracle.median_cost(x, y, 0.5), 0.5 * want, rel_tol=1e-12)
