@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define OTTX_ABI_VERSION 3
+#define OTTX_ABI_VERSION 4
 
 enum ottx_status {
   OTTX_OK = 0,
@@ -81,8 +81,24 @@ typedef struct ottx_problem {
                               0.05 * std(C), geometry.py:147-171); 0: use epsilon */
   int32_t epsilon_statistic; /* enum ottx_eps_statistic: the statistic relative_epsilon
                               multiplies */
-  int32_t _pad_problem;
+  int32_t cost;            /* enum ottx_cost: the translation-invariant cost h(x - y) */
+  double cost_p;           /* p of OTTX_COST_PNORMP, _SQPNORM, _EUCLIDEANP (>= 1) */
 } ottx_problem;
+
+/* Costs of a point cloud, c(x, y) = h(x - y) (geometry/costs.py):
+ * OTTX_COST_SQEUCLIDEAN runs the MFMA half-step on the cross term; the others
+ * sum a per-feature term of z = x - y on the vector ALUs, online (the cost is
+ * recomputed per tile, never stored).  With them a relative epsilon is
+ * resolved by the caller (ottx_cost_summary gives the mean and std of C);
+ * ottx_pointcloud_apply and ottx_cost_gradient need the squared-Euclidean
+ * cost.  inv_scale_cost multiplies every cost. */
+enum ottx_cost {
+  OTTX_COST_SQEUCLIDEAN = 0, /* ||z||_2^2 (costs.py:548-573) */
+  OTTX_COST_PNORMP = 1,      /* ||z||_p^p / p (costs.py:304-334) */
+  OTTX_COST_SQPNORM = 2,     /* ||z||_p^2 / 2 (costs.py:269-301) */
+  OTTX_COST_EUCLIDEANP = 3,  /* ||z||_2^p (costs.py:337-361) */
+  OTTX_COST_EUCLIDEAN = 4,   /* ||z||_2 (costs.py:528-545) */
+};
 
 /* The statistic of the cost matrix a relative epsilon multiplies
  * (geometry/geometry.py:112-136, both in closed form, never forming C). */
@@ -319,17 +335,29 @@ int ottx_pointcloud_apply(const ottx_problem* problem, int32_t axis, const float
 int ottx_cost_maxima(int64_t n, int64_t m, int32_t d, const float* x, const float* y, int32_t with_max_cost,
                      double* out_host, void* workspace, size_t workspace_bytes, void* stream);
 
-/* Median of the unscaled cost matrix |x_i - y_j|^2 over its n m entries,
- * the value behind scale_cost="median" (pointcloud.py:351-353, geometry.py:
+/* Median of the unscaled cost matrix c(x_i, y_j) over its n m entries, the
+ * value behind scale_cost="median" (pointcloud.py:351-353, geometry.py:
  * 106-109 median_cost_matrix: jnp.median, the mean of the two middle entries
  * when n m is even; NaN when an entry is NaN).  An exact radix select over
- * the entries' float bits, computed from the differences in float32, four
- * passes over the pairs; the matrix is never formed.  n m <= 2^40.
+ * the entries' float bits, costs computed from the differences in float32
+ * with the problem's cost, four passes over the pairs; the matrix is never
+ * formed.  n m <= 2^40.  Uses n, m, d, x, y, cost and cost_p of the problem.
  * workspace: ottx_cost_median_workspace_bytes(), 256-byte aligned.  Blocks
  * until the value is on the host. */
 size_t ottx_cost_median_workspace_bytes(void);
-int ottx_cost_median(int64_t n, int64_t m, int32_t d, const float* x, const float* y, double* out_host,
-                     void* workspace, size_t workspace_bytes, void* stream);
+int ottx_cost_median(const ottx_problem* problem, double* out_host, void* workspace, size_t workspace_bytes,
+                     void* stream);
+
+/* Sum, sum of squares and maximum of the unscaled cost matrix c(x_i, y_j)
+ * (out_host[0..2], the sums in float64), one pass over the pairs: the mean
+ * and std of the cost matrix (geometry.py:112-136) and scale_cost "mean" /
+ * "max_cost" (pointcloud.py:342-383) for costs without the squared
+ * Euclidean's closed form (the reference forms the matrix or applies it to a
+ * vector).  Uses n, m, d, x, y, cost and cost_p of the problem.  Blocks until
+ * the values are on the host. */
+size_t ottx_cost_summary_workspace_bytes(int64_t n, int64_t m);
+int ottx_cost_summary(const ottx_problem* problem, double* out_host, void* workspace, size_t workspace_bytes,
+                      void* stream);
 
 /* Dense transport matrix from potentials, geometry.py:361-365
  * (transport_from_potentials): out[i * m + j] = exp((f_i + g_j - C_ij) / eps)

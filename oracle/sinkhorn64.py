@@ -46,6 +46,7 @@ from scipy import special as sps
 
 __all__ = [
     "sqeuclidean_cost",
+    "ti_cost",
     "mean_cost",
     "std_cost",
     "median_cost",
@@ -103,6 +104,33 @@ def sqeuclidean_cost(x, y, scale: float = 1.0, *, expanded: bool = False,
     diff = x[:, None, :] - y[None, :, :]
     c = np.sum(diff * diff, axis=-1)
   return (np.asarray(scale, dtype=dtype) * c).astype(dtype)
+
+
+def ti_cost(x, y, cost_fn, scale: float = 1.0) -> np.ndarray:
+  """Cost matrix ``s * c(x_i, y_j)`` of a named cost (geometry/costs.py):
+  ("sqeuclidean",) ``||z||^2`` (:548-573), ("pnormp", p) ``||z||_p^p / p``
+  (:304-334), ("sqpnorm", p) ``||z||_p^2 / 2`` (:269-301), ("euclideanp", p)
+  ``||z||_2^p`` (:337-361), ("euclidean",) ``||z||_2`` (:528-545), with
+  z = x_i - y_j (TICost.__call__, :155-157) and the p-norm of ``mu.norm``."""
+  x = np.asarray(x, dtype=np.float64)
+  y = np.asarray(y, dtype=np.float64)
+  kind = cost_fn[0]
+  z = np.abs(x[:, None, :] - y[None, :, :])
+  if kind == "sqeuclidean":
+    c = np.sum(z * z, axis=-1)
+  elif kind == "euclidean":
+    c = np.sqrt(np.sum(z * z, axis=-1))
+  else:
+    p = float(cost_fn[1])
+    if kind == "pnormp":
+      c = np.sum(z ** p, axis=-1) / p
+    elif kind == "sqpnorm":
+      c = 0.5 * np.sum(z ** p, axis=-1) ** (2.0 / p)
+    elif kind == "euclideanp":
+      c = np.sqrt(np.sum(z * z, axis=-1)) ** p
+    else:
+      raise ValueError(f"unknown cost {cost_fn!r}")
+  return scale * c
 
 
 def mean_cost(x, y, scale: float = 1.0) -> float:
@@ -539,8 +567,19 @@ def solve_pointcloud(x, y, a=None, b=None, *, epsilon=None,
   n, m = x.shape[0], y.shape[0]
   a = np.full((n,), 1.0 / n) if a is None else np.asarray(a, np.float64)
   b = np.full((m,), 1.0 / m) if b is None else np.asarray(b, np.float64)
-  eps = resolve_epsilon(x, y, epsilon, relative_epsilon, 1.0 / scale_cost)
-  cost = sqeuclidean_cost(x, y, 1.0 / scale_cost, expanded=expanded, dtype=dtype)
+  cost_fn = kwargs.pop("cost_fn", None)
+  if cost_fn is None:
+    eps = resolve_epsilon(x, y, epsilon, relative_epsilon, 1.0 / scale_cost)
+    cost = sqeuclidean_cost(x, y, 1.0 / scale_cost, expanded=expanded, dtype=dtype)
+  else:
+    # geometry.py:147-171 on the dense matrix of another cost
+    cost = ti_cost(x, y, cost_fn, 1.0 / scale_cost)
+    stat = {"mean": np.mean(cost), "std": np.std(cost), None: np.std(cost)}[relative_epsilon]
+    if relative_epsilon is None and epsilon is not None:
+      eps = float(epsilon)
+    else:
+      eps = float((DEFAULT_EPSILON_SCALE if epsilon is None else epsilon) * stat)
+    cost = cost.astype(dtype)
   return sinkhorn(DenseLSE(cost), a, b, eps, dtype=dtype, **kwargs)
 
 

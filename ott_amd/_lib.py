@@ -54,12 +54,13 @@ class Problem(ctypes.Structure):
       ("tau_b", ctypes.c_double),
       ("relative_epsilon", ctypes.c_double),
       ("epsilon_statistic", ctypes.c_int32),
-      ("_pad_problem", ctypes.c_int32),
+      ("cost", ctypes.c_int32),
+      ("cost_p", ctypes.c_double),
   ]
 
 
 # OTTX_ABI_VERSION of include/ottx.h that these structures mirror
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 EPS_STAT_MEAN = 0
 EPS_STAT_STD = 1
@@ -206,8 +207,10 @@ _SIGNATURES = [
       ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     ("ottx_cost_median_workspace_bytes", ctypes.c_size_t, []),
     ("ottx_cost_median", ctypes.c_int,
-     [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
-      ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+     [ctypes.POINTER(Problem), ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    ("ottx_cost_summary_workspace_bytes", ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int64]),
+    ("ottx_cost_summary", ctypes.c_int,
+     [ctypes.POINTER(Problem), ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     ("ottx_transport_matrix", ctypes.c_int,
      [ctypes.POINTER(Problem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
       ctypes.c_void_p]),

@@ -130,6 +130,12 @@ struct HalfStepArgs {
   int32_t force_scan;
   float threshold;
   uint32_t* batch_done;  // batched solves: count of problems whose loop finished
+  // costs other than the squared Euclidean (PREC 2): in_frag / out_frag are
+  // the raw points [rows, d]; a pair's cost is s alpha (sum_k phi(z_k))^beta
+  int32_t d;
+  float cost_p;         // exponent of phi(z) = |z|^p
+  float cost_alpha;
+  float cost_beta;
 };
 
 }  // namespace ottx

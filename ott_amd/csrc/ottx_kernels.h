@@ -195,7 +195,7 @@ __device__ __forceinline__ float f4_get(const float4& v, int e) {
 // Number of 1 KiB fragment chunks per 16-row block.
 template <int PREC, int KK>
 struct FragChunks {
-  static constexpr int value = (PREC == 1) ? KK * 2 : (KK + 3) / 4;
+  static constexpr int value = (PREC == 1) ? KK * 2 : (PREC == 0 ? (KK + 3) / 4 : 1);
 };
 
 // Loads one streamed tile (contiguous fragment chunks of MT row blocks) and
@@ -684,6 +684,112 @@ __device__ __forceinline__ void block_sum2(double (*s_red)[NT], int tid, double 
 }
 
 // ---------------------------------------------------------------------------
+// Costs other than the squared Euclidean (geometry/costs.py TICost family and
+// Euclidean): c(x, y) = alpha (sum_k phi(x_k - y_k))^beta with phi(z) = |z|
+// (PHI 1), z^2 (PHI 2) or |z|^p (PHI 3).  No cross term exists for MFMA, so a
+// workgroup of 256 threads sums the per-feature terms on the vector ALUs for
+// 64 rows by 64 columns at a time, features staged through LDS 32 at a time
+// (4 x 4 pairs per thread), then folds the rows into each column's running
+// (max, sum) in base-2 exponent units, z_ij = (h_i - s c_ij) log2(e) / eps.
+// ---------------------------------------------------------------------------
+constexpr int kGcTile = 64;
+constexpr int kGcChunk = 32;
+constexpr int kGcPad = kGcTile + 4;
+
+template <int PHI>
+__device__ __forceinline__ float cost_phi_acc(float acc, float u, float p) {
+  if constexpr (PHI == 1) {
+    return acc + fabsf(u);
+  } else if constexpr (PHI == 2) {
+    return fmaf(u, u, acc);
+  } else {
+    const float a = fabsf(u);
+    return acc + (a > 0.f ? exp2f(p * log2f(a)) : (a == 0.f ? 0.f : a));  // NaN stays NaN
+  }
+}
+
+// alpha a^beta for a >= 0 (NaN propagates)
+__device__ __forceinline__ float cost_psi(float a, float alpha, float beta) {
+  if (beta == 1.f) return alpha * a;
+  if (beta == 0.5f) return alpha * sqrtf(a);
+  if (beta == 2.f) return alpha * a * a;
+  return alpha * (a > 0.f ? exp2f(beta * log2f(a)) : (a == 0.f ? 0.f : a));
+}
+
+// Column (max, sum) of the rows [row_begin, row_end) of the in side against
+// the 64 out-side points from col0, into s_ms[64].
+template <int PHI>
+__device__ __forceinline__ void stream_cost(const HalfStepArgs& p, int64_t col0, int64_t row_begin, int64_t row_end,
+                                            float cz, float s, int tid, float2* s_ms,
+                                            float (*xs)[kGcPad], float (*ys)[kGcPad], float2 (*part)[kGcTile]) {
+  const float* __restrict__ X = static_cast<const float*>(p.in_frag);
+  const float* __restrict__ Y = static_cast<const float*>(p.out_frag);
+  const int d = p.d;
+  const float cp = p.cost_p, alpha = p.cost_alpha, beta = p.cost_beta;
+  const int r0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
+  const int64_t n_in = p.n_in, n_out = p.n_out;
+  const int64_t rend = row_end < n_in ? row_end : n_in;
+  float mx[4], sm[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    mx[b] = -INFINITY;
+    sm[b] = 0.f;
+  }
+  for (int64_t i0 = row_begin; i0 < rend; i0 += kGcTile) {
+    float acc[4][4] = {};
+    for (int k0 = 0; k0 < d; k0 += kGcChunk) {
+      const int kc = min(kGcChunk, d - k0);
+      __syncthreads();
+      for (int e = tid; e < kGcTile * kGcChunk; e += 256) {
+        const int r = e / kGcChunk, k = e % kGcChunk;
+        const int64_t i = i0 + r, j = col0 + r;
+        xs[k][r] = (k < kc && i < rend) ? X[i * d + k0 + k] : 0.f;
+        ys[k][r] = (k < kc && j < n_out) ? Y[j * d + k0 + k] : 0.f;
+      }
+      __syncthreads();
+      for (int k = 0; k < kc; ++k) {
+        const float4 xv = *reinterpret_cast<const float4*>(&xs[k][r0]);
+        const float4 yv = *reinterpret_cast<const float4*>(&ys[k][c0]);
+        const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
+        const float ya[4] = {yv.x, yv.y, yv.z, yv.w};
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] = cost_phi_acc<PHI>(acc[a][b], xa[a] - ya[b], cp);
+      }
+    }
+    float z[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int64_t i = i0 + r0 + a;
+      const float hi = i < rend ? p.in_pot[i] : -INFINITY;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) z[a][b] = (hi - s * cost_psi(acc[a][b], alpha, beta)) * cz;
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      float cm = nan_max(nan_max(z[0][b], z[1][b]), nan_max(z[2][b], z[3][b]));
+      const float nm = nan_max(mx[b], cm);
+      if (nm == -INFINITY) continue;  // every term so far is exp(-inf) = 0
+      float acc_s = sm[b] * exp2_fast(mx[b] - nm);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc_s += exp2_fast(z[a][b] - nm);
+      sm[b] = acc_s;
+      mx[b] = nm;
+    }
+  }
+  // the 16 row groups of each column, merged in a fixed order
+#pragma unroll
+  for (int b = 0; b < 4; ++b) part[tid >> 4][c0 + b] = make_float2(mx[b], sm[b]);
+  __syncthreads();
+  if (tid < kGcTile) {
+    float M = -INFINITY, S = 0.f;
+    for (int g = 0; g < 16; ++g) lse_merge(M, S, part[g][tid].x, part[g][tid].y);
+    s_ms[tid] = make_float2(M, S);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // One workgroup of a half-step: column tile ``tile`` of the output side and
 // row split ``split`` of the input side.  W waves each own NB column blocks;
 // the last split to arrive on a column tile merges the partial (max, sum)
@@ -751,11 +857,21 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
   const float k = c / c_pack;
   const float kb = static_cast<float>(static_cast<double>(c_pack) / p.two_s);
 
+  const int rb_begin = split * p.rb_per_split;
+  const int rb_end = min(rb_begin + p.rb_per_split, p.n_in_rb);
+  if constexpr (PREC == 2) {
+    static_assert(COLS == kGcTile && NT == 256, "the cost stream covers 64 columns with 256 threads");
+    __shared__ __align__(16) float s_xs[kGcChunk][kGcPad];
+    __shared__ __align__(16) float s_ys[kGcChunk][kGcPad];
+    __shared__ float2 s_part[16][kGcTile];
+    (void)kb;
+    stream_cost<KK>(p, static_cast<int64_t>(tile) * COLS, static_cast<int64_t>(rb_begin) * 16,
+                    static_cast<int64_t>(rb_end) * 16, static_cast<float>(static_cast<double>(c) / p.two_s),
+                    static_cast<float>(0.5 * p.two_s), tid, s_ms, s_xs, s_ys, s_part);
+  } else {
   OutOperand<PREC, KK, NB> B;
   B.load(p.out_frag, static_cast<int64_t>(tile * W + wave) * NB, lane);
   float mx[NB], sm[NB];
-  const int rb_begin = split * p.rb_per_split;
-  const int rb_end = min(rb_begin + p.rb_per_split, p.n_in_rb);
   const int64_t cbase = static_cast<int64_t>(tile * W + wave) * NB * 16 + (lane & 15);
   // Fixed-shift streaming when the previous half-step of this side left a
   // finite log-sum-exp for every column of the tile; the workgroup falls back
@@ -845,6 +961,7 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
                                                     kb, 0.f, k, tid, mx, sm, s_tile, s_bias);
     combine_lanes<NB>(mx, sm, lane, wave, s_ms);
   }
+  }  // PREC 0 / 1
   __syncthreads();
 
   const int64_t col0 = static_cast<int64_t>(tile) * COLS;

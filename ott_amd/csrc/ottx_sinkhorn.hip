@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "ottx_common.h"
@@ -1026,12 +1027,52 @@ __global__ __launch_bounds__(256) void max_cost_kernel(const float* __restrict__
   block_max_store(v, out);
 }
 
-// Dense transport matrix exp((f_i + g_j - s |x_i - y_j|^2) / eps), row-major
+// Host view of a problem's cost: c = alpha (sum_k phi(z_k))^beta.
+struct CostForm {
+  int phi;
+  float p, alpha, beta;
+};
+
+inline bool cost_form(int cost, double p, CostForm* cf) {
+  switch (cost) {
+    case OTTX_COST_SQEUCLIDEAN: *cf = CostForm{2, 2.f, 1.f, 1.f}; return true;
+    case OTTX_COST_EUCLIDEAN: *cf = CostForm{2, 2.f, 1.f, 0.5f}; return true;
+    default: break;
+  }
+  if (!(p >= 1.0 && p <= 1e4)) return false;
+  const float pf = static_cast<float>(p);
+  switch (cost) {
+    case OTTX_COST_PNORMP:  // sum |z|^p / p
+      *cf = p == 1.0 ? CostForm{1, 1.f, 1.f, 1.f}
+                     : (p == 2.0 ? CostForm{2, 2.f, 0.5f, 1.f} : CostForm{3, pf, static_cast<float>(1.0 / p), 1.f});
+      return true;
+    case OTTX_COST_SQPNORM:  // (sum |z|^p)^(2 / p) / 2
+      *cf = p == 1.0 ? CostForm{1, 1.f, 0.5f, 2.f}
+                     : (p == 2.0 ? CostForm{2, 2.f, 0.5f, 1.f} : CostForm{3, pf, 0.5f, static_cast<float>(2.0 / p)});
+      return true;
+    case OTTX_COST_EUCLIDEANP:  // (sum z^2)^(p / 2)
+      *cf = CostForm{2, 2.f, 1.f, static_cast<float>(0.5 * p)};
+      return true;
+    default: return false;
+  }
+}
+
+// Key of the half-step kernel family: the precision for the squared-Euclidean
+// cost, 10 + phi for the vector-ALU cost stream (PREC 2, one variant).
+inline int kernel_key(const ottx_problem* pb) {
+  if (pb->cost == OTTX_COST_SQEUCLIDEAN) return pb->precision;
+  CostForm cf;
+  return cost_form(pb->cost, pb->cost_p, &cf) ? 10 + cf.phi : -1;
+}
+
+// Dense transport matrix exp((f_i + g_j - s c(x_i, y_j)) / eps), row-major
 // [n, m]: four rows by 64 columns per workgroup, one entry per thread, the
-// squared distance summed from the differences.
+// cost summed from the differences (cost_phi_acc / cost_psi).
+template <int PHI>
 __global__ void transport_matrix_kernel(const float* __restrict__ x, const float* __restrict__ y, int64_t n,
                                         int64_t m, int d, float s, const float* __restrict__ f,
-                                        const float* __restrict__ g, float inv_eps, float* __restrict__ out) {
+                                        const float* __restrict__ g, float inv_eps, float* __restrict__ out,
+                                        CostForm cf) {
   const int64_t i = static_cast<int64_t>(blockIdx.y) * 4 + (threadIdx.x >> 6);
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 64 + (threadIdx.x & 63);
   if (i >= n || j >= m) return;
@@ -1040,30 +1081,29 @@ __global__ void transport_matrix_kernel(const float* __restrict__ x, const float
   float c0 = 0.f, c1 = 0.f;
   int k = 0;
   for (; k + 1 < d; k += 2) {
-    const float u = xi[k] - yj[k], v = xi[k + 1] - yj[k + 1];
-    c0 = fmaf(u, u, c0);
-    c1 = fmaf(v, v, c1);
+    c0 = cost_phi_acc<PHI>(c0, xi[k] - yj[k], cf.p);
+    c1 = cost_phi_acc<PHI>(c1, xi[k + 1] - yj[k + 1], cf.p);
   }
-  if (k < d) {
-    const float u = xi[k] - yj[k];
-    c0 = fmaf(u, u, c0);
-  }
-  out[i * m + j] = expf((f[i] + g[j] - s * (c0 + c1)) * inv_eps);
+  if (k < d) c0 = cost_phi_acc<PHI>(c0, xi[k] - yj[k], cf.p);
+  out[i * m + j] = expf((f[i] + g[j] - s * cost_psi(c0 + c1, cf.alpha, cf.beta)) * inv_eps);
 }
 
 // <P, C'> in one pass over the pairs (SinkhornOutput.primal_cost and
 // transport_cost_at_geom, sinkhorn.py:345-411): P_ij = exp((f_i + g_j - s
-// |x_i - y_j|^2) / eps) and C'_ij = s' |x'_i - y'_j|^2, both squared distances
-// summed from the differences in float32.  A workgroup takes 64 rows by 64
+// c(x_i, y_j)) / eps) and C'_ij = s c(x_i, y_j) itself or s' |x'_i - y'_j|^2
+// of another pair of clouds, costs summed from the differences in float32.  A workgroup takes 64 rows by 64
 // columns; each thread 4 x 4 pairs, with the features staged through LDS,
 // feature-major, 32 at a time.  One float64 partial per workgroup.
 constexpr int kTcTile = 64;
 constexpr int kTcChunk = 32;
 constexpr int kTcPad = kTcTile + 4;  // keeps float4 rows aligned, spreads the staging stores
 
-__device__ __forceinline__ void tc_sq_dist(const float* __restrict__ x, const float* __restrict__ y, int64_t n,
-                                           int64_t m, int d, int64_t i0, int64_t j0, float (*xs)[kTcPad],
-                                           float (*ys)[kTcPad], int tid, float (&acc)[4][4]) {
+// Per-feature cost terms of a 64 x 64 block of pairs, sum_k phi(x_ik - y_jk)
+// (phi as cost_phi_acc), 4 x 4 pairs per thread, features staged through LDS.
+template <int PHI>
+__device__ __forceinline__ void tc_cost_acc(const float* __restrict__ x, const float* __restrict__ y, int64_t n,
+                                            int64_t m, int d, int64_t i0, int64_t j0, float (*xs)[kTcPad],
+                                            float (*ys)[kTcPad], int tid, float (&acc)[4][4], float p) {
   const int r0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
   for (int k0 = 0; k0 < d; k0 += kTcChunk) {
     const int kc = min(kTcChunk, d - k0);
@@ -1083,19 +1123,22 @@ __device__ __forceinline__ void tc_sq_dist(const float* __restrict__ x, const fl
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const float u = xa[a] - ya[b];
-          acc[a][b] = fmaf(u, u, acc[a][b]);
-        }
+        for (int b = 0; b < 4; ++b) acc[a][b] = cost_phi_acc<PHI>(acc[a][b], xa[a] - ya[b], p);
     }
   }
 }
 
-template <bool SAME>
+__device__ __forceinline__ void tc_sq_dist(const float* __restrict__ x, const float* __restrict__ y, int64_t n,
+                                           int64_t m, int d, int64_t i0, int64_t j0, float (*xs)[kTcPad],
+                                           float (*ys)[kTcPad], int tid, float (&acc)[4][4]) {
+  tc_cost_acc<2>(x, y, n, m, d, i0, j0, xs, ys, tid, acc, 2.f);
+}
+
+template <bool SAME, int PHI>
 __global__ __launch_bounds__(256) void transport_cost_kernel(
     const float* __restrict__ x, const float* __restrict__ y, int64_t n, int64_t m, int d, float s,
     const float* __restrict__ f, const float* __restrict__ g, float inv_eps, const float* __restrict__ xo,
-    const float* __restrict__ yo, int d_o, float s_o, double* __restrict__ partial) {
+    const float* __restrict__ yo, int d_o, float s_o, double* __restrict__ partial, CostForm cf) {
   __shared__ __align__(16) float xs[kTcChunk][kTcPad];
   __shared__ __align__(16) float ys[kTcChunk][kTcPad];
   __shared__ double red[256];
@@ -1103,8 +1146,12 @@ __global__ __launch_bounds__(256) void transport_cost_kernel(
   const int64_t i0 = static_cast<int64_t>(blockIdx.y) * kTcTile;
   const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kTcTile;
   float acc[4][4] = {}, acc_o[4][4] = {};
-  tc_sq_dist(x, y, n, m, d, i0, j0, xs, ys, tid, acc);
+  tc_cost_acc<PHI>(x, y, n, m, d, i0, j0, xs, ys, tid, acc, cf.p);
   if constexpr (!SAME) tc_sq_dist(xo, yo, n, m, d_o, i0, j0, xs, ys, tid, acc_o);
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = cost_psi(acc[a][b], cf.alpha, cf.beta);
   const int r0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
   float part = 0.f;
 #pragma unroll
@@ -1171,13 +1218,13 @@ __global__ void median_init_kernel(MedianState* st, unsigned long long* hist, un
   }
 }
 
-// One 64 x 64 block of costs per workgroup (tc_sq_dist); PASS 0-2 add the
+// One 64 x 64 block of costs per workgroup (tc_cost_acc); PASS 0-2 add the
 // block's matching entries to the global histogram, PASS 3 takes the minimum
 // above the lower middle entry.
-template <int PASS>
+template <int PASS, int PHI>
 __global__ __launch_bounds__(256) void median_pass_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                           int64_t n, int64_t m, int d, MedianState* st,
-                                                          unsigned long long* __restrict__ hist) {
+                                                          unsigned long long* __restrict__ hist, CostForm cf) {
   __shared__ __align__(16) float xs[kTcChunk][kTcPad];
   __shared__ __align__(16) float ys[kTcChunk][kTcPad];
   __shared__ uint32_t h[PASS < 3 ? kMedBins : 1];
@@ -1192,7 +1239,11 @@ __global__ __launch_bounds__(256) void median_pass_kernel(const float* __restric
   const int64_t i0 = static_cast<int64_t>(blockIdx.y) * kTcTile;
   const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kTcTile;
   float acc[4][4] = {};
-  tc_sq_dist(x, y, n, m, d, i0, j0, xs, ys, tid, acc);  // begins with a barrier
+  tc_cost_acc<PHI>(x, y, n, m, d, i0, j0, xs, ys, tid, acc, cf.p);  // begins with a barrier
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = cost_psi(acc[a][b], cf.alpha, cf.beta);
   const uint32_t prefix = st->prefix, mask = st->mask, lo = st->lo_bits;
   const int r0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
   bool nan = false;
@@ -1272,6 +1323,59 @@ __global__ __launch_bounds__(1024) void median_select_kernel(MedianState* st, un
       }
       before += c[q];
     }
+  }
+}
+
+// Sum, sum of squares and maximum of the unscaled costs of 64 x 64 pair
+// blocks (the mean and std of the cost matrix, geometry.py:112-136, and
+// scale_cost="mean" / "max_cost", pointcloud.py:342-383, for costs without a
+// closed form): float64 partials per block, the maximum through its float
+// bits (costs are >= 0).
+template <int PHI>
+__global__ __launch_bounds__(256) void cost_summary_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                                           int64_t n, int64_t m, int d, CostForm cf,
+                                                           double* __restrict__ partial, int64_t blocks,
+                                                           uint32_t* __restrict__ max_bits) {
+  __shared__ __align__(16) float xs[kTcChunk][kTcPad];
+  __shared__ __align__(16) float ys[kTcChunk][kTcPad];
+  __shared__ double red[2][256];
+  __shared__ uint32_t s_max;
+  const int tid = threadIdx.x;
+  const int64_t i0 = static_cast<int64_t>(blockIdx.y) * kTcTile;
+  const int64_t j0 = static_cast<int64_t>(blockIdx.x) * kTcTile;
+  if (tid == 0) s_max = 0u;
+  float acc[4][4] = {};
+  tc_cost_acc<PHI>(x, y, n, m, d, i0, j0, xs, ys, tid, acc, cf.p);  // begins with a barrier
+  const int r0 = (tid >> 4) * 4, c0 = (tid & 15) * 4;
+  double s1 = 0.0, s2 = 0.0;
+  uint32_t mx = 0u;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      if (i0 + r0 + a >= n || j0 + c0 + b >= m) continue;
+      const float c = cost_psi(acc[a][b], cf.alpha, cf.beta);
+      s1 += c;
+      s2 += static_cast<double>(c) * c;
+      const uint32_t u = __float_as_uint(c);
+      mx = u > mx ? u : mx;
+    }
+  red[0][tid] = s1;
+  red[1][tid] = s2;
+  atomicMax(&s_max, mx);
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      red[0][tid] += red[0][tid + w];
+      red[1][tid] += red[1][tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int64_t bid = static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x;
+    partial[bid] = red[0][0];
+    partial[blocks + bid] = red[1][0];
+    atomicMax(max_bits, s_max);
   }
 }
 
@@ -1779,7 +1883,8 @@ struct KernelCfg {
   HalfStepFn fn;
   BatchedFn bfn;
   int cols() const { return 16 * nb * w; }
-  int frag_chunks() const { return prec == 1 ? kk : (kk + 3) / 4; }  // 1 KiB chunks per 16 rows
+  // 1 KiB chunks per 16 rows (none for the cost stream, which reads the points)
+  int frag_chunks() const { return prec == 1 ? kk : (prec == 0 ? (kk + 3) / 4 : 0); }
 };
 
 template <int PREC, int KK, int NB, int MT, int W>
@@ -1797,6 +1902,14 @@ int kernel_variants(int prec, int d, KernelCfg* out, int max_out) {
     ++n;
   };
   if (d < 1 || d > 256) return 0;
+  // costs other than the squared Euclidean: the vector-ALU stream, 64 columns
+  // by 64 rows per step (kernel_key 10 + phi)
+  switch (prec) {
+    case 11: add(make_cfg<2, 1, 1, 4, 4>()); return n;
+    case 12: add(make_cfg<2, 2, 1, 4, 4>()); return n;
+    case 13: add(make_cfg<2, 3, 1, 4, 4>()); return n;
+    default: break;
+  }
   if (prec == OTTX_PREC_BF16_SPLIT) {
     const int kb = (d + 31) / 32;
     switch (kb) {
@@ -2065,7 +2178,7 @@ struct Plan {
 
 bool make_plan(const ottx_problem* pb, int tile_config, Plan* P) {
   int tm = 4;
-  if (!select_kernel(pb->precision, pb->d, tile_config, &P->k, &tm)) return false;
+  if (!select_kernel(kernel_key(pb), pb->d, tile_config, &P->k, &tm)) return false;
   const int cus = device_cus();
   // Every column tile (16 NB W points) and streamed tile (16 MT rows) must
   // divide the row padding, so tiles never straddle the end of an array.
@@ -2080,7 +2193,7 @@ bool make_plan(const ottx_problem* pb, int tile_config, Plan* P) {
 // Workspace large enough for every tile configuration of the problem.
 size_t max_workspace(const ottx_problem* pb) {
   KernelCfg v[8];
-  const int n = kernel_variants(pb->precision, pb->d, v, 8);
+  const int n = kernel_variants(kernel_key(pb), pb->d, v, 8);
   size_t best = 0;
   for (int vi = 0; vi < n; ++vi)
     for (int ti = 0; ti < kNumTargets; ++ti) {
@@ -2099,8 +2212,14 @@ int check_problem(const ottx_problem* pb) {
   if (!(pb->tau_a > 0.0 && pb->tau_a <= 1.0) || !(pb->tau_b > 0.0 && pb->tau_b <= 1.0)) return OTTX_EINVAL;
   if (pb->precision != OTTX_PREC_FP32 && pb->precision != OTTX_PREC_BF16_SPLIT) return OTTX_EINVAL;
   if (pb->epsilon_statistic != OTTX_EPS_STAT_MEAN && pb->epsilon_statistic != OTTX_EPS_STAT_STD) return OTTX_EINVAL;
+  CostForm cf;
+  if (!cost_form(pb->cost, pb->cost_p, &cf)) return OTTX_EINVAL;
+  // without a closed form for mean(C) and std(C) the caller resolves a
+  // relative epsilon (ottx_cost_summary)
+  if (pb->cost != OTTX_COST_SQEUCLIDEAN && pb->relative_epsilon > 0.0) return OTTX_EUNSUPPORTED;
   return OTTX_OK;
 }
+
 
 #define OTTX_HIP(call)                              \
   do {                                              \
@@ -2190,7 +2309,7 @@ int prepare(const ottx_problem* pb, const KernelCfg& k, const Layout& L, void* w
   pa.d = pb->d;
   pa.chunks = chunks;
   pa.prec = k.prec;
-  pa.s = pb->inv_scale_cost;
+  pa.s = k.prec == 2 ? 0.0 : pb->inv_scale_cost;  // no norm term in the cost stream's exponent
   auto blocks = [](int64_t threads) { return static_cast<int>((threads + 255) / 256); };
   int cur = 0;
   for (int side = 0; side < 2; ++side) {
@@ -2260,7 +2379,17 @@ HalfStepArgs base_args(const ottx_problem* pb, const KernelCfg& k, const PassPla
   a.scal = P.scal;
   a.pack_scal = P.scal;
   a.side = side_out;
-  (void)k;
+  if (k.prec == 2) {
+    // the cost stream reads the raw points and has no norm term
+    CostForm cf;
+    cost_form(pb->cost, pb->cost_p, &cf);
+    a.in_frag = g_upd ? pb->x : pb->y;
+    a.out_frag = g_upd ? pb->y : pb->x;
+    a.d = pb->d;
+    a.cost_p = cf.p;
+    a.cost_alpha = cf.alpha;
+    a.cost_beta = cf.beta;
+  }
   return a;
 }
 
@@ -2424,9 +2553,15 @@ size_t ottx_cost_median_workspace_bytes(void) {
   return align_up(sizeof(MedianState)) + align_up(sizeof(unsigned long long) * kMedBins);
 }
 
-int ottx_cost_median(int64_t n, int64_t m, int32_t d, const float* x, const float* y, double* out_host,
-                     void* workspace, size_t workspace_bytes, void* stream) {
-  if (n <= 0 || m <= 0 || d <= 0 || !x || !y || !out_host || !workspace) return OTTX_EINVAL;
+int ottx_cost_median(const ottx_problem* pb, double* out_host, void* workspace, size_t workspace_bytes,
+                     void* stream) {
+  int rc = check_problem(pb);
+  if (rc != OTTX_OK) return rc;
+  if (!out_host || !workspace) return OTTX_EINVAL;
+  const int64_t n = pb->n, m = pb->m;
+  const int d = pb->d;
+  CostForm cf;
+  if (!cost_form(pb->cost, pb->cost_p, &cf)) return OTTX_EINVAL;
   if (n > (int64_t{1} << 40) / m) return OTTX_EINVAL;  // n m <= 2^40 entries
   if (workspace_bytes < ottx_cost_median_workspace_bytes()) return OTTX_EWORKSPACE;
   if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
@@ -2436,15 +2571,23 @@ int ottx_cost_median(int64_t n, int64_t m, int32_t d, const float* x, const floa
   unsigned long long* hist =
       reinterpret_cast<unsigned long long*>(static_cast<char*>(workspace) + align_up(sizeof(MedianState)));
   const dim3 grid(static_cast<unsigned>((m + kTcTile - 1) / kTcTile), static_cast<unsigned>((n + kTcTile - 1) / kTcTile));
+  const float* x = pb->x;
+  const float* y = pb->y;
   hipLaunchKernelGGL(median_init_kernel, dim3(1), dim3(256), 0, st, ms, hist,
                      static_cast<unsigned long long>(n) * static_cast<unsigned long long>(m));
-  hipLaunchKernelGGL(median_pass_kernel<0>, grid, dim3(256), 0, st, x, y, n, m, d, ms, hist);
-  hipLaunchKernelGGL(median_select_kernel<0>, dim3(1), dim3(1024), 0, st, ms, hist);
-  hipLaunchKernelGGL(median_pass_kernel<1>, grid, dim3(256), 0, st, x, y, n, m, d, ms, hist);
-  hipLaunchKernelGGL(median_select_kernel<1>, dim3(1), dim3(1024), 0, st, ms, hist);
-  hipLaunchKernelGGL(median_pass_kernel<2>, grid, dim3(256), 0, st, x, y, n, m, d, ms, hist);
-  hipLaunchKernelGGL(median_select_kernel<2>, dim3(1), dim3(1024), 0, st, ms, hist);
-  hipLaunchKernelGGL(median_pass_kernel<3>, grid, dim3(256), 0, st, x, y, n, m, d, ms, hist);
+  auto passes = [&](auto phi) {
+    constexpr int PHI = decltype(phi)::value;
+    hipLaunchKernelGGL((median_pass_kernel<0, PHI>), grid, dim3(256), 0, st, x, y, n, m, d, ms, hist, cf);
+    hipLaunchKernelGGL(median_select_kernel<0>, dim3(1), dim3(1024), 0, st, ms, hist);
+    hipLaunchKernelGGL((median_pass_kernel<1, PHI>), grid, dim3(256), 0, st, x, y, n, m, d, ms, hist, cf);
+    hipLaunchKernelGGL(median_select_kernel<1>, dim3(1), dim3(1024), 0, st, ms, hist);
+    hipLaunchKernelGGL((median_pass_kernel<2, PHI>), grid, dim3(256), 0, st, x, y, n, m, d, ms, hist, cf);
+    hipLaunchKernelGGL(median_select_kernel<2>, dim3(1), dim3(1024), 0, st, ms, hist);
+    hipLaunchKernelGGL((median_pass_kernel<3, PHI>), grid, dim3(256), 0, st, x, y, n, m, d, ms, hist, cf);
+  };
+  if (cf.phi == 1) passes(std::integral_constant<int, 1>{});
+  else if (cf.phi == 2) passes(std::integral_constant<int, 2>{});
+  else passes(std::integral_constant<int, 3>{});
   OTTX_LAUNCHED();
   MedianState host;
   OTTX_HIP(hipMemcpyAsync(&host, ms, sizeof(host), hipMemcpyDeviceToHost, st));
@@ -2462,6 +2605,57 @@ int ottx_cost_median(int64_t n, int64_t m, int32_t d, const float* x, const floa
   return OTTX_OK;
 }
 
+size_t ottx_cost_summary_workspace_bytes(int64_t n, int64_t m) {
+  if (n <= 0 || m <= 0) return 0;
+  const int64_t blocks = ((n + kTcTile - 1) / kTcTile) * ((m + kTcTile - 1) / kTcTile);
+  return static_cast<size_t>(256 + align_up(static_cast<size_t>(blocks) * 2 * sizeof(double)));
+}
+
+int ottx_cost_summary(const ottx_problem* pb, double* out_host, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+  int rc = check_problem(pb);
+  if (rc != OTTX_OK) return rc;
+  if (!out_host || !workspace) return OTTX_EINVAL;
+  CostForm cf;
+  if (!cost_form(pb->cost, pb->cost_p, &cf)) return OTTX_EINVAL;
+  if (workspace_bytes < ottx_cost_summary_workspace_bytes(pb->n, pb->m)) return OTTX_EWORKSPACE;
+  if (reinterpret_cast<uintptr_t>(workspace) & 255) return OTTX_EWORKSPACE;
+  const int64_t gx = (pb->m + kTcTile - 1) / kTcTile, gy = (pb->n + kTcTile - 1) / kTcTile;
+  if (gy > 65535) return OTTX_EUNSUPPORTED;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  char* base = static_cast<char*>(workspace);
+  double* result = reinterpret_cast<double*>(base);            // [0] sum, [1] sum of squares
+  uint32_t* max_bits = reinterpret_cast<uint32_t*>(base + 64);
+  double* partial = reinterpret_cast<double*>(base + 256);
+  const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>(gy));
+  hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, st, max_bits, 1);
+  if (cf.phi == 1)
+    hipLaunchKernelGGL(cost_summary_kernel<1>, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d, cf,
+                       partial, gx * gy, max_bits);
+  else if (cf.phi == 2)
+    hipLaunchKernelGGL(cost_summary_kernel<2>, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d, cf,
+                       partial, gx * gy, max_bits);
+  else
+    hipLaunchKernelGGL(cost_summary_kernel<3>, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d, cf,
+                       partial, gx * gy, max_bits);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(1024), 0, st, static_cast<const double*>(partial), gx * gy,
+                     result);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(1024), 0, st,
+                     static_cast<const double*>(partial + gx * gy), gx * gy, result + 1);
+  OTTX_LAUNCHED();
+  double host[9];
+  OTTX_HIP(hipMemcpyAsync(host, result, sizeof(host), hipMemcpyDeviceToHost, st));
+  OTTX_HIP(hipStreamSynchronize(st));
+  uint32_t mb;
+  std::memcpy(&mb, &host[8], sizeof(mb));
+  float mx;
+  std::memcpy(&mx, &mb, sizeof(mx));
+  out_host[0] = host[0];
+  out_host[1] = host[1];
+  out_host[2] = mx;
+  return OTTX_OK;
+}
+
 size_t ottx_op_workspace_bytes(const ottx_problem* pb) {
   if (check_problem(pb) != OTTX_OK) return 0;
   return max_workspace(pb);
@@ -2474,8 +2668,18 @@ int ottx_transport_matrix(const ottx_problem* pb, const float* f, const float* g
   if (!f || !g || !out || !(eps > 0.0)) return OTTX_EINVAL;
   hipStream_t st = static_cast<hipStream_t>(stream);
   const dim3 grid(static_cast<unsigned>((pb->m + 63) / 64), static_cast<unsigned>((pb->n + 3) / 4));
-  hipLaunchKernelGGL(transport_matrix_kernel, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d,
-                     static_cast<float>(pb->inv_scale_cost), f, g, static_cast<float>(1.0 / eps), out);
+  CostForm cf;
+  if (!cost_form(pb->cost, pb->cost_p, &cf)) return OTTX_EINVAL;
+  const float s = static_cast<float>(pb->inv_scale_cost), inv_eps = static_cast<float>(1.0 / eps);
+  if (cf.phi == 1)
+    hipLaunchKernelGGL(transport_matrix_kernel<1>, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d, s, f,
+                       g, inv_eps, out, cf);
+  else if (cf.phi == 2)
+    hipLaunchKernelGGL(transport_matrix_kernel<2>, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d, s, f,
+                       g, inv_eps, out, cf);
+  else
+    hipLaunchKernelGGL(transport_matrix_kernel<3>, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d, s, f,
+                       g, inv_eps, out, cf);
   OTTX_LAUNCHED();
   return OTTX_OK;
 }
@@ -2504,12 +2708,30 @@ int ottx_transport_cost(const ottx_problem* pb, const float* f, const float* g, 
   const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>(gy));
   const float s = static_cast<float>(pb->inv_scale_cost);
   const float inv_eps = static_cast<float>(1.0 / eps);
-  if (same)
-    hipLaunchKernelGGL(transport_cost_kernel<true>, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d, s, f,
-                       g, inv_eps, pb->x, pb->y, pb->d, s, partial);
-  else
-    hipLaunchKernelGGL(transport_cost_kernel<false>, grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d, s, f,
-                       g, inv_eps, x_other, y_other, d_other, static_cast<float>(other_inv_scale_cost), partial);
+  CostForm cf;
+  if (!cost_form(pb->cost, pb->cost_p, &cf)) return OTTX_EINVAL;
+  const float so = static_cast<float>(other_inv_scale_cost);
+  auto launch = [&](auto same_c, auto phi_c) {
+    constexpr bool SAME = decltype(same_c)::value;
+    constexpr int PHI = decltype(phi_c)::value;
+    hipLaunchKernelGGL((transport_cost_kernel<SAME, PHI>), grid, dim3(256), 0, st, pb->x, pb->y, pb->n, pb->m, pb->d,
+                       s, f, g, inv_eps, SAME ? pb->x : x_other, SAME ? pb->y : y_other, SAME ? pb->d : d_other,
+                       SAME ? s : so, partial, cf);
+  };
+  using T = std::true_type;
+  using F = std::false_type;
+  using P1 = std::integral_constant<int, 1>;
+  using P2 = std::integral_constant<int, 2>;
+  using P3 = std::integral_constant<int, 3>;
+  if (same) {
+    if (cf.phi == 1) launch(T{}, P1{});
+    else if (cf.phi == 2) launch(T{}, P2{});
+    else launch(T{}, P3{});
+  } else {
+    if (cf.phi == 1) launch(F{}, P1{});
+    else if (cf.phi == 2) launch(F{}, P2{});
+    else launch(F{}, P3{});
+  }
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(1024), 0, st, static_cast<const double*>(partial), gx * gy,
                      result);
   OTTX_LAUNCHED();
@@ -2545,7 +2767,7 @@ int ottx_cost_gradient(const ottx_problem* pb, const float* f, const float* g, d
   int rc = check_problem(pb);
   if (rc != OTTX_OK) return rc;
   if (!f || !g || !grad || !workspace || !(eps > 0.0) || (side != 0 && side != 1)) return OTTX_EINVAL;
-  if (pb->d > kGdMaxD) return OTTX_EUNSUPPORTED;
+  if (pb->d > kGdMaxD || pb->cost != OTTX_COST_SQEUCLIDEAN) return OTTX_EUNSUPPORTED;
   if (workspace_bytes < ottx_cost_gradient_workspace_bytes(pb, side)) return OTTX_EWORKSPACE;
   const bool rows = side == 0;
   const int64_t np = rows ? pb->n : pb->m, nq = rows ? pb->m : pb->n;
@@ -3172,6 +3394,7 @@ ApplyCfg apply_cfg() {
 // One tile shape per fragment layout; wide features keep fewer column blocks
 // and rows in registers.
 bool select_apply(int prec, int kk, ApplyCfg* out) {
+  if (prec != OTTX_PREC_FP32 && prec != OTTX_PREC_BF16_SPLIT) return false;  // the MFMA cross term only
   if (prec == OTTX_PREC_BF16_SPLIT) {
     switch (kk) {
       case 1: *out = apply_cfg<1, 1, 2, 2>(); return true;

@@ -40,7 +40,9 @@ class PointCloud:
 
   Args mirror geometry/pointcloud.py:54-72 and geometry/geometry.py:76-89:
     x: ``[n, d]`` points.  y: ``[m, d]`` points, ``None`` means ``x``.
-    cost_fn: only :class:`~ott_amd.geometry.costs.SqEuclidean` (the default).
+    cost_fn: :class:`~ott_amd.geometry.costs.SqEuclidean` (the default, MFMA
+      cross term), or ``PNormP``, ``SqPNorm``, ``EuclideanP``, ``Euclidean``
+      (summed from the differences on the vector ALUs, online).
     batch_size: accepted for signature compatibility; the kernels are always
       online (the cost is recomputed per tile and never stored).
     scale_cost: a positive float ``s`` so that ``C = ||x - y||^2 / s``.
@@ -74,6 +76,9 @@ class PointCloud:
       raise ValueError(f"x and y must be [n, d] and [m, d], got {tuple(self.x.shape)} "
                        f"and {tuple(self.y.shape)}")
     self.cost_fn = costs.SqEuclidean() if cost_fn is None else cost_fn
+    # the kernels evaluate factor * cost(kind, p) (factor 0.5 for PNormP(2)
+    # and SqPNorm(2), which run as half the squared Euclidean cost)
+    self._cost_kind, self._cost_p, self._cost_factor = self.cost_fn.kernel_spec()
     if batch_size is not None:
       assert batch_size > 0, f"`batch_size={batch_size}` must be positive."
     self._batch_size = batch_size
@@ -88,6 +93,7 @@ class PointCloud:
     self._std_cache: Optional[float] = None
     self._inv_scale_cache: Optional[float] = None
     self._median_cache: Optional[float] = None
+    self._summary_cache: Optional[Tuple[float, float, float]] = None
 
   # -- shape / dtype ---------------------------------------------------------
   @property
@@ -101,6 +107,12 @@ class PointCloud:
   @property
   def is_squared_euclidean(self) -> bool:
     return isinstance(self.cost_fn, costs.SqEuclidean)
+
+  @property
+  def _closed_form(self) -> bool:
+    """The cost is (a multiple of) the squared Euclidean one: MFMA half-steps,
+    closed-form mean and std of the cost matrix."""
+    return self._cost_kind == costs.COST_SQEUCLIDEAN
 
   @property
   def is_online(self) -> bool:
@@ -141,45 +153,77 @@ class PointCloud:
       return self._inv_scale_cache
     raise ValueError(f"Scaling {s} not implemented.")
 
+  @property
+  def _kernel_scale(self) -> float:
+    """The s the kernels multiply their cost by: inv_scale_cost times the
+    factor of a cost that runs as a multiple of the squared Euclidean one."""
+    return self.inv_scale_cost * self._cost_factor
+
   def _cost_summary(self, mode: str) -> float:
     """The unscaled quantity a scale_cost mode divides by (pointcloud.py:343-377)."""
     lib = _lib.load_library()
     n, m = self.shape
     d = self.x.shape[1]
+    if mode == "median":
+      return self._unscaled_median()
+    if not self._closed_form and mode in ("mean", "max_cost"):
+      total, _, c_max = self._cost_summary_values()
+      return c_max if mode == "max_cost" else total / (float(n) * float(m))
     if mode == "mean":
       nbytes = lib.ottx_mean_cost_workspace_bytes(n, m, d)
       ws = _lib.workspace(nbytes, self.device)
       out = ctypes.c_double(0.0)
       _lib.check("ottx_mean_cost", lib.ottx_mean_cost(
-          n, m, d, self.x.data_ptr(), self.y.data_ptr(), 1.0, ctypes.byref(out), ws.data_ptr(), ws.numel(),
-          _lib.stream_ptr()))
+          n, m, d, self.x.data_ptr(), self.y.data_ptr(), self._cost_factor, ctypes.byref(out), ws.data_ptr(),
+          ws.numel(), _lib.stream_ptr()))
       return float(out.value)
-    if mode == "median":
-      return self._unscaled_median()
     ws = _lib.workspace(256, self.device)
     out = (ctypes.c_double * 3)()
     _lib.check("ottx_cost_maxima", lib.ottx_cost_maxima(
         n, m, d, self.x.data_ptr(), self.y.data_ptr(), 1 if mode == "max_cost" else 0, out, ws.data_ptr(),
         ws.numel(), _lib.stream_ptr()))
+    if not self.is_squared_euclidean and mode in ("max_norm", "max_bound"):
+      raise NotImplementedError(f"scale_cost={mode!r} needs the squared-Euclidean cost (cost_fn.norm)")
     x_max, y_max, c_max = float(out[0]), float(out[1]), float(out[2])
     if mode == "max_cost":
-      return c_max
+      return self._cost_factor * c_max
     if mode == "max_norm":
       return max(x_max, y_max)
     # max_bound for the squared-Euclidean cost
     return x_max + y_max + 2.0 * math.sqrt(x_max * y_max)
 
+  def _cost_problem(self) -> _lib.Problem:
+    """The clouds and the cost for the summary entry points (no epsilon)."""
+    n, m = self.shape
+    return _lib.Problem(n=n, m=m, d=self.x.shape[1], precision=_lib.PRECISIONS[self.precision],
+                        x=self.x.data_ptr(), y=self.y.data_ptr(), a=None, b=None, epsilon=1.0,
+                        inv_scale_cost=1.0, tau_a=1.0, tau_b=1.0, relative_epsilon=0.0,
+                        epsilon_statistic=_lib.EPS_STAT_MEAN, cost=self._cost_kind, cost_p=self._cost_p)
+
   def _unscaled_median(self) -> float:
     if self._median_cache is None:
       lib = _lib.load_library()
-      n, m = self.shape
       ws = _lib.workspace(lib.ottx_cost_median_workspace_bytes(), self.device)
       out = ctypes.c_double(0.0)
+      pb = self._cost_problem()
       _lib.check("ottx_cost_median", lib.ottx_cost_median(
-          n, m, self.x.shape[1], self.x.data_ptr(), self.y.data_ptr(), ctypes.byref(out), ws.data_ptr(),
-          ws.numel(), _lib.stream_ptr()))
-      self._median_cache = float(out.value)
+          ctypes.byref(pb), ctypes.byref(out), ws.data_ptr(), ws.numel(), _lib.stream_ptr()))
+      self._median_cache = self._cost_factor * float(out.value)
     return self._median_cache
+
+  def _cost_summary_values(self) -> Tuple[float, float, float]:
+    """(sum C, sum C^2, max C) of the unscaled cost matrix, one GPU pass."""
+    if self._summary_cache is None:
+      lib = _lib.load_library()
+      n, m = self.shape
+      ws = _lib.workspace(lib.ottx_cost_summary_workspace_bytes(n, m), self.device)
+      out = (ctypes.c_double * 3)()
+      pb = self._cost_problem()
+      _lib.check("ottx_cost_summary", lib.ottx_cost_summary(
+          ctypes.byref(pb), out, ws.data_ptr(), ws.numel(), _lib.stream_ptr()))
+      f = self._cost_factor
+      self._summary_cache = (f * float(out[0]), f * f * float(out[1]), f * float(out[2]))
+    return self._summary_cache
 
   @property
   def median_cost_matrix(self) -> float:
@@ -189,7 +233,11 @@ class PointCloud:
 
   @property
   def mean_cost_matrix(self) -> float:
-    """geometry.py:112-117 via pointcloud.py:170-186, closed form on the GPU."""
+    """geometry.py:112-117 via pointcloud.py:170-186, closed form on the GPU
+    for the squared Euclidean cost, one pass over the pairs for the others."""
+    if self._mean_cache is None and not self._closed_form:
+      n, m = self.shape
+      self._mean_cache = self.inv_scale_cost * self._cost_summary_values()[0] / (float(n) * float(m))
     if self._mean_cache is None:
       lib = _lib.load_library()
       n, m = self.shape
@@ -198,7 +246,7 @@ class PointCloud:
       ws = _lib.workspace(nbytes, self.device)
       out = ctypes.c_double(0.0)
       _lib.check("ottx_mean_cost", lib.ottx_mean_cost(
-          n, m, d, self.x.data_ptr(), self.y.data_ptr(), self.inv_scale_cost,
+          n, m, d, self.x.data_ptr(), self.y.data_ptr(), self._kernel_scale,
           ctypes.byref(out), ws.data_ptr(), ws.numel(), _lib.stream_ptr()))
       self._mean_cache = float(out.value)
     return self._mean_cache
@@ -206,7 +254,14 @@ class PointCloud:
   @property
   def std_cost_matrix(self) -> float:
     """geometry.py:119-136: sqrt(relu(mean(C^2) - mean(C)^2)), in closed form
-    on the GPU from the clouds' moments (the reference forms C^2)."""
+    on the GPU from the clouds' moments for the squared Euclidean cost (the
+    reference forms C^2), from float64 sums over the pairs for the others."""
+    if self._std_cache is None and not self._closed_form:
+      n, m = self.shape
+      s1, s2, _ = self._cost_summary_values()
+      nm = float(n) * float(m)
+      var = s2 / nm - (s1 / nm) ** 2
+      self._std_cache = self.inv_scale_cost * math.sqrt(max(var, 0.0))
     if self._std_cache is None:
       lib = _lib.load_library()
       n, m = self.shape
@@ -217,7 +272,7 @@ class PointCloud:
       ws = _lib.workspace(nbytes, self.device)
       out = (ctypes.c_double * 2)()
       _lib.check("ottx_cost_moments", lib.ottx_cost_moments(
-          n, m, d, self.x.data_ptr(), self.y.data_ptr(), self.inv_scale_cost, out, ws.data_ptr(), ws.numel(),
+          n, m, d, self.x.data_ptr(), self.y.data_ptr(), self._kernel_scale, out, ws.data_ptr(), ws.numel(),
           _lib.stream_ptr()))
       if self._mean_cache is None:
         self._mean_cache = float(out[0])
@@ -268,7 +323,7 @@ class PointCloud:
     device without a host round trip.
     """
     stat = self._relative_statistic()
-    if self._eps_cache is None and stat is not None:
+    if self._eps_cache is None and stat is not None and self._closed_form:
       mult = (eps_scheduler.DEFAULT_EPSILON_SCALE if self._epsilon_init is None
               else float(self._epsilon_init))
       return 0.0, mult, stat
@@ -312,16 +367,15 @@ class PointCloud:
                tau_a: float = 1.0, tau_b: float = 1.0, epsilon: Optional[float] = None,
                relative_epsilon: float = 0.0,
                epsilon_statistic: int = _lib.EPS_STAT_MEAN) -> Tuple[_lib.Problem, Tuple]:
-    if not self.is_squared_euclidean:
-      raise NotImplementedError("ott_amd accelerates the squared-Euclidean cost only")
     n, m = self.shape
     # None weights are passed as NULL: the library uses uniform 1/n, 1/m
     pb = _lib.Problem(
         n=n, m=m, d=self.x.shape[1], precision=_lib.PRECISIONS[self.precision],
         x=self.x.data_ptr(), y=self.y.data_ptr(), a=_lib.ptr(a), b=_lib.ptr(b),
         epsilon=float(self.epsilon if epsilon is None else epsilon),
-        inv_scale_cost=self.inv_scale_cost, tau_a=float(tau_a), tau_b=float(tau_b),
-        relative_epsilon=float(relative_epsilon), epsilon_statistic=int(epsilon_statistic))
+        inv_scale_cost=self._kernel_scale, tau_a=float(tau_a), tau_b=float(tau_b),
+        relative_epsilon=float(relative_epsilon), epsilon_statistic=int(epsilon_statistic),
+        cost=self._cost_kind, cost_p=self._cost_p)
     return pb, (a, b)
 
   def _op(self, op: int, f, g, eps: float, axis: int,
@@ -394,8 +448,11 @@ class PointCloud:
       xo = yo = None
       d_o, s_o = 0, 0.0
     else:
+      if not other._closed_form:
+        raise NotImplementedError("transport_cost_at_geom: another geometry's cost must be the squared "
+                                  "Euclidean one (or a multiple of it)")
       xo, yo = other.x, other.y
-      d_o, s_o = int(xo.shape[1]), float(other.inv_scale_cost)
+      d_o, s_o = int(xo.shape[1]), float(other._kernel_scale)
     _lib.check("ottx_transport_cost", lib.ottx_transport_cost(
         ctypes.byref(pb), f.data_ptr(), g.data_ptr(), float(self.epsilon), _lib.ptr(xo), _lib.ptr(yo), d_o, s_o,
         ctypes.byref(out), ws.data_ptr(), nbytes, _lib.stream_ptr()))
@@ -473,10 +530,12 @@ class PointCloud:
     C = cost_1 cost_2^T with cost_1 = [|x|^2, 1, -2 x] s and cost_2 = [1, |y|^2, y],
     here on points centred by the mean of both clouds (C is unchanged by a
     common shift, and the terms cancel less)."""
+    if not self._closed_form:
+      raise NotImplementedError("low_rank_cost_factors needs the squared-Euclidean cost (or a multiple of it)")
     mu = 0.5 * (self.x.double().mean(0) + self.y.double().mean(0))
     x = self.x.double() - mu
     y = self.y.double() - mu
-    s = self.inv_scale_cost
+    s = self._kernel_scale
     ones_x = torch.ones((x.shape[0], 1), dtype=torch.float64, device=self.device)
     ones_y = torch.ones((y.shape[0], 1), dtype=torch.float64, device=self.device)
     cost_1 = torch.cat([(x * x).sum(1, keepdim=True), ones_x, -2.0 * x], dim=1) * s

@@ -329,6 +329,8 @@ class DeepSetsInitializer(SinkhornInitializer):
 
   def _potential(self, ot_prob, side: str, out: Optional[torch.Tensor] = None) -> torch.Tensor:
     geom = ot_prob.geom
+    if not geom.is_squared_euclidean:
+      raise NotImplementedError("the learned warm start predicts potentials of the squared-Euclidean cost")
     inv_s = geom.inv_scale_cost  # the network works in units of |x - y|^2
     eps_abs, eps_rel, stat = geom._epsilon_spec()
     if eps_abs == 0.0 and stat != _lib.EPS_STAT_MEAN:

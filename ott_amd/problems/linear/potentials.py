@@ -38,8 +38,9 @@ class _CTransform:
   (w = 1 when ``weights`` is None)."""
 
   def __init__(self, support: torch.Tensor, h: torch.Tensor, weights: Optional[torch.Tensor], eps: float,
-               side: int, precision: str):
+               side: int, precision: str, cost_fn=None):
     self.support = support
+    self.cost_fn = cost_fn
     self.eps = float(eps)
     self.side = side  # 1: f (x against the y cloud), 0: g (y against the x cloud)
     self.precision = precision
@@ -53,8 +54,8 @@ class _CTransform:
 
   def _geom(self, pts: torch.Tensor) -> PointCloud:
     if self.side == 1:
-      return PointCloud(pts, self.support, epsilon=self.eps, precision=self.precision)
-    return PointCloud(self.support, pts, epsilon=self.eps, precision=self.precision)
+      return PointCloud(pts, self.support, cost_fn=self.cost_fn, epsilon=self.eps, precision=self.precision)
+    return PointCloud(self.support, pts, cost_fn=self.cost_fn, epsilon=self.eps, precision=self.precision)
 
   def values(self, pts: torch.Tensor):
     geom = self._geom(pts)
@@ -118,7 +119,7 @@ def from_solution(geom: PointCloud, f: torch.Tensor, g: torch.Tensor, a: torch.T
                   epsilon: Optional[float] = None) -> DualPotentials:
   """SinkhornOutput.to_dual_potentials (sinkhorn.py:480-499)."""
   eps = geom.epsilon if epsilon is None else float(epsilon)
-  f_fn = _CTransform(geom.y, g, b, eps, side=1, precision=geom.precision)
+  f_fn = _CTransform(geom.y, g, b, eps, side=1, precision=geom.precision, cost_fn=geom.cost_fn)
   del a  # the reference's g transform takes no weights on the solved rows
-  g_fn = _CTransform(geom.x, f, None, eps, side=0, precision=geom.precision)
+  g_fn = _CTransform(geom.x, f, None, eps, side=0, precision=geom.precision, cost_fn=geom.cost_fn)
   return DualPotentials(f_fn, g_fn, cost_fn=geom.cost_fn)

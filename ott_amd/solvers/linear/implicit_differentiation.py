@@ -142,12 +142,14 @@ class ImplicitDiff:
   def gradient(self, gr, geom: PointCloud, f: torch.Tensor, g: torch.Tensor, a: torch.Tensor, b: torch.Tensor,
                tau_a: float = 1.0, tau_b: float = 1.0):
     """implicit_differentiation.py:253-269: (dx, dy, da, db) in float64."""
+    if not geom._closed_form:
+      raise NotImplementedError("implicit gradients need the squared-Euclidean cost (or a multiple of it)")
     op = geom.transport_operator(f, g)
     v_a, v_b = self.solve(gr, geom, f, g, a, b, tau_a, tau_b, op=op)
     dev = geom.device
     n, m = geom.shape
     x, y = geom.x.double(), geom.y.double()
-    s = float(geom.inv_scale_cost)
+    s = float(geom._kernel_scale)
     eps = float(geom.epsilon)
     one_n = torch.ones((n,), dtype=torch.float32, device=dev)
     one_m = torch.ones((m,), dtype=torch.float32, device=dev)

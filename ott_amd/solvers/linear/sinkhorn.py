@@ -126,8 +126,8 @@ class SinkhornOutput(NamedTuple):
     C'_ij from the two pairs of clouds and sums P_ij C'_ij in float64, never
     materialising P or C' (``ottx_transport_cost``).
     """
-    if not isinstance(other_geom, PointCloud) or not other_geom.is_squared_euclidean:
-      raise NotImplementedError("transport_cost_at_geom needs a squared-Euclidean PointCloud")
+    if not isinstance(other_geom, PointCloud):
+      raise NotImplementedError("transport_cost_at_geom needs a PointCloud")
     if tuple(other_geom.shape) != tuple(self.geom.shape):
       raise ValueError(f"other_geom must be {tuple(self.geom.shape)}, got {tuple(other_geom.shape)}")
     return self.geom.transport_cost(self.f, self.g, None if other_geom is self.geom else other_geom)
@@ -205,8 +205,7 @@ class SinkhornOutput(NamedTuple):
     of x_i against y_i (pointcloud diag_cost)."""
     n, m = self.geom.shape
     assert n == m, "Problem must be square for transport matrix to have a diag."
-    diff = self.geom.x.double() - self.geom.y.double()
-    diag_cost = self.geom.inv_scale_cost * (diff * diff).sum(1)
+    diag_cost = self.geom.inv_scale_cost * self.geom.cost_fn(self.geom.x.double(), self.geom.y.double())
     return torch.exp((self.f.double() + self.g.double() - diag_cost) / self.geom.epsilon).float()
 
 
@@ -388,7 +387,7 @@ def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
       min_iterations=int(solver.min_iterations),
       max_iterations=int(solver.max_iterations),
       momentum=float(solver.momentum.value),
-      tile_config=tune.lookup(geom.precision, geom.x.shape[1], n, m),
+      tile_config=tune.lookup(geom.precision, geom.x.shape[1], n, m) if geom._closed_form else -1,
       use_graph=1 if solver.use_graph else 0,
       init_mode=int(init_mode),
       recenter_potentials=1 if solver.recenter_potentials else 0,

@@ -36,13 +36,13 @@ def test_library_exports_every_declared_symbol():
   missing = [n for n in sorted(_declared()) if not hasattr(lib, n)]
   assert not missing, f"missing exports: {missing}"
   assert set(_lib.EXPORTED_SYMBOLS) == _declared()
-  assert lib.ottx_abi_version() == 3
+  assert lib.ottx_abi_version() == 4
   assert lib.ottx_strerror(-1) == b"invalid argument"
 
 
 def test_struct_layouts_match_header():
   from ott_amd import _lib
-  assert ctypes.sizeof(_lib.Problem) == 8 + 8 + 4 + 4 + 4 * 8 + 5 * 8 + 2 * 4
+  assert ctypes.sizeof(_lib.Problem) == 8 + 8 + 4 + 4 + 4 * 8 + 5 * 8 + 2 * 4 + 8
   assert ctypes.sizeof(_lib.SinkhornOpts) == 88 + 2 * 8
   assert ctypes.sizeof(_lib.SinkhornOut) == 3 * 8 + 3 * 8 + 4 * 4
   assert ctypes.sizeof(_lib.Batch) == 4 * 4 + 4 * 8 + 3 * 8 + 6 * 8

@@ -472,3 +472,37 @@ def test_median_cost_against_sorted_entries():
     want = c[k // 2] if k % 2 else 0.5 * (c[k // 2 - 1] + c[k // 2])
     assert math.isclose(oracle.median_cost(x, y), want, rel_tol=1e-12)
     assert math.isclose(oracle.median_cost(x, y, 0.5), 0.5 * want, rel_tol=1e-12)
+
+
+@pytest.mark.parametrize("cost_fn", [("pnormp", 1.0), ("pnormp", 1.5), ("pnormp", 2.0), ("sqpnorm", 1.0),
+                                     ("sqpnorm", 3.0), ("euclideanp", 1.0), ("euclideanp", 3.0), ("euclidean",)])
+def test_ti_cost_against_textbook_norms(cost_fn):
+  """ti_cost entry by entry against numpy.linalg.norm of x_i - y_j with the
+  costs' definitions (the p-norm, its p-th power over p, half its square, the
+  Euclidean norm and its powers), and the special cases that fix the
+  constants: PNormP(2) = SqPNorm(2) = ||z||^2 / 2, EuclideanP(2) = ||z||^2,
+  PNormP(1) = SqPNorm(1)^(1/2) 2^(1/2) = the l1 norm."""
+  rng = np.random.default_rng(7)
+  x = rng.normal(size=(6, 4))
+  y = rng.normal(size=(5, 4))
+  c = oracle.ti_cost(x, y, cost_fn, scale=0.7)
+  kind = cost_fn[0]
+  for i in range(6):
+    for j in range(5):
+      z = x[i] - y[j]
+      if kind == "pnormp":
+        want = np.linalg.norm(z, ord=cost_fn[1]) ** cost_fn[1] / cost_fn[1]
+      elif kind == "sqpnorm":
+        want = 0.5 * np.linalg.norm(z, ord=cost_fn[1]) ** 2
+      elif kind == "euclideanp":
+        want = np.linalg.norm(z) ** cost_fn[1]
+      else:
+        want = np.linalg.norm(z)
+      assert math.isclose(c[i, j], 0.7 * want, rel_tol=1e-12)
+  sq = oracle.sqeuclidean_cost(x, y)
+  np.testing.assert_allclose(oracle.ti_cost(x, y, ("pnormp", 2.0)), 0.5 * sq, rtol=1e-12)
+  np.testing.assert_allclose(oracle.ti_cost(x, y, ("sqpnorm", 2.0)), 0.5 * sq, rtol=1e-12)
+  np.testing.assert_allclose(oracle.ti_cost(x, y, ("euclideanp", 2.0)), sq, rtol=1e-12)
+  l1 = np.abs(x[:, None] - y[None]).sum(-1)
+  np.testing.assert_allclose(oracle.ti_cost(x, y, ("pnormp", 1.0)), l1, rtol=1e-12)
+  np.testing.assert_allclose(np.sqrt(2.0 * oracle.ti_cost(x, y, ("sqpnorm", 1.0))), l1, rtol=1e-12)
