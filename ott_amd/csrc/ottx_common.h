@@ -30,6 +30,7 @@ enum : uint32_t {
   kModeMarginal = 1u << 5,   // out_vec = marginal_from_potentials
   kModeCheckDone = 1u << 6,  // return at once when state->done is set
   kModeWriteApp = 1u << 7,   // out_vec = eps LSE_i((h_i - C_ij) / eps) as it is (recentred updates)
+  kModeErrOld = 1u << 8,     // kModeErrRow measures the old potential (parallel updates)
 };
 
 // Loop state shared by the kernels of one solve (device memory).

@@ -650,7 +650,7 @@ __device__ __forceinline__ void column_epilogue(const EpiScalars& e, float2 ms, 
     float nv = upd;
     if (e.momentum != 1.f) nv = (1.f - e.momentum) * (isfinite(h_old) ? h_old : 0.f) + e.momentum * upd;
     out_pot[j] = nv;
-    if (e.mode & kModeErrRow) h_marg = nv;
+    if ((e.mode & kModeErrRow) && !(e.mode & kModeErrOld)) h_marg = nv;
   }
   if (e.mode & kModeSaveOld) out_save[j] = h_old;
   if (e.mode & (kModeErrCol | kModeErrRow)) {

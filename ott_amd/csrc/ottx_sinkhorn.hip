@@ -1588,6 +1588,7 @@ __global__ void solve_finalize_kernel(const float* f, const float* __restrict__ 
         }
       }
       if (side == 1) g_out[i] = g_save[i];
+      if (side == 0 && f_out != f) f_out[i] = pot[i];
     }
   } else if (blockIdx.x == 0) {
     for (int i = tid; i < outer; i += blockDim.x) {
@@ -2095,7 +2096,7 @@ int launch_std(const float* x, int64_t n, const float* y, int64_t m, int d, cons
 }
 
 struct Layout {
-  size_t frag_x, frag_y, norm_x, norm_y, g_save, lse_x, lse_y, partial, counters, err_col, mass, err_row, state,
+  size_t frag_x, frag_y, norm_x, norm_y, g_save, f_save, lse_x, lse_y, partial, counters, err_col, mass, err_row, state,
       dual, dual_part, scal, stats_x, stats_y, stats_out, mu, w_x, w_y, moments, rc_app, rc_col, rc, total;
   int chunks_x, chunks_y;
   int64_t rows_chunk_x, rows_chunk_y;
@@ -2131,6 +2132,7 @@ Layout make_layout(const ottx_problem* pb, const KernelCfg& k, const PassPlan& g
   L.norm_x = take(n_pad * sizeof(float));
   L.norm_y = take(m_pad * sizeof(float));
   L.g_save = take(m_pad * sizeof(float));
+  L.f_save = take(n_pad * sizeof(float));  // parallel updates: the f the g-update reads
   L.lse_x = take(n_pad * sizeof(float));
   L.lse_y = take(m_pad * sizeof(float));
   const size_t part_g = static_cast<size_t>(gp.splits) * gp.tiles * k.cols();
@@ -2957,25 +2959,29 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   if (o->recenter_potentials && total_blocks == 0) return OTTX_EUNSUPPORTED;
   const int recenter = (o->recenter_potentials && pb->tau_a == 1.0 && pb->tau_b == 1.0) ? 1 : 0;
   const bool rc_mode = o->recenter_potentials && both_unbalanced;
-  // parallel dual updates: the symmetric problems of a Sinkhorn divergence,
-  // where f = g throughout and one pass per iteration updates both
-  const bool symmetric = o->parallel_dual_updates != 0;
-  if (symmetric && (pb->x != pb->y || pb->n != pb->m || pb->a != pb->b || pb->tau_a != 1.0 ||
-                    pb->tau_b != 1.0 || o->init_mode != OTTX_INIT_ZEROS || o->recenter_potentials))
-    return OTTX_EUNSUPPORTED;
+  // parallel dual updates (sinkhorn.py:840-846): on the symmetric problems of
+  // a Sinkhorn divergence (x against x, a = b, balanced, zero start) f = g
+  // throughout and one pass per iteration updates both; any other problem
+  // runs both updates from the previous state, the f-update first (saving
+  // the old f, which the g-update then reads).
+  const bool parallel = o->parallel_dual_updates != 0;
+  if (parallel && o->recenter_potentials) return OTTX_EUNSUPPORTED;
+  const bool symmetric = parallel && pb->x == pb->y && pb->n == pb->m && pb->a == pb->b && pb->tau_a == 1.0 &&
+                         pb->tau_b == 1.0 && o->init_mode == OTTX_INIT_ZEROS;
+  const bool par_general = parallel && !symmetric;
   // epsilon schedule (epsilon_scheduler.py:47-60) with sequential updates, not
   // with the recentred iteration (whose soft minima take rho at the target)
   const double eps_init = o->epsilon_init > 1.0 ? o->epsilon_init : 1.0;
   const double eps_decay = o->epsilon_decay > 0.0 ? o->epsilon_decay : 1.0;
   if (!(eps_decay <= 1.0) || !(eps_init < 1e30)) return OTTX_EINVAL;
   const bool sched = eps_init > 1.0;
-  if (sched && (symmetric || rc_mode)) return OTTX_EUNSUPPORTED;
+  if (sched && (parallel || rc_mode)) return OTTX_EUNSUPPORTED;
   // adaptive momentum (acceleration.py:138-160) on L1 errors, from an
   // iteration that leaves two recorded blocks before it
   const int mom_start = o->momentum_start;
   if (mom_start < 0) return OTTX_EINVAL;
   const bool adaptive = mom_start > 0;
-  if (adaptive && (symmetric || o->norm_error != 1 || mom_start < 2 * inner)) return OTTX_EUNSUPPORTED;
+  if (adaptive && (parallel || o->norm_error != 1 || mom_start < 2 * inner)) return OTTX_EUNSUPPORTED;
   const bool per_launch = sched || adaptive;
   char* base = static_cast<char*>(workspace);
   SolveState* state = reinterpret_cast<SolveState*>(base + L.state);
@@ -3024,12 +3030,25 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   ga.out_save = g_save;
   ga.lse_shift = ga.lse_store = reinterpret_cast<float*>(base + L.lse_y);
   fa.lse_shift = fa.lse_store = reinterpret_cast<float*>(base + L.lse_x);
+  float* f_save = reinterpret_cast<float*>(base + L.f_save);
+  if (par_general) {
+    // the g-update reads the f of the previous state; the error is the row
+    // plus the column marginal error (sinkhorn.py:160-189) of that state,
+    // measured by the next iteration's two passes
+    ga.in_pot = f_save;
+    fa.out_save = f_save;
+    ga.balanced = fa.balanced = 0;
+  }
   const bool balanced = (pb->tau_a == 1.0 && pb->tau_b == 1.0);
   HalfStepArgs g_plain = ga, g_err = ga, f_plain = fa, f_err = fa;
   g_plain.mode = kModeWritePot | kModeCheckDone;
   g_err.mode = kModeWritePot | kModeCheckDone | kModeSaveOld | kModeErrCol;
   f_plain.mode = kModeWritePot | kModeCheckDone;
   f_err.mode = kModeWritePot | kModeCheckDone | (balanced ? 0u : kModeErrRow);
+  if (par_general) {
+    f_plain.mode |= kModeSaveOld;
+    f_err.mode = kModeWritePot | kModeCheckDone | kModeSaveOld | kModeErrRow | kModeErrOld;
+  }
 
   if (symmetric) g_err.symmetric = 1;
   // With a schedule every launch works out its iteration t = block * inner +
@@ -3197,7 +3216,8 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
                          static_cast<const SolveState*>(state));
       OTTX_LAUNCHED();
     }
-    hipLaunchKernelGGL(solve_finalize_kernel, dim3(kDualBlocks, 3), dim3(256), 0, s, out->f, P.a, pb->n,
+    hipLaunchKernelGGL(solve_finalize_kernel, dim3(kDualBlocks, 3), dim3(256), 0, s,
+                       par_general ? static_cast<const float*>(f_save) : out->f, P.a, pb->n,
                        rc_mode ? rc_col : g_save, out->g, P.b, pb->m, P.scal, pb->tau_a, pb->tau_b, out->errors,
                        outer, dual_part, state, mail, seq, recenter, out->f);
     OTTX_LAUNCHED();
@@ -3219,6 +3239,10 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
                          pb->n, pb->tau_a, static_cast<const SolveScalars*>(P.scal), &rcs->lse[0]);
       OTTX_LAUNCHED();
     } else {
+      if (par_general) {
+        rc = launch_halfstep(k, f_plain, st);  // f-update of iteration 0 (parallel)
+        if (rc != OTTX_OK) return rc;
+      }
       rc = launch_halfstep(k, g_plain, st);  // g-update of iteration 0
       if (rc != OTTX_OK) return rc;
     }

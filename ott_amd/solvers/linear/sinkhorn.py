@@ -216,10 +216,11 @@ class Sinkhorn:
   ``norm_error`` (p >= 1), ``inner_iterations``, ``min_iterations``,
   ``max_iterations``; constant ``Momentum`` and adaptive ``Momentum(start)``
   (norm_error 1, start >= 2 inner_iterations); epsilon schedules
-  ``Epsilon(target, init, decay)`` on balanced problems; sequential updates,
-  and parallel updates for the symmetric problems of a Sinkhorn divergence
-  (the same cloud and weights on both sides, from the default start); balanced
-  or unbalanced problems; warm starts through ``init`` or an ``initializer``;
+  ``Epsilon(target, init, decay)`` with sequential updates; sequential or
+  parallel updates (one pass per iteration on the symmetric problems of a
+  Sinkhorn divergence, both updates from the previous state otherwise);
+  balanced or unbalanced problems; warm starts through ``init`` or an
+  ``initializer``;
   ``progress_fn``, called per iteration once each block has run (the solve
   then waits for every block).  Options outside that set raise ``NotImplementedError``;
   there is no silent fallback.  ``implicit_diff`` and ``use_danskin`` only
@@ -289,11 +290,6 @@ class Sinkhorn:
                                 "or with recenter_potentials on problems with both taus < 1")
     if self.recenter_potentials and self.max_iterations == 0:
       raise NotImplementedError("recenter_potentials with max_iterations = 0 is not supported")
-    if self.parallel_dual_updates and not _is_symmetric(ot_prob):
-      # sinkhorn_divergence.py:298-318 runs parallel updates on x against x
-      # with a = b; there f = g throughout and one pass per iteration does both
-      raise NotImplementedError("parallel_dual_updates is supported for symmetric problems only "
-                                "(the same point cloud and weights on both sides, balanced)")
     if self.parallel_dual_updates and self.recenter_potentials:
       raise NotImplementedError("parallel_dual_updates with recenter_potentials is not supported")
 
@@ -319,8 +315,6 @@ class Sinkhorn:
     self._check_supported(ot_prob)
     geom = ot_prob.geom
     n, m = geom.shape
-    if self.parallel_dual_updates and (init is not None or type(self.initializer) is not init_lib.DefaultInitializer):
-      raise NotImplementedError("parallel_dual_updates runs from the default (zero) start only")
     if init is None and getattr(self.initializer, "predicts_g", False):
       # A warm start that predicts g only: the solver derives f0 from it with
       # its own first f-update instead of a separate pass over the geometry.
@@ -349,16 +343,6 @@ class Sinkhorn:
     f = as_device_f32(f0, geom.device).clone()
     g = as_device_f32(g0, geom.device).clone()
     return run(ot_prob, self, (f, g))
-
-
-def _is_symmetric(ot_prob: linear_problem.LinearProblem) -> bool:
-  """The same points and weights on both sides of a balanced problem."""
-  geom = ot_prob.geom
-  same_points = geom.x is geom.y or (geom.x.data_ptr() == geom.y.data_ptr() and geom.x.shape == geom.y.shape)
-  a, b = ot_prob._a, ot_prob._b
-  same_weights = (a is None and b is None) or (a is not None and b is not None and a.data_ptr() == b.data_ptr()
-                                                 and a.shape == b.shape)
-  return same_points and same_weights and ot_prob.tau_a == 1.0 and ot_prob.tau_b == 1.0
 
 
 def run(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,

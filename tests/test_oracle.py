@@ -506,3 +506,46 @@ def test_ti_cost_against_textbook_norms(cost_fn):
   l1 = np.abs(x[:, None] - y[None]).sum(-1)
   np.testing.assert_allclose(oracle.ti_cost(x, y, ("pnormp", 1.0)), l1, rtol=1e-12)
   np.testing.assert_allclose(np.sqrt(2.0 * oracle.ti_cost(x, y, ("sqpnorm", 1.0))), l1, rtol=1e-12)
+
+
+@pytest.mark.parametrize("taus", [(1.0, 1.0), (0.9, 0.8)])
+def test_oracle_parallel_updates_general(taus):
+  """Parallel updates on x against y (sinkhorn.py:840-846): one iteration
+  from (f0, g0) gives g1 = tau_b (eps log b - eps LSE_i((f0_i - C_ij) / eps)) and
+  f1 from g0 (not g1), written with scipy's logsumexp; the recorded error is
+  |P 1 - a'|_1 + |P^T 1 - b'|_1 of the state (sinkhorn.py:160-189, a' = a or
+  a exp(-f / rho) for tau < 1) from the dense coupling; and the converged
+  parallel iteration reaches the sequential one's optimum (the same coupling)."""
+  ta, tb = taus
+  rng = np.random.default_rng(12)
+  x = rng.normal(size=(30, 3))
+  y = rng.normal(size=(25, 3)) + 0.3
+  a = rng.uniform(0.2, 1.0, size=30)
+  b = rng.uniform(0.2, 1.0, size=25)
+  a /= a.sum()
+  b /= b.sum()
+  eps = 0.2
+  cost = ((x[:, None, :] - y[None]) ** 2).sum(-1)
+  f0 = rng.normal(size=30) * 0.1
+  g0 = rng.normal(size=25) * 0.1
+  one = oracle.solve_pointcloud(x, y, a, b, epsilon=eps, tau_a=ta, tau_b=tb, parallel_dual_updates=True,
+                                threshold=0.0, inner_iterations=1, max_iterations=1, init=(f0, g0))
+  g1 = tb * (eps * np.log(b) - eps * special.logsumexp((f0[:, None] - cost) / eps, axis=0))
+  f1 = ta * (eps * np.log(a) - eps * special.logsumexp((g0[None, :] - cost) / eps, axis=1))
+  np.testing.assert_allclose(one.g, g1, rtol=1e-12, atol=1e-12)
+  np.testing.assert_allclose(one.f, f1, rtol=1e-12, atol=1e-12)
+  plan = np.exp((one.f[:, None] + one.g[None] - cost) / eps)
+  rho = lambda t: eps * t / (1.0 - t)
+  ta_t = a if ta == 1.0 else a * np.exp(-one.f / rho(ta))
+  tb_t = b if tb == 1.0 else b * np.exp(-one.g / rho(tb))
+  want = np.abs(plan.sum(1) - ta_t).sum() + np.abs(plan.sum(0) - tb_t).sum()
+  np.testing.assert_allclose(one.errors[0], want, rtol=1e-10)
+  kw = dict(epsilon=eps, tau_a=ta, tau_b=tb, threshold=1e-10, inner_iterations=1, max_iterations=20000)
+  # balanced parallel updates oscillate in the gauge without momentum < 1
+  par = oracle.solve_pointcloud(x, y, a, b, parallel_dual_updates=True, momentum=0.5, **kw)
+  seq = oracle.solve_pointcloud(x, y, a, b, **kw)
+  assert par.converged and seq.converged
+  p_par = np.exp((par.f[:, None] + par.g[None] - cost) / eps)
+  p_seq = np.exp((seq.f[:, None] + seq.g[None] - cost) / eps)
+  np.testing.assert_allclose(p_par, p_seq, rtol=1e-6, atol=1e-12)
+  assert math.isclose(par.reg_ot_cost, seq.reg_ot_cost, rel_tol=1e-8)
