@@ -2981,7 +2981,11 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   const int mom_start = o->momentum_start;
   if (mom_start < 0) return OTTX_EINVAL;
   const bool adaptive = mom_start > 0;
-  if (adaptive && (parallel || o->norm_error != 1 || mom_start < 2 * inner)) return OTTX_EUNSUPPORTED;
+  // With norm_error != 1 the reference adds an L1 column to its errors
+  // (sinkhorn.py:1032-1038) but records the first norm's error in every
+  // column (one_iteration sets errors[it // inner, :] to solution_error(...)[0]),
+  // so the weight reads the errors this solve records.
+  if (adaptive && (parallel || mom_start < 2 * inner)) return OTTX_EUNSUPPORTED;
   const bool per_launch = sched || adaptive;
   char* base = static_cast<char*>(workspace);
   SolveState* state = reinterpret_cast<SolveState*>(base + L.state);

@@ -91,6 +91,30 @@ class _CTransform:
     return out[0] if single else out
 
 
+class _Debiased:
+  """x -> t1(x) - t2(x) of two c-transforms over the same new points: the
+  debiased potentials of a Sinkhorn divergence (sinkhorn_divergence.py:
+  76-114).  With the squared-Euclidean cost grad t(x) = 2 (x - bary_t(x)), so
+  the map x - grad(t1 - t2)(x) / 2 is bary_1(x) - bary_2(x) + x."""
+
+  def __init__(self, t1: _CTransform, t2: _CTransform):
+    self.t1 = t1
+    self.t2 = t2
+    self.support = t1.support
+
+  def values(self, pts: torch.Tensor):
+    v1, geom = self.t1.values(pts)
+    v2, _ = self.t2.values(pts)
+    return v1 - v2, geom
+
+  def __call__(self, x) -> torch.Tensor:
+    return self.t1(x) - self.t2(x)
+
+  def barycenter(self, x) -> torch.Tensor:
+    pts = as_device_f32(x, self.support.device)
+    return self.t1.barycenter(pts) - self.t2.barycenter(pts) + pts
+
+
 class DualPotentials:
   """potentials.py:37-113: the dual potential functions f and g of a solved
   problem, with ``transport`` and ``distance``."""
@@ -103,7 +127,7 @@ class DualPotentials:
   def transport(self, vec, forward: bool = True) -> torch.Tensor:
     """potentials.py:52-84: x - grad f(x) / 2 (forward) or y - grad g(y) / 2."""
     fn = self.f if forward else self.g
-    if not isinstance(fn, _CTransform):
+    if not isinstance(fn, (_CTransform, _Debiased)):
       raise NotImplementedError("transport needs the c-transform potentials of to_dual_potentials")
     return fn.barycenter(vec)
 

@@ -302,8 +302,11 @@ class Sinkhorn:
     if not self.momentum.is_constant:
       # Momentum(start > 0) (acceleration.py:138-160) reads the L1 errors of the
       # two blocks before ``start``
-      if int(self._norm_error) != 1 or self.parallel_dual_updates:
-        raise NotImplementedError("adaptive momentum is supported with norm_error=1 and sequential updates")
+      # with norm_error != 1 the reference records the first norm's error in
+      # the L1 column it adds (sinkhorn.py:1032-1038, 905-931), which is what
+      # the weight then reads: the errors of the requested norm
+      if self.parallel_dual_updates:
+        raise NotImplementedError("adaptive momentum is supported with sequential updates")
       if self.momentum.start < 2 * self.inner_iterations:
         raise NotImplementedError("adaptive momentum needs start >= 2 * inner_iterations")
     if int(self._norm_error) < 1:

@@ -41,6 +41,8 @@ class SegmentSinkhornOutput(NamedTuple):
   x_offsets: np.ndarray      # [K + 1]
   y_offsets: np.ndarray      # [K + 1]
   n_blocks: int
+  a: Optional[torch.Tensor] = None  # [sum n_k] the weights the solve used
+  b: Optional[torch.Tensor] = None  # [sum m_k]
 
 
 def _epsilon_args(epsilon, relative_epsilon) -> Tuple[float, float]:
@@ -63,13 +65,16 @@ def segment_sinkhorn_outputs(
     indices_are_sorted: bool = False, num_per_segment_x: Optional[Sequence[int]] = None,
     num_per_segment_y: Optional[Sequence[int]] = None, weights_x=None, weights_y=None,
     sinkhorn_kwargs: Mapping[str, Any] = MappingProxyType({}), tau_a: float = 1.0, tau_b: float = 1.0,
-    padded_statistics: bool = True, tile_config: int = -1, **kwargs: Any) -> SegmentSinkhornOutput:
+    padded_statistics: bool = True, tile_config: int = -1, epsilons=None, **kwargs: Any) -> SegmentSinkhornOutput:
   """All outputs of the batched solve (see ``segment_sinkhorn``).
 
   ``padded_statistics=True`` (OTT's behaviour) computes a relative epsilon
   over each segment padded to ``max_measure_size`` rows; ``False`` uses each
   segment's own points, i.e. what one ``PointCloud`` per segment would give.
   ``tile_config`` (beyond OTT) forces one of ``ottx_tile_configs``' variants.
+  ``epsilons`` (beyond OTT): an absolute epsilon per segment, which replaces
+  ``epsilon`` / ``relative_epsilon`` (a Sinkhorn divergence shares the
+  (x, y) terms' epsilons with the symmetric ones).
   """
   if cost_fn is not None and not isinstance(cost_fn, costs.SqEuclidean):
     raise NotImplementedError("ott_amd accelerates the squared-Euclidean cost only")
@@ -97,7 +102,7 @@ def segment_sinkhorn_outputs(
     raise NotImplementedError("adaptive momentum is not supported by the batched solve")
   if solver.initializer is not None and not isinstance(solver.initializer, sinkhorn.init_lib.DefaultInitializer):
     raise NotImplementedError("segment_sinkhorn starts every problem from zero potentials")
-  eps_abs, eps_rel = _epsilon_args(epsilon, relative_epsilon)
+  eps_abs, eps_rel = _epsilon_args(epsilon, relative_epsilon) if epsilons is None else (0.0, 0.0)
 
   device = _lib.require_device()
   x = as_device_f32(x, device)
@@ -130,7 +135,12 @@ def segment_sinkhorn_outputs(
   x_off_c = (ctypes.c_int64 * (K + 1))(*x_off.tolist())
   y_off_c = (ctypes.c_int64 * (K + 1))(*y_off.tolist())
   eps_c = None
-  if eps_rel == 0.0:
+  if epsilons is not None:
+    eps_list = [float(e) for e in epsilons]
+    if len(eps_list) != K or not all(e > 0.0 for e in eps_list):
+      raise ValueError(f"epsilons must hold {K} positive values")
+    eps_c = (ctypes.c_double * K)(*eps_list)
+  elif eps_rel == 0.0:
     eps_c = (ctypes.c_double * K)(*([eps_abs] * K))
   bt = _lib.Batch(count=K, d=x.shape[1], precision=_lib.PRECISIONS[precision], x=xs.data_ptr(),
                   y=ys.data_ptr(), a=a.data_ptr(), b=b.data_ptr(), x_off=x_off_c, y_off=y_off_c,
@@ -164,7 +174,7 @@ def segment_sinkhorn_outputs(
   return SegmentSinkhornOutput(
       f=f, g=g, errors=errors, reg_ot_cost=torch.tensor(list(reg), dtype=torch.float64),
       converged=torch.tensor(list(conv), dtype=torch.bool), n_iters=torch.tensor(list(iters)),
-      epsilon=torch.tensor(list(eps_out), dtype=torch.float64), x_offsets=x_off, y_offsets=y_off,
+      epsilon=torch.tensor(list(eps_out), dtype=torch.float64), x_offsets=x_off, y_offsets=y_off, a=a, b=b,
       n_blocks=int(out.n_blocks))
 
 

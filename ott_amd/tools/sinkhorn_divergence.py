@@ -11,23 +11,26 @@ run on the GPU:
 
 The divergence is
 ``reg_ot_cost(xy) - (reg_ot_cost(xx) + reg_ot_cost(yy)) / 2 + eps (sum a - sum b)^2 / 2``
-with eps the (x, y) geometry's epsilon (:320-324).  Low-rank solvers
-(``rank > 0``), ``to_dual_potentials`` and ``segment_sinkhorn_divergence`` are
-not provided.
+with eps the (x, y) geometry's epsilon (:320-324).
+``SinkhornDivergenceOutput.to_dual_potentials`` (:76-114) gives the debiased
+potential functions; ``segment_sinkhorn_divergence`` (:354-480) runs each of
+the three terms of every segment pair as one batched solve.  Low-rank solvers
+(``rank > 0``) are not provided.
 """
 
 from __future__ import annotations
 
 from types import MappingProxyType
-from typing import Any, Mapping, NamedTuple, Optional, Tuple, Type
+from typing import Any, Mapping, NamedTuple, Optional, Sequence, Tuple, Type
 
+import numpy as np
 import torch
 
 from ott_amd.geometry.pointcloud import PointCloud, as_device_f32
 from ott_amd.solvers.linear import acceleration
 from ott_amd.solvers.linear._solve import solve
 
-__all__ = ["sinkdiv", "sinkhorn_divergence", "SinkhornDivergenceOutput"]
+__all__ = ["sinkdiv", "sinkhorn_divergence", "segment_sinkhorn_divergence", "SinkhornDivergenceOutput"]
 
 Potentials = Tuple[torch.Tensor, torch.Tensor]
 
@@ -48,6 +51,31 @@ class SinkhornDivergenceOutput(NamedTuple):
   @property
   def is_low_rank(self) -> bool:
     return False
+
+  def to_dual_potentials(self, epsilon: Optional[float] = None):
+    """sinkhorn_divergence.py:76-114: the debiased dual potential functions
+    (Pooladian et al. 2022, eq. 8) of the (x, y) problem,
+
+      f(x) = f_xy(x) - g_x(x),   g(y) = g_xy(y) - f_y(y),
+
+    each a soft c-transform (``potential_fn_from_dual_vec`` of the (x, y)
+    problem) of a solved vector: g_xy against (y, b), f_x against x, f_xy
+    against x, g_y against (y, b).  With ``static_b`` g is not debiased.
+    ``transport`` of the squared-Euclidean cost, x - grad f(x) / 2, is then
+    bary_xy(x) - bary_x(x) + x with each bary the barycentric projection of
+    its c-transform."""
+    from ott_amd.problems.linear import potentials as pot
+    geom_xy = self.geoms[0]
+    eps = geom_xy.epsilon if epsilon is None else float(epsilon)
+    (f_xy, g_xy), (f_x, _), (_, g_y) = self.potentials
+    kw = dict(precision=geom_xy.precision, cost_fn=geom_xy.cost_fn)
+    f_xy_fn = pot._CTransform(geom_xy.y, g_xy, self.b, eps, side=1, **kw)
+    g_x_fn = pot._CTransform(geom_xy.x, f_x, None, eps, side=0, **kw)
+    g_xy_fn = pot._CTransform(geom_xy.x, f_xy, None, eps, side=0, **kw)
+    f_y_fn = None if g_y is None else pot._CTransform(geom_xy.y, g_y, self.b, eps, side=1, **kw)
+    return pot.DualPotentials(pot._Debiased(f_xy_fn, g_x_fn),
+                              g_xy_fn if f_y_fn is None else pot._Debiased(g_xy_fn, f_y_fn),
+                              cost_fn=geom_xy.cost_fn)
 
 
 def sinkdiv(x, y, *, cost_fn=None, epsilon: Optional[float] = None,
@@ -137,6 +165,65 @@ def _sinkhorn_divergence(geometry_xy: PointCloud, geometry_xx: PointCloud, geome
       converged=(bool(out_xy.converged), bool(out_xx.converged), conv_yy),
       n_iters=(out_xy.n_iters, out_xx.n_iters, it_yy),
   )
+
+
+def segment_sinkhorn_divergence(
+    x, y, num_segments: Optional[int] = None, max_measure_size: Optional[int] = None, cost_fn=None,
+    segment_ids_x=None, segment_ids_y=None, indices_are_sorted: bool = False,
+    num_per_segment_x: Optional[Sequence[int]] = None, num_per_segment_y: Optional[Sequence[int]] = None,
+    weights_x=None, weights_y=None, solve_kwargs: Mapping[str, Any] = MappingProxyType({}),
+    static_b: bool = False, share_epsilon: bool = True, symmetric_sinkhorn: bool = False,
+    **kwargs: Any) -> torch.Tensor:
+  """sinkhorn_divergence.py:354-480: the Sinkhorn divergence of every segment
+  pair of x and y, a float64 tensor [num_segments] on the CPU.
+
+  The reference pads every segment to ``max_measure_size`` and vmaps one
+  divergence per segment; here each of the three terms -- (x, y), (x, x) and,
+  unless ``static_b``, (y, y) -- is one batched solve over all segments at
+  their own sizes (``ottx_sinkhorn_batched_solve``).  With ``share_epsilon``
+  the symmetric terms take each segment's (x, y) epsilon, as the reference
+  does; a relative epsilon is taken over the padded clouds, as the reference's
+  padded geometries give it.  The symmetric terms run plain Sinkhorn updates,
+  the reference's default here (``symmetric_sinkhorn=False``); the symmetric
+  updates are not run by the batched solve and raise.
+  """
+  from ott_amd.tools import segment_sinkhorn as ss
+  if symmetric_sinkhorn:
+    raise NotImplementedError("segment_sinkhorn_divergence runs the symmetric terms with plain Sinkhorn updates "
+                              "(symmetric_sinkhorn=False, the reference's default)")
+  if int(dict(solve_kwargs).get("rank", -1)) > 0:
+    raise NotImplementedError("low-rank Sinkhorn (rank > 0) is outside ott_amd")
+  common = dict(num_segments=num_segments, max_measure_size=max_measure_size, cost_fn=cost_fn,
+                indices_are_sorted=indices_are_sorted, sinkhorn_kwargs=solve_kwargs)
+  out_xy = ss.segment_sinkhorn_outputs(x, y, segment_ids_x=segment_ids_x, segment_ids_y=segment_ids_y,
+                                       num_per_segment_x=num_per_segment_x, num_per_segment_y=num_per_segment_y,
+                                       weights_x=weights_x, weights_y=weights_y, **common, **kwargs)
+  sym_kwargs = dict(kwargs)
+  eps_xy = out_xy.epsilon.numpy()
+  if share_epsilon:
+    sym_kwargs.pop("epsilon", None)
+    sym_kwargs.pop("relative_epsilon", None)
+    sym_kwargs["epsilons"] = eps_xy.tolist()
+  out_xx = ss.segment_sinkhorn_outputs(x, x, segment_ids_x=segment_ids_x, segment_ids_y=segment_ids_x,
+                                       num_per_segment_x=num_per_segment_x, num_per_segment_y=num_per_segment_x,
+                                       weights_x=weights_x, weights_y=weights_x, **common, **sym_kwargs)
+  reg = out_xy.reg_ot_cost.numpy() - 0.5 * out_xx.reg_ot_cost.numpy()
+  if not static_b:
+    out_yy = ss.segment_sinkhorn_outputs(y, y, segment_ids_x=segment_ids_y, segment_ids_y=segment_ids_y,
+                                         num_per_segment_x=num_per_segment_y, num_per_segment_y=num_per_segment_y,
+                                         weights_x=weights_y, weights_y=weights_y, **common, **sym_kwargs)
+    reg = reg - 0.5 * out_yy.reg_ot_cost.numpy()
+  # the mass term eps (sum a - sum b)^2 / 2 of each segment, from the weights
+  # the solves used (the segments' own, or uniform)
+  sum_a = _segment_sums(out_xy.a, out_xy.x_offsets)
+  sum_b = _segment_sums(out_xy.b, out_xy.y_offsets)
+  div = reg + 0.5 * eps_xy * (sum_a - sum_b) ** 2
+  return torch.from_numpy(np.asarray(div, dtype=np.float64))
+
+
+def _segment_sums(w: torch.Tensor, offsets: np.ndarray) -> np.ndarray:
+  cs = np.concatenate([[0.0], np.cumsum(w.double().cpu().numpy())])
+  return cs[np.asarray(offsets[1:])] - cs[np.asarray(offsets[:-1])]
 
 
 def _is_uniform(w: torch.Tensor) -> bool:

@@ -16,7 +16,7 @@ import pytest
 import torch
 
 from oracle import sinkhorn64 as oracle
-from tests.test_gpu_parity import _check_solve
+from tests.test_gpu_parity import TOL, _check_solve
 
 pytestmark = pytest.mark.gpu
 
@@ -152,3 +152,81 @@ def test_sinkhorn_divergence_static_b():
   reg_yy = 2.0 * (static - full)  # full = xy - (xx + yy)/2, static = xy - xx/2
   offset, out_off = sd.sinkdiv(tx, ty, static_b=True, offset_static_b=reg_yy, **kw)
   assert math.isclose(offset, full, rel_tol=1e-6, abs_tol=1e-9)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("static_b", [False, True])
+def test_divergence_to_dual_potentials(precision, static_b):
+  """SinkhornDivergenceOutput.to_dual_potentials (sinkhorn_divergence.py:
+  76-114): f = f_xy - g_x and g = g_xy - f_y (g_xy alone with static_b) as
+  soft c-transforms of the three solves' vectors, the debiased transport map,
+  against the oracle's c-transforms and barycentric maps of the same vectors."""
+  from ott_amd.tools import sinkhorn_divergence as sd
+  x, a = _points(300, 6, seed=5)
+  y, b = _points(257, 6, seed=6, shift=0.3)
+  _, out = sd.sinkdiv(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), a=torch.from_numpy(a).cuda(),
+                      b=torch.from_numpy(b).cuda(), epsilon=0.1, static_b=static_b, precision=precision,
+                      solve_kwargs=dict(threshold=1e-5))
+  dp = out.to_dual_potentials()
+  (f_xy, g_xy), (f_x, _), (_, g_y) = [tuple(None if t is None else t.double().cpu().numpy() for t in p)
+                                      for p in out.potentials]
+  rng = np.random.default_rng(8)
+  xs = (rng.normal(size=(45, 6)) / np.sqrt(6)).astype(np.float32)
+  ys = (rng.normal(size=(33, 6)) / np.sqrt(6) + 0.3).astype(np.float32)
+  x64, y64 = x.astype(np.float64), y.astype(np.float64)
+  c_xs_y = oracle.sqeuclidean_cost(xs.astype(np.float64), y64)
+  c_x_xs = oracle.sqeuclidean_cost(x64, xs.astype(np.float64))
+  c_x_ys = oracle.sqeuclidean_cost(x64, ys.astype(np.float64))
+  c_ys_y = oracle.sqeuclidean_cost(ys.astype(np.float64), y64)
+  eps = 0.1
+  u45, u33 = np.full(45, 1.0 / 45), np.full(33, 1.0 / 33)
+  f_ref = (oracle.soft_c_transform(c_xs_y, g_xy, b, eps, axis=1)
+           - oracle.soft_c_transform(c_x_xs, f_x, u45, eps, axis=0))
+  g_ref = oracle.soft_c_transform(c_x_ys, f_xy, u33, eps, axis=0)
+  if not static_b:
+    g_ref = g_ref - oracle.soft_c_transform(c_ys_y, g_y, b, eps, axis=1)
+  tol = 1e-4 if precision == "fp32" else 1e-3
+  scale = max(np.abs(f_ref).max(), np.abs(g_ref).max(), eps)
+  np.testing.assert_allclose(dp.f(torch.from_numpy(xs).cuda()).double().cpu().numpy(), f_ref, atol=tol * scale)
+  np.testing.assert_allclose(dp.g(torch.from_numpy(ys).cuda()).double().cpu().numpy(), g_ref, atol=tol * scale)
+  # x - grad f(x) / 2 = bary_xy(x) - bary_x(x) + x
+  t_ref = (oracle.barycentric_map(c_xs_y, g_xy, b, eps, y64)
+           - oracle.barycentric_map(c_x_xs.T, f_x, np.ones(300), eps, x64) + xs.astype(np.float64))
+  np.testing.assert_allclose(dp.transport(torch.from_numpy(xs).cuda()).double().cpu().numpy(), t_ref,
+                             atol=tol * max(np.abs(x).max(), np.abs(y).max()))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("eps_kw,static_b", [(dict(epsilon=0.08), False), (dict(epsilon=0.05, relative_epsilon="mean"),
+                                                                           False),
+                                             (dict(epsilon=0.08), True)])
+def test_segment_sinkhorn_divergence_matches_oracle(precision, eps_kw, static_b):
+  """segment_sinkhorn_divergence (sinkhorn_divergence.py:354-480): three
+  batched solves over uneven segments, the symmetric terms at each segment's
+  (x, y) epsilon (share_epsilon), plain Sinkhorn updates; against the
+  oracle's per-segment solves (padded clouds for a relative epsilon)."""
+  from ott_amd.tools import sinkhorn_divergence as sd
+  sizes = [(120, 90), (37, 64), (300, 211), (5, 9)]
+  rng = np.random.default_rng(17)
+  xs = [(rng.normal(size=(n, 4)) / 2).astype(np.float32) for n, _ in sizes]
+  ys = [(rng.normal(size=(m, 4)) / 2 + 0.3).astype(np.float32) for _, m in sizes]
+  x, y = np.concatenate(xs), np.concatenate(ys)
+  nx, ny = [n for n, _ in sizes], [m for _, m in sizes]
+  P = 320
+  kw = dict(threshold=1e-5, inner_iterations=1, max_iterations=3000)
+  got = sd.segment_sinkhorn_divergence(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), num_per_segment_x=nx,
+                                       num_per_segment_y=ny, max_measure_size=P, solve_kwargs=kw, static_b=static_b,
+                                       precision=precision, **eps_kw)
+  refs = oracle.segment_sinkhorn(x, y, nx, ny, max_measure_size=P, sinkhorn_kwargs=kw, **eps_kw)
+  tol = TOL[precision]
+  for k, (n, m) in enumerate(sizes):
+    e = refs[k].epsilon
+    a = np.full(n, 1.0 / n)
+    b = np.full(m, 1.0 / m)
+    xx = oracle.solve_pointcloud(xs[k], xs[k], a, a, epsilon=e, **kw)
+    div = refs[k].reg_ot_cost - 0.5 * xx.reg_ot_cost
+    if not static_b:
+      yy = oracle.solve_pointcloud(ys[k], ys[k], b, b, epsilon=e, **kw)
+      div -= 0.5 * yy.reg_ot_cost
+    scale = max(abs(refs[k].reg_ot_cost), abs(xx.reg_ot_cost))
+    assert abs(float(got[k]) - div) <= tol * scale, (k, float(got[k]), div)

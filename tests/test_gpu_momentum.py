@@ -41,9 +41,6 @@ def test_adaptive_momentum_limits():
   x, y, a, b = _cloud(64, 48, 3, seed=2)
   with pytest.raises(NotImplementedError):  # needs two recorded blocks before start
     _solve(x, y, a, b, "fp32", dict(epsilon=0.05), inner_iterations=5, momentum=acceleration.Momentum(start=5))
-  with pytest.raises(NotImplementedError):  # the weight reads L1 errors
-    _solve(x, y, a, b, "fp32", dict(epsilon=0.05), inner_iterations=1, norm_error=2,
-           momentum=acceleration.Momentum(start=5))
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
@@ -81,4 +78,25 @@ def test_adaptive_momentum_unbalanced(precision, tau_a, tau_b, recenter):
   out = _solve(x, y, a, b, precision, dict(epsilon=0.02), momentum=mom, **kw)
   ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.02, momentum_start=9, **kw)
   assert ref.converged
+  _check_solve(out, ref, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("norm_error,taus", [(2, (1.0, 1.0)), (3, (1.0, 1.0)), (2, (0.9, 0.85))])
+def test_adaptive_momentum_with_other_norms(precision, norm_error, taus):
+  """Momentum(start) with norm_error != 1: the reference adds an L1 column to
+  its errors (sinkhorn.py:1032-1038) yet writes the first norm's error into
+  every column (one_iteration, :905-931), so Lehmann's weight reads the
+  p-norm errors; the oracle records them the same way."""
+  from ott_amd.solvers.linear import acceleration
+  x, y, a, b = _cloud(300, 257, 6, seed=14, zero_a=[3])
+  # epsilon and threshold chosen so that the threshold crossing is clear (the
+  # last two block errors at about 1.6 and 0.7 of it)
+  kw = dict(threshold=1e-5, inner_iterations=2, max_iterations=2000, norm_error=norm_error, tau_a=taus[0],
+            tau_b=taus[1])
+  out = _solve(x, y, a, b, precision, dict(epsilon=0.1), momentum=acceleration.Momentum(start=6), **kw)
+  ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.1, momentum_start=6, **kw)
+  assert ref.converged
+  w = oracle.momentum_weight(ref.errors, 6, 1.0, 6, math.inf, 2)
+  assert w > 1.0
   _check_solve(out, ref, precision)

@@ -15,8 +15,7 @@ def test_adaptive_momentum_limits_on_the_host():
   assert ok.momentum.start == 10 and ok.momentum.inner_iterations == 5
   with pytest.raises(NotImplementedError):  # two recorded blocks must precede start
     sinkhorn.Sinkhorn(inner_iterations=5, momentum=acceleration.Momentum(start=9))._check_options()
-  with pytest.raises(NotImplementedError):  # the weight reads L1 errors
-    sinkhorn.Sinkhorn(inner_iterations=1, norm_error=2, momentum=acceleration.Momentum(start=4))._check_options()
+  sinkhorn.Sinkhorn(inner_iterations=1, norm_error=2, momentum=acceleration.Momentum(start=4))._check_options()
   with pytest.raises(NotImplementedError):
     sinkhorn.Sinkhorn(inner_iterations=1, parallel_dual_updates=True,
                       momentum=acceleration.Momentum(start=4))._check_options()
