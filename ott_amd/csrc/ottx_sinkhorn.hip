@@ -4284,6 +4284,26 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
       for (int q = 0; q < norm_blocks; ++q) prep[wp++] = make_int4(i, side, q, 1);
     }
   }
+  // Longest items first (the classic longest-processing-time order): an
+  // item's cost is the rows its split streams times the tile's columns, the
+  // same for every column tile of a problem; dispatching the large problems'
+  // items before the small ones keeps the launch's tail short.
+  auto by_cost = [&](int4* work, int64_t count, bool g_dir) {
+    std::vector<std::pair<int64_t, int4>> v(static_cast<size_t>(count));
+    for (int64_t q = 0; q < count; ++q) {
+      const int4 w = work[q];
+      const PassPlan& pp = g_dir ? B.gp[w.x] : B.fp[w.x];
+      const int64_t rb0 = static_cast<int64_t>(w.z) * pp.rb_per_split;
+      const int64_t rows = std::min<int64_t>(pp.rb_per_split, pp.n_in_rb - rb0);
+      v[static_cast<size_t>(q)] = {rows, w};
+    }
+    std::stable_sort(v.begin(), v.end(), [](const std::pair<int64_t, int4>& l, const std::pair<int64_t, int4>& r) {
+      return l.first > r.first;
+    });
+    for (int64_t q = 0; q < count; ++q) work[q] = v[static_cast<size_t>(q)].second;
+  };
+  by_cost(work_g, wg, true);
+  by_cost(work_f, wf, false);
   BatchItem* d_items = reinterpret_cast<BatchItem*>(ws + B.off_items);
   HalfStepArgs* d_args_g = reinterpret_cast<HalfStepArgs*>(ws + B.off_args_g);
   HalfStepArgs* d_args_f = reinterpret_cast<HalfStepArgs*>(ws + B.off_args_f);
