@@ -71,7 +71,9 @@ def segment_sinkhorn_outputs(
   ``padded_statistics=True`` (OTT's behaviour) computes a relative epsilon
   over each segment padded to ``max_measure_size`` rows; ``False`` uses each
   segment's own points, i.e. what one ``PointCloud`` per segment would give.
-  ``tile_config`` (beyond OTT) forces one of ``ottx_tile_configs``' variants.
+  ``tile_config`` (beyond OTT) forces one of ``ottx_tile_configs``' variants;
+  -1 takes the autotuner's record for the batch's size class
+  (``tune.lookup_batch``), else the variant with the least row padding.
   ``epsilons`` (beyond OTT): an absolute epsilon per segment, which replaces
   ``epsilon`` / ``relative_epsilon`` (a Sinkhorn divergence shares the
   (x, y) terms' epsilons with the symmetric ones).
@@ -148,6 +150,9 @@ def segment_sinkhorn_outputs(
                   mean_rows_x=float(px) if eps_rel > 0 and padded_statistics else 0.0,
                   mean_rows_y=float(py) if eps_rel > 0 and padded_statistics else 0.0,
                   inv_scale_cost=1.0 / float(scale_cost), tau_a=float(tau_a), tau_b=float(tau_b))
+  if tile_config == -1:
+    from ott_amd import tune
+    tile_config = tune.lookup_batch(precision, int(x.shape[1]), K, float(np.sum(nx.astype(np.float64) * ny)))
   opts = _lib.SinkhornOpts(
       threshold=float(solver.threshold), norm_error=int(solver._norm_error),
       inner_iterations=int(solver.inner_iterations), min_iterations=int(solver.min_iterations),

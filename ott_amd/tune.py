@@ -112,6 +112,76 @@ def lookup(precision: str, d: int, n: int, m: int) -> int:
   return int(rec["tile_config"])
 
 
+def batch_key(precision: str, d: int, count: int, pairs: float) -> str:
+  """Cache key of a batched solve: precision, d, and the power-of-two classes
+  of the problem count and of the total pairs per half-step."""
+  return f"{_device_name()}/{precision}/d{d}/batch/k{_size_class(count)}/p{_size_class(int(pairs))}"
+
+
+def lookup_batch(precision: str, d: int, count: int, pairs: float) -> int:
+  """The tuned tile_config of a batched solve, or -1 (the variant with the
+  least row padding) when none is recorded or it names no listed variant."""
+  with _lock:
+    rec = _load().get(batch_key(precision, d, count, pairs))
+  if rec is None or not _valid(rec, precision, d):
+    return -1
+  return int(rec["tile_config"])
+
+
+def _record(k: str, best: dict, save: bool) -> None:
+  with _lock:
+    _load()[k] = entry(best)
+    if save:
+      path = cache_path()
+      os.makedirs(os.path.dirname(path), exist_ok=True)
+      existing = {}
+      if os.path.exists(path):
+        try:
+          with open(path) as fh:
+            existing = json.load(fh)
+        except (OSError, ValueError):
+          existing = {}
+      existing[k] = entry(best)
+      with open(path, "w") as fh:
+        json.dump(existing, fh, indent=1, sort_keys=True)
+
+
+def autotune_batch(x, y, num_per_segment_x, num_per_segment_y, precision: str = "bf16_split", reps: int = 3,
+                   iterations: int = 2, save: bool = True, verbose: bool = False, **pointcloud_kwargs) -> dict:
+  """Time every tile configuration on a batched solve (segment_sinkhorn of the
+  given segments, ``iterations`` fixed iterations so that convergence does
+  not enter) and record the fastest under :func:`batch_key`."""
+  import time
+  from ott_amd.tools import segment_sinkhorn as ss
+  d = int(x.shape[1])
+  kw = dict(min_iterations=iterations, max_iterations=iterations, inner_iterations=1)
+  pointcloud_kwargs.setdefault("epsilon", 0.05)
+  pointcloud_kwargs.setdefault("relative_epsilon", "mean")
+  results = []
+  for cfg in tile_configs(precision, d):
+    def one():
+      return ss.segment_sinkhorn_outputs(x, y, num_per_segment_x=num_per_segment_x,
+                                         num_per_segment_y=num_per_segment_y, sinkhorn_kwargs=kw,
+                                         precision=precision, tile_config=cfg["tile_config"], **pointcloud_kwargs)
+    one()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+      torch.cuda.synchronize()
+      t0 = time.perf_counter()
+      one()
+      torch.cuda.synchronize()
+      times.append((time.perf_counter() - t0) * 1e3)
+    results.append(dict(cfg, ms=min(times)))
+    if verbose:
+      print(f"  batch tile {cfg}: {min(times):.3f} ms", flush=True)
+  best = min(results, key=lambda r: r["ms"])
+  pairs = float(sum(int(n) * int(m) for n, m in zip(num_per_segment_x, num_per_segment_y)))
+  k = batch_key(precision, d, len(num_per_segment_x), pairs)
+  _record(k, best, save)
+  return dict(best=best, key=k, results=results)
+
+
 def benchmark(geom, tile_config: int, axis: int, reps: int = 10) -> float:
   """Mean milliseconds of one half-step of ``geom`` with ``tile_config``."""
   lib = _lib.load_library()

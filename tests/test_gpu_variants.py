@@ -114,3 +114,33 @@ def test_autotune_picks_a_listed_config_that_matches_oracle(precision, tmp_path,
   ref = oracle.solve_pointcloud(x, y, a, b, epsilon=0.05, relative_epsilon="mean", **KW)
   _close(out.f.cpu().numpy(), ref.f, TOL[precision], "f")
   assert abs(out.reg_ot_cost - ref.reg_ot_cost) <= TOL[precision] * abs(ref.reg_ot_cost)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+def test_autotune_batch_picks_a_listed_config_that_matches_oracle(precision, tmp_path, monkeypatch):
+  """tune.autotune_batch times every variant on the batch itself and records
+  the fastest; segment_sinkhorn then runs with it (tile_config -1) and matches
+  the oracle."""
+  from ott_amd import tune
+  from ott_amd.tools import segment_sinkhorn as ss
+  monkeypatch.setenv("OTTX_TUNE_CACHE", str(tmp_path / "tune.json"))
+  monkeypatch.setattr(tune, "_cache", None)
+  sizes = [(300, 257), (64, 900), (1000, 33)]
+  rng = np.random.default_rng(9)
+  xs = [(rng.normal(size=(n, 16)) / 4).astype(np.float32) for n, _ in sizes]
+  ys = [(rng.normal(size=(m, 16)) / 4 + 0.2).astype(np.float32) for _, m in sizes]
+  x = torch.from_numpy(np.concatenate(xs)).cuda()
+  y = torch.from_numpy(np.concatenate(ys)).cuda()
+  nx, ny = [n for n, _ in sizes], [m for _, m in sizes]
+  res = tune.autotune_batch(x, y, nx, ny, precision=precision, reps=1)
+  listed = {c["tile_config"] for c in tune.tile_configs(precision, 16)}
+  assert res["best"]["tile_config"] in listed
+  pairs = float(sum(n * m for n, m in sizes))
+  assert tune.lookup_batch(precision, 16, 3, pairs) == res["best"]["tile_config"]
+  out = ss.segment_sinkhorn_outputs(x, y, num_per_segment_x=nx, num_per_segment_y=ny, sinkhorn_kwargs=KW,
+                                    epsilon=0.05, relative_epsilon="mean", padded_statistics=False,
+                                    precision=precision)
+  tol = TOL[precision]
+  for k, (xk, yk) in enumerate(zip(xs, ys)):
+    ref = oracle.solve_pointcloud(xk, yk, None, None, epsilon=0.05, relative_epsilon="mean", **KW)
+    assert abs(float(out.reg_ot_cost[k]) - ref.reg_ot_cost) <= tol * abs(ref.reg_ot_cost), k

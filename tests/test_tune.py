@@ -58,5 +58,16 @@ def test_shipped_defaults_name_listed_variants():
   with open(tune.DEFAULTS_PATH) as fh:
     shipped = json.load(fh)
   for k, rec in shipped.items():
-    _, prec, d, _, _ = k.split("/")
+    _, prec, d = k.split("/")[:3]
     assert tune._valid(rec, prec, int(d[1:])), k
+
+
+def test_batch_records_are_keyed_by_size_class(cache):
+  cfg = tune.tile_configs("bf16_split", 32)[5]
+  cache({tune.batch_key("bf16_split", 32, 512, 5.35e8): tune.entry(cfg)})
+  assert tune.lookup_batch("bf16_split", 32, 512, 5.35e8) == cfg["tile_config"]
+  assert tune.lookup_batch("bf16_split", 32, 500, 5.0e8) == cfg["tile_config"]  # same classes
+  assert tune.lookup_batch("bf16_split", 32, 2048, 5.35e8) == -1
+  assert tune.lookup_batch("fp32", 32, 512, 5.35e8) == -1
+  # single-problem records are not batch records
+  assert tune.lookup("bf16_split", 32, 512, 512) == -1
