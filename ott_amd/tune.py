@@ -88,6 +88,7 @@ def _load() -> Dict[str, dict]:
         except (OSError, ValueError):
           pass
     _cache = merged
+    _resolved.clear()  # validated lookups of the previous cache
   return _cache
 
 
@@ -102,14 +103,28 @@ def _valid(rec: dict, precision: str, d: int) -> bool:
   return any(all(c[f] == want[f] for f in _FIELDS) for c in tile_configs(precision, d))
 
 
+_resolved: Dict[str, int] = {}
+
+
+def _resolve(k: str, precision: str, d: int) -> int:
+  """The validated tile_config of cache key ``k`` (-1 when none), memoised:
+  the loaded library's variant table does not change within a process, and a
+  solve on the hot path should not pay for the check every call."""
+  with _lock:
+    rec = _load().get(k)  # reloads (and forgets the memo) after a reset
+    hit = _resolved.get(k)
+  if hit is not None:
+    return hit
+  val = int(rec["tile_config"]) if rec is not None and _valid(rec, precision, d) else -1
+  with _lock:
+    _resolved[k] = val
+  return val
+
+
 def lookup(precision: str, d: int, n: int, m: int) -> int:
   """The tuned tile_config for the shape, or -1 when none is recorded or the
   record does not name a variant of the loaded library."""
-  with _lock:
-    rec = _load().get(key(precision, d, n, m))
-  if rec is None or not _valid(rec, precision, d):
-    return -1
-  return int(rec["tile_config"])
+  return _resolve(key(precision, d, n, m), precision, d)
 
 
 def batch_key(precision: str, d: int, count: int, pairs: float) -> str:
@@ -121,16 +136,13 @@ def batch_key(precision: str, d: int, count: int, pairs: float) -> str:
 def lookup_batch(precision: str, d: int, count: int, pairs: float) -> int:
   """The tuned tile_config of a batched solve, or -1 (the variant with the
   least row padding) when none is recorded or it names no listed variant."""
-  with _lock:
-    rec = _load().get(batch_key(precision, d, count, pairs))
-  if rec is None or not _valid(rec, precision, d):
-    return -1
-  return int(rec["tile_config"])
+  return _resolve(batch_key(precision, d, count, pairs), precision, d)
 
 
 def _record(k: str, best: dict, save: bool) -> None:
   with _lock:
     _load()[k] = entry(best)
+    _resolved.pop(k, None)
     if save:
       path = cache_path()
       os.makedirs(os.path.dirname(path), exist_ok=True)
@@ -214,21 +226,7 @@ def autotune(geom, reps: int = 10, save: bool = True, verbose: bool = False) -> 
       print(f"  tile {cfg}: {ms:.4f} ms per iteration", flush=True)
   best = min(results, key=lambda r: r["ms_per_iteration"])
   k = key(geom.precision, d, n, m)
-  with _lock:
-    _load()[k] = entry(best)
-    if save:
-      path = cache_path()
-      os.makedirs(os.path.dirname(path), exist_ok=True)
-      existing = {}
-      if os.path.exists(path):
-        try:
-          with open(path) as fh:
-            existing = json.load(fh)
-        except (OSError, ValueError):
-          existing = {}
-      existing[k] = entry(best)
-      with open(path, "w") as fh:
-        json.dump(existing, fh, indent=1, sort_keys=True)
+  _record(k, best, save)
   return dict(best=best, key=k, results=results)
 
 
