@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define OTTX_ABI_VERSION 4
+#define OTTX_ABI_VERSION 5
 
 enum ottx_status {
   OTTX_OK = 0,
@@ -196,11 +196,11 @@ typedef struct ottx_batch {
   int32_t count;            /* K problems (>= 1) */
   int32_t d;                /* feature dimension (1..256) */
   int32_t precision;        /* OTTX_PREC_* */
-  int32_t _pad0;
+  int32_t flags;            /* OTTX_BATCH_* bits */
   const float* x;           /* [x_off[K], d] device */
   const float* y;           /* [y_off[K], d] device */
-  const float* a;           /* [x_off[K]] device */
-  const float* b;           /* [y_off[K]] device */
+  float* a;                 /* [x_off[K]] device; read, or written (OTTX_BATCH_UNIFORM_A) */
+  float* b;                 /* [y_off[K]] device; read, or written (OTTX_BATCH_UNIFORM_B) */
   const int64_t* x_off;     /* [K + 1] host, x_off[0] = 0, non-decreasing with n_k >= 1 */
   const int64_t* y_off;     /* [K + 1] host */
   const double* epsilon;    /* [K] host absolute epsilons; NULL when relative_epsilon > 0 */
@@ -212,6 +212,12 @@ typedef struct ottx_batch {
   double tau_a;
   double tau_b;
 } ottx_batch;
+
+/* ottx_batch.flags: the solve writes the default weights 1 / n_k (1 / m_k)
+ * into a (b) for the rows of each problem, as segment_point_cloud's default
+ * (geometry/segment.py:100-104), instead of reading them. */
+#define OTTX_BATCH_UNIFORM_A 1
+#define OTTX_BATCH_UNIFORM_B 2
 
 typedef struct ottx_batch_out {
   float* f;                 /* [x_off[K]] device: initial potentials in, final out */

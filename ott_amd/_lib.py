@@ -60,7 +60,11 @@ class Problem(ctypes.Structure):
 
 
 # OTTX_ABI_VERSION of include/ottx.h that these structures mirror
-ABI_VERSION = 4
+ABI_VERSION = 5
+
+# ottx_batch.flags
+BATCH_UNIFORM_A = 1
+BATCH_UNIFORM_B = 2
 
 EPS_STAT_MEAN = 0
 EPS_STAT_STD = 1
@@ -112,7 +116,7 @@ class Batch(ctypes.Structure):
       ("count", ctypes.c_int32),
       ("d", ctypes.c_int32),
       ("precision", ctypes.c_int32),
-      ("_pad0", ctypes.c_int32),
+      ("flags", ctypes.c_int32),
       ("x", ctypes.c_void_p),
       ("y", ctypes.c_void_p),
       ("a", ctypes.c_void_p),

@@ -33,6 +33,11 @@ enum : uint32_t {
   kModeErrOld = 1u << 8,     // kModeErrRow measures the old potential (parallel updates)
 };
 
+// Batched solves: batch_done[0] counts finished problems, [1] completed
+// blocks, [2] finished problems at the last rebuild of the live work lists;
+// bit k of the words from kBatchDoneBits on is set when problem k finished.
+constexpr int kBatchDoneBits = 4;
+
 // Loop state shared by the kernels of one solve (device memory).
 struct SolveState {
   int32_t done;        // set by the block finalizer when the loop must stop
@@ -130,7 +135,8 @@ struct HalfStepArgs {
   int32_t outer;
   int32_t force_scan;
   float threshold;
-  uint32_t* batch_done;  // batched solves: count of problems whose loop finished
+  uint32_t* batch_done;  // batched solves: count of problems whose loop finished, then one bit per problem
+  int32_t problem;       // batched solves: index of the problem (its bit in batch_done + kBatchDoneBits)
   // costs other than the squared Euclidean (PREC 2): in_frag / out_frag are
   // the raw points [rows, d]; a pair's cost is s alpha (sum_k phi(z_k))^beta
   int32_t d;

@@ -146,7 +146,10 @@ __device__ inline void finalize_block(const HalfStepArgs& p, double col, double 
   if (!cont) {
     st->n_blocks = k + 1;
     st->done = 1;
-    if (p.batch_done != nullptr) atomicAdd(p.batch_done, 1u);
+    if (p.batch_done != nullptr) {
+      atomicOr(p.batch_done + kBatchDoneBits + (p.problem >> 5), 1u << (p.problem & 31));
+      atomicAdd(p.batch_done, 1u);
+    }
   }
 }
 
@@ -796,8 +799,9 @@ __device__ __forceinline__ void stream_cost(const HalfStepArgs& p, int64_t col0,
 // pairs and runs the epilogue.
 // ---------------------------------------------------------------------------
 template <int PREC, int KK, int NB, int MT, int W>
-__device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int tile, const int split) {
-  if ((p.mode & kModeCheckDone) && p.state->done) return;
+__device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, uint32_t mode, const int tile,
+                                               const int split, const int tid) {
+  if ((mode & kModeCheckDone) && p.state->done) return;
   constexpr int NT = W * 64;
   constexpr int COLS = W * NB * 16;
   constexpr int TILE_U4 = MT * FragChunks<PREC, KK>::value * 64;
@@ -809,7 +813,6 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
 
   __shared__ float s_mm[W][2];
 
-  const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   float c = p.c, eps = p.eps, eps_marg = p.eps_marg, inv_rho = p.inv_rho;
@@ -820,7 +823,6 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
   }
   // the exponent constant the y fragments were packed with
   const float c_pack = p.pack_scal != nullptr ? p.pack_scal->c : p.c;
-  uint32_t mode = p.mode;
   if (p.eps_sched) {
     // epsilon schedule (epsilon_scheduler.py:47-60): this launch's iteration
     // t from the block index, eps_t = max(init decay^t, 1) eps.  While eps_t >
@@ -1058,7 +1060,7 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, const int 
 // Single-problem half-step: grid (column tiles, row splits).
 template <int PREC, int KK, int NB, int MT, int W>
 __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) {
-  halfstep_block<PREC, KK, NB, MT, W>(p, blockIdx.x, blockIdx.y);
+  halfstep_block<PREC, KK, NB, MT, W>(p, p.mode, blockIdx.x, blockIdx.y, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1189,18 +1191,18 @@ __global__ __launch_bounds__(W * 64) void apply_pv_kernel(const ApplyArgs A) {
   if (tid == 0) p.tile_counter[tile] = 0u;
 }
 
-// Batched half-step over many independent problems in one launch.  Workgroup
-// b runs (problem, column tile, row split) = work[b].x, .y, .z with that
-// problem's arguments; ``mode`` replaces the per-problem mode.  Problems whose
-// loop is done exit at once (kModeCheckDone), so converged problems stop
-// costing time while the rest continue.
+// Batched half-step over many independent problems.  Workgroup b runs item
+// (problem, column tile, row split) = work[b].x, .y, .z of the live work
+// list, which holds the items of unfinished problems only, longest first (it
+// is rebuilt between blocks); workgroups past its length (*count) exit at
+// once.  ``mode`` replaces the per-problem mode.
 template <int PREC, int KK, int NB, int MT, int W>
 __global__ __launch_bounds__(W * 64) void halfstep_batched_kernel(const HalfStepArgs* __restrict__ args,
-                                                                  const int4* __restrict__ work, uint32_t mode) {
+                                                                  const int4* __restrict__ work,
+                                                                  const uint32_t* __restrict__ count, uint32_t mode) {
+  if (blockIdx.x >= *count) return;
   const int4 w = work[blockIdx.x];
-  HalfStepArgs p = args[w.x];
-  p.mode = mode;
-  halfstep_block<PREC, KK, NB, MT, W>(p, w.y, w.z);
+  halfstep_block<PREC, KK, NB, MT, W>(args[w.x], mode, w.y, w.z, threadIdx.x);
 }
 
 }  // namespace ottx

@@ -1875,7 +1875,7 @@ __global__ __launch_bounds__(kRcThreads) void rc_output_kernel(float* f, const f
 // Kernel selection
 // ---------------------------------------------------------------------------
 typedef void (*HalfStepFn)(const HalfStepArgs);
-typedef void (*BatchedFn)(const HalfStepArgs*, const int4*, uint32_t);
+typedef void (*BatchedFn)(const HalfStepArgs*, const int4*, const uint32_t*, uint32_t);
 
 struct KernelCfg {
   int prec;
@@ -3671,6 +3671,8 @@ struct BatchItem {
   const float* y;
   const float* a;
   const float* b;
+  float* a_fill;  // OTTX_BATCH_UNIFORM_A: a, written with 1 / n by the init kernel (else null)
+  float* b_fill;
   float* f;
   float* g;
   float* g_save;
@@ -3838,14 +3840,87 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchItem* __rest
 struct BatchMail {
   uint64_t progress;
   uint32_t done;
-  uint32_t pad;
+  uint32_t live[2];  // live work items of the g- and f-updates after the block
 };
 
-__global__ void batch_progress_kernel(uint32_t* counters, BatchMail* mail, uint32_t seq) {
-  if (threadIdx.x != 0) return;
+// End of a block.  When problems have finished since the last rebuild, the
+// live work lists are rebuilt from the full ones without the items of
+// finished problems, keeping their longest-first order (a stable compaction:
+// wave ballots and per-wave offsets, 1024 items per round).  Then the
+// progress (finished problems, live items, completed blocks) goes to the
+// mailbox.  counters: batch_done (kBatchDoneBits).
+constexpr int kCompactThreads = 1024;
+constexpr int kCompactPer = 8;  // items per thread per round of loads
+
+__global__ __launch_bounds__(kCompactThreads) void batch_progress_kernel(
+    uint32_t* counters, BatchMail* mail, uint32_t seq, int K,
+    const int4* __restrict__ work_g, const int4* __restrict__ work_f, int4* __restrict__ live_g,
+    int4* __restrict__ live_f, uint32_t* __restrict__ live_n, int n_g, int n_f) {
+  constexpr int kWaves = kCompactThreads / 64;
+  static_assert(kCompactPer * kWaves == 128, "one wave scans the round's counts, two per lane");
+  __shared__ uint32_t s_cnt[kCompactPer * kWaves];  // kept items per (sub-round, wave), list order
+  __shared__ uint32_t s_off[kCompactPer * kWaves];  // where each (sub-round, wave) writes
+  __shared__ uint32_t s_total;
+  __shared__ uint32_t s_done[65536 / 32];  // finished problems, one bit each (K <= 65535)
+  const int tid = threadIdx.x;
   const uint32_t dn = __hip_atomic_load(&counters[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (dn != counters[2]) {
+    for (int i = tid; i < (K + 31) / 32; i += kCompactThreads) s_done[i] = counters[kBatchDoneBits + i];
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int dir = 0; dir < 2; ++dir) {
+      const int4* __restrict__ src = dir ? work_f : work_g;
+      int4* __restrict__ dst = dir ? live_f : live_g;
+      const int n = dir ? n_f : n_g;
+      uint32_t base = 0;
+      for (int c0 = 0; c0 < n; c0 += kCompactThreads * kCompactPer) {
+        int4 w[kCompactPer];
+        bool keep[kCompactPer];
+        uint32_t rank[kCompactPer];  // kept items before this one in its wave
+#pragma unroll
+        for (int u = 0; u < kCompactPer; ++u) {
+          const int i = c0 + u * kCompactThreads + tid;
+          w[u] = i < n ? src[i] : make_int4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < kCompactPer; ++u) {
+          const int i = c0 + u * kCompactThreads + tid;
+          keep[u] = i < n && ((s_done[w[u].x >> 5] >> (w[u].x & 31)) & 1u) == 0u;
+          const uint64_t mask = __ballot(keep[u]);
+          if (lane == 0) s_cnt[u * kWaves + wave] = static_cast<uint32_t>(__popcll(mask));
+          rank[u] = static_cast<uint32_t>(__popcll(mask & ((uint64_t{1} << lane) - 1u)));
+        }
+        __syncthreads();
+        if (wave == 0) {
+          // exclusive scan of the 128 counts in list order, two per lane
+          const uint32_t a = s_cnt[2 * lane], b = s_cnt[2 * lane + 1];
+          uint32_t incl = a + b;
+#pragma unroll
+          for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+          }
+          const uint32_t excl = base + incl - (a + b);
+          s_off[2 * lane] = excl;
+          s_off[2 * lane + 1] = excl + a;
+          if (lane == 63) s_total = incl;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kCompactPer; ++u)
+          if (keep[u]) dst[s_off[u * kWaves + wave] + rank[u]] = w[u];
+        base += s_total;
+        __syncthreads();  // s_cnt, s_off and s_total are reused by the next round
+      }
+      if (tid == 0) live_n[dir] = base;
+    }
+    if (tid == 0) counters[2] = dn;
+  }
+  if (tid != 0) return;
   const uint32_t blocks = counters[1] + 1u;
   counters[1] = blocks;
+  __hip_atomic_store(&mail->live[0], live_n[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&mail->live[1], live_n[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   __hip_atomic_store(&mail->done, dn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   __hip_atomic_store(&mail->progress, (static_cast<uint64_t>(seq) << 32) | blocks, __ATOMIC_RELEASE,
                      __HIP_MEMORY_SCOPE_SYSTEM);
@@ -3871,11 +3946,22 @@ __global__ void batch_init_kernel(const BatchItem* __restrict__ items, int outer
     if (blockIdx.x == 0) {
       batch_done[0] = 0u;  // finished problems
       batch_done[1] = 0u;  // completed blocks
+      batch_done[2] = 0u;  // finished problems at the last rebuild of the live work lists
     }
+    if ((blockIdx.x & 31) == 0) batch_done[kBatchDoneBits + (blockIdx.x >> 5)] = 0u;
   }
   for (int k = t; k < outer; k += blockDim.x) it.errors[k] = -1.f;
   for (int k = t; k < it.n_counters; k += blockDim.x) it.counters[k] = 0u;
-  // DefaultInitializer: 0, or -inf where the weight is 0 (bit 0: f, bit 1: g)
+  // default weights (segment.py:100-104), float64 1 / n rounded to float32
+  if (it.a_fill != nullptr) {
+    const float w = static_cast<float>(1.0 / static_cast<double>(it.n));
+    for (int64_t k = t; k < it.n; k += blockDim.x) it.a_fill[k] = w;
+  }
+  if (it.b_fill != nullptr) {
+    const float w = static_cast<float>(1.0 / static_cast<double>(it.m));
+    for (int64_t k = t; k < it.m; k += blockDim.x) it.b_fill[k] = w;
+  }
+  // DefaultInitializer (each weight written above is read back by the thread that wrote it): 0, or -inf where the weight is 0 (bit 0: f, bit 1: g)
   if (zero_init & 1)
     for (int64_t k = t; k < it.n; k += blockDim.x) it.f[k] = it.a[k] > 0.f ? 0.f : -INFINITY;
   if (zero_init & 2)
@@ -3975,7 +4061,7 @@ struct BatchPlan {
   std::vector<Layout> L;
   std::vector<size_t> base;  // offset of each problem's layout
   size_t off_done = 0, off_items = 0, off_args_g = 0, off_args_f = 0, off_work_g = 0, off_work_f = 0,
-         off_out = 0, off_stats = 0, off_prep = 0, total = 0;
+         off_live_g = 0, off_live_f = 0, off_live_n = 0, off_out = 0, off_stats = 0, off_prep = 0, total = 0;
   int stat_chunks = 0;
   int64_t n_prep = 0;  // (problem, side, block, kind) items of the fragment / norm preparation
   int64_t n_work_g = 0, n_work_f = 0;
@@ -3990,9 +4076,11 @@ int batch_rows_per_split(const KernelCfg& k, const BatchPlan& B, bool g_dir, int
   double pairs = 0.0;
   int64_t max_in = 0;
   for (int i = 0; i < B.K; ++i) {
-    const int64_t n_pad = pad_rows(B.pbs[i].n, B.pad), m_pad = pad_rows(B.pbs[i].m, B.pad);
-    pairs += static_cast<double>(n_pad) * static_cast<double>(m_pad);
-    max_in = std::max(max_in, g_dir ? n_pad : m_pad);
+    // streamed rows are padded to the split unit, columns to the tile
+    const int64_t n_in = pad_rows(g_dir ? B.pbs[i].n : B.pbs[i].m, 32 * k.mt);
+    const int64_t n_out = pad_rows(g_dir ? B.pbs[i].m : B.pbs[i].n, k.cols());
+    pairs += static_cast<double>(n_in) * static_cast<double>(n_out);
+    max_in = std::max(max_in, n_in);
   }
   const int64_t unit = 32 * k.mt;  // an even number of streamed tiles per split
   int64_t rows = static_cast<int64_t>(pairs / (static_cast<double>(k.cols()) * target));
@@ -4004,6 +4092,7 @@ int check_batch(const ottx_batch* bt) {
   // count <= 65535: problems index the y and z grid dimensions of the batched kernels
   if (!bt || bt->count < 1 || bt->count > 65535 || bt->d < 1 || bt->d > 256) return OTTX_EINVAL;
   if (!bt->x || !bt->y || !bt->a || !bt->b || !bt->x_off || !bt->y_off) return OTTX_EINVAL;
+  if (bt->flags & ~(OTTX_BATCH_UNIFORM_A | OTTX_BATCH_UNIFORM_B)) return OTTX_EINVAL;
   if (bt->precision != OTTX_PREC_FP32 && bt->precision != OTTX_PREC_BF16_SPLIT) return OTTX_EINVAL;
   if (!(bt->inv_scale_cost > 0.0) || bt->relative_epsilon < 0.0 || bt->mean_rows_x < 0.0 || bt->mean_rows_y < 0.0)
     return OTTX_EINVAL;
@@ -4086,7 +4175,7 @@ int make_batch_plan(const ottx_batch* bt, const ottx_sinkhorn_opts* o, BatchPlan
     off = align_up(off + bytes);
     return at;
   };
-  B->off_done = take(256);
+  B->off_done = take(sizeof(uint32_t) * (kBatchDoneBits + (B->K + 31) / 32));
   B->off_items = take(sizeof(BatchItem) * B->K);
   B->off_args_g = take(sizeof(HalfStepArgs) * B->K);
   B->off_args_f = take(sizeof(HalfStepArgs) * B->K);
@@ -4096,8 +4185,10 @@ int make_batch_plan(const ottx_batch* bt, const ottx_sinkhorn_opts* o, BatchPlan
       const int64_t n_in = dir == 0 ? B->pbs[i].n : B->pbs[i].m;
       const int64_t n_out = dir == 0 ? B->pbs[i].m : B->pbs[i].n;
       PassPlan p;
-      p.tiles = static_cast<int>(pad_rows(n_out, B->pad) / k.cols());
-      p.n_in_rb = static_cast<int>(pad_rows(n_in, B->pad) / 16);
+      p.tiles = static_cast<int>(pad_rows(n_out, k.cols()) / k.cols());
+      // rows past n_in carry a -inf bias, so a split only streams the rows up
+      // to the next multiple of its unit (two tiles), not the storage padding
+      p.n_in_rb = static_cast<int>(pad_rows(n_in, 32 * k.mt) / 16);
       p.rb_per_split = (dir == 0 ? rows_g : rows_f) / 16;
       p.splits = (p.n_in_rb + p.rb_per_split - 1) / p.rb_per_split;
       if (!even_tiles(k, p)) return OTTX_EINVAL;
@@ -4108,6 +4199,9 @@ int make_batch_plan(const ottx_batch* bt, const ottx_sinkhorn_opts* o, BatchPlan
   if (B->n_work_g > (int64_t{1} << 31) - 1 || B->n_work_f > (int64_t{1} << 31) - 1) return OTTX_EINVAL;
   B->off_work_g = take(sizeof(int4) * B->n_work_g);
   B->off_work_f = take(sizeof(int4) * B->n_work_f);
+  B->off_live_g = take(sizeof(int4) * B->n_work_g);
+  B->off_live_f = take(sizeof(int4) * B->n_work_f);
+  B->off_live_n = take(sizeof(uint32_t) * 2);
   B->n_prep = 0;
   for (int i = 0; i < B->K; ++i)
     for (int side = 0; side < 2; ++side) {
@@ -4229,6 +4323,8 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
     it.y = pb.y;
     it.a = pb.a;
     it.b = pb.b;
+    it.a_fill = (bt->flags & OTTX_BATCH_UNIFORM_A) ? bt->a + bt->x_off[i] : nullptr;
+    it.b_fill = (bt->flags & OTTX_BATCH_UNIFORM_B) ? bt->b + bt->y_off[i] : nullptr;
     it.f = out->f + bt->x_off[i];
     it.g = out->g + bt->y_off[i];
     it.g_save = reinterpret_cast<float*>(base + L.g_save);
@@ -4266,6 +4362,7 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
       a->force_scan = force_scan ? 1 : 0;
       a->threshold = static_cast<float>(o->threshold);
       a->batch_done = batch_done;
+      a->problem = i;
     }
     ga.out_save = it.g_save;
     ga.lse_shift = ga.lse_store = reinterpret_cast<float*>(base + L.lse_y);
@@ -4304,11 +4401,20 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
   };
   by_cost(work_g, wg, true);
   by_cost(work_f, wf, false);
+  // the live lists start as the full ones; the queue heads at 0
+  std::memcpy(sb + (B.off_live_g - B.off_items), work_g, sizeof(int4) * wg);
+  std::memcpy(sb + (B.off_live_f - B.off_items), work_f, sizeof(int4) * wf);
+  uint32_t* live_n = reinterpret_cast<uint32_t*>(sb + (B.off_live_n - B.off_items));
+  live_n[0] = static_cast<uint32_t>(wg);
+  live_n[1] = static_cast<uint32_t>(wf);
   BatchItem* d_items = reinterpret_cast<BatchItem*>(ws + B.off_items);
   HalfStepArgs* d_args_g = reinterpret_cast<HalfStepArgs*>(ws + B.off_args_g);
   HalfStepArgs* d_args_f = reinterpret_cast<HalfStepArgs*>(ws + B.off_args_f);
   int4* d_work_g = reinterpret_cast<int4*>(ws + B.off_work_g);
   int4* d_work_f = reinterpret_cast<int4*>(ws + B.off_work_f);
+  int4* d_live_g = reinterpret_cast<int4*>(ws + B.off_live_g);
+  int4* d_live_f = reinterpret_cast<int4*>(ws + B.off_live_f);
+  uint32_t* d_live_n = reinterpret_cast<uint32_t*>(ws + B.off_live_n);
   OTTX_HIP(hipMemcpyAsync(d_items, stage, tbl_bytes, hipMemcpyHostToDevice, st));
 
   // Preparation of every problem.
@@ -4329,9 +4435,15 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
   const uint32_t f_plain = kModeWritePot | kModeCheckDone;
   const uint32_t f_err = kModeWritePot | kModeCheckDone | (balanced ? 0u : kModeErrRow);
   const dim3 block(64 * k.w);
+  // One workgroup per live item.  Launches made by the host loop take the
+  // live counts of the last block the host has seen finish; the lists only
+  // shrink, so those counts bound the ones the launches will read.  A
+  // captured block keeps the full counts.
+  uint32_t grid_g = static_cast<uint32_t>(B.n_work_g), grid_f = static_cast<uint32_t>(B.n_work_f);
   auto launch = [&](bool g_dir, uint32_t mode, hipStream_t s2) -> int {
-    hipLaunchKernelGGL(k.bfn, dim3(static_cast<unsigned>(g_dir ? B.n_work_g : B.n_work_f)), block, 0, s2,
-                       g_dir ? d_args_g : d_args_f, g_dir ? d_work_g : d_work_f, mode);
+    hipLaunchKernelGGL(k.bfn, dim3(std::max(1u, g_dir ? grid_g : grid_f)), block, 0, s2,
+                       g_dir ? d_args_g : d_args_f, g_dir ? d_live_g : d_live_f,
+                       static_cast<const uint32_t*>(d_live_n + (g_dir ? 0 : 1)), mode);
     OTTX_LAUNCHED();
     return OTTX_OK;
   };
@@ -4345,7 +4457,10 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
       r = launch(true, itn == inner - 1 ? g_err : g_plain, s2);
       if (r != OTTX_OK) return r;
     }
-    hipLaunchKernelGGL(batch_progress_kernel, dim3(1), dim3(64), 0, s2, batch_done, bmail, seq);
+    hipLaunchKernelGGL(batch_progress_kernel, dim3(1), dim3(kCompactThreads), 0, s2, batch_done, bmail, seq,
+                       K, static_cast<const int4*>(d_work_g),
+                       static_cast<const int4*>(d_work_f), d_live_g, d_live_f, d_live_n,
+                       static_cast<int>(B.n_work_g), static_cast<int>(B.n_work_f));
     OTTX_LAUNCHED();
     return OTTX_OK;
   };
@@ -4405,6 +4520,8 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
     if (blk >= 1) {
       if (!wait_block(static_cast<uint32_t>(blk))) { status = OTTX_EHIP; break; }
       if (__atomic_load_n(&bmail->done, __ATOMIC_ACQUIRE) >= static_cast<uint32_t>(K)) break;
+      grid_g = __atomic_load_n(&bmail->live[0], __ATOMIC_ACQUIRE);
+      grid_f = __atomic_load_n(&bmail->live[1], __ATOMIC_ACQUIRE);
     }
   }
   if (exec) (void)hipGraphExecDestroy(exec);

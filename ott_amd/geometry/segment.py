@@ -44,11 +44,13 @@ def _counts_and_order(num: int, num_segments: Optional[int], segment_ids, indice
 
 def contiguous_segments(x: torch.Tensor, a: Optional[torch.Tensor] = None, num_segments: Optional[int] = None,
                         segment_ids=None, indices_are_sorted: bool = False,
-                        num_per_segment: Optional[Sequence[int]] = None
-                        ) -> Tuple[torch.Tensor, torch.Tensor, np.ndarray]:
+                        num_per_segment: Optional[Sequence[int]] = None, default_weights: bool = True
+                        ) -> Tuple[torch.Tensor, Optional[torch.Tensor], np.ndarray]:
   """Rows grouped by segment, their weights and the offsets [K + 1].
 
-  Weights default to ``1 / n_k`` within each segment (segment.py:100-104).
+  Weights default to ``1 / n_k`` within each segment (segment.py:100-104);
+  with ``default_weights=False`` absent weights stay ``None`` (the batched
+  solve then writes them on the device).
   """
   num = x.shape[0]
   counts, order = _counts_and_order(num, num_segments, segment_ids, indices_are_sorted, num_per_segment)
@@ -60,10 +62,10 @@ def contiguous_segments(x: torch.Tensor, a: Optional[torch.Tensor] = None, num_s
     x = x.index_select(0, idx)
     if a is not None:
       a = a.index_select(0, idx)
-  if a is None:
+  if a is None and default_weights:
     per = torch.from_numpy(np.repeat(1.0 / counts, counts).astype(np.float32))
     a = per.to(x.device)
-  return x.contiguous(), a.contiguous(), offsets
+  return x.contiguous(), (None if a is None else a.contiguous()), offsets
 
 
 def segment_point_cloud(x, a=None, num_segments: Optional[int] = None, max_measure_size: Optional[int] = None,

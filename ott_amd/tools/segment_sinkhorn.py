@@ -119,9 +119,17 @@ def segment_sinkhorn_outputs(
   wx = None if weights_x is None else as_device_f32(weights_x, device)
   wy = None if weights_y is None else as_device_f32(weights_y, device)
   xs, a, x_off = segment.contiguous_segments(x, wx, num_segments, segment_ids_x, indices_are_sorted,
-                                             num_per_segment_x)
+                                             num_per_segment_x, default_weights=False)
   ys, b, y_off = segment.contiguous_segments(y, wy, num_segments, segment_ids_y, indices_are_sorted,
-                                             num_per_segment_y)
+                                             num_per_segment_y, default_weights=False)
+  # absent weights: the solve writes 1 / n_k (segment.py:100-104) into these
+  flags = 0
+  if a is None:
+    a = torch.empty(xs.shape[0], dtype=torch.float32, device=device)
+    flags |= _lib.BATCH_UNIFORM_A
+  if b is None:
+    b = torch.empty(ys.shape[0], dtype=torch.float32, device=device)
+    flags |= _lib.BATCH_UNIFORM_B
   K = len(x_off) - 1
   if len(y_off) - 1 != K:
     raise ValueError("x and y must hold the same number of segments")
@@ -144,7 +152,7 @@ def segment_sinkhorn_outputs(
     eps_c = (ctypes.c_double * K)(*eps_list)
   elif eps_rel == 0.0:
     eps_c = (ctypes.c_double * K)(*([eps_abs] * K))
-  bt = _lib.Batch(count=K, d=x.shape[1], precision=_lib.PRECISIONS[precision], x=xs.data_ptr(),
+  bt = _lib.Batch(count=K, d=x.shape[1], precision=_lib.PRECISIONS[precision], flags=flags, x=xs.data_ptr(),
                   y=ys.data_ptr(), a=a.data_ptr(), b=b.data_ptr(), x_off=x_off_c, y_off=y_off_c,
                   epsilon=eps_c, relative_epsilon=eps_rel,
                   mean_rows_x=float(px) if eps_rel > 0 and padded_statistics else 0.0,
@@ -176,11 +184,13 @@ def segment_sinkhorn_outputs(
   _lib.check("ottx_sinkhorn_batched_solve", lib.ottx_sinkhorn_batched_solve(
       ctypes.byref(bt), ctypes.byref(opts), ctypes.byref(out), ws.data_ptr(), ws.numel(), _lib.stream_ptr()))
   del ws
+  # the host arrays as tensors without per-element conversions
   return SegmentSinkhornOutput(
-      f=f, g=g, errors=errors, reg_ot_cost=torch.tensor(list(reg), dtype=torch.float64),
-      converged=torch.tensor(list(conv), dtype=torch.bool), n_iters=torch.tensor(list(iters)),
-      epsilon=torch.tensor(list(eps_out), dtype=torch.float64), x_offsets=x_off, y_offsets=y_off, a=a, b=b,
-      n_blocks=int(out.n_blocks))
+      f=f, g=g, errors=errors, reg_ot_cost=torch.from_numpy(np.ctypeslib.as_array(reg).copy()),
+      converged=torch.from_numpy(np.ctypeslib.as_array(conv) != 0),
+      n_iters=torch.from_numpy(np.ctypeslib.as_array(iters).astype(np.int64)),
+      epsilon=torch.from_numpy(np.ctypeslib.as_array(eps_out).copy()), x_offsets=x_off, y_offsets=y_off, a=a,
+      b=b, n_blocks=int(out.n_blocks))
 
 
 def segment_sinkhorn(

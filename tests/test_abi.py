@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
   missing = [n for n in sorted(_declared()) if not hasattr(lib, n)]
   assert not missing, f"missing exports: {missing}"
   assert set(_lib.EXPORTED_SYMBOLS) == _declared()
-  assert lib.ottx_abi_version() == 4
+  assert lib.ottx_abi_version() == 5
   assert lib.ottx_strerror(-1) == b"invalid argument"
 
 

@@ -57,6 +57,9 @@ def test_batched_matches_oracle_segment_sinkhorn(precision):
                                     precision=precision)
   refs = oracle.segment_sinkhorn(x, y, nx, ny, max_measure_size=P, sinkhorn_kwargs=kw, epsilon=0.05,
                                  relative_epsilon="mean")
+  # the default weights the solve wrote: 1 / n_k per segment (segment.py:100-104), float64 then float32
+  np.testing.assert_array_equal(out.a.cpu().numpy(), np.repeat(1.0 / np.array(nx), nx).astype(np.float32))
+  np.testing.assert_array_equal(out.b.cpu().numpy(), np.repeat(1.0 / np.array(ny), ny).astype(np.float32))
   tol = TOL[precision]
   for k, ref in enumerate(refs):
     n, m = SIZES[k]
