@@ -135,6 +135,8 @@ struct HalfStepArgs {
   int32_t outer;
   int32_t force_scan;
   float threshold;
+  SolveMail* mail;       // single solves: the block finalizer publishes the loop's progress here
+  uint32_t seq;          // ... tagged with the solve's sequence number
   uint32_t* batch_done;  // batched solves: count of problems whose loop finished, then one bit per problem
   int32_t problem;       // batched solves: index of the problem (its bit in batch_done + kBatchDoneBits)
   // costs other than the squared Euclidean (PREC 2): in_frag / out_frag are

@@ -151,6 +151,14 @@ __device__ inline void finalize_block(const HalfStepArgs& p, double col, double 
       atomicAdd(p.batch_done, 1u);
     }
   }
+  if (p.mail != nullptr) {
+    // the loop's progress (and this block's error) for the host, which keeps
+    // at most one block queued ahead; the stop bit tells it the results are
+    // due (the output finalizer writes them)
+    __hip_atomic_store(&p.mail->last_err, err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const uint64_t v = (static_cast<uint64_t>(p.seq) << 32) | (cont ? 0u : 0x80000000u) | static_cast<uint32_t>(k + 1);
+    __hip_atomic_store(&p.mail->progress, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -1550,14 +1550,6 @@ __global__ void solve_finalize_kernel(const float* f, const float* __restrict__ 
   __shared__ double red[2][256];
   __shared__ double cen[2][kDualBlocks];
   __shared__ int s_last;
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-    // progress of the loop (and the last error) for the host, which keeps
-    // at most one block queued ahead
-    __hip_atomic_store(&mail->last_err, st->last_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    const uint32_t done_bit = st->done ? 0x80000000u : 0u;  // the loop has stopped: results follow
-    const uint64_t v = (static_cast<uint64_t>(seq) << 32) | done_bit | static_cast<uint32_t>(st->block);
-    __hip_atomic_store(&mail->progress, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
   if (!st->done || st->finalized) return;
   const int tid = threadIdx.x;
   const int side = blockIdx.y;
@@ -3015,10 +3007,19 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
   rc = prepare(pb, k, L, workspace, st, &P, &pinit);
   if (rc != OTTX_OK) return rc;
 
+  // Results reach the host through pinned memory: the block finalizer
+  // publishes each block's progress, the output finalizer the results.
+  SolveMail* mail = reinterpret_cast<SolveMail*>(static_cast<char*>(hh->mailbox) + 1024);
+  const uint32_t seq = ++hh->seq;
+  mail->ready = 0u;
+  __atomic_store_n(&mail->progress, static_cast<uint64_t>(0), __ATOMIC_RELEASE);
+
   const double eps = pb->epsilon;
   HalfStepArgs ga = base_args(pb, k, gp, P, 1, out->f, out->g, eps);
   HalfStepArgs fa = base_args(pb, k, fp, P, 0, out->g, out->f, eps);
   for (HalfStepArgs* a : {&ga, &fa}) {
+    a->mail = mail;
+    a->seq = seq;
     a->norm_p = o->norm_error;
     a->momentum = static_cast<float>(o->momentum);
     a->errors = out->errors;
@@ -3203,12 +3204,11 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     if (rc != OTTX_OK) return rc;
   }
 
-  // Results reach the host through pinned memory: the finalizer kernel at the
-  // end of every block writes them (once) after the block that stops the loop.
-  SolveMail* mail = reinterpret_cast<SolveMail*>(static_cast<char*>(hh->mailbox) + 1024);
-  const uint32_t seq = ++hh->seq;
-  mail->ready = 0u;
-  __atomic_store_n(&mail->progress, static_cast<uint64_t>(0), __ATOMIC_RELEASE);
+  // The output finalizer writes the results (once) after the block that
+  // stops the loop.  It is queued behind each of the first kEagerFinalize
+  // blocks (and behind every block of a graph); a longer loop saves its
+  // launches and the host queues it once it has seen the loop stop.
+  constexpr int kEagerFinalize = 3;
   double* dual_part = reinterpret_cast<double*>(base + L.dual_part);
   auto launch_finalize = [&](hipStream_t s) -> int {
     if (rc_mode) {
@@ -3250,9 +3250,11 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
       rc = launch_halfstep(k, g_plain, st);  // g-update of iteration 0
       if (rc != OTTX_OK) return rc;
     }
-    auto launch_all = [&](hipStream_t s) -> int {
+    bool fin_queued = false;  // an output finalizer follows the last block queued
+    auto launch_all = [&](hipStream_t s, bool with_finalize) -> int {
       const int r = launch_block(s);
-      return r != OTTX_OK ? r : launch_finalize(s);
+      fin_queued = with_finalize;
+      return (r != OTTX_OK || !with_finalize) ? r : launch_finalize(s);
     };
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -3265,7 +3267,7 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     if (o->use_graph && inner >= 4 && static_cast<double>(pb->n) * static_cast<double>(pb->m) <= 16777216.0) {
       OTTX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
       OTTX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-      rc = launch_all(cap);
+      rc = launch_all(cap, true);
       hipGraph_t g = nullptr;
       const hipError_t e1 = hipStreamEndCapture(cap, &g);
       if (rc != OTTX_OK || e1 != hipSuccess) {
@@ -3282,7 +3284,7 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     }
     int status = OTTX_OK;
     // Waits until `blocks` blocks have finished (or the loop has stopped),
-    // spinning on the pinned mailbox that every block's finalizer updates (no
+    // spinning on the pinned mailbox that every block finalizer updates (no
     // event records in the stream, whose system-scope release would put a gap
     // between blocks, and no blocking wait with its wake-up latency).  The
     // stream itself is queried now and then, so an error or a drained stream
@@ -3291,19 +3293,7 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
       for (uint32_t spin = 1;; ++spin) {
         if (mail_ready()) return true;
         bool stopped = false;
-        if (progress(&stopped) >= blocks) {
-          if (!stopped) return mail_ready();
-          // the loop has stopped: its finalizer is writing the results
-          while (!mail_ready()) {
-            const hipError_t q = hipStreamQuery(st);
-            if (q != hipSuccess && q != hipErrorNotReady) {
-              status = OTTX_EHIP;
-              return true;
-            }
-            if (q == hipSuccess) return true;
-          }
-          return true;
-        }
+        if (progress(&stopped) >= blocks) return stopped;
         if ((spin & 1023u) == 0) {
           const hipError_t q = hipStreamQuery(st);
           if (q == hipSuccess) return mail_ready();
@@ -3318,8 +3308,9 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     for (int blk = 0; blk < total_blocks; ++blk) {
       if (exec) {
         if (hipGraphLaunch(exec, st) != hipSuccess) { status = OTTX_EHIP; break; }
+        fin_queued = true;
       } else {
-        status = launch_all(st);
+        status = launch_all(st, blk < kEagerFinalize);
         if (status != OTTX_OK) break;
       }
       if (o->progress_fn != nullptr) {
@@ -3355,10 +3346,17 @@ int ottx_sinkhorn_solve(const ottx_problem* pb, const ottx_sinkhorn_opts* o, ott
     if (graph) (void)hipGraphDestroy(graph);
     if (cap) (void)hipStreamDestroy(cap);
     if (status != OTTX_OK) return status;
-    // The loop stops at the latest in the last block, whose finalizer then
-    // writes the mailbox; wait for it if the speculation has not seen it yet.
+    // The loop stops at the latest in the last block queued; the output
+    // finalizer behind it (queued now when none is) writes the mailbox.
     if (!mail_ready()) {
-      OTTX_HIP(hipStreamSynchronize(st));
+      if (!fin_queued && (rc = launch_finalize(st)) != OTTX_OK) return rc;
+      for (uint32_t spin = 1; !mail_ready(); ++spin) {
+        if ((spin & 1023u) == 0) {
+          const hipError_t q = hipStreamQuery(st);
+          if (q == hipSuccess) break;
+          if (q != hipErrorNotReady) return OTTX_EHIP;
+        }
+      }
       if (!mail_ready()) return OTTX_EHIP;
     }
     const SolveMail* fm = mail;
