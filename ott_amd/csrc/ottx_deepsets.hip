@@ -479,28 +479,37 @@ __global__ void head_out_kernel(const float* __restrict__ A, const float* __rest
 // acc = In . W^T over K, In a [16 RB x K] bf16 tile in LDS (row stride LI),
 // W [256 x K] bf16 rows in global memory (read by every workgroup, so from
 // L2).  Wave cw / 64 owns the output columns [cw, cw + 64): acc[i][j] holds
-// rows 16 i + 4 (lane / 16) + r and column cw + 16 j + lane % 16.
+// rows 16 i + 4 (lane / 16) + r and column cw + 16 j + lane % 16.  The wave's
+// whole slice of W (K / 32 steps x 4 fragments, at most 128 registers) is
+// loaded before the first MFMA: one L2 round trip per layer instead of one per
+// step (the fused kernels run one workgroup per CU, so registers are free).
 template <int RB, int K, int LI>
 __device__ __forceinline__ void layer_tile(const bf16_t* In, const bf16_t* __restrict__ W, int cw, int lane,
                                            f32x4 (&acc)[RB][4]) {
+  constexpr int KS = K / 32;
   const int lr = lane & 15, lk = 8 * (lane >> 4);
+  bf16x8 bw[KS][4];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      bw[ks][j] =
+          __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(W + (cw + 16 * j + lr) * K + 32 * ks + lk));
 #pragma unroll
   for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    bf16x8 bw[4], ai[RB];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      bw[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(W + (cw + 16 * j + lr) * K + k0 + lk));
+  for (int ks = 0; ks < KS; ++ks) {
+    bf16x8 ai[RB];
 #pragma unroll
     for (int i = 0; i < RB; ++i)
-      ai[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&In[(16 * i + lr) * LI + k0 + lk]));
+      ai[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(&In[(16 * i + lr) * LI + 32 * ks + lk]));
 #pragma unroll
     for (int i = 0; i < RB; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai[i], bw[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai[i], bw[ks][j], acc[i][j], 0, 0, 0);
   }
 }
 
@@ -514,11 +523,16 @@ __device__ __forceinline__ void load_tile(const bf16_t* __restrict__ src, int64_
 }
 
 // relu(acc + bias) rounded to bf16 into an LDS tile; rows >= valid are zero
-// (as the unfused GEMM epilogue writes them).
+// (as the unfused GEMM epilogue writes them).  The lane's four bias values are
+// loaded before the first store: the stores go through a generic pointer, so a
+// load left after one of them would be issued (and waited for) after it.
 template <int RB, int LO>
 __device__ __forceinline__ void relu_bf16_tile(const f32x4 (&acc)[RB][4], const float* __restrict__ bias, int cw,
                                                int lane, int64_t r0, int64_t valid, bf16_t* dst) {
   const int lr = lane & 15;
+  float bv[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) bv[j] = bias[cw + 16 * j + lr];
 #pragma unroll
   for (int i = 0; i < RB; ++i)
 #pragma unroll
@@ -527,7 +541,7 @@ __device__ __forceinline__ void relu_bf16_tile(const f32x4 (&acc)[RB][4], const 
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int col = cw + 16 * j + lr;
-        const float v = r0 + row < valid ? fmaxf(acc[i][j][r] + bias[col], 0.f) : 0.f;
+        const float v = r0 + row < valid ? fmaxf(acc[i][j][r] + bv[j], 0.f) : 0.f;
         dst[row * LO + col] = to_bf16(v);
       }
     }
@@ -561,6 +575,9 @@ __global__ __launch_bounds__(256) void encoder_fused_kernel(const bf16_t* __rest
   __syncthreads();
   layer_tile<RB, 256, LZ>(Zb, W2, cw, lane, acc);
   const int lr = lane & 15;
+  float bv[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) bv[j] = b2[cw + 16 * j + lr];
 #pragma unroll
   for (int i = 0; i < RB; ++i)
 #pragma unroll
@@ -570,7 +587,7 @@ __global__ __launch_bounds__(256) void encoder_fused_kernel(const bf16_t* __rest
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int col = cw + 16 * j + lr;
-        H2[q * 256 + col] = q < rows ? acc[i][j][r] + b2[col] : 0.f;
+        H2[q * 256 + col] = q < rows ? acc[i][j][r] + bv[j] : 0.f;
       }
     }
 }
@@ -601,19 +618,36 @@ __global__ __launch_bounds__(256) void head_fused_kernel(const bf16_t* __restric
   const int cw = wave * 64;  // first output column of this wave
   const int lr = lane & 15;
   f32x4 acc[4][4];
-  // layer 1, with the per-problem bias of each row
+  // layer 1, with the per-problem bias of each row.  The lane's 16 rows'
+  // problems and their 64 bias values are all loaded before the first store
+  // into LDS (a generic-pointer store orders every later load after it).
   layer_tile<4, KIN, LY>(Ys, Wy, cw, lane, acc);
+  int pb[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      // clamped, not skipped: a load under a branch would be waited for alone
+      const int64_t q = min(r0 + 16 * i + 4 * (lane >> 4) + r, rows - 1);
+      pb[i][r] = prob[q];
+    }
+  float hv[4][4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) hv[i][r][j] = hctx[static_cast<int64_t>(pb[i][r]) * 256 + cw + 16 * j + lr];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = 16 * i + 4 * (lane >> 4) + r;
       const int64_t q = r0 + row;
-      const float* hb = hctx + static_cast<int64_t>(q < rows ? prob[q] : 0) * 256;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int col = cw + 16 * j + lr;
-        const float v = q < rows ? fmaxf(acc[i][j][r] + hb[col], 0.f) : 0.f;
+        const float v = q < rows ? fmaxf(acc[i][j][r] + hv[i][r][j], 0.f) : 0.f;
         Zs[row * LZ + col] = to_bf16(v);
       }
     }
