@@ -38,8 +38,13 @@ class SinkhornOutput(NamedTuple):
   threshold: Optional[float] = None
   converged: Optional[bool] = None
   inner_iterations: Optional[int] = None
+  # the count the solve itself reported (beyond OTT's fields): n_iters without
+  # a reduction over ``errors`` and its device-to-host wait
+  solve_n_iters: Optional[int] = None
 
   def set(self, **kwargs: Any) -> "SinkhornOutput":
+    if "errors" in kwargs or "inner_iterations" in kwargs:
+      kwargs.setdefault("solve_n_iters", None)  # no longer the count of these errors
     return self._replace(**kwargs)
 
   @property
@@ -65,6 +70,8 @@ class SinkhornOutput(NamedTuple):
   @property
   def n_iters(self) -> int:
     """sinkhorn.py:431-434."""
+    if self.solve_n_iters is not None:
+      return self.solve_n_iters
     return int(torch.sum(self.errors != -1).item()) * self.inner_iterations
 
   @property
@@ -403,7 +410,8 @@ def _prepare(ot_prob: linear_problem.LinearProblem, solver: Sinkhorn,
     return SinkhornOutput(
         potentials=(f, g), errors=errors, reg_ot_cost=float(out.reg_ot_cost),
         ot_prob=ot_prob, threshold=float(solver.threshold),
-        converged=bool(out.converged), inner_iterations=int(solver.inner_iterations))
+        converged=bool(out.converged), inner_iterations=int(solver.inner_iterations),
+        solve_n_iters=int(out.n_iters))
 
   launch.keep = (keep, progress)  # weights referenced by pb, the callback, alive until the call
   return launch

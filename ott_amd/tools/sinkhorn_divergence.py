@@ -109,17 +109,20 @@ def sinkhorn_divergence(
   geom_xy, geom_x, geom_y = (tuple(geoms) + (None,) * 3)[:3]
   num_a, num_b = geom_xy.shape
   device = geom_xy.device
+  # default weights are known to be uniform: no device check, no host sync
+  uniform = (a is None, b is None)
   a = torch.full((num_a,), 1.0 / num_a, device=device) if a is None else as_device_f32(a, device)
   b = torch.full((num_b,), 1.0 / num_b, device=device) if b is None else as_device_f32(b, device)
   out = _sinkhorn_divergence(geom_xy, geom_x, geom_y, a=a, b=b, share_epsilon=share_epsilon,
                              symmetric_sinkhorn=symmetric_sinkhorn, offset_yy=offset_static_b,
-                             **dict(solve_kwargs))
+                             uniform=uniform, **dict(solve_kwargs))
   return out.divergence, out
 
 
 def _sinkhorn_divergence(geometry_xy: PointCloud, geometry_xx: PointCloud, geometry_yy: Optional[PointCloud],
                          a: torch.Tensor, b: torch.Tensor, share_epsilon: bool, symmetric_sinkhorn: bool,
-                         offset_yy: Optional[float], **kwargs: Any) -> SinkhornDivergenceOutput:
+                         offset_yy: Optional[float], uniform: Tuple[bool, bool] = (False, False),
+                         **kwargs: Any) -> SinkhornDivergenceOutput:
   """sinkhorn_divergence.py:253-351 (Sinkhorn solver only).
 
   With ``share_epsilon`` the symmetric terms use the (x, y) geometry's
@@ -133,8 +136,9 @@ def _sinkhorn_divergence(geometry_xy: PointCloud, geometry_xx: PointCloud, geome
                             momentum=acceleration.Momentum(start=0, value=0.5), anderson=None)
   # Uniform weights are passed to the solves as None (the library sets
   # 1 / n itself); the symmetric solves need the same weights on both sides.
-  a_solve = None if _is_uniform(a) else a
-  b_solve = None if _is_uniform(b) else b
+  # ``uniform`` marks weights built here as 1 / n (no check needed).
+  a_solve = None if uniform[0] or _is_uniform(a) else a
+  b_solve = None if uniform[1] or _is_uniform(b) else b
   out_xy = solve(geometry_xy, a=a_solve, b=b_solve, **kwargs)
   if share_epsilon:
     geometry_xx = geometry_xx.copy_epsilon(geometry_xy)
@@ -152,7 +156,11 @@ def _sinkhorn_divergence(geometry_xy: PointCloud, geometry_xx: PointCloud, geome
     reg_yy, pot_yy = out_yy.reg_ot_cost, (out_yy.f, out_yy.g)
     err_yy, conv_yy, it_yy = out_yy.errors, out_yy.converged, out_yy.n_iters
   eps = geometry_xy.epsilon
-  mass = float(a.double().sum() - b.double().sum())
+  # sum a - sum b in float64; for the default weights n float32(1 / n) sums to
+  # n * float32(1 / n) exactly, so it is formed on the host
+  sum_a = a.shape[0] * float(np.float32(1.0 / a.shape[0])) if uniform[0] else float(a.double().sum())
+  sum_b = b.shape[0] * float(np.float32(1.0 / b.shape[0])) if uniform[1] else float(b.double().sum())
+  mass = sum_a - sum_b
   div = out_xy.reg_ot_cost - 0.5 * (out_xx.reg_ot_cost + reg_yy) + 0.5 * eps * mass ** 2
   return SinkhornDivergenceOutput(
       divergence=float(div),

@@ -108,6 +108,9 @@ def _solve(x, y, a, b, precision, eps_kw, **kw):
 def _check_solve(out, ref, precision, same_iters=True):
   tol = TOL[precision]
   assert out.converged == ref.converged
+  if getattr(out, "solve_n_iters", None) is not None:
+    # the count the solve reported is the one its errors give (sinkhorn.py:431-434)
+    assert out.solve_n_iters == int((out.errors != -1).sum()) * out.inner_iterations
   if same_iters:
     assert out.n_iters == ref.n_iters, (out.n_iters, ref.n_iters)
     errs = out.errors.cpu().numpy()
