@@ -135,3 +135,49 @@ def test_batched_segment_ids_and_weights():
     sl = slice(ox[k], ox[k + 1])
     ref = oracle.solve_pointcloud(x[sl], y[oy[k]:oy[k + 1]], wx[sl], wy[oy[k]:oy[k + 1]], epsilon=0.1, **kw)
     assert abs(base[k].item() - ref.reg_ot_cost) <= 1e-4 * abs(ref.reg_ot_cost)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+def test_batched_thousands_of_problems_retire_in_rounds(precision):
+  """9000 problems (three templates repeated, each with its own epsilon, so
+  they stop at different blocks): the live work lists are rebuilt on the
+  device as problems finish, over more items than one 8192-item round of
+  the rebuild.  Every copy must give its template's result bit for bit, and
+  the template the oracle's (tools/segment_sinkhorn.py:24-138 per problem)."""
+  from ott_amd.tools import segment_sinkhorn as ss
+  d = 4
+  templates = [(20, 30, 0.5), (45, 17, 0.1), (33, 60, 0.05)]
+  reps = 3000
+  rng = np.random.default_rng(11)
+  tx = [(rng.normal(size=(n, d)) / np.sqrt(d)).astype(np.float32) for n, _, _ in templates]
+  ty = [(rng.normal(size=(m, d)) / np.sqrt(d) + 0.3).astype(np.float32) for _, m, _ in templates]
+  order = np.tile(np.arange(3), reps)
+  x = np.concatenate([tx[t] for t in order])
+  y = np.concatenate([ty[t] for t in order])
+  nx = [templates[t][0] for t in order]
+  ny = [templates[t][1] for t in order]
+  eps = [templates[t][2] for t in order]
+  kw = dict(threshold=1e-4, inner_iterations=1, max_iterations=3000)
+  print(f"batched {len(order)} problems {precision}", flush=True)
+  out = ss.segment_sinkhorn_outputs(torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), num_per_segment_x=nx,
+                                    num_per_segment_y=ny, sinkhorn_kwargs=kw, epsilons=eps, precision=precision)
+  f = out.f.cpu().numpy()
+  g = out.g.cpu().numpy()
+  reg = out.reg_ot_cost.numpy()
+  iters = out.n_iters.numpy()
+  ox, oy = out.x_offsets, out.y_offsets
+  tol = TOL[precision]
+  for t, (n, m, e) in enumerate(templates):
+    ks = np.nonzero(order == t)[0]
+    k0 = ks[0]
+    for k in ks[1:]:
+      assert reg[k] == reg[k0] and iters[k] == iters[k0], (t, k)
+      assert np.array_equal(f[ox[k]:ox[k + 1]], f[ox[k0]:ox[k0 + 1]], equal_nan=True), (t, k)
+      assert np.array_equal(g[oy[k]:oy[k + 1]], g[oy[k0]:oy[k0 + 1]], equal_nan=True), (t, k)
+    ref = oracle.solve_pointcloud(tx[t], ty[t], None, None, epsilon=e, **kw)
+    assert ref.converged and bool(out.converged[k0])
+    assert iters[k0] == ref.n_iters, (t, iters[k0], ref.n_iters)
+    assert abs(reg[k0] - ref.reg_ot_cost) <= tol * abs(ref.reg_ot_cost), (t, reg[k0], ref.reg_ot_cost)
+    _close(f[ox[k0]:ox[k0 + 1]].astype(np.float64), ref.f, tol, f"f[{t}]")
+  # the templates stop at different blocks, so the lists shrink in steps
+  assert len(set(int(v) for v in iters[:3])) == 3, iters[:3]
