@@ -141,14 +141,16 @@ typedef struct ottx_sinkhorn_opts {
                                runs, the errors are those of the recentred state and the outputs
                                are recentred (the Sejourne shift). */
   int32_t parallel_dual_updates; /* 1: Sinkhorn(parallel_dual_updates=True) (sinkhorn.py:840-846):
-                               the f-update reads the g of the previous iteration.  Supported for
-                               the symmetric problems of a Sinkhorn divergence
-                               (sinkhorn_divergence.py:298-318): x and y the same buffer, a and b
-                               the same buffer (or both NULL), tau_a = tau_b = 1, init_mode
-                               OTTX_INIT_ZEROS, no recentring.  There f and g stay equal, so each
-                               iteration is one pass: g <- (1 - w) g + w g^(f), then f <- g; the
-                               error, row plus column marginal (sinkhorn.py:160-189), is twice the
-                               column one.  Other problems return OTTX_EUNSUPPORTED. */
+                               both updates read the previous state; the error is the row plus
+                               the column marginal error (sinkhorn.py:160-189).  On the symmetric
+                               problems of a Sinkhorn divergence (sinkhorn_divergence.py:298-318:
+                               x and y the same buffer, a and b the same buffer or both NULL,
+                               tau_a = tau_b = 1, init_mode OTTX_INIT_ZEROS) f and g stay equal, so
+                               each iteration is one pass, g <- (1 - w) g + w g^(f), then f <- g,
+                               and the error is twice the column one.  Any other problem runs the
+                               f-update first, saving the old f for the g-update.  Not with
+                               recenter_potentials, a schedule or adaptive momentum
+                               (OTTX_EUNSUPPORTED). */
   double epsilon_init;      /* epsilon schedule (epsilon_scheduler.py:24-60): iteration t uses
                                max(epsilon_init epsilon_decay^t, 1) epsilon; the marginal error
                                and the outputs use epsilon itself.  Values <= 1 (0 included) run
@@ -161,9 +163,10 @@ typedef struct ottx_sinkhorn_opts {
                                (acceleration.py:138-160): iterations from momentum_start on use
                                Lehmann's weight 2 / (1 + sqrt(1 - r^(1 / inner))),
                                r = min(err[k-1] / err[k-2], 0.99), k = momentum_start /
-                               inner_iterations, if err[k-1] < momentum_error_threshold.  Needs
-                               norm_error = 1 and momentum_start >= 2 inner_iterations; not with
-                               parallel updates or in the batched solve (OTTX_EUNSUPPORTED). */
+                               inner_iterations, if err[k-1] < momentum_error_threshold, with the
+                               errors of the requested norm (as the reference records them).
+                               Needs momentum_start >= 2 inner_iterations; not with parallel
+                               updates or in the batched solve (OTTX_EUNSUPPORTED). */
   int32_t _pad0;
   double momentum_error_threshold; /* Momentum.error_threshold (inf when unused) */
   ottx_progress_fn progress_fn; /* NULL: none */
