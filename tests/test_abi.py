@@ -67,8 +67,11 @@ def test_batched_workspace_query_rejects_bad_batches():
                            max_iterations=100, momentum=1.0, tile_config=-1, use_graph=1)
   assert lib.ottx_sinkhorn_batched_workspace_bytes(ctypes.byref(batch()), ctypes.byref(opts)) > 0
   bad_off = (ctypes.c_int64 * 3)(0, 5, 5)  # an empty problem
+  # the default-weight flags are accepted, unknown flag bits are not
+  for flags in (_lib.BATCH_UNIFORM_A, _lib.BATCH_UNIFORM_B, _lib.BATCH_UNIFORM_A | _lib.BATCH_UNIFORM_B):
+    assert lib.ottx_sinkhorn_batched_workspace_bytes(ctypes.byref(batch(flags=flags)), ctypes.byref(opts)) > 0
   for kw in (dict(count=0), dict(d=0), dict(x_off=bad_off), dict(epsilon=None), dict(mean_rows_x=4.0),
-             dict(tau_a=0.0), dict(precision=7)):
+             dict(tau_a=0.0), dict(precision=7), dict(flags=4), dict(flags=-1)):
     assert lib.ottx_sinkhorn_batched_workspace_bytes(ctypes.byref(batch(**kw)), ctypes.byref(opts)) == 0, kw
   # relative epsilon needs no per-problem epsilons
   assert lib.ottx_sinkhorn_batched_workspace_bytes(
