@@ -123,7 +123,9 @@ typedef struct ottx_sinkhorn_opts {
   int32_t min_iterations;   /* default 0 */
   int32_t max_iterations;   /* default 2000 */
   double momentum;          /* constant Momentum weight (acceleration.py:161-172), 1 = none */
-  int32_t tile_config;      /* -1 = default, else a value listed by ottx_tile_configs */
+  int32_t tile_config;      /* -1 = the library's default for the precision, d and size (the
+                               operator entry points always use it), else a value listed by
+                               ottx_tile_configs */
   int32_t use_graph;        /* 1 = replay each block of iterations as a hipGraph */
   int32_t init_mode;        /* OTTX_INIT_GIVEN: f and g are used as given;
                                OTTX_INIT_F_FROM_G: f starts as the f-update of the given g (a warm

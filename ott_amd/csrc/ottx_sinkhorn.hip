@@ -2170,8 +2170,27 @@ struct Plan {
   Layout L;
 };
 
+// The tile configuration of a problem that names none (-1), which is also what
+// the operator entry points (no tile_config argument) run with.  From 2^24
+// pairs on: the choices the tuner makes on MI355X at n = m = 16384, d = 64 and
+// at n = 32768, m = 8192, d = 128 (bf16 split: variant 1, NB 4 or 2 by d, MT
+// 2, two workgroups per CU; fp32: variant 2, NB 2, MT 4, four per CU, or
+// beyond 64 features variant 1 with three per CU).  Smaller problems: variant
+// 0 at four workgroups per CU.
+int default_tile_config(int key, int d, int64_t n, int64_t m) {
+  if (static_cast<double>(n) * static_cast<double>(m) < 16777216.0) return 0;
+  if (key == OTTX_PREC_BF16_SPLIT && d <= 128) return 1 | (1 << 8);
+  if (key == OTTX_PREC_FP32) {
+    const int ks = (d + 3) / 4;
+    if (ks >= 3 && ks <= 16) return 2;
+    if (ks > 16 && ks <= 32) return 1 | (5 << 8);
+  }
+  return 0;
+}
+
 bool make_plan(const ottx_problem* pb, int tile_config, Plan* P) {
   int tm = 4;
+  if (tile_config < 0) tile_config = default_tile_config(kernel_key(pb), pb->d, pb->n, pb->m);
   if (!select_kernel(kernel_key(pb), pb->d, tile_config, &P->k, &tm)) return false;
   const int cus = device_cus();
   // Every column tile (16 NB W points) and streamed tile (16 MT rows) must

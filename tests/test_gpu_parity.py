@@ -71,6 +71,24 @@ def test_apply_lse_kernel(precision, d, axis):
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
 @pytest.mark.parametrize("axis", [0, 1])
+def test_apply_lse_kernel_at_2_24_pairs(precision, axis):
+  """apply_lse_kernel (geometry.py:260-299) where the library's default tile
+  configuration switches to the large-problem one (2^24 pairs, d = 64)."""
+  x, y, a, b = _cloud(4096, 4096, 64, seed=64)
+  rng = np.random.default_rng(8)
+  f = rng.normal(size=4096).astype(np.float32) * 0.2
+  g = rng.normal(size=4096).astype(np.float32) * 0.2
+  eps = 0.1
+  print(f"apply_lse 4096x4096 d=64 {precision} axis={axis}", flush=True)
+  geom = _geom(x, y, precision, epsilon=eps)
+  got, _ = geom.apply_lse_kernel(torch.from_numpy(f).cuda(), torch.from_numpy(g).cuda(), eps, axis=axis)
+  ops = oracle.DenseLSE(oracle.sqeuclidean_cost(x, y))
+  want = oracle.apply_lse_kernel(ops, f.astype(np.float64), g.astype(np.float64), eps, axis)
+  _close(got.cpu().numpy(), want, TOL[precision], f"apply_lse 2^24 axis={axis}")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_split"])
+@pytest.mark.parametrize("axis", [0, 1])
 def test_update_potential_and_marginal(precision, axis):
   x, y, a, b = _cloud(257, 129, 32, seed=11, zero_a=[0], zero_b=[5])
   f = np.where(a > 0, 0.0, -np.inf).astype(np.float32)
