@@ -3863,7 +3863,8 @@ struct BatchMail {
 // End of a block.  When problems have finished since the last rebuild, the
 // live work lists are rebuilt from the full ones without the items of
 // finished problems, keeping their longest-first order (a stable compaction:
-// wave ballots and per-wave offsets, 1024 items per round).  Then the
+// wave ballots and a scan of the per-wave counts, 8192 items per round, eight
+// per thread).  Then the
 // progress (finished problems, live items, completed blocks) goes to the
 // mailbox.  counters: batch_done (kBatchDoneBits).
 constexpr int kCompactThreads = 1024;
@@ -3978,7 +3979,8 @@ __global__ void batch_init_kernel(const BatchItem* __restrict__ items, int outer
     const float w = static_cast<float>(1.0 / static_cast<double>(it.m));
     for (int64_t k = t; k < it.m; k += blockDim.x) it.b_fill[k] = w;
   }
-  // DefaultInitializer (each weight written above is read back by the thread that wrote it): 0, or -inf where the weight is 0 (bit 0: f, bit 1: g)
+  // DefaultInitializer: 0, or -inf where the weight is 0 (bit 0: f, bit 1: g);
+  // each weight written above is read back by the thread that wrote it
   if (zero_init & 1)
     for (int64_t k = t; k < it.n; k += blockDim.x) it.f[k] = it.a[k] > 0.f ? 0.f : -INFINITY;
   if (zero_init & 2)
