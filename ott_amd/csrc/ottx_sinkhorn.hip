@@ -3987,10 +3987,10 @@ __global__ void batch_init_kernel(const BatchItem* __restrict__ items, int outer
     for (int64_t k = t; k < it.m; k += blockDim.x) it.g[k] = it.b[k] > 0.f ? 0.f : -INFINITY;
 }
 
-// Stage 1 of the float64 dual terms of every problem: grid (kDualBlocks, 3, K),
-// as dual_partial_kernel for one problem.
+// Stage 1 of the float64 dual terms of every problem: grid (nblk, 3, K),
+// nblk <= kDualBlocks blocks per side, as dual_partial_kernel for one problem.
 __global__ void batch_dual_partial_kernel(const BatchItem* __restrict__ items, double tau_a, double tau_b,
-                                          int outer) {
+                                          int outer, int nblk) {
   __shared__ double red[2][256];
   const BatchItem& it = items[blockIdx.z];
   const int tid = threadIdx.x;
@@ -4003,7 +4003,7 @@ __global__ void batch_dual_partial_kernel(const BatchItem* __restrict__ items, d
     const double tau = side == 0 ? tau_a : tau_b;
     const double eps = it.scal->eps;
     const double rho = eps * tau / (1.0 - tau);
-    const int64_t per = (len + kDualBlocks - 1) / kDualBlocks;
+    const int64_t per = (len + nblk - 1) / nblk;
     const int64_t i0 = blockIdx.x * per, i1 = min(len, i0 + per);
     for (int64_t i = i0 + tid; i < i1; i += blockDim.x) {
       const double wi = w[i];
@@ -4042,13 +4042,14 @@ __global__ void batch_dual_partial_kernel(const BatchItem* __restrict__ items, d
 
 // Stage 2: out[16 k + ...] = div_a, sum a, div_b, sum b, #NaN errors,
 // #(errors != -1), errors[-1], eps, total mass, blocks run.  Grid K.
-__global__ void batch_dual_final_kernel(const BatchItem* __restrict__ items, int outer, double* __restrict__ out) {
+__global__ void batch_dual_final_kernel(const BatchItem* __restrict__ items, int outer, int nblk,
+                                        double* __restrict__ out) {
   if (threadIdx.x != 0) return;
   const BatchItem& it = items[blockIdx.x];
   double* o = out + 16 * static_cast<int64_t>(blockIdx.x);
   for (int side = 0; side < 3; ++side) {
     double s0 = 0.0, s1 = 0.0;
-    for (int b = 0; b < kDualBlocks; ++b) {
+    for (int b = 0; b < nblk; ++b) {
       s0 += it.dual_part[(side * kDualBlocks + b) * 2 + 0];
       s1 += it.dual_part[(side * kDualBlocks + b) * 2 + 1];
     }
@@ -4549,9 +4550,11 @@ extern "C" int ottx_sinkhorn_batched_solve(const ottx_batch* bt, const ottx_sink
   if (status != OTTX_OK) return status;
 
   double* d_out = reinterpret_cast<double*>(ws + B.off_out);
-  hipLaunchKernelGGL(batch_dual_partial_kernel, dim3(kDualBlocks, 3, K), dim3(256), 0, st, d_items, bt->tau_a,
-                     bt->tau_b, outer);
-  hipLaunchKernelGGL(batch_dual_final_kernel, dim3(K), dim3(64), 0, st, d_items, outer, d_out);
+  // one workgroup per 4096 points of the longest cloud (per side and problem)
+  const int dual_blocks = static_cast<int>(std::min<int64_t>(kDualBlocks, (B.max_np + 4095) / 4096));
+  hipLaunchKernelGGL(batch_dual_partial_kernel, dim3(dual_blocks, 3, K), dim3(256), 0, st, d_items, bt->tau_a,
+                     bt->tau_b, outer, dual_blocks);
+  hipLaunchKernelGGL(batch_dual_final_kernel, dim3(K), dim3(64), 0, st, d_items, outer, dual_blocks, d_out);
   hipLaunchKernelGGL(batch_copy_g_kernel, dim3(static_cast<unsigned>((B.max_np + 255) / 256), K), dim3(256), 0, st,
                      d_items);
   OTTX_LAUNCHED();
