@@ -519,7 +519,10 @@ inline int zero_init_bits(int init_mode) {
 }
 
 // Row r < K of the shift probe's sample: evenly strided over the n rows.
-__device__ __forceinline__ int64_t probe_row(int r, int64_t n, int K) { return (static_cast<int64_t>(r) * n) / K; }
+__device__ __forceinline__ int64_t probe_row(int r, int64_t n, int K) {
+  const int64_t p = static_cast<int64_t>(r) * n;
+  return K == kProbeRows ? p / kProbeRows : p / K;  // K = 64 unless n < 64: a shift, not a 64-bit division
+}
 
 constexpr int kProbeCols = 64;
 constexpr int kProbeChunk = 64;
@@ -535,115 +538,148 @@ constexpr int kProbeChunk = 64;
 // only sets the streaming shift of the first g-update: a tile whose sums
 // leave range falls back to the running max.
 //
-// A workgroup of four waves takes 64 columns, one per lane, and wave w the
-// sampled rows w, w + 4, ...  The probe is short, so it is bound by memory
-// latency: each chunk of 64 features takes one round trip, with the sampled
-// rows, x_0 and f staged in LDS and the lane's column in registers, every load
-// issued before the first wait.  The waves merge their maxima through LDS.
+// A workgroup of four waves takes 64 columns and wave w the sampled rows
+// 16w .. 16w + 15: a 64 x 64 product over d, on the fp32 MFMA
+// (v_mfma_f32_16x16x4f32, four 16 x 16 tiles a wave).  Lane (q, l) of a wave
+// (q = lane / 16, l = lane % 16) feeds features 16q .. 16q + 15 of each 64-wide
+// chunk, of sampled row 16w + l and of column 16nb + l; both operands come
+// from LDS, where the chunk of the sampled rows, of the block's columns
+// (read coalesced) and of x_0 is staged with padded rows.  Each chunk takes
+// one round trip of loads, all issued before the first wait.  The waves
+// merge their maxima through LDS.
 __device__ void probe_max_block(int blk, const float* __restrict__ x, int64_t n, const float* __restrict__ y,
                                 int64_t m, int64_t m_pad, int d, const float* __restrict__ f, double s,
                                 float* __restrict__ out) {
-  constexpr int R = kProbeRows / 4;
   constexpr int KC = kProbeChunk;
-  __shared__ float4 s_x[kProbeRows][KC / 4];
+  constexpr int PS = KC / 4 + 1;               // padded row of s_x and s_y in float4: no bank conflicts
+  constexpr int NX = kProbeRows * KC / 256;    // floats of the row chunk per thread
+  constexpr int NY = kProbeCols * KC / 1024;   // float4 of the column chunk per thread
+  static_assert(kProbeRows == 64 && kProbeCols == 64 && KC == 64, "four waves of four 16 x 16 tiles");
+  __shared__ float4 s_x[kProbeRows][PS];
+  __shared__ float4 s_y[kProbeCols][PS];
   __shared__ float4 s_c[KC / 4];
   __shared__ float s_f[kProbeRows], s_nrm[kProbeRows];
   __shared__ float s_mx[4][kProbeCols];
-  const int tid = threadIdx.x, lane = tid & 63;
+  const int tid = threadIdx.x, lane = tid & 63, q = lane >> 4, l16 = lane & 15;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int K = static_cast<int>(min<int64_t>(kProbeRows, n));
   const int64_t col0 = static_cast<int64_t>(blk) * kProbeCols;
-  const float* yj = y + min(col0 + lane, m - 1) * d;
   const bool vec16 = (d & 3) == 0 && (reinterpret_cast<uintptr_t>(y) & 15) == 0;
   if (tid < kProbeRows) {
     s_f[tid] = tid < K ? f[probe_row(tid, n, K)] : 0.f;
     s_nrm[tid] = 0.f;
   }
-  float acc[R], yc = 0.f;
+  f32x4 acc[4];
+  float ycp[4];
 #pragma unroll
-  for (int t = 0; t < R; ++t) acc[t] = 0.f;
+  for (int nb = 0; nb < 4; ++nb) {
+    acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    ycp[nb] = 0.f;
+  }
   for (int k0 = 0; k0 < d; k0 += KC) {
     const int kc = min(KC, d - k0);
-    // the lane's column, the chunk of the sampled rows and of x_0, zero past d
-    float yv[KC];
-    if (vec16) {
+    // the chunk of the block's columns, of the sampled rows and of x_0, zero past d, m and K
+    float4 ys[NY];
 #pragma unroll
-      for (int k4 = 0; k4 < KC / 4; ++k4) {
-        const float4 v = 4 * k4 < kc ? *reinterpret_cast<const float4*>(yj + k0 + 4 * k4) : make_float4(0.f, 0.f, 0.f, 0.f);
-        yv[4 * k4] = v.x;
-        yv[4 * k4 + 1] = v.y;
-        yv[4 * k4 + 2] = v.z;
-        yv[4 * k4 + 3] = v.w;
+    for (int i = 0; i < NY; ++i) {
+      const int e = tid + 256 * i, k = 4 * (e % (KC / 4));
+      const int64_t j = col0 + e / (KC / 4);
+      const float* src = y + j * d + k0 + k;
+      if (j >= m || k >= kc) {
+        ys[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      } else if (vec16) {
+        ys[i] = *reinterpret_cast<const float4*>(src);
+      } else {
+        ys[i].x = src[0];
+        ys[i].y = k + 1 < kc ? src[1] : 0.f;
+        ys[i].z = k + 2 < kc ? src[2] : 0.f;
+        ys[i].w = k + 3 < kc ? src[3] : 0.f;
       }
-    } else {
-#pragma unroll
-      for (int k = 0; k < KC; ++k) yv[k] = k < kc ? yj[k0 + k] : 0.f;
     }
-    float xs[kProbeRows * KC / 256];
+    float xs[NX];
 #pragma unroll
-    for (int i = 0; i < kProbeRows * KC / 256; ++i) {
+    for (int i = 0; i < NX; ++i) {
       const int e = tid + 256 * i, r = e / KC, k = e % KC;
       xs[i] = (r < K && k < kc) ? x[probe_row(r, n, K) * d + k0 + k] : 0.f;
     }
     const float ck = (tid < KC && tid < kc) ? x[k0 + tid] : 0.f;
     __syncthreads();  // the previous chunk's readers are done
-    float* sx = reinterpret_cast<float*>(s_x);
+    float* sxf = reinterpret_cast<float*>(s_x);
 #pragma unroll
-    for (int i = 0; i < kProbeRows * KC / 256; ++i) sx[tid + 256 * i] = xs[i];
+    for (int i = 0; i < NX; ++i) {
+      const int e = tid + 256 * i;
+      sxf[(e / KC) * (4 * PS) + e % KC] = xs[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NY; ++i) {
+      const int e = tid + 256 * i;
+      s_y[e / (KC / 4)][e % (KC / 4)] = ys[i];
+    }
     if (tid < KC) reinterpret_cast<float*>(s_c)[tid] = ck;
     __syncthreads();
     // |x_r - x_0|^2 over the chunk: four threads a row
     {
-      const int r = tid >> 2, q = tid & 3;
+      const int r = tid >> 2, qr = tid & 3;
       float nr = 0.f;
 #pragma unroll
       for (int i = 0; i < KC / 16; ++i) {
-        const float4 v = s_x[r][q * (KC / 16) + i], u = s_c[q * (KC / 16) + i];
+        const float4 v = s_x[r][qr * (KC / 16) + i], u = s_c[qr * (KC / 16) + i];
         const float a = v.x - u.x, b = v.y - u.y, c = v.z - u.z, e = v.w - u.w;
         nr += a * a + b * b + c * c + e * e;
       }
       nr += __shfl_xor(nr, 1);
       nr += __shfl_xor(nr, 2);
-      if (q == 0) s_nrm[r] += nr;
+      if (qr == 0) s_nrm[r] += nr;
     }
-    // <x_r, y_j - x_0> for the wave's rows, and <x_0, y_j - x_0>
+    // <x_r, y_j - x_0> on the MFMA, and the lane's part of <x_0, y_j - x_0>
+    float4 cv[4], av[4];
 #pragma unroll
-    for (int k4 = 0; k4 < KC / 4; ++k4) {
-      const float4 u = s_c[k4];
-      yv[4 * k4 + 0] -= u.x;
-      yv[4 * k4 + 1] -= u.y;
-      yv[4 * k4 + 2] -= u.z;
-      yv[4 * k4 + 3] -= u.w;
-      yc += u.x * yv[4 * k4] + u.y * yv[4 * k4 + 1] + u.z * yv[4 * k4 + 2] + u.w * yv[4 * k4 + 3];
+    for (int i = 0; i < 4; ++i) {
+      cv[i] = s_c[4 * q + i];
+      av[i] = s_x[16 * w + l16][4 * q + i];
     }
 #pragma unroll
-    for (int t = 0; t < R; ++t) {
-      const int r = w + 4 * t;
-      float a = 0.f;
+    for (int nb = 0; nb < 4; ++nb) {
 #pragma unroll
-      for (int k4 = 0; k4 < KC / 4; ++k4) {
-        const float4 xv = s_x[r][k4];
-        a += xv.x * yv[4 * k4] + xv.y * yv[4 * k4 + 1] + xv.z * yv[4 * k4 + 2] + xv.w * yv[4 * k4 + 3];
+      for (int i = 0; i < 4; ++i) {
+        float4 v = s_y[16 * nb + l16][4 * q + i];
+        v.x -= cv[i].x;
+        v.y -= cv[i].y;
+        v.z -= cv[i].z;
+        v.w -= cv[i].w;
+        ycp[nb] += cv[i].x * v.x + cv[i].y * v.y + cv[i].z * v.z + cv[i].w * v.w;
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i].x, v.x, acc[nb], 0, 0, 0);
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i].y, v.y, acc[nb], 0, 0, 0);
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i].z, v.z, acc[nb], 0, 0, 0);
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i].w, v.w, acc[nb], 0, 0, 0);
       }
-      acc[t] += a;
     }
   }
   __syncthreads();
+  // element i of tile nb in lane (q, l): sampled row 16w + 4q + i, column 16nb + l
   const float inv2s = static_cast<float>(0.5 / s);
-  float M = -INFINITY;
 #pragma unroll
-  for (int t = 0; t < R; ++t) {
-    const int r = w + 4 * t;
-    if (r >= K) continue;
-    const float a = acc[t] - yc + s_f[r] * inv2s - 0.5f * s_nrm[r];
-    if (isfinite(a)) M = fmaxf(M, a);
+  for (int nb = 0; nb < 4; ++nb) {
+    float yc = ycp[nb];
+    yc += __shfl_xor(yc, 16);
+    yc += __shfl_xor(yc, 32);
+    float M = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 16 * w + 4 * q + i;
+      if (r >= K) continue;
+      const float a = acc[nb][i] - yc + s_f[r] * inv2s - 0.5f * s_nrm[r];
+      if (isfinite(a)) M = fmaxf(M, a);
+    }
+    M = fmaxf(M, __shfl_xor(M, 16));
+    M = fmaxf(M, __shfl_xor(M, 32));
+    if (q == 0) s_mx[w][16 * nb + l16] = M;
   }
-  s_mx[w][lane] = M;
   __syncthreads();
   if (tid < kProbeCols) {
     const int64_t j = col0 + tid;
     if (j >= m_pad) return;
-    M = fmaxf(fmaxf(s_mx[0][tid], s_mx[1][tid]), fmaxf(s_mx[2][tid], s_mx[3][tid]));
+    const float M = fmaxf(fmaxf(s_mx[0][tid], s_mx[1][tid]), fmaxf(s_mx[2][tid], s_mx[3][tid]));
     out[j] = j < m ? M : -INFINITY;
   }
 }
