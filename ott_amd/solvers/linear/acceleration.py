@@ -1,9 +1,11 @@
 """Momentum (solvers/linear/acceleration.py:127-172 in OTT).
 
 Constant momentum (``start == 0``) and the adaptive Lehmann momentum
-(``start > 0``, acceleration.py:138-160) run on the GPU path; the adaptive
-form needs ``norm_error=1``, ``start >= 2 inner_iterations`` and sequential
-updates (:class:`~ott_amd.solvers.linear.sinkhorn.Sinkhorn` raises
+(``start > 0``, acceleration.py:138-160) run on the GPU path, on balanced
+and unbalanced problems and with any ``norm_error`` (the weight reads the
+errors of the requested norm, as the reference records them); the adaptive
+form needs ``start >= 2 inner_iterations`` and sequential updates
+(:class:`~ott_amd.solvers.linear.sinkhorn.Sinkhorn` raises
 ``NotImplementedError`` otherwise).  Anderson acceleration is not run; its
 class exists so that passing it raises the same error.
 """
