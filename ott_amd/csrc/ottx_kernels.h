@@ -1068,7 +1068,25 @@ __device__ __forceinline__ void halfstep_block(const HalfStepArgs& p, uint32_t m
 // Single-problem half-step: grid (column tiles, row splits).
 template <int PREC, int KK, int NB, int MT, int W>
 __global__ __launch_bounds__(W * 64) void halfstep_kernel(const HalfStepArgs p) {
-  halfstep_block<PREC, KK, NB, MT, W>(p, p.mode, blockIdx.x, blockIdx.y, threadIdx.x);
+  // Workgroups reach the 8 XCDs round robin in dispatch order (x fastest), and
+  // each XCD has its own L2.  In grid order an XCD would take every split of
+  // its tiles: all of the streamed cloud and 1/8 of the other.  Here XCD k
+  // takes half of x and a quarter of y, the same parts in both directions
+  // (g-update: split group k % 2 of x's rows and tile group k / 2 of y's
+  // columns; f-update: split group k / 2 of y's rows and tile group k % 2 of
+  // x's), 3/8 of the fragments instead of 9/16.  Any bijection is correct: the
+  // grouping only moves which L2 serves a workgroup.
+  int tile = blockIdx.x, split = blockIdx.y;
+  const int T = gridDim.x, S = gridDim.y;
+  const bool g_dir = p.side == 1;
+  const int sg_n = g_dir ? 2 : 4, tg_n = 8 / sg_n;
+  if (S % sg_n == 0 && T % tg_n == 0) {
+    const int b = blockIdx.x + blockIdx.y * T, k = b & 7, local = b >> 3;
+    const int Tg = T / tg_n;
+    split = (g_dir ? (k & 1) : (k >> 1)) * (S / sg_n) + local / Tg;
+    tile = (g_dir ? (k >> 1) : (k & 1)) * Tg + local % Tg;
+  }
+  halfstep_block<PREC, KK, NB, MT, W>(p, p.mode, tile, split, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
